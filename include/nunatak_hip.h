@@ -1,0 +1,245 @@
+/*
+ * nunatak_hip.h — C ABI of libnunatak_hip.so, the MI355X (gfx950) velocity solver for Nunatak.jl.
+ *
+ * The library replaces the reference's depth-integrated velocity solve:
+ *   update_velocities!  (src/Models/ice_sheet_model_methods.jl:116-147)
+ *   -> compute_driving_stress!  (src/StressBalances/driving_stress.jl:43-53)
+ *   -> picard_solve!            (src/Solvers/picard.jl:71-90)
+ *      -> update_stress_balance_coefficients! / update_diva_coefficients!
+ *                               (src/StressBalances/stress_balance_coefficients.jl:10-20, src/StressBalances/diva.jl:61-75)
+ *      -> solve! (PCG)          (src/Solvers/conjugate_gradient.jl:128-223)
+ *         -> ssa_operator!      (src/StressBalances/ssa_operator.jl:47-89)
+ *         -> precondition!      (src/StressBalances/diagonal_preconditioner.jl:11-18,
+ *                                src/StressBalances/multigrid_preconditioner.jl:122-128)
+ *   and the geometry diagnostics feeding it, update_geometry! (src/Models/ice_sheet_model_methods.jl:103-111).
+ *
+ * Model of the boundary. A handle owns one grid's device state: every field the reference keeps in
+ * `model.geometry`, `model.auxiliary` and `model.velocities` (src/Models/ice_sheet_fields.jl) lives in HBM under a
+ * field id (nk_field_id). The host copies fields in and out with nk_set_field / nk_get_field and then calls
+ * operations that mirror the reference functions one to one. Device pointers of the FP64 fields are exposed for
+ * zero-copy callers. All entry points return an nk_status; nothing aborts and no C++ exception crosses the ABI.
+ *
+ * Host array convention (the reference's column-major layout): element (i, j[, k]) of a field, 1-based as in
+ * Julia, is at data[(i-1) + (j-1)*ld + (k-1)*lds]. ld = 0 means ld = the field's x extent; lds = 0 means
+ * lds = ld * (y extent). For an Oceananigans parent array with halo (Hx, Hy) pass
+ * pointer(parent(f), Hx + 1 + Hy * size(parent(f), 1)) with ld = size(parent(f), 1).
+ * Field extents follow Oceananigans `size(field)` on a RectilinearGrid with Flat z: a Face location on a
+ * Bounded direction has N + 1 points, every other location has N.
+ *
+ * Threading: a handle is used by one host thread at a time; distinct handles are independent and each owns its
+ * own HIP stream.
+ */
+#ifndef NUNATAK_HIP_H
+#define NUNATAK_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NK_ABI_VERSION 1
+
+typedef enum nk_status {
+    NK_OK = 0,
+    NK_INVALID_ARGUMENT = 1,  /* the reference throws ArgumentError (e.g. src/StressBalances/ssa_operator.jl:56) */
+    NK_NOT_CONVERGED = 2,     /* reported only when the caller asks for it; the reference is silent */
+    NK_NONFINITE = 3,         /* NaN or Inf found in a solution */
+    NK_OUT_OF_MEMORY = 4,
+    NK_DEVICE_ERROR = 5,
+    NK_UNSUPPORTED = 6
+} nk_status;
+
+/* Topology of a horizontal direction: Oceananigans Periodic or Bounded. */
+enum { NK_PERIODIC = 0, NK_BOUNDED = 1 };
+
+/* Grid: a uniform RectilinearGrid (Flat z) plus the σ column of SigmaGrid (src/Grids/sigma_grid.jl:20-62). */
+typedef struct nk_grid {
+    int32_t Nx, Ny;                 /* cells */
+    int32_t topology_x, topology_y; /* NK_PERIODIC or NK_BOUNDED */
+    int32_t halo;                   /* halo of the host grid; enters the multigrid coarsening rule */
+    int32_t Nsigma;                 /* σ layers, default 10 (src/Models/ice_sheet_model.jl:119) */
+    double dx, dy;                  /* uniform cell widths (m) */
+    double bed_refinement;          /* σ faces ζ_k = (k/Nσ)^bed_refinement, default 2 */
+} nk_grid;
+
+enum { NK_GROUNDING_NONE = 0, NK_GROUNDING_SUBGRID = 1, NK_GROUNDING_SUBGRID_DRIVING_STRESS = 2 };
+enum { NK_FRICTION_NONE = 0, NK_FRICTION_WEERTMAN = 1, NK_FRICTION_REGULARIZED_COULOMB = 2 };
+enum { NK_FREE_SLIP = 0, NK_NO_SLIP = 1 };
+enum { NK_SSA = 0, NK_DIVA = 1 };
+enum { NK_RATE_FACTOR_2D = 0, NK_RATE_FACTOR_3D = 1 };
+
+/* Physics of the model: IceSheetConstants, SmoothIceMask, GlenFlowLaw, the friction law, the grounding treatment
+ * and DepthIntegrated (src/Models/ice_sheet_constants.jl:33, src/Models/smooth_ice_mask.jl:41,
+ * src/Rheologies/glen_flow_law.jl:114, src/Frictions/frictions.jl:255,286, src/Groundings/subgrid_grounded_fraction.jl:146,
+ * src/StressBalances/depth_integrated.jl:138). */
+typedef struct nk_physics {
+    double ice_density, ocean_density, gravitational_acceleration;   /* 918, 1028, 9.81 */
+    int32_t grounding;                                               /* NK_GROUNDING_* */
+    int32_t masking;                                                 /* 1 = SmoothIceMask, 0 = none (μ ≡ 1 not used) */
+    double mask_minimum_thickness, mask_transition_thickness;        /* 1, 1 */
+    double glen_exponent, strain_rate_regularization;                /* 3, 1e-13 */
+    int32_t rate_factor_depth;                                       /* NK_RATE_FACTOR_2D or _3D */
+    int32_t friction;                                                /* NK_FRICTION_* */
+    double friction_exponent, coulomb_coefficient, velocity_regularization;  /* 1/3, α², 1e-8 */
+    int32_t walls;                                                   /* NK_FREE_SLIP or NK_NO_SLIP */
+    int32_t stress_balance;                                          /* NK_SSA or NK_DIVA */
+} nk_physics;
+
+/* Iteration control: Convergence(reltol, abstol, maxiter) or FixedIterations(n) (src/Solvers/iteration_controls.jl). */
+enum { NK_CONVERGENCE = 0, NK_FIXED_ITERATIONS = 1 };
+typedef struct nk_control {
+    int32_t kind;
+    int32_t maxiter;     /* Convergence */
+    int32_t iterations;  /* FixedIterations */
+    int32_t pad;
+    double reltol, abstol;
+} nk_control;
+
+enum { NK_PRECONDITIONER_NONE = 0, NK_PRECONDITIONER_DIAGONAL = 1, NK_PRECONDITIONER_MULTIGRID = 2 };
+enum { NK_POINT_JACOBI = 0, NK_LINE_JACOBI = 1 };
+enum { NK_PRECISION_FP64 = 0, NK_PRECISION_MIXED = 1 };
+
+/* StressBalanceSolver options (src/StressBalances/stress_balance_solver.jl:126-135) and the multigrid defaults
+ * (src/StressBalances/multigrid_preconditioner.jl:57). */
+typedef struct nk_solver_options {
+    nk_control linear;                /* Convergence(reltol = 1e-8, maxiter = 10_000) */
+    nk_control picard;                /* Convergence(reltol = 1e-6, maxiter = 100) */
+    int32_t preconditioner;           /* NK_PRECONDITIONER_DIAGONAL */
+    int32_t smoother;                 /* NK_POINT_JACOBI */
+    int32_t smoothing_sweeps;         /* 2 */
+    int32_t coarsest_sweeps;          /* 20 */
+    int32_t minimum_coarse_size;      /* 2 */
+    int32_t precision;                /* NK_PRECISION_FP64; NK_PRECISION_MIXED = FP32 inner PCG + V-cycle
+                                         inside FP64 iterative refinement */
+    double inner_reltol;              /* mixed: relative reduction asked of each FP32 inner solve */
+    int32_t inner_maxiter;            /* mixed: FP32 PCG iterations per refinement step */
+    int32_t check_interval;           /* PCG iterations launched between host checks of the on-device stop flag */
+} nk_solver_options;
+
+/* Field ids. Location (x, y[, σ]) in brackets; Face on a Bounded direction has N + 1 points. */
+typedef enum nk_field_id {
+    NK_FIELD_U = 0,        /* depth-averaged u            (Face, Center)      */
+    NK_FIELD_V,            /* depth-averaged v            (Center, Face)      */
+    NK_FIELD_H,            /* thickness                   (Center, Center)    */
+    NK_FIELD_BED,          /* bed elevation b             (Center, Center)    */
+    NK_FIELD_S,            /* surface elevation           (Center, Center)    */
+    NK_FIELD_PHI,          /* grounded fraction φ         (Center, Center)    */
+    NK_FIELD_PHIU,         /* φᵘ                          (Face, Center)      */
+    NK_FIELD_PHIV,         /* φᵛ                          (Center, Face)      */
+    NK_FIELD_N,            /* effective pressure          (Center, Center)    */
+    NK_FIELD_MU,           /* ice mask μ                  (Center, Center)    */
+    NK_FIELD_TAUU,         /* driving stress τᵘ           (Face, Center)      */
+    NK_FIELD_TAUV,         /* driving stress τᵛ           (Center, Face)      */
+    NK_FIELD_ETAH,         /* η̄H                          (Center, Center)    */
+    NK_FIELD_ETAHFF,       /* η̄Hᶠᶠ                        (Face, Face)        */
+    NK_FIELD_BETAU,        /* operator drag βᵘ            (Face, Center)      */
+    NK_FIELD_BETAV,        /* operator drag βᵛ            (Center, Face)      */
+    NK_FIELD_F2,           /* DIVA integral F₂            (Center, Center)    */
+    NK_FIELD_ETASIGMA,     /* ησ                          (Center, Center, σ) */
+    NK_FIELD_BETABU,       /* friction-law drag βᵇᵘ       (Face, Center)      */
+    NK_FIELD_BETABV,       /* βᵇᵛ                         (Center, Face)      */
+    NK_FIELD_UB,           /* basal velocity uᵇ           (Face, Center)      */
+    NK_FIELD_VB,           /* vᵇ                          (Center, Face)      */
+    NK_FIELD_USIGMA,       /* uσ                          (Face, Center, σ)   */
+    NK_FIELD_VSIGMA,       /* vσ                          (Center, Face, σ)   */
+    NK_FIELD_A,            /* rate factor A               (Center, Center[, σ]) per rate_factor_depth */
+    NK_FIELD_C,            /* friction coefficient C      (Center, Center)    */
+    NK_FIELD_TEMPERATURE,  /* temperature (K) input of nk_rate_factor_from_temperature (Center, Center, σ) */
+    NK_FIELD_PU,           /* operand vector p / r        (Face, Center)      */
+    NK_FIELD_PV,           /*                             (Center, Face)      */
+    NK_FIELD_QU,           /* result vector q / z         (Face, Center)      */
+    NK_FIELD_QV,           /*                             (Center, Face)      */
+    NK_FIELD_COUNT
+} nk_field_id;
+
+/* Face classes written by nk_compute_driving_stress, one byte per velocity point, bit-exact with the reference's
+ * branch selection in _compute_x/_y_driving_stress! (src/StressBalances/driving_stress.jl:109-125). */
+enum {
+    NK_CLASS_WALL = 1,       /* peripheral_node: Bounded wall face */
+    NK_CLASS_ICE_FREE = 2,   /* μ = 0 on both sides */
+    NK_CLASS_FRONT = 4,      /* μ > 0 on exactly one side: calving-front force */
+    NK_CLASS_GROUNDING = 8   /* flotation changes sign across the face: grounding-line split */
+};
+
+typedef struct nk_info {
+    int32_t picard_iterations;      /* solver.iteration of StressBalanceSolver */
+    int32_t linear_iterations;      /* iterations of the last linear solve (VelocityConjugateGradientSolver.iteration) */
+    int64_t total_linear_iterations;
+    int32_t refinement_steps;       /* mixed precision: FP64 defect corrections in the last linear solve */
+    int32_t pad;
+    double last_picard_change;      /* Δ = ‖x − x⁻‖ */
+    double last_picard_norm;        /* X = ‖x‖ */
+    double last_residual_norm;      /* ‖r‖ at the end of the last linear solve */
+    double elapsed_ms;              /* device time of the call, from HIP events */
+} nk_info;
+
+typedef struct nk_solver nk_solver;
+
+/* Version of this ABI (NK_ABI_VERSION). */
+int32_t nk_abi_version(void);
+
+/* Fill defaults of the reference's constructors. */
+void nk_default_grid(nk_grid *grid);
+void nk_default_physics(nk_physics *physics);
+void nk_default_options(nk_solver_options *options);
+
+/* StressBalanceSolver(grid; ...) + the field set of IceSheetModel (src/StressBalances/stress_balance_solver.jl:126,
+ * src/Models/ice_sheet_fields.jl:182-233), and StressBalanceMultigridPreconditioner(grid; ...) when options ask for
+ * it (src/StressBalances/multigrid_preconditioner.jl:57). Allocates all device state of the handle. */
+nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_solver_options *options,
+                    nk_solver **out);
+nk_status nk_destroy(nk_solver *solver);
+const char *nk_last_error(const nk_solver *solver);
+
+/* Change physics or options of an existing handle (same grid). */
+nk_status nk_set_physics(nk_solver *solver, const nk_physics *physics);
+nk_status nk_set_options(nk_solver *solver, const nk_solver_options *options);
+
+/* Field extents (nx, ny, nz) of a field id on this handle's grid. */
+nk_status nk_field_shape(const nk_solver *solver, nk_field_id id, int32_t shape[3]);
+/* Copy a host array into / out of a device field (FP64), synchronously. */
+nk_status nk_set_field(nk_solver *solver, nk_field_id id, const double *data, int64_t ld, int64_t lds);
+nk_status nk_get_field(nk_solver *solver, nk_field_id id, double *data, int64_t ld, int64_t lds);
+nk_status nk_fill_field(nk_solver *solver, nk_field_id id, double value);
+/* Face classes (NK_CLASS_* bits) of the last nk_compute_driving_stress. */
+nk_status nk_get_classes(nk_solver *solver, uint8_t *classu, int64_t ldu, uint8_t *classv, int64_t ldv);
+/* Device address, pitch (elements between rows) and layer stride of an FP64 field. */
+nk_status nk_field_device_pointer(nk_solver *solver, nk_field_id id, double **ptr, int64_t *pitch, int64_t *layer);
+/* hipStream_t of the handle, as void*. */
+void *nk_stream(nk_solver *solver);
+nk_status nk_synchronize(nk_solver *solver);
+
+/* update_geometry!: s, φ, φᵘ, φᵛ, N, μ from H and bed (src/Models/ice_sheet_model_methods.jl:103-111). */
+nk_status nk_update_geometry(nk_solver *solver);
+/* compute_driving_stress!(τ, H, s, ρᵢ, ρw, g, μ, b) (src/StressBalances/driving_stress.jl:43-53); the bed is used
+ * for the grounding-line split only with NK_GROUNDING_SUBGRID_DRIVING_STRESS, μ only when masking is on. */
+nk_status nk_compute_driving_stress(nk_solver *solver);
+/* update_stress_balance_coefficients! (SSA) or update_diva_coefficients! (DIVA) at the current (u, v). */
+nk_status nk_update_coefficients(nk_solver *solver);
+/* ssa_operator!(q, p, η̄H, η̄Hᶠᶠ, βᵘ, βᵛ, walls): (QU, QV) = A (PU, PV). */
+nk_status nk_apply_operator(nk_solver *solver);
+/* precondition!(z, P, r, coefficients...): (QU, QV) = M⁻¹ (PU, PV) with the handle's preconditioner; the multigrid
+ * coarse coefficients are rebuilt first (update_preconditioner!). */
+nk_status nk_precondition(nk_solver *solver);
+/* solve!(x, solver, b, coefficients...): PCG on (U, V) with right-hand side (TAUU, TAUV), frozen coefficients. */
+nk_status nk_solve(nk_solver *solver, nk_info *info);
+/* picard_solve!(x, picard, solver, update!, τ, coefficients...) with the current τ. */
+nk_status nk_picard_solve(nk_solver *solver, nk_info *info);
+/* update_velocities!(model, solver, ::SSA or ::DIVA): driving stress, Picard, then extend_depth_average! (SSA) or
+ * compute_velocity_profile! (DIVA). With geometry != 0 update_geometry! runs first. */
+nk_status nk_update_velocities(nk_solver *solver, int32_t geometry, nk_info *info);
+/* Arrhenius rate factor A on the σ grid from TEMPERATURE via the enthalpy round trip of set!(model; T)
+ * (src/Models/ice_sheet_model_methods.jl:47-57, src/Thermodynamics/column_energy_solve.jl:105-117,
+ * src/Rheologies/arrhenius_rate_factor.jl:102-110) with the reference's default parameters. */
+nk_status nk_rate_factor_from_temperature(nk_solver *solver);
+
+/* Multigrid level shapes of the handle (StressBalanceMultigridPreconditioner(grid).levels). Writes up to
+ * max_levels (Nx, Ny) pairs; returns the level count in *levels. */
+nk_status nk_multigrid_levels(const nk_solver *solver, int32_t *levels, int32_t *shapes, int32_t max_levels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUNATAK_HIP_H */
