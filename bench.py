@@ -1,0 +1,192 @@
+"""Benchmark: velocity solves/s at 1024² (DIVA, MG-PCG), mixed precision vs FP64, on one MI355X.
+
+A step is one full velocity solve (update_velocities!: driving stress, Picard with DIVA coefficient updates and
+multigrid-preconditioned CG, velocity-profile reconstruction) of the 1024 × 1024 floating-channel workload
+(config 3 of the benchmark set), from u = v = 0 and zero DIVA state, with every input already in HBM.
+`value` is the mixed-precision form's solves/s; the FP64 form's solves/s and the mixed result's relative L2 error
+against it are reported beside it.
+
+Roofline: the dominant kernel of the mixed solve is the fused FP32 ℓ1-Jacobi sweep on the finest grid. Its
+launches are bracketed with HIP events on the solver's own stream (nk_solver_options.profile_kernel) during one
+extra solve; algorithmic bytes per launch = Nx·Ny·(10·4 + 1) (read x, b: 4 arrays; η̄H, η̄Hᶠᶠ, βᵘ, βᵛ: 4 arrays;
+activity flags: 1 byte; write x: 2 arrays).
+
+CPU baseline: the CPU oracle (plain C, one thread) timed on a bounded sample of the same workload — one DIVA
+coefficient update and three MG-PCG iterations at 1024² — and scaled to one solve with the FP64 GPU run's Picard
+and CG counts.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nunatak.jl_amd"))
+
+from nunatak_hip import abi, workloads  # noqa: E402
+from nunatak_hip.model import HIPModel  # noqa: E402
+
+METRIC = "velocity solves/s @1024² mixed vs FP64 at rel err≤1e-8; served p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+STATE = ("U", "V", "ETASIGMA", "F2", "BETABU", "BETABV", "UB", "VB")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make(N, precision, profile=0, check_interval=8):
+    g, phys, opts, fields = workloads.diva_shelf(N=N, dx=512e3 / N, precision=precision)
+    opts.profile_kernel = profile
+    opts.check_interval = check_interval
+    m = HIPModel(g, phys, opts)
+    workloads.load_into(m, fields)
+    return m, (g, phys, opts, fields)
+
+
+def reset_state(m):
+    for name in STATE:
+        m.fill(name, 0.0)
+
+
+def timed_solves(m, warmup, steps, label):
+    infos = []
+    for k in range(warmup):
+        reset_state(m)
+        info = m.update_velocities(geometry=True)
+        log(f"[bench] {label} warmup {k + 1}/{warmup}: {info['elapsed_ms']:.1f} ms, picard {info['picard_iterations']}, "
+            f"cg {info['total_linear_iterations']}")
+    m.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        reset_state(m)
+        infos.append(m.update_velocities(geometry=True))
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    for k, info in enumerate(infos):
+        log(f"[bench] {label} step {k + 1}/{steps}: {info['elapsed_ms']:.1f} ms, picard {info['picard_iterations']}, "
+            f"cg {info['total_linear_iterations']}, refinement {info['refinement_steps']}")
+    return wall, infos
+
+
+def cpu_baseline(setup, fp64_info):
+    """Oracle (C, 1 thread) on a bounded sample of the workload, scaled to one solve."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    g, phys, opts, fields = setup
+    o_opts = abi.default_options(preconditioner=abi.PRECONDITIONER_MULTIGRID, linear=abi.fixed_iterations(3))
+    o = oracle.OracleModel(g, phys, o_opts)
+    workloads.load_into(o, fields)
+    o.update_geometry()
+    o.compute_driving_stress()
+    t0 = time.perf_counter()
+    o.update_coefficients()
+    t_coef = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.solve()
+    t_solve3 = time.perf_counter() - t0
+    o_opts0 = abi.default_options(preconditioner=abi.PRECONDITIONER_MULTIGRID, linear=abi.fixed_iterations(0))
+    o0 = oracle.OracleModel(g, phys, o_opts0)
+    workloads.load_into(o0, fields)
+    o0.update_geometry()
+    o0.compute_driving_stress()
+    o0.update_coefficients()
+    t0 = time.perf_counter()
+    o0.solve()
+    t_setup = time.perf_counter() - t0
+    t_iter = max(t_solve3 - t_setup, 0.0) / 3.0
+    picard = fp64_info["picard_iterations"]
+    cg = fp64_info["total_linear_iterations"]
+    t_solve = picard * (t_coef + t_setup) + cg * t_iter
+    log(f"[bench] cpu oracle: coefficient update {t_coef:.2f} s, solve setup {t_setup:.2f} s, CG iteration {t_iter:.2f} s "
+        f"→ {t_solve:.1f} s per solve ({picard} Picard, {cg} CG iterations)")
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle at 1024²: one DIVA coefficient update ({t_coef:.2f} s), MG-PCG setup ({t_setup:.2f} s) and "
+                      f"3 MG-PCG iterations ({t_iter:.2f} s each), scaled by the FP64 GPU solve's {picard} Picard and "
+                      f"{cg} CG iterations"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the velocity solve of one grid is not "
+              "sharded across GPUs in this build.", file=sys.stderr)
+        sys.exit(2)
+    N, K, Wm = args.size, args.steps, args.warmup
+
+    log(f"[bench] building FP64 and mixed handles at {N}x{N}")
+    results = {}
+    models = {}
+    for label, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
+        if args.only and args.only != label:
+            continue
+        m, setup = make(N, prec)
+        models[label] = (m, setup)
+        wall, infos = timed_solves(m, Wm, K, label)
+        results[label] = {"wall": wall, "infos": infos, "u": m.get("U"), "v": m.get("V")}
+
+    out = {"metric": METRIC, "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": Wm, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "data": "synthetic (seeded numpy PCG64 generator of config 3)"}
+    if "fp64" in results and "mixed" in results:
+        a = np.concatenate([results["mixed"]["u"].ravel(), results["mixed"]["v"].ravel()])
+        b = np.concatenate([results["fp64"]["u"].ravel(), results["fp64"]["v"].ravel()])
+        err = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    else:
+        err = None
+    main_label = "mixed" if "mixed" in results else "fp64"
+    r = results[main_label]
+    out["value"] = K / r["wall"]
+    out["ms_per_step"] = 1e3 * r["wall"] / K
+    out["dtype"] = "f32+f64" if main_label == "mixed" else "f64"
+    info_last = r["infos"][-1]
+    out["config"] = {"workload": f"config3 DIVA floating channel {N}x{N} dx={512 / N:g} km, Nsigma=10, Arrhenius A, "
+                                 f"MG-PCG (PointJacobi V(2,2)), Picard 1e-6, linear 1e-8",
+                     "grid": [N, N], "precision": main_label,
+                     "picard_iterations": info_last["picard_iterations"],
+                     "cg_iterations": info_last["total_linear_iterations"]}
+    if "fp64" in results:
+        f = results["fp64"]
+        out["fp64_solves_per_s"] = K / f["wall"]
+        out["fp64_ms_per_solve"] = 1e3 * f["wall"] / K
+        out["fp64_picard_iterations"] = f["infos"][-1]["picard_iterations"]
+        out["fp64_cg_iterations"] = f["infos"][-1]["total_linear_iterations"]
+    if err is not None:
+        out["mixed_rel_l2_err_vs_fp64"] = err
+        out["mixed_speedup_vs_fp64"] = out["value"] / out["fp64_solves_per_s"]
+
+    # roofline of the dominant kernel: one more solve with event brackets on the FP32 (or FP64) sweep
+    m, setup = models[main_label]
+    opts = setup[2]
+    opts.profile_kernel = abi.PROFILE_SWEEP
+    opts.check_interval = 1
+    m.set_options(opts)
+    reset_state(m)
+    pinfo = m.update_velocities(geometry=True)
+    launches = max(pinfo["profiled_launches"], 1)
+    dur_s = pinfo["profiled_ms"] / launches / 1e3
+    T = pinfo["profiled_precision"] or 8
+    bytes_per_launch = N * N * (10 * T + 1)
+    achieved = bytes_per_launch / dur_s / 1e9 if dur_s > 0 else 0.0
+    log(f"[bench] profiled sweep: {launches} launches, avg {dur_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": f"k_sweep<{'float' if T == 4 else 'double'}> (finest grid)",
+                       "avg_launch_us": dur_s * 1e6, "launches": launches, "algorithmic_bytes_per_launch": bytes_per_launch}
+    if not args.no_cpu_baseline and "fp64" in results:
+        out["cpu_baseline"] = cpu_baseline(models["fp64"][1], results["fp64"]["infos"][-1])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -116,7 +116,17 @@ typedef struct nk_solver_options {
     double inner_reltol;              /* mixed: relative reduction asked of each FP32 inner solve */
     int32_t inner_maxiter;            /* mixed: FP32 PCG iterations per refinement step */
     int32_t check_interval;           /* PCG iterations launched between host checks of the on-device stop flag */
+    int32_t profile_kernel;           /* NK_PROFILE_*: bracket every finest-level launch of that kernel with HIP events
+                                         on the handle's stream and report the summed device time in nk_info */
 } nk_solver_options;
+
+/* Kernels that nk_solver_options.profile_kernel can time (finest grid only). */
+enum {
+    NK_PROFILE_NONE = 0,
+    NK_PROFILE_OPERATOR = 1,   /* q = A p of the PCG iteration */
+    NK_PROFILE_SWEEP = 2,      /* fused ℓ1-Jacobi sweep x + (b − A x)/s (not the first sweep from zero) */
+    NK_PROFILE_UPDATE = 3      /* PCG vector update x += αp, r −= αq (+ Jacobi z = r/d) */
+};
 
 /* Field ids. Location (x, y[, σ]) in brackets; Face on a Bounded direction has N + 1 points. */
 typedef enum nk_field_id {
@@ -173,6 +183,11 @@ typedef struct nk_info {
     double last_picard_norm;        /* X = ‖x‖ */
     double last_residual_norm;      /* ‖r‖ at the end of the last linear solve */
     double elapsed_ms;              /* device time of the call, from HIP events */
+    double profiled_ms;             /* summed device time of the profiled kernel's launches (profile_kernel) */
+    int64_t profiled_launches;      /* launches of the profiled kernel */
+    int64_t profiled_points;        /* Σ Nx·Ny of those launches (finest grid) */
+    int32_t profiled_precision;     /* bytes per element of the profiled launches (8 FP64, 4 FP32) */
+    int32_t pad2;
 } nk_info;
 
 typedef struct nk_solver nk_solver;

@@ -1,0 +1,709 @@
+// nk_device.hpp — gfx950 device code of libnunatak_hip.so: stencil kernels of the velocity solve.
+//
+// Layout in HBM. Every horizontal field of a grid level is one pitched 2D array of (Ny + 1) rows of `pitch`
+// elements (pitch = Nx + 1 rounded up to 64 elements = 512 B in FP64), element (i, j) (1-based, i fastest) at
+// (j - 1) * pitch + (i - 1); σ fields stack Nσ such planes. There are no halo cells: neighbour indices are mapped
+// in the kernels with the reference's halo semantics (Periodic wraps; Bounded Center clamps, which equals
+// Oceananigans' zero-gradient halo fill; Face values beyond a Bounded wall are peripheral and never used unmasked).
+//
+// Arithmetic. Kernels are compiled with -ffp-contract=off and keep the reference's expression order, so the
+// FP64 operator, masks, grounded fractions, driving stress and multigrid transfers reproduce the reference's
+// IEEE results operation for operation. Reductions (dot products) are deterministic: each block writes a partial
+// in a fixed slot and the last block to finish sums the slots in a fixed order, then runs a "finisher" that
+// updates the PCG scalars on the device, so no host round trip is needed inside an iteration.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nk {
+
+constexpr int BX = 64;   // threads along x per block: one wavefront per row segment
+constexpr int BY = 4;    // rows per block
+constexpr int NT = BX * BY;
+constexpr int MAXRED = 4;  // values reduced by one kernel at most
+
+struct KG {
+    int Nx, Ny, tx, ty, pitch;
+    double dx, dy;
+};
+
+__device__ __forceinline__ int wrapi(int i, int N) {
+    if (i >= 1 && i <= N) return i;
+    int r = (i - 1) % N;
+    return (r < 0 ? r + N : r) + 1;
+}
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+__device__ __forceinline__ long off(const KG &g, int i, int j) { return (long)(j - 1) * g.pitch + (i - 1); }
+// index maps by location (Center / Face) along x and y
+__device__ __forceinline__ int cx(const KG &g, int i) { return g.tx ? clampi(i, 1, g.Nx) : wrapi(i, g.Nx); }
+__device__ __forceinline__ int fx(const KG &g, int i) { return g.tx ? clampi(i, 1, g.Nx + 1) : wrapi(i, g.Nx); }
+__device__ __forceinline__ int cy(const KG &g, int j) { return g.ty ? clampi(j, 1, g.Ny) : wrapi(j, g.Ny); }
+__device__ __forceinline__ int fy(const KG &g, int j) { return g.ty ? clampi(j, 1, g.Ny + 1) : wrapi(j, g.Ny); }
+__device__ __forceinline__ int nxf(const KG &g) { return g.Nx + g.tx; }
+__device__ __forceinline__ int nyf(const KG &g) { return g.Ny + g.ty; }
+
+// peripheral_node(i, j, k, grid, LX, LY, Center()) (Oceananigans Grids/inactive_node.jl)
+__device__ __forceinline__ bool periph_fc(const KG &g, int i, int j) {
+    return (g.tx && (i <= 1 || i >= g.Nx + 1)) || (g.ty && (j < 1 || j > g.Ny));
+}
+__device__ __forceinline__ bool periph_cf(const KG &g, int i, int j) {
+    return (g.ty && (j <= 1 || j >= g.Ny + 1)) || (g.tx && (i < 1 || i > g.Nx));
+}
+__device__ __forceinline__ bool periph_ff(const KG &g, int i, int j) {
+    return (g.tx && (i <= 1 || i >= g.Nx + 1)) || (g.ty && (j <= 1 || j >= g.Ny + 1));
+}
+
+// Julia Base.max / min / clamp on Float64 (signed zeros as Julia orders them).
+__device__ __forceinline__ double jmax(double x, double y) { double d = x - y; return (isnan(x) || isnan(y)) ? d : (signbit(d) ? y : x); }
+__device__ __forceinline__ double jmin(double x, double y) { double d = x - y; return (isnan(x) || isnan(y)) ? d : (signbit(d) ? x : y); }
+__device__ __forceinline__ double jclamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
+
+// ---------------------------------------------------------------------------------------------------------------
+// Activity flags: bit 0 = u point active, bit 1 = v point active (not a wall, not ice free on both sides).
+// inactiveᶠᶜᶜ / inactiveᶜᶠᶜ of src/StressBalances/ssa_operator.jl:91-92 evaluated once per coefficient update.
+// ---------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool uact(const KG &g, const uint8_t *act, int i, int j) {
+    if (g.tx) { if (i < 1 || i > g.Nx + 1) return false; } else i = wrapi(i, g.Nx);
+    if (g.ty) { if (j < 1 || j > g.Ny) return false; } else j = wrapi(j, g.Ny);
+    return act[off(g, i, j)] & 1;
+}
+__device__ __forceinline__ bool vact(const KG &g, const uint8_t *act, int i, int j) {
+    if (g.tx) { if (i < 1 || i > g.Nx) return false; } else i = wrapi(i, g.Nx);
+    if (g.ty) { if (j < 1 || j > g.Ny + 1) return false; } else j = wrapi(j, g.Ny);
+    return (act[off(g, i, j)] >> 1) & 1;
+}
+// interior_u / interior_v with the stiffness mask (ssa_operator.jl:94-95)
+template <typename T> __device__ __forceinline__ T iu(const KG &g, const uint8_t *act, const T *u, int i, int j) {
+    return uact(g, act, i, j) ? u[off(g, fx(g, i), cy(g, j))] : T(0);
+}
+template <typename T> __device__ __forceinline__ T iv(const KG &g, const uint8_t *act, const T *v, int i, int j) {
+    return vact(g, act, i, j) ? v[off(g, cx(g, i), fy(g, j))] : T(0);
+}
+// interior_u / interior_v without a stiffness (walls only), as the viscosity sees the velocity
+template <typename T> __device__ __forceinline__ T iu_w(const KG &g, const T *u, int i, int j) {
+    return periph_fc(g, i, j) ? T(0) : u[off(g, fx(g, i), cy(g, j))];
+}
+template <typename T> __device__ __forceinline__ T iv_w(const KG &g, const T *v, int i, int j) {
+    return periph_cf(g, i, j) ? T(0) : v[off(g, cx(g, i), fy(g, j))];
+}
+template <typename T> __device__ __forceinline__ T ldc(const KG &g, const T *f, int i, int j) { return f[off(g, cx(g, i), cy(g, j))]; }
+template <typename T> __device__ __forceinline__ T ldfc(const KG &g, const T *f, int i, int j) { return f[off(g, fx(g, i), cy(g, j))]; }
+template <typename T> __device__ __forceinline__ T ldcf(const KG &g, const T *f, int i, int j) { return f[off(g, cx(g, i), fy(g, j))]; }
+template <typename T> __device__ __forceinline__ T ldff(const KG &g, const T *f, int i, int j) { return f[off(g, fx(g, i), fy(g, j))]; }
+
+template <typename T> struct Coef {
+    const T *eta, *etaff, *bu, *bv;
+    const uint8_t *act;
+};
+
+// ---------------------------------------------------------------------------------------------------------------
+// The SSA operator at one (i, j): q = Az β p − ∇·T (src/StressBalances/ssa_operator.jl:60-89), with the
+// reference's expression order. Centres (i, j), (i−1, j), (i, j−1) and corners (i, j), (i, j+1), (i+1, j).
+// ---------------------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu, const T *pv, const Coef<T> &c, T wf,
+                                          T &qu, T &qv) {
+    const T dx = (T)g.dx, dy = (T)g.dy, Az = (T)(g.dx * g.dy);
+    const uint8_t *a = c.act;
+    const T u00 = iu(g, a, pu, i, j), u10 = iu(g, a, pu, i + 1, j), um0 = iu(g, a, pu, i - 1, j);
+    const T u0m = iu(g, a, pu, i, j - 1), u0p = iu(g, a, pu, i, j + 1), u1m = iu(g, a, pu, i + 1, j - 1);
+    const T v00 = iv(g, a, pv, i, j), v01 = iv(g, a, pv, i, j + 1), vm0 = iv(g, a, pv, i - 1, j);
+    const T vm1 = iv(g, a, pv, i - 1, j + 1), v0m = iv(g, a, pv, i, j - 1), v10 = iv(g, a, pv, i + 1, j);
+    // strain rates at the three centres
+    const T exx_c = (u10 - u00) / dx, eyy_c = (v01 - v00) / dy;      // (i, j)
+    const T exx_w = (u00 - um0) / dx, eyy_w = (vm1 - vm0) / dy;      // (i-1, j)
+    const T exx_s = (u1m - u0m) / dx, eyy_s = (v00 - v0m) / dy;      // (i, j-1)
+    const T e_c = ldc(g, c.eta, i, j), e_w = ldc(g, c.eta, i - 1, j), e_s = ldc(g, c.eta, i, j - 1);
+    const T txx_c = T(2) * e_c * (T(2) * exx_c + eyy_c), txx_w = T(2) * e_w * (T(2) * exx_w + eyy_w);
+    const T tyy_c = T(2) * e_c * (T(2) * eyy_c + exx_c), tyy_s = T(2) * e_s * (T(2) * eyy_s + exx_s);
+    // shear strain at the three corners
+    T exy_00 = ((u00 - u0m) / dy + (v00 - vm0) / dx) / T(2);
+    T exy_01 = ((u0p - u00) / dy + (v01 - vm1) / dx) / T(2);
+    T exy_10 = ((u10 - u1m) / dy + (v10 - v00) / dx) / T(2);
+    if (periph_ff(g, i, j)) exy_00 = wf * exy_00;
+    if (periph_ff(g, i, j + 1)) exy_01 = wf * exy_01;
+    if (periph_ff(g, i + 1, j)) exy_10 = wf * exy_10;
+    const T txy_00 = T(2) * ldff(g, c.etaff, i, j) * exy_00;
+    const T txy_01 = T(2) * ldff(g, c.etaff, i, j + 1) * exy_01;
+    const T txy_10 = T(2) * ldff(g, c.etaff, i + 1, j) * exy_10;
+    const long o = off(g, i, j);
+    const T pu0 = pu[o], pv0 = pv[o];
+    const T divu = (dy * txx_c - dy * txx_w) + (dx * txy_01 - dx * txy_00);
+    const T divv = (dx * tyy_c - dx * tyy_s) + (dy * txy_10 - dy * txy_00);
+    const T qu_ = Az * c.bu[o] * pu0 - divu;
+    const T qv_ = Az * c.bv[o] * pv0 - divv;
+    const uint8_t f = a[o];
+    qu = (f & 1) ? qu_ : pu0;
+    qv = (f & 2) ? qv_ : pv0;
+}
+
+// Operator diagonal (src/StressBalances/diagonal_preconditioner.jl:20-44)
+template <typename T> __device__ __forceinline__ T corner_factor(const KG &g, int i, int j, T wf) { return periph_ff(g, i, j) ? wf : T(1); }
+template <typename T> __device__ __forceinline__ T x_nd(const KG &g, const T *eta, int i, int j) { return T(4) * (T)g.dy * ldc(g, eta, i, j) / (T)g.dx; }
+template <typename T> __device__ __forceinline__ T y_nd(const KG &g, const T *eta, int i, int j) { return T(4) * (T)g.dx * ldc(g, eta, i, j) / (T)g.dy; }
+template <typename T> __device__ __forceinline__ T x_sd(const KG &g, const T *ff, int i, int j, T wf) { return corner_factor(g, i, j, wf) * (T)g.dx * ldff(g, ff, i, j) / (T)g.dy; }
+template <typename T> __device__ __forceinline__ T y_sd(const KG &g, const T *ff, int i, int j, T wf) { return corner_factor(g, i, j, wf) * (T)g.dy * ldff(g, ff, i, j) / (T)g.dx; }
+
+template <typename T> __device__ __forceinline__ T x_diag(const KG &g, const Coef<T> &c, int i, int j, T wf) {
+    T d = (T)(g.dx * g.dy) * c.bu[off(g, i, j)] + x_nd(g, c.eta, i, j) + x_nd(g, c.eta, i - 1, j) + x_sd(g, c.etaff, i, j, wf) +
+          x_sd(g, c.etaff, i, j + 1, wf);
+    return (!(c.act[off(g, i, j)] & 1) || d == T(0)) ? T(1) : d;
+}
+template <typename T> __device__ __forceinline__ T y_diag(const KG &g, const Coef<T> &c, int i, int j, T wf) {
+    T d = (T)(g.dx * g.dy) * c.bv[off(g, i, j)] + y_nd(g, c.eta, i, j) + y_nd(g, c.eta, i, j - 1) + y_sd(g, c.etaff, i, j, wf) +
+          y_sd(g, c.etaff, i + 1, j, wf);
+    return (!(c.act[off(g, i, j)] & 2) || d == T(0)) ? T(1) : d;
+}
+// ℓ1 row sums (src/StressBalances/multigrid_preconditioner.jl:195-211)
+template <typename T> __device__ __forceinline__ T x_l1(const KG &g, const Coef<T> &c, int i, int j, T wf) {
+    T s = (T)(g.dx * g.dy) * c.bu[off(g, i, j)] + T(2) * (x_nd(g, c.eta, i, j) + x_nd(g, c.eta, i - 1, j)) +
+          T(4) * (ldc(g, c.eta, i, j) + ldc(g, c.eta, i - 1, j)) + T(2) * (x_sd(g, c.etaff, i, j, wf) + x_sd(g, c.etaff, i, j + 1, wf)) +
+          T(2) * (corner_factor(g, i, j, wf) * ldff(g, c.etaff, i, j) + corner_factor(g, i, j + 1, wf) * ldff(g, c.etaff, i, j + 1));
+    return (!(c.act[off(g, i, j)] & 1) || s == T(0)) ? T(1) : s;
+}
+template <typename T> __device__ __forceinline__ T y_l1(const KG &g, const Coef<T> &c, int i, int j, T wf) {
+    T s = (T)(g.dx * g.dy) * c.bv[off(g, i, j)] + T(2) * (y_nd(g, c.eta, i, j) + y_nd(g, c.eta, i, j - 1)) +
+          T(4) * (ldc(g, c.eta, i, j) + ldc(g, c.eta, i, j - 1)) + T(2) * (y_sd(g, c.etaff, i, j, wf) + y_sd(g, c.etaff, i + 1, j, wf)) +
+          T(2) * (corner_factor(g, i, j, wf) * ldff(g, c.etaff, i, j) + corner_factor(g, i + 1, j, wf) * ldff(g, c.etaff, i + 1, j));
+    return (!(c.act[off(g, i, j)] & 2) || s == T(0)) ? T(1) : s;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Device scalars of one PCG solve and the reduction finishers.
+// ---------------------------------------------------------------------------------------------------------------
+struct Scal {
+    double rho, alpha, beta, pq, rr, rz, res, r0, tol;
+    double reltol, abstol;
+    int maxiter, fixed, it, done;
+    double out[MAXRED];
+    double picard_D, picard_X;
+};
+
+enum Fin : int {
+    FIN_NONE = 0,     // store totals in out[]
+    FIN_ALPHA = 1,    // pq → α
+    FIN_BETA_RES = 2, // (r·z, r·r) → β, ρ, res, it, done
+    FIN_INIT = 3,     // (r·z, r·r) → ρ, r₀, tol, res, it = 0, done
+    FIN_RES = 4,      // r·r → res, it, done (multigrid path: β follows after the V-cycle)
+    FIN_BETA = 5,     // r·z → β, ρ
+    FIN_PICARD = 6,   // (Δ², X²) → picard_D, picard_X
+};
+
+__device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
+    switch (fin) {
+    case FIN_ALPHA: S->pq = t[0]; S->alpha = t[0] == 0.0 ? 0.0 : S->rho / t[0]; break;
+    case FIN_BETA_RES: {
+        double rz = t[0];
+        S->beta = S->rho == 0.0 ? 0.0 : rz / S->rho;
+        S->rho = rz;
+        S->rr = t[1];
+        S->res = sqrt(t[1]);
+        S->it += 1;
+        S->done = S->fixed ? (S->it >= S->maxiter) : !(S->res > S->tol && S->it < S->maxiter);
+        break;
+    }
+    case FIN_INIT: {
+        S->rho = t[0];
+        S->rr = t[1];
+        S->r0 = sqrt(t[1]);
+        double a = S->reltol * S->r0, b = S->abstol;
+        S->tol = jmax(a, b);
+        S->res = S->r0;
+        S->it = 0;
+        S->done = S->fixed ? (S->maxiter <= 0) : !(S->res > S->tol && 0 < S->maxiter);
+        break;
+    }
+    case FIN_RES:
+        S->rr = t[0];
+        S->res = sqrt(t[0]);
+        S->it += 1;
+        S->done = S->fixed ? (S->it >= S->maxiter) : !(S->res > S->tol && S->it < S->maxiter);
+        break;
+    case FIN_BETA: S->beta = S->rho == 0.0 ? 0.0 : t[0] / S->rho; S->rho = t[0]; break;
+    case FIN_PICARD: S->picard_D = sqrt(t[0]); S->picard_X = sqrt(t[1]); break;
+    default: break;
+    }
+    for (int k = 0; k < MAXRED; k++) S->out[k] = t[k];
+}
+
+struct Red {
+    double *partials;       // MAXRED * capacity slots
+    unsigned int *counter;  // zero between kernels
+    Scal *S;
+    int fin;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Block-reduce NV values, then the last block sums every block's slot in block order and runs the finisher.
+template <int NV> __device__ void reduce_finish(double (&v)[NV], const Red &r) {
+    __shared__ double sh[NV][NT / 64];
+    __shared__ int last;
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int nwaves = (blockDim.x * blockDim.y) >> 6;
+    const int nblocks = gridDim.x * gridDim.y;
+    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][wave] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int w = 0; w < nwaves; w++) s += sh[k][w];
+            r.partials[(long)k * nblocks + bid] = s;
+        }
+        __threadfence();
+        unsigned int prev = atomicAdd(r.counter, 1u);
+        last = (prev == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double tot[MAXRED] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = 0.0;
+        for (int b = tid; b < nblocks; b += blockDim.x * blockDim.y)
+            s += __hip_atomic_load(&r.partials[(long)k * nblocks + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s = wave_sum(s);
+        __syncthreads();
+        if (lane == 0) sh[k][wave] = s;
+        __syncthreads();
+        double t = 0.0;
+        for (int w = 0; w < nwaves; w++) t += sh[k][w];
+        tot[k] = t;
+    }
+    if (tid == 0) {
+        finish(r.S, r.fin, tot);
+        *r.counter = 0u;
+        __threadfence();
+    }
+}
+
+__device__ __forceinline__ bool solve_done(const Scal *S) { return S != nullptr && S->done; }
+
+#define NK_IJ                                                 \
+    const int i = blockIdx.x * BX + threadIdx.x + 1;          \
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+
+// ---------------------------------------------------------------------------------------------------------------
+// PCG kernels
+// ---------------------------------------------------------------------------------------------------------------
+
+// q = A p (+ p·q → α). With b != nullptr writes r = b − A p instead (initial residual, iterative refinement).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_apply(KG g, const T *__restrict__ pu, const T *__restrict__ pv, Coef<T> c, T wf,
+                                              T *__restrict__ qu, T *__restrict__ qv, const T *__restrict__ bu,
+                                              const T *__restrict__ bv, Red red, int nred, const Scal *gate) {
+    if (solve_done(gate)) return;
+    NK_IJ
+    double acc[1] = {0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        T a, b;
+        ssa_apply(g, i, j, pu, pv, c, wf, a, b);
+        const long o = off(g, i, j);
+        if (bu) {
+            a = T(1) * bu[o] + T(-1) * a;
+            b = T(1) * bv[o] + T(-1) * b;
+        }
+        qu[o] = a;
+        qv[o] = b;
+        if (nred) acc[0] = (double)pu[o] * (double)a + (double)pv[o] * (double)b;
+    }
+    if (nred) reduce_finish<1>(acc, red);
+}
+
+// Jacobi step of the PCG with the diagonal preconditioner:
+//   x += α p; r += (−α) q; z = r / d; (r·z, r·r) → β, ρ, res, done.  With init != 0: z = r / d, p = z, (r·z, r·r).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int precond, T *__restrict__ xu, T *__restrict__ xv,
+                                                    const T *__restrict__ pu, const T *__restrict__ pv, const T *__restrict__ qu,
+                                                    const T *__restrict__ qv, T *__restrict__ ru, T *__restrict__ rv,
+                                                    T *__restrict__ zu, T *__restrict__ zv, T *__restrict__ pu_out,
+                                                    T *__restrict__ pv_out, int init, Red red, const Scal *S) {
+    if (solve_done(S)) return;
+    NK_IJ
+    double acc[2] = {0.0, 0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        T r_u, r_v;
+        if (init) {
+            r_u = ru[o];
+            r_v = rv[o];
+        } else {
+            const T al = (T)S->alpha;
+            xu[o] = T(1) * xu[o] + al * pu[o];
+            xv[o] = T(1) * xv[o] + al * pv[o];
+            r_u = T(1) * ru[o] + (-al) * qu[o];
+            r_v = T(1) * rv[o] + (-al) * qv[o];
+            ru[o] = r_u;
+            rv[o] = r_v;
+        }
+        T z_u, z_v;
+        if (precond == 1) {
+            z_u = r_u / x_diag(g, c, i, j, wf);
+            z_v = r_v / y_diag(g, c, i, j, wf);
+        } else {
+            z_u = T(1) * r_u + T(0) * r_u;
+            z_v = T(1) * r_v + T(0) * r_v;
+        }
+        zu[o] = z_u;
+        zv[o] = z_v;
+        if (init) {
+            pu_out[o] = z_u;
+            pv_out[o] = z_v;
+        }
+        acc[0] = (double)r_u * (double)z_u + (double)r_v * (double)z_v;
+        acc[1] = (double)r_u * (double)r_u + (double)r_v * (double)r_v;
+    }
+    reduce_finish<2>(acc, red);
+}
+
+// Multigrid PCG: x += α p; r += (−α) q; r·r → res, it, done.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_update_xr(KG g, T *__restrict__ xu, T *__restrict__ xv, const T *__restrict__ pu,
+                                                  const T *__restrict__ pv, const T *__restrict__ qu, const T *__restrict__ qv,
+                                                  T *__restrict__ ru, T *__restrict__ rv, Red red, const Scal *S) {
+    if (solve_done(S)) return;
+    NK_IJ
+    double acc[1] = {0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        const T al = (T)S->alpha;
+        xu[o] = T(1) * xu[o] + al * pu[o];
+        xv[o] = T(1) * xv[o] + al * pv[o];
+        const T a = T(1) * ru[o] + (-al) * qu[o], b = T(1) * rv[o] + (-al) * qv[o];
+        ru[o] = a;
+        rv[o] = b;
+        acc[0] = (double)a * (double)a + (double)b * (double)b;
+    }
+    reduce_finish<1>(acc, red);
+}
+
+// p = z + β p (or p = z when init)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_update_p(KG g, const T *__restrict__ zu, const T *__restrict__ zv, T *__restrict__ pu,
+                                                 T *__restrict__ pv, int init, const Scal *S) {
+    if (solve_done(S) && !init) return;
+    NK_IJ
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        if (init) {
+            pu[o] = zu[o];
+            pv[o] = zv[o];
+        } else {
+            const T be = (T)S->beta;
+            pu[o] = T(1) * zu[o] + be * pu[o];
+            pv[o] = T(1) * zv[o] + be * pv[o];
+        }
+    }
+}
+
+// Σ over :xyz of a·b (+ c·d) → finisher
+template <typename T>
+__global__ __launch_bounds__(NT) void k_dot(KG g, const T *__restrict__ au, const T *__restrict__ av, const T *__restrict__ bu,
+                                            const T *__restrict__ bv, const T *__restrict__ cu, const T *__restrict__ cv,
+                                            Red red, const Scal *gate) {
+    if (solve_done(gate)) return;
+    NK_IJ
+    double acc[2] = {0.0, 0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        acc[0] = (double)au[o] * (double)bu[o] + (double)av[o] * (double)bv[o];
+        if (cu) acc[1] = (double)cu[o] * (double)cu[o] + (double)cv[o] * (double)cv[o];
+    }
+    reduce_finish<2>(acc, red);
+}
+
+// Picard bookkeeping: x⁻ = 1·x⁻ + (−1)·x, then (x⁻·x⁻, x·x) → Δ, X.
+__global__ __launch_bounds__(NT) void k_picard_change(KG g, double *__restrict__ mu_, double *__restrict__ mv_,
+                                                      const double *__restrict__ xu, const double *__restrict__ xv, Red red) {
+    NK_IJ
+    double acc[2] = {0.0, 0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        const double a = 1.0 * mu_[o] + (-1.0) * xu[o], b = 1.0 * mv_[o] + (-1.0) * xv[o];
+        mu_[o] = a;
+        mv_[o] = b;
+        acc[0] = a * a + b * b;
+        acc[1] = xu[o] * xu[o] + xv[o] * xv[o];
+    }
+    reduce_finish<2>(acc, red);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Multigrid kernels
+// ---------------------------------------------------------------------------------------------------------------
+
+// Fused ℓ1-Jacobi sweep x_out = x + (b − A x) / s (src/StressBalances/multigrid_preconditioner.jl:165-219).
+// from_zero: x = 0, so A x = 0 and x_out = 0 + (b − 0) / s exactly as the reference computes it.
+// With nred: Σ b·x_out → finisher (the r·z of the PCG at the finest level).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c, T wf,
+                                              const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                              T *__restrict__ ov, int from_zero, Red red, int nred, const Scal *gate) {
+    if (solve_done(gate)) return;
+    NK_IJ
+    double acc[1] = {0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        T qa, qb, x_u, x_v;
+        if (from_zero) {
+            qa = T(0); qb = T(0); x_u = T(0); x_v = T(0);
+        } else {
+            ssa_apply(g, i, j, xu, xv, c, wf, qa, qb);
+            x_u = xu[o];
+            x_v = xv[o];
+        }
+        const T nu = x_u + (bu[o] - qa) / x_l1(g, c, i, j, wf);
+        const T nv = x_v + (bv[o] - qb) / y_l1(g, c, i, j, wf);
+        ou[o] = nu;
+        ov[o] = nv;
+        if (nred) acc[0] = (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
+    }
+    if (nred) reduce_finish<1>(acc, red);
+}
+
+// Thomas algorithm of ℓ1 line Jacobi along y (multigrid_preconditioner.jl:221-259): one thread per x index.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_line_relax(KG g, T *__restrict__ xu, T *__restrict__ xv, const T *__restrict__ bu,
+                                                   const T *__restrict__ bv, T *__restrict__ qu, T *__restrict__ qv,
+                                                   T *__restrict__ t, Coef<T> c, T wf, const Scal *gate) {
+    if (solve_done(gate)) return;
+    const int i = blockIdx.x * NT + threadIdx.x + threadIdx.y * BX + 1;
+    if (i > g.Nx) return;
+    const int Ny = g.Ny;
+    for (int comp = 0; comp < 2; comp++) {
+        T *x = comp == 0 ? xu : xv;
+        const T *b = comp == 0 ? bu : bv;
+        T *q = comp == 0 ? qu : qv;
+        T cc = 0, tj = 0, d = 0;
+        for (int j = 1; j <= Ny; j++) {
+            const T s = comp == 0 ? x_l1(g, c, i, j, wf) : y_l1(g, c, i, j, wf);
+            T cn;
+            if (comp == 0) cn = (uact(g, c.act, i, j) && uact(g, c.act, i, j + 1)) ? x_sd(g, c.etaff, i, j + 1, wf) : T(0);
+            else cn = (vact(g, c.act, i, j) && vact(g, c.act, i, j + 1)) ? y_nd(g, c.eta, i, j) : T(0);
+            cn = (j == Ny) ? T(0) : cn;
+            const T m = s - cc - cn + cc * tj;
+            const long o = off(g, i, j);
+            d = (b[o] - q[o] + cc * d) / m;
+            tj = -cn / m;
+            t[o] = tj;
+            q[o] = d;
+            cc = cn;
+        }
+        T delta = 0;
+        for (int j = Ny; j >= 1; j--) {
+            const long o = off(g, i, j);
+            delta = q[o] - t[o] * delta;
+            x[o] += delta;
+        }
+    }
+}
+
+// restriction_index / restriction_weight / parents (multigrid_preconditioner.jl:264-305)
+__device__ __forceinline__ int r_index(int I, int a, int N, int bounded, int face, int h) {
+    if (!h) return I;
+    if (!bounded) return wrapi(2 * I - 1 + a, N);
+    if (face) { int i = 2 * I - 1 + a; return i > 1 ? i : 1; }
+    return clampi(2 * I - 1 + a, 1, N);
+}
+template <typename T> __device__ __forceinline__ T r_weight(int I, int a, int Nc, int bounded, int face, int h) {
+    if (!h) return T(1);
+    if (face) {
+        T w = (a == 0) ? T(1) : T(1) / T(2);
+        return (bounded && (2 * I - 1 + a < 1)) ? T(0) : w;
+    }
+    T w = ((a == 0) | (a == 1)) ? T(3) * T(1) / T(4) : T(1) / T(4);
+    if (!bounded) return w;
+    const int outside = ((I == 1) & (a == -1)) | ((I == Nc) & (a == 2));
+    const int folded = ((I == 1) & (a == 0)) | ((I == Nc) & (a == 1));
+    w = w + (folded ? T(1) / T(4) : T(0));
+    return outside ? T(0) : w;
+}
+template <typename T>
+__device__ __forceinline__ void parents(int i, int Nc, int bounded, int face, int h, int &I1, int &I2, T &w1, T &w2) {
+    if (!h) { I1 = i; I2 = i; w1 = T(1); w2 = T(0); return; }
+    const int I = (i + 1) / 2, odd = i & 1;
+    if (face) {
+        const int up = bounded ? I + 1 : wrapi(I + 1, Nc);
+        I1 = I; I2 = odd ? I : up; w1 = odd ? T(1) : T(1) / T(2); w2 = odd ? T(0) : T(1) / T(2);
+    } else {
+        const int Ip = odd ? I - 1 : I + 1;
+        I1 = I; I2 = bounded ? clampi(Ip, 1, Nc) : wrapi(Ip, Nc); w1 = T(3) * T(1) / T(4); w2 = T(1) / T(4);
+    }
+}
+
+// _restrict! (multigrid_preconditioner.jl:346-352): coarse b = R (fine r with inactive points zeroed)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_restrict(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
+                                                 const uint8_t *__restrict__ fact, const T *__restrict__ ru,
+                                                 const T *__restrict__ rv, T *__restrict__ bu, T *__restrict__ bv,
+                                                 const Scal *gate) {
+    if (solve_done(gate)) return;
+    const int I = blockIdx.x * BX + threadIdx.x + 1, J = blockIdx.y * BY + threadIdx.y + 1;
+    if (I > cg.Nx || J > cg.Ny) return;
+    const long o = off(cg, I, J);
+    T su = T(0), sv = T(0);
+    // u: Face along x, Center along y
+    for (int a = (hx ? -1 : 0); a <= (hx ? 1 : 0); a++)
+        for (int b = (hy ? -1 : 0); b <= (hy ? 2 : 0); b++) {
+            const int i = r_index(I, a, fg.Nx, fg.tx, 1, hx), j = r_index(J, b, fg.Ny, fg.ty, 0, hy);
+            const T w = r_weight<T>(I, a, cg.Nx, fg.tx, 1, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 0, hy);
+            su += w * iu(fg, fact, ru, i, j);
+        }
+    // v: Center along x, Face along y
+    for (int a = (hx ? -1 : 0); a <= (hx ? 2 : 0); a++)
+        for (int b = (hy ? -1 : 0); b <= (hy ? 1 : 0); b++) {
+            const int i = r_index(I, a, fg.Nx, fg.tx, 0, hx), j = r_index(J, b, fg.Ny, fg.ty, 1, hy);
+            const T w = r_weight<T>(I, a, cg.Nx, fg.tx, 0, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 1, hy);
+            sv += w * iv(fg, fact, rv, i, j);
+        }
+    bu[o] = (cact[o] & 1) ? su : T(0);
+    bv[o] = (cact[o] & 2) ? sv : T(0);
+}
+
+// _prolongate! (multigrid_preconditioner.jl:354-360): fine x += P (coarse x with inactive points zeroed)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_prolongate(KG fg, KG cg, int hx, int hy, const uint8_t *__restrict__ fact,
+                                                   const uint8_t *__restrict__ cact, const T *__restrict__ cu,
+                                                   const T *__restrict__ cv, T *__restrict__ xu, T *__restrict__ xv,
+                                                   const Scal *gate) {
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > fg.Nx || j > fg.Ny) return;
+    const long o = off(fg, i, j);
+    int I1, I2, J1, J2;
+    T wx1, wx2, wy1, wy2;
+    parents<T>(i, cg.Nx, fg.tx, 1, hx, I1, I2, wx1, wx2);
+    parents<T>(j, cg.Ny, fg.ty, 0, hy, J1, J2, wy1, wy2);
+    const T pu = wx1 * (wy1 * iu(cg, cact, cu, I1, J1) + wy2 * iu(cg, cact, cu, I1, J2)) +
+                 wx2 * (wy1 * iu(cg, cact, cu, I2, J1) + wy2 * iu(cg, cact, cu, I2, J2));
+    parents<T>(i, cg.Nx, fg.tx, 0, hx, I1, I2, wx1, wx2);
+    parents<T>(j, cg.Ny, fg.ty, 1, hy, J1, J2, wy1, wy2);
+    const T pv = wx1 * (wy1 * iv(cg, cact, cv, I1, J1) + wy2 * iv(cg, cact, cv, I1, J2)) +
+                 wx2 * (wy1 * iv(cg, cact, cv, I2, J1) + wy2 * iv(cg, cact, cv, I2, J2));
+    xu[o] += (fact[o] & 1) ? pu : T(0);
+    xv[o] += (fact[o] & 2) ? pv : T(0);
+}
+
+// _coarsen_coefficients! (multigrid_preconditioner.jl:331-344), FP64
+__global__ __launch_bounds__(NT) void k_coarsen(KG cg, KG fg, int hx, int hy, const double *__restrict__ feta,
+                                                const double *__restrict__ fbu, const double *__restrict__ fbv,
+                                                double *__restrict__ eta, double *__restrict__ bu, double *__restrict__ bv) {
+    const int I = blockIdx.x * BX + threadIdx.x + 1, J = blockIdx.y * BY + threadIdx.y + 1;
+    if (I > cg.Nx || J > cg.Ny) return;
+    const int fxf = hx ? 2 : 1, fyf = hy ? 2 : 1;
+    const int i = fxf * (I - 1) + 1, j = fyf * (J - 1) + 1;
+    const double fAz = fg.dx * fg.dy, cAz = cg.dx * cg.dy;
+    double S = 0.0;
+    for (int a = 0; a <= fxf - 1; a++)
+        for (int b = 0; b <= fyf - 1; b++) S += fAz * ldc(fg, feta, i + a, j + b);
+    const long o = off(cg, I, J);
+    eta[o] = S / cAz;
+    double su = 0.0, sv = 0.0;
+    for (int a = (hx ? -1 : 0); a <= (hx ? 1 : 0); a++)
+        for (int b = (hy ? -1 : 0); b <= (hy ? 2 : 0); b++) {
+            const int ii = r_index(I, a, fg.Nx, fg.tx, 1, hx), jj = r_index(J, b, fg.Ny, fg.ty, 0, hy);
+            const double w = r_weight<double>(I, a, cg.Nx, fg.tx, 1, hx) * r_weight<double>(J, b, cg.Ny, fg.ty, 0, hy);
+            su += w * (fAz * ldfc(fg, fbu, ii, jj));
+        }
+    for (int a = (hx ? -1 : 0); a <= (hx ? 2 : 0); a++)
+        for (int b = (hy ? -1 : 0); b <= (hy ? 1 : 0); b++) {
+            const int ii = r_index(I, a, fg.Nx, fg.tx, 0, hx), jj = r_index(J, b, fg.Ny, fg.ty, 1, hy);
+            const double w = r_weight<double>(I, a, cg.Nx, fg.tx, 0, hx) * r_weight<double>(J, b, cg.Ny, fg.ty, 1, hy);
+            sv += w * (fAz * ldcf(fg, fbv, ii, jj));
+        }
+    bu[o] = su / cAz;
+    bv[o] = sv / cAz;
+}
+
+// η̄Hᶠᶠ = ℑxyᶠᶠ(η̄H) (src/StressBalances/stress_balance_coefficients.jl:80-83) and the activity flags, over the
+// (Nx + 1) × (Ny + 1) extent of one level.
+__global__ __launch_bounds__(NT) void k_corners_flags(KG g, const double *__restrict__ eta, double *__restrict__ etaff,
+                                                      uint8_t *__restrict__ act) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    if (etaff && i <= nxf(g) && j <= nyf(g))
+        etaff[o] = ((ldc(g, eta, i - 1, j - 1) + ldc(g, eta, i, j - 1)) / 2.0 + (ldc(g, eta, i - 1, j) + ldc(g, eta, i, j)) / 2.0) / 2.0;
+    uint8_t f = 0;
+    if (i <= nxf(g) && j <= g.Ny) {
+        const bool fr = (ldc(g, eta, i - 1, j) + ldc(g, eta, i, j)) == 0.0;
+        if (!(periph_fc(g, i, j) || fr)) f |= 1;
+    }
+    if (i <= g.Nx && j <= nyf(g)) {
+        const bool fr = (ldc(g, eta, i, j - 1) + ldc(g, eta, i, j)) == 0.0;
+        if (!(periph_cf(g, i, j) || fr)) f |= 2;
+    }
+    act[o] = f;
+}
+
+// FP64 → FP32 copies of a level's coefficients for the mixed-precision inner solve.
+__global__ __launch_bounds__(NT) void k_cast_coeffs(KG g, const double *__restrict__ a, const double *__restrict__ b,
+                                                    const double *__restrict__ c, const double *__restrict__ d,
+                                                    float *__restrict__ a32, float *__restrict__ b32, float *__restrict__ c32,
+                                                    float *__restrict__ d32) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    a32[o] = (float)a[o];
+    b32[o] = (float)b[o];
+    c32[o] = (float)c[o];
+    d32[o] = (float)d[o];
+}
+
+// Mixed precision: r32 = (float) r64 and d32 = 0 over :xyz (inner solve from zero)
+__global__ __launch_bounds__(NT) void k_demote(KG g, const double *__restrict__ ru, const double *__restrict__ rv,
+                                               float *__restrict__ r32u, float *__restrict__ r32v, float *__restrict__ du,
+                                               float *__restrict__ dv) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    r32u[o] = (float)ru[o];
+    r32v[o] = (float)rv[o];
+    du[o] = 0.0f;
+    dv[o] = 0.0f;
+}
+
+// Mixed precision: x64 += (double) d32 over :xyz
+__global__ __launch_bounds__(NT) void k_promote_add(KG g, const float *__restrict__ du, const float *__restrict__ dv,
+                                                    double *__restrict__ xu, double *__restrict__ xv) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    xu[o] += (double)du[o];
+    xv[o] += (double)dv[o];
+}
+
+// fill over the whole extent (used for fill!(x, 0) and workspace clearing)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fill2(KG g, T *__restrict__ a, T *__restrict__ b, T value, const Scal *gate) {
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    a[o] = value;
+    if (b) b[o] = value;
+}
+
+// z = 1·x + 0·x over :xyz (copyto!), both components
+template <typename T>
+__global__ __launch_bounds__(NT) void k_copy2(KG g, const T *__restrict__ xu, const T *__restrict__ xv, T *__restrict__ zu,
+                                              T *__restrict__ zv) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    zu[o] = T(1) * xu[o] + T(0) * xu[o];
+    zv[o] = T(1) * xv[o] + T(0) * xv[o];
+}
+
+}  // namespace nk

@@ -1,0 +1,1063 @@
+// nk_solver.hip — host side of libnunatak_hip.so: the handle, its device fields, and the drivers of the
+// reference's velocity-solve functions (update_velocities!, picard_solve!, solve!, precondition!, ssa_operator!)
+// over the gfx950 kernels of nk_device.hpp / nk_physics.hpp. C ABI in include/nunatak_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nunatak_hip.h"
+#include "nk_device.hpp"
+#include "nk_physics.hpp"
+
+using namespace nk;
+
+namespace {
+
+struct LocInfo {
+    int lx, ly, sigma;  // 1 = Face
+};
+const LocInfo LOC[NK_FIELD_COUNT] = {
+    {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0},
+    {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0},
+    {1, 0, 0}, {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0},
+    {0, 1, 0},
+};
+
+template <typename T> struct Vecs {
+    T *x0u = nullptr, *x0v = nullptr, *x1u = nullptr, *x1v = nullptr, *bu = nullptr, *bv = nullptr, *qu = nullptr,
+      *qv = nullptr, *t = nullptr;
+};
+
+struct Level {
+    KG g{};
+    int cx = 0, cy = 0;  // halved relative to the next finer level
+    double *eta = nullptr, *etaff = nullptr, *bu = nullptr, *bv = nullptr;
+    float *eta32 = nullptr, *etaff32 = nullptr, *bu32 = nullptr, *bv32 = nullptr;
+    uint8_t *act = nullptr;
+    Vecs<double> v64;
+    Vecs<float> v32;
+};
+
+template <typename T> struct Work {  // PCG workspace of one precision on the finest grid
+    T *ru, *rv, *zu, *zv, *z2u, *z2v, *pu, *pv, *qu, *qv, *du, *dv;
+};
+
+double jmax_h(double x, double y) { double d = x - y; return (std::isnan(x) || std::isnan(y)) ? d : (std::signbit(d) ? y : x); }
+
+}  // namespace
+
+struct nk_solver {
+    nk_grid grid{};
+    nk_physics ph{};
+    nk_solver_options op{};
+    KG g{};
+    long plane = 0;  // elements per 2D plane
+    int Ns = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *f[NK_FIELD_COUNT] = {};
+    uint8_t *cls_u = nullptr, *cls_v = nullptr;
+    double *sig = nullptr;
+    Work<double> w64{};
+    Work<float> w32{};
+    double *xmu = nullptr, *xmv = nullptr;
+    std::vector<Level> lev;  // lev[0] = the finest grid
+    double *partials = nullptr;
+    unsigned int *counter = nullptr;
+    Scal *S = nullptr;
+    Scal *S_host = nullptr;
+    std::vector<void *> allocs;
+    std::string err;
+    bool flags_valid = false;
+    // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
+    std::vector<hipEvent_t> prof_events;
+    size_t prof_used = 0;
+    double prof_ms = 0.0;
+    long prof_launches = 0, prof_points = 0;
+    int prof_precision = 0;
+};
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            s->err = std::string(#expr) + ": " + hipGetErrorString(e_);                                \
+            return NK_DEVICE_ERROR;                                                                    \
+        }                                                                                              \
+    } while (0)
+
+#define CHK(expr)                               \
+    do {                                        \
+        nk_status st_ = (expr);                 \
+        if (st_ != NK_OK) return st_;           \
+    } while (0)
+
+namespace {
+
+KG make_kg(int Nx, int Ny, int tx, int ty, double dx, double dy) {
+    KG g;
+    g.Nx = Nx; g.Ny = Ny; g.tx = tx; g.ty = ty; g.dx = dx; g.dy = dy;
+    g.pitch = ((Nx + 1) + 63) / 64 * 64;
+    return g;
+}
+long plane_of(const KG &g) { return (long)g.pitch * (long)(g.Ny + 1); }
+
+dim3 blocks_for(int ex, int ey) { return dim3((ex + BX - 1) / BX, (ey + BY - 1) / BY); }
+const dim3 THREADS(BX, BY);
+
+nk_status alloc_bytes(nk_solver *s, void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        s->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return NK_OUT_OF_MEMORY;
+    }
+    s->allocs.push_back(*p);
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) { s->err = "hipMemset failed"; return NK_DEVICE_ERROR; }
+    return NK_OK;
+}
+template <typename T> nk_status alloc_plane(nk_solver *s, T **p, const KG &g, int nz = 1) {
+    return alloc_bytes(s, (void **)p, sizeof(T) * (size_t)plane_of(g) * (size_t)nz);
+}
+
+int coarsening(int N, int H, int minimum) { int m = minimum > H ? minimum : H; return (N % 2 == 0) && (N / 2 >= m); }
+
+// Julia Float64^Int is correctly rounded up to its compensated power; long double reproduces that.
+double jpow_int(double x, int n) { long double r = 1.0L; for (int k = 0; k < n; k++) r *= (long double)x; return (double)r; }
+double sigma_face(int k0, int Ns, double r) {
+    double q = (double)k0 / (double)Ns;
+    if (r == std::floor(r) && r >= 0 && r <= 16) return jpow_int(q, (int)r);
+    return std::pow(q, r);
+}
+double power_integral(double zm, double zp, int p) { return (jpow_int(1.0 - zm, p + 1) - jpow_int(1.0 - zp, p + 1)) / (double)(p + 1); }
+
+PhysK physk(const nk_solver *s) {
+    const nk_physics &ph = s->ph;
+    PhysK p;
+    p.ri = ph.ice_density; p.rw = ph.ocean_density; p.gr = ph.gravitational_acceleration;
+    p.ri_rw = ph.ice_density / ph.ocean_density;
+    p.rw_ri = ph.ocean_density / ph.ice_density;
+    p.rig = ph.ice_density * ph.gravitational_acceleration;
+    p.rwg = ph.ocean_density * ph.gravitational_acceleration;
+    p.grounding = ph.grounding; p.masking = ph.masking;
+    p.hmin = ph.mask_minimum_thickness; p.htr = ph.mask_transition_thickness;
+    p.n = ph.glen_exponent; p.e0 = ph.strain_rate_regularization;
+    p.inv_n = -1.0 / ph.glen_exponent;
+    p.exp_e = (1.0 - ph.glen_exponent) / (2.0 * ph.glen_exponent);
+    p.A3d = ph.rate_factor_depth == NK_RATE_FACTOR_3D; p.friction = ph.friction;
+    p.m = ph.friction_exponent; p.alpha2 = ph.coulomb_coefficient; p.u0 = ph.velocity_regularization;
+    p.walls = ph.walls; p.Ns = s->Ns;
+    p.wf = ph.walls == NK_NO_SLIP ? 2.0 : 0.0;
+    return p;
+}
+
+Sig sig_tables(const nk_solver *s) {
+    int Ns = s->Ns;
+    Sig t;
+    t.zf = s->sig;
+    t.zc = s->sig + (Ns + 1);
+    t.dz = t.zc + Ns;
+    t.w2 = t.dz + Ns;
+    t.pw = t.w2 + Ns;
+    return t;
+}
+
+Red red(nk_solver *s, int fin) { Red r; r.partials = s->partials; r.counter = s->counter; r.S = s->S; r.fin = fin; return r; }
+
+template <typename T> Coef<T> coef(const Level &L);
+template <> Coef<double> coef<double>(const Level &L) { return Coef<double>{L.eta, L.etaff, L.bu, L.bv, L.act}; }
+template <> Coef<float> coef<float>(const Level &L) { return Coef<float>{L.eta32, L.etaff32, L.bu32, L.bv32, L.act}; }
+template <typename T> Vecs<T> &vecs(Level &L);
+template <> Vecs<double> &vecs<double>(Level &L) { return L.v64; }
+template <> Vecs<float> &vecs<float>(Level &L) { return L.v32; }
+template <typename T> Work<T> &work(nk_solver *s);
+template <> Work<double> &work<double>(nk_solver *s) { return s->w64; }
+template <> Work<float> &work<float>(nk_solver *s) { return s->w32; }
+
+// Event bracket around one launch of the profiled kernel class on the finest grid.
+struct ProfScope {
+    nk_solver *s;
+    int idx = -1;
+    ProfScope(nk_solver *s_, int kind, int level, int precision_bytes) : s(s_) {
+        if (s->op.profile_kernel != kind || kind == NK_PROFILE_NONE || level != 0) return;
+        if (s->prof_used + 2 > s->prof_events.size()) {
+            for (int k = 0; k < 256; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return;
+                s->prof_events.push_back(e);
+            }
+        }
+        idx = (int)s->prof_used;
+        s->prof_used += 2;
+        (void)hipEventRecord(s->prof_events[idx], s->stream);
+        s->prof_launches += 1;
+        s->prof_points += (long)s->g.Nx * s->g.Ny;
+        s->prof_precision = precision_bytes;
+    }
+    ~ProfScope() {
+        if (idx >= 0) (void)hipEventRecord(s->prof_events[idx + 1], s->stream);
+    }
+};
+
+// Sum the recorded event pairs (after the stream has been synchronised).
+void prof_harvest(nk_solver *s) {
+    for (size_t k = 0; k + 1 < s->prof_used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s->prof_events[k], s->prof_events[k + 1]) == hipSuccess) s->prof_ms += ms;
+    }
+    s->prof_used = 0;
+}
+
+nk_status launch_check(nk_solver *s) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { s->err = std::string("kernel launch: ") + hipGetErrorString(e); return NK_DEVICE_ERROR; }
+    return NK_OK;
+}
+
+// ---- coefficient bookkeeping ------------------------------------------------------------------------------------
+
+// η̄Hᶠᶠ (optional) and activity flags of level l from its η̄H
+nk_status corners_flags(nk_solver *s, int l, bool write_corners) {
+    Level &L = s->lev[l];
+    k_corners_flags<<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, L.eta, write_corners ? L.etaff : nullptr, L.act);
+    return launch_check(s);
+}
+
+// update_preconditioner!: coarse coefficients of every level (multigrid) and FP32 copies (mixed precision)
+nk_status update_preconditioner(nk_solver *s) {
+    if (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID) {
+        for (size_t l = 1; l < s->lev.size(); l++) {
+            Level &F = s->lev[l - 1], &C = s->lev[l];
+            k_coarsen<<<blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream>>>(C.g, F.g, C.cx, C.cy, F.eta, F.bu, F.bv, C.eta, C.bu, C.bv);
+            CHK(launch_check(s));
+            CHK(corners_flags(s, (int)l, true));
+        }
+    }
+    if (s->op.precision == NK_PRECISION_MIXED) {
+        for (size_t l = 0; l < s->lev.size(); l++) {
+            Level &L = s->lev[l];
+            k_cast_coeffs<<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, L.eta, L.etaff, L.bu, L.bv, L.eta32,
+                                                                                          L.etaff32, L.bu32, L.bv32);
+            CHK(launch_check(s));
+        }
+    }
+    return NK_OK;
+}
+
+// ---- multigrid V-cycle ------------------------------------------------------------------------------------------
+
+template <typename T> struct Pair { T *u, *v; };
+
+// smooth! from a zero initial guess (fill!(x, 0) then `sweeps` sweeps); result in x0 or x1 (returned)
+template <typename T>
+nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, const T *bv, int sweeps, bool final_dot,
+                           const Scal *gate, Pair<T> *out) {
+    Level &L = s->lev[l];
+    Coef<T> c = coef<T>(L);
+    const T wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
+    const dim3 B = blocks_for(L.g.Nx, L.g.Ny);
+    Vecs<T> &V = vecs<T>(L);
+    if (s->op.smoother == NK_LINE_JACOBI || sweeps == 0) {
+        k_fill2<T><<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, x0.u, x0.v, T(0), gate);
+        CHK(launch_check(s));
+        T *qu = (l == 0) ? work<T>(s).qu : V.qu, *qv = (l == 0) ? work<T>(s).qv : V.qv;
+        T *t = V.t;
+        for (int k = 0; k < sweeps; k++) {
+            k_apply<T><<<B, THREADS, 0, s->stream>>>(L.g, x0.u, x0.v, c, wf, qu, qv, nullptr, nullptr, red(s, FIN_NONE), 0, gate);
+            CHK(launch_check(s));
+            k_line_relax<T><<<dim3((L.g.Nx + NT - 1) / NT), THREADS, 0, s->stream>>>(L.g, x0.u, x0.v, bu, bv, qu, qv, t, c, wf, gate);
+            CHK(launch_check(s));
+        }
+        if (final_dot) {
+            k_dot<T><<<B, THREADS, 0, s->stream>>>(L.g, bu, bv, x0.u, x0.v, nullptr, nullptr, red(s, FIN_BETA), gate);
+            CHK(launch_check(s));
+        }
+        *out = x0;
+        return NK_OK;
+    }
+    Pair<T> cur = x0, oth = x1;
+    for (int k = 0; k < sweeps; k++) {
+        const bool last = (k == sweeps - 1) && final_dot;
+        {
+            ProfScope ps(s, k > 0 ? NK_PROFILE_SWEEP : NK_PROFILE_NONE, l, (int)sizeof(T));
+            k_sweep<T><<<B, THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, bu, bv, (k == 0 ? cur : oth).u, (k == 0 ? cur : oth).v,
+                                                     k == 0 ? 1 : 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+        }
+        CHK(launch_check(s));
+        if (k > 0) std::swap(cur, oth);
+    }
+    *out = cur;
+    return NK_OK;
+}
+
+// smooth! from the current x (post-smoothing)
+template <typename T>
+nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu, const T *bv, int sweeps, bool final_dot,
+                      const Scal *gate, Pair<T> *out) {
+    Level &L = s->lev[l];
+    Coef<T> c = coef<T>(L);
+    const T wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
+    const dim3 B = blocks_for(L.g.Nx, L.g.Ny);
+    Vecs<T> &V = vecs<T>(L);
+    if (s->op.smoother == NK_LINE_JACOBI) {
+        T *qu = (l == 0) ? work<T>(s).qu : V.qu, *qv = (l == 0) ? work<T>(s).qv : V.qv;
+        for (int k = 0; k < sweeps; k++) {
+            k_apply<T><<<B, THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, qu, qv, nullptr, nullptr, red(s, FIN_NONE), 0, gate);
+            CHK(launch_check(s));
+            k_line_relax<T><<<dim3((L.g.Nx + NT - 1) / NT), THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, bu, bv, qu, qv, V.t, c, wf, gate);
+            CHK(launch_check(s));
+        }
+    } else {
+        for (int k = 0; k < sweeps; k++) {
+            const bool last = (k == sweeps - 1) && final_dot;
+            {
+                ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
+                k_sweep<T><<<B, THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+            }
+            CHK(launch_check(s));
+            std::swap(cur, oth);
+        }
+        if (final_dot && sweeps > 0) { *out = cur; return NK_OK; }
+    }
+    if (final_dot) {
+        k_dot<T><<<B, THREADS, 0, s->stream>>>(L.g, bu, bv, cur.u, cur.v, nullptr, nullptr, red(s, FIN_BETA), gate);
+        CHK(launch_check(s));
+    }
+    *out = cur;
+    return NK_OK;
+}
+
+// v_cycle! (src/StressBalances/multigrid_preconditioner.jl:144-163). x ≈ A⁻¹ b from zero at level l; the result
+// lands in x0 or x1 (returned in *out). final_dot: Σ b·x at the end → FIN_BETA (finest level of the PCG).
+template <typename T>
+nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, const T *bv, bool final_dot, const Scal *gate,
+                  Pair<T> *out) {
+    const int nlev = (int)s->lev.size();
+    Level &L = s->lev[l];
+    Vecs<T> &V = vecs<T>(L);
+    if (l == nlev - 1) return smooth_from_zero<T>(s, l, x0, x1, bu, bv, s->op.coarsest_sweeps, final_dot, gate, out);
+    Pair<T> cur;
+    CHK(smooth_from_zero<T>(s, l, x0, x1, bu, bv, s->op.smoothing_sweeps, false, gate, &cur));
+    Pair<T> oth = (cur.u == x0.u) ? x1 : x0;
+    Coef<T> c = coef<T>(L);
+    const T wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
+    T *qu = (l == 0) ? work<T>(s).qu : V.qu, *qv = (l == 0) ? work<T>(s).qv : V.qv;
+    k_apply<T><<<blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, qu, qv, bu, bv, red(s, FIN_NONE), 0, gate);
+    CHK(launch_check(s));
+    Level &C = s->lev[l + 1];
+    Vecs<T> &CV = vecs<T>(C);
+    k_restrict<T><<<blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream>>>(C.g, L.g, C.cx, C.cy, C.act, L.act, qu, qv, CV.bu, CV.bv, gate);
+    CHK(launch_check(s));
+    Pair<T> cres;
+    CHK(v_cycle<T>(s, l + 1, Pair<T>{CV.x0u, CV.x0v}, Pair<T>{CV.x1u, CV.x1v}, CV.bu, CV.bv, false, gate, &cres));
+    k_prolongate<T><<<blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream>>>(L.g, C.g, C.cx, C.cy, L.act, C.act, cres.u, cres.v, cur.u,
+                                                                          cur.v, gate);
+    CHK(launch_check(s));
+    return smooth_from<T>(s, l, cur, oth, bu, bv, s->op.smoothing_sweeps, final_dot, gate, out);
+}
+
+// ---- PCG --------------------------------------------------------------------------------------------------------
+
+nk_status read_scalars(nk_solver *s) {
+    HIPCHK(hipMemcpyAsync(s->S_host, s->S, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    prof_harvest(s);
+    return NK_OK;
+}
+
+nk_status write_scalars(nk_solver *s, const nk_control &ctl) {
+    Scal h;
+    std::memset(&h, 0, sizeof(h));
+    h.reltol = ctl.reltol;
+    h.abstol = ctl.abstol;
+    h.fixed = ctl.kind == NK_FIXED_ITERATIONS;
+    h.maxiter = h.fixed ? ctl.iterations : ctl.maxiter;
+    *s->S_host = h;
+    HIPCHK(hipMemcpyAsync(s->S, s->S_host, sizeof(Scal), hipMemcpyHostToDevice, s->stream));
+    return NK_OK;
+}
+
+// conjugate_gradient! (src/Solvers/conjugate_gradient.jl:299-364) at precision T on the finest grid. When
+// r_ready, the residual r of the initial guess is already in the workspace (the mixed inner solve from zero).
+template <typename T>
+nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_control &ctl, bool r_ready, int *iters, double *res) {
+    Level &L0 = s->lev[0];
+    Work<T> &W = work<T>(s);
+    Coef<T> c = coef<T>(L0);
+    const T wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
+    const KG &g = L0.g;
+    const dim3 B = blocks_for(g.Nx, g.Ny);
+    const int pre = s->op.preconditioner;
+    CHK(write_scalars(s, ctl));
+    if (!r_ready) {
+        k_apply<T><<<B, THREADS, 0, s->stream>>>(g, xu, xv, c, wf, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
+        CHK(launch_check(s));
+    }
+    Pair<T> z{W.zu, W.zv};
+    if (pre == NK_PRECONDITIONER_MULTIGRID) {
+        CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, false, nullptr, &z));
+        k_dot<T><<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, z.u, z.v, W.ru, W.rv, red(s, FIN_INIT), nullptr);
+        CHK(launch_check(s));
+        k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, z.u, z.v, W.pu, W.pv, 1, nullptr);
+        CHK(launch_check(s));
+    } else {
+        k_update_diag<T><<<B, THREADS, 0, s->stream>>>(g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv, W.pu,
+                                                        W.pv, 1, red(s, FIN_INIT), nullptr);
+        CHK(launch_check(s));
+    }
+    const int chunk = s->op.check_interval > 0 ? s->op.check_interval : 8;
+    for (;;) {
+        CHK(read_scalars(s));
+        if (s->S_host->done) break;
+        for (int k = 0; k < chunk; k++) {
+            {
+                ProfScope ps(s, NK_PROFILE_OPERATOR, 0, (int)sizeof(T));
+                k_apply<T><<<B, THREADS, 0, s->stream>>>(g, W.pu, W.pv, c, wf, W.qu, W.qv, nullptr, nullptr, red(s, FIN_ALPHA), 1, s->S);
+            }
+            CHK(launch_check(s));
+            if (pre == NK_PRECONDITIONER_MULTIGRID) {
+                {
+                    ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
+                    k_update_xr<T><<<B, THREADS, 0, s->stream>>>(g, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, red(s, FIN_RES), s->S);
+                }
+                CHK(launch_check(s));
+                Pair<T> zz;
+                CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, true, s->S, &zz));
+                k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, zz.u, zz.v, W.pu, W.pv, 0, s->S);
+                CHK(launch_check(s));
+            } else {
+                {
+                    ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
+                    k_update_diag<T><<<B, THREADS, 0, s->stream>>>(g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv,
+                                                                    nullptr, nullptr, 0, red(s, FIN_BETA_RES), s->S);
+                }
+                CHK(launch_check(s));
+                k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, W.zu, W.zv, W.pu, W.pv, 0, s->S);
+                CHK(launch_check(s));
+            }
+        }
+    }
+    *iters = s->S_host->it;
+    *res = s->S_host->res;
+    return NK_OK;
+}
+
+// Mixed precision linear solve: FP64 defect correction around FP32 PCG + FP32 V-cycle. The outer stopping rule
+// is the reference's, on the true FP64 residual: ‖b − A x‖ ≤ max(reltol ‖r₀‖, abstol) or maxiter inner iterations.
+nk_status solve_mixed(nk_solver *s, nk_info *info) {
+    Level &L0 = s->lev[0];
+    const KG &g = L0.g;
+    const dim3 B = blocks_for(g.Nx, g.Ny);
+    Coef<double> c = coef<double>(L0);
+    const double wf = s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0;
+    double *xu = s->f[NK_FIELD_U], *xv = s->f[NK_FIELD_V];
+    const double *bu = s->f[NK_FIELD_TAUU], *bv = s->f[NK_FIELD_TAUV];
+    Work<double> &W = s->w64;
+    Work<float> &V = s->w32;
+    const nk_control &ctl = s->op.linear;
+    const bool fixed = ctl.kind == NK_FIXED_ITERATIONS;
+    const int maxiter = fixed ? ctl.iterations : ctl.maxiter;
+    auto residual = [&](double *out) -> nk_status {
+        k_apply<double><<<B, THREADS, 0, s->stream>>>(g, xu, xv, c, wf, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
+        CHK(launch_check(s));
+        k_dot<double><<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, W.ru, W.rv, nullptr, nullptr, red(s, FIN_NONE), nullptr);
+        CHK(launch_check(s));
+        CHK(read_scalars(s));
+        *out = std::sqrt(s->S_host->out[0]);
+        return NK_OK;
+    };
+    double r0;
+    CHK(residual(&r0));
+    const double tol = fixed ? -1.0 : jmax_h(ctl.reltol * r0, ctl.abstol);
+    double res = r0;
+    int total = 0, steps = 0;
+    while ((fixed ? total < maxiter : (res > tol && total < maxiter))) {
+        k_demote<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, V.ru, V.rv, V.du, V.dv);
+        CHK(launch_check(s));
+        nk_control inner;
+        std::memset(&inner, 0, sizeof(inner));
+        inner.kind = NK_CONVERGENCE;
+        inner.reltol = s->op.inner_reltol;
+        inner.abstol = 0.0;
+        int budget = maxiter - total;
+        inner.maxiter = s->op.inner_maxiter < budget ? s->op.inner_maxiter : budget;
+        int it = 0;
+        double ires = 0.0;
+        CHK(pcg<float>(s, V.du, V.dv, nullptr, nullptr, inner, true, &it, &ires));
+        k_promote_add<<<B, THREADS, 0, s->stream>>>(g, V.du, V.dv, xu, xv);
+        CHK(launch_check(s));
+        total += it > 0 ? it : 1;
+        steps++;
+        CHK(residual(&res));
+        if (it == 0) break;
+    }
+    if (info) {
+        info->linear_iterations = total;
+        info->total_linear_iterations += total;
+        info->refinement_steps = steps;
+        info->last_residual_norm = res;
+    }
+    return NK_OK;
+}
+
+nk_status solve_linear(nk_solver *s, nk_info *info) {
+    CHK(update_preconditioner(s));
+    if (s->op.precision == NK_PRECISION_MIXED) return solve_mixed(s, info);
+    int it = 0;
+    double res = 0.0;
+    CHK(pcg<double>(s, s->f[NK_FIELD_U], s->f[NK_FIELD_V], s->f[NK_FIELD_TAUU], s->f[NK_FIELD_TAUV], s->op.linear, false, &it, &res));
+    if (info) {
+        info->linear_iterations = it;
+        info->total_linear_iterations += it;
+        info->last_residual_norm = res;
+    }
+    return NK_OK;
+}
+
+// ---- coefficient update ------------------------------------------------------------------------------------------
+
+nk_status update_coefficients(nk_solver *s) {
+    const KG &g = s->g;
+    const PhysK p = physk(s);
+    const Sig sg = sig_tables(s);
+    const dim3 Bc = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    double **f = s->f;
+    const double *mu = s->ph.masking ? f[NK_FIELD_MU] : nullptr;
+    if (s->ph.stress_balance == NK_SSA) {
+        k_viscosity<<<Bc, THREADS, 0, s->stream>>>(g, p, sg, 0, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_A], s->plane,
+                                                    f[NK_FIELD_ETASIGMA], f[NK_FIELD_H], mu, nullptr, nullptr, nullptr, nullptr,
+                                                    f[NK_FIELD_ETAH], f[NK_FIELD_F2]);
+        CHK(launch_check(s));
+        CHK(corners_flags(s, 0, true));
+        if (s->ph.friction != NK_FRICTION_NONE) {
+            k_drag<<<Be, THREADS, 0, s->stream>>>(g, p, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_C], f[NK_FIELD_N], f[NK_FIELD_PHIU],
+                                                   f[NK_FIELD_PHIV], f[NK_FIELD_BETAU], f[NK_FIELD_BETAV]);
+            CHK(launch_check(s));
+        }
+    } else {
+        k_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                         f[NK_FIELD_F2], f[NK_FIELD_UB], f[NK_FIELD_VB]);
+        CHK(launch_check(s));
+        if (s->ph.friction != NK_FRICTION_NONE) {
+            k_drag<<<Be, THREADS, 0, s->stream>>>(g, p, f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_C], f[NK_FIELD_N], f[NK_FIELD_PHIU],
+                                                   f[NK_FIELD_PHIV], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV]);
+            CHK(launch_check(s));
+        }
+        k_viscosity<<<Bc, THREADS, 0, s->stream>>>(g, p, sg, 1, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_A], s->plane,
+                                                    f[NK_FIELD_ETASIGMA], f[NK_FIELD_H], mu, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                    f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_ETAH], f[NK_FIELD_F2]);
+        CHK(launch_check(s));
+        CHK(corners_flags(s, 0, true));
+        k_effective_drag<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_F2], f[NK_FIELD_BETAU],
+                                                         f[NK_FIELD_BETAV]);
+        CHK(launch_check(s));
+    }
+    s->flags_valid = true;
+    return NK_OK;
+}
+
+nk_status driving_stress(nk_solver *s) {
+    const KG &g = s->g;
+    double **f = s->f;
+    k_driving_stress<<<blocks_for(g.Nx + 1, g.Ny + 1), THREADS, 0, s->stream>>>(
+        g, physk(s), f[NK_FIELD_H], f[NK_FIELD_S], s->ph.masking ? f[NK_FIELD_MU] : nullptr,
+        s->ph.grounding == NK_GROUNDING_SUBGRID_DRIVING_STRESS ? f[NK_FIELD_BED] : nullptr, f[NK_FIELD_TAUU], f[NK_FIELD_TAUV],
+        s->cls_u, s->cls_v);
+    return launch_check(s);
+}
+
+nk_status post_process(nk_solver *s) {
+    const KG &g = s->g;
+    double **f = s->f;
+    const dim3 Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    if (s->ph.stress_balance == NK_SSA) {
+        k_extend<<<Be, THREADS, 0, s->stream>>>(g, s->Ns, s->plane, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_UB], f[NK_FIELD_VB],
+                                                 f[NK_FIELD_USIGMA], f[NK_FIELD_VSIGMA]);
+        return launch_check(s);
+    }
+    k_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                     f[NK_FIELD_F2], f[NK_FIELD_UB], f[NK_FIELD_VB]);
+    CHK(launch_check(s));
+    k_velocity_profile<<<Be, THREADS, 0, s->stream>>>(g, sig_tables(s), s->Ns, s->plane, f[NK_FIELD_ETASIGMA], f[NK_FIELD_H],
+                                                       f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                       f[NK_FIELD_USIGMA], f[NK_FIELD_VSIGMA]);
+    return launch_check(s);
+}
+
+// picard_solve! (src/Solvers/picard.jl:71-90)
+nk_status picard(nk_solver *s, nk_info *info) {
+    const nk_control &ctl = s->op.picard;
+    const bool fixed = ctl.kind == NK_FIXED_ITERATIONS;
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny);
+    double D = INFINITY, X = 0.0;
+    int it = 0;
+    if (info) info->total_linear_iterations = 0;
+    for (;;) {
+        if (fixed) { if (it >= ctl.iterations) break; }
+        else if (!(D > jmax_h(ctl.reltol * X, ctl.abstol) && it < ctl.maxiter)) break;
+        if (!fixed) {
+            k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->f[NK_FIELD_U], s->f[NK_FIELD_V], s->xmu, s->xmv);
+            CHK(launch_check(s));
+        }
+        CHK(update_coefficients(s));
+        CHK(solve_linear(s, info));
+        if (!fixed) {
+            k_picard_change<<<B, THREADS, 0, s->stream>>>(g, s->xmu, s->xmv, s->f[NK_FIELD_U], s->f[NK_FIELD_V], red(s, FIN_PICARD));
+            CHK(launch_check(s));
+            CHK(read_scalars(s));
+            D = s->S_host->picard_D;
+            X = s->S_host->picard_X;
+        }
+        it++;
+    }
+    if (info) {
+        info->picard_iterations = it;
+        info->last_picard_change = D;
+        info->last_picard_norm = X;
+    }
+    return NK_OK;
+}
+
+nk_status check_field(const nk_solver *s, int id) {
+    return (id < 0 || id >= NK_FIELD_COUNT) ? NK_INVALID_ARGUMENT : NK_OK;
+}
+
+void field_shape(const nk_solver *s, int id, int32_t shape[3]) {
+    const LocInfo &L = LOC[id];
+    shape[0] = s->grid.Nx + ((L.lx && s->grid.topology_x == NK_BOUNDED) ? 1 : 0);
+    shape[1] = s->grid.Ny + ((L.ly && s->grid.topology_y == NK_BOUNDED) ? 1 : 0);
+    shape[2] = (L.sigma == 1 || (L.sigma == 2 && s->ph.rate_factor_depth == NK_RATE_FACTOR_3D)) ? s->Ns : 1;
+}
+
+struct Timer {
+    nk_solver *s;
+    nk_info *info;
+    Timer(nk_solver *s_, nk_info *i_) : s(s_), info(i_) {
+        s->prof_ms = 0.0;
+        s->prof_launches = s->prof_points = 0;
+        s->prof_used = 0;
+        (void)hipEventRecord(s->ev0, s->stream);
+    }
+    nk_status done(nk_status st) {
+        (void)hipEventRecord(s->ev1, s->stream);
+        (void)hipEventSynchronize(s->ev1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+        prof_harvest(s);
+        if (info) {
+            info->elapsed_ms = ms;
+            info->profiled_ms = s->prof_ms;
+            info->profiled_launches = s->prof_launches;
+            info->profiled_points = s->prof_points;
+            info->profiled_precision = s->prof_precision;
+        }
+        hipError_t e = hipGetLastError();
+        if (st == NK_OK && e != hipSuccess) { s->err = hipGetErrorString(e); return NK_DEVICE_ERROR; }
+        return st;
+    }
+};
+
+nk_status validate(const nk_grid *g, const nk_physics *ph, const nk_solver_options *op, std::string &err) {
+    if (!g || !ph || !op) { err = "null argument"; return NK_INVALID_ARGUMENT; }
+    if (g->Nx < 1 || g->Ny < 1 || g->Nsigma < 1 || g->Nsigma > 256) { err = "grid sizes must be positive (Nσ ≤ 256)"; return NK_INVALID_ARGUMENT; }
+    if (!(g->dx > 0) || !(g->dy > 0)) { err = "grid spacing must be positive"; return NK_INVALID_ARGUMENT; }
+    if (ph->masking && !(ph->mask_transition_thickness > 0)) {
+        err = "transition_thickness must be positive";  // SmoothIceMask (src/Models/smooth_ice_mask.jl:42)
+        return NK_INVALID_ARGUMENT;
+    }
+    if (ph->walls != NK_FREE_SLIP && ph->walls != NK_NO_SLIP) {
+        err = "the coefficients must include the side walls, FreeSlipWalls() or NoSlipWalls()";  // ssa_operator.jl:56
+        return NK_INVALID_ARGUMENT;
+    }
+    if (op->preconditioner < 0 || op->preconditioner > 2 || op->precision < 0 || op->precision > 1) {
+        err = "unknown preconditioner or precision";
+        return NK_INVALID_ARGUMENT;
+    }
+    if (op->precision == NK_PRECISION_MIXED && op->preconditioner == NK_PRECONDITIONER_NONE) {
+        err = "mixed precision needs a preconditioner";
+        return NK_INVALID_ARGUMENT;
+    }
+    return NK_OK;
+}
+
+}  // namespace
+
+// =====================================================================================================================
+// C ABI
+// =====================================================================================================================
+
+extern "C" {
+
+int32_t nk_abi_version(void) { return NK_ABI_VERSION; }
+
+void nk_default_grid(nk_grid *g) {
+    std::memset(g, 0, sizeof(*g));
+    g->Nx = g->Ny = 4;
+    g->topology_x = g->topology_y = NK_BOUNDED;
+    g->halo = 4;
+    g->Nsigma = 10;
+    g->dx = g->dy = 1.0;
+    g->bed_refinement = 2.0;
+}
+
+void nk_default_physics(nk_physics *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->ice_density = 918.0; p->ocean_density = 1028.0; p->gravitational_acceleration = 9.81;
+    p->grounding = NK_GROUNDING_NONE; p->masking = 1;
+    p->mask_minimum_thickness = 1.0; p->mask_transition_thickness = 1.0;
+    p->glen_exponent = 3.0; p->strain_rate_regularization = 1e-13;
+    p->rate_factor_depth = NK_RATE_FACTOR_2D;
+    p->friction = NK_FRICTION_NONE; p->friction_exponent = 1.0 / 3.0; p->coulomb_coefficient = 0.0;
+    p->velocity_regularization = 1e-8;
+    p->walls = NK_FREE_SLIP; p->stress_balance = NK_SSA;
+}
+
+void nk_default_options(nk_solver_options *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->linear.kind = NK_CONVERGENCE; o->linear.reltol = 1e-8; o->linear.maxiter = 10000;
+    o->picard.kind = NK_CONVERGENCE; o->picard.reltol = 1e-6; o->picard.maxiter = 100;
+    o->preconditioner = NK_PRECONDITIONER_DIAGONAL;
+    o->smoother = NK_POINT_JACOBI; o->smoothing_sweeps = 2; o->coarsest_sweeps = 20; o->minimum_coarse_size = 2;
+    o->precision = NK_PRECISION_FP64; o->inner_reltol = 1e-3; o->inner_maxiter = 200; o->check_interval = 8;
+    o->profile_kernel = NK_PROFILE_NONE;
+}
+
+nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_solver_options *options, nk_solver **out) {
+    if (!out) return NK_INVALID_ARGUMENT;
+    *out = nullptr;
+    std::string err;
+    nk_status st = validate(grid, physics, options, err);
+    if (st != NK_OK) {
+        std::fprintf(stderr, "nk_create: %s\n", err.c_str());
+        return st;
+    }
+    nk_solver *s = new nk_solver();
+    s->grid = *grid; s->ph = *physics; s->op = *options;
+    s->Ns = grid->Nsigma;
+    s->g = make_kg(grid->Nx, grid->Ny, grid->topology_x, grid->topology_y, grid->dx, grid->dy);
+    s->plane = plane_of(s->g);
+    auto fail = [&](nk_status e) { *out = s; return e; };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { s->err = "hipStreamCreate failed"; return fail(NK_DEVICE_ERROR); }
+    (void)hipEventCreate(&s->ev0);
+    (void)hipEventCreate(&s->ev1);
+    for (int id = 0; id < NK_FIELD_COUNT; id++) {
+        int nz = LOC[id].sigma ? s->Ns : 1;
+        st = alloc_plane(s, &s->f[id], s->g, nz);
+        if (st != NK_OK) return fail(st);
+    }
+    if ((st = alloc_plane(s, &s->cls_u, s->g)) != NK_OK) return fail(st);
+    if ((st = alloc_plane(s, &s->cls_v, s->g)) != NK_OK) return fail(st);
+    // σ tables: zf (Ns+1), zc, dz, w2 (Ns each), pw (Ns × Ns), as the reference's SigmaGrid computes them
+    {
+        int Ns = s->Ns;
+        std::vector<double> t((Ns + 1) + 3 * Ns + Ns * Ns, 0.0);
+        double *zf = t.data(), *zc = zf + Ns + 1, *dz = zc + Ns, *w2 = dz + Ns, *pw = w2 + Ns;
+        for (int k = 0; k <= Ns; k++) zf[k] = sigma_face(k, Ns, grid->bed_refinement);
+        for (int k = 0; k < Ns; k++) {
+            zc[k] = (zf[k] + zf[k + 1]) / 2.0;
+            dz[k] = zf[k + 1] - zf[k];
+            w2[k] = power_integral(zf[k], zf[k + 1], 2);
+        }
+        for (int k = 0; k < Ns; k++)
+            for (int kp = 0; kp <= k; kp++) pw[k * Ns + kp] = power_integral(zf[kp], kp == k ? zc[k] : zf[kp + 1], 1);
+        if ((st = alloc_bytes(s, (void **)&s->sig, t.size() * sizeof(double))) != NK_OK) return fail(st);
+        if (hipMemcpy(s->sig, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { s->err = "σ tables"; return fail(NK_DEVICE_ERROR); }
+    }
+    // levels: the finest uses the model's fields; coarse levels own theirs
+    {
+        Level L0;
+        L0.g = s->g;
+        L0.eta = s->f[NK_FIELD_ETAH]; L0.etaff = s->f[NK_FIELD_ETAHFF]; L0.bu = s->f[NK_FIELD_BETAU]; L0.bv = s->f[NK_FIELD_BETAV];
+        s->lev.push_back(L0);
+        if (options->preconditioner == NK_PRECONDITIONER_MULTIGRID) {
+            KG cur = s->g;
+            for (;;) {
+                int cxh = coarsening(cur.Nx, grid->halo, options->minimum_coarse_size);
+                int cyh = coarsening(cur.Ny, grid->halo, options->minimum_coarse_size);
+                if (!cxh && !cyh) break;
+                Level L;
+                L.g = make_kg(cur.Nx / (cxh ? 2 : 1), cur.Ny / (cyh ? 2 : 1), cur.tx, cur.ty, cur.dx * (cxh ? 2 : 1), cur.dy * (cyh ? 2 : 1));
+                L.cx = cxh; L.cy = cyh;
+                s->lev.push_back(L);
+                cur = L.g;
+            }
+        }
+        for (size_t l = 0; l < s->lev.size(); l++) {
+            Level &L = s->lev[l];
+            if (l > 0) {
+                double **d[] = {&L.eta, &L.etaff, &L.bu, &L.bv};
+                for (auto p : d) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
+            }
+            if ((st = alloc_plane(s, &L.act, L.g)) != NK_OK) return fail(st);
+            double **v64[] = {&L.v64.x0u, &L.v64.x0v, &L.v64.x1u, &L.v64.x1v, &L.v64.bu, &L.v64.bv, &L.v64.qu, &L.v64.qv, &L.v64.t};
+            for (auto p : v64) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
+            if (options->precision == NK_PRECISION_MIXED) {
+                float **c32[] = {&L.eta32, &L.etaff32, &L.bu32, &L.bv32};
+                for (auto p : c32) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
+                float **v32[] = {&L.v32.x0u, &L.v32.x0v, &L.v32.x1u, &L.v32.x1v, &L.v32.bu, &L.v32.bv, &L.v32.qu, &L.v32.qv, &L.v32.t};
+                for (auto p : v32) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
+            }
+        }
+    }
+    {
+        double **w[] = {&s->w64.ru, &s->w64.rv, &s->w64.zu, &s->w64.zv, &s->w64.z2u, &s->w64.z2v, &s->w64.pu, &s->w64.pv,
+                        &s->w64.qu, &s->w64.qv, &s->w64.du, &s->w64.dv, &s->xmu, &s->xmv};
+        for (auto p : w) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
+        if (options->precision == NK_PRECISION_MIXED) {
+            float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
+                             &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv};
+            for (auto p : w32) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
+        }
+    }
+    {
+        long nb = (long)((s->g.Nx + BX - 1) / BX) * (long)((s->g.Ny + BY - 1) / BY);
+        if ((st = alloc_bytes(s, (void **)&s->partials, sizeof(double) * MAXRED * (size_t)nb)) != NK_OK) return fail(st);
+        if ((st = alloc_bytes(s, (void **)&s->counter, 64)) != NK_OK) return fail(st);
+        if ((st = alloc_bytes(s, (void **)&s->S, sizeof(Scal))) != NK_OK) return fail(st);
+        if (hipHostMalloc((void **)&s->S_host, sizeof(Scal), hipHostMallocDefault) != hipSuccess) { s->err = "hipHostMalloc failed"; return fail(NK_OUT_OF_MEMORY); }
+    }
+    *out = s;
+    return NK_OK;
+}
+
+nk_status nk_destroy(nk_solver *s) {
+    if (!s) return NK_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void *p : s->allocs) (void)hipFree(p);
+    if (s->S_host) (void)hipHostFree(s->S_host);
+    for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return NK_OK;
+}
+
+const char *nk_last_error(const nk_solver *s) { return s ? s->err.c_str() : "null handle"; }
+
+nk_status nk_set_physics(nk_solver *s, const nk_physics *physics) {
+    if (!s || !physics) return NK_INVALID_ARGUMENT;
+    nk_status st = validate(&s->grid, physics, &s->op, s->err);
+    if (st != NK_OK) return st;
+    s->ph = *physics;
+    return NK_OK;
+}
+
+nk_status nk_set_options(nk_solver *s, const nk_solver_options *o) {
+    if (!s || !o) return NK_INVALID_ARGUMENT;
+    nk_status st = validate(&s->grid, &s->ph, o, s->err);
+    if (st != NK_OK) return st;
+    if (o->preconditioner != s->op.preconditioner || o->precision != s->op.precision || o->minimum_coarse_size != s->op.minimum_coarse_size) {
+        if ((o->preconditioner == NK_PRECONDITIONER_MULTIGRID) != (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID) ||
+            o->precision != s->op.precision || o->minimum_coarse_size != s->op.minimum_coarse_size) {
+            s->err = "the multigrid hierarchy and precision are fixed at nk_create";
+            return NK_UNSUPPORTED;
+        }
+    }
+    s->op = *o;
+    return NK_OK;
+}
+
+nk_status nk_field_shape(const nk_solver *s, nk_field_id id, int32_t shape[3]) {
+    if (!s || !shape || check_field(s, id) != NK_OK) return NK_INVALID_ARGUMENT;
+    field_shape(s, id, shape);
+    return NK_OK;
+}
+
+nk_status nk_set_field(nk_solver *s, nk_field_id id, const double *data, int64_t ld, int64_t lds) {
+    if (!s || !data || check_field(s, id) != NK_OK) return NK_INVALID_ARGUMENT;
+    int32_t sh[3];
+    field_shape(s, id, sh);
+    if (ld == 0) ld = sh[0];
+    if (lds == 0) lds = ld * sh[1];
+    if (ld < sh[0] || lds < ld * sh[1]) { s->err = "leading dimension smaller than the field extent"; return NK_INVALID_ARGUMENT; }
+    for (int k = 0; k < sh[2]; k++)
+        HIPCHK(hipMemcpy2DAsync(s->f[id] + (long)k * s->plane, sizeof(double) * s->g.pitch, data + (long)k * lds, sizeof(double) * ld,
+                                sizeof(double) * sh[0], sh[1], hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (id == NK_FIELD_ETAH) s->flags_valid = false;
+    return NK_OK;
+}
+
+nk_status nk_get_field(nk_solver *s, nk_field_id id, double *data, int64_t ld, int64_t lds) {
+    if (!s || !data || check_field(s, id) != NK_OK) return NK_INVALID_ARGUMENT;
+    int32_t sh[3];
+    field_shape(s, id, sh);
+    if (ld == 0) ld = sh[0];
+    if (lds == 0) lds = ld * sh[1];
+    if (ld < sh[0] || lds < ld * sh[1]) { s->err = "leading dimension smaller than the field extent"; return NK_INVALID_ARGUMENT; }
+    for (int k = 0; k < sh[2]; k++)
+        HIPCHK(hipMemcpy2DAsync(data + (long)k * lds, sizeof(double) * ld, s->f[id] + (long)k * s->plane, sizeof(double) * s->g.pitch,
+                                sizeof(double) * sh[0], sh[1], hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_fill_field(nk_solver *s, nk_field_id id, double value) {
+    if (!s || check_field(s, id) != NK_OK) return NK_INVALID_ARGUMENT;
+    int32_t sh[3];
+    field_shape(s, id, sh);
+    for (int k = 0; k < sh[2]; k++) {
+        k_fill2<double><<<blocks_for(s->g.Nx + 1, s->g.Ny + 1), THREADS, 0, s->stream>>>(s->g, s->f[id] + (long)k * s->plane, nullptr,
+                                                                                         value, nullptr);
+        CHK(launch_check(s));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (id == NK_FIELD_ETAH) s->flags_valid = false;
+    return NK_OK;
+}
+
+nk_status nk_get_classes(nk_solver *s, uint8_t *cu, int64_t ldu, uint8_t *cv, int64_t ldv) {
+    if (!s || !cu || !cv) return NK_INVALID_ARGUMENT;
+    int32_t su[3], sv[3];
+    field_shape(s, NK_FIELD_U, su);
+    field_shape(s, NK_FIELD_V, sv);
+    if (ldu == 0) ldu = su[0];
+    if (ldv == 0) ldv = sv[0];
+    HIPCHK(hipMemcpy2DAsync(cu, ldu, s->cls_u, s->g.pitch, su[0], su[1], hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpy2DAsync(cv, ldv, s->cls_v, s->g.pitch, sv[0], sv[1], hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_field_device_pointer(nk_solver *s, nk_field_id id, double **ptr, int64_t *pitch, int64_t *layer) {
+    if (!s || check_field(s, id) != NK_OK) return NK_INVALID_ARGUMENT;
+    if (ptr) *ptr = s->f[id];
+    if (pitch) *pitch = s->g.pitch;
+    if (layer) *layer = s->plane;
+    return NK_OK;
+}
+
+void *nk_stream(nk_solver *s) { return s ? (void *)s->stream : nullptr; }
+
+nk_status nk_synchronize(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_update_geometry(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    double **f = s->f;
+    k_geometry<<<blocks_for(s->g.Nx + 1, s->g.Ny + 1), THREADS, 0, s->stream>>>(s->g, physk(s), f[NK_FIELD_H], f[NK_FIELD_BED],
+                                                                                f[NK_FIELD_S], f[NK_FIELD_PHI], f[NK_FIELD_PHIU],
+                                                                                f[NK_FIELD_PHIV], f[NK_FIELD_N], f[NK_FIELD_MU]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_compute_driving_stress(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    CHK(driving_stress(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_update_coefficients(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    CHK(update_coefficients(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_apply_operator(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    CHK(corners_flags(s, 0, false));
+    Level &L = s->lev[0];
+    k_apply<double><<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(
+        s->g, s->f[NK_FIELD_PU], s->f[NK_FIELD_PV], coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, s->f[NK_FIELD_QU],
+        s->f[NK_FIELD_QV], nullptr, nullptr, red(s, FIN_NONE), 0, nullptr);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_precondition(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    if (s->op.precision == NK_PRECISION_MIXED) { s->err = "nk_precondition runs the FP64 preconditioner only"; return NK_UNSUPPORTED; }
+    CHK(corners_flags(s, 0, false));
+    const nk_control fixed0 = {NK_FIXED_ITERATIONS, 0, 0, 0, 0.0, 0.0};
+    CHK(write_scalars(s, fixed0));  // done = 0: no gating
+    CHK(update_preconditioner(s));
+    Level &L = s->lev[0];
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny);
+    double *ru = s->f[NK_FIELD_PU], *rv = s->f[NK_FIELD_PV], *zu = s->f[NK_FIELD_QU], *zv = s->f[NK_FIELD_QV];
+    const int pre = s->op.preconditioner;
+    if (pre == NK_PRECONDITIONER_MULTIGRID) {
+        Pair<double> z;
+        CHK(v_cycle<double>(s, 0, Pair<double>{s->w64.zu, s->w64.zv}, Pair<double>{s->w64.z2u, s->w64.z2v}, ru, rv, false, nullptr, &z));
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, z.u, z.v, zu, zv);
+        CHK(launch_check(s));
+    } else {
+        k_update_diag<double><<<B, THREADS, 0, s->stream>>>(g, coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, pre, nullptr, nullptr,
+                                                             nullptr, nullptr, nullptr, nullptr, ru, rv, zu, zv, s->w64.pu, s->w64.pv, 1,
+                                                             red(s, FIN_NONE), nullptr);
+        CHK(launch_check(s));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_solve(nk_solver *s, nk_info *info) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    Timer t(s, info);
+    if (info) info->total_linear_iterations = 0;
+    nk_status st = corners_flags(s, 0, false);
+    if (st == NK_OK) st = solve_linear(s, info);
+    return t.done(st);
+}
+
+nk_status nk_picard_solve(nk_solver *s, nk_info *info) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    Timer t(s, info);
+    return t.done(picard(s, info));
+}
+
+nk_status nk_update_velocities(nk_solver *s, int32_t geometry, nk_info *info) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    Timer t(s, info);
+    nk_status st = NK_OK;
+    if (geometry) {
+        double **f = s->f;
+        k_geometry<<<blocks_for(s->g.Nx + 1, s->g.Ny + 1), THREADS, 0, s->stream>>>(s->g, physk(s), f[NK_FIELD_H], f[NK_FIELD_BED],
+                                                                                    f[NK_FIELD_S], f[NK_FIELD_PHI], f[NK_FIELD_PHIU],
+                                                                                    f[NK_FIELD_PHIV], f[NK_FIELD_N], f[NK_FIELD_MU]);
+        st = launch_check(s);
+    }
+    if (st == NK_OK) st = driving_stress(s);
+    if (st == NK_OK) st = picard(s, info);
+    if (st == NK_OK) st = post_process(s);
+    return t.done(st);
+}
+
+nk_status nk_rate_factor_from_temperature(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    if (s->ph.rate_factor_depth != NK_RATE_FACTOR_3D) { s->err = "the rate factor must be on the σ grid (NK_RATE_FACTOR_3D)"; return NK_INVALID_ARGUMENT; }
+    k_rate_factor<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(s->g, sig_tables(s), s->Ns, s->plane,
+                                                                           s->ph.ice_density * s->ph.gravitational_acceleration,
+                                                                           s->f[NK_FIELD_TEMPERATURE], s->f[NK_FIELD_H], s->f[NK_FIELD_A]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_multigrid_levels(const nk_solver *s, int32_t *levels, int32_t *shapes, int32_t max_levels) {
+    if (!s || !levels) return NK_INVALID_ARGUMENT;
+    int n = (int)s->lev.size() - 1;
+    *levels = n;
+    for (int l = 0; l < n && l < max_levels && shapes; l++) {
+        shapes[2 * l] = s->lev[l + 1].g.Nx;
+        shapes[2 * l + 1] = s->lev[l + 1].g.Ny;
+    }
+    return NK_OK;
+}
+
+}  // extern "C"

@@ -21,6 +21,7 @@ POINT_JACOBI, LINE_JACOBI = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 CLASS_WALL, CLASS_ICE_FREE, CLASS_FRONT, CLASS_GROUNDING = 1, 2, 4, 8
+PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE = 0, 1, 2, 3
 
 FIELDS = ["U", "V", "H", "BED", "S", "PHI", "PHIU", "PHIV", "N", "MU", "TAUU", "TAUV", "ETAH", "ETAHFF",
           "BETAU", "BETAV", "F2", "ETASIGMA", "BETABU", "BETABV", "UB", "VB", "USIGMA", "VSIGMA", "A", "C",
@@ -66,17 +67,19 @@ class SolverOptions(ct.Structure):
     _fields_ = [("linear", Control), ("picard", Control), ("preconditioner", ct.c_int32),
                 ("smoother", ct.c_int32), ("smoothing_sweeps", ct.c_int32), ("coarsest_sweeps", ct.c_int32),
                 ("minimum_coarse_size", ct.c_int32), ("precision", ct.c_int32), ("inner_reltol", ct.c_double),
-                ("inner_maxiter", ct.c_int32), ("check_interval", ct.c_int32)]
+                ("inner_maxiter", ct.c_int32), ("check_interval", ct.c_int32), ("profile_kernel", ct.c_int32)]
 
 
 class Info(ct.Structure):
     _fields_ = [("picard_iterations", ct.c_int32), ("linear_iterations", ct.c_int32),
                 ("total_linear_iterations", ct.c_int64), ("refinement_steps", ct.c_int32), ("pad", ct.c_int32),
                 ("last_picard_change", ct.c_double), ("last_picard_norm", ct.c_double),
-                ("last_residual_norm", ct.c_double), ("elapsed_ms", ct.c_double)]
+                ("last_residual_norm", ct.c_double), ("elapsed_ms", ct.c_double), ("profiled_ms", ct.c_double),
+                ("profiled_launches", ct.c_int64), ("profiled_points", ct.c_int64), ("profiled_precision", ct.c_int32),
+                ("pad2", ct.c_int32)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+        return {name: getattr(self, name) for name, _ in self._fields_ if not name.startswith("pad")}
 
 
 def default_grid(Nx, Ny, dx, dy, topology=(BOUNDED, BOUNDED), halo=4, Nsigma=10, bed_refinement=2.0):
@@ -108,7 +111,7 @@ def default_options(**kw):
     o = SolverOptions(linear=convergence(1e-8, 0.0, 10_000), picard=convergence(1e-6, 0.0, 100),
                       preconditioner=PRECONDITIONER_DIAGONAL, smoother=POINT_JACOBI, smoothing_sweeps=2,
                       coarsest_sweeps=20, minimum_coarse_size=2, precision=PRECISION_FP64, inner_reltol=1e-3,
-                      inner_maxiter=200, check_interval=8)
+                      inner_maxiter=200, check_interval=8, profile_kernel=0)
     for k, v in kw.items():
         setattr(o, k, v)
     return o

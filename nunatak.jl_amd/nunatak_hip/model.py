@@ -1,0 +1,195 @@
+"""ctypes binding of libnunatak_hip.so (include/nunatak_hip.h).
+
+``HIPModel`` owns one solver handle: the device-resident fields of one grid and the reference operations on them
+(update_geometry!, compute_driving_stress!, coefficient update, ssa_operator!, precondition!, solve!,
+picard_solve!, update_velocities!). It raises when the library is missing or a call fails; there is no CPU
+fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from . import abi
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libnunatak_hip.so")
+
+# every symbol declared in include/nunatak_hip.h
+EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_default_options", "nk_create", "nk_destroy",
+           "nk_last_error", "nk_set_physics", "nk_set_options", "nk_field_shape", "nk_set_field", "nk_get_field",
+           "nk_fill_field", "nk_get_classes", "nk_field_device_pointer", "nk_stream", "nk_synchronize",
+           "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients", "nk_apply_operator",
+           "nk_precondition", "nk_solve", "nk_picard_solve", "nk_update_velocities",
+           "nk_rate_factor_from_temperature", "nk_multigrid_levels"]
+
+_lib = None
+
+
+class NunatakError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load libnunatak_hip.so and declare its signatures. Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NunatakError(f"{path} is not built; run __graft_entry__.build() (make -C nunatak.jl_amd)")
+    L = ct.CDLL(path)
+    P, I32, I64, D = ct.c_void_p, ct.c_int32, ct.c_int64, ct.c_double
+    dp = ct.POINTER(ct.c_double)
+    L.nk_abi_version.restype = I32
+    L.nk_default_grid.argtypes = [ct.POINTER(abi.Grid)]
+    L.nk_default_physics.argtypes = [ct.POINTER(abi.Physics)]
+    L.nk_default_options.argtypes = [ct.POINTER(abi.SolverOptions)]
+    L.nk_create.argtypes = [ct.POINTER(abi.Grid), ct.POINTER(abi.Physics), ct.POINTER(abi.SolverOptions), ct.POINTER(P)]
+    L.nk_destroy.argtypes = [P]
+    L.nk_last_error.argtypes = [P]
+    L.nk_last_error.restype = ct.c_char_p
+    L.nk_set_physics.argtypes = [P, ct.POINTER(abi.Physics)]
+    L.nk_set_options.argtypes = [P, ct.POINTER(abi.SolverOptions)]
+    L.nk_field_shape.argtypes = [P, ct.c_int, ct.POINTER(ct.c_int32)]
+    L.nk_set_field.argtypes = [P, ct.c_int, dp, I64, I64]
+    L.nk_get_field.argtypes = [P, ct.c_int, dp, I64, I64]
+    L.nk_fill_field.argtypes = [P, ct.c_int, D]
+    L.nk_get_classes.argtypes = [P, P, I64, P, I64]
+    L.nk_field_device_pointer.argtypes = [P, ct.c_int, ct.POINTER(dp), ct.POINTER(I64), ct.POINTER(I64)]
+    L.nk_stream.argtypes = [P]
+    L.nk_stream.restype = P
+    for name in ("nk_synchronize", "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients",
+                 "nk_apply_operator", "nk_precondition", "nk_rate_factor_from_temperature"):
+        getattr(L, name).argtypes = [P]
+    L.nk_solve.argtypes = [P, ct.POINTER(abi.Info)]
+    L.nk_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
+    L.nk_update_velocities.argtypes = [P, I32, ct.POINTER(abi.Info)]
+    L.nk_multigrid_levels.argtypes = [P, ct.POINTER(I32), ct.POINTER(I32), I32]
+    for name in EXPORTS:
+        if name not in ("nk_last_error", "nk_stream", "nk_abi_version", "nk_default_grid", "nk_default_physics",
+                        "nk_default_options"):
+            getattr(L, name).restype = ct.c_int
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+class HIPModel:
+    """One solver handle and its device fields (field-id protocol of include/nunatak_hip.h)."""
+
+    def __init__(self, grid, physics, options):
+        self.grid, self.physics, self.options = grid, physics, options
+        self._L = load()
+        h = ct.c_void_p()
+        st = self._L.nk_create(ct.byref(grid), ct.byref(physics), ct.byref(options), ct.byref(h))
+        self._h = h
+        if st != abi.NK_OK:
+            msg = self._L.nk_last_error(h).decode() if h.value else ""
+            if h.value:
+                self._L.nk_destroy(h)
+                self._h = ct.c_void_p()
+            raise NunatakError(f"nk_create: {abi.STATUS_NAMES.get(st, st)} {msg}")
+
+    def _check(self, st, what):
+        if st != abi.NK_OK:
+            raise NunatakError(f"{what}: {abi.STATUS_NAMES.get(st, st)}: {self._L.nk_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h and self._h.value:
+            self._L.nk_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_physics(self, physics):
+        self._check(self._L.nk_set_physics(self._h, ct.byref(physics)), "nk_set_physics")
+        self.physics = physics
+
+    def set_options(self, options):
+        self._check(self._L.nk_set_options(self._h, ct.byref(options)), "nk_set_options")
+        self.options = options
+
+    def shape(self, name):
+        s = (ct.c_int32 * 3)()
+        self._check(self._L.nk_field_shape(self._h, abi.FIELD_ID[name], s), "nk_field_shape")
+        return tuple(s)
+
+    def set(self, name, value):
+        nx, ny, nz = self.shape(name)
+        a = np.empty((nz, ny, nx), dtype=np.float64)
+        a[...] = value
+        a = np.ascontiguousarray(a)
+        self._check(self._L.nk_set_field(self._h, abi.FIELD_ID[name], _dptr(a), nx, nx * ny), f"nk_set_field({name})")
+
+    def get(self, name):
+        nx, ny, nz = self.shape(name)
+        a = np.empty((nz, ny, nx), dtype=np.float64)
+        self._check(self._L.nk_get_field(self._h, abi.FIELD_ID[name], _dptr(a), nx, nx * ny), f"nk_get_field({name})")
+        return a[0] if nz == 1 else a
+
+    def fill(self, name, value):
+        self._check(self._L.nk_fill_field(self._h, abi.FIELD_ID[name], float(value)), f"nk_fill_field({name})")
+
+    def classes(self):
+        nxu, nyu, _ = self.shape("U")
+        nxv, nyv, _ = self.shape("V")
+        cu = np.zeros((nyu, nxu), dtype=np.uint8)
+        cv = np.zeros((nyv, nxv), dtype=np.uint8)
+        self._check(self._L.nk_get_classes(self._h, cu.ctypes.data, nxu, cv.ctypes.data, nxv), "nk_get_classes")
+        return cu, cv
+
+    def device_pointer(self, name):
+        p = ct.POINTER(ct.c_double)()
+        pitch, layer = ct.c_int64(), ct.c_int64()
+        self._check(self._L.nk_field_device_pointer(self._h, abi.FIELD_ID[name], ct.byref(p), ct.byref(pitch),
+                                                    ct.byref(layer)), "nk_field_device_pointer")
+        return ct.cast(p, ct.c_void_p).value, pitch.value, layer.value
+
+    def synchronize(self):
+        self._check(self._L.nk_synchronize(self._h), "nk_synchronize")
+
+    def update_geometry(self):
+        self._check(self._L.nk_update_geometry(self._h), "nk_update_geometry")
+
+    def compute_driving_stress(self):
+        self._check(self._L.nk_compute_driving_stress(self._h), "nk_compute_driving_stress")
+
+    def update_coefficients(self):
+        self._check(self._L.nk_update_coefficients(self._h), "nk_update_coefficients")
+
+    def apply_operator(self):
+        self._check(self._L.nk_apply_operator(self._h), "nk_apply_operator")
+
+    def precondition(self):
+        self._check(self._L.nk_precondition(self._h), "nk_precondition")
+
+    def rate_factor_from_temperature(self):
+        self._check(self._L.nk_rate_factor_from_temperature(self._h), "nk_rate_factor_from_temperature")
+
+    def solve(self):
+        info = abi.Info()
+        self._check(self._L.nk_solve(self._h, ct.byref(info)), "nk_solve")
+        return info.as_dict()
+
+    def picard_solve(self):
+        info = abi.Info()
+        self._check(self._L.nk_picard_solve(self._h, ct.byref(info)), "nk_picard_solve")
+        return info.as_dict()
+
+    def update_velocities(self, geometry=True):
+        info = abi.Info()
+        self._check(self._L.nk_update_velocities(self._h, 1 if geometry else 0, ct.byref(info)), "nk_update_velocities")
+        return info.as_dict()
+
+    def multigrid_levels(self):
+        n = ct.c_int32()
+        buf = (ct.c_int32 * 128)()
+        self._check(self._L.nk_multigrid_levels(self._h, ct.byref(n), buf, 64), "nk_multigrid_levels")
+        return [(buf[2 * k], buf[2 * k + 1]) for k in range(n.value)]
