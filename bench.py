@@ -38,10 +38,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make(N, precision, profile=0, check_interval=8):
+def make(N, precision, profile=0, check_interval=8, picard=None):
     g, phys, opts, fields = workloads.diva_shelf(N=N, dx=512e3 / N, precision=precision)
     opts.profile_kernel = profile
     opts.check_interval = check_interval
+    if picard:
+        opts.picard = abi.fixed_iterations(picard)
     m = HIPModel(g, phys, opts)
     workloads.load_into(m, fields)
     return m, (g, phys, opts, fields)
@@ -118,12 +120,14 @@ def main():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the velocity solve of one grid is not "
               "sharded across GPUs in this build.", file=sys.stderr)
         sys.exit(2)
     N, K, Wm = args.size, args.steps, args.warmup
+    os.environ.setdefault("NK_VERBOSE", "1")  # one library progress line per Picard iteration
 
     log(f"[bench] building FP64 and mixed handles at {N}x{N}")
     results = {}
@@ -131,7 +135,7 @@ def main():
     for label, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
         if args.only and args.only != label:
             continue
-        m, setup = make(N, prec)
+        m, setup = make(N, prec, picard=args.picard)
         models[label] = (m, setup)
         wall, infos = timed_solves(m, Wm, K, label)
         results[label] = {"wall": wall, "infos": infos, "u": m.get("U"), "v": m.get("V")}

@@ -118,6 +118,8 @@ typedef struct nk_solver_options {
     int32_t check_interval;           /* PCG iterations launched between host checks of the on-device stop flag */
     int32_t profile_kernel;           /* NK_PROFILE_*: bracket every finest-level launch of that kernel with HIP events
                                          on the handle's stream and report the summed device time in nk_info */
+    int32_t graphs;                   /* 1: capture the launches of check_interval PCG iterations in a hipGraph once
+                                         and replay it (not while profiling); 0: launch kernel by kernel */
 } nk_solver_options;
 
 /* Kernels that nk_solver_options.profile_kernel can time (finest grid only). */

@@ -674,6 +674,35 @@ __global__ __launch_bounds__(NT) void k_demote(KG g, const double *__restrict__ 
     dv[o] = 0.0f;
 }
 
+// Mixed precision preconditioning: r32 = (float) r64 over :xyz (gated like the PCG kernels)
+__global__ __launch_bounds__(NT) void k_demote_r(KG g, const double *__restrict__ ru, const double *__restrict__ rv,
+                                                 float *__restrict__ r32u, float *__restrict__ r32v, const Scal *gate) {
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    r32u[o] = (float)ru[o];
+    r32v[o] = (float)rv[o];
+}
+
+// Mixed precision preconditioning: z64 = (double) z32 and (r·z[, r·r]) → finisher
+__global__ __launch_bounds__(NT) void k_promote_dot(KG g, const float *__restrict__ z32u, const float *__restrict__ z32v,
+                                                    double *__restrict__ zu, double *__restrict__ zv, const double *__restrict__ ru,
+                                                    const double *__restrict__ rv, Red red, const Scal *gate) {
+    if (solve_done(gate)) return;
+    NK_IJ
+    double acc[2] = {0.0, 0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        const double a = (double)z32u[o], b = (double)z32v[o];
+        zu[o] = a;
+        zv[o] = b;
+        acc[0] = ru[o] * a + rv[o] * b;
+        acc[1] = ru[o] * ru[o] + rv[o] * rv[o];
+    }
+    reduce_finish<2>(acc, red);
+}
+
 // Mixed precision: x64 += (double) d32 over :xyz
 __global__ __launch_bounds__(NT) void k_promote_add(KG g, const float *__restrict__ du, const float *__restrict__ dv,
                                                     double *__restrict__ xu, double *__restrict__ xv) {

@@ -74,6 +74,9 @@ struct nk_solver {
     std::vector<void *> allocs;
     std::string err;
     bool flags_valid = false;
+    bool verbose = false;  // NK_VERBOSE=1: one progress line per Picard iteration on stderr
+    hipGraphExec_t chunk_graph = nullptr;  // captured PCG chunk (check_interval iterations)
+    int chunk_graph_key = -1;
     // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
     std::vector<hipEvent_t> prof_events;
     size_t prof_used = 0;
@@ -382,6 +385,21 @@ nk_status write_scalars(nk_solver *s, const nk_control &ctl) {
     return NK_OK;
 }
 
+// Mixed-precision preconditioner: z = V₃₂(r) with r cast to FP32 and z promoted back to FP64, then the PCG dot(s)
+// r·z (and r·r at the start) in FP64 → finisher `fin`.
+nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate) {
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny);
+    Work<double> &W = s->w64;
+    Work<float> &V = s->w32;
+    k_demote_r<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, V.ru, V.rv, gate);
+    CHK(launch_check(s));
+    Pair<float> z32;
+    CHK(v_cycle<float>(s, 0, Pair<float>{V.zu, V.zv}, Pair<float>{V.z2u, V.z2v}, V.ru, V.rv, false, gate, &z32));
+    k_promote_dot<<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.zu, W.zv, W.ru, W.rv, red(s, fin), gate);
+    return launch_check(s);
+}
+
 // conjugate_gradient! (src/Solvers/conjugate_gradient.jl:299-364) at precision T on the finest grid. When
 // r_ready, the residual r of the initial guess is already in the workspace (the mixed inner solve from zero).
 template <typename T>
@@ -399,7 +417,12 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         CHK(launch_check(s));
     }
     Pair<T> z{W.zu, W.zv};
-    if (pre == NK_PRECONDITIONER_MULTIGRID) {
+    const bool mixed = sizeof(T) == 8 && s->op.precision == NK_PRECISION_MIXED && pre == NK_PRECONDITIONER_MULTIGRID;
+    if (mixed) {
+        CHK(precondition_fp32(s, FIN_INIT, nullptr));
+        k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, W.zu, W.zv, W.pu, W.pv, 1, nullptr);
+        CHK(launch_check(s));
+    } else if (pre == NK_PRECONDITIONER_MULTIGRID) {
         CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, false, nullptr, &z));
         k_dot<T><<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, z.u, z.v, W.ru, W.rv, red(s, FIN_INIT), nullptr);
         CHK(launch_check(s));
@@ -411,9 +434,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         CHK(launch_check(s));
     }
     const int chunk = s->op.check_interval > 0 ? s->op.check_interval : 8;
-    for (;;) {
-        CHK(read_scalars(s));
-        if (s->S_host->done) break;
+    auto launch_chunk = [&]() -> nk_status {
         for (int k = 0; k < chunk; k++) {
             {
                 ProfScope ps(s, NK_PROFILE_OPERATOR, 0, (int)sizeof(T));
@@ -426,8 +447,9 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
                     k_update_xr<T><<<B, THREADS, 0, s->stream>>>(g, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, red(s, FIN_RES), s->S);
                 }
                 CHK(launch_check(s));
-                Pair<T> zz;
-                CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, true, s->S, &zz));
+                Pair<T> zz{W.zu, W.zv};
+                if (mixed) CHK(precondition_fp32(s, FIN_BETA, s->S));
+                else CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, true, s->S, &zz));
                 k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, zz.u, zz.v, W.pu, W.pv, 0, s->S);
                 CHK(launch_check(s));
             } else {
@@ -441,73 +463,38 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
                 CHK(launch_check(s));
             }
         }
+        return NK_OK;
+    };
+    // The chunk's launches have fixed arguments for a given solve path and vector buffers, so they are captured once
+    // and replayed; the kernels read the device stop flag, so a replay after convergence does no work.
+    const bool use_graph = s->op.graphs && s->op.profile_kernel == NK_PROFILE_NONE;
+    const int key = (int)sizeof(T) * 1000 + (mixed ? 500 : 0) + pre * 100 + (xu == s->f[NK_FIELD_U] ? 1 : 0) + chunk * 10000;
+    if (use_graph && s->chunk_graph_key != key) {
+        if (s->chunk_graph) { (void)hipGraphExecDestroy(s->chunk_graph); s->chunk_graph = nullptr; }
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        nk_status cst = launch_chunk();
+        hipError_t e = hipStreamEndCapture(s->stream, &graph);
+        if (cst != NK_OK) return cst;
+        if (e != hipSuccess) { s->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e); return NK_DEVICE_ERROR; }
+        e = hipGraphInstantiate(&s->chunk_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) { s->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e); return NK_DEVICE_ERROR; }
+        s->chunk_graph_key = key;
+    }
+    for (;;) {
+        CHK(read_scalars(s));
+        if (s->S_host->done) break;
+        if (use_graph) HIPCHK(hipGraphLaunch(s->chunk_graph, s->stream));
+        else CHK(launch_chunk());
     }
     *iters = s->S_host->it;
     *res = s->S_host->res;
     return NK_OK;
 }
 
-// Mixed precision linear solve: FP64 defect correction around FP32 PCG + FP32 V-cycle. The outer stopping rule
-// is the reference's, on the true FP64 residual: ‖b − A x‖ ≤ max(reltol ‖r₀‖, abstol) or maxiter inner iterations.
-nk_status solve_mixed(nk_solver *s, nk_info *info) {
-    Level &L0 = s->lev[0];
-    const KG &g = L0.g;
-    const dim3 B = blocks_for(g.Nx, g.Ny);
-    Coef<double> c = coef<double>(L0);
-    const double wf = s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0;
-    double *xu = s->f[NK_FIELD_U], *xv = s->f[NK_FIELD_V];
-    const double *bu = s->f[NK_FIELD_TAUU], *bv = s->f[NK_FIELD_TAUV];
-    Work<double> &W = s->w64;
-    Work<float> &V = s->w32;
-    const nk_control &ctl = s->op.linear;
-    const bool fixed = ctl.kind == NK_FIXED_ITERATIONS;
-    const int maxiter = fixed ? ctl.iterations : ctl.maxiter;
-    auto residual = [&](double *out) -> nk_status {
-        k_apply<double><<<B, THREADS, 0, s->stream>>>(g, xu, xv, c, wf, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
-        CHK(launch_check(s));
-        k_dot<double><<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, W.ru, W.rv, nullptr, nullptr, red(s, FIN_NONE), nullptr);
-        CHK(launch_check(s));
-        CHK(read_scalars(s));
-        *out = std::sqrt(s->S_host->out[0]);
-        return NK_OK;
-    };
-    double r0;
-    CHK(residual(&r0));
-    const double tol = fixed ? -1.0 : jmax_h(ctl.reltol * r0, ctl.abstol);
-    double res = r0;
-    int total = 0, steps = 0;
-    while ((fixed ? total < maxiter : (res > tol && total < maxiter))) {
-        k_demote<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, V.ru, V.rv, V.du, V.dv);
-        CHK(launch_check(s));
-        nk_control inner;
-        std::memset(&inner, 0, sizeof(inner));
-        inner.kind = NK_CONVERGENCE;
-        inner.reltol = s->op.inner_reltol;
-        inner.abstol = 0.0;
-        int budget = maxiter - total;
-        inner.maxiter = s->op.inner_maxiter < budget ? s->op.inner_maxiter : budget;
-        int it = 0;
-        double ires = 0.0;
-        CHK(pcg<float>(s, V.du, V.dv, nullptr, nullptr, inner, true, &it, &ires));
-        k_promote_add<<<B, THREADS, 0, s->stream>>>(g, V.du, V.dv, xu, xv);
-        CHK(launch_check(s));
-        total += it > 0 ? it : 1;
-        steps++;
-        CHK(residual(&res));
-        if (it == 0) break;
-    }
-    if (info) {
-        info->linear_iterations = total;
-        info->total_linear_iterations += total;
-        info->refinement_steps = steps;
-        info->last_residual_norm = res;
-    }
-    return NK_OK;
-}
-
 nk_status solve_linear(nk_solver *s, nk_info *info) {
     CHK(update_preconditioner(s));
-    if (s->op.precision == NK_PRECISION_MIXED) return solve_mixed(s, info);
     int it = 0;
     double res = 0.0;
     CHK(pcg<double>(s, s->f[NK_FIELD_U], s->f[NK_FIELD_V], s->f[NK_FIELD_TAUU], s->f[NK_FIELD_TAUV], s->op.linear, false, &it, &res));
@@ -615,6 +602,11 @@ nk_status picard(nk_solver *s, nk_info *info) {
             X = s->S_host->picard_X;
         }
         it++;
+        if (s->verbose) {
+            std::fprintf(stderr, "[nunatak_hip] picard %d: linear %d (total %lld), change %.3e, norm %.3e\n", it,
+                         info ? info->linear_iterations : -1, info ? (long long)info->total_linear_iterations : -1LL, D, X);
+            std::fflush(stderr);
+        }
     }
     if (info) {
         info->picard_iterations = it;
@@ -679,8 +671,8 @@ nk_status validate(const nk_grid *g, const nk_physics *ph, const nk_solver_optio
         err = "unknown preconditioner or precision";
         return NK_INVALID_ARGUMENT;
     }
-    if (op->precision == NK_PRECISION_MIXED && op->preconditioner == NK_PRECONDITIONER_NONE) {
-        err = "mixed precision needs a preconditioner";
+    if (op->precision == NK_PRECISION_MIXED && op->preconditioner != NK_PRECONDITIONER_MULTIGRID) {
+        err = "mixed precision runs the multigrid V-cycle in FP32: it needs the multigrid preconditioner";
         return NK_INVALID_ARGUMENT;
     }
     return NK_OK;
@@ -726,6 +718,7 @@ void nk_default_options(nk_solver_options *o) {
     o->smoother = NK_POINT_JACOBI; o->smoothing_sweeps = 2; o->coarsest_sweeps = 20; o->minimum_coarse_size = 2;
     o->precision = NK_PRECISION_FP64; o->inner_reltol = 1e-3; o->inner_maxiter = 200; o->check_interval = 8;
     o->profile_kernel = NK_PROFILE_NONE;
+    o->graphs = 1;
 }
 
 nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_solver_options *options, nk_solver **out) {
@@ -739,6 +732,10 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
     }
     nk_solver *s = new nk_solver();
     s->grid = *grid; s->ph = *physics; s->op = *options;
+    {
+        const char *v = std::getenv("NK_VERBOSE");
+        s->verbose = v && v[0] && v[0] != '0';
+    }
     s->Ns = grid->Nsigma;
     s->g = make_kg(grid->Nx, grid->Ny, grid->topology_x, grid->topology_y, grid->dx, grid->dy);
     s->plane = plane_of(s->g);
@@ -831,6 +828,7 @@ nk_status nk_destroy(nk_solver *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void *p : s->allocs) (void)hipFree(p);
     if (s->S_host) (void)hipHostFree(s->S_host);
+    if (s->chunk_graph) (void)hipGraphExecDestroy(s->chunk_graph);
     for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -861,6 +859,8 @@ nk_status nk_set_options(nk_solver *s, const nk_solver_options *o) {
         }
     }
     s->op = *o;
+    if (s->chunk_graph) { (void)hipGraphExecDestroy(s->chunk_graph); s->chunk_graph = nullptr; }
+    s->chunk_graph_key = -1;
     return NK_OK;
 }
 

@@ -67,7 +67,8 @@ class SolverOptions(ct.Structure):
     _fields_ = [("linear", Control), ("picard", Control), ("preconditioner", ct.c_int32),
                 ("smoother", ct.c_int32), ("smoothing_sweeps", ct.c_int32), ("coarsest_sweeps", ct.c_int32),
                 ("minimum_coarse_size", ct.c_int32), ("precision", ct.c_int32), ("inner_reltol", ct.c_double),
-                ("inner_maxiter", ct.c_int32), ("check_interval", ct.c_int32), ("profile_kernel", ct.c_int32)]
+                ("inner_maxiter", ct.c_int32), ("check_interval", ct.c_int32), ("profile_kernel", ct.c_int32),
+                ("graphs", ct.c_int32)]
 
 
 class Info(ct.Structure):
@@ -111,7 +112,8 @@ def default_options(**kw):
     o = SolverOptions(linear=convergence(1e-8, 0.0, 10_000), picard=convergence(1e-6, 0.0, 100),
                       preconditioner=PRECONDITIONER_DIAGONAL, smoother=POINT_JACOBI, smoothing_sweeps=2,
                       coarsest_sweeps=20, minimum_coarse_size=2, precision=PRECISION_FP64, inner_reltol=1e-3,
-                      inner_maxiter=200, check_interval=8, profile_kernel=0)
+                      inner_maxiter=200, check_interval=8, profile_kernel=0,
+                      graphs=1)
     for k, v in kw.items():
         setattr(o, k, v)
     return o

@@ -253,3 +253,28 @@ def test_mixed_precision_matches_fp64():
     say(f"  fp64 picard {out[0][2]['picard_iterations']} cg {out[0][2]['total_linear_iterations']}; mixed picard "
         f"{out[1][2]['picard_iterations']} inner {out[1][2]['total_linear_iterations']}; rel L2 {err:.2e}")
     assert err <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
+def test_graph_replays_match_eager_launches(precision):
+    # PCG chunks replayed from a captured hipGraph must give the eager result bit for bit, solve after solve
+    say(f"graph vs eager precision={precision}")
+    res = {}
+    for graphs in (0, 1):
+        g, phys, opts, f = W.diva_shelf(N=96, dx=512e3 / 96, precision=precision)
+        opts.graphs = graphs
+        opts.check_interval = 3
+        m = HIPModel(g, phys, opts)
+        W.load_into(m, f)
+        runs = []
+        for rep in range(3):
+            for name in ("U", "V", "ETASIGMA", "F2", "BETABU", "BETABV"):
+                m.fill(name, 0.0)
+            info = m.update_velocities()
+            runs.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+            m.synchronize()
+        res[graphs] = runs
+    for k in range(3):
+        assert res[0][k][2] == res[1][k][2]
+        assert np.array_equal(res[0][k][0], res[1][k][0]) and np.array_equal(res[0][k][1], res[1][k][1])
+        assert np.array_equal(res[1][k][0], res[1][0][0])
