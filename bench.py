@@ -38,9 +38,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make(N, precision, profile=0, check_interval=8, picard=None):
+def make(N, precision, profile=0, check_interval=8, picard=None, graphs=1):
     g, phys, opts, fields = workloads.diva_shelf(N=N, dx=512e3 / N, precision=precision)
     opts.profile_kernel = profile
+    opts.graphs = graphs
     opts.check_interval = check_interval
     if picard:
         opts.picard = abi.fixed_iterations(picard)
@@ -121,6 +122,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
     ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
+    ap.add_argument("--graphs", type=int, default=1, help="0: launch kernel by kernel (for external profilers)")
+    ap.add_argument("--no-roofline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the velocity solve of one grid is not "
@@ -135,7 +138,7 @@ def main():
     for label, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
         if args.only and args.only != label:
             continue
-        m, setup = make(N, prec, picard=args.picard)
+        m, setup = make(N, prec, picard=args.picard, graphs=args.graphs)
         models[label] = (m, setup)
         wall, infos = timed_solves(m, Wm, K, label)
         results[label] = {"wall": wall, "infos": infos, "u": m.get("U"), "v": m.get("V")}
@@ -170,6 +173,9 @@ def main():
         out["mixed_speedup_vs_fp64"] = out["value"] / out["fp64_solves_per_s"]
 
     # roofline of the dominant kernel: one more solve with event brackets on the FP32 (or FP64) sweep
+    if args.no_roofline:
+        print(json.dumps(out), flush=True)
+        return
     m, setup = models[main_label]
     opts = setup[2]
     opts.profile_kernel = abi.PROFILE_SWEEP

@@ -27,10 +27,13 @@ struct KG {
     double dx, dy;
 };
 
+// Periodic index map onto 1..N for |offset| ≤ 2N, branch-free (selects only).
 __device__ __forceinline__ int wrapi(int i, int N) {
-    if (i >= 1 && i <= N) return i;
-    int r = (i - 1) % N;
-    return (r < 0 ? r + N : r) + 1;
+    i = i < 1 ? i + N : i;
+    i = i < 1 ? i + N : i;
+    i = i > N ? i - N : i;
+    i = i > N ? i - N : i;
+    return i;
 }
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 __device__ __forceinline__ long off(const KG &g, int i, int j) { return (long)(j - 1) * g.pitch + (i - 1); }
@@ -62,29 +65,37 @@ __device__ __forceinline__ double jclamp(double x, double lo, double hi) { retur
 // Activity flags: bit 0 = u point active, bit 1 = v point active (not a wall, not ice free on both sides).
 // inactiveᶠᶜᶜ / inactiveᶜᶠᶜ of src/StressBalances/ssa_operator.jl:91-92 evaluated once per coefficient update.
 // ---------------------------------------------------------------------------------------------------------------
+// Branch-free: the flag byte is always loaded from an in-range (clamped or wrapped) address and the range test is a
+// select, so every load of a stencil point can be issued before any of them returns.
 __device__ __forceinline__ bool uact(const KG &g, const uint8_t *act, int i, int j) {
-    if (g.tx) { if (i < 1 || i > g.Nx + 1) return false; } else i = wrapi(i, g.Nx);
-    if (g.ty) { if (j < 1 || j > g.Ny) return false; } else j = wrapi(j, g.Ny);
-    return act[off(g, i, j)] & 1;
+    const bool in = (!g.tx || (i >= 1 && i <= g.Nx + 1)) && (!g.ty || (j >= 1 && j <= g.Ny));
+    const int ii = g.tx ? clampi(i, 1, g.Nx + 1) : wrapi(i, g.Nx);
+    const int jj = g.ty ? clampi(j, 1, g.Ny) : wrapi(j, g.Ny);
+    return in & (bool)(act[off(g, ii, jj)] & 1);
 }
 __device__ __forceinline__ bool vact(const KG &g, const uint8_t *act, int i, int j) {
-    if (g.tx) { if (i < 1 || i > g.Nx) return false; } else i = wrapi(i, g.Nx);
-    if (g.ty) { if (j < 1 || j > g.Ny + 1) return false; } else j = wrapi(j, g.Ny);
-    return (act[off(g, i, j)] >> 1) & 1;
+    const bool in = (!g.tx || (i >= 1 && i <= g.Nx)) && (!g.ty || (j >= 1 && j <= g.Ny + 1));
+    const int ii = g.tx ? clampi(i, 1, g.Nx) : wrapi(i, g.Nx);
+    const int jj = g.ty ? clampi(j, 1, g.Ny + 1) : wrapi(j, g.Ny);
+    return in & (bool)((act[off(g, ii, jj)] >> 1) & 1);
 }
 // interior_u / interior_v with the stiffness mask (ssa_operator.jl:94-95)
 template <typename T> __device__ __forceinline__ T iu(const KG &g, const uint8_t *act, const T *u, int i, int j) {
-    return uact(g, act, i, j) ? u[off(g, fx(g, i), cy(g, j))] : T(0);
+    const T val = u[off(g, fx(g, i), cy(g, j))];
+    return uact(g, act, i, j) ? val : T(0);
 }
 template <typename T> __device__ __forceinline__ T iv(const KG &g, const uint8_t *act, const T *v, int i, int j) {
-    return vact(g, act, i, j) ? v[off(g, cx(g, i), fy(g, j))] : T(0);
+    const T val = v[off(g, cx(g, i), fy(g, j))];
+    return vact(g, act, i, j) ? val : T(0);
 }
 // interior_u / interior_v without a stiffness (walls only), as the viscosity sees the velocity
 template <typename T> __device__ __forceinline__ T iu_w(const KG &g, const T *u, int i, int j) {
-    return periph_fc(g, i, j) ? T(0) : u[off(g, fx(g, i), cy(g, j))];
+    const T val = u[off(g, fx(g, i), cy(g, j))];
+    return periph_fc(g, i, j) ? T(0) : val;
 }
 template <typename T> __device__ __forceinline__ T iv_w(const KG &g, const T *v, int i, int j) {
-    return periph_cf(g, i, j) ? T(0) : v[off(g, cx(g, i), fy(g, j))];
+    const T val = v[off(g, cx(g, i), fy(g, j))];
+    return periph_cf(g, i, j) ? T(0) : val;
 }
 template <typename T> __device__ __forceinline__ T ldc(const KG &g, const T *f, int i, int j) { return f[off(g, cx(g, i), cy(g, j))]; }
 template <typename T> __device__ __forceinline__ T ldfc(const KG &g, const T *f, int i, int j) { return f[off(g, fx(g, i), cy(g, j))]; }
@@ -120,9 +131,9 @@ __device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu
     T exy_00 = ((u00 - u0m) / dy + (v00 - vm0) / dx) / T(2);
     T exy_01 = ((u0p - u00) / dy + (v01 - vm1) / dx) / T(2);
     T exy_10 = ((u10 - u1m) / dy + (v10 - v00) / dx) / T(2);
-    if (periph_ff(g, i, j)) exy_00 = wf * exy_00;
-    if (periph_ff(g, i, j + 1)) exy_01 = wf * exy_01;
-    if (periph_ff(g, i + 1, j)) exy_10 = wf * exy_10;
+    exy_00 = periph_ff(g, i, j) ? wf * exy_00 : exy_00;
+    exy_01 = periph_ff(g, i, j + 1) ? wf * exy_01 : exy_01;
+    exy_10 = periph_ff(g, i + 1, j) ? wf * exy_10 : exy_10;
     const T txy_00 = T(2) * ldff(g, c.etaff, i, j) * exy_00;
     const T txy_01 = T(2) * ldff(g, c.etaff, i, j + 1) * exy_01;
     const T txy_10 = T(2) * ldff(g, c.etaff, i + 1, j) * exy_10;
@@ -238,10 +249,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block-reduce NV values, then the last block sums every block's slot in block order and runs the finisher.
+// Block-reduce NV values and store the block's sums in its own slots (plain stores; the kernel boundary makes them
+// visible). k_finish then sums every block's slots in block order and runs the finisher: no atomics and no
+// device-scope fences, which on a multi-XCD part would write back and invalidate the XCD's L2 once per block.
 template <int NV> __device__ void reduce_finish(double (&v)[NV], const Red &r) {
     __shared__ double sh[NV][NT / 64];
-    __shared__ int last;
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int nwaves = (blockDim.x * blockDim.y) >> 6;
@@ -260,35 +272,33 @@ template <int NV> __device__ void reduce_finish(double (&v)[NV], const Red &r) {
             for (int w = 0; w < nwaves; w++) s += sh[k][w];
             r.partials[(long)k * nblocks + bid] = s;
         }
-        __threadfence();
-        unsigned int prev = atomicAdd(r.counter, 1u);
-        last = (prev == (unsigned)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double tot[MAXRED] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double s = 0.0;
-        for (int b = tid; b < nblocks; b += blockDim.x * blockDim.y)
-            s += __hip_atomic_load(&r.partials[(long)k * nblocks + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = wave_sum(s);
-        __syncthreads();
-        if (lane == 0) sh[k][wave] = s;
-        __syncthreads();
-        double t = 0.0;
-        for (int w = 0; w < nwaves; w++) t += sh[k][w];
-        tot[k] = t;
-    }
-    if (tid == 0) {
-        finish(r.S, r.fin, tot);
-        *r.counter = 0u;
-        __threadfence();
     }
 }
 
 __device__ __forceinline__ bool solve_done(const Scal *S) { return S != nullptr && S->done; }
+
+// Sum NV block partials of the preceding kernel in block order and run the finisher (one block).
+__global__ __launch_bounds__(NT) void k_finish(const double *__restrict__ partials, int nblocks, int nv, Scal *S, int fin,
+                                               const Scal *gate) {
+    if (solve_done(gate)) return;
+    __shared__ double sh[NT / 64];
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int nth = blockDim.x * blockDim.y, nwaves = nth >> 6;
+    double tot[MAXRED] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nv; k++) {
+        double s = 0.0;
+        for (int b = tid; b < nblocks; b += nth) s += partials[(long)k * nblocks + b];
+        s = wave_sum(s);
+        __syncthreads();
+        if (lane == 0) sh[wave] = s;
+        __syncthreads();
+        double t = 0.0;
+        for (int w = 0; w < nwaves; w++) t += sh[w];
+        tot[k] = t;
+    }
+    if (tid == 0) finish(S, fin, tot);
+}
 
 #define NK_IJ                                                 \
     const int i = blockIdx.x * BX + threadIdx.x + 1;          \
@@ -299,10 +309,11 @@ __device__ __forceinline__ bool solve_done(const Scal *S) { return S != nullptr 
 // ---------------------------------------------------------------------------------------------------------------
 
 // q = A p (+ p·q → α). With b != nullptr writes r = b − A p instead (initial residual, iterative refinement).
-template <typename T>
+template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_apply(KG g, const T *__restrict__ pu, const T *__restrict__ pv, Coef<T> c, T wf,
                                               T *__restrict__ qu, T *__restrict__ qv, const T *__restrict__ bu,
                                               const T *__restrict__ bv, Red red, int nred, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ
     double acc[1] = {0.0};
@@ -323,12 +334,13 @@ __global__ __launch_bounds__(NT) void k_apply(KG g, const T *__restrict__ pu, co
 
 // Jacobi step of the PCG with the diagonal preconditioner:
 //   x += α p; r += (−α) q; z = r / d; (r·z, r·r) → β, ρ, res, done.  With init != 0: z = r / d, p = z, (r·z, r·r).
-template <typename T>
+template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int precond, T *__restrict__ xu, T *__restrict__ xv,
                                                     const T *__restrict__ pu, const T *__restrict__ pv, const T *__restrict__ qu,
                                                     const T *__restrict__ qv, T *__restrict__ ru, T *__restrict__ rv,
                                                     T *__restrict__ zu, T *__restrict__ zv, T *__restrict__ pu_out,
                                                     T *__restrict__ pv_out, int init, Red red, const Scal *S) {
+    g.tx = TX; g.ty = TY;
     if (solve_done(S)) return;
     NK_IJ
     double acc[2] = {0.0, 0.0};
@@ -446,10 +458,11 @@ __global__ __launch_bounds__(NT) void k_picard_change(KG g, double *__restrict__
 // Fused ℓ1-Jacobi sweep x_out = x + (b − A x) / s (src/StressBalances/multigrid_preconditioner.jl:165-219).
 // from_zero: x = 0, so A x = 0 and x_out = 0 + (b − 0) / s exactly as the reference computes it.
 // With nred: Σ b·x_out → finisher (the r·z of the PCG at the finest level).
-template <typename T>
+template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c, T wf,
                                               const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
                                               T *__restrict__ ov, int from_zero, Red red, int nred, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ
     double acc[1] = {0.0};
@@ -472,11 +485,31 @@ __global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, co
     if (nred) reduce_finish<1>(acc, red);
 }
 
+// First ℓ1-Jacobi sweep from x = 0: A·0 = 0, so x = 0 + (b − 0)/s exactly as the reference evaluates it.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_zero(KG g, Coef<T> c, T wf, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                   T *__restrict__ ou, T *__restrict__ ov, Red red, int nred, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    double acc[1] = {0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        const T nu = T(0) + (bu[o] - T(0)) / x_l1(g, c, i, j, wf);
+        const T nv = T(0) + (bv[o] - T(0)) / y_l1(g, c, i, j, wf);
+        ou[o] = nu;
+        ov[o] = nv;
+        if (nred) acc[0] = (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
+    }
+    if (nred) reduce_finish<1>(acc, red);
+}
+
 // Thomas algorithm of ℓ1 line Jacobi along y (multigrid_preconditioner.jl:221-259): one thread per x index.
-template <typename T>
+template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_line_relax(KG g, T *__restrict__ xu, T *__restrict__ xv, const T *__restrict__ bu,
                                                    const T *__restrict__ bv, T *__restrict__ qu, T *__restrict__ qv,
                                                    T *__restrict__ t, Coef<T> c, T wf, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     const int i = blockIdx.x * NT + threadIdx.x + threadIdx.y * BX + 1;
     if (i > g.Nx) return;
@@ -543,11 +576,13 @@ __device__ __forceinline__ void parents(int i, int Nc, int bounded, int face, in
 }
 
 // _restrict! (multigrid_preconditioner.jl:346-352): coarse b = R (fine r with inactive points zeroed)
-template <typename T>
+template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_restrict(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
                                                  const uint8_t *__restrict__ fact, const T *__restrict__ ru,
                                                  const T *__restrict__ rv, T *__restrict__ bu, T *__restrict__ bv,
                                                  const Scal *gate) {
+    cg.tx = TX; cg.ty = TY;
+    fg.tx = TX; fg.ty = TY;
     if (solve_done(gate)) return;
     const int I = blockIdx.x * BX + threadIdx.x + 1, J = blockIdx.y * BY + threadIdx.y + 1;
     if (I > cg.Nx || J > cg.Ny) return;
@@ -572,11 +607,13 @@ __global__ __launch_bounds__(NT) void k_restrict(KG cg, KG fg, int hx, int hy, c
 }
 
 // _prolongate! (multigrid_preconditioner.jl:354-360): fine x += P (coarse x with inactive points zeroed)
-template <typename T>
+template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_prolongate(KG fg, KG cg, int hx, int hy, const uint8_t *__restrict__ fact,
                                                    const uint8_t *__restrict__ cact, const T *__restrict__ cu,
                                                    const T *__restrict__ cv, T *__restrict__ xu, T *__restrict__ xv,
                                                    const Scal *gate) {
+    fg.tx = TX; fg.ty = TY;
+    cg.tx = TX; cg.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ
     if (i > fg.Nx || j > fg.Ny) return;
@@ -733,6 +770,181 @@ __global__ __launch_bounds__(NT) void k_copy2(KG g, const T *__restrict__ xu, co
     const long o = off(g, i, j);
     zu[o] = T(1) * xu[o] + T(0) * xu[o];
     zv[o] = T(1) * xv[o] + T(0) * xv[o];
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Coarse tail of the V-cycle in one workgroup: every level from `top` down to the coarsest lives in LDS, and each
+// step of v_cycle! (fill, ℓ1 sweep, residual, restriction, prolongation) is one barrier-separated pass of the
+// workgroup over a level. The stencil functions are the ones the grid-wide kernels use, so the result is the same
+// bit for bit; only the launch latency of ~40 tiny kernels per V-cycle disappears.
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int TAIL_MAX = 12;
+constexpr int TAIL_THREADS = 1024;
+
+template <typename T> struct TailLevel {
+    KG lg;                    // LDS geometry (pitch = Nx + 1)
+    int gpitch;               // pitch of the level's global arrays
+    int hx, hy;               // halved relative to the next finer level
+    const T *eta, *etaff, *bu, *bv;
+    const uint8_t *act;
+    int o_eta, o_etaff, o_bu, o_bv, o_xu, o_xv, o_bu_, o_bv_, o_qu, o_qv, o_act;  // LDS offsets in elements of T / bytes
+};
+
+template <typename T> struct TailDesc {
+    int n, sweeps, coarsest;
+    T wf;
+    const T *gbu, *gbv;  // right-hand side of the top tail level (global, top level's pitch)
+    T *gxu, *gxv;        // result of the top tail level (global)
+    TailLevel<T> lv[TAIL_MAX];
+};
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TAIL_THREADS) void k_coarse_cycle(TailDesc<T> d, const Scal *gate) {
+    if (solve_done(gate)) return;
+    auto geom = [](const KG &k) { KG g = k; g.tx = TX; g.ty = TY; return g; };
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *sm = reinterpret_cast<T *>(smem);
+    const int tid = threadIdx.x;
+    const int nth = blockDim.x;
+    // load coefficients and flags of every tail level; zero the vectors
+    for (int l = 0; l < d.n; l++) {
+        const TailLevel<T> &L = d.lv[l];
+        const int nx = L.lg.Nx + 1, ny = L.lg.Ny + 1;
+        for (int p = tid; p < nx * ny; p += nth) {
+            const int i = p % nx, j = p / nx;
+            const long go = (long)j * L.gpitch + i;
+            const int lo = j * L.lg.pitch + i;
+            sm[L.o_eta + lo] = L.eta[go];
+            sm[L.o_etaff + lo] = L.etaff[go];
+            sm[L.o_bu + lo] = L.bu[go];
+            sm[L.o_bv + lo] = L.bv[go];
+            smem[L.o_act + lo] = L.act[go];
+            sm[L.o_xu + lo] = T(0); sm[L.o_xv + lo] = T(0);
+            sm[L.o_qu + lo] = T(0); sm[L.o_qv + lo] = T(0);
+            sm[L.o_bu_ + lo] = l == 0 ? d.gbu[go] : T(0);
+            sm[L.o_bv_ + lo] = l == 0 ? d.gbv[go] : T(0);
+        }
+    }
+    __syncthreads();
+    auto coef = [&](int l) {
+        const TailLevel<T> &L = d.lv[l];
+        return Coef<T>{sm + L.o_eta, sm + L.o_etaff, sm + L.o_bu, sm + L.o_bv, smem + L.o_act};
+    };
+    // x = 0 + (b − 0)/s, then sweeps−1 two-phase sweeps q = A x; x += (b − q)/s (Jacobi: all q before any x)
+    auto smooth = [&](int l, int sweeps, bool from_zero) {
+        const TailLevel<T> &L = d.lv[l];
+        const KG g = geom(L.lg);
+        const Coef<T> c = coef(l);
+        T *xu = sm + L.o_xu, *xv = sm + L.o_xv, *qu = sm + L.o_qu, *qv = sm + L.o_qv;
+        const T *bu = sm + L.o_bu_, *bv = sm + L.o_bv_;
+        for (int k = 0; k < sweeps; k++) {
+            const bool z = from_zero && k == 0;
+            if (!z) {
+                for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+                    const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+                    T a, b;
+                    ssa_apply(g, i, j, xu, xv, c, d.wf, a, b);
+                    const long o = off(g, i, j);
+                    qu[o] = a; qv[o] = b;
+                }
+                __syncthreads();
+            }
+            for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+                const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+                const long o = off(g, i, j);
+                const T x_u = z ? T(0) : xu[o], x_v = z ? T(0) : xv[o];
+                const T q_u = z ? T(0) : qu[o], q_v = z ? T(0) : qv[o];
+                xu[o] = x_u + (bu[o] - q_u) / x_l1(g, c, i, j, d.wf);
+                xv[o] = x_v + (bv[o] - q_v) / y_l1(g, c, i, j, d.wf);
+            }
+            __syncthreads();
+        }
+    };
+    auto zero_x = [&](int l) {
+        const TailLevel<T> &L = d.lv[l];
+        const int nx = L.lg.Nx + 1, ny = L.lg.Ny + 1;
+        for (int p = tid; p < nx * ny; p += nth) {
+            const int lo = (p / nx) * L.lg.pitch + p % nx;
+            sm[L.o_xu + lo] = T(0); sm[L.o_xv + lo] = T(0);
+        }
+        __syncthreads();
+    };
+    // down leg
+    int l = 0;
+    for (; l < d.n - 1; l++) {
+        const TailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
+        const KG g = geom(L.lg), cg = geom(C.lg);
+        const Coef<T> c = coef(l);
+        if (d.sweeps == 0) zero_x(l);
+        smooth(l, d.sweeps, true);
+        T *xu = sm + L.o_xu, *xv = sm + L.o_xv, *qu = sm + L.o_qu, *qv = sm + L.o_qv;
+        const T *bu = sm + L.o_bu_, *bv = sm + L.o_bv_;
+        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+            const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+            T a, b;
+            ssa_apply(g, i, j, xu, xv, c, d.wf, a, b);
+            const long o = off(g, i, j);
+            qu[o] = T(1) * bu[o] + T(-1) * a;
+            qv[o] = T(1) * bv[o] + T(-1) * b;
+        }
+        __syncthreads();
+        const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
+        T *cbu = sm + C.o_bu_, *cbv = sm + C.o_bv_;
+        for (int p = tid; p < cg.Nx * cg.Ny; p += nth) {
+            const int I = p % cg.Nx + 1, J = p / cg.Nx + 1;
+            T su = T(0), sv = T(0);
+            for (int a = (C.hx ? -1 : 0); a <= (C.hx ? 1 : 0); a++)
+                for (int b = (C.hy ? -1 : 0); b <= (C.hy ? 2 : 0); b++) {
+                    const int i = r_index(I, a, g.Nx, g.tx, 1, C.hx), j = r_index(J, b, g.Ny, g.ty, 0, C.hy);
+                    su += r_weight<T>(I, a, cg.Nx, g.tx, 1, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 0, C.hy) * iu(g, fact, qu, i, j);
+                }
+            for (int a = (C.hx ? -1 : 0); a <= (C.hx ? 2 : 0); a++)
+                for (int b = (C.hy ? -1 : 0); b <= (C.hy ? 1 : 0); b++) {
+                    const int i = r_index(I, a, g.Nx, g.tx, 0, C.hx), j = r_index(J, b, g.Ny, g.ty, 1, C.hy);
+                    sv += r_weight<T>(I, a, cg.Nx, g.tx, 0, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 1, C.hy) * iv(g, fact, qv, i, j);
+                }
+            const long o = off(cg, I, J);
+            cbu[o] = (cact[o] & 1) ? su : T(0);
+            cbv[o] = (cact[o] & 2) ? sv : T(0);
+        }
+        __syncthreads();
+    }
+    // coarsest level
+    if (d.coarsest == 0) zero_x(l);
+    smooth(l, d.coarsest, true);
+    // up leg
+    for (l = d.n - 2; l >= 0; l--) {
+        const TailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
+        const KG g = geom(L.lg), cg = geom(C.lg);
+        const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
+        const T *cu = sm + C.o_xu, *cv = sm + C.o_xv;
+        T *xu = sm + L.o_xu, *xv = sm + L.o_xv;
+        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+            const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+            int I1, I2, J1, J2;
+            T wx1, wx2, wy1, wy2;
+            parents<T>(i, cg.Nx, g.tx, 1, C.hx, I1, I2, wx1, wx2);
+            parents<T>(j, cg.Ny, g.ty, 0, C.hy, J1, J2, wy1, wy2);
+            const T pu = wx1 * (wy1 * iu(cg, cact, cu, I1, J1) + wy2 * iu(cg, cact, cu, I1, J2)) +
+                         wx2 * (wy1 * iu(cg, cact, cu, I2, J1) + wy2 * iu(cg, cact, cu, I2, J2));
+            parents<T>(i, cg.Nx, g.tx, 0, C.hx, I1, I2, wx1, wx2);
+            parents<T>(j, cg.Ny, g.ty, 1, C.hy, J1, J2, wy1, wy2);
+            const T pv = wx1 * (wy1 * iv(cg, cact, cv, I1, J1) + wy2 * iv(cg, cact, cv, I1, J2)) +
+                         wx2 * (wy1 * iv(cg, cact, cv, I2, J1) + wy2 * iv(cg, cact, cv, I2, J2));
+            const long o = off(g, i, j);
+            xu[o] += (fact[o] & 1) ? pu : T(0);
+            xv[o] += (fact[o] & 2) ? pv : T(0);
+        }
+        __syncthreads();
+        smooth(l, d.sweeps, false);
+    }
+    // result of the top tail level to global memory
+    const TailLevel<T> &L = d.lv[0];
+    for (int p = tid; p < L.lg.Nx * L.lg.Ny; p += nth) {
+        const int i = p % L.lg.Nx, j = p / L.lg.Nx;
+        d.gxu[(long)j * L.gpitch + i] = sm[L.o_xu + j * L.lg.pitch + i];
+        d.gxv[(long)j * L.gpitch + i] = sm[L.o_xv + j * L.lg.pitch + i];
+    }
 }
 
 }  // namespace nk

@@ -75,6 +75,8 @@ struct nk_solver {
     std::string err;
     bool flags_valid = false;
     bool verbose = false;  // NK_VERBOSE=1: one progress line per Picard iteration on stderr
+    int tail_start[2] = {-1, -1};           // first multigrid level run by the LDS tail kernel (FP64, FP32); −1: none
+    size_t tail_bytes[2] = {0, 0};
     hipGraphExec_t chunk_graph = nullptr;  // captured PCG chunk (check_interval iterations)
     int chunk_graph_key = -1;
     // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
@@ -112,6 +114,19 @@ long plane_of(const KG &g) { return (long)g.pitch * (long)(g.Ny + 1); }
 
 dim3 blocks_for(int ex, int ey) { return dim3((ex + BX - 1) / BX, (ey + BY - 1) / BY); }
 const dim3 THREADS(BX, BY);
+
+// Launch a stencil kernel instantiated for the grid's topology (Bounded = 1 / Periodic = 0 along x and y), so the
+// index maps compile to straight-line code.
+#define NK_LAUNCH_TOPO(G, KERNEL, T, GRID, BLOCK, SHM, STREAM, ...)                                  \
+    do {                                                                                             \
+        if ((G).tx) {                                                                                \
+            if ((G).ty) KERNEL<T, 1, 1><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                   \
+            else KERNEL<T, 1, 0><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                          \
+        } else {                                                                                     \
+            if ((G).ty) KERNEL<T, 0, 1><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                   \
+            else KERNEL<T, 0, 0><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                          \
+        }                                                                                            \
+    } while (0)
 
 nk_status alloc_bytes(nk_solver *s, void **p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);
@@ -222,6 +237,12 @@ nk_status launch_check(nk_solver *s) {
     return NK_OK;
 }
 
+// Finish a reduction of nv values over the blocks of grid B (after the kernel that wrote the partials).
+nk_status finish_red(nk_solver *s, dim3 B, int nv, int fin, const Scal *gate) {
+    k_finish<<<1, THREADS, 0, s->stream>>>(s->partials, (int)(B.x * B.y), nv, s->S, fin, gate);
+    return launch_check(s);
+}
+
 // ---- coefficient bookkeeping ------------------------------------------------------------------------------------
 
 // η̄Hᶠᶠ (optional) and activity flags of level l from its η̄H
@@ -271,14 +292,15 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
         T *qu = (l == 0) ? work<T>(s).qu : V.qu, *qv = (l == 0) ? work<T>(s).qv : V.qv;
         T *t = V.t;
         for (int k = 0; k < sweeps; k++) {
-            k_apply<T><<<B, THREADS, 0, s->stream>>>(L.g, x0.u, x0.v, c, wf, qu, qv, nullptr, nullptr, red(s, FIN_NONE), 0, gate);
+            NK_LAUNCH_TOPO(L.g, k_apply, T, B, THREADS, 0, s->stream, L.g, x0.u, x0.v, c, wf, qu, qv, nullptr, nullptr, red(s, FIN_NONE), 0, gate);
             CHK(launch_check(s));
-            k_line_relax<T><<<dim3((L.g.Nx + NT - 1) / NT), THREADS, 0, s->stream>>>(L.g, x0.u, x0.v, bu, bv, qu, qv, t, c, wf, gate);
+            NK_LAUNCH_TOPO(L.g, k_line_relax, T, dim3((L.g.Nx + NT - 1) / NT), THREADS, 0, s->stream, L.g, x0.u, x0.v, bu, bv, qu, qv, t, c, wf, gate);
             CHK(launch_check(s));
         }
         if (final_dot) {
             k_dot<T><<<B, THREADS, 0, s->stream>>>(L.g, bu, bv, x0.u, x0.v, nullptr, nullptr, red(s, FIN_BETA), gate);
             CHK(launch_check(s));
+            CHK(finish_red(s, B, 2, FIN_BETA, gate));
         }
         *out = x0;
         return NK_OK;
@@ -286,12 +308,16 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     Pair<T> cur = x0, oth = x1;
     for (int k = 0; k < sweeps; k++) {
         const bool last = (k == sweeps - 1) && final_dot;
-        {
-            ProfScope ps(s, k > 0 ? NK_PROFILE_SWEEP : NK_PROFILE_NONE, l, (int)sizeof(T));
-            k_sweep<T><<<B, THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, bu, bv, (k == 0 ? cur : oth).u, (k == 0 ? cur : oth).v,
-                                                     k == 0 ? 1 : 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+        if (k == 0) {
+            NK_LAUNCH_TOPO(L.g, k_sweep_zero, T, B, THREADS, 0, s->stream, L.g, c, wf, bu, bv, cur.u, cur.v, red(s, FIN_BETA),
+                           last ? 1 : 0, gate);
+        } else {
+            ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
+            NK_LAUNCH_TOPO(L.g, k_sweep, T, B, THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, bu, bv, oth.u, oth.v, 0,
+                           red(s, FIN_BETA), last ? 1 : 0, gate);
         }
         CHK(launch_check(s));
+        if (last) CHK(finish_red(s, B, 1, FIN_BETA, gate));
         if (k > 0) std::swap(cur, oth);
     }
     *out = cur;
@@ -310,9 +336,9 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
     if (s->op.smoother == NK_LINE_JACOBI) {
         T *qu = (l == 0) ? work<T>(s).qu : V.qu, *qv = (l == 0) ? work<T>(s).qv : V.qv;
         for (int k = 0; k < sweeps; k++) {
-            k_apply<T><<<B, THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, qu, qv, nullptr, nullptr, red(s, FIN_NONE), 0, gate);
+            NK_LAUNCH_TOPO(L.g, k_apply, T, B, THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, qu, qv, nullptr, nullptr, red(s, FIN_NONE), 0, gate);
             CHK(launch_check(s));
-            k_line_relax<T><<<dim3((L.g.Nx + NT - 1) / NT), THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, bu, bv, qu, qv, V.t, c, wf, gate);
+            NK_LAUNCH_TOPO(L.g, k_line_relax, T, dim3((L.g.Nx + NT - 1) / NT), THREADS, 0, s->stream, L.g, cur.u, cur.v, bu, bv, qu, qv, V.t, c, wf, gate);
             CHK(launch_check(s));
         }
     } else {
@@ -320,9 +346,10 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
             const bool last = (k == sweeps - 1) && final_dot;
             {
                 ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-                k_sweep<T><<<B, THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+                NK_LAUNCH_TOPO(L.g, k_sweep, T, B, THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
             }
             CHK(launch_check(s));
+            if (last) CHK(finish_red(s, B, 1, FIN_BETA, gate));
             std::swap(cur, oth);
         }
         if (final_dot && sweeps > 0) { *out = cur; return NK_OK; }
@@ -330,6 +357,7 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
     if (final_dot) {
         k_dot<T><<<B, THREADS, 0, s->stream>>>(L.g, bu, bv, cur.u, cur.v, nullptr, nullptr, red(s, FIN_BETA), gate);
         CHK(launch_check(s));
+        CHK(finish_red(s, B, 2, FIN_BETA, gate));
     }
     *out = cur;
     return NK_OK;
@@ -337,12 +365,61 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
 
 // v_cycle! (src/StressBalances/multigrid_preconditioner.jl:144-163). x ≈ A⁻¹ b from zero at level l; the result
 // lands in x0 or x1 (returned in *out). final_dot: Σ b·x at the end → FIN_BETA (finest level of the PCG).
+// The coarse tail (levels tail_start..coarsest) in one LDS-resident workgroup: x at level l from b at level l.
+template <typename T>
+nk_status coarse_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv, const Scal *gate) {
+    const int pi = sizeof(T) == 8 ? 0 : 1;
+    TailDesc<T> d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = (int)s->lev.size() - l;
+    d.sweeps = s->op.smoothing_sweeps;
+    d.coarsest = s->op.coarsest_sweeps;
+    d.wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
+    d.gbu = bu; d.gbv = bv; d.gxu = x.u; d.gxv = x.v;
+    const KG L0g = s->lev[l].g;
+    int off_t = 0;
+    for (int k = 0; k < d.n; k++) {
+        Level &L = s->lev[l + k];
+        TailLevel<T> &t = d.lv[k];
+        t.lg = L.g;
+        t.lg.pitch = L.g.Nx + 1;
+        t.gpitch = L.g.pitch;
+        t.hx = L.cx; t.hy = L.cy;
+        Coef<T> c = coef<T>(L);
+        t.eta = c.eta; t.etaff = c.etaff; t.bu = c.bu; t.bv = c.bv; t.act = L.act;
+        const int plane = (L.g.Nx + 1) * (L.g.Ny + 1);
+        int *offs[] = {&t.o_eta, &t.o_etaff, &t.o_bu, &t.o_bv, &t.o_xu, &t.o_xv, &t.o_bu_, &t.o_bv_, &t.o_qu, &t.o_qv};
+        for (int *o : offs) { *o = off_t; off_t += plane; }
+    }
+    int off_b = off_t * (int)sizeof(T);
+    for (int k = 0; k < d.n; k++) {
+        Level &L = s->lev[l + k];
+        d.lv[k].o_act = off_b;
+        off_b += (L.g.Nx + 1) * (L.g.Ny + 1);
+    }
+    NK_LAUNCH_TOPO(L0g, k_coarse_cycle, T, 1, TAIL_THREADS, (size_t)off_b, s->stream, d, gate);
+    (void)pi;
+    return launch_check(s);
+}
+
 template <typename T>
 nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, const T *bv, bool final_dot, const Scal *gate,
                   Pair<T> *out) {
     const int nlev = (int)s->lev.size();
     Level &L = s->lev[l];
     Vecs<T> &V = vecs<T>(L);
+    const int ts = s->tail_start[sizeof(T) == 8 ? 0 : 1];
+    if (ts >= 0 && l >= ts) {
+        CHK(coarse_tail<T>(s, l, x0, bu, bv, gate));
+        if (final_dot) {
+            const dim3 B = blocks_for(L.g.Nx, L.g.Ny);
+            k_dot<T><<<B, THREADS, 0, s->stream>>>(L.g, bu, bv, x0.u, x0.v, nullptr, nullptr, red(s, FIN_BETA), gate);
+            CHK(launch_check(s));
+            CHK(finish_red(s, B, 2, FIN_BETA, gate));
+        }
+        *out = x0;
+        return NK_OK;
+    }
     if (l == nlev - 1) return smooth_from_zero<T>(s, l, x0, x1, bu, bv, s->op.coarsest_sweeps, final_dot, gate, out);
     Pair<T> cur;
     CHK(smooth_from_zero<T>(s, l, x0, x1, bu, bv, s->op.smoothing_sweeps, false, gate, &cur));
@@ -350,15 +427,15 @@ nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, cons
     Coef<T> c = coef<T>(L);
     const T wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
     T *qu = (l == 0) ? work<T>(s).qu : V.qu, *qv = (l == 0) ? work<T>(s).qv : V.qv;
-    k_apply<T><<<blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, c, wf, qu, qv, bu, bv, red(s, FIN_NONE), 0, gate);
+    NK_LAUNCH_TOPO(L.g, k_apply, T, blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, qu, qv, bu, bv, red(s, FIN_NONE), 0, gate);
     CHK(launch_check(s));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
-    k_restrict<T><<<blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream>>>(C.g, L.g, C.cx, C.cy, C.act, L.act, qu, qv, CV.bu, CV.bv, gate);
+    NK_LAUNCH_TOPO(C.g, k_restrict, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, L.g, C.cx, C.cy, C.act, L.act, qu, qv, CV.bu, CV.bv, gate);
     CHK(launch_check(s));
     Pair<T> cres;
     CHK(v_cycle<T>(s, l + 1, Pair<T>{CV.x0u, CV.x0v}, Pair<T>{CV.x1u, CV.x1v}, CV.bu, CV.bv, false, gate, &cres));
-    k_prolongate<T><<<blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream>>>(L.g, C.g, C.cx, C.cy, L.act, C.act, cres.u, cres.v, cur.u,
+    NK_LAUNCH_TOPO(L.g, k_prolongate, T, blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream, L.g, C.g, C.cx, C.cy, L.act, C.act, cres.u, cres.v, cur.u,
                                                                           cur.v, gate);
     CHK(launch_check(s));
     return smooth_from<T>(s, l, cur, oth, bu, bv, s->op.smoothing_sweeps, final_dot, gate, out);
@@ -397,7 +474,8 @@ nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate) {
     Pair<float> z32;
     CHK(v_cycle<float>(s, 0, Pair<float>{V.zu, V.zv}, Pair<float>{V.z2u, V.z2v}, V.ru, V.rv, false, gate, &z32));
     k_promote_dot<<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.zu, W.zv, W.ru, W.rv, red(s, fin), gate);
-    return launch_check(s);
+    CHK(launch_check(s));
+    return finish_red(s, B, 2, fin, gate);
 }
 
 // conjugate_gradient! (src/Solvers/conjugate_gradient.jl:299-364) at precision T on the finest grid. When
@@ -413,7 +491,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
     const int pre = s->op.preconditioner;
     CHK(write_scalars(s, ctl));
     if (!r_ready) {
-        k_apply<T><<<B, THREADS, 0, s->stream>>>(g, xu, xv, c, wf, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
+        NK_LAUNCH_TOPO(g, k_apply, T, B, THREADS, 0, s->stream, g, xu, xv, c, wf, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
         CHK(launch_check(s));
     }
     Pair<T> z{W.zu, W.zv};
@@ -426,27 +504,31 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, false, nullptr, &z));
         k_dot<T><<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, z.u, z.v, W.ru, W.rv, red(s, FIN_INIT), nullptr);
         CHK(launch_check(s));
+        CHK(finish_red(s, B, 2, FIN_INIT, nullptr));
         k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, z.u, z.v, W.pu, W.pv, 1, nullptr);
         CHK(launch_check(s));
     } else {
-        k_update_diag<T><<<B, THREADS, 0, s->stream>>>(g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv, W.pu,
+        NK_LAUNCH_TOPO(g, k_update_diag, T, B, THREADS, 0, s->stream, g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv, W.pu,
                                                         W.pv, 1, red(s, FIN_INIT), nullptr);
         CHK(launch_check(s));
+        CHK(finish_red(s, B, 2, FIN_INIT, nullptr));
     }
     const int chunk = s->op.check_interval > 0 ? s->op.check_interval : 8;
     auto launch_chunk = [&]() -> nk_status {
         for (int k = 0; k < chunk; k++) {
             {
                 ProfScope ps(s, NK_PROFILE_OPERATOR, 0, (int)sizeof(T));
-                k_apply<T><<<B, THREADS, 0, s->stream>>>(g, W.pu, W.pv, c, wf, W.qu, W.qv, nullptr, nullptr, red(s, FIN_ALPHA), 1, s->S);
+                NK_LAUNCH_TOPO(g, k_apply, T, B, THREADS, 0, s->stream, g, W.pu, W.pv, c, wf, W.qu, W.qv, nullptr, nullptr, red(s, FIN_ALPHA), 1, s->S);
             }
             CHK(launch_check(s));
+            CHK(finish_red(s, B, 1, FIN_ALPHA, s->S));
             if (pre == NK_PRECONDITIONER_MULTIGRID) {
                 {
                     ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
                     k_update_xr<T><<<B, THREADS, 0, s->stream>>>(g, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, red(s, FIN_RES), s->S);
                 }
                 CHK(launch_check(s));
+                CHK(finish_red(s, B, 1, FIN_RES, s->S));
                 Pair<T> zz{W.zu, W.zv};
                 if (mixed) CHK(precondition_fp32(s, FIN_BETA, s->S));
                 else CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, true, s->S, &zz));
@@ -455,10 +537,11 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
             } else {
                 {
                     ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
-                    k_update_diag<T><<<B, THREADS, 0, s->stream>>>(g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv,
+                    NK_LAUNCH_TOPO(g, k_update_diag, T, B, THREADS, 0, s->stream, g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv,
                                                                     nullptr, nullptr, 0, red(s, FIN_BETA_RES), s->S);
                 }
                 CHK(launch_check(s));
+                CHK(finish_red(s, B, 2, FIN_BETA_RES, s->S));
                 k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, W.zu, W.zv, W.pu, W.pv, 0, s->S);
                 CHK(launch_check(s));
             }
@@ -597,6 +680,7 @@ nk_status picard(nk_solver *s, nk_info *info) {
         if (!fixed) {
             k_picard_change<<<B, THREADS, 0, s->stream>>>(g, s->xmu, s->xmv, s->f[NK_FIELD_U], s->f[NK_FIELD_V], red(s, FIN_PICARD));
             CHK(launch_check(s));
+            CHK(finish_red(s, B, 2, FIN_PICARD, nullptr));
             CHK(read_scalars(s));
             D = s->S_host->picard_D;
             X = s->S_host->picard_X;
@@ -812,6 +896,32 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             for (auto p : w32) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
         }
     }
+    // coarse tail: the longest run of coarsest levels whose LDS image fits the budget (point Jacobi only)
+    if (options->preconditioner == NK_PRECONDITIONER_MULTIGRID && options->smoother == NK_POINT_JACOBI && s->lev.size() > 1) {
+        const size_t budget = 150 * 1024;
+        for (int pi = 0; pi < 2; pi++) {
+            const size_t es = pi == 0 ? 8 : 4;
+            size_t bytes = 0;
+            int start = -1;
+            for (int l = (int)s->lev.size() - 1; l >= 1; l--) {
+                const Level &L = s->lev[l];
+                const size_t plane = (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1);
+                const size_t b = plane * (10 * es + 1);
+                if (bytes + b > budget || (int)s->lev.size() - l > TAIL_MAX) break;
+                bytes += b;
+                start = l;
+            }
+            if (start >= 0 && (int)s->lev.size() - start >= 2) { s->tail_start[pi] = start; s->tail_bytes[pi] = bytes + 64; }
+        }
+        const void *tails[] = {(const void *)k_coarse_cycle<double, 0, 0>, (const void *)k_coarse_cycle<double, 0, 1>,
+                               (const void *)k_coarse_cycle<double, 1, 0>, (const void *)k_coarse_cycle<double, 1, 1>,
+                               (const void *)k_coarse_cycle<float, 0, 0>, (const void *)k_coarse_cycle<float, 0, 1>,
+                               (const void *)k_coarse_cycle<float, 1, 0>, (const void *)k_coarse_cycle<float, 1, 1>};
+        for (const void *f : tails)
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) s->tail_start[0] = s->tail_start[1] = -1;
+        const char *nt = std::getenv("NK_NO_TAIL");
+        if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
+    }
     {
         long nb = (long)((s->g.Nx + BX - 1) / BX) * (long)((s->g.Ny + BY - 1) / BY);
         if ((st = alloc_bytes(s, (void **)&s->partials, sizeof(double) * MAXRED * (size_t)nb)) != NK_OK) return fail(st);
@@ -971,7 +1081,7 @@ nk_status nk_apply_operator(nk_solver *s) {
     if (!s) return NK_INVALID_ARGUMENT;
     CHK(corners_flags(s, 0, false));
     Level &L = s->lev[0];
-    k_apply<double><<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(
+    NK_LAUNCH_TOPO(s->g, k_apply, double, blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream,
         s->g, s->f[NK_FIELD_PU], s->f[NK_FIELD_PV], coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, s->f[NK_FIELD_QU],
         s->f[NK_FIELD_QV], nullptr, nullptr, red(s, FIN_NONE), 0, nullptr);
     CHK(launch_check(s));
@@ -997,7 +1107,7 @@ nk_status nk_precondition(nk_solver *s) {
         k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, z.u, z.v, zu, zv);
         CHK(launch_check(s));
     } else {
-        k_update_diag<double><<<B, THREADS, 0, s->stream>>>(g, coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, pre, nullptr, nullptr,
+        NK_LAUNCH_TOPO(g, k_update_diag, double, B, THREADS, 0, s->stream, g, coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, pre, nullptr, nullptr,
                                                              nullptr, nullptr, nullptr, nullptr, ru, rv, zu, zv, s->w64.pu, s->w64.pv, 1,
                                                              red(s, FIN_NONE), nullptr);
         CHK(launch_check(s));
