@@ -163,6 +163,12 @@ typedef enum nk_field_id {
     NK_FIELD_PV,           /*                             (Center, Face)      */
     NK_FIELD_QU,           /* result vector q / z         (Face, Center)      */
     NK_FIELD_QV,           /*                             (Center, Face)      */
+    NK_FIELD_UBAR,         /* adjoint input x̄ = ∂J/∂u     (Face, Center)      */
+    NK_FIELD_VBAR,         /* ∂J/∂v                       (Center, Face)      */
+    NK_FIELD_LAMBDAU,      /* adjoint state λ = ∂J/∂τᵘ    (Face, Center)      */
+    NK_FIELD_LAMBDAV,      /* ∂J/∂τᵛ                      (Center, Face)      */
+    NK_FIELD_CBAR,         /* ∂J/∂C                       (Center, Center)    */
+    NK_FIELD_ABAR,         /* ∂J/∂A                       (Center, Center[, σ]) like NK_FIELD_A */
     NK_FIELD_COUNT
 } nk_field_id;
 
@@ -251,6 +257,14 @@ nk_status nk_update_velocities(nk_solver *solver, int32_t geometry, nk_info *inf
  * (src/Models/ice_sheet_model_methods.jl:47-57, src/Thermodynamics/column_energy_solve.jl:105-117,
  * src/Rheologies/arrhenius_rate_factor.jl:102-110) with the reference's default parameters. */
 nk_status nk_rate_factor_from_temperature(nk_solver *solver);
+
+/* Fixed-point adjoint of the converged SSA Picard solve (the EnzymeRules reverse rule of picard_solve!,
+ * ext/NunatakEnzymeExt.jl:175-240, Christianson 1994), hand-written. Inputs: the converged solution in (U, V) and the
+ * cost gradient x̄ in (UBAR, VBAR). Iterates λ = A⁻¹(x̄ + (∂q/∂x)ᵀ(−λ)) with q(x) = A(c(x)) x* until
+ * ‖Δλ‖ ≤ max(reltol‖λ‖, abstol) under `control` (NULL: the Picard control, as the reference defaults), each linear
+ * solve being the handle's PCG. Outputs: λ in (LAMBDAU, LAMBDAV) — the gradient with respect to the driving stress —
+ * and the parameter gradients ∂J/∂C (CBAR) and ∂J/∂A (ABAR). SSA only (NK_UNSUPPORTED for DIVA). */
+nk_status nk_adjoint(nk_solver *solver, const nk_control *control, nk_info *info);
 
 /* Multigrid level shapes of the handle (StressBalanceMultigridPreconditioner(grid).levels). Writes up to
  * max_levels (Nx, Ny) pairs; returns the level count in *levels. */

@@ -13,6 +13,7 @@
 #include "../../include/nunatak_hip.h"
 #include "nk_device.hpp"
 #include "nk_physics.hpp"
+#include "nk_adjoint.hpp"
 
 using namespace nk;
 
@@ -25,8 +26,9 @@ const LocInfo LOC[NK_FIELD_COUNT] = {
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0},
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0},
     {1, 0, 0}, {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0},
-    {0, 1, 0},
+    {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2},
 };
+static_assert(sizeof(LOC) / sizeof(LOC[0]) == NK_FIELD_COUNT, "one location per field id");
 
 template <typename T> struct Vecs {
     T *x0u = nullptr, *x0v = nullptr, *x1u = nullptr, *x1v = nullptr, *bu = nullptr, *bv = nullptr, *qu = nullptr,
@@ -66,6 +68,9 @@ struct nk_solver {
     Work<double> w64{};
     Work<float> w32{};
     double *xmu = nullptr, *xmv = nullptr;
+    // adjoint work planes: q̄, ȳ, λ⁺, r (u and v), η̄H̄ (centres), ε̇xy adjoint (corners)
+    double *ad_qu = nullptr, *ad_qv = nullptr, *ad_yu = nullptr, *ad_yv = nullptr, *ad_lu = nullptr, *ad_lv = nullptr,
+           *ad_ru = nullptr, *ad_rv = nullptr, *ad_eta = nullptr, *ad_exy = nullptr;
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
     unsigned int *counter = nullptr;
@@ -79,6 +84,7 @@ struct nk_solver {
     size_t tail_bytes[2] = {0, 0};
     hipGraphExec_t chunk_graph = nullptr;  // captured PCG chunk (check_interval iterations)
     int chunk_graph_key = -1;
+    const void *chunk_graph_x = nullptr;  // solution buffer the captured chunk updates
     // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
     std::vector<hipEvent_t> prof_events;
     size_t prof_used = 0;
@@ -552,7 +558,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
     // and replayed; the kernels read the device stop flag, so a replay after convergence does no work.
     const bool use_graph = s->op.graphs && s->op.profile_kernel == NK_PROFILE_NONE;
     const int key = (int)sizeof(T) * 1000 + (mixed ? 500 : 0) + pre * 100 + (xu == s->f[NK_FIELD_U] ? 1 : 0) + chunk * 10000;
-    if (use_graph && s->chunk_graph_key != key) {
+    if (use_graph && (s->chunk_graph_key != key || s->chunk_graph_x != (const void *)xu)) {
         if (s->chunk_graph) { (void)hipGraphExecDestroy(s->chunk_graph); s->chunk_graph = nullptr; }
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -564,6 +570,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) { s->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e); return NK_DEVICE_ERROR; }
         s->chunk_graph_key = key;
+        s->chunk_graph_x = xu;
     }
     for (;;) {
         CHK(read_scalars(s));
@@ -888,7 +895,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
     }
     {
         double **w[] = {&s->w64.ru, &s->w64.rv, &s->w64.zu, &s->w64.zv, &s->w64.z2u, &s->w64.z2v, &s->w64.pu, &s->w64.pv,
-                        &s->w64.qu, &s->w64.qv, &s->w64.du, &s->w64.dv, &s->xmu, &s->xmv};
+                        &s->w64.qu, &s->w64.qv, &s->w64.du, &s->w64.dv, &s->xmu, &s->xmv, &s->ad_qu, &s->ad_qv,
+                        &s->ad_yu, &s->ad_yv, &s->ad_lu, &s->ad_lv, &s->ad_ru, &s->ad_rv, &s->ad_eta, &s->ad_exy};
         for (auto p : w) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
         if (options->precision == NK_PRECISION_MIXED) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
@@ -1157,6 +1165,117 @@ nk_status nk_rate_factor_from_temperature(nk_solver *s) {
     CHK(launch_check(s));
     HIPCHK(hipStreamSynchronize(s->stream));
     return NK_OK;
+}
+
+namespace {
+
+// One transposed coefficient step: ȳ = (∂q/∂x)ᵀ q̄ (velocity) and/or C̄, Ā (parameters), with q̄ in ad_q.
+nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters) {
+    const KG &g = s->g;
+    const dim3 Bc = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    double **f = s->f;
+    const PhysK p = physk(s);
+    const uint8_t *act = s->lev[0].act;
+    k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->ad_eta, s->ad_exy, 0.0, nullptr);
+    CHK(launch_check(s));
+    if (velocity) {
+        k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->ad_yu, s->ad_yv, 0.0, nullptr);
+        CHK(launch_check(s));
+    }
+    k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    CHK(launch_check(s));
+    k_adj_corners<<<Be, THREADS, 0, s->stream>>>(g, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    CHK(launch_check(s));
+    k_adj_viscosity<<<Bc, THREADS, 0, s->stream>>>(g, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETAH],
+                                                    f[NK_FIELD_ETASIGMA], s->plane, f[NK_FIELD_A], f[NK_FIELD_H],
+                                                    s->ph.masking ? f[NK_FIELD_MU] : nullptr, s->ad_eta, s->ad_yu, s->ad_yv,
+                                                    s->ad_exy, f[NK_FIELD_ABAR], velocity ? 1 : 0, parameters ? 1 : 0);
+    CHK(launch_check(s));
+    if (velocity) {
+        k_adj_shear<<<Be, THREADS, 0, s->stream>>>(g, p.wf, s->ad_exy, s->ad_yu, s->ad_yv);
+        CHK(launch_check(s));
+    }
+    k_adj_drag<<<Be, THREADS, 0, s->stream>>>(g, p, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, f[NK_FIELD_C], f[NK_FIELD_N],
+                                               f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_yu, s->ad_yv, f[NK_FIELD_CBAR],
+                                               velocity ? 1 : 0, parameters ? 1 : 0);
+    return launch_check(s);
+}
+
+// EnzymeRules.reverse for converged picard_solve! (ext/NunatakEnzymeExt.jl:175-240), SSA coefficients.
+nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    double **f = s->f;
+    double *lu = f[NK_FIELD_LAMBDAU], *lv = f[NK_FIELD_LAMBDAV];
+    CHK(update_coefficients(s));   // coefficients at the fixed point x*
+    CHK(update_preconditioner(s));
+    k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, lu, lv, 0.0, nullptr);
+    CHK(launch_check(s));
+    k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->ad_lu, s->ad_lv, 0.0, nullptr);
+    CHK(launch_check(s));
+    k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_CBAR], nullptr, 0.0, nullptr);
+    CHK(launch_check(s));
+    k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->ad_qu, s->ad_qv, 0.0, nullptr);
+    CHK(launch_check(s));
+    for (int k = 0; k < (s->ph.rate_factor_depth == NK_RATE_FACTOR_3D ? s->Ns : 1); k++) {
+        k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_ABAR] + (long)k * s->plane, nullptr, 0.0, nullptr);
+        CHK(launch_check(s));
+    }
+    const bool fixed = ctl.kind == NK_FIXED_ITERATIONS;
+    double D = INFINITY, M = 0.0;
+    int it = 0;
+    long total = 0;
+    for (;;) {
+        if (fixed) { if (it >= ctl.iterations) break; }
+        else if (!(D > jmax_h(ctl.reltol * M, ctl.abstol) && it < ctl.maxiter)) break;
+        k_negate<<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_qu, s->ad_qv);   // q̄ = −λ
+        CHK(launch_check(s));
+        CHK(adjoint_step(s, true, false));                                      // ȳ
+        k_add<<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_yu, s->ad_yv, s->ad_ru, s->ad_rv);
+        CHK(launch_check(s));
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_lu, s->ad_lv);     // λ⁺ = λ (warm start)
+        CHK(launch_check(s));
+        int cg = 0;
+        double res = 0.0;
+        CHK(pcg<double>(s, s->ad_lu, s->ad_lv, s->ad_ru, s->ad_rv, s->op.linear, false, &cg, &res));   // λ⁺ = A⁻ᵀ r
+        total += cg;
+        k_picard_change<<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_lu, s->ad_lv, red(s, FIN_PICARD));   // Δ, M
+        CHK(launch_check(s));
+        CHK(finish_red(s, B, 2, FIN_PICARD, nullptr));
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->ad_lu, s->ad_lv, lu, lv);     // λ = λ⁺
+        CHK(launch_check(s));
+        CHK(read_scalars(s));
+        D = s->S_host->picard_D;
+        M = s->S_host->picard_X;
+        it++;
+        if (s->verbose) {
+            std::fprintf(stderr, "[nunatak_hip] adjoint %d: linear %d, change %.3e, norm %.3e\n", it, cg, D, M);
+            std::fflush(stderr);
+        }
+    }
+    if (!fixed && D > jmax_h(ctl.reltol * M, ctl.abstol))
+        std::fprintf(stderr, "nunatak_hip: Picard adjoint did not converge in %d iterations: |dλ| = %g, |λ| = %g\n", ctl.maxiter, D, M);
+    // parameter pullback with the converged λ: p̄ = (∂q/∂p)ᵀ(−λ)
+    k_negate<<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_qu, s->ad_qv);
+    CHK(launch_check(s));
+    CHK(adjoint_step(s, false, true));
+    if (info) {
+        info->picard_iterations = it;
+        info->total_linear_iterations = total;
+        info->last_picard_change = D;
+        info->last_picard_norm = M;
+    }
+    return NK_OK;
+}
+
+}  // namespace
+
+nk_status nk_adjoint(nk_solver *s, const nk_control *control, nk_info *info) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    if (s->ph.stress_balance != NK_SSA) { s->err = "the hand-written adjoint covers the SSA coefficient update"; return NK_UNSUPPORTED; }
+    Timer t(s, info);
+    const nk_control ctl = control ? *control : s->op.picard;
+    return t.done(adjoint(s, ctl, info));
 }
 
 nk_status nk_multigrid_levels(const nk_solver *s, int32_t *levels, int32_t *shapes, int32_t max_levels) {

@@ -62,7 +62,8 @@ end
 
 # field ids (nk_field_id)
 const U, V, H, BED, S, PHI, PHIU, PHIV, N, MU, TAUU, TAUV, ETAH, ETAHFF, BETAU, BETAV, F2, ETASIGMA,
-      BETABU, BETABV, UB, VB, USIGMA, VSIGMA, A, C = 0:25
+      BETABU, BETABV, UB, VB, USIGMA, VSIGMA, A, C, TEMPERATURE, PU, PV, QU, QV,
+      UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR = 0:36
 
 control(c::Convergence) = NKControl(0, c.maxiter, 0, 0, c.reltol, c.abstol)
 control(c::FixedIterations) = NKControl(1, c.iterations, c.iterations, 0, 0.0, 0.0)
@@ -182,8 +183,25 @@ end
 
 update_velocities!(model, solver::HIPStressBalanceSolver) = update_velocities!(model, solver, model.stress_balance)
 
+# Reverse pass of a converged SSA velocity solve (the fixed-point rule of ext/NunatakEnzymeExt.jl:175-240): given the
+# cost gradient with respect to the velocities (ū, v̄), accumulate ∂J/∂τ into (τ̄ᵘ, τ̄ᵛ), ∂J/∂C into C̄ and ∂J/∂A into
+# Ā. The handle must still hold the converged solution of the forward call.
+function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalanceSolver, ū, v̄; control = nothing)
+    set_field!(solver, UBAR, ū); set_field!(solver, VBAR, v̄)
+    info = solver.info
+    c = control === nothing ? C_NULL : Ref(NunatakHIP.control(control))
+    check(ccall((:nk_adjoint, libnunatak_hip), Cint, (Ptr{Cvoid}, Ptr{NKControl}, Ref{NKInfo}), solver.handle, c, info),
+          solver.handle, "nk_adjoint")
+    for (f, id) in ((τ̄ᵘ, LAMBDAU), (τ̄ᵛ, LAMBDAV), (C̄, CBAR), (Ā, ABAR))
+        f === nothing && continue
+        g = similar(f); get_field!(g, solver, id)
+        interior(f) .+= interior(g)
+    end
+    return info.picard_iterations
+end
+
 Base.summary(s::HIPStressBalanceSolver) = "HIPStressBalanceSolver(libnunatak_hip, iteration = $(s.iteration))"
 
-export HIPStressBalanceSolver
+export HIPStressBalanceSolver, hip_velocity_adjoint!
 
 end # module

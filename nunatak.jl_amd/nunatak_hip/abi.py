@@ -25,7 +25,7 @@ PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE = 0, 1, 2, 3
 
 FIELDS = ["U", "V", "H", "BED", "S", "PHI", "PHIU", "PHIV", "N", "MU", "TAUU", "TAUV", "ETAH", "ETAHFF",
           "BETAU", "BETAV", "F2", "ETASIGMA", "BETABU", "BETABV", "UB", "VB", "USIGMA", "VSIGMA", "A", "C",
-          "TEMPERATURE", "PU", "PV", "QU", "QV"]
+          "TEMPERATURE", "PU", "PV", "QU", "QV", "UBAR", "VBAR", "LAMBDAU", "LAMBDAV", "CBAR", "ABAR"]
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 
 # (x location, y location, on the σ grid); "f" = Face, "c" = Center.
@@ -38,7 +38,8 @@ LOCATIONS = {
     "BETABU": ("f", "c", False), "BETABV": ("c", "f", False), "UB": ("f", "c", False), "VB": ("c", "f", False),
     "USIGMA": ("f", "c", True), "VSIGMA": ("c", "f", True), "A": ("c", "c", None), "C": ("c", "c", False),
     "TEMPERATURE": ("c", "c", True), "PU": ("f", "c", False), "PV": ("c", "f", False),
-    "QU": ("f", "c", False), "QV": ("c", "f", False),
+    "QU": ("f", "c", False), "QV": ("c", "f", False), "UBAR": ("f", "c", False), "VBAR": ("c", "f", False),
+    "LAMBDAU": ("f", "c", False), "LAMBDAV": ("c", "f", False), "CBAR": ("c", "c", False), "ABAR": ("c", "c", None),
 }
 
 

@@ -21,7 +21,7 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_fill_field", "nk_get_classes", "nk_field_device_pointer", "nk_stream", "nk_synchronize",
            "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients", "nk_apply_operator",
            "nk_precondition", "nk_solve", "nk_picard_solve", "nk_update_velocities",
-           "nk_rate_factor_from_temperature", "nk_multigrid_levels"]
+           "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint"]
 
 _lib = None
 
@@ -65,6 +65,7 @@ def load(path=LIB_PATH):
     L.nk_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
     L.nk_update_velocities.argtypes = [P, I32, ct.POINTER(abi.Info)]
     L.nk_multigrid_levels.argtypes = [P, ct.POINTER(I32), ct.POINTER(I32), I32]
+    L.nk_adjoint.argtypes = [P, ct.POINTER(abi.Control), ct.POINTER(abi.Info)]
     for name in EXPORTS:
         if name not in ("nk_last_error", "nk_stream", "nk_abi_version", "nk_default_grid", "nk_default_physics",
                         "nk_default_options"):
@@ -186,6 +187,13 @@ class HIPModel:
     def update_velocities(self, geometry=True):
         info = abi.Info()
         self._check(self._L.nk_update_velocities(self._h, 1 if geometry else 0, ct.byref(info)), "nk_update_velocities")
+        return info.as_dict()
+
+    def adjoint(self, control=None):
+        """nk_adjoint: λ, ∂J/∂C, ∂J/∂A from the cost gradient in UBAR, VBAR at the converged (U, V)."""
+        info = abi.Info()
+        c = ct.byref(control) if control is not None else None
+        self._check(self._L.nk_adjoint(self._h, c, ct.byref(info)), "nk_adjoint")
         return info.as_dict()
 
     def multigrid_levels(self):

@@ -111,6 +111,54 @@ def diva_shelf(N=1024, dx=500.0, seed=3, Nsigma=10, precision=abi.PRECISION_FP64
     return g, phys, opts, fields
 
 
+def friction_inversion(N=512, dx=1e3, seed=4, A=1e-25, preconditioner=abi.PRECONDITIONER_MULTIGRID,
+                       linear_reltol=1e-12, picard_reltol=1e-10):
+    """Config 4: gradient of J = ½Σ(|u| − u_obs)² with respect to log C on config 1's geometry at N × N, dx (512 km at
+    the defaults): N − 1 ice columns and one open-ocean column, H = 1000 − 8·(x/km)·(64 km / L) + box3(N(0, 10 m)),
+    no-slip walls, ρᵢ = 917, ρw = 1028, Glen n = 3 with constant A, SSA, Weertman m = 1/3.
+
+    Config 1's bed (−2000 m) floats the whole shelf, where friction has no effect and ∂J/∂C ≡ 0, so the bed here is
+    B = −300 − 1400·x/L: grounded upstream, afloat downstream, with a subgrid grounding line between.
+    The prior is C₀ = 3.16e6 everywhere; the truth is log C = log C₀ + GRF(σ = 0.2, ℓ = 20 km). Returns
+    (grid, physics, options, fields with C = C₀, log C_true)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = N * dx
+    g = abi.default_grid(N, N, dx, dx, topology=(abi.BOUNDED, abi.BOUNDED), halo=4)
+    X, _ = _centers(N, N, dx, dx)
+    H = 1000.0 - 8.0 * (X / 1e3) * (64e3 / L) + box3(rng.normal(0.0, 10.0, size=(N, N)))
+    H[:, N - 1:] = 0.0
+    B = -300.0 - 1400.0 * X / L
+    logC0 = np.full((N, N), np.log(3.16e6))
+    logC_true = logC0 + gaussian_random_field(rng, (N, N), (dx, dx), 20e3, 0.2)
+    phys = abi.default_physics(ice_density=917.0, ocean_density=1028.0, grounding=abi.GROUNDING_SUBGRID,
+                               walls=abi.NO_SLIP, friction=abi.FRICTION_WEERTMAN, friction_exponent=1.0 / 3.0)
+    opts = abi.default_options(linear=abi.convergence(linear_reltol, 0.0, 10_000),
+                               picard=abi.convergence(picard_reltol, 0.0, 500), preconditioner=preconditioner)
+    fields = {"H": H, "BED": B, "A": np.full((N, N), A), "C": np.exp(logC0)}
+    return g, phys, opts, fields, logC_true
+
+
+def centre_speed(u, v):
+    """|u| at cell centres of a Bounded × Bounded grid: √(ℑx(u)² + ℑy(v)²), u shaped (Ny, Nx + 1), v (Ny + 1, Nx)."""
+    uc = (u[:, :-1] + u[:, 1:]) / 2
+    vc = (v[:-1, :] + v[1:, :]) / 2
+    return np.sqrt(uc * uc + vc * vc), uc, vc
+
+
+def speed_misfit(u, v, s_obs, mask):
+    """J = ½ Σ_mask (|u| − s_obs)² and its gradient (ū, v̄) with respect to the face velocities."""
+    s, uc, vc = centre_speed(u, v)
+    d = np.where(mask, s - s_obs, 0.0)
+    J = 0.5 * float(np.sum(d * d))
+    w = np.where(s > 0, d / np.where(s > 0, s, 1.0), 0.0)
+    gu, gv = np.zeros_like(u), np.zeros_like(v)
+    gu[:, :-1] += w * uc / 2
+    gu[:, 1:] += w * uc / 2
+    gv[:-1, :] += w * vc / 2
+    gv[1:, :] += w * vc / 2
+    return J, gu, gv
+
+
 def load_into(model, fields, zero_velocity=True):
     """Copy a workload's inputs into a model (HIPModel or any object with the same protocol)."""
     for name, a in fields.items():

@@ -42,7 +42,9 @@ static const nko_loc LOCS[NK_FIELD_COUNT] = {
     [NK_FIELD_VB] = {C, F, 0},     [NK_FIELD_USIGMA] = {F, C, 1},  [NK_FIELD_VSIGMA] = {C, F, 1},
     [NK_FIELD_A] = {C, C, 0},      [NK_FIELD_C] = {C, C, 0},       [NK_FIELD_TEMPERATURE] = {C, C, 1},
     [NK_FIELD_PU] = {F, C, 0},     [NK_FIELD_PV] = {C, F, 0},      [NK_FIELD_QU] = {F, C, 0},
-    [NK_FIELD_QV] = {C, F, 0},
+    [NK_FIELD_QV] = {C, F, 0},     [NK_FIELD_UBAR] = {F, C, 0},    [NK_FIELD_VBAR] = {C, F, 0},
+    [NK_FIELD_LAMBDAU] = {F, C, 0},[NK_FIELD_LAMBDAV] = {C, F, 0}, [NK_FIELD_CBAR] = {C, C, 0},
+    [NK_FIELD_ABAR] = {C, C, 0},
 };
 
 /* ------------------------------------------------------------------------------------------------------------ */
@@ -569,7 +571,7 @@ typedef struct nko_model {
 
 static int nz_of(const nko_model *m, int id) {
     if (LOCS[id].sigma) return m->Ns;
-    if (id == NK_FIELD_A && m->ph.rate_factor_depth == NK_RATE_FACTOR_3D) return m->Ns;
+    if ((id == NK_FIELD_A || id == NK_FIELD_ABAR) && m->ph.rate_factor_depth == NK_RATE_FACTOR_3D) return m->Ns;
     return 1;
 }
 
@@ -594,7 +596,8 @@ EXPORT nko_model *nko_create(const nk_grid *grid, const nk_physics *ph, const nk
     m->grid = *grid; m->ph = *ph; m->op = *op;
     og_init(&m->g, grid->Nx, grid->Ny, grid->topology_x, grid->topology_y, grid->dx, grid->dy);
     m->Ns = grid->Nsigma;
-    for (int id = 0; id < NK_FIELD_COUNT; id++) m->f[id] = og_alloc(&m->g, (LOCS[id].sigma || id == NK_FIELD_A) ? m->Ns : 1);
+    for (int id = 0; id < NK_FIELD_COUNT; id++)
+        m->f[id] = og_alloc(&m->g, (LOCS[id].sigma || id == NK_FIELD_A || id == NK_FIELD_ABAR) ? m->Ns : 1);
     m->class_u = calloc(m->g.plane, 1);
     m->class_v = calloc(m->g.plane, 1);
     int Ns = m->Ns;

@@ -1,0 +1,246 @@
+"""Fixed-point adjoint of the converged SSA Picard solve (nk_adjoint) against central finite differences.
+
+Mirrors test/test_enzyme.jl:352-410 of the reference: the ridge model (16 × 4, Bounded × Periodic, Glen A = 1e-24,
+Weertman C = 3e6(1 + x/L), J = Σu² + Σv² over the solved cells), ∂J/∂C against a centred difference with
+h = 1e-4·C (bar 1e-5 relative), the multigrid and diagonal preconditioners giving the same gradient (bar 1e-8
+relative), and the warning of an unconverged adjoint. Added: ∂J/∂A (2D and per σ layer), ∂J/∂τ (= λ), the regularized
+Coulomb drag, and the friction-inversion workload's ∂J/∂log C along seeded random directions.
+
+No oracle adjoint exists: the gradient is pinned by finite differences of the forward solve, which is itself pinned
+to the oracle by the parity tests.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import B, P, abi, field_values, grid
+from nunatak_hip import workloads as W
+from nunatak_hip.model import HIPModel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+L, LY = 64e3, 16e3
+
+
+def ridge(preconditioner=abi.PRECONDITIONER_DIAGONAL, friction=abi.FRICTION_WEERTMAN, Nsigma=1, A3d=False,
+          adjoint_maxiter=None):
+    """ridge_model of test/test_enzyme.jl:92-101 (bed 0: grounded everywhere)."""
+    g = grid(16, 4, L, LY, topology=(B, P), halo=4, Nsigma=Nsigma)
+    kw = {}
+    if friction == abi.FRICTION_REGULARIZED_COULOMB:
+        kw = dict(coulomb_coefficient=0.5, velocity_regularization=300.0 / W.YEAR)
+    phys = abi.default_physics(friction=friction, friction_exponent=1.0 / 3.0,
+                               rate_factor_depth=abi.RATE_FACTOR_3D if A3d else abi.RATE_FACTOR_2D, **kw)
+    opts = abi.default_options(linear=abi.convergence(1e-14, 0.0, 10_000), picard=abi.convergence(1e-12, 0.0, 300),
+                               preconditioner=preconditioner)
+    m = HIPModel(g, phys, opts)
+    H = field_values(g, "H", lambda x, y: 500 + 300 * np.exp(-((x - L / 2) / 12e3) ** 2) * (1 + np.sin(2 * np.pi * y / LY) / 5))
+    m.set("H", H)
+    m.set("BED", 0.0)
+    if A3d:
+        A = 1e-24 * (1 + 0.3 * np.arange(Nsigma)[:, None, None] / Nsigma) * np.ones((Nsigma, 4, 16))
+        m.set("A", A)
+    else:
+        m.set("A", 1e-24)
+    m.set("C", field_values(g, "C", lambda x, y: 3e6 * (1 + x / L)))
+    return m, g
+
+
+def cost(m, g):
+    """sum_of_squares(u) + sum_of_squares(v) over the solved cells, after a solve from u = v = 0."""
+    m.set("U", 0.0)
+    m.set("V", 0.0)
+    m.update_velocities(geometry=True)
+    u, v = m.get("U"), m.get("V")
+    return float(np.sum(u[: g.Ny, : g.Nx] ** 2) + np.sum(v[: g.Ny, : g.Nx] ** 2))
+
+
+def adjoint_of_cost(m, g, control=None):
+    J = cost(m, g)
+    u, v = m.get("U"), m.get("V")
+    ub, vb = np.zeros_like(u), np.zeros_like(v)
+    ub[: g.Ny, : g.Nx] = 2 * u[: g.Ny, : g.Nx]
+    vb[: g.Ny, : g.Nx] = 2 * v[: g.Ny, : g.Nx]
+    m.set("UBAR", ub)
+    m.set("VBAR", vb)
+    info = m.adjoint(control)
+    return J, info
+
+
+def centred_difference(m, g, name, index, h):
+    a = m.get(name)
+    a0 = a.copy()
+    a[index] += h
+    m.set(name, a)
+    Jp = cost(m, g)
+    a[index] -= 2 * h
+    m.set(name, a)
+    Jm = cost(m, g)
+    m.set(name, a0)
+    return (Jp - Jm) / (2 * h)
+
+
+@pytest.mark.parametrize("preconditioner", [abi.PRECONDITIONER_DIAGONAL, abi.PRECONDITIONER_MULTIGRID])
+def test_friction_and_rate_factor_gradients_match_finite_differences(preconditioner):
+    say(f"[adjoint] ridge 16x4, preconditioner {preconditioner}")
+    m, g = ridge(preconditioner)
+    J, info = adjoint_of_cost(m, g)
+    say(f"[adjoint] J = {J:.6e}, adjoint iterations {info['picard_iterations']}, linear {info['total_linear_iterations']}")
+    assert info["picard_iterations"] >= 1
+    Cb, Ab = m.get("CBAR").copy(), m.get("ABAR").copy()
+    C, A = m.get("C"), m.get("A")
+    for name, grad, field, index in (("C", Cb, C, (1, 6)), ("C", Cb, C, (3, 11)), ("C", Cb, C, (0, 2)),
+                                     ("A", Ab, A, (1, 2)), ("A", Ab, A, (2, 9))):
+        fd = centred_difference(m, g, name, index, 1e-4 * field[index])
+        ad = grad[index]
+        say(f"[adjoint] dJ/d{name}{index}: adjoint {ad:.10e}, fd {fd:.10e}")
+        assert abs(ad - fd) < 1e-5 * abs(fd)
+
+
+def test_multigrid_and_diagonal_adjoints_agree():
+    grads = []
+    for pre in (abi.PRECONDITIONER_DIAGONAL, abi.PRECONDITIONER_MULTIGRID):
+        m, g = ridge(pre)
+        adjoint_of_cost(m, g)
+        grads.append((m.get("CBAR").copy(), m.get("ABAR").copy(), m.get("LAMBDAU").copy()))
+    for d, mg in zip(*grads):
+        err = np.max(np.abs(mg - d)) / np.max(np.abs(d))
+        say(f"[adjoint] multigrid vs diagonal: {err:.3e}")
+        assert err < 1e-8
+
+
+def test_driving_stress_gradient_is_lambda():
+    """b̄ = λ: ∂J/∂τ through picard_solve! with τ held fixed."""
+    m, g = ridge()
+    m.update_geometry()
+    m.compute_driving_stress()
+    tu = m.get("TAUU").copy()
+
+    def J():
+        m.set("U", 0.0)
+        m.set("V", 0.0)
+        m.picard_solve()
+        u, v = m.get("U"), m.get("V")
+        return float(np.sum(u[: g.Ny, : g.Nx] ** 2) + np.sum(v[: g.Ny, : g.Nx] ** 2))
+
+    J()
+    u, v = m.get("U"), m.get("V")
+    ub, vb = np.zeros_like(u), np.zeros_like(v)
+    ub[: g.Ny, : g.Nx] = 2 * u[: g.Ny, : g.Nx]
+    vb[: g.Ny, : g.Nx] = 2 * v[: g.Ny, : g.Nx]
+    m.set("UBAR", ub)
+    m.set("VBAR", vb)
+    m.adjoint()
+    lam = m.get("LAMBDAU").copy()
+    for index in ((1, 5), (2, 10)):
+        h = 1e-4 * abs(tu[index])
+        t = tu.copy()
+        t[index] += h
+        m.set("TAUU", t)
+        Jp = J()
+        t[index] -= 2 * h
+        m.set("TAUU", t)
+        Jm = J()
+        m.set("TAUU", tu)
+        fd = (Jp - Jm) / (2 * h)
+        say(f"[adjoint] dJ/dtau_u{index}: lambda {lam[index]:.10e}, fd {fd:.10e}")
+        assert abs(lam[index] - fd) < 1e-5 * abs(fd)
+
+
+def test_regularized_coulomb_gradient_matches_finite_differences():
+    m, g = ridge(friction=abi.FRICTION_REGULARIZED_COULOMB)
+    adjoint_of_cost(m, g)
+    Cb = m.get("CBAR").copy()
+    C = m.get("C")
+    for index in ((1, 6), (2, 12)):
+        fd = centred_difference(m, g, "C", index, 1e-4 * C[index])
+        say(f"[adjoint] regularized Coulomb dJ/dC{index}: adjoint {Cb[index]:.10e}, fd {fd:.10e}")
+        assert abs(Cb[index] - fd) < 1e-5 * abs(fd)
+
+
+def test_layered_rate_factor_gradient_matches_finite_differences():
+    m, g = ridge(Nsigma=4, A3d=True)
+    adjoint_of_cost(m, g)
+    Ab = m.get("ABAR").copy()
+    A = m.get("A")
+    assert Ab.shape == A.shape == (4, 4, 16)
+    for index in ((0, 1, 6), (3, 2, 9)):
+        fd = centred_difference(m, g, "A", index, 1e-4 * A[index])
+        say(f"[adjoint] dJ/dA{index}: adjoint {Ab[index]:.10e}, fd {fd:.10e}")
+        assert abs(Ab[index] - fd) < 1e-5 * abs(fd)
+
+
+def test_unconverged_adjoint_warns(capfd):
+    m, g = ridge()
+    info = adjoint_of_cost(m, g, control=abi.convergence(1e-12, 0.0, 1))[1]
+    assert info["picard_iterations"] == 1
+    err = capfd.readouterr().err
+    assert "Picard adjoint did not converge" in err
+
+
+def test_diva_adjoint_is_rejected():
+    g = grid(16, 4, L, LY, topology=(B, P))
+    m = HIPModel(g, abi.default_physics(stress_balance=abi.DIVA), abi.default_options())
+    with pytest.raises(Exception, match="UNSUPPORTED"):
+        m.adjoint()
+
+
+FULL = os.environ.get("NK_FULL_SIZE") == "1"
+
+
+@pytest.mark.parametrize("N,directions", [(128, 16), pytest.param(512, 16, marks=pytest.mark.skipif(
+    not FULL, reason="512² check takes minutes; run with NK_FULL_SIZE=1"))])
+def test_friction_inversion_gradient_along_random_directions(N, directions):
+    """Friction-inversion workload: ∂J/∂log C · d against centred differences along 16 seeded smooth directions d
+    (GRF, ℓ = 20 km, unit maximum), Richardson-extrapolated from steps h = 1e-3 and h/2 so the O(h²) truncation term
+    cancels: fd = (4 D(h/2) − D(h)) / 3, D(h) = (J(C e^{hd}) − J(C e^{−hd})) / 2h. Bar: 1e-8 relative."""
+    g, phys, opts, fields, logC_true = W.friction_inversion(N=N, dx=512e3 / N)
+    say(f"[adjoint] friction inversion {N}x{N}")
+    m = HIPModel(g, phys, opts)
+    W.load_into(m, fields)
+    m.set("C", np.exp(logC_true))
+    m.update_velocities(geometry=True)
+    rng = np.random.Generator(np.random.PCG64(44))
+    s_true, _, _ = W.centre_speed(m.get("U"), m.get("V"))
+    s_obs = s_true + rng.normal(0.0, 5.0 / W.YEAR, size=s_true.shape)
+    mask = fields["H"] > 0
+    C0 = fields["C"]
+
+    def J_of(C):
+        m.set("C", C)
+        m.set("U", 0.0)
+        m.set("V", 0.0)
+        info = m.update_velocities(geometry=True)
+        return W.speed_misfit(m.get("U"), m.get("V"), s_obs, mask)[0], info
+
+    J0, info = J_of(C0)
+    _, gu, gv = W.speed_misfit(m.get("U"), m.get("V"), s_obs, mask)
+    m.set("UBAR", gu)
+    m.set("VBAR", gv)
+    ainfo = m.adjoint()
+    say(f"[adjoint] J0 = {J0:.6e}; forward picard {info['picard_iterations']}, adjoint iterations "
+        f"{ainfo['picard_iterations']}, {ainfo['elapsed_ms']:.1f} ms")
+    glogC = m.get("CBAR") * C0
+    h = 1e-3
+
+    def D(d, h):
+        return (J_of(C0 * np.exp(h * d))[0] - J_of(C0 * np.exp(-h * d))[0]) / (2 * h)
+
+    worst = 0.0
+    for k in range(directions):
+        d = W.gaussian_random_field(rng, (N, N), (g.dx, g.dy), 20e3, 1.0)
+        d /= np.max(np.abs(d))
+        fd = (4 * D(d, h / 2) - D(d, h)) / 3
+        ad = float(np.sum(glogC * d))
+        rel = abs(ad - fd) / abs(fd)
+        worst = max(worst, rel)
+        say(f"[adjoint] direction {k}: adjoint {ad:.10e}, fd {fd:.10e}, rel {rel:.2e}")
+    assert worst < 1e-8
