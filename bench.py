@@ -14,6 +14,10 @@ activity flags: 1 byte; write x: 2 arrays).
 CPU baseline: the CPU oracle (plain C, one thread) timed on a bounded sample of the same workload — one DIVA
 coefficient update and three MG-PCG iterations at 1024² — and scaled to one solve with the FP64 GPU run's Picard
 and CG counts.
+
+Serving (config 5, bounded sample): --serve requests of the config-5 stream replayed at their Poisson arrival times
+through nk_server (--serve-slots workers); p50/p99 of finish − arrival and solves/s over the run, beside serial FP64
+GPU solves/s of the first --serve-serial requests.
 """
 import argparse
 import json
@@ -113,6 +117,35 @@ def cpu_baseline(setup, fp64_info):
                       f"{cg} CG iterations"}
 
 
+def serving_leg(n, rate_hz, slots, serial_sample):
+    """Config 5 on a bounded sample: n requests of the stated stream (Poisson arrivals at rate_hz, sides log-uniform
+    in [64, 1024] rounded to 16) through the server; the serial FP64 baseline solves the first serial_sample
+    requests one after another on fresh FP64 handles."""
+    from nunatak_hip.server import serve
+    reqs = workloads.serving_stream(n=n, rate_hz=rate_hz, seed=5)
+    log(f"[bench] serving {n} requests at {rate_hz} /s on {slots} slots")
+    out = serve(reqs, slots=slots, progress=max(1, n // 10))
+    t_serial = 0.0
+    for k, (_, g, p, o, f) in enumerate(reqs[:serial_sample]):
+        o64 = abi.default_options(linear=o.linear, picard=o.picard, preconditioner=o.preconditioner,
+                                  precision=abi.PRECISION_FP64)
+        m = HIPModel(g, p, o64)
+        workloads.load_into(m, f)
+        t0 = time.perf_counter()
+        m.update_velocities(geometry=True)
+        t_serial += time.perf_counter() - t0
+        m.close()
+    serial = serial_sample / t_serial if serial_sample else None
+    sides = [r[1].Nx for r in reqs]
+    log(f"[bench] served: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, {out['solves_per_s']:.2f} solves/s; "
+        f"serial FP64 {serial:.2f} solves/s on the first {serial_sample}")
+    return {"requests": n, "rate_hz": rate_hz, "slots": slots, "sides_min_max": [min(sides), max(sides)],
+            "p50_ms": out["p50_ms"], "p99_ms": out["p99_ms"], "solves_per_s": out["solves_per_s"],
+            "serial_fp64_solves_per_s": serial, "serial_sample": serial_sample,
+            "workload": "config5 stream (seed 5): SSA grounded/floating channel per request, mixed precision, "
+                        "MG-PCG linear 1e-8, Picard 1e-6"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +157,10 @@ def main():
     ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
     ap.add_argument("--graphs", type=int, default=1, help="0: launch kernel by kernel (for external profilers)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--serve", type=int, default=48, help="requests in the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-serial", type=int, default=12, help="requests timed serially in FP64 for the baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the velocity solve of one grid is not "
@@ -195,6 +232,12 @@ def main():
                        "avg_launch_us": dur_s * 1e6, "launches": launches, "algorithmic_bytes_per_launch": bytes_per_launch}
     if not args.no_cpu_baseline and "fp64" in results:
         out["cpu_baseline"] = cpu_baseline(models["fp64"][1], results["fp64"]["infos"][-1])
+    if args.serve > 0:
+        for m, _ in models.values():
+            m.close()
+        out["serving"] = serving_leg(args.serve, args.serve_rate, args.serve_slots, min(args.serve_serial, args.serve))
+        out["served_p50_ms"] = out["serving"]["p50_ms"]
+        out["served_p99_ms"] = out["serving"]["p99_ms"]
     print(json.dumps(out), flush=True)
 
 

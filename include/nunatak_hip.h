@@ -270,6 +270,61 @@ nk_status nk_adjoint(nk_solver *solver, const nk_control *control, nk_info *info
  * max_levels (Nx, Ny) pairs; returns the level count in *levels. */
 nk_status nk_multigrid_levels(const nk_solver *solver, int32_t *levels, int32_t *shapes, int32_t max_levels);
 
+/* Device memory a handle holds: nk_device_bytes of an existing handle, nk_estimate_device_bytes of one that
+ * nk_create would build from these arguments (the same figure, without allocating). No reference counterpart:
+ * the serving form's HBM admission uses them. */
+nk_status nk_device_bytes(const nk_solver *solver, int64_t *bytes);
+nk_status nk_estimate_device_bytes(const nk_grid *grid, const nk_physics *physics, const nk_solver_options *options,
+                                   int64_t *bytes);
+
+/* ---- Serving: continuous batching of velocity solves ---------------------------------------------------------
+ * No reference counterpart (the reference solves one model at a time through update_velocities!,
+ * src/Models/ice_sheet_model_methods.jl:119-147); each request is that call on its own inputs.
+ *
+ * A server owns `slots` worker threads, each driving one request at a time on its own HIP stream, and a pool of
+ * solver handles keyed by (grid, options): a request reuses an idle handle of its size class (device buffers and
+ * captured PCG graphs stay resident) or, when the HBM budget allows, a new one is built; idle handles of other
+ * classes are evicted (least recently used first) to make room. Requests are admitted in arrival order when a slot
+ * is free and their handle fits the budget, so requests that arrive while others run join the device work as slots
+ * and memory free up. A reused handle has every field not given by the request reset to zero, so a request's
+ * result does not depend on what ran before it. */
+typedef struct nk_server nk_server;
+
+typedef struct {
+    int32_t slots;               /* requests in flight at once (worker threads, one HIP stream each), ≥ 1 */
+    int32_t pad;
+    int64_t hbm_budget_bytes;    /* device memory the server's handles may hold; 0 = 80 % of free device memory */
+} nk_server_config;
+
+typedef struct {
+    int64_t id;
+    int32_t status;              /* nk_status of the solve */
+    int32_t reused_handle;       /* 1 when an idle handle of the size class was reused */
+    int32_t picard_iterations;
+    int32_t total_linear_iterations;
+    double submit_ms;            /* server clock (ms since nk_server_create) */
+    double admit_ms;             /* slot and memory granted */
+    double start_ms;             /* inputs on the device, solve started */
+    double finish_ms;            /* result copied back to host memory */
+    int64_t device_bytes;        /* device memory of the handle that served the request */
+} nk_request_info;
+
+nk_status nk_server_create(const nk_server_config *config, nk_server **server);
+/* Waits for every submitted request to finish, then frees the handles and joins the workers. */
+nk_status nk_server_destroy(nk_server *server);
+const char *nk_server_last_error(const nk_server *server);
+/* Queue one update_velocities! (geometry on the device). fields[k] holds field field_ids[k] packed to the extents
+ * of nk_field_shape (column-major, σ layers last); the server copies them before returning. U and V default to
+ * zero (the initial guess); TEMPERATURE, when given, sets A through nk_rate_factor_from_temperature. */
+nk_status nk_server_submit(nk_server *server, const nk_grid *grid, const nk_physics *physics,
+                           const nk_solver_options *options, int32_t nfields, const int32_t *field_ids,
+                           const double *const *fields, int64_t *request_id);
+/* Non-blocking: *done = 1 when the request has finished. */
+nk_status nk_server_poll(nk_server *server, int64_t request_id, int32_t *done);
+/* Blocks until the request finishes; writes u and v packed to their nk_field_shape extents (either may be NULL)
+ * and the request's record, then forgets the request. Returns the solve's status. */
+nk_status nk_server_wait(nk_server *server, int64_t request_id, double *u, double *v, nk_request_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,7 @@
 #include "nk_device.hpp"
 #include "nk_physics.hpp"
 #include "nk_adjoint.hpp"
+#include "nk_internal.hpp"
 
 using namespace nk;
 
@@ -77,6 +79,7 @@ struct nk_solver {
     Scal *S = nullptr;
     Scal *S_host = nullptr;
     std::vector<void *> allocs;
+    size_t device_bytes = 0;
     std::string err;
     bool flags_valid = false;
     bool verbose = false;  // NK_VERBOSE=1: one progress line per Picard iteration on stderr
@@ -110,6 +113,9 @@ struct nk_solver {
 
 namespace {
 
+// Serialises graph capture against device allocation and release across the handles of one process.
+std::mutex g_device_mu;
+
 KG make_kg(int Nx, int Ny, int tx, int ty, double dx, double dy) {
     KG g;
     g.Nx = Nx; g.Ny = Ny; g.tx = tx; g.ty = ty; g.dx = dx; g.dy = dy;
@@ -141,7 +147,8 @@ nk_status alloc_bytes(nk_solver *s, void **p, size_t bytes) {
         return NK_OUT_OF_MEMORY;
     }
     s->allocs.push_back(*p);
-    e = hipMemset(*p, 0, bytes);
+    s->device_bytes += bytes;
+    e = hipMemsetAsync(*p, 0, bytes, s->stream);
     if (e != hipSuccess) { s->err = "hipMemset failed"; return NK_DEVICE_ERROR; }
     return NK_OK;
 }
@@ -561,7 +568,10 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
     if (use_graph && (s->chunk_graph_key != key || s->chunk_graph_x != (const void *)xu)) {
         if (s->chunk_graph) { (void)hipGraphExecDestroy(s->chunk_graph); s->chunk_graph = nullptr; }
         hipGraph_t graph;
-        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        // Handles may be driven from several host threads (the server's slots): one capture at a time, never
+        // beside an allocation or a free, and relaxed mode so other threads' stream work goes on meanwhile.
+        std::lock_guard<std::mutex> lk(g_device_mu);
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
         nk_status cst = launch_chunk();
         hipError_t e = hipStreamEndCapture(s->stream, &graph);
         if (cst != NK_OK) return cst;
@@ -821,6 +831,7 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         std::fprintf(stderr, "nk_create: %s\n", err.c_str());
         return st;
     }
+    std::lock_guard<std::mutex> lk(g_device_mu);
     nk_solver *s = new nk_solver();
     s->grid = *grid; s->ph = *physics; s->op = *options;
     {
@@ -855,7 +866,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         for (int k = 0; k < Ns; k++)
             for (int kp = 0; kp <= k; kp++) pw[k * Ns + kp] = power_integral(zf[kp], kp == k ? zc[k] : zf[kp + 1], 1);
         if ((st = alloc_bytes(s, (void **)&s->sig, t.size() * sizeof(double))) != NK_OK) return fail(st);
-        if (hipMemcpy(s->sig, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { s->err = "σ tables"; return fail(NK_DEVICE_ERROR); }
+        if (hipMemcpyAsync(s->sig, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipStreamSynchronize(s->stream) != hipSuccess) { s->err = "σ tables"; return fail(NK_DEVICE_ERROR); }
     }
     // levels: the finest uses the model's fields; coarse levels own theirs
     {
@@ -937,12 +949,67 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if ((st = alloc_bytes(s, (void **)&s->S, sizeof(Scal))) != NK_OK) return fail(st);
         if (hipHostMalloc((void **)&s->S_host, sizeof(Scal), hipHostMallocDefault) != hipSuccess) { s->err = "hipHostMalloc failed"; return fail(NK_OUT_OF_MEMORY); }
     }
+    if (hipStreamSynchronize(s->stream) != hipSuccess) { s->err = "clearing device buffers failed"; return fail(NK_DEVICE_ERROR); }
     *out = s;
+    return NK_OK;
+}
+
+namespace {
+
+// Device bytes nk_create allocates for these arguments: the same planes, levels and work vectors, counted.
+size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
+    const KG g0 = make_kg(grid->Nx, grid->Ny, grid->topology_x, grid->topology_y, grid->dx, grid->dy);
+    const size_t p0 = (size_t)plane_of(g0), Ns = (size_t)grid->Nsigma;
+    const bool mixed = o->precision == NK_PRECISION_MIXED;
+    size_t b = 0;
+    for (int id = 0; id < NK_FIELD_COUNT; id++) b += sizeof(double) * p0 * (LOC[id].sigma ? Ns : 1);
+    b += 2 * p0;                                                              // face classes
+    b += sizeof(double) * ((Ns + 1) + 3 * Ns + Ns * Ns);                      // σ tables
+    std::vector<KG> lev{g0};
+    if (o->preconditioner == NK_PRECONDITIONER_MULTIGRID) {
+        KG cur = g0;
+        for (;;) {
+            int cxh = coarsening(cur.Nx, grid->halo, o->minimum_coarse_size);
+            int cyh = coarsening(cur.Ny, grid->halo, o->minimum_coarse_size);
+            if (!cxh && !cyh) break;
+            cur = make_kg(cur.Nx / (cxh ? 2 : 1), cur.Ny / (cyh ? 2 : 1), cur.tx, cur.ty, cur.dx, cur.dy);
+            lev.push_back(cur);
+        }
+    }
+    for (size_t l = 0; l < lev.size(); l++) {
+        const size_t p = (size_t)plane_of(lev[l]);
+        if (l > 0) b += 4 * sizeof(double) * p;
+        b += p + 9 * sizeof(double) * p;                                      // activity flags, FP64 vectors
+        if (mixed) b += (4 + 9) * sizeof(float) * p;
+    }
+    b += 24 * sizeof(double) * p0;
+    if (mixed) b += 12 * sizeof(float) * p0;
+    const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
+    b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
+    return b;
+}
+
+}  // namespace
+
+nk_status nk_device_bytes(const nk_solver *s, int64_t *bytes) {
+    if (!s || !bytes) return NK_INVALID_ARGUMENT;
+    *bytes = (int64_t)s->device_bytes;
+    return NK_OK;
+}
+
+nk_status nk_estimate_device_bytes(const nk_grid *grid, const nk_physics *physics, const nk_solver_options *options,
+                                   int64_t *bytes) {
+    if (!grid || !physics || !options || !bytes) return NK_INVALID_ARGUMENT;
+    std::string err;
+    nk_status st = validate(grid, physics, options, err);
+    if (st != NK_OK) return st;
+    *bytes = (int64_t)planned_bytes(grid, options);
     return NK_OK;
 }
 
 nk_status nk_destroy(nk_solver *s) {
     if (!s) return NK_OK;
+    std::lock_guard<std::mutex> lk(g_device_mu);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void *p : s->allocs) (void)hipFree(p);
     if (s->S_host) (void)hipHostFree(s->S_host);
@@ -1014,6 +1081,28 @@ nk_status nk_get_field(nk_solver *s, nk_field_id id, double *data, int64_t ld, i
         HIPCHK(hipMemcpy2DAsync(data + (long)k * lds, sizeof(double) * ld, s->f[id] + (long)k * s->plane, sizeof(double) * s->g.pitch,
                                 sizeof(double) * sh[0], sh[1], hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+void nk_internal_field_extents(const nk_grid *grid, const nk_physics *physics, int id, int32_t shape[3]) {
+    const LocInfo &L = LOC[id];
+    shape[0] = grid->Nx + ((L.lx && grid->topology_x == NK_BOUNDED) ? 1 : 0);
+    shape[1] = grid->Ny + ((L.ly && grid->topology_y == NK_BOUNDED) ? 1 : 0);
+    shape[2] = (L.sigma == 1 || (L.sigma == 2 && physics->rate_factor_depth == NK_RATE_FACTOR_3D)) ? grid->Nsigma : 1;
+}
+
+nk_status nk_internal_reset_fields(nk_solver *s, const uint8_t *keep) {
+    for (int id = 0; id < NK_FIELD_COUNT; id++) {
+        if (keep && keep[id]) continue;
+        const int nz = LOC[id].sigma ? s->Ns : 1;
+        for (int k = 0; k < nz; k++) {
+            k_fill2<double><<<blocks_for(s->g.Nx + 1, s->g.Ny + 1), THREADS, 0, s->stream>>>(s->g, s->f[id] + (long)k * s->plane, nullptr,
+                                                                                             0.0, nullptr);
+            CHK(launch_check(s));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->flags_valid = false;
     return NK_OK;
 }
 

@@ -84,6 +84,20 @@ class Info(ct.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if not name.startswith("pad")}
 
 
+class ServerConfig(ct.Structure):
+    _fields_ = [("slots", ct.c_int32), ("pad", ct.c_int32), ("hbm_budget_bytes", ct.c_int64)]
+
+
+class RequestInfo(ct.Structure):
+    _fields_ = [("id", ct.c_int64), ("status", ct.c_int32), ("reused_handle", ct.c_int32),
+                ("picard_iterations", ct.c_int32), ("total_linear_iterations", ct.c_int32),
+                ("submit_ms", ct.c_double), ("admit_ms", ct.c_double), ("start_ms", ct.c_double),
+                ("finish_ms", ct.c_double), ("device_bytes", ct.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def default_grid(Nx, Ny, dx, dy, topology=(BOUNDED, BOUNDED), halo=4, Nsigma=10, bed_refinement=2.0):
     return Grid(Nx, Ny, topology[0], topology[1], halo, Nsigma, dx, dy, bed_refinement)
 
@@ -126,6 +140,17 @@ def field_shape(grid, name):
     nx = grid.Nx + (1 if (lx == "f" and grid.topology_x == BOUNDED) else 0)
     ny = grid.Ny + (1 if (ly == "f" and grid.topology_y == BOUNDED) else 0)
     return nx, ny, sig
+
+
+def packed_shape(grid, physics, name):
+    """(nx, ny, nz) of a field as nk_field_shape reports it: σ fields have Nsigma layers, the rate factor (and its
+    gradient) Nsigma layers when physics.rate_factor_depth is 3D."""
+    nx, ny, sig = field_shape(grid, name)
+    if sig is None:
+        nz = grid.Nsigma if physics.rate_factor_depth == RATE_FACTOR_3D else 1
+    else:
+        nz = grid.Nsigma if sig else 1
+    return nx, ny, nz
 
 
 def node_coordinates(grid, name, x0=0.0, y0=0.0):

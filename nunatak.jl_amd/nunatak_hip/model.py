@@ -21,7 +21,9 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_fill_field", "nk_get_classes", "nk_field_device_pointer", "nk_stream", "nk_synchronize",
            "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients", "nk_apply_operator",
            "nk_precondition", "nk_solve", "nk_picard_solve", "nk_update_velocities",
-           "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint"]
+           "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint", "nk_device_bytes",
+           "nk_estimate_device_bytes", "nk_server_create", "nk_server_destroy", "nk_server_last_error",
+           "nk_server_submit", "nk_server_poll", "nk_server_wait"]
 
 _lib = None
 
@@ -66,9 +68,20 @@ def load(path=LIB_PATH):
     L.nk_update_velocities.argtypes = [P, I32, ct.POINTER(abi.Info)]
     L.nk_multigrid_levels.argtypes = [P, ct.POINTER(I32), ct.POINTER(I32), I32]
     L.nk_adjoint.argtypes = [P, ct.POINTER(abi.Control), ct.POINTER(abi.Info)]
+    L.nk_device_bytes.argtypes = [P, ct.POINTER(I64)]
+    L.nk_estimate_device_bytes.argtypes = [ct.POINTER(abi.Grid), ct.POINTER(abi.Physics), ct.POINTER(abi.SolverOptions),
+                                           ct.POINTER(I64)]
+    L.nk_server_create.argtypes = [ct.POINTER(abi.ServerConfig), ct.POINTER(P)]
+    L.nk_server_destroy.argtypes = [P]
+    L.nk_server_last_error.argtypes = [P]
+    L.nk_server_last_error.restype = ct.c_char_p
+    L.nk_server_submit.argtypes = [P, ct.POINTER(abi.Grid), ct.POINTER(abi.Physics), ct.POINTER(abi.SolverOptions), I32,
+                                   ct.POINTER(I32), ct.POINTER(dp), ct.POINTER(I64)]
+    L.nk_server_poll.argtypes = [P, I64, ct.POINTER(I32)]
+    L.nk_server_wait.argtypes = [P, I64, dp, dp, ct.POINTER(abi.RequestInfo)]
     for name in EXPORTS:
         if name not in ("nk_last_error", "nk_stream", "nk_abi_version", "nk_default_grid", "nk_default_physics",
-                        "nk_default_options"):
+                        "nk_default_options", "nk_server_last_error"):
             getattr(L, name).restype = ct.c_int
     _lib = L
     return L
@@ -196,8 +209,23 @@ class HIPModel:
         self._check(self._L.nk_adjoint(self._h, c, ct.byref(info)), "nk_adjoint")
         return info.as_dict()
 
+    def device_bytes(self):
+        b = ct.c_int64()
+        self._check(self._L.nk_device_bytes(self._h, ct.byref(b)), "nk_device_bytes")
+        return b.value
+
     def multigrid_levels(self):
         n = ct.c_int32()
         buf = (ct.c_int32 * 128)()
         self._check(self._L.nk_multigrid_levels(self._h, ct.byref(n), buf, 64), "nk_multigrid_levels")
         return [(buf[2 * k], buf[2 * k + 1]) for k in range(n.value)]
+
+
+def estimate_device_bytes(grid, physics, options):
+    """nk_estimate_device_bytes: device memory nk_create would allocate for these arguments."""
+    L = load()
+    b = ct.c_int64()
+    st = L.nk_estimate_device_bytes(ct.byref(grid), ct.byref(physics), ct.byref(options), ct.byref(b))
+    if st != abi.NK_OK:
+        raise NunatakError(f"nk_estimate_device_bytes: {abi.STATUS_NAMES.get(st, st)}")
+    return b.value

@@ -138,6 +138,42 @@ def friction_inversion(N=512, dx=1e3, seed=4, A=1e-25, preconditioner=abi.PRECON
     return g, phys, opts, fields, logC_true
 
 
+def serving_request(N, seed, dx=1e3, precision=abi.PRECISION_MIXED, linear_reltol=1e-8, picard_reltol=1e-6):
+    """One request of the serving workload: config 1's geometry at N × N, dx (N − 1 ice columns and one open-ocean
+    column, H = 1000 − 8·(x/km)·(64 km / L) + box3(N(0, 10 m)), no-slip walls) on the friction-inversion bed
+    B = −300 − 1400·x/L (grounded upstream, afloat downstream) with its own seeded friction
+    C = 3.16e6·exp(GRF(σ = 0.2, ℓ = 20 km)); SSA, Glen A = 1e-25, Weertman m = 1/3, MG-PCG at the given precision,
+    linear reltol 1e-8, Picard 1e-6."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = N * dx
+    g = abi.default_grid(N, N, dx, dx, topology=(abi.BOUNDED, abi.BOUNDED), halo=4)
+    X, _ = _centers(N, N, dx, dx)
+    H = 1000.0 - 8.0 * (X / 1e3) * (64e3 / L) + box3(rng.normal(0.0, 10.0, size=(N, N)))
+    H[:, N - 1:] = 0.0
+    B = -300.0 - 1400.0 * X / L
+    C = 3.16e6 * np.exp(gaussian_random_field(rng, (N, N), (dx, dx), 20e3, 0.2))
+    phys = abi.default_physics(ice_density=917.0, ocean_density=1028.0, grounding=abi.GROUNDING_SUBGRID,
+                               walls=abi.NO_SLIP, friction=abi.FRICTION_WEERTMAN, friction_exponent=1.0 / 3.0)
+    opts = abi.default_options(linear=abi.convergence(linear_reltol, 0.0, 10_000),
+                               picard=abi.convergence(picard_reltol, 0.0, 100),
+                               preconditioner=abi.PRECONDITIONER_MULTIGRID, precision=precision)
+    fields = {"H": H, "BED": B, "A": np.full((N, N), 1e-25), "C": C}
+    return g, phys, opts, fields
+
+
+def serving_stream(n=2000, rate_hz=4.0, seed=5, smallest=64, largest=1024, precision=abi.PRECISION_MIXED):
+    """Config 5: n requests with Poisson arrivals at rate_hz; side = 16·round(exp(U[ln 64, ln 1024]) / 16); request
+    k is serving_request(side, seed·100003 + k). Returns [(arrival_s, grid, physics, options, fields), ...]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sides = 16 * np.round(np.exp(rng.uniform(np.log(smallest), np.log(largest), size=n)) / 16).astype(int)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, size=n))
+    out = []
+    for k in range(n):
+        g, p, o, f = serving_request(int(sides[k]), seed * 100003 + k, precision=precision)
+        out.append((float(arrivals[k]), g, p, o, f))
+    return out
+
+
 def centre_speed(u, v):
     """|u| at cell centres of a Bounded × Bounded grid: √(ℑx(u)² + ℑy(v)²), u shaped (Ny, Nx + 1), v (Ny + 1, Nx)."""
     uc = (u[:, :-1] + u[:, 1:]) / 2

@@ -63,6 +63,8 @@ def test_struct_sizes_match_header_layout():
     assert ct.sizeof(abi.Grid) == 6 * 4 + 3 * 8
     assert ct.sizeof(abi.Control) == 4 * 4 + 2 * 8
     assert ct.sizeof(abi.Info) == 4 + 4 + 8 + 4 + 4 + 4 * 8 + 8 + 8 + 8 + 4 + 4
+    assert ct.sizeof(abi.ServerConfig) == 4 + 4 + 8
+    assert ct.sizeof(abi.RequestInfo) == 8 + 4 * 4 + 4 * 8 + 8
 
 
 def test_create_rejects_bad_arguments_without_touching_a_device():
