@@ -42,8 +42,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make(N, precision, profile=0, check_interval=8, picard=None, graphs=1):
+def make(N, precision, profile=0, check_interval=8, picard=None, graphs=1, coarse=abi.COARSE_REDISCRETIZED):
     g, phys, opts, fields = workloads.diva_shelf(N=N, dx=512e3 / N, precision=precision)
+    opts.coarse_operator = coarse
     opts.profile_kernel = profile
     opts.graphs = graphs
     opts.check_interval = check_interval
@@ -157,6 +158,8 @@ def main():
     ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
     ap.add_argument("--graphs", type=int, default=1, help="0: launch kernel by kernel (for external profilers)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--coarse", choices=["galerkin", "rediscretized"], default="galerkin",
+                    help="coarse operators of the mixed form's FP32 V-cycle (the FP64 form keeps the reference's)")
     ap.add_argument("--serve", type=int, default=48, help="requests in the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
@@ -175,7 +178,8 @@ def main():
     for label, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
         if args.only and args.only != label:
             continue
-        m, setup = make(N, prec, picard=args.picard, graphs=args.graphs)
+        coarse = abi.COARSE_GALERKIN if (label == "mixed" and args.coarse == "galerkin") else abi.COARSE_REDISCRETIZED
+        m, setup = make(N, prec, picard=args.picard, graphs=args.graphs, coarse=coarse)
         models[label] = (m, setup)
         wall, infos = timed_solves(m, Wm, K, label)
         results[label] = {"wall": wall, "infos": infos, "u": m.get("U"), "v": m.get("V")}
@@ -195,7 +199,8 @@ def main():
     out["dtype"] = "f32+f64" if main_label == "mixed" else "f64"
     info_last = r["infos"][-1]
     out["config"] = {"workload": f"config3 DIVA floating channel {N}x{N} dx={512 / N:g} km, Nsigma=10, Arrhenius A, "
-                                 f"MG-PCG (PointJacobi V(2,2)), Picard 1e-6, linear 1e-8",
+                                 f"MG-PCG (PointJacobi V(2,2)), Picard 1e-6, linear 1e-8; mixed: FP32 V-cycle with "
+                                 f"{args.coarse} coarse operators",
                      "grid": [N, N], "precision": main_label,
                      "picard_iterations": info_last["picard_iterations"],
                      "cg_iterations": info_last["total_linear_iterations"]}

@@ -100,6 +100,11 @@ typedef struct nk_control {
 enum { NK_PRECONDITIONER_NONE = 0, NK_PRECONDITIONER_DIAGONAL = 1, NK_PRECONDITIONER_MULTIGRID = 2 };
 enum { NK_POINT_JACOBI = 0, NK_LINE_JACOBI = 1 };
 enum { NK_PRECISION_FP64 = 0, NK_PRECISION_MIXED = 1 };
+/* Coarse-level operators of the multigrid preconditioner: rediscretized from coarsened coefficients, as the reference
+ * builds them (src/StressBalances/multigrid_preconditioner.jl:128-141), or Galerkin products Pᵀ A P of the level
+ * above with the same transfers (no reference counterpart; keeps a calving front free on coarse levels where a
+ * coarse cell straddles the front and an ice-free strip narrower than the cell). */
+enum { NK_COARSE_REDISCRETIZED = 0, NK_COARSE_GALERKIN = 1 };
 
 /* StressBalanceSolver options (src/StressBalances/stress_balance_solver.jl:126-135) and the multigrid defaults
  * (src/StressBalances/multigrid_preconditioner.jl:57). */
@@ -111,15 +116,15 @@ typedef struct nk_solver_options {
     int32_t smoothing_sweeps;         /* 2 */
     int32_t coarsest_sweeps;          /* 20 */
     int32_t minimum_coarse_size;      /* 2 */
-    int32_t precision;                /* NK_PRECISION_FP64; NK_PRECISION_MIXED = FP32 inner PCG + V-cycle
-                                         inside FP64 iterative refinement */
-    double inner_reltol;              /* mixed: relative reduction asked of each FP32 inner solve */
-    int32_t inner_maxiter;            /* mixed: FP32 PCG iterations per refinement step */
+    int32_t precision;                /* NK_PRECISION_FP64; NK_PRECISION_MIXED = FP64 PCG (operator, residual, α/β,
+                                         stopping rule) preconditioned by an FP32 multigrid V-cycle (multigrid only) */
+    int32_t coarse_operator;          /* NK_COARSE_REDISCRETIZED; NK_COARSE_GALERKIN (multigrid, point Jacobi) */
     int32_t check_interval;           /* PCG iterations launched between host checks of the on-device stop flag */
     int32_t profile_kernel;           /* NK_PROFILE_*: bracket every finest-level launch of that kernel with HIP events
                                          on the handle's stream and report the summed device time in nk_info */
     int32_t graphs;                   /* 1: capture the launches of check_interval PCG iterations in a hipGraph once
                                          and replay it (not while profiling); 0: launch kernel by kernel */
+    int32_t pad;
 } nk_solver_options;
 
 /* Kernels that nk_solver_options.profile_kernel can time (finest grid only). */

@@ -1,0 +1,229 @@
+// nk_galerkin.hpp — Galerkin coarse operators A_{l+1} = Pᵀ A_l P for the multigrid preconditioner.
+//
+// The reference rediscretizes each coarse level from coarsened coefficients (src/StressBalances/
+// multigrid_preconditioner.jl:128-141). Where a coarse cell straddles a calving front and an ice-free strip narrower
+// than the cell (the one-column ocean of the channel workloads), the coarsened viscosity reaches the domain wall and
+// the coarse problem pins the front: the coarse correction then misses the outflow mode and PCG needs hundreds of
+// iterations at 1024². The Galerkin product with the V-cycle's own transfers (restriction R = Pᵀ of
+// k_restrict / k_prolongate) keeps what the fine operator says about the front.
+//
+// A coarse operator is an explicit stencil: with the C-grid transfers the product closes on 31 couplings per row.
+// Layout: 62 planes per level, plane-major (coefficient k of point o at st[k·plane + o]):
+//   u row: [ 0,15) u columns at (dx, dy) ∈ [−1, 1] × [−2, 2], [15,31) v columns at [−2, 1] × [−1, 2]
+//   v row: [31,47) u columns at [−1, 2] × [−2, 1], [47,62) v columns at [−2, 2] × [−1, 1]
+// (dx, dy) are index offsets of the column from the row point; an inactive row is the identity row. Two more planes
+// hold 1/ℓ1 (the inverse absolute row sums) for the ℓ1-Jacobi smoother.
+#pragma once
+#include "nk_device.hpp"
+
+namespace nk {
+
+constexpr int GS_PLANES = 62;   // stencil planes per level
+constexpr int GS_ALLOC = 64;    // + 1/ℓ1 of u rows and of v rows
+
+struct GBlk {
+    int base, x0, x1, y0, y1;
+};
+__host__ __device__ constexpr GBlk gblk(int rc, int cc) {
+    return rc == 0 ? (cc == 0 ? GBlk{0, -1, 1, -2, 2} : GBlk{15, -2, 1, -1, 2})
+                   : (cc == 0 ? GBlk{31, -1, 2, -2, 1} : GBlk{47, -2, 2, -1, 1});
+}
+__host__ __device__ constexpr int gidx(const GBlk &b, int dx, int dy) { return b.base + (dx - b.x0) * (b.y1 - b.y0 + 1) + (dy - b.y0); }
+
+__device__ __forceinline__ long gplane(const KG &g) { return (long)g.pitch * (long)(g.Ny + 1); }
+
+// Row RC (0 = u, 1 = v) of A x at (i, j); x read through the location's index maps (wrap on Periodic, clamp on
+// Bounded, where every out-of-range column has a zero coefficient).
+template <typename T, int RC>
+__device__ __forceinline__ T g_row(const KG &g, const T *__restrict__ st, long plane, long o, int i, int j,
+                                   const T *__restrict__ xu, const T *__restrict__ xv) {
+    T q = T(0);
+    constexpr GBlk bu = gblk(RC, 0), bv = gblk(RC, 1);
+#pragma unroll
+    for (int dx = bu.x0; dx <= bu.x1; dx++)
+#pragma unroll
+        for (int dy = bu.y0; dy <= bu.y1; dy++)
+            q += st[gidx(bu, dx, dy) * plane + o] * xu[off(g, fx(g, i + dx), cy(g, j + dy))];
+#pragma unroll
+    for (int dx = bv.x0; dx <= bv.x1; dx++)
+#pragma unroll
+        for (int dy = bv.y0; dy <= bv.y1; dy++)
+            q += st[gidx(bv, dx, dy) * plane + o] * xv[off(g, cx(g, i + dx), fy(g, j + dy))];
+    return q;
+}
+
+// The finest level's operator as a stencil (the ssa_operator! couplings expanded, same masks as ssa_apply).
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_g_assemble(KG g, Coef<T> c, T wf, T *__restrict__ st) {
+    g.tx = TX; g.ty = TY;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j), P = gplane(g);
+    const T dx = (T)g.dx, dy = (T)g.dy, Az = (T)(g.dx * g.dy);
+    for (int k = 0; k < GS_PLANES; k++) st[k * P + o] = T(0);
+    const T ec = ldc(g, c.eta, i, j), ew = ldc(g, c.eta, i - 1, j), es = ldc(g, c.eta, i, j - 1);
+    const T f00 = ldff(g, c.etaff, i, j) * (periph_ff(g, i, j) ? wf : T(1));
+    const T f01 = ldff(g, c.etaff, i, j + 1) * (periph_ff(g, i, j + 1) ? wf : T(1));
+    const T f10 = ldff(g, c.etaff, i + 1, j) * (periph_ff(g, i + 1, j) ? wf : T(1));
+    const uint8_t *a = c.act;
+    constexpr GBlk UU = gblk(0, 0), UV = gblk(0, 1), VU = gblk(1, 0), VV = gblk(1, 1);
+    if (uact(g, a, i, j)) {
+        st[gidx(UU, 0, 0) * P + o] = Az * c.bu[o] + T(4) * dy * ec / dx + T(4) * dy * ew / dx + dx * f01 / dy + dx * f00 / dy;
+        if (uact(g, a, i + 1, j)) st[gidx(UU, 1, 0) * P + o] = -T(4) * dy * ec / dx;
+        if (uact(g, a, i - 1, j)) st[gidx(UU, -1, 0) * P + o] = -T(4) * dy * ew / dx;
+        if (uact(g, a, i, j + 1)) st[gidx(UU, 0, 1) * P + o] = -dx * f01 / dy;
+        if (uact(g, a, i, j - 1)) st[gidx(UU, 0, -1) * P + o] = -dx * f00 / dy;
+        if (vact(g, a, i, j + 1)) st[gidx(UV, 0, 1) * P + o] = -T(2) * ec - f01;
+        if (vact(g, a, i, j)) st[gidx(UV, 0, 0) * P + o] = T(2) * ec + f00;
+        if (vact(g, a, i - 1, j + 1)) st[gidx(UV, -1, 1) * P + o] = T(2) * ew + f01;
+        if (vact(g, a, i - 1, j)) st[gidx(UV, -1, 0) * P + o] = -T(2) * ew - f00;
+    } else {
+        st[gidx(UU, 0, 0) * P + o] = T(1);
+    }
+    if (vact(g, a, i, j)) {
+        st[gidx(VV, 0, 0) * P + o] = Az * c.bv[o] + T(4) * dx * ec / dy + T(4) * dx * es / dy + dy * f10 / dx + dy * f00 / dx;
+        if (vact(g, a, i, j + 1)) st[gidx(VV, 0, 1) * P + o] = -T(4) * dx * ec / dy;
+        if (vact(g, a, i, j - 1)) st[gidx(VV, 0, -1) * P + o] = -T(4) * dx * es / dy;
+        if (vact(g, a, i + 1, j)) st[gidx(VV, 1, 0) * P + o] = -dy * f10 / dx;
+        if (vact(g, a, i - 1, j)) st[gidx(VV, -1, 0) * P + o] = -dy * f00 / dx;
+        if (uact(g, a, i + 1, j)) st[gidx(VU, 1, 0) * P + o] = -T(2) * ec - f10;
+        if (uact(g, a, i, j)) st[gidx(VU, 0, 0) * P + o] = T(2) * ec + f00;
+        if (uact(g, a, i + 1, j - 1)) st[gidx(VU, 1, -1) * P + o] = T(2) * es + f10;
+        if (uact(g, a, i, j - 1)) st[gidx(VU, 0, -1) * P + o] = -T(2) * es - f00;
+    } else {
+        st[gidx(VV, 0, 0) * P + o] = T(1);
+    }
+}
+
+// Offset d of a coarse column from the row along one direction, as the pattern range [lo, hi] stores it (periodic
+// directions fold modulo N). Returns false when no representative lies in the range.
+__device__ __forceinline__ bool g_fold(int &d, int lo, int hi, int N, int periodic) {
+    if (periodic) {
+        while (d < lo) d += N;
+        while (d > hi) d -= N;
+    }
+    return d >= lo && d <= hi;
+}
+
+// One coarse row RC at (I, J): Σ over the fine points f of the restriction (weight w_f, fine-active) of
+// w_f · A_l(f, f′) · P(f′, C) over the fine couplings f′ of f and the coarse parents C of f′ (coarse-active).
+// acc: this thread's 31 accumulators in LDS (stride NT).
+template <typename T, int RC>
+__device__ __forceinline__ void g_rap_row(const KG &cg, const KG &fg, int hx, int hy, const uint8_t *cact, const uint8_t *fact,
+                                          const T *__restrict__ fst, long fplane, int I, int J, T *acc, int *err) {
+    constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
+    const int fr = RC == 0 ? 1 : 0;   // restriction: u is Face along x, v is Face along y
+    for (int a = (hx ? -1 : 0); a <= (hx ? (RC == 0 ? 1 : 2) : 0); a++)
+        for (int b = (hy ? -1 : 0); b <= (hy ? (RC == 0 ? 2 : 1) : 0); b++) {
+            const int i = r_index(I, a, fg.Nx, fg.tx, fr, hx), j = r_index(J, b, fg.Ny, fg.ty, 1 - fr, hy);
+            const T w = r_weight<T>(I, a, cg.Nx, fg.tx, fr, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 1 - fr, hy);
+            if (w == T(0)) continue;
+            if (!(RC == 0 ? uact(fg, fact, i, j) : vact(fg, fact, i, j))) continue;
+            const long fo = off(fg, i, j);
+#pragma unroll
+            for (int cc = 0; cc < 2; cc++) {
+                const GBlk fb = gblk(RC, cc);
+                for (int dx = fb.x0; dx <= fb.x1; dx++)
+                    for (int dy = fb.y0; dy <= fb.y1; dy++) {
+                        const T av = fst[gidx(fb, dx, dy) * fplane + fo];
+                        if (av == T(0)) continue;
+                        // fine column f′ (component cc) and its coarse parents
+                        int i2 = i + dx, j2 = j + dy;
+                        if (!fg.tx) i2 = wrapi(i2, fg.Nx);
+                        if (!fg.ty) j2 = wrapi(j2, fg.Ny);
+                        int I1, I2, J1, J2;
+                        T wx1, wx2, wy1, wy2;
+                        parents<T>(i2, cg.Nx, fg.tx, cc == 0 ? 1 : 0, hx, I1, I2, wx1, wx2);
+                        parents<T>(j2, cg.Ny, fg.ty, cc == 0 ? 0 : 1, hy, J1, J2, wy1, wy2);
+                        const GBlk cb = cc == 0 ? B0 : B1;
+                        const int Is[2] = {I1, I2}, Js[2] = {J1, J2};
+                        const T wxs[2] = {wx1, wx2}, wys[2] = {wy1, wy2};
+                        for (int p = 0; p < 2; p++)
+                            for (int q = 0; q < 2; q++) {
+                                const T wp = wxs[p] * wys[q];
+                                if (wp == T(0)) continue;
+                                const int Ic = Is[p], Jc = Js[q];
+                                if (!(cc == 0 ? uact(cg, cact, Ic, Jc) : vact(cg, cact, Ic, Jc))) continue;
+                                int ddx = Ic - I, ddy = Jc - J;
+                                if (!g_fold(ddx, cb.x0, cb.x1, cg.Nx, !cg.tx) || !g_fold(ddy, cb.y0, cb.y1, cg.Ny, !cg.ty)) {
+                                    atomicAdd(err, 1);
+                                    continue;
+                                }
+                                acc[(gidx(cb, ddx, ddy) - gblk(RC, 0).base) * NT] += w * av * wp;
+                            }
+                    }
+            }
+        }
+}
+
+// A_{l+1} = Pᵀ A_l P and 1/ℓ1 of its rows; inactive coarse rows become identity rows.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
+                                              const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
+                                              int *err) {
+    cg.tx = TX; cg.ty = TY;
+    fg.tx = TX; fg.ty = TY;
+    __shared__ T acc_s[31 * NT];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    T *acc = acc_s + tid;
+    const int I = blockIdx.x * BX + threadIdx.x + 1, J = blockIdx.y * BY + threadIdx.y + 1;
+    if (I > cg.Nx || J > cg.Ny) return;
+    const long o = off(cg, I, J), P = gplane(cg), fP = gplane(fg);
+#pragma unroll
+    for (int rc = 0; rc < 2; rc++) {
+        for (int k = 0; k < 31; k++) acc[k * NT] = T(0);
+        const bool active = rc == 0 ? uact(cg, cact, I, J) : vact(cg, cact, I, J);
+        if (active) {
+            if (rc == 0) g_rap_row<T, 0>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, acc, err);
+            else g_rap_row<T, 1>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, acc, err);
+        } else {
+            acc[(gidx(gblk(rc, rc), 0, 0) - gblk(rc, 0).base) * NT] = T(1);
+        }
+        T l1 = T(0);
+        const int base = gblk(rc, 0).base;
+        for (int k = 0; k < 31; k++) {
+            const T v = acc[k * NT];
+            cst[(base + k) * P + o] = v;
+            l1 += fabs(v);
+        }
+        cst[(GS_PLANES + rc) * P + o] = l1 > T(0) ? T(1) / l1 : T(1);
+    }
+}
+
+// ℓ1-Jacobi sweep with a stencil operator: x_out = x + (b − A x)/ℓ1, or b/ℓ1 from a zero x.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_g_sweep(KG g, const T *__restrict__ st, const T *__restrict__ xu, const T *__restrict__ xv,
+                                                const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                                T *__restrict__ ov, int from_zero, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j), P = gplane(g);
+    const T su = st[GS_PLANES * P + o], sv = st[(GS_PLANES + 1) * P + o];
+    if (from_zero) {
+        ou[o] = bu[o] * su;
+        ov[o] = bv[o] * sv;
+        return;
+    }
+    const T qu = g_row<T, 0>(g, st, P, o, i, j, xu, xv);
+    const T qv = g_row<T, 1>(g, st, P, o, i, j, xu, xv);
+    ou[o] = xu[o] + (bu[o] - qu) * su;
+    ov[o] = xv[o] + (bv[o] - qv) * sv;
+}
+
+// r = b − A x with a stencil operator
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_g_residual(KG g, const T *__restrict__ st, const T *__restrict__ xu, const T *__restrict__ xv,
+                                                   const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ru,
+                                                   T *__restrict__ rv, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j), P = gplane(g);
+    ru[o] = bu[o] - g_row<T, 0>(g, st, P, o, i, j, xu, xv);
+    rv[o] = bv[o] - g_row<T, 1>(g, st, P, o, i, j, xu, xv);
+}
+
+}  // namespace nk

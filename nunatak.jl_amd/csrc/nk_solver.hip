@@ -15,6 +15,7 @@
 #include "nk_device.hpp"
 #include "nk_physics.hpp"
 #include "nk_adjoint.hpp"
+#include "nk_galerkin.hpp"
 #include "nk_internal.hpp"
 
 using namespace nk;
@@ -45,6 +46,8 @@ struct Level {
     uint8_t *act = nullptr;
     Vecs<double> v64;
     Vecs<float> v32;
+    double *gs64 = nullptr;  // Galerkin stencil + 1/ℓ1 planes (GS_ALLOC) at the V-cycle's precision
+    float *gs32 = nullptr;
 };
 
 template <typename T> struct Work {  // PCG workspace of one precision on the finest grid
@@ -88,6 +91,7 @@ struct nk_solver {
     hipGraphExec_t chunk_graph = nullptr;  // captured PCG chunk (check_interval iterations)
     int chunk_graph_key = -1;
     const void *chunk_graph_x = nullptr;  // solution buffer the captured chunk updates
+    int *gerr = nullptr;                   // Galerkin products: couplings that fell outside the stencil pattern
     // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
     std::vector<hipEvent_t> prof_events;
     size_t prof_used = 0;
@@ -206,6 +210,9 @@ template <> Coef<float> coef<float>(const Level &L) { return Coef<float>{L.eta32
 template <typename T> Vecs<T> &vecs(Level &L);
 template <> Vecs<double> &vecs<double>(Level &L) { return L.v64; }
 template <> Vecs<float> &vecs<float>(Level &L) { return L.v32; }
+template <typename T> T *gs(Level &L);
+template <> double *gs<double>(Level &L) { return L.gs64; }
+template <> float *gs<float>(Level &L) { return L.gs32; }
 template <typename T> Work<T> &work(nk_solver *s);
 template <> Work<double> &work<double>(nk_solver *s) { return s->w64; }
 template <> Work<float> &work<float>(nk_solver *s) { return s->w32; }
@@ -266,6 +273,29 @@ nk_status corners_flags(nk_solver *s, int l, bool write_corners) {
 }
 
 // update_preconditioner!: coarse coefficients of every level (multigrid) and FP32 copies (mixed precision)
+// Galerkin coarse operators: the finest operator as a stencil, then A_{l+1} = Pᵀ A_l P level by level.
+template <typename T> nk_status galerkin_build(nk_solver *s) {
+    Level &L0 = s->lev[0];
+    const T wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
+    NK_LAUNCH_TOPO(L0.g, k_g_assemble, T, blocks_for(L0.g.Nx, L0.g.Ny), THREADS, 0, s->stream, L0.g, coef<T>(L0), wf, gs<T>(L0));
+    CHK(launch_check(s));
+    HIPCHK(hipMemsetAsync(s->gerr, 0, sizeof(int), s->stream));
+    for (size_t l = 1; l < s->lev.size(); l++) {
+        Level &F = s->lev[l - 1], &C = s->lev[l];
+        NK_LAUNCH_TOPO(C.g, k_g_rap, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
+                       gs<T>(C), s->gerr);
+        CHK(launch_check(s));
+    }
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, s->gerr, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (bad) {
+        s->err = "Galerkin product: " + std::to_string(bad) + " couplings outside the coarse stencil pattern";
+        return NK_DEVICE_ERROR;
+    }
+    return NK_OK;
+}
+
 nk_status update_preconditioner(nk_solver *s) {
     if (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID) {
         for (size_t l = 1; l < s->lev.size(); l++) {
@@ -282,6 +312,10 @@ nk_status update_preconditioner(nk_solver *s) {
                                                                                           L.etaff32, L.bu32, L.bv32);
             CHK(launch_check(s));
         }
+    }
+    if (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID && s->op.coarse_operator == NK_COARSE_GALERKIN && s->lev.size() > 1) {
+        if (s->op.precision == NK_PRECISION_MIXED) CHK(galerkin_build<float>(s));
+        else CHK(galerkin_build<double>(s));
     }
     return NK_OK;
 }
@@ -417,6 +451,57 @@ nk_status coarse_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv, 
 
 template <typename T>
 nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, const T *bv, bool final_dot, const Scal *gate,
+                  Pair<T> *out);
+
+// The V-cycle below a level with a Galerkin operator (l ≥ 1): ℓ1-Jacobi sweeps and residuals with the stencil,
+// the same restriction and prolongation kernels as the rediscretized cycle.
+template <typename T>
+nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, const T *bv, const Scal *gate, Pair<T> *out) {
+    const int nlev = (int)s->lev.size();
+    Level &L = s->lev[l];
+    Vecs<T> &V = vecs<T>(L);
+    const T *st = gs<T>(L);
+    const dim3 B = blocks_for(L.g.Nx, L.g.Ny);
+    auto smooth = [&](Pair<T> &cur, Pair<T> &oth, int sweeps, bool from_zero) -> nk_status {
+        if (from_zero && sweeps == 0) {
+            k_fill2<T><<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, T(0), gate);
+            return launch_check(s);
+        }
+        for (int k = 0; k < sweeps; k++) {
+            if (from_zero && k == 0) {
+                NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, cur.u, cur.v, 1, gate);
+            } else {
+                NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, oth.u, oth.v, 0, gate);
+                std::swap(cur, oth);
+            }
+            CHK(launch_check(s));
+        }
+        return NK_OK;
+    };
+    Pair<T> cur = x0, oth = x1;
+    if (l == nlev - 1) {
+        CHK(smooth(cur, oth, s->op.coarsest_sweeps, true));
+        *out = cur;
+        return NK_OK;
+    }
+    CHK(smooth(cur, oth, s->op.smoothing_sweeps, true));
+    NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+    CHK(launch_check(s));
+    Level &C = s->lev[l + 1];
+    Vecs<T> &CV = vecs<T>(C);
+    NK_LAUNCH_TOPO(C.g, k_restrict, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, L.g, C.cx, C.cy, C.act, L.act, V.qu, V.qv, CV.bu, CV.bv, gate);
+    CHK(launch_check(s));
+    Pair<T> cres;
+    CHK(v_cycle<T>(s, l + 1, Pair<T>{CV.x0u, CV.x0v}, Pair<T>{CV.x1u, CV.x1v}, CV.bu, CV.bv, false, gate, &cres));
+    NK_LAUNCH_TOPO(L.g, k_prolongate, T, B, THREADS, 0, s->stream, L.g, C.g, C.cx, C.cy, L.act, C.act, cres.u, cres.v, cur.u, cur.v, gate);
+    CHK(launch_check(s));
+    CHK(smooth(cur, oth, s->op.smoothing_sweeps, false));
+    *out = cur;
+    return NK_OK;
+}
+
+template <typename T>
+nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, const T *bv, bool final_dot, const Scal *gate,
                   Pair<T> *out) {
     const int nlev = (int)s->lev.size();
     Level &L = s->lev[l];
@@ -433,6 +518,7 @@ nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, cons
         *out = x0;
         return NK_OK;
     }
+    if (l >= 1 && s->op.coarse_operator == NK_COARSE_GALERKIN) return v_cycle_galerkin<T>(s, l, x0, x1, bu, bv, gate, out);
     if (l == nlev - 1) return smooth_from_zero<T>(s, l, x0, x1, bu, bv, s->op.coarsest_sweeps, final_dot, gate, out);
     Pair<T> cur;
     CHK(smooth_from_zero<T>(s, l, x0, x1, bu, bv, s->op.smoothing_sweeps, false, gate, &cur));
@@ -776,6 +862,15 @@ nk_status validate(const nk_grid *g, const nk_physics *ph, const nk_solver_optio
         err = "mixed precision runs the multigrid V-cycle in FP32: it needs the multigrid preconditioner";
         return NK_INVALID_ARGUMENT;
     }
+    if (op->coarse_operator != NK_COARSE_REDISCRETIZED && op->coarse_operator != NK_COARSE_GALERKIN) {
+        err = "unknown coarse operator";
+        return NK_INVALID_ARGUMENT;
+    }
+    if (op->coarse_operator == NK_COARSE_GALERKIN &&
+        (op->preconditioner != NK_PRECONDITIONER_MULTIGRID || op->smoother != NK_POINT_JACOBI)) {
+        err = "Galerkin coarse operators need the multigrid preconditioner with point-Jacobi smoothing";
+        return NK_INVALID_ARGUMENT;
+    }
     return NK_OK;
 }
 
@@ -817,7 +912,7 @@ void nk_default_options(nk_solver_options *o) {
     o->picard.kind = NK_CONVERGENCE; o->picard.reltol = 1e-6; o->picard.maxiter = 100;
     o->preconditioner = NK_PRECONDITIONER_DIAGONAL;
     o->smoother = NK_POINT_JACOBI; o->smoothing_sweeps = 2; o->coarsest_sweeps = 20; o->minimum_coarse_size = 2;
-    o->precision = NK_PRECISION_FP64; o->inner_reltol = 1e-3; o->inner_maxiter = 200; o->check_interval = 8;
+    o->precision = NK_PRECISION_FP64; o->coarse_operator = NK_COARSE_REDISCRETIZED; o->check_interval = 8;
     o->profile_kernel = NK_PROFILE_NONE;
     o->graphs = 1;
 }
@@ -903,6 +998,11 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
                 float **v32[] = {&L.v32.x0u, &L.v32.x0v, &L.v32.x1u, &L.v32.x1v, &L.v32.bu, &L.v32.bv, &L.v32.qu, &L.v32.qv, &L.v32.t};
                 for (auto p : v32) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
             }
+            if (options->coarse_operator == NK_COARSE_GALERKIN) {
+                if (options->precision == NK_PRECISION_MIXED) st = alloc_plane(s, &L.gs32, L.g, GS_ALLOC);
+                else st = alloc_plane(s, &L.gs64, L.g, GS_ALLOC);
+                if (st != NK_OK) return fail(st);
+            }
         }
     }
     {
@@ -941,11 +1041,14 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) s->tail_start[0] = s->tail_start[1] = -1;
         const char *nt = std::getenv("NK_NO_TAIL");
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
+        // the LDS tail runs the rediscretized operator: levels with Galerkin operators run level by level
+        if (options->coarse_operator == NK_COARSE_GALERKIN) s->tail_start[options->precision == NK_PRECISION_MIXED ? 1 : 0] = -1;
     }
     {
         long nb = (long)((s->g.Nx + BX - 1) / BX) * (long)((s->g.Ny + BY - 1) / BY);
         if ((st = alloc_bytes(s, (void **)&s->partials, sizeof(double) * MAXRED * (size_t)nb)) != NK_OK) return fail(st);
         if ((st = alloc_bytes(s, (void **)&s->counter, 64)) != NK_OK) return fail(st);
+        if (options->coarse_operator == NK_COARSE_GALERKIN && (st = alloc_bytes(s, (void **)&s->gerr, 64)) != NK_OK) return fail(st);
         if ((st = alloc_bytes(s, (void **)&s->S, sizeof(Scal))) != NK_OK) return fail(st);
         if (hipHostMalloc((void **)&s->S_host, sizeof(Scal), hipHostMallocDefault) != hipSuccess) { s->err = "hipHostMalloc failed"; return fail(NK_OUT_OF_MEMORY); }
     }
@@ -981,11 +1084,13 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
         if (l > 0) b += 4 * sizeof(double) * p;
         b += p + 9 * sizeof(double) * p;                                      // activity flags, FP64 vectors
         if (mixed) b += (4 + 9) * sizeof(float) * p;
+        if (o->coarse_operator == NK_COARSE_GALERKIN) b += GS_ALLOC * (mixed ? sizeof(float) : sizeof(double)) * p;
     }
     b += 24 * sizeof(double) * p0;
     if (mixed) b += 12 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
+    if (o->coarse_operator == NK_COARSE_GALERKIN) b += 64;
     return b;
 }
 

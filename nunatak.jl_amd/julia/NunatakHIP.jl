@@ -49,7 +49,7 @@ struct NKOptions
     linear::NKControl; picard::NKControl
     preconditioner::Int32; smoother::Int32; smoothing_sweeps::Int32; coarsest_sweeps::Int32
     minimum_coarse_size::Int32; precision::Int32
-    inner_reltol::Float64; inner_maxiter::Int32; check_interval::Int32; profile_kernel::Int32; graphs::Int32
+    coarse_operator::Int32; check_interval::Int32; profile_kernel::Int32; graphs::Int32; pad::Int32
 end
 
 mutable struct NKInfo
@@ -92,14 +92,16 @@ end
 
 function HIPStressBalanceSolver(grid; linear_iteration_control = Convergence(eltype(grid); reltol = 1e-8, maxiter = 10_000),
                                 picard_iteration_control = Convergence(eltype(grid); reltol = 1e-6, maxiter = 100),
-                                preconditioner = :diagonal, precision = :fp64, schedule = IterationInterval(1),
+                                preconditioner = :diagonal, precision = :fp64, coarse_operator = :rediscretized,
+                                schedule = IterationInterval(1),
                                 physics = nothing)
     Nx, Ny, _ = size(grid)
     TX, TY, _ = topology(grid)
     Hx, _, _ = halo_size(grid)
     g = NKGrid(Nx, Ny, TX === Bounded, TY === Bounded, Hx, 10, grid.Δxᶜᵃᵃ, grid.Δyᵃᶜᵃ, 2.0)
     o = NKOptions(control(linear_iteration_control), control(picard_iteration_control),
-                  preconditioner === :multigrid ? 2 : 1, 0, 2, 20, 2, precision === :mixed ? 1 : 0, 1e-3, 200, 8, 0, 1)
+                  preconditioner === :multigrid ? 2 : 1, 0, 2, 20, 2, precision === :mixed ? 1 : 0,
+                  coarse_operator === :galerkin ? 1 : 0, 8, 0, 1, 0)
     h = Ref{Ptr{Cvoid}}(C_NULL)   # physics are set from the model at every solve
     check(ccall((:nk_create, libnunatak_hip), Cint, (Ref{NKGrid}, Ref{NKPhysics}, Ref{NKOptions}, Ref{Ptr{Cvoid}}),
                 g, default_physics(), o, h), h[], "nk_create")

@@ -18,6 +18,7 @@ RATE_FACTOR_2D, RATE_FACTOR_3D = 0, 1
 CONVERGENCE, FIXED_ITERATIONS = 0, 1
 PRECONDITIONER_NONE, PRECONDITIONER_DIAGONAL, PRECONDITIONER_MULTIGRID = 0, 1, 2
 POINT_JACOBI, LINE_JACOBI = 0, 1
+COARSE_REDISCRETIZED, COARSE_GALERKIN = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 CLASS_WALL, CLASS_ICE_FREE, CLASS_FRONT, CLASS_GROUNDING = 1, 2, 4, 8
@@ -67,9 +68,9 @@ class Control(ct.Structure):
 class SolverOptions(ct.Structure):
     _fields_ = [("linear", Control), ("picard", Control), ("preconditioner", ct.c_int32),
                 ("smoother", ct.c_int32), ("smoothing_sweeps", ct.c_int32), ("coarsest_sweeps", ct.c_int32),
-                ("minimum_coarse_size", ct.c_int32), ("precision", ct.c_int32), ("inner_reltol", ct.c_double),
-                ("inner_maxiter", ct.c_int32), ("check_interval", ct.c_int32), ("profile_kernel", ct.c_int32),
-                ("graphs", ct.c_int32)]
+                ("minimum_coarse_size", ct.c_int32), ("precision", ct.c_int32), ("coarse_operator", ct.c_int32),
+                ("check_interval", ct.c_int32), ("profile_kernel", ct.c_int32), ("graphs", ct.c_int32),
+                ("pad", ct.c_int32)]
 
 
 class Info(ct.Structure):
@@ -126,9 +127,8 @@ def default_options(**kw):
     """StressBalanceSolver and StressBalanceMultigridPreconditioner defaults of the reference."""
     o = SolverOptions(linear=convergence(1e-8, 0.0, 10_000), picard=convergence(1e-6, 0.0, 100),
                       preconditioner=PRECONDITIONER_DIAGONAL, smoother=POINT_JACOBI, smoothing_sweeps=2,
-                      coarsest_sweeps=20, minimum_coarse_size=2, precision=PRECISION_FP64, inner_reltol=1e-3,
-                      inner_maxiter=200, check_interval=8, profile_kernel=0,
-                      graphs=1)
+                      coarsest_sweeps=20, minimum_coarse_size=2, precision=PRECISION_FP64,
+                      coarse_operator=COARSE_REDISCRETIZED, check_interval=8, profile_kernel=0, graphs=1, pad=0)
     for k, v in kw.items():
         setattr(o, k, v)
     return o

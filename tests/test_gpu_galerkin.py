@@ -1,0 +1,98 @@
+"""Galerkin coarse operators (NK_COARSE_GALERKIN) of the multigrid preconditioner.
+
+No reference counterpart: the reference rediscretizes coarse levels. Checks: the Galerkin-preconditioned PCG
+converges to the same solution as the reference's rediscretized MG-PCG (bar 1e-7 relative L2: both stop at a
+relative residual of 1e-10), in fewer iterations on the calving-front channel (the case the Galerkin product is
+for; the CPU prototype of the same construction needs 29 / 39 iterations at 64² / 128² against the reference's
+48 / 80), with every coupling inside the 31-point coarse stencil on all four topologies and on semi-coarsened
+hierarchies; and the mixed-precision form with Galerkin operators stays within 1e-8 of the FP64 solve.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import B, P, abi
+from nunatak_hip import workloads as W
+from nunatak_hip.model import HIPModel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def channel(N, coarse, precision=abi.PRECISION_FP64, picard=1, linear=1e-10, topology=(B, B), Ny=None):
+    g, phys, opts, fields = W.diva_shelf(N=N, dx=512e3 / N)
+    Ny = Ny or N
+    if Ny != N or topology != (B, B):
+        g = abi.default_grid(N, Ny, g.dx, g.dy, topology=topology, halo=4, Nsigma=g.Nsigma)
+        fields = {k: (v[:Ny] if v.ndim == 2 else v[:, :Ny]) for k, v in fields.items()}
+    phys.stress_balance = abi.SSA
+    phys.rate_factor_depth = abi.RATE_FACTOR_2D
+    fields = dict(fields)
+    fields.pop("TEMPERATURE")
+    fields["A"] = np.full((Ny, N), 1e-25)
+    opts.linear = abi.convergence(linear, 0.0, 20_000)
+    opts.picard = abi.fixed_iterations(picard)
+    opts.coarse_operator = coarse
+    opts.precision = precision
+    m = HIPModel(g, phys, opts)
+    W.load_into(m, fields)
+    info = m.update_velocities(geometry=True)
+    return m, info
+
+
+def residual(m):
+    """‖A x − τ‖ / ‖τ‖ of the handle's solution through nk_apply_operator (coefficients at the last Picard iterate)."""
+    m.set("PU", m.get("U"))
+    m.set("PV", m.get("V"))
+    m.apply_operator()
+    qu, qv, tu, tv = m.get("QU"), m.get("QV"), m.get("TAUU"), m.get("TAUV")
+    g = m.grid
+    r = np.concatenate([(qu - tu)[: g.Ny, : g.Nx].ravel(), (qv - tv)[: g.Ny, : g.Nx].ravel()])
+    b = np.concatenate([tu[: g.Ny, : g.Nx].ravel(), tv[: g.Ny, : g.Nx].ravel()])
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def rel(m1, m2):
+    a = np.concatenate([m1.get("U").ravel(), m1.get("V").ravel()])
+    b = np.concatenate([m2.get("U").ravel(), m2.get("V").ravel()])
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("N,bound", [(64, 40), (128, 50)])
+def test_galerkin_converges_to_the_same_solution_in_fewer_iterations(N, bound):
+    mr, ir = channel(N, abi.COARSE_REDISCRETIZED)
+    mg, ig = channel(N, abi.COARSE_GALERKIN)
+    e = rel(mg, mr)
+    say(f"[galerkin] {N}²: rediscretized {ir['total_linear_iterations']} iterations, Galerkin "
+        f"{ig['total_linear_iterations']}, rel diff {e:.2e}")
+    assert ig["total_linear_iterations"] <= bound
+    assert ig["total_linear_iterations"] < ir["total_linear_iterations"]
+    assert e < 1e-7
+
+
+@pytest.mark.parametrize("topology", [(B, B), (B, P), (P, B), (P, P)])
+@pytest.mark.parametrize("shape", [(96, 96), (128, 40)])
+def test_galerkin_pattern_closes_on_every_topology(topology, shape):
+    N, Ny = shape
+    mr, ir = channel(N, abi.COARSE_REDISCRETIZED, topology=topology, Ny=Ny)
+    mg, ig = channel(N, abi.COARSE_GALERKIN, topology=topology, Ny=Ny)
+    e = rel(mg, mr)
+    res = residual(mg)
+    say(f"[galerkin] {shape} {topology}: levels {mg.multigrid_levels()}, iterations {ir['total_linear_iterations']} → "
+        f"{ig['total_linear_iterations']}, rel diff {e:.2e}, residual {res:.2e}")
+    assert res < 1e-9
+    if topology != (P, P):   # doubly periodic and afloat: rigid translations are a null space, solutions may differ by one
+        assert e < 1e-7
+
+
+def test_mixed_galerkin_matches_fp64():
+    m64, i64 = channel(128, abi.COARSE_REDISCRETIZED, picard=4, linear=1e-8)
+    mmx, imx = channel(128, abi.COARSE_GALERKIN, precision=abi.PRECISION_MIXED, picard=4, linear=1e-8)
+    e = rel(mmx, m64)
+    say(f"[galerkin] mixed+Galerkin vs FP64: {e:.2e}; CG {i64['total_linear_iterations']} → {imx['total_linear_iterations']}")
+    assert e < 1e-8
