@@ -226,4 +226,176 @@ __global__ __launch_bounds__(NT) void k_g_residual(KG g, const T *__restrict__ s
     rv[o] = bv[o] - g_row<T, 1>(g, st, P, o, i, j, xu, xv);
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Coarse tail with Galerkin operators: the levels below tail_start of a V-cycle in one 1024-thread workgroup.
+// Vectors live in LDS (two x buffers for Jacobi ping-pong, b, r: 8 values and 1 flag byte per point); stencils and
+// 1/ℓ1 stay in global memory (L2-resident at these sizes). Same operations and order as the level-by-level cycle.
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int GTAIL_MAX = 12;
+
+template <typename T> struct GTailLevel {
+    KG lg;                  // LDS geometry (pitch = Nx + 1)
+    int gpitch;             // pitch of the level's global planes
+    int hx, hy;             // halved relative to the next finer level
+    const T *st;            // stencil + 1/ℓ1 planes (global)
+    const uint8_t *act;     // activity flags (global)
+    int o_x0u, o_x0v, o_x1u, o_x1v, o_bu, o_bv, o_ru, o_rv, o_act;   // LDS offsets (elements of T; bytes for o_act)
+};
+
+template <typename T> struct GTailDesc {
+    int n, sweeps, coarsest;
+    const T *gbu, *gbv;     // right-hand side of the top tail level (global)
+    T *gxu, *gxv;           // result of the top tail level (global)
+    GTailLevel<T> lv[GTAIL_MAX];
+};
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const Scal *gate) {
+    if (solve_done(gate)) return;
+    auto geom = [](const KG &k) { KG g = k; g.tx = TX; g.ty = TY; return g; };
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *sm = reinterpret_cast<T *>(smem);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int l = 0; l < d.n; l++) {
+        const GTailLevel<T> &L = d.lv[l];
+        const int nx = L.lg.Nx + 1, ny = L.lg.Ny + 1;
+        for (int p = tid; p < nx * ny; p += nth) {
+            const int i = p % nx, j = p / nx;
+            const long go = (long)j * L.gpitch + i;
+            const int lo = j * L.lg.pitch + i;
+            smem[L.o_act + lo] = L.act[go];
+            sm[L.o_x0u + lo] = T(0); sm[L.o_x0v + lo] = T(0);
+            sm[L.o_x1u + lo] = T(0); sm[L.o_x1v + lo] = T(0);
+            sm[L.o_ru + lo] = T(0); sm[L.o_rv + lo] = T(0);
+            sm[L.o_bu + lo] = l == 0 ? d.gbu[go] : T(0);
+            sm[L.o_bv + lo] = l == 0 ? d.gbv[go] : T(0);
+        }
+    }
+    __syncthreads();
+    int cur[GTAIL_MAX];   // which x buffer holds the level's iterate (0: x0, 1: x1)
+    auto X = [&](int l, int which, int comp) -> T * {
+        const GTailLevel<T> &L = d.lv[l];
+        return sm + (which == 0 ? (comp == 0 ? L.o_x0u : L.o_x0v) : (comp == 0 ? L.o_x1u : L.o_x1v));
+    };
+    // sweeps of x ← x + (b − A x)/ℓ1 (x = b/ℓ1 first when from_zero); one barrier per sweep (ping-pong buffers)
+    auto smooth = [&](int l, int sweeps, bool from_zero) {
+        const GTailLevel<T> &L = d.lv[l];
+        const KG g = geom(L.lg);
+        const long P = (long)L.gpitch * (long)(L.lg.Ny + 1);
+        const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
+        for (int k = 0; k < sweeps; k++) {
+            const bool z = from_zero && k == 0;
+            const int src = cur[l], dst = z ? src : 1 - src;
+            const T *xu = X(l, src, 0), *xv = X(l, src, 1);
+            T *ou = X(l, dst, 0), *ov = X(l, dst, 1);
+            for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+                const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+                const long go = (long)(j - 1) * L.gpitch + (i - 1);
+                const long o = off(g, i, j);
+                const T su = L.st[GS_PLANES * P + go], sv = L.st[(GS_PLANES + 1) * P + go];
+                if (z) {
+                    ou[o] = bu[o] * su;
+                    ov[o] = bv[o] * sv;
+                } else {
+                    const T qu = g_row<T, 0>(g, L.st, P, go, i, j, xu, xv);
+                    const T qv = g_row<T, 1>(g, L.st, P, go, i, j, xu, xv);
+                    ou[o] = xu[o] + (bu[o] - qu) * su;
+                    ov[o] = xv[o] + (bv[o] - qv) * sv;
+                }
+            }
+            cur[l] = dst;
+            __syncthreads();
+        }
+    };
+    for (int l = 0; l < d.n; l++) cur[l] = 0;
+    int l = 0;
+    for (; l < d.n - 1; l++) {
+        const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
+        const KG g = geom(L.lg), cg = geom(C.lg);
+        const long P = (long)L.gpitch * (long)(L.lg.Ny + 1);
+        smooth(l, d.sweeps, true);
+        if (d.sweeps == 0) {   // x = 0
+            for (int p = tid; p < (g.Nx + 1) * (g.Ny + 1); p += nth) {
+                const int lo = (p / (g.Nx + 1)) * g.pitch + p % (g.Nx + 1);
+                X(l, cur[l], 0)[lo] = T(0); X(l, cur[l], 1)[lo] = T(0);
+            }
+            __syncthreads();
+        }
+        const T *xu = X(l, cur[l], 0), *xv = X(l, cur[l], 1);
+        const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
+        T *ru = sm + L.o_ru, *rv = sm + L.o_rv;
+        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+            const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+            const long go = (long)(j - 1) * L.gpitch + (i - 1);
+            const long o = off(g, i, j);
+            ru[o] = bu[o] - g_row<T, 0>(g, L.st, P, go, i, j, xu, xv);
+            rv[o] = bv[o] - g_row<T, 1>(g, L.st, P, go, i, j, xu, xv);
+        }
+        __syncthreads();
+        const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
+        T *cbu = sm + C.o_bu, *cbv = sm + C.o_bv;
+        for (int p = tid; p < cg.Nx * cg.Ny; p += nth) {
+            const int I = p % cg.Nx + 1, J = p / cg.Nx + 1;
+            T su = T(0), sv = T(0);
+            for (int a = (C.hx ? -1 : 0); a <= (C.hx ? 1 : 0); a++)
+                for (int b = (C.hy ? -1 : 0); b <= (C.hy ? 2 : 0); b++) {
+                    const int i = r_index(I, a, g.Nx, g.tx, 1, C.hx), j = r_index(J, b, g.Ny, g.ty, 0, C.hy);
+                    su += r_weight<T>(I, a, cg.Nx, g.tx, 1, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 0, C.hy) * iu(g, fact, ru, i, j);
+                }
+            for (int a = (C.hx ? -1 : 0); a <= (C.hx ? 2 : 0); a++)
+                for (int b = (C.hy ? -1 : 0); b <= (C.hy ? 1 : 0); b++) {
+                    const int i = r_index(I, a, g.Nx, g.tx, 0, C.hx), j = r_index(J, b, g.Ny, g.ty, 1, C.hy);
+                    sv += r_weight<T>(I, a, cg.Nx, g.tx, 0, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 1, C.hy) * iv(g, fact, rv, i, j);
+                }
+            const long o = off(cg, I, J);
+            cbu[o] = (cact[o] & 1) ? su : T(0);
+            cbv[o] = (cact[o] & 2) ? sv : T(0);
+        }
+        __syncthreads();
+    }
+    // coarsest level
+    smooth(l, d.coarsest, true);
+    if (d.coarsest == 0) {
+        const KG g = geom(d.lv[l].lg);
+        for (int p = tid; p < (g.Nx + 1) * (g.Ny + 1); p += nth) {
+            const int lo = (p / (g.Nx + 1)) * g.pitch + p % (g.Nx + 1);
+            X(l, cur[l], 0)[lo] = T(0); X(l, cur[l], 1)[lo] = T(0);
+        }
+        __syncthreads();
+    }
+    // up leg
+    for (l = d.n - 2; l >= 0; l--) {
+        const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
+        const KG g = geom(L.lg), cg = geom(C.lg);
+        const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
+        const T *cu = X(l + 1, cur[l + 1], 0), *cv = X(l + 1, cur[l + 1], 1);
+        T *xu = X(l, cur[l], 0), *xv = X(l, cur[l], 1);
+        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+            const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+            int I1, I2, J1, J2;
+            T wx1, wx2, wy1, wy2;
+            parents<T>(i, cg.Nx, g.tx, 1, C.hx, I1, I2, wx1, wx2);
+            parents<T>(j, cg.Ny, g.ty, 0, C.hy, J1, J2, wy1, wy2);
+            const T pu = wx1 * (wy1 * iu(cg, cact, cu, I1, J1) + wy2 * iu(cg, cact, cu, I1, J2)) +
+                         wx2 * (wy1 * iu(cg, cact, cu, I2, J1) + wy2 * iu(cg, cact, cu, I2, J2));
+            parents<T>(i, cg.Nx, g.tx, 0, C.hx, I1, I2, wx1, wx2);
+            parents<T>(j, cg.Ny, g.ty, 1, C.hy, J1, J2, wy1, wy2);
+            const T pv = wx1 * (wy1 * iv(cg, cact, cv, I1, J1) + wy2 * iv(cg, cact, cv, I1, J2)) +
+                         wx2 * (wy1 * iv(cg, cact, cv, I2, J1) + wy2 * iv(cg, cact, cv, I2, J2));
+            const long o = off(g, i, j);
+            xu[o] += (fact[o] & 1) ? pu : T(0);
+            xv[o] += (fact[o] & 2) ? pv : T(0);
+        }
+        __syncthreads();
+        smooth(l, d.sweeps, false);
+    }
+    const GTailLevel<T> &L = d.lv[0];
+    const T *xu = X(0, cur[0], 0), *xv = X(0, cur[0], 1);
+    for (int p = tid; p < L.lg.Nx * L.lg.Ny; p += nth) {
+        const int i = p % L.lg.Nx, j = p / L.lg.Nx;
+        d.gxu[(long)j * L.gpitch + i] = xu[j * L.lg.pitch + i];
+        d.gxv[(long)j * L.gpitch + i] = xv[j * L.lg.pitch + i];
+    }
+}
+
 }  // namespace nk

@@ -92,6 +92,7 @@ struct nk_solver {
     int chunk_graph_key = -1;
     const void *chunk_graph_x = nullptr;  // solution buffer the captured chunk updates
     int *gerr = nullptr;                   // Galerkin products: couplings that fell outside the stencil pattern
+    int gtail_start = -1;                  // first level of the Galerkin LDS tail (-1: none)
     // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
     std::vector<hipEvent_t> prof_events;
     size_t prof_used = 0;
@@ -455,9 +456,47 @@ nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, cons
 
 // The V-cycle below a level with a Galerkin operator (l ≥ 1): ℓ1-Jacobi sweeps and residuals with the stencil,
 // the same restriction and prolongation kernels as the rediscretized cycle.
+// Levels l..coarsest of a Galerkin V-cycle in one LDS-resident workgroup: x at level l from b at level l.
+template <typename T>
+nk_status galerkin_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv, const Scal *gate) {
+    GTailDesc<T> d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = (int)s->lev.size() - l;
+    d.sweeps = s->op.smoothing_sweeps;
+    d.coarsest = s->op.coarsest_sweeps;
+    d.gbu = bu; d.gbv = bv; d.gxu = x.u; d.gxv = x.v;
+    int off_t = 0;
+    for (int k = 0; k < d.n; k++) {
+        Level &L = s->lev[l + k];
+        GTailLevel<T> &t = d.lv[k];
+        t.lg = L.g;
+        t.lg.pitch = L.g.Nx + 1;
+        t.gpitch = L.g.pitch;
+        t.hx = L.cx; t.hy = L.cy;
+        t.st = gs<T>(L);
+        t.act = L.act;
+        const int plane = (L.g.Nx + 1) * (L.g.Ny + 1);
+        int *offs[] = {&t.o_x0u, &t.o_x0v, &t.o_x1u, &t.o_x1v, &t.o_bu, &t.o_bv, &t.o_ru, &t.o_rv};
+        for (int *o : offs) { *o = off_t; off_t += plane; }
+    }
+    int off_b = off_t * (int)sizeof(T);
+    for (int k = 0; k < d.n; k++) {
+        Level &L = s->lev[l + k];
+        d.lv[k].o_act = off_b;
+        off_b += (L.g.Nx + 1) * (L.g.Ny + 1);
+    }
+    NK_LAUNCH_TOPO(s->lev[l].g, k_g_tail, T, 1, TAIL_THREADS, (size_t)off_b, s->stream, d, gate);
+    return launch_check(s);
+}
+
 template <typename T>
 nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, const T *bv, const Scal *gate, Pair<T> *out) {
     const int nlev = (int)s->lev.size();
+    if (s->gtail_start >= 1 && l >= s->gtail_start) {
+        CHK(galerkin_tail<T>(s, l, x0, bu, bv, gate));
+        *out = x0;
+        return NK_OK;
+    }
     Level &L = s->lev[l];
     Vecs<T> &V = vecs<T>(L);
     const T *st = gs<T>(L);
@@ -1041,8 +1080,27 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) s->tail_start[0] = s->tail_start[1] = -1;
         const char *nt = std::getenv("NK_NO_TAIL");
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
-        // the LDS tail runs the rediscretized operator: levels with Galerkin operators run level by level
-        if (options->coarse_operator == NK_COARSE_GALERKIN) s->tail_start[options->precision == NK_PRECISION_MIXED ? 1 : 0] = -1;
+        // the LDS tail runs the rediscretized operator: levels with Galerkin operators have a tail of their own
+        if (options->coarse_operator == NK_COARSE_GALERKIN) {
+            s->tail_start[options->precision == NK_PRECISION_MIXED ? 1 : 0] = -1;
+            const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
+            size_t bytes = 0;
+            int start = -1;
+            for (int l = (int)s->lev.size() - 1; l >= 1; l--) {
+                const Level &L = s->lev[l];
+                const size_t b = (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1) * (8 * es + 1);
+                if (bytes + b > budget || (int)s->lev.size() - l > GTAIL_MAX) break;
+                bytes += b;
+                start = l;
+            }
+            const void *gt[] = {(const void *)k_g_tail<double, 0, 0>, (const void *)k_g_tail<double, 0, 1>,
+                                (const void *)k_g_tail<double, 1, 0>, (const void *)k_g_tail<double, 1, 1>,
+                                (const void *)k_g_tail<float, 0, 0>, (const void *)k_g_tail<float, 0, 1>,
+                                (const void *)k_g_tail<float, 1, 0>, (const void *)k_g_tail<float, 1, 1>};
+            bool ok = true;
+            for (const void *f : gt) ok &= hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+            if (ok && start >= 1 && (int)s->lev.size() - start >= 2 && !(nt && nt[0] == '1')) s->gtail_start = start;
+        }
     }
     {
         long nb = (long)((s->g.Nx + BX - 1) / BX) * (long)((s->g.Ny + BY - 1) / BY);
