@@ -142,8 +142,8 @@ def serving_request(N, seed, dx=1e3, precision=abi.PRECISION_MIXED, linear_relto
     """One request of the serving workload: config 1's geometry at N × N, dx (N − 1 ice columns and one open-ocean
     column, H = 1000 − 8·(x/km)·(64 km / L) + box3(N(0, 10 m)), no-slip walls) on the friction-inversion bed
     B = −300 − 1400·x/L (grounded upstream, afloat downstream) with its own seeded friction
-    C = 3.16e6·exp(GRF(σ = 0.2, ℓ = 20 km)); SSA, Glen A = 1e-25, Weertman m = 1/3, MG-PCG at the given precision,
-    linear reltol 1e-8, Picard 1e-6."""
+    C = 3.16e6·exp(GRF(σ = 0.2, ℓ = 20 km)); SSA, Glen A = 1e-25, Weertman m = 1/3, MG-PCG at the given precision
+    (the mixed form with Galerkin coarse operators), linear reltol 1e-8, Picard 1e-6."""
     rng = np.random.Generator(np.random.PCG64(seed))
     L = N * dx
     g = abi.default_grid(N, N, dx, dx, topology=(abi.BOUNDED, abi.BOUNDED), halo=4)
@@ -156,7 +156,9 @@ def serving_request(N, seed, dx=1e3, precision=abi.PRECISION_MIXED, linear_relto
                                walls=abi.NO_SLIP, friction=abi.FRICTION_WEERTMAN, friction_exponent=1.0 / 3.0)
     opts = abi.default_options(linear=abi.convergence(linear_reltol, 0.0, 10_000),
                                picard=abi.convergence(picard_reltol, 0.0, 100),
-                               preconditioner=abi.PRECONDITIONER_MULTIGRID, precision=precision)
+                               preconditioner=abi.PRECONDITIONER_MULTIGRID, precision=precision,
+                               coarse_operator=(abi.COARSE_GALERKIN if precision == abi.PRECISION_MIXED
+                                                else abi.COARSE_REDISCRETIZED))
     fields = {"H": H, "BED": B, "A": np.full((N, N), 1e-25), "C": C}
     return g, phys, opts, fields
 
