@@ -158,6 +158,9 @@ def main():
     ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
     ap.add_argument("--graphs", type=int, default=1, help="0: launch kernel by kernel (for external profilers)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: one device copy of 1 GiB (torch) as the counter calibration, then one "
+                         "Picard iteration of the mixed solve launched kernel by kernel; prints nothing else")
     ap.add_argument("--coarse", choices=["galerkin", "rediscretized"], default="galerkin",
                     help="coarse operators of the mixed form's FP32 V-cycle (the FP64 form keeps the reference's)")
     ap.add_argument("--serve", type=int, default=48, help="requests in the serving leg (0: skip)")
@@ -171,6 +174,18 @@ def main():
         sys.exit(2)
     N, K, Wm = args.size, args.steps, args.warmup
     os.environ.setdefault("NK_VERBOSE", "1")  # one library progress line per Picard iteration
+    if args.traffic_probe:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] traffic probe: copied 1 GiB")
+        m, _ = make(N, abi.PRECISION_MIXED, picard=1, graphs=0, coarse=abi.COARSE_GALERKIN)
+        reset_state(m)
+        info = m.update_velocities(geometry=True)
+        log(f"[bench] traffic probe: 1 Picard iteration, {info['total_linear_iterations']} CG iterations")
+        return
 
     log(f"[bench] building FP64 and mixed handles at {N}x{N}")
     results = {}
@@ -231,8 +246,12 @@ def main():
     bytes_per_launch = N * N * (10 * T + 1)
     achieved = bytes_per_launch / dur_s / 1e9 if dur_s > 0 else 0.0
     log(f"[bench] profiled sweep: {launches} launches, avg {dur_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_k_sweep_f32_1024.json")
+    if T == 4 and N == 1024 and os.path.exists(tfile):   # measured by rocprofv3 --pmc runs of --traffic-probe
+        traffic = json.load(open(tfile))["traffic_bytes_per_launch"]
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                        "kernel": f"k_sweep<{'float' if T == 4 else 'double'}> (finest grid)",
                        "avg_launch_us": dur_s * 1e6, "launches": launches, "algorithmic_bytes_per_launch": bytes_per_launch}
     if not args.no_cpu_baseline and "fp64" in results:

@@ -300,9 +300,25 @@ __global__ __launch_bounds__(NT) void k_finish(const double *__restrict__ partia
     if (tid == 0) finish(S, fin, tot);
 }
 
+// XCD-aware tile order: the dispatcher hands consecutive workgroup ids to the 8 XCDs in turn, so id → tile is
+// remapped to give each XCD one contiguous run of tiles (row-major). Stencil neighbours — the halo rows and columns
+// of a tile — are then mostly fetched by the same XCD's L2 instead of by all eight.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void xcd_tile(int &bx, int &by) {
+    const int n = gridDim.x * gridDim.y;
+    const int id = blockIdx.x + blockIdx.y * gridDim.x;
+    const int q = n / NXCD, r = n % NXCD;
+    const int x = id % NXCD, k = id / NXCD;
+    const int t = x * q + (x < r ? x : r) + k;
+    bx = t % gridDim.x;
+    by = t / gridDim.x;
+}
+
 #define NK_IJ                                                 \
-    const int i = blockIdx.x * BX + threadIdx.x + 1;          \
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    int bx_, by_;                                             \
+    xcd_tile(bx_, by_);                                       \
+    const int i = bx_ * BX + threadIdx.x + 1;                 \
+    const int j = by_ * BY + threadIdx.y + 1;
 
 // ---------------------------------------------------------------------------------------------------------------
 // PCG kernels
@@ -584,7 +600,9 @@ __global__ __launch_bounds__(NT) void k_restrict(KG cg, KG fg, int hx, int hy, c
     cg.tx = TX; cg.ty = TY;
     fg.tx = TX; fg.ty = TY;
     if (solve_done(gate)) return;
-    const int I = blockIdx.x * BX + threadIdx.x + 1, J = blockIdx.y * BY + threadIdx.y + 1;
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int I = bx_ * BX + threadIdx.x + 1, J = by_ * BY + threadIdx.y + 1;
     if (I > cg.Nx || J > cg.Ny) return;
     const long o = off(cg, I, J);
     T su = T(0), sv = T(0);
@@ -636,7 +654,9 @@ __global__ __launch_bounds__(NT) void k_prolongate(KG fg, KG cg, int hx, int hy,
 __global__ __launch_bounds__(NT) void k_coarsen(KG cg, KG fg, int hx, int hy, const double *__restrict__ feta,
                                                 const double *__restrict__ fbu, const double *__restrict__ fbv,
                                                 double *__restrict__ eta, double *__restrict__ bu, double *__restrict__ bv) {
-    const int I = blockIdx.x * BX + threadIdx.x + 1, J = blockIdx.y * BY + threadIdx.y + 1;
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int I = bx_ * BX + threadIdx.x + 1, J = by_ * BY + threadIdx.y + 1;
     if (I > cg.Nx || J > cg.Ny) return;
     const int fxf = hx ? 2 : 1, fyf = hy ? 2 : 1;
     const int i = fxf * (I - 1) + 1, j = fyf * (J - 1) + 1;

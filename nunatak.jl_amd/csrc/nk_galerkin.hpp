@@ -111,7 +111,7 @@ __device__ __forceinline__ bool g_fold(int &d, int lo, int hi, int N, int period
 template <typename T, int RC>
 __device__ __forceinline__ void g_rap_row(const KG &cg, const KG &fg, int hx, int hy, const uint8_t *cact, const uint8_t *fact,
                                           const T *__restrict__ fst, long fplane, int I, int J, T *acc, int *err) {
-    constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
+    constexpr int base = gblk(RC, 0).base;
     const int fr = RC == 0 ? 1 : 0;   // restriction: u is Face along x, v is Face along y
     for (int a = (hx ? -1 : 0); a <= (hx ? (RC == 0 ? 1 : 2) : 0); a++)
         for (int b = (hy ? -1 : 0); b <= (hy ? (RC == 0 ? 2 : 1) : 0); b++) {
@@ -120,13 +120,20 @@ __device__ __forceinline__ void g_rap_row(const KG &cg, const KG &fg, int hx, in
             if (w == T(0)) continue;
             if (!(RC == 0 ? uact(fg, fact, i, j) : vact(fg, fact, i, j))) continue;
             const long fo = off(fg, i, j);
+            // the fine row's 31 couplings, loaded together before any is used
+            T av[31];
+#pragma unroll
+            for (int k = 0; k < 31; k++) av[k] = fst[(base + k) * fplane + fo];
 #pragma unroll
             for (int cc = 0; cc < 2; cc++) {
-                const GBlk fb = gblk(RC, cc);
+                constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
+                const GBlk fb = cc == 0 ? B0 : B1;
+#pragma unroll
                 for (int dx = fb.x0; dx <= fb.x1; dx++)
+#pragma unroll
                     for (int dy = fb.y0; dy <= fb.y1; dy++) {
-                        const T av = fst[gidx(fb, dx, dy) * fplane + fo];
-                        if (av == T(0)) continue;
+                        const T c = av[gidx(fb, dx, dy) - base];
+                        if (c == T(0)) continue;
                         // fine column f′ (component cc) and its coarse parents
                         int i2 = i + dx, j2 = j + dy;
                         if (!fg.tx) i2 = wrapi(i2, fg.Nx);
@@ -135,21 +142,23 @@ __device__ __forceinline__ void g_rap_row(const KG &cg, const KG &fg, int hx, in
                         T wx1, wx2, wy1, wy2;
                         parents<T>(i2, cg.Nx, fg.tx, cc == 0 ? 1 : 0, hx, I1, I2, wx1, wx2);
                         parents<T>(j2, cg.Ny, fg.ty, cc == 0 ? 0 : 1, hy, J1, J2, wy1, wy2);
-                        const GBlk cb = cc == 0 ? B0 : B1;
                         const int Is[2] = {I1, I2}, Js[2] = {J1, J2};
                         const T wxs[2] = {wx1, wx2}, wys[2] = {wy1, wy2};
+#pragma unroll
                         for (int p = 0; p < 2; p++)
+#pragma unroll
                             for (int q = 0; q < 2; q++) {
                                 const T wp = wxs[p] * wys[q];
                                 if (wp == T(0)) continue;
                                 const int Ic = Is[p], Jc = Js[q];
                                 if (!(cc == 0 ? uact(cg, cact, Ic, Jc) : vact(cg, cact, Ic, Jc))) continue;
+                                // the coarse row couples to component cc through the same block of the pattern
                                 int ddx = Ic - I, ddy = Jc - J;
-                                if (!g_fold(ddx, cb.x0, cb.x1, cg.Nx, !cg.tx) || !g_fold(ddy, cb.y0, cb.y1, cg.Ny, !cg.ty)) {
+                                if (!g_fold(ddx, fb.x0, fb.x1, cg.Nx, !cg.tx) || !g_fold(ddy, fb.y0, fb.y1, cg.Ny, !cg.ty)) {
                                     atomicAdd(err, 1);
                                     continue;
                                 }
-                                acc[(gidx(cb, ddx, ddy) - gblk(RC, 0).base) * NT] += w * av * wp;
+                                acc[(gidx(fb, ddx, ddy) - base) * NT] += w * c * wp;
                             }
                     }
             }
@@ -166,7 +175,9 @@ __global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, cons
     __shared__ T acc_s[31 * NT];
     const int tid = threadIdx.y * BX + threadIdx.x;
     T *acc = acc_s + tid;
-    const int I = blockIdx.x * BX + threadIdx.x + 1, J = blockIdx.y * BY + threadIdx.y + 1;
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int I = bx_ * BX + threadIdx.x + 1, J = by_ * BY + threadIdx.y + 1;
     if (I > cg.Nx || J > cg.Ny) return;
     const long o = off(cg, I, J), P = gplane(cg), fP = gplane(fg);
 #pragma unroll

@@ -32,13 +32,14 @@ def alone(g, p, o, f):
     return u, v
 
 
-@pytest.mark.parametrize("case", ["fp64-diag", "fp64-mg", "mixed-mg", "diva-3d"])
+@pytest.mark.parametrize("case", ["fp64-diag", "fp64-mg", "mixed-mg-galerkin", "diva-3d"])
 def test_estimate_matches_allocation(case):
     g, p, o, f = W.serving_request(96, 1)
     if case == "fp64-diag":
         o.preconditioner, o.precision = abi.PRECONDITIONER_DIAGONAL, abi.PRECISION_FP64
+        o.coarse_operator = abi.COARSE_REDISCRETIZED
     elif case == "fp64-mg":
-        o.precision = abi.PRECISION_FP64
+        o.precision, o.coarse_operator = abi.PRECISION_FP64, abi.COARSE_REDISCRETIZED
     elif case == "diva-3d":
         g, p, o, f = W.diva_shelf(N=80, dx=1e3)
     m = HIPModel(g, p, o)
