@@ -606,19 +606,29 @@ __global__ __launch_bounds__(NT) void k_restrict(KG cg, KG fg, int hx, int hy, c
     if (I > cg.Nx || J > cg.Ny) return;
     const long o = off(cg, I, J);
     T su = T(0), sv = T(0);
+    // Fixed trip counts, so every load of the support is issued before the first is used; offsets outside the
+    // support of an un-halved direction read an in-range point with weight 0 (adding +0 leaves each sum unchanged).
     // u: Face along x, Center along y
-    for (int a = (hx ? -1 : 0); a <= (hx ? 1 : 0); a++)
-        for (int b = (hy ? -1 : 0); b <= (hy ? 2 : 0); b++) {
+#pragma unroll
+    for (int a = -1; a <= 1; a++)
+#pragma unroll
+        for (int b = -1; b <= 2; b++) {
+            const bool on = (hx || a == 0) && (hy || b == 0);
             const int i = r_index(I, a, fg.Nx, fg.tx, 1, hx), j = r_index(J, b, fg.Ny, fg.ty, 0, hy);
             const T w = r_weight<T>(I, a, cg.Nx, fg.tx, 1, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 0, hy);
-            su += w * iu(fg, fact, ru, i, j);
+            const T val = iu(fg, fact, ru, i, j);
+            if (on) su += w * val;
         }
     // v: Center along x, Face along y
-    for (int a = (hx ? -1 : 0); a <= (hx ? 2 : 0); a++)
-        for (int b = (hy ? -1 : 0); b <= (hy ? 1 : 0); b++) {
+#pragma unroll
+    for (int a = -1; a <= 2; a++)
+#pragma unroll
+        for (int b = -1; b <= 1; b++) {
+            const bool on = (hx || a == 0) && (hy || b == 0);
             const int i = r_index(I, a, fg.Nx, fg.tx, 0, hx), j = r_index(J, b, fg.Ny, fg.ty, 1, hy);
             const T w = r_weight<T>(I, a, cg.Nx, fg.tx, 0, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 1, hy);
-            sv += w * iv(fg, fact, rv, i, j);
+            const T val = iv(fg, fact, rv, i, j);
+            if (on) sv += w * val;
         }
     bu[o] = (cact[o] & 1) ? su : T(0);
     bv[o] = (cact[o] & 2) ? sv : T(0);

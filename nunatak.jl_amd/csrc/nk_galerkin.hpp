@@ -105,99 +105,139 @@ __device__ __forceinline__ bool g_fold(int &d, int lo, int hi, int N, int period
     return d >= lo && d <= hi;
 }
 
-// One coarse row RC at (I, J): Σ over the fine points f of the restriction (weight w_f, fine-active) of
-// w_f · A_l(f, f′) · P(f′, C) over the fine couplings f′ of f and the coarse parents C of f′ (coarse-active).
-// acc: this thread's 31 accumulators in LDS (stride NT).
+// Contribution of one fine point f = support point (a, b) of the restriction of coarse row RC at (I, J):
+// w_f · Σ over the fine couplings f′ of f of A_l(f, f′) · P(f′, C), over the coarse parents C of f′ (coarse-active),
+// added into acc[0..31) (the row's couplings in pattern order).
 template <typename T, int RC>
-__device__ __forceinline__ void g_rap_row(const KG &cg, const KG &fg, int hx, int hy, const uint8_t *cact, const uint8_t *fact,
-                                          const T *__restrict__ fst, long fplane, int I, int J, T *acc, int *err) {
+__device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, int hy, const uint8_t *cact, const uint8_t *fact,
+                                            const T *__restrict__ fst, long fplane, int I, int J, int a, int b, T *acc, int *err) {
     constexpr int base = gblk(RC, 0).base;
     const int fr = RC == 0 ? 1 : 0;   // restriction: u is Face along x, v is Face along y
-    for (int a = (hx ? -1 : 0); a <= (hx ? (RC == 0 ? 1 : 2) : 0); a++)
-        for (int b = (hy ? -1 : 0); b <= (hy ? (RC == 0 ? 2 : 1) : 0); b++) {
-            const int i = r_index(I, a, fg.Nx, fg.tx, fr, hx), j = r_index(J, b, fg.Ny, fg.ty, 1 - fr, hy);
-            const T w = r_weight<T>(I, a, cg.Nx, fg.tx, fr, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 1 - fr, hy);
-            if (w == T(0)) continue;
-            if (!(RC == 0 ? uact(fg, fact, i, j) : vact(fg, fact, i, j))) continue;
-            const long fo = off(fg, i, j);
-            // the fine row's 31 couplings, loaded together before any is used
-            T av[31];
+    const int i = r_index(I, a, fg.Nx, fg.tx, fr, hx), j = r_index(J, b, fg.Ny, fg.ty, 1 - fr, hy);
+    const T w = r_weight<T>(I, a, cg.Nx, fg.tx, fr, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 1 - fr, hy);
+    if (w == T(0)) return;
+    if (!(RC == 0 ? uact(fg, fact, i, j) : vact(fg, fact, i, j))) return;
+    // activity of the coarse columns within ±2 of (I, J): bit (dx + 2)·5 + (dy + 2) for u, + 25 for v
+    uint64_t cmask = 0;
 #pragma unroll
-            for (int k = 0; k < 31; k++) av[k] = fst[(base + k) * fplane + fo];
+    for (int dx = -2; dx <= 2; dx++)
 #pragma unroll
-            for (int cc = 0; cc < 2; cc++) {
-                constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
-                const GBlk fb = cc == 0 ? B0 : B1;
+        for (int dy = -2; dy <= 2; dy++) {
+            const int bit = (dx + 2) * 5 + (dy + 2);
+            int ic = I + dx, jc = J + dy;
+            if (!cg.tx) ic = wrapi(ic, cg.Nx);
+            if (!cg.ty) jc = wrapi(jc, cg.Ny);
+            cmask |= (uint64_t)uact(cg, cact, ic, jc) << bit;
+            cmask |= (uint64_t)vact(cg, cact, ic, jc) << (25 + bit);
+        }
+    const long fo = off(fg, i, j);
+    T av[31];
 #pragma unroll
-                for (int dx = fb.x0; dx <= fb.x1; dx++)
+    for (int k = 0; k < 31; k++) av[k] = fst[(base + k) * fplane + fo];
 #pragma unroll
-                    for (int dy = fb.y0; dy <= fb.y1; dy++) {
-                        const T c = av[gidx(fb, dx, dy) - base];
-                        if (c == T(0)) continue;
-                        // fine column f′ (component cc) and its coarse parents
-                        int i2 = i + dx, j2 = j + dy;
-                        if (!fg.tx) i2 = wrapi(i2, fg.Nx);
-                        if (!fg.ty) j2 = wrapi(j2, fg.Ny);
-                        int I1, I2, J1, J2;
-                        T wx1, wx2, wy1, wy2;
-                        parents<T>(i2, cg.Nx, fg.tx, cc == 0 ? 1 : 0, hx, I1, I2, wx1, wx2);
-                        parents<T>(j2, cg.Ny, fg.ty, cc == 0 ? 0 : 1, hy, J1, J2, wy1, wy2);
-                        const int Is[2] = {I1, I2}, Js[2] = {J1, J2};
-                        const T wxs[2] = {wx1, wx2}, wys[2] = {wy1, wy2};
+    for (int cc = 0; cc < 2; cc++) {
+        constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
+        const GBlk fb = cc == 0 ? B0 : B1;
 #pragma unroll
-                        for (int p = 0; p < 2; p++)
+        for (int dx = fb.x0; dx <= fb.x1; dx++)
 #pragma unroll
-                            for (int q = 0; q < 2; q++) {
-                                const T wp = wxs[p] * wys[q];
-                                if (wp == T(0)) continue;
-                                const int Ic = Is[p], Jc = Js[q];
-                                if (!(cc == 0 ? uact(cg, cact, Ic, Jc) : vact(cg, cact, Ic, Jc))) continue;
-                                // the coarse row couples to component cc through the same block of the pattern
-                                int ddx = Ic - I, ddy = Jc - J;
-                                if (!g_fold(ddx, fb.x0, fb.x1, cg.Nx, !cg.tx) || !g_fold(ddy, fb.y0, fb.y1, cg.Ny, !cg.ty)) {
-                                    atomicAdd(err, 1);
-                                    continue;
-                                }
-                                acc[(gidx(fb, ddx, ddy) - base) * NT] += w * c * wp;
-                            }
+            for (int dy = fb.y0; dy <= fb.y1; dy++) {
+                const T c = av[gidx(fb, dx, dy) - base];
+                if (c == T(0)) continue;
+                int i2 = i + dx, j2 = j + dy;
+                if (!fg.tx) i2 = wrapi(i2, fg.Nx);
+                if (!fg.ty) j2 = wrapi(j2, fg.Ny);
+                int I1, I2, J1, J2;
+                T wx1, wx2, wy1, wy2;
+                parents<T>(i2, cg.Nx, fg.tx, cc == 0 ? 1 : 0, hx, I1, I2, wx1, wx2);
+                parents<T>(j2, cg.Ny, fg.ty, cc == 0 ? 0 : 1, hy, J1, J2, wy1, wy2);
+                const int Is[2] = {I1, I2}, Js[2] = {J1, J2};
+                const T wxs[2] = {wx1, wx2}, wys[2] = {wy1, wy2};
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+#pragma unroll
+                    for (int q = 0; q < 2; q++) {
+                        const T wp = wxs[p] * wys[q];
+                        if (wp == T(0)) continue;
+                        int ddx = Is[p] - I, ddy = Js[q] - J;
+                        int mx = ddx, my = ddy;
+                        if (!g_fold(mx, -2, 2, cg.Nx, !cg.tx) || !g_fold(my, -2, 2, cg.Ny, !cg.ty)) {
+                            atomicAdd(err, 1);
+                            continue;
+                        }
+                        if (!((cmask >> (cc * 25 + (mx + 2) * 5 + (my + 2))) & 1)) continue;
+                        // the coarse row couples to component cc through the same block of the pattern
+                        if (!g_fold(ddx, fb.x0, fb.x1, cg.Nx, !cg.tx) || !g_fold(ddy, fb.y0, fb.y1, cg.Ny, !cg.ty)) {
+                            atomicAdd(err, 1);
+                            continue;
+                        }
+                        acc[gidx(fb, ddx, ddy) - base] += w * c * wp;
                     }
             }
-        }
+    }
 }
 
-// A_{l+1} = Pᵀ A_l P and 1/ℓ1 of its rows; inactive coarse rows become identity rows.
+// A_{l+1} = Pᵀ A_l P and 1/ℓ1 of its rows; inactive coarse rows become identity rows. One 256-thread block per 8
+// coarse points: 16 row slots (u and v of each point) × 16 lanes, lane s < 12 handling support point s of the row's
+// restriction into its own 31 partial sums in LDS; the partials are then summed over the lanes in lane order
+// (deterministic). One-dimensional grid of ⌈Nx·Ny / 8⌉ blocks.
+constexpr int GRAP_POINTS = 8;
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
                                               const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
                                               int *err) {
     cg.tx = TX; cg.ty = TY;
     fg.tx = TX; fg.ty = TY;
-    __shared__ T acc_s[31 * NT];
+    __shared__ T part[2 * GRAP_POINTS][12][31];
     const int tid = threadIdx.y * BX + threadIdx.x;
-    T *acc = acc_s + tid;
-    int bx_, by_;
-    xcd_tile(bx_, by_);
-    const int I = bx_ * BX + threadIdx.x + 1, J = by_ * BY + threadIdx.y + 1;
-    if (I > cg.Nx || J > cg.Ny) return;
-    const long o = off(cg, I, J), P = gplane(cg), fP = gplane(fg);
-#pragma unroll
-    for (int rc = 0; rc < 2; rc++) {
-        for (int k = 0; k < 31; k++) acc[k * NT] = T(0);
-        const bool active = rc == 0 ? uact(cg, cact, I, J) : vact(cg, cact, I, J);
+    const int slot = tid / 16, sub = tid % 16, rc = slot & 1;
+    const int npts = cg.Nx * cg.Ny;
+    const int q = blockIdx.x * GRAP_POINTS + (slot >> 1);
+    const int I = q % cg.Nx + 1, J = q / cg.Nx + 1;
+    const bool active = q < npts && (rc == 0 ? uact(cg, cact, I, J) : vact(cg, cact, I, J));
+    if (sub < 12) {
+        T *acc = part[slot][sub];
+        for (int k = 0; k < 31; k++) acc[k] = T(0);
         if (active) {
-            if (rc == 0) g_rap_row<T, 0>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, acc, err);
-            else g_rap_row<T, 1>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, acc, err);
-        } else {
-            acc[(gidx(gblk(rc, rc), 0, 0) - gblk(rc, 0).base) * NT] = T(1);
+            const int a = rc == 0 ? sub / 4 - 1 : sub / 3 - 1, b = rc == 0 ? sub % 4 - 1 : sub % 3 - 1;
+            const bool on = (hx || a == 0) && (hy || b == 0);
+            if (on) {
+                const long fP = gplane(fg);
+                if (rc == 0) g_rap_point<T, 0>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, a, b, acc, err);
+                else g_rap_point<T, 1>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, a, b, acc, err);
+            }
         }
-        T l1 = T(0);
-        const int base = gblk(rc, 0).base;
-        for (int k = 0; k < 31; k++) {
-            const T v = acc[k * NT];
+    }
+    __syncthreads();
+    // lanes of each slot sum couplings k = lane, lane + 16 over the 12 support points, in support order
+    T vals[2] = {T(0), T(0)};
+    const long P = gplane(cg);
+    const int base = gblk(rc, 0).base, diag = gidx(gblk(rc, rc), 0, 0) - base;
+    if (q < npts) {
+        const long o = off(cg, I, J);
+        for (int t = 0; t < 2; t++) {
+            const int k = sub + 16 * t;
+            if (k >= 31) continue;
+            T v = T(0);
+            if (active) {
+                for (int m = 0; m < 12; m++) v += part[slot][m][k];
+            } else {
+                v = k == diag ? T(1) : T(0);
+            }
             cst[(base + k) * P + o] = v;
-            l1 += fabs(v);
+            vals[t] = fabs(v);
         }
-        cst[(GS_PLANES + rc) * P + o] = l1 > T(0) ? T(1) / l1 : T(1);
+    }
+    __syncthreads();
+    if (q < npts) {
+        part[slot][0][sub] = vals[0];
+        if (sub < 15) part[slot][1][sub] = vals[1];
+    }
+    __syncthreads();
+    if (sub == 0 && q < npts) {
+        T l1 = T(0);
+        for (int k = 0; k < 31; k++) l1 += k < 16 ? part[slot][0][k] : part[slot][1][k - 16];
+        cst[(GS_PLANES + rc) * P + off(cg, I, J)] = l1 > T(0) ? T(1) / l1 : T(1);
     }
 }
 
@@ -238,9 +278,109 @@ __global__ __launch_bounds__(NT) void k_g_residual(KG g, const T *__restrict__ s
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Coarsest level: the Galerkin operator as a dense matrix, inverted once per build (Gauss-Jordan without pivoting:
+// the operator is symmetric positive semidefinite, an inactive row is an identity row). A pivot that falls below
+// a relative tolerance of its original diagonal marks a null direction (a floating, doubly periodic domain has the
+// rigid translations): that unknown is left out and its row and column of the result are zero, which leaves the
+// inverse of the remaining block — a symmetric positive semidefinite generalized inverse. Unknown k < n/2 is u at
+// (k % Nx + 1, k / Nx + 1), k ≥ n/2 is v likewise; n = 2·Nx·Ny.
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int GDENSE_MAX = 512;   // largest coarsest system inverted densely; above it the coarsest level is swept
+
+__device__ __forceinline__ int g_unknown(const KG &g, int comp, int i, int j) { return comp * g.Nx * g.Ny + (j - 1) * g.Nx + (i - 1); }
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const T *__restrict__ st, T *__restrict__ M) {
+    g.tx = TX; g.ty = TY;
+    __shared__ T fcol[GDENSE_MAX];
+    __shared__ T dorig[GDENSE_MAX];
+    __shared__ unsigned char sing[GDENSE_MAX];
+    __shared__ T piv;
+    __shared__ int skip;
+    const T tol = sizeof(T) == 4 ? T(1e-5) : T(1e-10);
+    const int n = 2 * g.Nx * g.Ny, tid = threadIdx.x, nth = blockDim.x;
+    const long P = gplane(g);
+    for (int e = tid; e < n * n; e += nth) M[e] = T(0);
+    __syncthreads();
+    // assemble: row r of component rc at (i, j) from its 31 couplings (out-of-range columns carry zero couplings)
+    for (int r = tid; r < n; r += nth) {
+        const int rc = r >= n / 2, k = r - rc * (n / 2), i = k % g.Nx + 1, j = k / g.Nx + 1;
+        const long o = off(g, i, j);
+        for (int cc = 0; cc < 2; cc++) {
+            const GBlk b = gblk(rc, cc);
+            for (int dx = b.x0; dx <= b.x1; dx++)
+                for (int dy = b.y0; dy <= b.y1; dy++) {
+                    const T a = st[gidx(b, dx, dy) * P + o];
+                    if (a == T(0)) continue;
+                    int ic = i + dx, jc = j + dy;
+                    if (!g.tx) ic = wrapi(ic, g.Nx);
+                    if (!g.ty) jc = wrapi(jc, g.Ny);
+                    if (ic < 1 || ic > g.Nx || jc < 1 || jc > g.Ny) continue;
+                    M[(long)r * n + g_unknown(g, cc, ic, jc)] += a;
+                }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nth) {
+        dorig[i] = M[(long)i * n + i];
+        sing[i] = 0;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        if (tid == 0) {
+            const T p = M[(long)k * n + k];
+            skip = !(p > tol * fabs(dorig[k]));
+            if (skip) {
+                sing[k] = 1;
+            } else {
+                piv = T(1) / p;
+                M[(long)k * n + k] = T(1);
+            }
+        }
+        __syncthreads();
+        const int sk = skip;   // every thread reads the flag before thread 0 may write the next one
+        __syncthreads();
+        if (sk) continue;
+        for (int j = tid; j < n; j += nth) M[(long)k * n + j] *= piv;
+        for (int i = tid; i < n; i += nth) fcol[i] = M[(long)i * n + k];
+        __syncthreads();
+        for (int e = tid; e < n * n; e += nth) {
+            const int i = e / n, j = e % n;
+            if (i == k) continue;
+            const T f = fcol[i];
+            M[e] = (j == k) ? -f * M[(long)k * n + k] : M[e] - f * M[(long)k * n + j];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < n * n; e += nth)
+        if (sing[e / n] || sing[e % n]) M[e] = T(0);
+}
+
+// x = M⁻¹ b on the coarsest level (one workgroup)
+template <typename T>
+__global__ __launch_bounds__(TAIL_THREADS) void k_g_dense_solve(KG g, const T *__restrict__ M, const T *__restrict__ bu,
+                                                                const T *__restrict__ bv, T *__restrict__ xu, T *__restrict__ xv,
+                                                                const Scal *gate) {
+    if (solve_done(gate)) return;
+    const int n = 2 * g.Nx * g.Ny;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        T acc = T(0);
+        for (int c = 0; c < n; c++) {
+            const int cc = c >= n / 2, k = c - cc * (n / 2);
+            const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
+            acc += M[(long)r * n + c] * (cc ? bv[o] : bu[o]);
+        }
+        const int rc = r >= n / 2, k = r - rc * (n / 2);
+        const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
+        (rc ? xv : xu)[o] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Coarse tail with Galerkin operators: the levels below tail_start of a V-cycle in one 1024-thread workgroup.
-// Vectors live in LDS (two x buffers for Jacobi ping-pong, b, r: 8 values and 1 flag byte per point); stencils and
-// 1/ℓ1 stay in global memory (L2-resident at these sizes). Same operations and order as the level-by-level cycle.
+// Vectors live in LDS (two x buffers for Jacobi ping-pong, b, r: 8 values and 1 flag byte per point); the stencil
+// planes of the smallest levels are copied into LDS too (o_st ≥ 0), the others are read from global memory
+// (L2-resident at these sizes). The coarsest level is solved with its dense inverse when it has one.
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int GTAIL_MAX = 12;
 
@@ -251,12 +391,14 @@ template <typename T> struct GTailLevel {
     const T *st;            // stencil + 1/ℓ1 planes (global)
     const uint8_t *act;     // activity flags (global)
     int o_x0u, o_x0v, o_x1u, o_x1v, o_bu, o_bv, o_ru, o_rv, o_act;   // LDS offsets (elements of T; bytes for o_act)
+    int o_st;               // LDS offset of the stencil planes, or −1 (read from global memory)
 };
 
 template <typename T> struct GTailDesc {
     int n, sweeps, coarsest;
     const T *gbu, *gbv;     // right-hand side of the top tail level (global)
     T *gxu, *gxv;           // result of the top tail level (global)
+    const T *cinv;          // dense inverse of the coarsest level, or null (sweeps)
     GTailLevel<T> lv[GTAIL_MAX];
 };
 
@@ -270,6 +412,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
     for (int l = 0; l < d.n; l++) {
         const GTailLevel<T> &L = d.lv[l];
         const int nx = L.lg.Nx + 1, ny = L.lg.Ny + 1;
+        const long P = (long)L.gpitch * (long)ny;
         for (int p = tid; p < nx * ny; p += nth) {
             const int i = p % nx, j = p / nx;
             const long go = (long)j * L.gpitch + i;
@@ -280,67 +423,71 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
             sm[L.o_ru + lo] = T(0); sm[L.o_rv + lo] = T(0);
             sm[L.o_bu + lo] = l == 0 ? d.gbu[go] : T(0);
             sm[L.o_bv + lo] = l == 0 ? d.gbv[go] : T(0);
+            if (L.o_st >= 0)
+                for (int k = 0; k < GS_ALLOC; k++) sm[L.o_st + k * nx * ny + lo] = L.st[k * P + go];
         }
     }
     __syncthreads();
-    int cur[GTAIL_MAX];   // which x buffer holds the level's iterate (0: x0, 1: x1)
+    unsigned cur = 0;     // bit l: which x buffer holds level l's iterate (0: x0, 1: x1); a register, not an array
+    auto C_ = [&](int l) { return (int)((cur >> l) & 1u); };
     auto X = [&](int l, int which, int comp) -> T * {
         const GTailLevel<T> &L = d.lv[l];
         return sm + (which == 0 ? (comp == 0 ? L.o_x0u : L.o_x0v) : (comp == 0 ? L.o_x1u : L.o_x1v));
+    };
+    // the level's stencil: base, plane stride and the offset of point (i, j) (LDS copy or global planes)
+    struct St { const T *base; long plane; };
+    auto stencil = [&](const GTailLevel<T> &L) -> St {
+        if (L.o_st >= 0) return St{sm + L.o_st, (long)(L.lg.Nx + 1) * (long)(L.lg.Ny + 1)};
+        return St{L.st, (long)L.gpitch * (long)(L.lg.Ny + 1)};
+    };
+    auto st_off = [&](const GTailLevel<T> &L, int i, int j) -> long {
+        return L.o_st >= 0 ? (long)(j - 1) * L.lg.pitch + (i - 1) : (long)(j - 1) * L.gpitch + (i - 1);
     };
     // sweeps of x ← x + (b − A x)/ℓ1 (x = b/ℓ1 first when from_zero); one barrier per sweep (ping-pong buffers)
     auto smooth = [&](int l, int sweeps, bool from_zero) {
         const GTailLevel<T> &L = d.lv[l];
         const KG g = geom(L.lg);
-        const long P = (long)L.gpitch * (long)(L.lg.Ny + 1);
+        const St S = stencil(L);
         const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
         for (int k = 0; k < sweeps; k++) {
             const bool z = from_zero && k == 0;
-            const int src = cur[l], dst = z ? src : 1 - src;
+            const int src = C_(l), dst = z ? src : 1 - src;
             const T *xu = X(l, src, 0), *xv = X(l, src, 1);
             T *ou = X(l, dst, 0), *ov = X(l, dst, 1);
             for (int p = tid; p < g.Nx * g.Ny; p += nth) {
                 const int i = p % g.Nx + 1, j = p / g.Nx + 1;
-                const long go = (long)(j - 1) * L.gpitch + (i - 1);
+                const long so = st_off(L, i, j);
                 const long o = off(g, i, j);
-                const T su = L.st[GS_PLANES * P + go], sv = L.st[(GS_PLANES + 1) * P + go];
+                const T su = S.base[GS_PLANES * S.plane + so], sv = S.base[(GS_PLANES + 1) * S.plane + so];
                 if (z) {
                     ou[o] = bu[o] * su;
                     ov[o] = bv[o] * sv;
                 } else {
-                    const T qu = g_row<T, 0>(g, L.st, P, go, i, j, xu, xv);
-                    const T qv = g_row<T, 1>(g, L.st, P, go, i, j, xu, xv);
+                    const T qu = g_row<T, 0>(g, S.base, S.plane, so, i, j, xu, xv);
+                    const T qv = g_row<T, 1>(g, S.base, S.plane, so, i, j, xu, xv);
                     ou[o] = xu[o] + (bu[o] - qu) * su;
                     ov[o] = xv[o] + (bv[o] - qv) * sv;
                 }
             }
-            cur[l] = dst;
+            cur = (cur & ~(1u << l)) | ((unsigned)dst << l);
             __syncthreads();
         }
     };
-    for (int l = 0; l < d.n; l++) cur[l] = 0;
     int l = 0;
     for (; l < d.n - 1; l++) {
         const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
         const KG g = geom(L.lg), cg = geom(C.lg);
-        const long P = (long)L.gpitch * (long)(L.lg.Ny + 1);
+        const St S = stencil(L);
         smooth(l, d.sweeps, true);
-        if (d.sweeps == 0) {   // x = 0
-            for (int p = tid; p < (g.Nx + 1) * (g.Ny + 1); p += nth) {
-                const int lo = (p / (g.Nx + 1)) * g.pitch + p % (g.Nx + 1);
-                X(l, cur[l], 0)[lo] = T(0); X(l, cur[l], 1)[lo] = T(0);
-            }
-            __syncthreads();
-        }
-        const T *xu = X(l, cur[l], 0), *xv = X(l, cur[l], 1);
+        const T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
         const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
         T *ru = sm + L.o_ru, *rv = sm + L.o_rv;
         for (int p = tid; p < g.Nx * g.Ny; p += nth) {
             const int i = p % g.Nx + 1, j = p / g.Nx + 1;
-            const long go = (long)(j - 1) * L.gpitch + (i - 1);
+            const long so = st_off(L, i, j);
             const long o = off(g, i, j);
-            ru[o] = bu[o] - g_row<T, 0>(g, L.st, P, go, i, j, xu, xv);
-            rv[o] = bv[o] - g_row<T, 1>(g, L.st, P, go, i, j, xu, xv);
+            ru[o] = bu[o] - g_row<T, 0>(g, S.base, S.plane, so, i, j, xu, xv);
+            rv[o] = bv[o] - g_row<T, 1>(g, S.base, S.plane, so, i, j, xu, xv);
         }
         __syncthreads();
         const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
@@ -364,23 +511,35 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         }
         __syncthreads();
     }
-    // coarsest level
-    smooth(l, d.coarsest, true);
-    if (d.coarsest == 0) {
-        const KG g = geom(d.lv[l].lg);
-        for (int p = tid; p < (g.Nx + 1) * (g.Ny + 1); p += nth) {
-            const int lo = (p / (g.Nx + 1)) * g.pitch + p % (g.Nx + 1);
-            X(l, cur[l], 0)[lo] = T(0); X(l, cur[l], 1)[lo] = T(0);
+    // coarsest level: dense inverse, or sweeps
+    if (d.cinv) {
+        const GTailLevel<T> &L = d.lv[l];
+        const KG g = geom(L.lg);
+        const int n = 2 * g.Nx * g.Ny;
+        const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
+        T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
+        for (int r = tid; r < n; r += nth) {
+            T acc = T(0);
+            for (int c = 0; c < n; c++) {
+                const int cc = c >= n / 2, k = c - cc * (n / 2);
+                const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
+                acc += d.cinv[(long)r * n + c] * (cc ? bv[o] : bu[o]);
+            }
+            const int rc = r >= n / 2, k = r - rc * (n / 2);
+            const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
+            (rc ? xv : xu)[o] = acc;
         }
         __syncthreads();
+    } else {
+        smooth(l, d.coarsest, true);
     }
     // up leg
     for (l = d.n - 2; l >= 0; l--) {
         const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
         const KG g = geom(L.lg), cg = geom(C.lg);
         const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
-        const T *cu = X(l + 1, cur[l + 1], 0), *cv = X(l + 1, cur[l + 1], 1);
-        T *xu = X(l, cur[l], 0), *xv = X(l, cur[l], 1);
+        const T *cu = X(l + 1, C_(l + 1), 0), *cv = X(l + 1, C_(l + 1), 1);
+        T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
         for (int p = tid; p < g.Nx * g.Ny; p += nth) {
             const int i = p % g.Nx + 1, j = p / g.Nx + 1;
             int I1, I2, J1, J2;
@@ -401,7 +560,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         smooth(l, d.sweeps, false);
     }
     const GTailLevel<T> &L = d.lv[0];
-    const T *xu = X(0, cur[0], 0), *xv = X(0, cur[0], 1);
+    const T *xu = X(0, C_(0), 0), *xv = X(0, C_(0), 1);
     for (int p = tid; p < L.lg.Nx * L.lg.Ny; p += nth) {
         const int i = p % L.lg.Nx, j = p / L.lg.Nx;
         d.gxu[(long)j * L.gpitch + i] = xu[j * L.lg.pitch + i];

@@ -2,6 +2,7 @@
 // reference's velocity-solve functions (update_velocities!, picard_solve!, solve!, precondition!, ssa_operator!)
 // over the gfx950 kernels of nk_device.hpp / nk_physics.hpp. C ABI in include/nunatak_hip.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdio>
@@ -93,6 +94,7 @@ struct nk_solver {
     const void *chunk_graph_x = nullptr;  // solution buffer the captured chunk updates
     int *gerr = nullptr;                   // Galerkin products: couplings that fell outside the stencil pattern
     int gtail_start = -1;                  // first level of the Galerkin LDS tail (-1: none)
+    void *cinv = nullptr;                  // dense inverse of the coarsest Galerkin operator (V-cycle precision), or null
     // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
     std::vector<hipEvent_t> prof_events;
     size_t prof_used = 0;
@@ -134,6 +136,17 @@ const dim3 THREADS(BX, BY);
 
 // Launch a stencil kernel instantiated for the grid's topology (Bounded = 1 / Periodic = 0 along x and y), so the
 // index maps compile to straight-line code.
+#define NK_LAUNCH_TOPO_EV(G, KERNEL, T, GRID, BLOCK, SHM, STREAM, E0, E1, ...)                                 \
+    do {                                                                                                      \
+        if ((G).tx) {                                                                                         \
+            if ((G).ty) hipExtLaunchKernelGGL(KERNEL<T, 1, 1>, GRID, BLOCK, SHM, STREAM, E0, E1, 0, __VA_ARGS__); \
+            else hipExtLaunchKernelGGL(KERNEL<T, 1, 0>, GRID, BLOCK, SHM, STREAM, E0, E1, 0, __VA_ARGS__);        \
+        } else {                                                                                              \
+            if ((G).ty) hipExtLaunchKernelGGL(KERNEL<T, 0, 1>, GRID, BLOCK, SHM, STREAM, E0, E1, 0, __VA_ARGS__); \
+            else hipExtLaunchKernelGGL(KERNEL<T, 0, 0>, GRID, BLOCK, SHM, STREAM, E0, E1, 0, __VA_ARGS__);        \
+        }                                                                                                     \
+    } while (0)
+
 #define NK_LAUNCH_TOPO(G, KERNEL, T, GRID, BLOCK, SHM, STREAM, ...)                                  \
     do {                                                                                             \
         if ((G).tx) {                                                                                \
@@ -219,9 +232,11 @@ template <> Work<double> &work<double>(nk_solver *s) { return s->w64; }
 template <> Work<float> &work<float>(nk_solver *s) { return s->w32; }
 
 // Event bracket around one launch of the profiled kernel class on the finest grid.
+// The launch inside a ProfScope goes through hipExtLaunchKernelGGL with the scope's event pair, which the runtime
+// records at the kernel's start and end (null events when the scope is inactive: a plain launch).
 struct ProfScope {
     nk_solver *s;
-    int idx = -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     ProfScope(nk_solver *s_, int kind, int level, int precision_bytes) : s(s_) {
         if (s->op.profile_kernel != kind || kind == NK_PROFILE_NONE || level != 0) return;
         if (s->prof_used + 2 > s->prof_events.size()) {
@@ -231,15 +246,12 @@ struct ProfScope {
                 s->prof_events.push_back(e);
             }
         }
-        idx = (int)s->prof_used;
+        e0 = s->prof_events[s->prof_used];
+        e1 = s->prof_events[s->prof_used + 1];
         s->prof_used += 2;
-        (void)hipEventRecord(s->prof_events[idx], s->stream);
         s->prof_launches += 1;
         s->prof_points += (long)s->g.Nx * s->g.Ny;
         s->prof_precision = precision_bytes;
-    }
-    ~ProfScope() {
-        if (idx >= 0) (void)hipEventRecord(s->prof_events[idx + 1], s->stream);
     }
 };
 
@@ -283,8 +295,14 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
     HIPCHK(hipMemsetAsync(s->gerr, 0, sizeof(int), s->stream));
     for (size_t l = 1; l < s->lev.size(); l++) {
         Level &F = s->lev[l - 1], &C = s->lev[l];
-        NK_LAUNCH_TOPO(C.g, k_g_rap, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
+        const dim3 Br((C.g.Nx * C.g.Ny + GRAP_POINTS - 1) / GRAP_POINTS);
+        NK_LAUNCH_TOPO(C.g, k_g_rap, T, Br, THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
                        gs<T>(C), s->gerr);
+        CHK(launch_check(s));
+    }
+    if (s->cinv) {
+        Level &C = s->lev.back();
+        NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, 0, s->stream, C.g, gs<T>(C), (T *)s->cinv);
         CHK(launch_check(s));
     }
     int bad = 0;
@@ -361,8 +379,8 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
                            last ? 1 : 0, gate);
         } else {
             ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-            NK_LAUNCH_TOPO(L.g, k_sweep, T, B, THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, bu, bv, oth.u, oth.v, 0,
-                           red(s, FIN_BETA), last ? 1 : 0, gate);
+            NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu,
+                              bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
         }
         CHK(launch_check(s));
         if (last) CHK(finish_red(s, B, 1, FIN_BETA, gate));
@@ -394,7 +412,8 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
             const bool last = (k == sweeps - 1) && final_dot;
             {
                 ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-                NK_LAUNCH_TOPO(L.g, k_sweep, T, B, THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+                NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf,
+                                  bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
             }
             CHK(launch_check(s));
             if (last) CHK(finish_red(s, B, 1, FIN_BETA, gate));
@@ -465,6 +484,7 @@ nk_status galerkin_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv
     d.sweeps = s->op.smoothing_sweeps;
     d.coarsest = s->op.coarsest_sweeps;
     d.gbu = bu; d.gbv = bv; d.gxu = x.u; d.gxv = x.v;
+    d.cinv = (const T *)s->cinv;
     int off_t = 0;
     for (int k = 0; k < d.n; k++) {
         Level &L = s->lev[l + k];
@@ -475,9 +495,22 @@ nk_status galerkin_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv
         t.hx = L.cx; t.hy = L.cy;
         t.st = gs<T>(L);
         t.act = L.act;
+        t.o_st = -1;
         const int plane = (L.g.Nx + 1) * (L.g.Ny + 1);
         int *offs[] = {&t.o_x0u, &t.o_x0v, &t.o_x1u, &t.o_x1v, &t.o_bu, &t.o_bv, &t.o_ru, &t.o_rv};
         for (int *o : offs) { *o = off_t; off_t += plane; }
+    }
+    {   // copy the stencils of the smallest levels into LDS while they fit (the coarsest needs none with a dense inverse)
+        int flags = 0;
+        for (int k = 0; k < d.n; k++) flags += (s->lev[l + k].g.Nx + 1) * (s->lev[l + k].g.Ny + 1);
+        const int cap = 160 * 1024;
+        for (int k = d.n - (d.cinv ? 2 : 1); k >= 0; k--) {
+            const Level &L = s->lev[l + k];
+            const int need = GS_ALLOC * (L.g.Nx + 1) * (L.g.Ny + 1);
+            if ((off_t + need) * (int)sizeof(T) + flags > cap) break;
+            d.lv[k].o_st = off_t;
+            off_t += need;
+        }
     }
     int off_b = off_t * (int)sizeof(T);
     for (int k = 0; k < d.n; k++) {
@@ -519,7 +552,12 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     };
     Pair<T> cur = x0, oth = x1;
     if (l == nlev - 1) {
-        CHK(smooth(cur, oth, s->op.coarsest_sweeps, true));
+        if (s->cinv) {
+            k_g_dense_solve<T><<<1, TAIL_THREADS, 0, s->stream>>>(L.g, (const T *)s->cinv, bu, bv, cur.u, cur.v, gate);
+            CHK(launch_check(s));
+        } else {
+            CHK(smooth(cur, oth, s->op.coarsest_sweeps, true));
+        }
         *out = cur;
         return NK_OK;
     }
@@ -656,14 +694,16 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         for (int k = 0; k < chunk; k++) {
             {
                 ProfScope ps(s, NK_PROFILE_OPERATOR, 0, (int)sizeof(T));
-                NK_LAUNCH_TOPO(g, k_apply, T, B, THREADS, 0, s->stream, g, W.pu, W.pv, c, wf, W.qu, W.qv, nullptr, nullptr, red(s, FIN_ALPHA), 1, s->S);
+                NK_LAUNCH_TOPO_EV(g, k_apply, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, g, (const T *)W.pu, (const T *)W.pv, c, wf, W.qu, W.qv,
+                                  (const T *)nullptr, (const T *)nullptr, red(s, FIN_ALPHA), 1, (const Scal *)s->S);
             }
             CHK(launch_check(s));
             CHK(finish_red(s, B, 1, FIN_ALPHA, s->S));
             if (pre == NK_PRECONDITIONER_MULTIGRID) {
                 {
                     ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
-                    k_update_xr<T><<<B, THREADS, 0, s->stream>>>(g, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, red(s, FIN_RES), s->S);
+                    hipExtLaunchKernelGGL(k_update_xr<T>, B, THREADS, 0, s->stream, ps.e0, ps.e1, 0, g, xu, xv, (const T *)W.pu,
+                                          (const T *)W.pv, (const T *)W.qu, (const T *)W.qv, W.ru, W.rv, red(s, FIN_RES), (const Scal *)s->S);
                 }
                 CHK(launch_check(s));
                 CHK(finish_red(s, B, 1, FIN_RES, s->S));
@@ -675,8 +715,9 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
             } else {
                 {
                     ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
-                    NK_LAUNCH_TOPO(g, k_update_diag, T, B, THREADS, 0, s->stream, g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv,
-                                                                    nullptr, nullptr, 0, red(s, FIN_BETA_RES), s->S);
+                    NK_LAUNCH_TOPO_EV(g, k_update_diag, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, g, c, wf, pre, xu, xv, (const T *)W.pu,
+                                      (const T *)W.pv, (const T *)W.qu, (const T *)W.qv, W.ru, W.rv, W.zu, W.zv, (T *)nullptr, (T *)nullptr, 0,
+                                      red(s, FIN_BETA_RES), (const Scal *)s->S);
                 }
                 CHK(launch_check(s));
                 CHK(finish_red(s, B, 2, FIN_BETA_RES, s->S));
@@ -1084,12 +1125,14 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if (options->coarse_operator == NK_COARSE_GALERKIN) {
             s->tail_start[options->precision == NK_PRECISION_MIXED ? 1 : 0] = -1;
             const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
+            const char *gb = std::getenv("NK_GTAIL_BYTES");   // tuning: LDS bytes of vectors the tail may take
+            const size_t gbudget = gb ? (size_t)std::strtoul(gb, nullptr, 10) : budget;
             size_t bytes = 0;
             int start = -1;
             for (int l = (int)s->lev.size() - 1; l >= 1; l--) {
                 const Level &L = s->lev[l];
                 const size_t b = (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1) * (8 * es + 1);
-                if (bytes + b > budget || (int)s->lev.size() - l > GTAIL_MAX) break;
+                if (bytes + b > gbudget || bytes + b > budget || (int)s->lev.size() - l > GTAIL_MAX) break;
                 bytes += b;
                 start = l;
             }
@@ -1107,6 +1150,12 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if ((st = alloc_bytes(s, (void **)&s->partials, sizeof(double) * MAXRED * (size_t)nb)) != NK_OK) return fail(st);
         if ((st = alloc_bytes(s, (void **)&s->counter, 64)) != NK_OK) return fail(st);
         if (options->coarse_operator == NK_COARSE_GALERKIN && (st = alloc_bytes(s, (void **)&s->gerr, 64)) != NK_OK) return fail(st);
+        if (options->coarse_operator == NK_COARSE_GALERKIN && s->lev.size() > 1) {
+            const KG &gc = s->lev.back().g;
+            const size_t n = 2 * (size_t)gc.Nx * (size_t)gc.Ny;
+            const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
+            if (n <= (size_t)GDENSE_MAX && (st = alloc_bytes(s, &s->cinv, n * n * es)) != NK_OK) return fail(st);
+        }
         if ((st = alloc_bytes(s, (void **)&s->S, sizeof(Scal))) != NK_OK) return fail(st);
         if (hipHostMalloc((void **)&s->S_host, sizeof(Scal), hipHostMallocDefault) != hipSuccess) { s->err = "hipHostMalloc failed"; return fail(NK_OUT_OF_MEMORY); }
     }
@@ -1148,7 +1197,11 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
     if (mixed) b += 12 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
-    if (o->coarse_operator == NK_COARSE_GALERKIN) b += 64;
+    if (o->coarse_operator == NK_COARSE_GALERKIN) {
+        b += 64;
+        const size_t n = 2 * (size_t)lev.back().Nx * (size_t)lev.back().Ny;
+        if (lev.size() > 1 && n <= (size_t)GDENSE_MAX) b += n * n * (mixed ? sizeof(float) : sizeof(double));
+    }
     return b;
 }
 
