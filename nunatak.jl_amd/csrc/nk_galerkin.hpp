@@ -423,8 +423,10 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
             sm[L.o_ru + lo] = T(0); sm[L.o_rv + lo] = T(0);
             sm[L.o_bu + lo] = l == 0 ? d.gbu[go] : T(0);
             sm[L.o_bv + lo] = l == 0 ? d.gbv[go] : T(0);
-            if (L.o_st >= 0)
+            if (L.o_st >= 0) {
+#pragma unroll 16
                 for (int k = 0; k < GS_ALLOC; k++) sm[L.o_st + k * nx * ny + lo] = L.st[k * P + go];
+            }
         }
     }
     __syncthreads();
@@ -495,15 +497,26 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         for (int p = tid; p < cg.Nx * cg.Ny; p += nth) {
             const int I = p % cg.Nx + 1, J = p / cg.Nx + 1;
             T su = T(0), sv = T(0);
-            for (int a = (C.hx ? -1 : 0); a <= (C.hx ? 1 : 0); a++)
-                for (int b = (C.hy ? -1 : 0); b <= (C.hy ? 2 : 0); b++) {
+            // fixed trip counts (as k_restrict): all reads issued together, off-support terms masked
+#pragma unroll
+            for (int a = -1; a <= 1; a++)
+#pragma unroll
+                for (int b = -1; b <= 2; b++) {
+                    const bool on = (C.hx || a == 0) && (C.hy || b == 0);
                     const int i = r_index(I, a, g.Nx, g.tx, 1, C.hx), j = r_index(J, b, g.Ny, g.ty, 0, C.hy);
-                    su += r_weight<T>(I, a, cg.Nx, g.tx, 1, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 0, C.hy) * iu(g, fact, ru, i, j);
+                    const T w = r_weight<T>(I, a, cg.Nx, g.tx, 1, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 0, C.hy);
+                    const T val = iu(g, fact, ru, i, j);
+                    if (on) su += w * val;
                 }
-            for (int a = (C.hx ? -1 : 0); a <= (C.hx ? 2 : 0); a++)
-                for (int b = (C.hy ? -1 : 0); b <= (C.hy ? 1 : 0); b++) {
+#pragma unroll
+            for (int a = -1; a <= 2; a++)
+#pragma unroll
+                for (int b = -1; b <= 1; b++) {
+                    const bool on = (C.hx || a == 0) && (C.hy || b == 0);
                     const int i = r_index(I, a, g.Nx, g.tx, 0, C.hx), j = r_index(J, b, g.Ny, g.ty, 1, C.hy);
-                    sv += r_weight<T>(I, a, cg.Nx, g.tx, 0, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 1, C.hy) * iv(g, fact, rv, i, j);
+                    const T w = r_weight<T>(I, a, cg.Nx, g.tx, 0, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 1, C.hy);
+                    const T val = iv(g, fact, rv, i, j);
+                    if (on) sv += w * val;
                 }
             const long o = off(cg, I, J);
             cbu[o] = (cact[o] & 1) ? su : T(0);
