@@ -6,6 +6,7 @@ arrival times from one submitting thread while another collects results, so requ
 are still on the device, and reports per-request latency (finish − arrival) and throughput.
 """
 import ctypes as ct
+import sys
 import threading
 import time
 
@@ -128,7 +129,7 @@ def serve(requests, slots=4, hbm_budget_bytes=0, keep_results=False, progress=No
                 results[k] = (u, v)
             if progress and (k + 1) % progress == 0:
                 lat = [r["latency_ms"] for r in records[: k + 1]]
-                print(f"[serve] {k + 1}/{len(requests)} done, p50 {np.percentile(lat, 50):.1f} ms", flush=True)
+                print(f"[serve] {k + 1}/{len(requests)} done, p50 {np.percentile(lat, 50):.1f} ms", file=sys.stderr, flush=True)
         th.join()
     if errors:
         raise errors[0]
