@@ -314,6 +314,15 @@ __device__ __forceinline__ void xcd_tile(int &bx, int &by) {
     by = t / gridDim.x;
 }
 
+// Two vertically adjacent points per thread (rows j0, j0 + 1) for the finest-level smoother: the stencil rows the
+// two points share are loaded once, and half as many waves cover the grid. Grid: blocks_for_rows(Nx, Ny, SROWS).
+constexpr int SROWS = 2;
+#define NK_IJ2                                                \
+    int bx_, by_;                                             \
+    xcd_tile(bx_, by_);                                       \
+    const int i = bx_ * BX + threadIdx.x + 1;                 \
+    const int j0 = by_ * (BY * SROWS) + threadIdx.y * SROWS + 1;
+
 #define NK_IJ                                                 \
     int bx_, by_;                                             \
     xcd_tile(bx_, by_);                                       \
@@ -480,23 +489,27 @@ __global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, co
                                               T *__restrict__ ov, int from_zero, Red red, int nred, const Scal *gate) {
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
-    NK_IJ
+    NK_IJ2
     double acc[1] = {0.0};
-    if (i <= g.Nx && j <= g.Ny) {
-        const long o = off(g, i, j);
-        T qa, qb, x_u, x_v;
-        if (from_zero) {
-            qa = T(0); qb = T(0); x_u = T(0); x_v = T(0);
-        } else {
-            ssa_apply(g, i, j, xu, xv, c, wf, qa, qb);
-            x_u = xu[o];
-            x_v = xv[o];
+#pragma unroll
+    for (int r = 0; r < SROWS; r++) {
+        const int j = j0 + r;
+        if (i <= g.Nx && j <= g.Ny) {
+            const long o = off(g, i, j);
+            T qa, qb, x_u, x_v;
+            if (from_zero) {
+                qa = T(0); qb = T(0); x_u = T(0); x_v = T(0);
+            } else {
+                ssa_apply(g, i, j, xu, xv, c, wf, qa, qb);
+                x_u = xu[o];
+                x_v = xv[o];
+            }
+            const T nu = x_u + (bu[o] - qa) / x_l1(g, c, i, j, wf);
+            const T nv = x_v + (bv[o] - qb) / y_l1(g, c, i, j, wf);
+            ou[o] = nu;
+            ov[o] = nv;
+            if (nred) acc[0] += (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
         }
-        const T nu = x_u + (bu[o] - qa) / x_l1(g, c, i, j, wf);
-        const T nv = x_v + (bv[o] - qb) / y_l1(g, c, i, j, wf);
-        ou[o] = nu;
-        ov[o] = nv;
-        if (nred) acc[0] = (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
     }
     if (nred) reduce_finish<1>(acc, red);
 }
@@ -507,15 +520,19 @@ __global__ __launch_bounds__(NT) void k_sweep_zero(KG g, Coef<T> c, T wf, const 
                                                    T *__restrict__ ou, T *__restrict__ ov, Red red, int nred, const Scal *gate) {
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
-    NK_IJ
+    NK_IJ2
     double acc[1] = {0.0};
-    if (i <= g.Nx && j <= g.Ny) {
-        const long o = off(g, i, j);
-        const T nu = T(0) + (bu[o] - T(0)) / x_l1(g, c, i, j, wf);
-        const T nv = T(0) + (bv[o] - T(0)) / y_l1(g, c, i, j, wf);
-        ou[o] = nu;
-        ov[o] = nv;
-        if (nred) acc[0] = (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
+#pragma unroll
+    for (int r = 0; r < SROWS; r++) {
+        const int j = j0 + r;
+        if (i <= g.Nx && j <= g.Ny) {
+            const long o = off(g, i, j);
+            const T nu = T(0) + (bu[o] - T(0)) / x_l1(g, c, i, j, wf);
+            const T nv = T(0) + (bv[o] - T(0)) / y_l1(g, c, i, j, wf);
+            ou[o] = nu;
+            ov[o] = nv;
+            if (nred) acc[0] += (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
+        }
     }
     if (nred) reduce_finish<1>(acc, red);
 }

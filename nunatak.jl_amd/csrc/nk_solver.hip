@@ -132,6 +132,7 @@ KG make_kg(int Nx, int Ny, int tx, int ty, double dx, double dy) {
 long plane_of(const KG &g) { return (long)g.pitch * (long)(g.Ny + 1); }
 
 dim3 blocks_for(int ex, int ey) { return dim3((ex + BX - 1) / BX, (ey + BY - 1) / BY); }
+dim3 blocks_for_rows(int ex, int ey, int rows) { return dim3((ex + BX - 1) / BX, (ey + BY * rows - 1) / (BY * rows)); }
 const dim3 THREADS(BX, BY);
 
 // Launch a stencil kernel instantiated for the grid's topology (Bounded = 1 / Periodic = 0 along x and y), so the
@@ -372,18 +373,19 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
         return NK_OK;
     }
     Pair<T> cur = x0, oth = x1;
+    const dim3 B2 = blocks_for_rows(L.g.Nx, L.g.Ny, SROWS);
     for (int k = 0; k < sweeps; k++) {
         const bool last = (k == sweeps - 1) && final_dot;
         if (k == 0) {
-            NK_LAUNCH_TOPO(L.g, k_sweep_zero, T, B, THREADS, 0, s->stream, L.g, c, wf, bu, bv, cur.u, cur.v, red(s, FIN_BETA),
+            NK_LAUNCH_TOPO(L.g, k_sweep_zero, T, B2, THREADS, 0, s->stream, L.g, c, wf, bu, bv, cur.u, cur.v, red(s, FIN_BETA),
                            last ? 1 : 0, gate);
         } else {
             ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-            NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu,
+            NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu,
                               bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
         }
         CHK(launch_check(s));
-        if (last) CHK(finish_red(s, B, 1, FIN_BETA, gate));
+        if (last) CHK(finish_red(s, B2, 1, FIN_BETA, gate));
         if (k > 0) std::swap(cur, oth);
     }
     *out = cur;
@@ -408,15 +410,16 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
             CHK(launch_check(s));
         }
     } else {
+        const dim3 B2 = blocks_for_rows(L.g.Nx, L.g.Ny, SROWS);
         for (int k = 0; k < sweeps; k++) {
             const bool last = (k == sweeps - 1) && final_dot;
             {
                 ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-                NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf,
+                NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf,
                                   bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
             }
             CHK(launch_check(s));
-            if (last) CHK(finish_red(s, B, 1, FIN_BETA, gate));
+            if (last) CHK(finish_red(s, B2, 1, FIN_BETA, gate));
             std::swap(cur, oth);
         }
         if (final_dot && sweeps > 0) { *out = cur; return NK_OK; }
