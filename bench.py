@@ -158,6 +158,9 @@ def main():
     ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
     ap.add_argument("--graphs", type=int, default=1, help="0: launch kernel by kernel (for external profilers)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--stream-curve", action="store_true",
+                    help="device-to-device copy bandwidth from 16 MB to 4 GB (the 256 MB Infinity Cache shows in it); "
+                         "prints one JSON line and exits")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: one device copy of 1 GiB (torch) as the counter calibration, then one "
                          "Picard iteration of the mixed solve launched kernel by kernel; prints nothing else")
@@ -174,6 +177,29 @@ def main():
         sys.exit(2)
     N, K, Wm = args.size, args.steps, args.warmup
     os.environ.setdefault("NK_VERBOSE", "1")  # one library progress line per Picard iteration
+    if args.stream_curve:
+        import torch
+        curve = []
+        for mb in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096):
+            n = mb * (1 << 20) // 4
+            a = torch.ones(n, dtype=torch.float32, device="cuda")
+            b = torch.empty_like(a)
+            for _ in range(3):
+                b.copy_(a)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = max(5, min(200, 4096 // mb))
+            e0.record()
+            for _ in range(reps):
+                b.copy_(a)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            gbs = 2 * mb * (1 << 20) / (ms * 1e-3) / 1e9
+            curve.append({"MB": mb, "ms": ms, "GB_s": gbs})
+            log(f"[bench] copy {mb:5d} MB: {ms:8.3f} ms, {gbs:7.0f} GB/s (read + write)")
+            del a, b
+        print(json.dumps({"stream_copy_curve": curve}), flush=True)
+        return
     if args.traffic_probe:
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB

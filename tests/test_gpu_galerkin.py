@@ -96,3 +96,22 @@ def test_mixed_galerkin_matches_fp64():
     e = rel(mmx, m64)
     say(f"[galerkin] mixed+Galerkin vs FP64: {e:.2e}; CG {i64['total_linear_iterations']} → {imx['total_linear_iterations']}")
     assert e < 1e-8
+
+
+def test_mixed_galerkin_on_the_marine_sheet_matches_fp64():
+    """Config 2 (640 × 80, grounded + floating, Weertman, semi-coarsened hierarchy): the mixed form with Galerkin
+    coarse operators against the reference algorithm in FP64, full Picard solve. Bar: 1e-8 relative L2."""
+    out = {}
+    for label, prec, coarse in (("fp64", abi.PRECISION_FP64, abi.COARSE_REDISCRETIZED),
+                                ("mixed", abi.PRECISION_MIXED, abi.COARSE_GALERKIN)):
+        g, phys, opts, fields = W.marine_sheet()
+        opts.precision, opts.coarse_operator = prec, coarse
+        m = HIPModel(g, phys, opts)
+        W.load_into(m, fields)
+        info = m.update_velocities(geometry=True)
+        out[label] = (m, info)
+    e = rel(out["mixed"][0], out["fp64"][0])
+    say(f"[galerkin] marine sheet: Picard {out['fp64'][1]['picard_iterations']} / {out['mixed'][1]['picard_iterations']}, "
+        f"CG {out['fp64'][1]['total_linear_iterations']} → {out['mixed'][1]['total_linear_iterations']}, rel {e:.2e}")
+    assert out["mixed"][1]["picard_iterations"] == out["fp64"][1]["picard_iterations"]
+    assert e < 1e-8

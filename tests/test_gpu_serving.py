@@ -86,3 +86,25 @@ def test_request_larger_than_budget_fails_cleanly():
         rid = srv.submit(g, p, o, f)
         with pytest.raises(NunatakError, match="OUT_OF_MEMORY"):
             srv.wait(rid)
+
+
+@pytest.mark.parametrize("kind", ["diva", "fp64"])
+def test_other_request_kinds_same_alone_and_in_the_mix(kind):
+    """DIVA requests (config 3's generator, 3D Arrhenius A from temperature) and FP64 requests through the server."""
+    reqs = []
+    for k, n in enumerate([64, 80, 64, 96]):
+        if kind == "diva":
+            g, p, o, f = W.diva_shelf(N=n, dx=4e3, seed=300 + k)
+        else:
+            g, p, o, f = W.serving_request(n, 3000 + k, precision=abi.PRECISION_FP64)
+        reqs.append((g, p, o, f))
+    ref = []
+    for g, p, o, f in reqs:
+        m = HIPModel(g, p, o)
+        W.load_into(m, f)
+        m.update_velocities(geometry=True)
+        ref.append((m.get("U"), m.get("V")))
+        m.close()
+    out = serve([(0.0, *r) for r in reqs], slots=3, keep_results=True)
+    for k, (u, v) in enumerate(out["results"]):
+        assert np.array_equal(u, ref[k][0]) and np.array_equal(v, ref[k][1]), f"request {k} differs"
