@@ -268,7 +268,8 @@ nk_status nk_rate_factor_from_temperature(nk_solver *solver);
  * cost gradient x̄ in (UBAR, VBAR). Iterates λ = A⁻¹(x̄ + (∂q/∂x)ᵀ(−λ)) with q(x) = A(c(x)) x* until
  * ‖Δλ‖ ≤ max(reltol‖λ‖, abstol) under `control` (NULL: the Picard control, as the reference defaults), each linear
  * solve being the handle's PCG. Outputs: λ in (LAMBDAU, LAMBDAV) — the gradient with respect to the driving stress —
- * and the parameter gradients ∂J/∂C (CBAR) and ∂J/∂A (ABAR). SSA only (NK_UNSUPPORTED for DIVA). */
+ * and the parameter gradients ∂J/∂C (CBAR) and ∂J/∂A (ABAR). DIVA carries the lagged state (βᵇ, F₂, ησ) through the
+ * fixed point as the reference rule does (its shadows start at zero: the cost reads the depth-averaged velocities). */
 nk_status nk_adjoint(nk_solver *solver, const nk_control *control, nk_info *info);
 
 /* Multigrid level shapes of the handle (StressBalanceMultigridPreconditioner(grid).levels). Writes up to

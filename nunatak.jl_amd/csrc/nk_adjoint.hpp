@@ -129,13 +129,15 @@ __device__ __forceinline__ void drag_derivatives(const PhysK &p, double U, doubl
     dC = pow(U, m - 1.0) * tc * Dm1 * pow(tc, 1.0 / m);
 }
 
-// Faces: β̄ = Az x* q̄ on active rows; β = φ·drag(U(u, v), ℑx C, ℑx N) → velocity adjoint and, in the parameter pass, C̄.
+// Faces: β̄ = Az x* q̄ on active rows (or the given β̄ — the DIVA chain passes βᵇ̄); β = φ·drag(U(u, v), ℑx C, ℑx N)
+// → velocity adjoint and, in the parameter pass, C̄.
 __global__ __launch_bounds__(NT) void k_adj_drag(KG g, PhysK p, const uint8_t *__restrict__ act, const double *__restrict__ xu,
                                                  const double *__restrict__ xv, const double *__restrict__ qbu,
                                                  const double *__restrict__ qbv, const double *__restrict__ C,
                                                  const double *__restrict__ N, const double *__restrict__ phiu,
                                                  const double *__restrict__ phiv, double *__restrict__ yu, double *__restrict__ yv,
-                                                 double *__restrict__ Cb, int velocity, int parameters) {
+                                                 double *__restrict__ Cb, int velocity, int parameters,
+                                                 const double *__restrict__ bbar_u, const double *__restrict__ bbar_v) {
     NK_IJ
     if (i > g.Nx + 1 || j > g.Ny + 1) return;
     if (p.friction == 0) return;
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_adj_drag(KG g, PhysK p, const uint8_t *_
     const bool useN = p.friction == 2;
     const double u0 = p.u0;
     if (i <= nxf(g) && j <= g.Ny && !periph_fc(g, i, j)) {
-        const double bb = qbar_u(g, act, qbu, i, j) * Az * xu[o];
+        const double bb = bbar_u ? bbar_u[o] : qbar_u(g, act, qbu, i, j) * Az * xu[o];
         if (bb != 0.0) {
             const double vm = ((ldcf(g, xv, i - 1, j) + ldcf(g, xv, i, j)) / 2.0 + (ldcf(g, xv, i - 1, j + 1) + ldcf(g, xv, i, j + 1)) / 2.0) / 2.0;
             const double uu = xu[o];
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(NT) void k_adj_drag(KG g, PhysK p, const uint8_t *_
         }
     }
     if (i <= g.Nx && j <= nyf(g) && !periph_cf(g, i, j)) {
-        const double bb = qbar_v(g, act, qbv, i, j) * Az * xv[o];
+        const double bb = bbar_v ? bbar_v[o] : qbar_v(g, act, qbv, i, j) * Az * xv[o];
         if (bb != 0.0) {
             const double um = ((ldfc(g, xu, i, j - 1) + ldfc(g, xu, i + 1, j - 1)) / 2.0 + (ldfc(g, xu, i, j) + ldfc(g, xu, i + 1, j)) / 2.0) / 2.0;
             const double vv = xv[o];
@@ -197,6 +199,178 @@ __global__ __launch_bounds__(NT) void k_adj_drag(KG g, PhysK p, const uint8_t *_
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// DIVA (update_diva_coefficients!, src/StressBalances/diva.jl:60-75). One step reads the lagged state
+// s = (βᵇ, F₂, ησ) and writes its successor: uᵇ = u/(1 + βᵇ ℑx F₂) → βᵇ′ = φ·drag(uᵇ) → ησ′(u, βᵇ′, uᵇ, ησ) →
+// η̄H, F₂′ → β = βᵇ′/(1 + βᵇ′ ℑx F₂′). The kernels below run that chain backwards; the shadows of s′ come in, those
+// of s go out (the fixed-point sweep of the reference rule).
+// ---------------------------------------------------------------------------------------------------------------
+
+// Faces: β̄ = q̄·Az·x* (operator rows) through β = βᵇ/(1 + βᵇ F̄ˣ): βᵇ̄ += β̄/den², F₂̄ += −β̄ βᵇ²/den² / 2 to the two
+// centres (diva.jl:35-51).
+__global__ __launch_bounds__(NT) void k_adj_effective_drag(KG g, const uint8_t *__restrict__ act, const double *__restrict__ xu,
+                                                           const double *__restrict__ xv, const double *__restrict__ qbu,
+                                                           const double *__restrict__ qbv, const double *__restrict__ bbu,
+                                                           const double *__restrict__ bbv, const double *__restrict__ F2,
+                                                           double *__restrict__ bbu_b, double *__restrict__ bbv_b,
+                                                           double *__restrict__ F2_b) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    const double Az = g.dx * g.dy;
+    if (i <= nxf(g) && j <= g.Ny) {
+        const double bb = qbar_u(g, act, qbu, i, j) * Az * xu[o];
+        if (bb != 0.0) {
+            const double b = bbu[o];
+            const double den = 1.0 + b * ((ldc(g, F2, i - 1, j) + ldc(g, F2, i, j)) / 2.0);
+            bbu_b[o] += bb / (den * den);
+            const double fb = -bb * b * b / (den * den) / 2.0;
+            aadd(&F2_b[off(g, cx(g, i - 1), cy(g, j))], fb);
+            aadd(&F2_b[off(g, cx(g, i), cy(g, j))], fb);
+        }
+    }
+    if (i <= g.Nx && j <= nyf(g)) {
+        const double bb = qbar_v(g, act, qbv, i, j) * Az * xv[o];
+        if (bb != 0.0) {
+            const double b = bbv[o];
+            const double den = 1.0 + b * ((ldc(g, F2, i, j - 1) + ldc(g, F2, i, j)) / 2.0);
+            bbv_b[o] += bb / (den * den);
+            const double fb = -bb * b * b / (den * den) / 2.0;
+            aadd(&F2_b[off(g, cx(g, i), cy(g, j - 1))], fb);
+            aadd(&F2_b[off(g, cx(g, i), cy(g, j))], fb);
+        }
+    }
+}
+
+// Centres: DIVA viscosity and column integrals backwards (glen_flow_law.jl:122-143, column_integrals.jl). Inputs:
+// η̄H̄ (etab), F̄₂′ (F2_b), ησ̄′ (es_b, σ layers). Outputs: strain-rate adjoints onto u, v and the corners (as the SSA
+// kernel), βᵇ̄ and uᵇ̄ of the faces i, i + 1 (j, j + 1) through the basal shear, ησ̄ of the lagged layers (es_b_old),
+// and Ā in the parameter pass.
+__global__ __launch_bounds__(NT) void k_adj_viscosity_diva(KG g, PhysK p, Sig sg, const double *__restrict__ xu,
+                                                           const double *__restrict__ xv, const double *__restrict__ es, long plane,
+                                                           const double *__restrict__ A, const double *__restrict__ H,
+                                                           const double *__restrict__ mu, const double *__restrict__ bbu,
+                                                           const double *__restrict__ bbv, const double *__restrict__ ub,
+                                                           const double *__restrict__ vb, const double *__restrict__ etab,
+                                                           const double *__restrict__ F2_b, const double *__restrict__ es_b,
+                                                           double *__restrict__ yu, double *__restrict__ yv, double *__restrict__ exyb,
+                                                           double *__restrict__ bbu_b, double *__restrict__ bbv_b,
+                                                           double *__restrict__ ub_b, double *__restrict__ vb_b,
+                                                           double *__restrict__ es_b_old, double *__restrict__ Ab, int velocity,
+                                                           int parameters) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    const double Hm = H[o] * (mu ? mu[o] : 1.0);
+    const double eb = etab[o], fb = F2_b[o];
+    // forward quantities at the fixed point
+    const double exx = (iu_w(g, xu, i + 1, j) - iu_w(g, xu, i, j)) / g.dx;
+    const double eyy = (iv_w(g, xv, i, j + 1) - iv_w(g, xv, i, j)) / g.dy;
+    auto exy = [&](int a, int c) {
+        double e = ((iu_w(g, xu, a, c) - iu_w(g, xu, a, c - 1)) / g.dy + (iv_w(g, xv, a, c) - iv_w(g, xv, a - 1, c)) / g.dx) / 2.0;
+        return periph_ff(g, a, c) ? p.wf * e : e;
+    };
+    const double e00 = exy(i, j), e10 = exy(i + 1, j), e01 = exy(i, j + 1), e11 = exy(i + 1, j + 1);
+    const double e2m = exx * exx + eyy * eyy + exx * eyy + ((e00 * e00 + e10 * e10) / 2.0 + (e01 * e01 + e11 * e11) / 2.0) / 2.0;
+    const double tx = (ldfc(g, bbu, i, j) * ldfc(g, ub, i, j) + ldfc(g, bbu, i + 1, j) * ldfc(g, ub, i + 1, j)) / 2.0;
+    const double ty = (ldcf(g, bbv, i, j) * ldcf(g, vb, i, j) + ldcf(g, bbv, i, j + 1) * ldcf(g, vb, i, j + 1)) / 2.0;
+    double e2m_b = 0.0, tx_b = 0.0, ty_b = 0.0;
+    for (int k = 0; k < p.Ns; k++) {
+        const long ok = o + (long)k * plane;
+        const double zeta = sg.zc[k];
+        const double es0 = es[ok];
+        const double em = es0 == 0.0 ? INFINITY : es0;
+        const double exz = tx * (1.0 - zeta) / (2.0 * em), eyz = ty * (1.0 - zeta) / (2.0 * em);
+        const double E = e2m + exz * exz + eyz * eyz + p.e0 * p.e0;
+        const double pa = pow(A[p.A3d ? ok : o], p.inv_n);
+        const double ek = pa * pow(E, p.exp_e) / 2.0;
+        // ek feeds ησ′, η̄H = H μ Σ ek dz and F₂′ = H Σ w₂/ek
+        const double ekb = es_b[ok] + eb * Hm * sg.dz[k] - fb * H[o] * sg.w2[k] / (ek * ek);
+        if (parameters && Ab) {
+            const double ab = ekb * ek * p.inv_n / A[p.A3d ? ok : o];
+            if (p.A3d) Ab[ok] += ab;
+            else Ab[o] += ab;
+        }
+        if (!velocity) continue;
+        const double Eb = ekb * ek * p.exp_e / E;
+        e2m_b += Eb;
+        const double exzb = Eb * 2.0 * exz, eyzb = Eb * 2.0 * eyz;
+        tx_b += exzb * (1.0 - zeta) / (2.0 * em);
+        ty_b += eyzb * (1.0 - zeta) / (2.0 * em);
+        // lagged layer viscosity: ∂exz/∂em = −exz/em (no dependence where the lag was still 0)
+        es_b_old[ok] = es0 == 0.0 ? 0.0 : -(exzb * exz + eyzb * eyz) / em;
+    }
+    if (!velocity) return;
+    // strain rates (as k_adj_viscosity with Ē = ∂J/∂ε̇ₑ²)
+    const double exxb = e2m_b * (2.0 * exx + eyy), eyyb = e2m_b * (2.0 * eyy + exx);
+    if (!periph_fc(g, i + 1, j)) aadd(&yu[off(g, fx(g, i + 1), cy(g, j))], exxb / g.dx);
+    if (!periph_fc(g, i, j)) aadd(&yu[off(g, fx(g, i), cy(g, j))], -exxb / g.dx);
+    if (!periph_cf(g, i, j + 1)) aadd(&yv[off(g, cx(g, i), fy(g, j + 1))], eyyb / g.dy);
+    if (!periph_cf(g, i, j)) aadd(&yv[off(g, cx(g, i), fy(g, j))], -eyyb / g.dy);
+    aadd(&exyb[off(g, fx(g, i), fy(g, j))], e2m_b * e00 / 2.0);
+    aadd(&exyb[off(g, fx(g, i + 1), fy(g, j))], e2m_b * e10 / 2.0);
+    aadd(&exyb[off(g, fx(g, i), fy(g, j + 1))], e2m_b * e01 / 2.0);
+    aadd(&exyb[off(g, fx(g, i + 1), fy(g, j + 1))], e2m_b * e11 / 2.0);
+    // basal shear: tx = (βᵇ(i) uᵇ(i) + βᵇ(i+1) uᵇ(i+1)) / 2
+    const long ou0 = off(g, fx(g, i), cy(g, j)), ou1 = off(g, fx(g, i + 1), cy(g, j));
+    const long ov0 = off(g, cx(g, i), fy(g, j)), ov1 = off(g, cx(g, i), fy(g, j + 1));
+    aadd(&bbu_b[ou0], tx_b * ub[ou0] / 2.0);
+    aadd(&bbu_b[ou1], tx_b * ub[ou1] / 2.0);
+    aadd(&ub_b[ou0], tx_b * bbu[ou0] / 2.0);
+    aadd(&ub_b[ou1], tx_b * bbu[ou1] / 2.0);
+    aadd(&bbv_b[ov0], ty_b * vb[ov0] / 2.0);
+    aadd(&bbv_b[ov1], ty_b * vb[ov1] / 2.0);
+    aadd(&vb_b[ov0], ty_b * bbv[ov0] / 2.0);
+    aadd(&vb_b[ov1], ty_b * bbv[ov1] / 2.0);
+}
+
+// Faces: uᵇ = u / (1 + βᵇ ℑx F₂) backwards (diva.jl:1-27): ū += uᵇ̄/D; the lagged βᵇ and F₂ get −uᵇ̄ uᵇ/D times
+// the other factor (F₂ halved onto the two centres).
+__global__ __launch_bounds__(NT) void k_adj_basal_velocity(KG g, const double *__restrict__ ub, const double *__restrict__ vb,
+                                                           const double *__restrict__ bbu, const double *__restrict__ bbv,
+                                                           const double *__restrict__ F2, const double *__restrict__ ub_b,
+                                                           const double *__restrict__ vb_b, double *__restrict__ yu,
+                                                           double *__restrict__ yv, double *__restrict__ bbu_b_old,
+                                                           double *__restrict__ bbv_b_old, double *__restrict__ F2_b_old) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    if (i <= nxf(g) && j <= g.Ny) {
+        const double a = ub_b[o];
+        if (a != 0.0) {
+            const double F2x = (ldc(g, F2, i - 1, j) + ldc(g, F2, i, j)) / 2.0;
+            const double D = 1.0 + bbu[o] * F2x;
+            aadd(&yu[o], a / D);
+            const double Db = -a * ub[o] / D;
+            bbu_b_old[o] += Db * F2x;
+            const double fb = Db * bbu[o] / 2.0;
+            aadd(&F2_b_old[off(g, cx(g, i - 1), cy(g, j))], fb);
+            aadd(&F2_b_old[off(g, cx(g, i), cy(g, j))], fb);
+        }
+    }
+    if (i <= g.Nx && j <= nyf(g)) {
+        const double a = vb_b[o];
+        if (a != 0.0) {
+            const double F2y = (ldc(g, F2, i, j - 1) + ldc(g, F2, i, j)) / 2.0;
+            const double D = 1.0 + bbv[o] * F2y;
+            aadd(&yv[o], a / D);
+            const double Db = -a * vb[o] / D;
+            bbv_b_old[o] += Db * F2y;
+            const double fb = Db * bbv[o] / 2.0;
+            aadd(&F2_b_old[off(g, cx(g, i), cy(g, j - 1))], fb);
+            aadd(&F2_b_old[off(g, cx(g, i), cy(g, j))], fb);
+        }
+    }
+}
+
+// one plane over the (Nx + 1) × (Ny + 1) footprint
+__global__ __launch_bounds__(NT) void k_copy_plane(KG g, const double *__restrict__ a, double *__restrict__ b) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    b[o] = a[o];
 }
 
 // q̄ = −λ over :xyz (zero elsewhere is kept by the caller's fills)

@@ -77,6 +77,11 @@ struct nk_solver {
     // adjoint work planes: q̄, ȳ, λ⁺, r (u and v), η̄H̄ (centres), ε̇xy adjoint (corners)
     double *ad_qu = nullptr, *ad_qv = nullptr, *ad_yu = nullptr, *ad_yv = nullptr, *ad_lu = nullptr, *ad_lv = nullptr,
            *ad_ru = nullptr, *ad_rv = nullptr, *ad_eta = nullptr, *ad_exy = nullptr;
+    // DIVA adjoint: βᵇ̄ (total), uᵇ̄, F̄₂ of this step's state; the sweep state (st_*) and the step's outputs for the
+    // lagged state (out_*), swapped after each fixed-point iteration; ησ̄ planes (Nσ layers)
+    double *ad_bbu = nullptr, *ad_bbv = nullptr, *ad_ubb = nullptr, *ad_vbb = nullptr, *ad_F2n = nullptr;
+    double *st_bbu = nullptr, *st_bbv = nullptr, *st_F2 = nullptr, *out_bbu = nullptr, *out_bbv = nullptr, *out_F2 = nullptr;
+    double *st_es = nullptr, *out_es = nullptr;
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
     unsigned int *counter = nullptr;
@@ -1091,8 +1096,12 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
     {
         double **w[] = {&s->w64.ru, &s->w64.rv, &s->w64.zu, &s->w64.zv, &s->w64.z2u, &s->w64.z2v, &s->w64.pu, &s->w64.pv,
                         &s->w64.qu, &s->w64.qv, &s->w64.du, &s->w64.dv, &s->xmu, &s->xmv, &s->ad_qu, &s->ad_qv,
-                        &s->ad_yu, &s->ad_yv, &s->ad_lu, &s->ad_lv, &s->ad_ru, &s->ad_rv, &s->ad_eta, &s->ad_exy};
+                        &s->ad_yu, &s->ad_yv, &s->ad_lu, &s->ad_lv, &s->ad_ru, &s->ad_rv, &s->ad_eta, &s->ad_exy,
+                        &s->ad_bbu, &s->ad_bbv, &s->ad_ubb, &s->ad_vbb, &s->ad_F2n, &s->st_bbu, &s->st_bbv, &s->st_F2,
+                        &s->out_bbu, &s->out_bbv, &s->out_F2};
         for (auto p : w) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
+        if ((st = alloc_plane(s, &s->st_es, s->g, s->Ns)) != NK_OK) return fail(st);
+        if ((st = alloc_plane(s, &s->out_es, s->g, s->Ns)) != NK_OK) return fail(st);
         if (options->precision == NK_PRECISION_MIXED) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
                              &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv};
@@ -1196,7 +1205,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
         if (mixed) b += (4 + 9) * sizeof(float) * p;
         if (o->coarse_operator == NK_COARSE_GALERKIN) b += GS_ALLOC * (mixed ? sizeof(float) : sizeof(double)) * p;
     }
-    b += 24 * sizeof(double) * p0;
+    b += (24 + 11) * sizeof(double) * p0 + 2 * Ns * sizeof(double) * p0;
     if (mixed) b += 12 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
@@ -1505,7 +1514,54 @@ nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters) {
     }
     k_adj_drag<<<Be, THREADS, 0, s->stream>>>(g, p, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, f[NK_FIELD_C], f[NK_FIELD_N],
                                                f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_yu, s->ad_yv, f[NK_FIELD_CBAR],
-                                               velocity ? 1 : 0, parameters ? 1 : 0);
+                                               velocity ? 1 : 0, parameters ? 1 : 0, nullptr, nullptr);
+    return launch_check(s);
+}
+
+// One transposed DIVA coefficient step (update_diva_coefficients!, src/StressBalances/diva.jl:60-75), q̄ in ad_q and the
+// shadows of the step's state (βᵇ′, F₂′, ησ′) in st_*: ȳ in ad_y, the lagged state's shadows in out_*, and C̄, Ā in the
+// parameter pass. The whole chain runs in both passes (C̄ needs βᵇ̄ from the basal-shear terms of the viscosity).
+nk_status adjoint_step_diva(nk_solver *s, bool parameters) {
+    const KG &g = s->g;
+    const dim3 Bc = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    double **f = s->f;
+    const PhysK p = physk(s);
+    const uint8_t *act = s->lev[0].act;
+    auto fill = [&](double *a, double *b) { k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, a, b, 0.0, nullptr); return launch_check(s); };
+    CHK(fill(s->ad_eta, s->ad_exy));
+    CHK(fill(s->ad_yu, s->ad_yv));
+    CHK(fill(s->ad_ubb, s->ad_vbb));
+    CHK(fill(s->out_bbu, s->out_bbv));
+    CHK(fill(s->out_F2, nullptr));
+    k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->st_bbu, s->ad_bbu);
+    CHK(launch_check(s));
+    k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->st_bbv, s->ad_bbv);
+    CHK(launch_check(s));
+    k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->st_F2, s->ad_F2n);
+    CHK(launch_check(s));
+    k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    CHK(launch_check(s));
+    k_adj_corners<<<Be, THREADS, 0, s->stream>>>(g, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    CHK(launch_check(s));
+    k_adj_effective_drag<<<Be, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, f[NK_FIELD_BETABU],
+                                                         f[NK_FIELD_BETABV], f[NK_FIELD_F2], s->ad_bbu, s->ad_bbv, s->ad_F2n);
+    CHK(launch_check(s));
+    k_adj_viscosity_diva<<<Bc, THREADS, 0, s->stream>>>(g, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETASIGMA], s->plane,
+                                                         f[NK_FIELD_A], f[NK_FIELD_H], s->ph.masking ? f[NK_FIELD_MU] : nullptr,
+                                                         f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_UB], f[NK_FIELD_VB],
+                                                         s->ad_eta, s->ad_F2n, s->st_es, s->ad_yu, s->ad_yv, s->ad_exy, s->ad_bbu,
+                                                         s->ad_bbv, s->ad_ubb, s->ad_vbb, s->out_es, f[NK_FIELD_ABAR], 1,
+                                                         parameters ? 1 : 0);
+    CHK(launch_check(s));
+    k_adj_shear<<<Be, THREADS, 0, s->stream>>>(g, p.wf, s->ad_exy, s->ad_yu, s->ad_yv);
+    CHK(launch_check(s));
+    k_adj_drag<<<Be, THREADS, 0, s->stream>>>(g, p, act, f[NK_FIELD_UB], f[NK_FIELD_VB], s->ad_qu, s->ad_qv, f[NK_FIELD_C], f[NK_FIELD_N],
+                                               f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_ubb, s->ad_vbb, f[NK_FIELD_CBAR], 1,
+                                               parameters ? 1 : 0, s->ad_bbu, s->ad_bbv);
+    CHK(launch_check(s));
+    k_adj_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                         f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->out_bbu,
+                                                         s->out_bbv, s->out_F2);
     return launch_check(s);
 }
 
@@ -1529,6 +1585,18 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
         k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_ABAR] + (long)k * s->plane, nullptr, 0.0, nullptr);
         CHK(launch_check(s));
     }
+    const bool diva = s->ph.stress_balance == NK_DIVA;
+    if (diva) {   // the cost reads no DIVA state directly: the sweep starts from zero shadows
+        double *z[] = {s->st_bbu, s->st_bbv, s->st_F2};
+        for (double *a : z) {
+            k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, a, nullptr, 0.0, nullptr);
+            CHK(launch_check(s));
+        }
+        for (int k = 0; k < s->Ns; k++) {
+            k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->st_es + (long)k * s->plane, s->out_es + (long)k * s->plane, 0.0, nullptr);
+            CHK(launch_check(s));
+        }
+    }
     const bool fixed = ctl.kind == NK_FIXED_ITERATIONS;
     double D = INFINITY, M = 0.0;
     int it = 0;
@@ -1538,7 +1606,15 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
         else if (!(D > jmax_h(ctl.reltol * M, ctl.abstol) && it < ctl.maxiter)) break;
         k_negate<<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_qu, s->ad_qv);   // q̄ = −λ
         CHK(launch_check(s));
-        CHK(adjoint_step(s, true, false));                                      // ȳ
+        if (diva) {
+            CHK(adjoint_step_diva(s, false));                                    // ȳ and the lagged state's shadows
+            std::swap(s->st_bbu, s->out_bbu);                                    // sweep: they become the next step's
+            std::swap(s->st_bbv, s->out_bbv);
+            std::swap(s->st_F2, s->out_F2);
+            std::swap(s->st_es, s->out_es);
+        } else {
+            CHK(adjoint_step(s, true, false));                                   // ȳ
+        }
         k_add<<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_yu, s->ad_yv, s->ad_ru, s->ad_rv);
         CHK(launch_check(s));
         k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_lu, s->ad_lv);     // λ⁺ = λ (warm start)
@@ -1566,7 +1642,8 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
     // parameter pullback with the converged λ: p̄ = (∂q/∂p)ᵀ(−λ)
     k_negate<<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_qu, s->ad_qv);
     CHK(launch_check(s));
-    CHK(adjoint_step(s, false, true));
+    if (diva) CHK(adjoint_step_diva(s, true));
+    else CHK(adjoint_step(s, false, true));
     if (info) {
         info->picard_iterations = it;
         info->total_linear_iterations = total;
@@ -1580,7 +1657,6 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
 
 nk_status nk_adjoint(nk_solver *s, const nk_control *control, nk_info *info) {
     if (!s) return NK_INVALID_ARGUMENT;
-    if (s->ph.stress_balance != NK_SSA) { s->err = "the hand-written adjoint covers the SSA coefficient update"; return NK_UNSUPPORTED; }
     Timer t(s, info);
     const nk_control ctl = control ? *control : s->op.picard;
     return t.done(adjoint(s, ctl, info));

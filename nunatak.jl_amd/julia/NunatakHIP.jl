@@ -185,7 +185,7 @@ end
 
 update_velocities!(model, solver::HIPStressBalanceSolver) = update_velocities!(model, solver, model.stress_balance)
 
-# Reverse pass of a converged SSA velocity solve (the fixed-point rule of ext/NunatakEnzymeExt.jl:175-240): given the
+# Reverse pass of a converged SSA or DIVA velocity solve (the fixed-point rule of ext/NunatakEnzymeExt.jl:175-240): given the
 # cost gradient with respect to the velocities (ū, v̄), accumulate ∂J/∂τ into (τ̄ᵘ, τ̄ᵛ), ∂J/∂C into C̄ and ∂J/∂A into
 # Ā. The handle must still hold the converged solution of the forward call.
 function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalanceSolver, ū, v̄; control = nothing)

@@ -207,11 +207,66 @@ def test_unconverged_adjoint_warns(capfd):
     assert "Picard adjoint did not converge" in err
 
 
-def test_diva_adjoint_is_rejected():
-    g = grid(16, 4, L, LY, topology=(B, P))
-    m = HIPModel(g, abi.default_physics(stress_balance=abi.DIVA), abi.default_options())
-    with pytest.raises(Exception, match="UNSUPPORTED"):
-        m.adjoint()
+def diva_problem():
+    """diva_problem of test/test_enzyme.jl:63-90: 8 × 6 (Bounded, Periodic), Nσ = 6, τ given, φ = 1, no ice mask,
+    A = 1e-21(1 + cos(2πy/Ly)/5), Weertman C = 1e6(1 + x/Lx), PCG 1e-14, Picard 1e-12."""
+    Lx, Ly, tau0 = 8e4, 6e4, 1e4
+    g = grid(8, 6, Lx, Ly, topology=(B, P), halo=4, Nsigma=6)
+    phys = abi.default_physics(stress_balance=abi.DIVA, masking=0, friction=abi.FRICTION_WEERTMAN,
+                               strain_rate_regularization=1e-16)
+    opts = abi.default_options(linear=abi.convergence(1e-14, 0.0, 10_000), picard=abi.convergence(1e-12, 0.0, 300))
+    m = HIPModel(g, phys, opts)
+    Az = (Lx / 8) * (Ly / 6)
+    m.set("H", field_values(g, "H", lambda x, y: 1000 + x / 1e2 + 100 * np.sin(2 * np.pi * y / Ly)))
+    m.set("PHIU", 1.0)
+    m.set("PHIV", 1.0)
+    m.set("TAUU", field_values(g, "TAUU", lambda x, y: tau0 * Az * np.sin(np.pi * x / Lx) * (1 + np.sin(2 * np.pi * y / Ly) / 5)))
+    m.set("TAUV", field_values(g, "TAUV", lambda x, y: tau0 * Az * np.cos(np.pi * x / Lx) / 5))
+    m.set("A", field_values(g, "A", lambda x, y: 1e-21 * (1 + np.cos(2 * np.pi * y / Ly) / 5)))
+    m.set("C", field_values(g, "C", lambda x, y: 1e6 * (1 + x / Lx)))
+    return m, g
+
+
+def diva_cost(m, g):
+    for name in ("U", "V", "ETASIGMA", "F2", "BETABU", "BETABV"):
+        m.fill(name, 0.0)
+    m.picard_solve()
+    u, v = m.get("U"), m.get("V")
+    return float(np.sum(u[: g.Ny, : g.Nx] ** 2) + np.sum(v[: g.Ny, : g.Nx] ** 2))
+
+
+def test_diva_fixed_point_adjoint_matches_finite_differences():
+    """Fixed-point adjoint of converged DIVA Picard iterations (test/test_enzyme.jl:319-350) with the cost on the
+    depth-averaged velocities: ∂J/∂A and ∂J/∂C against centred differences (h = 1e-4 of the value), bar 1e-5."""
+    m, g = diva_problem()
+    J = diva_cost(m, g)
+    u, v = m.get("U"), m.get("V")
+    ub, vb = np.zeros_like(u), np.zeros_like(v)
+    ub[: g.Ny, : g.Nx] = 2 * u[: g.Ny, : g.Nx]
+    vb[: g.Ny, : g.Nx] = 2 * v[: g.Ny, : g.Nx]
+    m.set("UBAR", ub)
+    m.set("VBAR", vb)
+    info = m.adjoint()
+    say(f"[adjoint] DIVA J = {J:.6e}; adjoint iterations {info['picard_iterations']}")
+    Cb, Ab, lam = m.get("CBAR").copy(), m.get("ABAR").copy(), m.get("LAMBDAU").copy()
+    C, A, tu = m.get("C").copy(), m.get("A").copy(), m.get("TAUU").copy()
+
+    def fd(name, arr, index, h):
+        a = arr.copy()
+        a[index] += h
+        m.set(name, a)
+        Jp = diva_cost(m, g)
+        a[index] -= 2 * h
+        m.set(name, a)
+        Jm = diva_cost(m, g)
+        m.set(name, arr)
+        return (Jp - Jm) / (2 * h)
+
+    for name, grad, arr, index in (("A", Ab, A, (1, 2)), ("A", Ab, A, (4, 5)), ("C", Cb, C, (2, 3)), ("C", Cb, C, (5, 6)),
+                                   ("TAUU", lam, tu, (2, 4))):
+        d = fd(name, arr, index, 1e-4 * abs(arr[index]))
+        say(f"[adjoint] DIVA dJ/d{name}{index}: adjoint {grad[index]:.10e}, fd {d:.10e}")
+        assert abs(grad[index] - d) < 1e-5 * abs(d)
 
 
 FULL = os.environ.get("NK_FULL_SIZE") == "1"
