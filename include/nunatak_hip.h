@@ -174,6 +174,8 @@ typedef enum nk_field_id {
     NK_FIELD_LAMBDAV,      /* ∂J/∂τᵛ                      (Center, Face)      */
     NK_FIELD_CBAR,         /* ∂J/∂C                       (Center, Center)    */
     NK_FIELD_ABAR,         /* ∂J/∂A                       (Center, Center[, σ]) like NK_FIELD_A */
+    NK_FIELD_USIGMABAR,    /* adjoint input ∂J/∂uσ (DIVA) (Face, Center, σ)   */
+    NK_FIELD_VSIGMABAR,    /* ∂J/∂vσ                      (Center, Face, σ)   */
     NK_FIELD_COUNT
 } nk_field_id;
 
@@ -258,6 +260,10 @@ nk_status nk_picard_solve(nk_solver *solver, nk_info *info);
 /* update_velocities!(model, solver, ::SSA or ::DIVA): driving stress, Picard, then extend_depth_average! (SSA) or
  * compute_velocity_profile! (DIVA). With geometry != 0 update_geometry! runs first. */
 nk_status nk_update_velocities(nk_solver *solver, int32_t geometry, nk_info *info);
+/* The post-processing of update_velocities! alone: compute_velocity_profile! (src/StressBalances/diva.jl:89-109:
+ * UB, VB and the layer velocities USIGMA, VSIGMA from U, V and the DIVA state) or extend_depth_average!
+ * (src/Models/ice_sheet_model_methods.jl:167-180) for SSA. */
+nk_status nk_compute_velocity_profile(nk_solver *solver);
 /* Arrhenius rate factor A on the σ grid from TEMPERATURE via the enthalpy round trip of set!(model; T)
  * (src/Models/ice_sheet_model_methods.jl:47-57, src/Thermodynamics/column_energy_solve.jl:105-117,
  * src/Rheologies/arrhenius_rate_factor.jl:102-110) with the reference's default parameters. */
@@ -269,7 +275,8 @@ nk_status nk_rate_factor_from_temperature(nk_solver *solver);
  * ‖Δλ‖ ≤ max(reltol‖λ‖, abstol) under `control` (NULL: the Picard control, as the reference defaults), each linear
  * solve being the handle's PCG. Outputs: λ in (LAMBDAU, LAMBDAV) — the gradient with respect to the driving stress —
  * and the parameter gradients ∂J/∂C (CBAR) and ∂J/∂A (ABAR). DIVA carries the lagged state (βᵇ, F₂, ησ) through the
- * fixed point as the reference rule does (its shadows start at zero: the cost reads the depth-averaged velocities). */
+ * fixed point as the reference rule does; a cost on the layer velocities enters through (USIGMABAR, VSIGMABAR),
+ * whose pullback through compute_velocity_profile! (src/StressBalances/diva.jl:89-109) gives the state's shadows. */
 nk_status nk_adjoint(nk_solver *solver, const nk_control *control, nk_info *info);
 
 /* Multigrid level shapes of the handle (StressBalanceMultigridPreconditioner(grid).levels). Writes up to

@@ -365,6 +365,56 @@ __global__ __launch_bounds__(NT) void k_adj_basal_velocity(KG g, const double *_
     }
 }
 
+// Faces: compute_velocity_profile! backwards (diva.jl:89-109): uσₖ = uᵇ (1 + βᵇ F₁ₖ), F₁ₖ = ℑx SIₖ,
+// SIₖ(c) = H(c) Σ_{k′≤k} pw[k][k′] / ησ(c, k′). From ūσ: uᵇ̄ += Σ ūσₖ (1 + βᵇ F₁ₖ), βᵇ̄ += Σ ūσₖ uᵇ F₁ₖ, and
+// ησ̄(c, k′) −= ūσₖ uᵇ βᵇ / 2 · H pw[k][k′] / ησ(c, k′)² for both centres c of the face.
+__global__ __launch_bounds__(NT) void k_adj_velocity_profile(KG g, Sig sg, int Ns, long plane, const double *__restrict__ es,
+                                                             const double *__restrict__ H, const double *__restrict__ ub,
+                                                             const double *__restrict__ vb, const double *__restrict__ bbu,
+                                                             const double *__restrict__ bbv, const double *__restrict__ usb,
+                                                             const double *__restrict__ vsb, double *__restrict__ ub_b,
+                                                             double *__restrict__ vb_b, double *__restrict__ bbu_b,
+                                                             double *__restrict__ bbv_b, double *__restrict__ es_b) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    auto SI = [&](long oc, int k) {
+        double I = 0.0;
+        for (int kp = 0; kp <= k; kp++) I += 1.0 / es[oc + (long)kp * plane] * sg.pw[k * Ns + kp];
+        return I * H[oc];
+    };
+    auto face = [&](double u, double b, const double *sb, long c0, long c1, double *u_b, double *b_b) {
+        double ubar = 0.0, bbar = 0.0;
+        for (int k = 0; k < Ns; k++) {
+            const double a = sb[o + (long)k * plane];
+            if (a == 0.0) continue;
+            const double F1 = (SI(c0, k) + SI(c1, k)) / 2.0;
+            ubar += a * (1.0 + b * F1);
+            bbar += a * u * F1;
+            const double sib = a * u * b / 2.0;
+            for (int kp = 0; kp <= k; kp++) {
+                const double e0 = es[c0 + (long)kp * plane], e1 = es[c1 + (long)kp * plane];
+                aadd(&es_b[c0 + (long)kp * plane], -sib * H[c0] * sg.pw[k * Ns + kp] / (e0 * e0));
+                aadd(&es_b[c1 + (long)kp * plane], -sib * H[c1] * sg.pw[k * Ns + kp] / (e1 * e1));
+            }
+        }
+        u_b[o] += ubar;
+        b_b[o] += bbar;
+    };
+    if (i <= nxf(g) && j <= g.Ny)
+        face(ub[o], bbu[o], usb, off(g, cx(g, i - 1), cy(g, j)), off(g, cx(g, i), cy(g, j)), ub_b, bbu_b);
+    if (i <= g.Nx && j <= nyf(g))
+        face(vb[o], bbv[o], vsb, off(g, cx(g, i), cy(g, j - 1)), off(g, cx(g, i), cy(g, j)), vb_b, bbv_b);
+}
+
+// b = a + c over the (Nx + 1) × (Ny + 1) footprint of one plane
+__global__ __launch_bounds__(NT) void k_add_plane(KG g, const double *__restrict__ a, const double *__restrict__ c, double *__restrict__ b) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    b[o] = a[o] + c[o];
+}
+
 // one plane over the (Nx + 1) × (Ny + 1) footprint
 __global__ __launch_bounds__(NT) void k_copy_plane(KG g, const double *__restrict__ a, double *__restrict__ b) {
     NK_IJ

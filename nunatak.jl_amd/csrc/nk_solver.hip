@@ -30,7 +30,7 @@ const LocInfo LOC[NK_FIELD_COUNT] = {
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0},
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0},
     {1, 0, 0}, {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0},
-    {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2},
+    {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2}, {1, 0, 1}, {0, 1, 1},
 };
 static_assert(sizeof(LOC) / sizeof(LOC[0]) == NK_FIELD_COUNT, "one location per field id");
 
@@ -82,6 +82,8 @@ struct nk_solver {
     double *ad_bbu = nullptr, *ad_bbv = nullptr, *ad_ubb = nullptr, *ad_vbb = nullptr, *ad_F2n = nullptr;
     double *st_bbu = nullptr, *st_bbv = nullptr, *st_F2 = nullptr, *out_bbu = nullptr, *out_bbv = nullptr, *out_F2 = nullptr;
     double *st_es = nullptr, *out_es = nullptr;
+    // shadows the cost gives the DIVA state (through the layer velocities) and the resulting x̄ of the adjoint
+    double *cs_bbu = nullptr, *cs_bbv = nullptr, *cs_F2 = nullptr, *cs_es = nullptr, *ad_xbu = nullptr, *ad_xbv = nullptr;
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
     unsigned int *counter = nullptr;
@@ -1098,10 +1100,10 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
                         &s->w64.qu, &s->w64.qv, &s->w64.du, &s->w64.dv, &s->xmu, &s->xmv, &s->ad_qu, &s->ad_qv,
                         &s->ad_yu, &s->ad_yv, &s->ad_lu, &s->ad_lv, &s->ad_ru, &s->ad_rv, &s->ad_eta, &s->ad_exy,
                         &s->ad_bbu, &s->ad_bbv, &s->ad_ubb, &s->ad_vbb, &s->ad_F2n, &s->st_bbu, &s->st_bbv, &s->st_F2,
-                        &s->out_bbu, &s->out_bbv, &s->out_F2};
+                        &s->out_bbu, &s->out_bbv, &s->out_F2, &s->cs_bbu, &s->cs_bbv, &s->cs_F2, &s->ad_xbu, &s->ad_xbv};
         for (auto p : w) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
-        if ((st = alloc_plane(s, &s->st_es, s->g, s->Ns)) != NK_OK) return fail(st);
-        if ((st = alloc_plane(s, &s->out_es, s->g, s->Ns)) != NK_OK) return fail(st);
+        double **wσ[] = {&s->st_es, &s->out_es, &s->cs_es};
+        for (auto p : wσ) if ((st = alloc_plane(s, p, s->g, s->Ns)) != NK_OK) return fail(st);
         if (options->precision == NK_PRECISION_MIXED) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
                              &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv};
@@ -1205,7 +1207,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
         if (mixed) b += (4 + 9) * sizeof(float) * p;
         if (o->coarse_operator == NK_COARSE_GALERKIN) b += GS_ALLOC * (mixed ? sizeof(float) : sizeof(double)) * p;
     }
-    b += (24 + 11) * sizeof(double) * p0 + 2 * Ns * sizeof(double) * p0;
+    b += (24 + 16) * sizeof(double) * p0 + 3 * Ns * sizeof(double) * p0;
     if (mixed) b += 12 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
@@ -1456,6 +1458,13 @@ nk_status nk_picard_solve(nk_solver *s, nk_info *info) {
     return t.done(picard(s, info));
 }
 
+nk_status nk_compute_velocity_profile(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    CHK(post_process(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
 nk_status nk_update_velocities(nk_solver *s, int32_t geometry, nk_info *info) {
     if (!s) return NK_INVALID_ARGUMENT;
     Timer t(s, info);
@@ -1586,14 +1595,46 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
         CHK(launch_check(s));
     }
     const bool diva = s->ph.stress_balance == NK_DIVA;
-    if (diva) {   // the cost reads no DIVA state directly: the sweep starts from zero shadows
-        double *z[] = {s->st_bbu, s->st_bbv, s->st_F2};
+    const double *xbu = f[NK_FIELD_UBAR], *xbv = f[NK_FIELD_VBAR];
+    if (diva) {
+        // the cost's pullback through the layer velocities (compute_velocity_profile!, after the Picard loop): ū joins x̄,
+        // and the DIVA state gets the shadows s̄ that every fixed-point step adds to its sweep
+        double *z[] = {s->cs_bbu, s->cs_bbv, s->cs_F2, s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv};
         for (double *a : z) {
             k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, a, nullptr, 0.0, nullptr);
             CHK(launch_check(s));
         }
         for (int k = 0; k < s->Ns; k++) {
-            k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->st_es + (long)k * s->plane, s->out_es + (long)k * s->plane, 0.0, nullptr);
+            k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->cs_es + (long)k * s->plane, s->out_es + (long)k * s->plane, 0.0, nullptr);
+            CHK(launch_check(s));
+        }
+        // uᵇ of the post-processing, at x* and the state there, into out_bbu/out_bbv (free until the first step clears them)
+        double *ubp = s->out_bbu, *vbp = s->out_bbv;
+        k_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                         f[NK_FIELD_F2], ubp, vbp);
+        CHK(launch_check(s));
+        k_adj_velocity_profile<<<Be, THREADS, 0, s->stream>>>(g, sig_tables(s), s->Ns, s->plane, f[NK_FIELD_ETASIGMA], f[NK_FIELD_H],
+                                                               ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                               f[NK_FIELD_USIGMABAR], f[NK_FIELD_VSIGMABAR], s->ad_ubb, s->ad_vbb,
+                                                               s->cs_bbu, s->cs_bbv, s->cs_es);
+        CHK(launch_check(s));
+        k_adj_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                             f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->cs_bbu,
+                                                             s->cs_bbv, s->cs_F2);
+        CHK(launch_check(s));
+        k_add<<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_yu, s->ad_yv, s->ad_xbu, s->ad_xbv);
+        CHK(launch_check(s));
+        xbu = s->ad_xbu;
+        xbv = s->ad_xbv;
+        // first step's state shadows: s̄ + 0
+        k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->cs_bbu, s->st_bbu);
+        CHK(launch_check(s));
+        k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->cs_bbv, s->st_bbv);
+        CHK(launch_check(s));
+        k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->cs_F2, s->st_F2);
+        CHK(launch_check(s));
+        for (int k = 0; k < s->Ns; k++) {
+            k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->cs_es + (long)k * s->plane, s->st_es + (long)k * s->plane);
             CHK(launch_check(s));
         }
     }
@@ -1608,14 +1649,21 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
         CHK(launch_check(s));
         if (diva) {
             CHK(adjoint_step_diva(s, false));                                    // ȳ and the lagged state's shadows
-            std::swap(s->st_bbu, s->out_bbu);                                    // sweep: they become the next step's
-            std::swap(s->st_bbv, s->out_bbv);
-            std::swap(s->st_F2, s->out_F2);
-            std::swap(s->st_es, s->out_es);
+            // next step's state shadows: s̄ + this step's sweep (ext/NunatakEnzymeExt.jl:205-212)
+            double *pairs[][3] = {{s->cs_bbu, s->out_bbu, s->st_bbu}, {s->cs_bbv, s->out_bbv, s->st_bbv}, {s->cs_F2, s->out_F2, s->st_F2}};
+            for (auto &q : pairs) {
+                k_add_plane<<<Be, THREADS, 0, s->stream>>>(g, q[0], q[1], q[2]);
+                CHK(launch_check(s));
+            }
+            for (int k = 0; k < s->Ns; k++) {
+                const long ko = (long)k * s->plane;
+                k_add_plane<<<Be, THREADS, 0, s->stream>>>(g, s->cs_es + ko, s->out_es + ko, s->st_es + ko);
+                CHK(launch_check(s));
+            }
         } else {
             CHK(adjoint_step(s, true, false));                                   // ȳ
         }
-        k_add<<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_yu, s->ad_yv, s->ad_ru, s->ad_rv);
+        k_add<<<B, THREADS, 0, s->stream>>>(g, xbu, xbv, s->ad_yu, s->ad_yv, s->ad_ru, s->ad_rv);
         CHK(launch_check(s));
         k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, lu, lv, s->ad_lu, s->ad_lv);     // λ⁺ = λ (warm start)
         CHK(launch_check(s));

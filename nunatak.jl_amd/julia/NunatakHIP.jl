@@ -63,7 +63,7 @@ end
 # field ids (nk_field_id)
 const U, V, H, BED, S, PHI, PHIU, PHIV, N, MU, TAUU, TAUV, ETAH, ETAHFF, BETAU, BETAV, F2, ETASIGMA,
       BETABU, BETABV, UB, VB, USIGMA, VSIGMA, A, C, TEMPERATURE, PU, PV, QU, QV,
-      UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR = 0:36
+      UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR, USIGMABAR, VSIGMABAR = 0:38
 
 control(c::Convergence) = NKControl(0, c.maxiter, 0, 0, c.reltol, c.abstol)
 control(c::FixedIterations) = NKControl(1, c.iterations, c.iterations, 0, 0.0, 0.0)
@@ -186,10 +186,16 @@ end
 update_velocities!(model, solver::HIPStressBalanceSolver) = update_velocities!(model, solver, model.stress_balance)
 
 # Reverse pass of a converged SSA or DIVA velocity solve (the fixed-point rule of ext/NunatakEnzymeExt.jl:175-240): given the
-# cost gradient with respect to the velocities (ū, v̄), accumulate ∂J/∂τ into (τ̄ᵘ, τ̄ᵛ), ∂J/∂C into C̄ and ∂J/∂A into
-# Ā. The handle must still hold the converged solution of the forward call.
-function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalanceSolver, ū, v̄; control = nothing)
+# cost gradient with respect to the velocities (ū, v̄) and, for DIVA, the layer velocities (ūσ, v̄σ, pulled back through
+# compute_velocity_profile!), accumulate ∂J/∂τ into (τ̄ᵘ, τ̄ᵛ), ∂J/∂C into C̄ and ∂J/∂A into Ā. The handle must still
+# hold the converged solution of the forward call.
+function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalanceSolver, ū, v̄; ūσ = nothing, v̄σ = nothing,
+                               control = nothing)
     set_field!(solver, UBAR, ū); set_field!(solver, VBAR, v̄)
+    for (id, f) in ((USIGMABAR, ūσ), (VSIGMABAR, v̄σ))
+        f === nothing ? check(ccall((:nk_fill_field, libnunatak_hip), Cint, (Ptr{Cvoid}, Cint, Float64), solver.handle, id, 0.0),
+                              solver.handle, "nk_fill_field") : set_field!(solver, id, f)
+    end
     info = solver.info
     c = control === nothing ? C_NULL : Ref(NunatakHIP.control(control))
     check(ccall((:nk_adjoint, libnunatak_hip), Cint, (Ptr{Cvoid}, Ptr{NKControl}, Ref{NKInfo}), solver.handle, c, info),

@@ -26,7 +26,8 @@ PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE = 0, 1, 2, 3
 
 FIELDS = ["U", "V", "H", "BED", "S", "PHI", "PHIU", "PHIV", "N", "MU", "TAUU", "TAUV", "ETAH", "ETAHFF",
           "BETAU", "BETAV", "F2", "ETASIGMA", "BETABU", "BETABV", "UB", "VB", "USIGMA", "VSIGMA", "A", "C",
-          "TEMPERATURE", "PU", "PV", "QU", "QV", "UBAR", "VBAR", "LAMBDAU", "LAMBDAV", "CBAR", "ABAR"]
+          "TEMPERATURE", "PU", "PV", "QU", "QV", "UBAR", "VBAR", "LAMBDAU", "LAMBDAV", "CBAR", "ABAR", "USIGMABAR",
+          "VSIGMABAR"]
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 
 # (x location, y location, on the σ grid); "f" = Face, "c" = Center.
@@ -41,6 +42,7 @@ LOCATIONS = {
     "TEMPERATURE": ("c", "c", True), "PU": ("f", "c", False), "PV": ("c", "f", False),
     "QU": ("f", "c", False), "QV": ("c", "f", False), "UBAR": ("f", "c", False), "VBAR": ("c", "f", False),
     "LAMBDAU": ("f", "c", False), "LAMBDAV": ("c", "f", False), "CBAR": ("c", "c", False), "ABAR": ("c", "c", None),
+    "USIGMABAR": ("f", "c", True), "VSIGMABAR": ("c", "f", True),
 }
 
 

@@ -44,7 +44,7 @@ static const nko_loc LOCS[NK_FIELD_COUNT] = {
     [NK_FIELD_PU] = {F, C, 0},     [NK_FIELD_PV] = {C, F, 0},      [NK_FIELD_QU] = {F, C, 0},
     [NK_FIELD_QV] = {C, F, 0},     [NK_FIELD_UBAR] = {F, C, 0},    [NK_FIELD_VBAR] = {C, F, 0},
     [NK_FIELD_LAMBDAU] = {F, C, 0},[NK_FIELD_LAMBDAV] = {C, F, 0}, [NK_FIELD_CBAR] = {C, C, 0},
-    [NK_FIELD_ABAR] = {C, C, 0},
+    [NK_FIELD_ABAR] = {C, C, 0},   [NK_FIELD_USIGMABAR] = {F, C, 1}, [NK_FIELD_VSIGMABAR] = {C, F, 1},
 };
 
 /* ------------------------------------------------------------------------------------------------------------ */

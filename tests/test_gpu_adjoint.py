@@ -269,6 +269,59 @@ def test_diva_fixed_point_adjoint_matches_finite_differences():
         assert abs(grad[index] - d) < 1e-5 * abs(d)
 
 
+def diva_layer_cost(m, g):
+    """diva_velocity_cost of test/test_enzyme.jl:44-61: Picard from zero, then compute_velocity_profile!, and
+    J = Σu² + Σv² + Σuσ² + Σvσ² over every stored face (wall faces hold zero)."""
+    for name in ("U", "V", "ETASIGMA", "F2", "BETABU", "BETABV"):
+        m.fill(name, 0.0)
+    m.picard_solve()
+    m.compute_velocity_profile()
+    u, v, us, vs = m.get("U"), m.get("V"), m.get("USIGMA"), m.get("VSIGMA")
+    return float(np.sum(u ** 2) + np.sum(v ** 2) + np.sum(us ** 2) + np.sum(vs ** 2)), (u, v, us, vs)
+
+
+@pytest.mark.gpu
+def test_diva_adjoint_with_layer_velocity_cost_matches_finite_differences():
+    """Fixed-point adjoint of converged DIVA Picard iterations (test/test_enzyme.jl:307-326) with the reference's full cost,
+    which also reads the layer velocities uσ, vσ of compute_velocity_profile!: their shadows (USIGMABAR, VSIGMABAR)
+    enter x̄ and the state's shadows of every fixed-point step. ∂J/∂A and ∂J/∂C at the reference's four points against
+    centred differences (h = 1e-4 of the value); bar 1e-6 relative, the reference's."""
+    m, g = diva_problem()
+    J, (u, v, us, vs) = diva_layer_cost(m, g)
+    assert np.abs(us).max() > 1.05 * np.abs(u).max() or np.abs(vs).max() > 1.05 * np.abs(v).max()   # shear is resolved
+    m.set("UBAR", 2 * u)
+    m.set("VBAR", 2 * v)
+    m.set("USIGMABAR", 2 * us)
+    m.set("VSIGMABAR", 2 * vs)
+    info = m.adjoint()
+    say(f"[adjoint] DIVA layer cost J = {J:.6e}; adjoint iterations {info['picard_iterations']}")
+    Cb, Ab = m.get("CBAR").copy(), m.get("ABAR").copy()
+    C, A = m.get("C").copy(), m.get("A").copy()
+
+    def fd(name, arr, index, h):
+        a = arr.copy()
+        a[index] += h
+        m.set(name, a)
+        Jp = diva_layer_cost(m, g)[0]
+        a[index] -= 2 * h
+        m.set(name, a)
+        Jm = diva_layer_cost(m, g)[0]
+        m.set(name, arr)
+        return (Jp - Jm) / (2 * h)
+
+    # Julia (i, j) 1-based → packed [j − 1, i − 1]: A (3, 2), (6, 5); C (4, 3), (7, 6)
+    for name, grad, arr, index in (("A", Ab, A, (1, 2)), ("A", Ab, A, (4, 5)), ("C", Cb, C, (2, 3)), ("C", Cb, C, (5, 6))):
+        d = fd(name, arr, index, 1e-4 * abs(arr[index]))
+        say(f"[adjoint] DIVA layer cost dJ/d{name}{index}: adjoint {grad[index]:.10e}, fd {d:.10e}")
+        assert abs(grad[index] - d) < 1e-6 * abs(d)
+    # the layer term matters: dropping it from the adjoint changes the gradient well beyond the bar
+    m.fill("USIGMABAR", 0.0)
+    m.fill("VSIGMABAR", 0.0)
+    diva_layer_cost(m, g)
+    m.adjoint()
+    assert abs(m.get("CBAR")[2, 3] - Cb[2, 3]) > 1e-3 * abs(Cb[2, 3])
+
+
 FULL = os.environ.get("NK_FULL_SIZE") == "1"
 
 
