@@ -131,7 +131,8 @@ typedef struct nk_solver_options {
 enum {
     NK_PROFILE_NONE = 0,
     NK_PROFILE_OPERATOR = 1,   /* q = A p of the PCG iteration */
-    NK_PROFILE_SWEEP = 2,      /* fused ℓ1-Jacobi sweep x + (b − A x)/s (not the first sweep from zero) */
+    NK_PROFILE_SWEEP = 2,      /* fused ℓ1-Jacobi sweep x + (b − A x)/s (not the first sweep from zero, nor the mixed
+                                  form's last one, which also writes the FP64 z and r·z) */
     NK_PROFILE_UPDATE = 3      /* PCG vector update x += αp, r −= αq (+ Jacobi z = r/d) */
 };
 

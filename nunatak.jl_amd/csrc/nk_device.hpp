@@ -102,6 +102,12 @@ template <typename T> __device__ __forceinline__ T ldfc(const KG &g, const T *f,
 template <typename T> __device__ __forceinline__ T ldcf(const KG &g, const T *f, int i, int j) { return f[off(g, cx(g, i), fy(g, j))]; }
 template <typename T> __device__ __forceinline__ T ldff(const KG &g, const T *f, int i, int j) { return f[off(g, fx(g, i), fy(g, j))]; }
 
+// Division in the operator and smoother: IEEE a / d in FP64 (the reference's operation order, bit for bit); in the
+// FP32 multigrid preconditioner a · rcp(d) with the hardware reciprocal (≤ 1 ulp), which the compiler computes once
+// per divisor instead of expanding every quotient into a denormal-safe division sequence.
+__device__ __forceinline__ double qd(double a, double d) { return a / d; }
+__device__ __forceinline__ float qd(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
+
 template <typename T> struct Coef {
     const T *eta, *etaff, *bu, *bv;
     const uint8_t *act;
@@ -111,26 +117,35 @@ template <typename T> struct Coef {
 // The SSA operator at one (i, j): q = Az β p − ∇·T (src/StressBalances/ssa_operator.jl:60-89), with the
 // reference's expression order. Centres (i, j), (i−1, j), (i, j−1) and corners (i, j), (i, j+1), (i+1, j).
 // ---------------------------------------------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu, const T *pv, const Coef<T> &c, T wf,
-                                          T &qu, T &qv) {
+// The iterate is read through accessors fu(offset), fv(offset) (plain arrays, or values formed on the fly).
+template <typename T, typename FU> __device__ __forceinline__ T iu_f(const KG &g, const uint8_t *act, FU fu, int i, int j) {
+    const T val = fu(off(g, fx(g, i), cy(g, j)));
+    return uact(g, act, i, j) ? val : T(0);
+}
+template <typename T, typename FV> __device__ __forceinline__ T iv_f(const KG &g, const uint8_t *act, FV fv, int i, int j) {
+    const T val = fv(off(g, cx(g, i), fy(g, j)));
+    return vact(g, act, i, j) ? val : T(0);
+}
+
+template <typename T, typename FU, typename FV>
+__device__ __forceinline__ void ssa_apply_f(const KG &g, int i, int j, FU fu, FV fv, const Coef<T> &c, T wf, T &qu, T &qv) {
     const T dx = (T)g.dx, dy = (T)g.dy, Az = (T)(g.dx * g.dy);
     const uint8_t *a = c.act;
-    const T u00 = iu(g, a, pu, i, j), u10 = iu(g, a, pu, i + 1, j), um0 = iu(g, a, pu, i - 1, j);
-    const T u0m = iu(g, a, pu, i, j - 1), u0p = iu(g, a, pu, i, j + 1), u1m = iu(g, a, pu, i + 1, j - 1);
-    const T v00 = iv(g, a, pv, i, j), v01 = iv(g, a, pv, i, j + 1), vm0 = iv(g, a, pv, i - 1, j);
-    const T vm1 = iv(g, a, pv, i - 1, j + 1), v0m = iv(g, a, pv, i, j - 1), v10 = iv(g, a, pv, i + 1, j);
+    const T u00 = iu_f<T>(g, a, fu, i, j), u10 = iu_f<T>(g, a, fu, i + 1, j), um0 = iu_f<T>(g, a, fu, i - 1, j);
+    const T u0m = iu_f<T>(g, a, fu, i, j - 1), u0p = iu_f<T>(g, a, fu, i, j + 1), u1m = iu_f<T>(g, a, fu, i + 1, j - 1);
+    const T v00 = iv_f<T>(g, a, fv, i, j), v01 = iv_f<T>(g, a, fv, i, j + 1), vm0 = iv_f<T>(g, a, fv, i - 1, j);
+    const T vm1 = iv_f<T>(g, a, fv, i - 1, j + 1), v0m = iv_f<T>(g, a, fv, i, j - 1), v10 = iv_f<T>(g, a, fv, i + 1, j);
     // strain rates at the three centres
-    const T exx_c = (u10 - u00) / dx, eyy_c = (v01 - v00) / dy;      // (i, j)
-    const T exx_w = (u00 - um0) / dx, eyy_w = (vm1 - vm0) / dy;      // (i-1, j)
-    const T exx_s = (u1m - u0m) / dx, eyy_s = (v00 - v0m) / dy;      // (i, j-1)
+    const T exx_c = qd(u10 - u00, dx), eyy_c = qd(v01 - v00, dy);      // (i, j)
+    const T exx_w = qd(u00 - um0, dx), eyy_w = qd(vm1 - vm0, dy);      // (i-1, j)
+    const T exx_s = qd(u1m - u0m, dx), eyy_s = qd(v00 - v0m, dy);      // (i, j-1)
     const T e_c = ldc(g, c.eta, i, j), e_w = ldc(g, c.eta, i - 1, j), e_s = ldc(g, c.eta, i, j - 1);
     const T txx_c = T(2) * e_c * (T(2) * exx_c + eyy_c), txx_w = T(2) * e_w * (T(2) * exx_w + eyy_w);
     const T tyy_c = T(2) * e_c * (T(2) * eyy_c + exx_c), tyy_s = T(2) * e_s * (T(2) * eyy_s + exx_s);
     // shear strain at the three corners
-    T exy_00 = ((u00 - u0m) / dy + (v00 - vm0) / dx) / T(2);
-    T exy_01 = ((u0p - u00) / dy + (v01 - vm1) / dx) / T(2);
-    T exy_10 = ((u10 - u1m) / dy + (v10 - v00) / dx) / T(2);
+    T exy_00 = (qd(u00 - u0m, dy) + qd(v00 - vm0, dx)) / T(2);
+    T exy_01 = (qd(u0p - u00, dy) + qd(v01 - vm1, dx)) / T(2);
+    T exy_10 = (qd(u10 - u1m, dy) + qd(v10 - v00, dx)) / T(2);
     exy_00 = periph_ff(g, i, j) ? wf * exy_00 : exy_00;
     exy_01 = periph_ff(g, i, j + 1) ? wf * exy_01 : exy_01;
     exy_10 = periph_ff(g, i + 1, j) ? wf * exy_10 : exy_10;
@@ -138,7 +153,7 @@ __device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu
     const T txy_01 = T(2) * ldff(g, c.etaff, i, j + 1) * exy_01;
     const T txy_10 = T(2) * ldff(g, c.etaff, i + 1, j) * exy_10;
     const long o = off(g, i, j);
-    const T pu0 = pu[o], pv0 = pv[o];
+    const T pu0 = fu(o), pv0 = fv(o);
     const T divu = (dy * txx_c - dy * txx_w) + (dx * txy_01 - dx * txy_00);
     const T divv = (dx * tyy_c - dx * tyy_s) + (dy * txy_10 - dy * txy_00);
     const T qu_ = Az * c.bu[o] * pu0 - divu;
@@ -148,12 +163,18 @@ __device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu
     qv = (f & 2) ? qv_ : pv0;
 }
 
+template <typename T>
+__device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu, const T *pv, const Coef<T> &c, T wf,
+                                          T &qu, T &qv) {
+    ssa_apply_f<T>(g, i, j, [pu](long q) { return pu[q]; }, [pv](long q) { return pv[q]; }, c, wf, qu, qv);
+}
+
 // Operator diagonal (src/StressBalances/diagonal_preconditioner.jl:20-44)
 template <typename T> __device__ __forceinline__ T corner_factor(const KG &g, int i, int j, T wf) { return periph_ff(g, i, j) ? wf : T(1); }
-template <typename T> __device__ __forceinline__ T x_nd(const KG &g, const T *eta, int i, int j) { return T(4) * (T)g.dy * ldc(g, eta, i, j) / (T)g.dx; }
-template <typename T> __device__ __forceinline__ T y_nd(const KG &g, const T *eta, int i, int j) { return T(4) * (T)g.dx * ldc(g, eta, i, j) / (T)g.dy; }
-template <typename T> __device__ __forceinline__ T x_sd(const KG &g, const T *ff, int i, int j, T wf) { return corner_factor(g, i, j, wf) * (T)g.dx * ldff(g, ff, i, j) / (T)g.dy; }
-template <typename T> __device__ __forceinline__ T y_sd(const KG &g, const T *ff, int i, int j, T wf) { return corner_factor(g, i, j, wf) * (T)g.dy * ldff(g, ff, i, j) / (T)g.dx; }
+template <typename T> __device__ __forceinline__ T x_nd(const KG &g, const T *eta, int i, int j) { return qd(T(4) * (T)g.dy * ldc(g, eta, i, j), (T)g.dx); }
+template <typename T> __device__ __forceinline__ T y_nd(const KG &g, const T *eta, int i, int j) { return qd(T(4) * (T)g.dx * ldc(g, eta, i, j), (T)g.dy); }
+template <typename T> __device__ __forceinline__ T x_sd(const KG &g, const T *ff, int i, int j, T wf) { return qd(corner_factor(g, i, j, wf) * (T)g.dx * ldff(g, ff, i, j), (T)g.dy); }
+template <typename T> __device__ __forceinline__ T y_sd(const KG &g, const T *ff, int i, int j, T wf) { return qd(corner_factor(g, i, j, wf) * (T)g.dy * ldff(g, ff, i, j), (T)g.dx); }
 
 template <typename T> __device__ __forceinline__ T x_diag(const KG &g, const Coef<T> &c, int i, int j, T wf) {
     T d = (T)(g.dx * g.dy) * c.bu[off(g, i, j)] + x_nd(g, c.eta, i, j) + x_nd(g, c.eta, i - 1, j) + x_sd(g, c.etaff, i, j, wf) +
@@ -277,27 +298,44 @@ template <int NV> __device__ void reduce_finish(double (&v)[NV], const Red &r) {
 
 __device__ __forceinline__ bool solve_done(const Scal *S) { return S != nullptr && S->done; }
 
-// Sum NV block partials of the preceding kernel in block order and run the finisher (one block).
+// Sum NV block partials of the preceding kernel and run the finisher (one block). Each thread first loads its
+// share of every value's slots in batches of FIN_BATCH (all loads of a batch in flight together), then adds them in a
+// fixed order, so the totals are the same on every run.
+constexpr int FIN_BATCH = 8;
 __global__ __launch_bounds__(NT) void k_finish(const double *__restrict__ partials, int nblocks, int nv, Scal *S, int fin,
                                                const Scal *gate) {
     if (solve_done(gate)) return;
-    __shared__ double sh[NT / 64];
+    __shared__ double sh[MAXRED][NT / 64];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int nth = blockDim.x * blockDim.y, nwaves = nth >> 6;
-    double tot[MAXRED] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < nv; k++) {
-        double s = 0.0;
-        for (int b = tid; b < nblocks; b += nth) s += partials[(long)k * nblocks + b];
-        s = wave_sum(s);
-        __syncthreads();
-        if (lane == 0) sh[wave] = s;
-        __syncthreads();
-        double t = 0.0;
-        for (int w = 0; w < nwaves; w++) t += sh[w];
-        tot[k] = t;
+    double s[MAXRED] = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = 0; b0 < nblocks; b0 += nth * FIN_BATCH) {
+        double x[MAXRED][FIN_BATCH];
+#pragma unroll
+        for (int k = 0; k < MAXRED; k++)
+#pragma unroll
+            for (int u = 0; u < FIN_BATCH; u++) {
+                const int b = b0 + u * nth + tid;
+                x[k][u] = (k < nv && b < nblocks) ? partials[(long)k * nblocks + b] : 0.0;
+            }
+#pragma unroll
+        for (int k = 0; k < MAXRED; k++)
+#pragma unroll
+            for (int u = 0; u < FIN_BATCH; u++) s[k] += x[k][u];
     }
-    if (tid == 0) finish(S, fin, tot);
+#pragma unroll
+    for (int k = 0; k < MAXRED; k++) {
+        const double w = wave_sum(s[k]);
+        if (lane == 0) sh[k][wave] = w;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double tot[MAXRED] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < nv; k++)
+            for (int w = 0; w < nwaves; w++) tot[k] += sh[k][w];
+        finish(S, fin, tot);
+    }
 }
 
 // XCD-aware tile order: the dispatcher hands consecutive workgroup ids to the 8 XCDs in turn, so id → tile is
@@ -316,7 +354,10 @@ __device__ __forceinline__ void xcd_tile(int &bx, int &by) {
 
 // Two vertically adjacent points per thread (rows j0, j0 + 1) for the finest-level smoother: the stencil rows the
 // two points share are loaded once, and half as many waves cover the grid. Grid: blocks_for_rows(Nx, Ny, SROWS).
-constexpr int SROWS = 2;
+#ifndef NK_SROWS   // compile-time override for tuning (-DNK_SROWS=4 measured no faster at 1024²)
+#define NK_SROWS 2
+#endif
+constexpr int SROWS = NK_SROWS;
 #define NK_IJ2                                                \
     int bx_, by_;                                             \
     xcd_tile(bx_, by_);                                       \
@@ -405,10 +446,12 @@ __global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int p
 }
 
 // Multigrid PCG: x += α p; r += (−α) q; r·r → res, it, done.
+// With r32u: also r32 = (float) r, the FP32 preconditioner's right-hand side (mixed precision).
 template <typename T>
 __global__ __launch_bounds__(NT) void k_update_xr(KG g, T *__restrict__ xu, T *__restrict__ xv, const T *__restrict__ pu,
                                                   const T *__restrict__ pv, const T *__restrict__ qu, const T *__restrict__ qv,
-                                                  T *__restrict__ ru, T *__restrict__ rv, Red red, const Scal *S) {
+                                                  T *__restrict__ ru, T *__restrict__ rv, float *__restrict__ r32u,
+                                                  float *__restrict__ r32v, Red red, const Scal *S) {
     if (solve_done(S)) return;
     NK_IJ
     double acc[1] = {0.0};
@@ -420,6 +463,10 @@ __global__ __launch_bounds__(NT) void k_update_xr(KG g, T *__restrict__ xu, T *_
         const T a = T(1) * ru[o] + (-al) * qu[o], b = T(1) * rv[o] + (-al) * qv[o];
         ru[o] = a;
         rv[o] = b;
+        if (r32u) {
+            r32u[o] = (float)a;
+            r32v[o] = (float)b;
+        }
         acc[0] = (double)a * (double)a + (double)b * (double)b;
     }
     reduce_finish<1>(acc, red);
@@ -482,14 +529,14 @@ __global__ __launch_bounds__(NT) void k_picard_change(KG g, double *__restrict__
 
 // Fused ℓ1-Jacobi sweep x_out = x + (b − A x) / s (src/StressBalances/multigrid_preconditioner.jl:165-219).
 // from_zero: x = 0, so A x = 0 and x_out = 0 + (b − 0) / s exactly as the reference computes it.
-// With nred: Σ b·x_out → finisher (the r·z of the PCG at the finest level).
-template <typename T, int TX, int TY>
-__global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c, T wf,
-                                              const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
-                                              T *__restrict__ ov, int from_zero, Red red, int nred, const Scal *gate) {
-    g.tx = TX; g.ty = TY;
-    if (solve_done(gate)) return;
-    NK_IJ2
+// With nred: Σ b·x_out → finisher (the r·z of the PCG at the finest level). FZ (the last sweep of the mixed form's
+// FP32 V-cycle, k_sweep_z64): also z64 = (double) x_out, and the sum is Σ r64·z64 with the FP64 residual r64.
+template <typename T, bool FZ>
+__device__ __forceinline__ void sweep_body(KG g, int i, int j0, const T *__restrict__ xu, const T *__restrict__ xv, const Coef<T> &c,
+                                           T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                           T *__restrict__ ov, int from_zero, const Red &red, int nred,
+                                           const double *__restrict__ r64u, const double *__restrict__ r64v,
+                                           double *__restrict__ z64u, double *__restrict__ z64v) {
     double acc[1] = {0.0};
 #pragma unroll
     for (int r = 0; r < SROWS; r++) {
@@ -504,14 +551,79 @@ __global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, co
                 x_u = xu[o];
                 x_v = xv[o];
             }
-            const T nu = x_u + (bu[o] - qa) / x_l1(g, c, i, j, wf);
-            const T nv = x_v + (bv[o] - qb) / y_l1(g, c, i, j, wf);
+            const T nu = x_u + qd(bu[o] - qa, x_l1(g, c, i, j, wf));
+            const T nv = x_v + qd(bv[o] - qb, y_l1(g, c, i, j, wf));
             ou[o] = nu;
             ov[o] = nv;
-            if (nred) acc[0] += (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
+            if (FZ) {
+                const double a = (double)nu, b = (double)nv;
+                z64u[o] = a;
+                z64v[o] = b;
+                acc[0] += r64u[o] * a + r64v[o] * b;
+            } else if (nred) {
+                acc[0] += (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
+            }
         }
     }
-    if (nred) reduce_finish<1>(acc, red);
+    if (FZ || nred) reduce_finish<1>(acc, red);
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c, T wf,
+                                              const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                              T *__restrict__ ov, int from_zero, Red red, int nred, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ2
+    sweep_body<T, false>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, from_zero, red, nred, nullptr, nullptr, nullptr, nullptr);
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_z64(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c, T wf,
+                                                  const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                                  T *__restrict__ ov, Red red, const Scal *gate, const double *__restrict__ r64u,
+                                                  const double *__restrict__ r64v, double *__restrict__ z64u,
+                                                  double *__restrict__ z64v) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ2
+    sweep_body<T, true>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, 0, red, 1, r64u, r64v, z64u, z64v);
+}
+
+// 1/ℓ1 of every row at the finest level (FP32 V-cycle of the mixed form): qd(1, ℓ1) = rcp(ℓ1), the factor every FP32
+// sweep applies, stored so the fused first sweeps can form x₁ = b/ℓ1 at a row's neighbours.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_l1_recip(KG g, Coef<T> c, T wf, T *__restrict__ ru, T *__restrict__ rv) {
+    g.tx = TX; g.ty = TY;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    ru[o] = qd(T(1), x_l1(g, c, i, j, wf));
+    rv[o] = qd(T(1), y_l1(g, c, i, j, wf));
+}
+
+// The first two ℓ1-Jacobi sweeps from x = 0 in one pass (FP32): x₁ = b·rcp(ℓ1) is pointwise, so the second sweep
+// x₂ = x₁ + (b − A x₁)·rcp(ℓ1) reads x₁ at its neighbours as b·(1/ℓ1) there. Bit for bit the two separate FP32 sweeps.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_pre(KG g, Coef<T> c, T wf, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                  const T *__restrict__ rlu, const T *__restrict__ rlv, T *__restrict__ ou,
+                                                  T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ2
+    auto x1u = [&](long q) { return T(0) + bu[q] * rlu[q]; };   // as k_sweep_zero forms it
+    auto x1v = [&](long q) { return T(0) + bv[q] * rlv[q]; };
+#pragma unroll
+    for (int r = 0; r < SROWS; r++) {
+        const int j = j0 + r;
+        if (i <= g.Nx && j <= g.Ny) {
+            const long o = off(g, i, j);
+            T qa, qb;
+            ssa_apply_f<T>(g, i, j, x1u, x1v, c, wf, qa, qb);
+            ou[o] = x1u(o) + (bu[o] - qa) * rlu[o];
+            ov[o] = x1v(o) + (bv[o] - qb) * rlv[o];
+        }
+    }
 }
 
 // First ℓ1-Jacobi sweep from x = 0: A·0 = 0, so x = 0 + (b − 0)/s exactly as the reference evaluates it.
@@ -527,8 +639,8 @@ __global__ __launch_bounds__(NT) void k_sweep_zero(KG g, Coef<T> c, T wf, const 
         const int j = j0 + r;
         if (i <= g.Nx && j <= g.Ny) {
             const long o = off(g, i, j);
-            const T nu = T(0) + (bu[o] - T(0)) / x_l1(g, c, i, j, wf);
-            const T nv = T(0) + (bv[o] - T(0)) / y_l1(g, c, i, j, wf);
+            const T nu = T(0) + qd(bu[o] - T(0), x_l1(g, c, i, j, wf));
+            const T nv = T(0) + qd(bv[o] - T(0), y_l1(g, c, i, j, wf));
             ou[o] = nu;
             ov[o] = nv;
             if (nred) acc[0] += (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
@@ -901,8 +1013,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_coarse_cycle(TailDesc<T> d, co
                 const long o = off(g, i, j);
                 const T x_u = z ? T(0) : xu[o], x_v = z ? T(0) : xv[o];
                 const T q_u = z ? T(0) : qu[o], q_v = z ? T(0) : qv[o];
-                xu[o] = x_u + (bu[o] - q_u) / x_l1(g, c, i, j, d.wf);
-                xv[o] = x_v + (bv[o] - q_v) / y_l1(g, c, i, j, d.wf);
+                xu[o] = x_u + qd(bu[o] - q_u, x_l1(g, c, i, j, d.wf));
+                xv[o] = x_v + qd(bv[o] - q_v, y_l1(g, c, i, j, d.wf));
             }
             __syncthreads();
         }

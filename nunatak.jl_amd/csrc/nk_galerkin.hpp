@@ -34,6 +34,24 @@ __device__ __forceinline__ long gplane(const KG &g) { return (long)g.pitch * (lo
 
 // Row RC (0 = u, 1 = v) of A x at (i, j); x read through the location's index maps (wrap on Periodic, clamp on
 // Bounded, where every out-of-range column has a zero coefficient).
+// g_row with the iterate given by accessors fu(offset), fv(offset) instead of arrays
+template <typename T, int RC, typename FU, typename FV>
+__device__ __forceinline__ T g_row_f(const KG &g, const T *__restrict__ st, long plane, long o, int i, int j, FU fu, FV fv) {
+    T q = T(0);
+    constexpr GBlk bu = gblk(RC, 0), bv = gblk(RC, 1);
+#pragma unroll
+    for (int dx = bu.x0; dx <= bu.x1; dx++)
+#pragma unroll
+        for (int dy = bu.y0; dy <= bu.y1; dy++)
+            q += st[gidx(bu, dx, dy) * plane + o] * fu(off(g, fx(g, i + dx), cy(g, j + dy)));
+#pragma unroll
+    for (int dx = bv.x0; dx <= bv.x1; dx++)
+#pragma unroll
+        for (int dy = bv.y0; dy <= bv.y1; dy++)
+            q += st[gidx(bv, dx, dy) * plane + o] * fv(off(g, cx(g, i + dx), fy(g, j + dy)));
+    return q;
+}
+
 template <typename T, int RC>
 __device__ __forceinline__ T g_row(const KG &g, const T *__restrict__ st, long plane, long o, int i, int j,
                                    const T *__restrict__ xu, const T *__restrict__ xv) {
@@ -263,6 +281,27 @@ __global__ __launch_bounds__(NT) void k_g_sweep(KG g, const T *__restrict__ st, 
     ov[o] = xv[o] + (bv[o] - qv) * sv;
 }
 
+// The first two ℓ1-Jacobi sweeps from a zero x in one pass: the first sweep is pointwise (x₁ = b/ℓ1), so the second,
+// x₂ = x₁ + (b − A x₁)/ℓ1, reads x₁ at the stencil's neighbours as b·(1/ℓ1) there. Same products and order as the two
+// separate sweeps, so the result is the same bit for bit.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_g_sweep_pre(KG g, const T *__restrict__ st, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                    T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j), P = gplane(g);
+    const T *su_p = st + GS_PLANES * P, *sv_p = st + (GS_PLANES + 1) * P;
+    const T su = su_p[o], sv = sv_p[o];
+    auto x1u = [&](long q) { return bu[q] * su_p[q]; };
+    auto x1v = [&](long q) { return bv[q] * sv_p[q]; };
+    const T qu = g_row_f<T, 0>(g, st, P, o, i, j, x1u, x1v);
+    const T qv = g_row_f<T, 1>(g, st, P, o, i, j, x1u, x1v);
+    ou[o] = bu[o] * su + (bu[o] - qu) * su;
+    ov[o] = bv[o] * sv + (bv[o] - qv) * sv;
+}
+
 // r = b − A x with a stencil operator
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_g_residual(KG g, const T *__restrict__ st, const T *__restrict__ xu, const T *__restrict__ xv,
@@ -286,6 +325,7 @@ __global__ __launch_bounds__(NT) void k_g_residual(KG g, const T *__restrict__ s
 // (k % Nx + 1, k / Nx + 1), k ≥ n/2 is v likewise; n = 2·Nx·Ny.
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int GDENSE_MAX = 512;   // largest coarsest system inverted densely; above it the coarsest level is swept
+constexpr int GDENSE_STOP = 128;  // a Galerkin hierarchy ends at the first coarse level with at most this many unknowns
 
 __device__ __forceinline__ int g_unknown(const KG &g, int comp, int i, int j) { return comp * g.Nx * g.Ny + (j - 1) * g.Nx + (i - 1); }
 
@@ -354,6 +394,16 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const T *__res
     }
     for (int e = tid; e < n * n; e += nth)
         if (sing[e / n] || sing[e % n]) M[e] = T(0);
+    __syncthreads();
+    // store column-major (element (r, c) at c·n + r) so that the solves read it with coalesced loads
+    for (int e = tid; e < n * n; e += nth) {
+        const int i = e / n, j = e % n;
+        if (i < j) {
+            const T a = M[(long)i * n + j];
+            M[(long)i * n + j] = M[(long)j * n + i];
+            M[(long)j * n + i] = a;
+        }
+    }
 }
 
 // x = M⁻¹ b on the coarsest level (one workgroup)
@@ -368,7 +418,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_dense_solve(KG g, const T *_
         for (int c = 0; c < n; c++) {
             const int cc = c >= n / 2, k = c - cc * (n / 2);
             const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-            acc += M[(long)r * n + c] * (cc ? bv[o] : bu[o]);
+            acc += M[(long)c * n + r] * (cc ? bv[o] : bu[o]);
         }
         const int rc = r >= n / 2, k = r - rc * (n / 2);
         const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
@@ -382,29 +432,62 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_dense_solve(KG g, const T *_
 // planes of the smallest levels are copied into LDS too (o_st ≥ 0), the others are read from global memory
 // (L2-resident at these sizes). The coarsest level is solved with its dense inverse when it has one.
 // ---------------------------------------------------------------------------------------------------------------
+// Compact copy of a level's stencil for the LDS tail: the GS_ALLOC planes with pitch Nx + 1 and no padding rows,
+// plane k of point (i, j) at k·(Nx + 1)(Ny + 1) + (j − 1)(Nx + 1) + i − 1 — the layout the tail keeps in LDS, so
+// loading it is one contiguous 16-byte copy, and a level read from global memory is read with coalesced loads.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_g_compact(KG g, const T *__restrict__ st, T *__restrict__ cst) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long P = gplane(g), o = off(g, i, j);
+    const long NP = (long)(g.Nx + 1) * (g.Ny + 1), q = (long)(j - 1) * (g.Nx + 1) + (i - 1);
+    const bool row = i <= g.Nx && j <= g.Ny;
+#pragma unroll 8
+    for (int k = 0; k < GS_ALLOC; k++) cst[k * NP + q] = row ? st[k * P + o] : T(0);
+}
+
+template <typename T> struct V16;
+template <> struct V16<float> { using type = float4; static constexpr int n = 4; };
+template <> struct V16<double> { using type = double2; static constexpr int n = 2; };
+
 constexpr int GTAIL_MAX = 12;
 
 template <typename T> struct GTailLevel {
     KG lg;                  // LDS geometry (pitch = Nx + 1)
     int gpitch;             // pitch of the level's global planes
     int hx, hy;             // halved relative to the next finer level
-    const T *st;            // stencil + 1/ℓ1 planes (global)
+    const T *cst;           // compact stencil (k_g_compact layout, global)
     const uint8_t *act;     // activity flags (global)
     int o_x0u, o_x0v, o_x1u, o_x1v, o_bu, o_bv, o_ru, o_rv, o_act;   // LDS offsets (elements of T; bytes for o_act)
-    int o_st;               // LDS offset of the stencil planes, or −1 (read from global memory)
+    int o_st;               // LDS offset of a copy of the compact stencil (16-byte aligned), or −1 (read cst)
 };
 
 template <typename T> struct GTailDesc {
     int n, sweeps, coarsest;
+    int dgroups, o_dp;      // dense solve: column groups and the LDS offset of their partial products (G·n values)
     const T *gbu, *gbv;     // right-hand side of the top tail level (global)
     T *gxu, *gxv;           // result of the top tail level (global)
     const T *cinv;          // dense inverse of the coarsest level, or null (sweeps)
     GTailLevel<T> lv[GTAIL_MAX];
 };
 
+// Development aid: built with -DNK_TAIL_TIMING, thread 0 of the tail stamps the device clock (100 MHz) at each phase
+// (load, each level down, coarsest, each level up, store) and prints the 41st call's phase times. Off by default.
+#ifdef NK_TAIL_TIMING
+__device__ unsigned int nk_tail_calls = 0;
+#define NK_TT(k) do { const int k_ = (k); if (threadIdx.x == 0 && k_ < 40) tt[k_] = wall_clock64(); } while (0)
+#else
+#define NK_TT(k) do { } while (0)
+#endif
+
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const Scal *gate) {
     if (solve_done(gate)) return;
+#ifdef NK_TAIL_TIMING
+    unsigned long long tt[40];
+    int nt = 0;
+#endif
+    NK_TT(nt++);
     auto geom = [](const KG &k) { KG g = k; g.tx = TX; g.ty = TY; return g; };
     extern __shared__ __align__(16) unsigned char smem[];
     T *sm = reinterpret_cast<T *>(smem);
@@ -412,7 +495,6 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
     for (int l = 0; l < d.n; l++) {
         const GTailLevel<T> &L = d.lv[l];
         const int nx = L.lg.Nx + 1, ny = L.lg.Ny + 1;
-        const long P = (long)L.gpitch * (long)ny;
         for (int p = tid; p < nx * ny; p += nth) {
             const int i = p % nx, j = p / nx;
             const long go = (long)j * L.gpitch + i;
@@ -423,54 +505,67 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
             sm[L.o_ru + lo] = T(0); sm[L.o_rv + lo] = T(0);
             sm[L.o_bu + lo] = l == 0 ? d.gbu[go] : T(0);
             sm[L.o_bv + lo] = l == 0 ? d.gbv[go] : T(0);
-            if (L.o_st >= 0) {
-#pragma unroll 16
-                for (int k = 0; k < GS_ALLOC; k++) sm[L.o_st + k * nx * ny + lo] = L.st[k * P + go];
-            }
         }
     }
+    // compact stencils of the LDS-resident levels: contiguous 16-byte copies
+    for (int l = 0; l < d.n; l++) {
+        const GTailLevel<T> &L = d.lv[l];
+        if (L.o_st < 0) continue;
+        using V = typename V16<T>::type;
+        const int nv = (L.lg.Nx + 1) * (L.lg.Ny + 1) * GS_ALLOC / V16<T>::n;
+        const V *src = reinterpret_cast<const V *>(L.cst);
+        V *dst = reinterpret_cast<V *>(sm + L.o_st);
+#pragma unroll 4
+        for (int q = tid; q < nv; q += nth) dst[q] = src[q];
+    }
     __syncthreads();
+    NK_TT(nt++);
     unsigned cur = 0;     // bit l: which x buffer holds level l's iterate (0: x0, 1: x1); a register, not an array
     auto C_ = [&](int l) { return (int)((cur >> l) & 1u); };
     auto X = [&](int l, int which, int comp) -> T * {
         const GTailLevel<T> &L = d.lv[l];
         return sm + (which == 0 ? (comp == 0 ? L.o_x0u : L.o_x0v) : (comp == 0 ? L.o_x1u : L.o_x1v));
     };
-    // the level's stencil: base, plane stride and the offset of point (i, j) (LDS copy or global planes)
-    struct St { const T *base; long plane; };
-    auto stencil = [&](const GTailLevel<T> &L) -> St {
-        if (L.o_st >= 0) return St{sm + L.o_st, (long)(L.lg.Nx + 1) * (long)(L.lg.Ny + 1)};
-        return St{L.st, (long)L.gpitch * (long)(L.lg.Ny + 1)};
+    // One pass over the level's rows with the compact stencil at stb (LDS copy or global): an ℓ1 sweep
+    // o = x + (b − A x)/ℓ1 (o = b/ℓ1 when z), or the residual o = b − A x (res). Called with a pointer of one
+    // address space per call site, so the stencil loads are LDS or global loads, not flat ones.
+    auto pass = [&](const T *stb, const KG &g, const T *xu, const T *xv, const T *bu, const T *bv, T *ou, T *ov, bool z,
+                    bool res) __attribute__((always_inline)) {
+        const long NP = (long)(g.Nx + 1) * (long)(g.Ny + 1);
+        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+            const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+            const long o = off(g, i, j);   // pitch Nx + 1: also the point's index in the compact stencil
+            const T su = stb[GS_PLANES * NP + o], sv = stb[(GS_PLANES + 1) * NP + o];
+            if (z) {
+                ou[o] = bu[o] * su;
+                ov[o] = bv[o] * sv;
+            } else {
+                const T qu = g_row<T, 0>(g, stb, NP, o, i, j, xu, xv);
+                const T qv = g_row<T, 1>(g, stb, NP, o, i, j, xu, xv);
+                if (res) {
+                    ou[o] = bu[o] - qu;
+                    ov[o] = bv[o] - qv;
+                } else {
+                    ou[o] = xu[o] + (bu[o] - qu) * su;
+                    ov[o] = xv[o] + (bv[o] - qv) * sv;
+                }
+            }
+        }
     };
-    auto st_off = [&](const GTailLevel<T> &L, int i, int j) -> long {
-        return L.o_st >= 0 ? (long)(j - 1) * L.lg.pitch + (i - 1) : (long)(j - 1) * L.gpitch + (i - 1);
+    auto level_pass = [&](const GTailLevel<T> &L, const KG &g, const T *xu, const T *xv, const T *bu, const T *bv, T *ou, T *ov,
+                          bool z, bool res) {
+        if (L.o_st >= 0) pass(sm + L.o_st, g, xu, xv, bu, bv, ou, ov, z, res);
+        else pass(L.cst, g, xu, xv, bu, bv, ou, ov, z, res);
     };
     // sweeps of x ← x + (b − A x)/ℓ1 (x = b/ℓ1 first when from_zero); one barrier per sweep (ping-pong buffers)
     auto smooth = [&](int l, int sweeps, bool from_zero) {
         const GTailLevel<T> &L = d.lv[l];
         const KG g = geom(L.lg);
-        const St S = stencil(L);
         const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
         for (int k = 0; k < sweeps; k++) {
             const bool z = from_zero && k == 0;
             const int src = C_(l), dst = z ? src : 1 - src;
-            const T *xu = X(l, src, 0), *xv = X(l, src, 1);
-            T *ou = X(l, dst, 0), *ov = X(l, dst, 1);
-            for (int p = tid; p < g.Nx * g.Ny; p += nth) {
-                const int i = p % g.Nx + 1, j = p / g.Nx + 1;
-                const long so = st_off(L, i, j);
-                const long o = off(g, i, j);
-                const T su = S.base[GS_PLANES * S.plane + so], sv = S.base[(GS_PLANES + 1) * S.plane + so];
-                if (z) {
-                    ou[o] = bu[o] * su;
-                    ov[o] = bv[o] * sv;
-                } else {
-                    const T qu = g_row<T, 0>(g, S.base, S.plane, so, i, j, xu, xv);
-                    const T qv = g_row<T, 1>(g, S.base, S.plane, so, i, j, xu, xv);
-                    ou[o] = xu[o] + (bu[o] - qu) * su;
-                    ov[o] = xv[o] + (bv[o] - qv) * sv;
-                }
-            }
+            level_pass(L, g, X(l, src, 0), X(l, src, 1), bu, bv, X(l, dst, 0), X(l, dst, 1), z, false);
             cur = (cur & ~(1u << l)) | ((unsigned)dst << l);
             __syncthreads();
         }
@@ -479,18 +574,9 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
     for (; l < d.n - 1; l++) {
         const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
         const KG g = geom(L.lg), cg = geom(C.lg);
-        const St S = stencil(L);
         smooth(l, d.sweeps, true);
-        const T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
-        const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
         T *ru = sm + L.o_ru, *rv = sm + L.o_rv;
-        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
-            const int i = p % g.Nx + 1, j = p / g.Nx + 1;
-            const long so = st_off(L, i, j);
-            const long o = off(g, i, j);
-            ru[o] = bu[o] - g_row<T, 0>(g, S.base, S.plane, so, i, j, xu, xv);
-            rv[o] = bv[o] - g_row<T, 1>(g, S.base, S.plane, so, i, j, xu, xv);
-        }
+        level_pass(L, g, X(l, C_(l), 0), X(l, C_(l), 1), sm + L.o_bu, sm + L.o_bv, ru, rv, false, true);
         __syncthreads();
         const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
         T *cbu = sm + C.o_bu, *cbv = sm + C.o_bv;
@@ -523,21 +609,43 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
             cbv[o] = (cact[o] & 2) ? sv : T(0);
         }
         __syncthreads();
+        NK_TT(nt++);
     }
     // coarsest level: dense inverse, or sweeps
     if (d.cinv) {
+        // x = M⁻¹ b: d.dgroups groups of n threads, group q summing columns [q·chunk, (q + 1)·chunk) of row r
+        // (coalesced: M is column-major), then the groups' partials added in group order
         const GTailLevel<T> &L = d.lv[l];
         const KG g = geom(L.lg);
-        const int n = 2 * g.Nx * g.Ny;
+        const int n = 2 * g.Nx * g.Ny, G = d.dgroups, chunk = (n + G - 1) / G;
         const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
+        T *dp = sm + d.o_dp;
+        if (tid < G * n) {
+            const int r = tid % n, q = tid / n;
+            const int c0 = q * chunk, c1 = c0 + chunk < n ? c0 + chunk : n;
+            T acc = T(0);
+            constexpr int U = 16;   // loads of a batch issued together
+            for (int cb = c0; cb < c1; cb += U) {
+                T m[U];
+#pragma unroll
+                for (int t = 0; t < U; t++) m[t] = cb + t < c1 ? d.cinv[(long)(cb + t) * n + r] : T(0);
+#pragma unroll
+                for (int t = 0; t < U; t++) {
+                    const int c = cb + t;
+                    if (c < c1) {
+                        const int cc = c >= n / 2, k = c - cc * (n / 2);
+                        const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
+                        acc += m[t] * (cc ? bv[o] : bu[o]);
+                    }
+                }
+            }
+            dp[q * n + r] = acc;
+        }
+        __syncthreads();
         T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
         for (int r = tid; r < n; r += nth) {
             T acc = T(0);
-            for (int c = 0; c < n; c++) {
-                const int cc = c >= n / 2, k = c - cc * (n / 2);
-                const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-                acc += d.cinv[(long)r * n + c] * (cc ? bv[o] : bu[o]);
-            }
+            for (int q = 0; q < G; q++) acc += dp[q * n + r];
             const int rc = r >= n / 2, k = r - rc * (n / 2);
             const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
             (rc ? xv : xu)[o] = acc;
@@ -546,6 +654,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
     } else {
         smooth(l, d.coarsest, true);
     }
+    NK_TT(nt++);
     // up leg
     for (l = d.n - 2; l >= 0; l--) {
         const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
@@ -571,6 +680,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         }
         __syncthreads();
         smooth(l, d.sweeps, false);
+        NK_TT(nt++);
     }
     const GTailLevel<T> &L = d.lv[0];
     const T *xu = X(0, C_(0), 0), *xv = X(0, C_(0), 1);
@@ -579,6 +689,15 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         d.gxu[(long)j * L.gpitch + i] = xu[j * L.lg.pitch + i];
         d.gxv[(long)j * L.gpitch + i] = xv[j * L.lg.pitch + i];
     }
+#ifdef NK_TAIL_TIMING
+    __syncthreads();
+    NK_TT(nt++);
+    if (threadIdx.x == 0 && atomicAdd(&nk_tail_calls, 1u) == 40u) {
+        printf("tail n=%d cinv=%d sweeps=%d coarsest=%d\n", d.n, d.cinv != nullptr, d.sweeps, d.coarsest);
+        for (int k = 0; k < d.n; k++) printf("  level %d: %d x %d stencil %s\n", k, d.lv[k].lg.Nx, d.lv[k].lg.Ny, d.lv[k].o_st >= 0 ? "LDS" : "global");
+        for (int k = 1; k < nt && k < 40; k++) printf("  stamp %d: +%.2f us\n", k, (double)(tt[k] - tt[k - 1]) * 0.01);
+    }
+#endif
 }
 
 }  // namespace nk

@@ -49,7 +49,15 @@ struct Level {
     Vecs<float> v32;
     double *gs64 = nullptr;  // Galerkin stencil + 1/ℓ1 planes (GS_ALLOC) at the V-cycle's precision
     float *gs32 = nullptr;
+    void *gcst = nullptr;    // compact copy of the Galerkin stencil (k_g_compact) for levels the LDS tail may hold
 };
+
+// Levels small enough for the Galerkin LDS tail keep a compact copy of their stencil (shape-only rule, so the
+// device-memory estimate can mirror it).
+constexpr size_t GTAIL_BUDGET = 150 * 1024;
+bool wants_gcst(const KG &g, int l, size_t es) {
+    return l >= 1 && (size_t)(g.Nx + 1) * (size_t)(g.Ny + 1) * (8 * es + 1) <= GTAIL_BUDGET;
+}
 
 template <typename T> struct Work {  // PCG workspace of one precision on the finest grid
     T *ru, *rv, *zu, *zv, *z2u, *z2v, *pu, *pv, *qu, *qv, *du, *dv;
@@ -84,6 +92,11 @@ struct nk_solver {
     double *st_es = nullptr, *out_es = nullptr;
     // shadows the cost gives the DIVA state (through the layer velocities) and the resulting x̄ of the adjoint
     double *cs_bbu = nullptr, *cs_bbv = nullptr, *cs_F2 = nullptr, *cs_es = nullptr, *ad_xbu = nullptr, *ad_xbv = nullptr;
+    // mixed precision: while set, the finest level's last FP32 sweep also writes z64 and sums r64·z64
+    const double *fz_r64u = nullptr, *fz_r64v = nullptr;
+    double *fz_z64u = nullptr, *fz_z64v = nullptr;
+    float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
+    bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
     unsigned int *counter = nullptr;
@@ -313,6 +326,12 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
         NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, 0, s->stream, C.g, gs<T>(C), (T *)s->cinv);
         CHK(launch_check(s));
     }
+    for (size_t l = 1; l < s->lev.size(); l++) {
+        Level &L = s->lev[l];
+        if (!L.gcst) continue;
+        k_g_compact<T><<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, gs<T>(L), (T *)L.gcst);
+        CHK(launch_check(s));
+    }
     int bad = 0;
     HIPCHK(hipMemcpyAsync(&bad, s->gerr, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -337,6 +356,13 @@ nk_status update_preconditioner(nk_solver *s) {
             Level &L = s->lev[l];
             k_cast_coeffs<<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, L.eta, L.etaff, L.bu, L.bv, L.eta32,
                                                                                           L.etaff32, L.bu32, L.bv32);
+            CHK(launch_check(s));
+        }
+        if (s->rl1u32 && s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID) {
+            Level &L0 = s->lev[0];
+            const float wf = (float)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
+            NK_LAUNCH_TOPO(L0.g, k_l1_recip, float, blocks_for(L0.g.Nx, L0.g.Ny), THREADS, 0, s->stream, L0.g, coef<float>(L0), wf,
+                           s->rl1u32, s->rl1v32);
             CHK(launch_check(s));
         }
     }
@@ -381,7 +407,15 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     }
     Pair<T> cur = x0, oth = x1;
     const dim3 B2 = blocks_for_rows(L.g.Nx, L.g.Ny, SROWS);
-    for (int k = 0; k < sweeps; k++) {
+    int k0 = 0;
+    if (sizeof(T) == 4 && l == 0 && s->rl1u32 && s->fuse_pre && sweeps >= 2 && !final_dot) {   // first two FP32 sweeps in one pass
+        NK_LAUNCH_TOPO(L.g, k_sweep_pre, T, B2, THREADS, 0, s->stream, L.g, c, wf, bu, bv, (const T *)s->rl1u32, (const T *)s->rl1v32,
+                       oth.u, oth.v, gate);
+        CHK(launch_check(s));
+        std::swap(cur, oth);
+        k0 = 2;
+    }
+    for (int k = k0; k < sweeps; k++) {
         const bool last = (k == sweeps - 1) && final_dot;
         if (k == 0) {
             NK_LAUNCH_TOPO(L.g, k_sweep_zero, T, B2, THREADS, 0, s->stream, L.g, c, wf, bu, bv, cur.u, cur.v, red(s, FIN_BETA),
@@ -421,9 +455,16 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
         for (int k = 0; k < sweeps; k++) {
             const bool last = (k == sweeps - 1) && final_dot;
             {
-                ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-                NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf,
-                                  bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+                const bool fz = last && l == 0 && sizeof(T) == 4 && s->fz_z64u;   // mixed: z64 and r64·z64 here
+                if (fz) {
+                    NK_LAUNCH_TOPO(L.g, k_sweep_z64, T, B2, THREADS, 0, s->stream, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu, bv,
+                                   oth.u, oth.v, red(s, FIN_BETA), gate, (const double *)s->fz_r64u, (const double *)s->fz_r64v,
+                                   s->fz_z64u, s->fz_z64v);
+                } else {
+                    ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
+                    NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
+                                      wf, bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+                }
             }
             CHK(launch_check(s));
             if (last) CHK(finish_red(s, B2, 1, FIN_BETA, gate));
@@ -503,23 +544,32 @@ nk_status galerkin_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv
         t.lg.pitch = L.g.Nx + 1;
         t.gpitch = L.g.pitch;
         t.hx = L.cx; t.hy = L.cy;
-        t.st = gs<T>(L);
+        t.cst = (const T *)L.gcst;
         t.act = L.act;
         t.o_st = -1;
         const int plane = (L.g.Nx + 1) * (L.g.Ny + 1);
         int *offs[] = {&t.o_x0u, &t.o_x0v, &t.o_x1u, &t.o_x1v, &t.o_bu, &t.o_bv, &t.o_ru, &t.o_rv};
         for (int *o : offs) { *o = off_t; off_t += plane; }
     }
+    if (d.cinv) {   // partial products of the dense solve
+        const KG &gc = s->lev.back().g;
+        const int n = 2 * gc.Nx * gc.Ny;
+        d.dgroups = TAIL_THREADS / n > 0 ? TAIL_THREADS / n : 1;
+        d.o_dp = off_t;
+        off_t += d.dgroups * n;
+    }
     {   // copy the stencils of the smallest levels into LDS while they fit (the coarsest needs none with a dense inverse)
         int flags = 0;
         for (int k = 0; k < d.n; k++) flags += (s->lev[l + k].g.Nx + 1) * (s->lev[l + k].g.Ny + 1);
         const int cap = 160 * 1024;
+        constexpr int align = 16 / (int)sizeof(T);   // 16-byte vector copies and row loads
         for (int k = d.n - (d.cinv ? 2 : 1); k >= 0; k--) {
             const Level &L = s->lev[l + k];
             const int need = GS_ALLOC * (L.g.Nx + 1) * (L.g.Ny + 1);
-            if ((off_t + need) * (int)sizeof(T) + flags > cap) break;
-            d.lv[k].o_st = off_t;
-            off_t += need;
+            const int at = (off_t + align - 1) / align * align;
+            if ((at + need) * (int)sizeof(T) + flags > cap) break;
+            d.lv[k].o_st = at;
+            off_t = at + need;
         }
     }
     int off_b = off_t * (int)sizeof(T);
@@ -549,7 +599,14 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
             k_fill2<T><<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, T(0), gate);
             return launch_check(s);
         }
-        for (int k = 0; k < sweeps; k++) {
+        int k0 = 0;
+        if (from_zero && sweeps >= 2 && s->fuse_pre) {   // the first two sweeps in one pass
+            NK_LAUNCH_TOPO(L.g, k_g_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, bu, bv, oth.u, oth.v, gate);
+            CHK(launch_check(s));
+            std::swap(cur, oth);
+            k0 = 2;
+        }
+        for (int k = k0; k < sweeps; k++) {
             if (from_zero && k == 0) {
                 NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, cur.u, cur.v, 1, gate);
             } else {
@@ -649,14 +706,25 @@ nk_status write_scalars(nk_solver *s, const nk_control &ctl) {
 }
 
 // Mixed-precision preconditioner: z = V₃₂(r) with r cast to FP32 and z promoted back to FP64, then the PCG dot(s)
-// r·z (and r·r at the start) in FP64 → finisher `fin`.
+// r·z (and r·r at the start) in FP64 → finisher `fin`. Inside the PCG loop (FIN_BETA) the FP32 copy of r comes from
+// k_update_xr, and with point Jacobi the V-cycle's last finest-level sweep writes z64 and sums r·z itself.
 nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate) {
     const KG &g = s->g;
     const dim3 B = blocks_for(g.Nx, g.Ny);
     Work<double> &W = s->w64;
     Work<float> &V = s->w32;
-    k_demote_r<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, V.ru, V.rv, gate);
-    CHK(launch_check(s));
+    if (fin == FIN_BETA && s->op.smoother == NK_POINT_JACOBI && s->op.smoothing_sweeps > 0 && s->lev.size() >= 2 &&
+        s->tail_start[1] != 0) {
+        s->fz_r64u = W.ru; s->fz_r64v = W.rv; s->fz_z64u = W.zu; s->fz_z64v = W.zv;
+        Pair<float> z32;
+        const nk_status st = v_cycle<float>(s, 0, Pair<float>{V.zu, V.zv}, Pair<float>{V.z2u, V.z2v}, V.ru, V.rv, true, gate, &z32);
+        s->fz_r64u = s->fz_r64v = nullptr; s->fz_z64u = s->fz_z64v = nullptr;
+        return st;
+    }
+    if (fin != FIN_BETA) {
+        k_demote_r<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, V.ru, V.rv, gate);
+        CHK(launch_check(s));
+    }
     Pair<float> z32;
     CHK(v_cycle<float>(s, 0, Pair<float>{V.zu, V.zv}, Pair<float>{V.z2u, V.z2v}, V.ru, V.rv, false, gate, &z32));
     k_promote_dot<<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.zu, W.zv, W.ru, W.rv, red(s, fin), gate);
@@ -713,7 +781,8 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
                 {
                     ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
                     hipExtLaunchKernelGGL(k_update_xr<T>, B, THREADS, 0, s->stream, ps.e0, ps.e1, 0, g, xu, xv, (const T *)W.pu,
-                                          (const T *)W.pv, (const T *)W.qu, (const T *)W.qv, W.ru, W.rv, red(s, FIN_RES), (const Scal *)s->S);
+                                          (const T *)W.pv, (const T *)W.qu, (const T *)W.qv, W.ru, W.rv, mixed ? s->w32.ru : nullptr,
+                                          mixed ? s->w32.rv : nullptr, red(s, FIN_RES), (const Scal *)s->S);
                 }
                 CHK(launch_check(s));
                 CHK(finish_red(s, B, 1, FIN_RES, s->S));
@@ -1063,6 +1132,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if (options->preconditioner == NK_PRECONDITIONER_MULTIGRID) {
             KG cur = s->g;
             for (;;) {
+                // a Galerkin hierarchy stops where the dense inverse takes over
+                if (options->coarse_operator == NK_COARSE_GALERKIN && s->lev.size() >= 2 && 2 * cur.Nx * cur.Ny <= GDENSE_STOP) break;
                 int cxh = coarsening(cur.Nx, grid->halo, options->minimum_coarse_size);
                 int cyh = coarsening(cur.Ny, grid->halo, options->minimum_coarse_size);
                 if (!cxh && !cyh) break;
@@ -1092,6 +1163,10 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
                 if (options->precision == NK_PRECISION_MIXED) st = alloc_plane(s, &L.gs32, L.g, GS_ALLOC);
                 else st = alloc_plane(s, &L.gs64, L.g, GS_ALLOC);
                 if (st != NK_OK) return fail(st);
+                const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
+                if (wants_gcst(L.g, (int)l, es) &&
+                    (st = alloc_bytes(s, &L.gcst, (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1) * GS_ALLOC * es)) != NK_OK)
+                    return fail(st);
             }
         }
     }
@@ -1106,7 +1181,7 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         for (auto p : wσ) if ((st = alloc_plane(s, p, s->g, s->Ns)) != NK_OK) return fail(st);
         if (options->precision == NK_PRECISION_MIXED) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
-                             &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv};
+                             &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv, &s->rl1u32, &s->rl1v32};
             for (auto p : w32) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
         }
     }
@@ -1133,6 +1208,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
                                (const void *)k_coarse_cycle<float, 1, 0>, (const void *)k_coarse_cycle<float, 1, 1>};
         for (const void *f : tails)
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) s->tail_start[0] = s->tail_start[1] = -1;
+        const char *nf = std::getenv("NK_NO_PRE_FUSE");
+        s->fuse_pre = !(nf && nf[0] == '1');
         const char *nt = std::getenv("NK_NO_TAIL");
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
         // the LDS tail runs the rediscretized operator: levels with Galerkin operators have a tail of their own
@@ -1140,13 +1217,13 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             s->tail_start[options->precision == NK_PRECISION_MIXED ? 1 : 0] = -1;
             const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
             const char *gb = std::getenv("NK_GTAIL_BYTES");   // tuning: LDS bytes of vectors the tail may take
-            const size_t gbudget = gb ? (size_t)std::strtoul(gb, nullptr, 10) : budget;
+            const size_t gbudget = gb ? (size_t)std::strtoul(gb, nullptr, 10) : GTAIL_BUDGET;
             size_t bytes = 0;
             int start = -1;
             for (int l = (int)s->lev.size() - 1; l >= 1; l--) {
                 const Level &L = s->lev[l];
                 const size_t b = (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1) * (8 * es + 1);
-                if (bytes + b > gbudget || bytes + b > budget || (int)s->lev.size() - l > GTAIL_MAX) break;
+                if (bytes + b > gbudget || bytes + b > GTAIL_BUDGET || !L.gcst || (int)s->lev.size() - l > GTAIL_MAX) break;
                 bytes += b;
                 start = l;
             }
@@ -1193,6 +1270,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
     if (o->preconditioner == NK_PRECONDITIONER_MULTIGRID) {
         KG cur = g0;
         for (;;) {
+            if (o->coarse_operator == NK_COARSE_GALERKIN && lev.size() >= 2 && 2 * cur.Nx * cur.Ny <= GDENSE_STOP) break;
             int cxh = coarsening(cur.Nx, grid->halo, o->minimum_coarse_size);
             int cyh = coarsening(cur.Ny, grid->halo, o->minimum_coarse_size);
             if (!cxh && !cyh) break;
@@ -1205,10 +1283,14 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
         if (l > 0) b += 4 * sizeof(double) * p;
         b += p + 9 * sizeof(double) * p;                                      // activity flags, FP64 vectors
         if (mixed) b += (4 + 9) * sizeof(float) * p;
-        if (o->coarse_operator == NK_COARSE_GALERKIN) b += GS_ALLOC * (mixed ? sizeof(float) : sizeof(double)) * p;
+        if (o->coarse_operator == NK_COARSE_GALERKIN) {
+            const size_t es = mixed ? sizeof(float) : sizeof(double);
+            b += GS_ALLOC * es * p;
+            if (wants_gcst(lev[l], (int)l, es)) b += (size_t)(lev[l].Nx + 1) * (size_t)(lev[l].Ny + 1) * GS_ALLOC * es;
+        }
     }
     b += (24 + 16) * sizeof(double) * p0 + 3 * Ns * sizeof(double) * p0;
-    if (mixed) b += 12 * sizeof(float) * p0;
+    if (mixed) b += 14 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
     if (o->coarse_operator == NK_COARSE_GALERKIN) {

@@ -115,3 +115,23 @@ def test_mixed_galerkin_on_the_marine_sheet_matches_fp64():
         f"CG {out['fp64'][1]['total_linear_iterations']} → {out['mixed'][1]['total_linear_iterations']}, rel {e:.2e}")
     assert out["mixed"][1]["picard_iterations"] == out["fp64"][1]["picard_iterations"]
     assert e < 1e-8
+
+
+@pytest.mark.parametrize("precision,coarse", [(abi.PRECISION_MIXED, abi.COARSE_GALERKIN), (abi.PRECISION_FP64, abi.COARSE_GALERKIN),
+                                              (abi.PRECISION_MIXED, abi.COARSE_REDISCRETIZED)])
+def test_fused_first_sweeps_equal_separate_sweeps(monkeypatch, precision, coarse):
+    """The first two smoothing sweeps from zero run as one kernel (k_sweep_pre at the finest FP32 level, k_g_sweep_pre on
+    Galerkin levels); with NK_NO_PRE_FUSE=1 at creation they run as two. The solutions and iteration counts must be
+    identical bit for bit."""
+    out = []
+    for fuse in (True, False):
+        if fuse:
+            monkeypatch.delenv("NK_NO_PRE_FUSE", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_PRE_FUSE", "1")
+        m, info = channel(96, coarse, precision=precision, picard=2, linear=1e-8)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    say(f"[galerkin] fused vs separate first sweeps (precision {precision}, coarse {coarse}): CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
