@@ -177,6 +177,8 @@ typedef enum nk_field_id {
     NK_FIELD_ABAR,         /* ∂J/∂A                       (Center, Center[, σ]) like NK_FIELD_A */
     NK_FIELD_USIGMABAR,    /* adjoint input ∂J/∂uσ (DIVA) (Face, Center, σ)   */
     NK_FIELD_VSIGMABAR,    /* ∂J/∂vσ                      (Center, Face, σ)   */
+    NK_FIELD_ENTHALPY,     /* enthalpy E (J/kg), input of nk_thermal_diagnostics (Center, Center, σ) */
+    NK_FIELD_WATER_FRACTION, /* liquid water fraction ω    (Center, Center, σ)   */
     NK_FIELD_COUNT
 } nk_field_id;
 
@@ -269,6 +271,11 @@ nk_status nk_compute_velocity_profile(nk_solver *solver);
  * (src/Models/ice_sheet_model_methods.jl:47-57, src/Thermodynamics/column_energy_solve.jl:105-117,
  * src/Rheologies/arrhenius_rate_factor.jl:102-110) with the reference's default parameters. */
 nk_status nk_rate_factor_from_temperature(nk_solver *solver);
+/* update_thermal_diagnostics! (src/Thermodynamics/column_energy_solve.jl:85-117) with the reference's default
+ * parameters: TEMPERATURE = min(Tₚₘ, Tᵣ + E/cᵢ) and WATER_FRACTION = max(0, E − Eₚₘ)/L from ENTHALPY at each layer's
+ * pressure (enthalpy_conversions.jl), then A = arrhenius_rate_factor(T − Tₚₘ + T₀, ω) on the σ grid
+ * (NK_RATE_FACTOR_3D). */
+nk_status nk_thermal_diagnostics(nk_solver *solver);
 
 /* Fixed-point adjoint of the converged SSA Picard solve (the EnzymeRules reverse rule of picard_solve!,
  * ext/NunatakEnzymeExt.jl:175-240, Christianson 1994), hand-written. Inputs: the converged solution in (U, V) and the

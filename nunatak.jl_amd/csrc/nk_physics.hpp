@@ -313,29 +313,58 @@ __global__ __launch_bounds__(NT) void k_extend(KG g, int Ns, long plane, const d
 // Arrhenius rate factor from temperature through the enthalpy round trip
 // (src/Thermodynamics/enthalpy_conversions.jl:116-174, src/Thermodynamics/column_energy_solve.jl:105-117,
 //  src/Rheologies/arrhenius_rate_factor.jl:102-110), default parameters of the reference.
+// Defaults of EnthalpyThermodynamics (src/Thermodynamics/enthalpy_thermodynamics.jl:96-105) and ArrheniusRateFactor
+// (src/Rheologies/arrhenius_rate_factor.jl:60-69).
+struct Thermo {
+    static constexpr double ci = 2009.0, L = 3.34e5, be = 7.9e-8, T0 = 273.15, Tr = 223.15;
+    static constexpr double Ac = 3.61e-13, Aw = 1.73e3, Qc = 6.0e4, Qw = 13.9e4, Tt = 263.15, Tw = 1.0, R = 8.31441,
+                            cw = 181.25, wmax = 0.01;
+};
+// temperature / water_fraction of enthalpy E at pressure p (enthalpy_conversions.jl:116-152), then
+// arrhenius_rate_factor(T − Tₚₘ + T₀, ω) (arrhenius_rate_factor.jl:102-110, column_energy_solve.jl:106-111)
+__device__ __forceinline__ void thermal_state(double E, double pr, double &Tk, double &w, double &A) {
+    using C = Thermo;
+    Tk = jmin(C::T0 - C::be * pr, C::Tr + E / C::ci);
+    const double Epm = C::ci * ((C::T0 - C::be * pr) - C::Tr);
+    w = jmax(0.0, E - Epm) / C::L;
+    const double Tp = Tk - (C::T0 - C::be * pr) + C::T0;
+    const double lc = log(C::Ac) - C::Qc / (C::R * Tp);
+    const double lw = log(C::Aw) - C::Qw / (C::R * Tp);
+    const double x = jclamp((Tp - C::Tt) / C::Tw + 1.0 / 2.0, 0.0, 1.0);
+    const double s = x * x * (3.0 - 2.0 * x);
+    const double soft = 1.0 + C::cw * jmin(w, C::wmax);
+    A = exp((1.0 - s) * lc + s * lw) * soft;
+}
+
+// A from TEMPERATURE through the enthalpy round trip of set!(model; T) (ice_sheet_model_methods.jl:47-57)
 __global__ __launch_bounds__(NT) void k_rate_factor(KG g, Sig sg, int Ns, long plane, double rig, const double *__restrict__ T,
                                                     const double *__restrict__ H, double *__restrict__ A) {
     NK_IJ
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
-    const double ci = 2009.0, L = 3.34e5, be = 7.9e-8, T0 = 273.15, Tr = 223.15;
-    const double Ac = 3.61e-13, Aw = 1.73e3, Qc = 6.0e4, Qw = 13.9e4, Tt = 263.15, Tw = 1.0, R = 8.31441, cw = 181.25, wmax = 0.01;
+    using C = Thermo;
     for (int k = 0; k < Ns; k++) {
         const long ok = o + (long)k * plane;
         const double pr = rig * H[o] * (1.0 - sg.zc[k]);
-        const double Tpm = T0 - be * pr;
+        const double Tpm = C::T0 - C::be * pr;
         const double Tc = jmin(T[ok], Tpm);
-        const double E = ci * (Tc - Tr) + 0.0 * L;
-        const double Tk = jmin(T0 - be * pr, Tr + E / ci);
-        const double Epm = ci * ((T0 - be * pr) - Tr);
-        const double w = jmax(0.0, E - Epm) / L;
-        const double Tp = Tk - (T0 - be * pr) + T0;
-        const double lc = log(Ac) - Qc / (R * Tp);
-        const double lw = log(Aw) - Qw / (R * Tp);
-        const double x = jclamp((Tp - Tt) / Tw + 1.0 / 2.0, 0.0, 1.0);
-        const double s = x * x * (3.0 - 2.0 * x);
-        const double soft = 1.0 + cw * jmin(w, wmax);
-        A[ok] = exp((1.0 - s) * lc + s * lw) * soft;
+        const double E = C::ci * (Tc - C::Tr) + 0.0 * C::L;
+        double Tk, w;
+        thermal_state(E, pr, Tk, w, A[ok]);
+    }
+}
+
+// update_thermal_diagnostics! (column_energy_solve.jl:85-117): T, ω and A from the enthalpy E of every layer
+__global__ __launch_bounds__(NT) void k_thermal_diagnostics(KG g, Sig sg, int Ns, long plane, double rig, const double *__restrict__ E,
+                                                            const double *__restrict__ H, double *__restrict__ T,
+                                                            double *__restrict__ W, double *__restrict__ A) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    for (int k = 0; k < Ns; k++) {
+        const long ok = o + (long)k * plane;
+        const double pr = rig * H[o] * (1.0 - sg.zc[k]);
+        thermal_state(E[ok], pr, T[ok], W[ok], A[ok]);
     }
 }
 

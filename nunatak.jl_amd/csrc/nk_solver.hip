@@ -30,7 +30,8 @@ const LocInfo LOC[NK_FIELD_COUNT] = {
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0},
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0},
     {1, 0, 0}, {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0},
-    {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2}, {1, 0, 1}, {0, 1, 1},
+    {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2}, {1, 0, 1}, {0, 1, 1}, {0, 0, 1},
+    {0, 0, 1},
 };
 static_assert(sizeof(LOC) / sizeof(LOC[0]) == NK_FIELD_COUNT, "one location per field id");
 
@@ -1562,6 +1563,17 @@ nk_status nk_update_velocities(nk_solver *s, int32_t geometry, nk_info *info) {
     if (st == NK_OK) st = picard(s, info);
     if (st == NK_OK) st = post_process(s);
     return t.done(st);
+}
+
+nk_status nk_thermal_diagnostics(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    if (s->ph.rate_factor_depth != NK_RATE_FACTOR_3D) { s->err = "the rate factor must be on the σ grid (NK_RATE_FACTOR_3D)"; return NK_INVALID_ARGUMENT; }
+    k_thermal_diagnostics<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(
+        s->g, sig_tables(s), s->Ns, s->plane, s->ph.ice_density * s->ph.gravitational_acceleration, s->f[NK_FIELD_ENTHALPY],
+        s->f[NK_FIELD_H], s->f[NK_FIELD_TEMPERATURE], s->f[NK_FIELD_WATER_FRACTION], s->f[NK_FIELD_A]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
 }
 
 nk_status nk_rate_factor_from_temperature(nk_solver *s) {

@@ -63,7 +63,7 @@ end
 # field ids (nk_field_id)
 const U, V, H, BED, S, PHI, PHIU, PHIV, N, MU, TAUU, TAUV, ETAH, ETAHFF, BETAU, BETAV, F2, ETASIGMA,
       BETABU, BETABV, UB, VB, USIGMA, VSIGMA, A, C, TEMPERATURE, PU, PV, QU, QV,
-      UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR, USIGMABAR, VSIGMABAR = 0:38
+      UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR, USIGMABAR, VSIGMABAR, ENTHALPY, WATER_FRACTION = 0:40
 
 control(c::Convergence) = NKControl(0, c.maxiter, 0, 0, c.reltol, c.abstol)
 control(c::FixedIterations) = NKControl(1, c.iterations, c.iterations, 0, 0.0, 0.0)

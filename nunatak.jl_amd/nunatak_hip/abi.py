@@ -27,7 +27,7 @@ PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE = 0, 1, 2, 3
 FIELDS = ["U", "V", "H", "BED", "S", "PHI", "PHIU", "PHIV", "N", "MU", "TAUU", "TAUV", "ETAH", "ETAHFF",
           "BETAU", "BETAV", "F2", "ETASIGMA", "BETABU", "BETABV", "UB", "VB", "USIGMA", "VSIGMA", "A", "C",
           "TEMPERATURE", "PU", "PV", "QU", "QV", "UBAR", "VBAR", "LAMBDAU", "LAMBDAV", "CBAR", "ABAR", "USIGMABAR",
-          "VSIGMABAR"]
+          "VSIGMABAR", "ENTHALPY", "WATER_FRACTION"]
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 
 # (x location, y location, on the σ grid); "f" = Face, "c" = Center.
@@ -42,7 +42,8 @@ LOCATIONS = {
     "TEMPERATURE": ("c", "c", True), "PU": ("f", "c", False), "PV": ("c", "f", False),
     "QU": ("f", "c", False), "QV": ("c", "f", False), "UBAR": ("f", "c", False), "VBAR": ("c", "f", False),
     "LAMBDAU": ("f", "c", False), "LAMBDAV": ("c", "f", False), "CBAR": ("c", "c", False), "ABAR": ("c", "c", None),
-    "USIGMABAR": ("f", "c", True), "VSIGMABAR": ("c", "f", True),
+    "USIGMABAR": ("f", "c", True), "VSIGMABAR": ("c", "f", True), "ENTHALPY": ("c", "c", True),
+    "WATER_FRACTION": ("c", "c", True),
 }
 
 

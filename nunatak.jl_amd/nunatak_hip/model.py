@@ -21,6 +21,7 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_fill_field", "nk_get_classes", "nk_field_device_pointer", "nk_stream", "nk_synchronize",
            "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients", "nk_apply_operator",
            "nk_precondition", "nk_solve", "nk_picard_solve", "nk_update_velocities", "nk_compute_velocity_profile",
+           "nk_thermal_diagnostics",
            "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint", "nk_device_bytes",
            "nk_estimate_device_bytes", "nk_server_create", "nk_server_destroy", "nk_server_last_error",
            "nk_server_submit", "nk_server_poll", "nk_server_wait"]
@@ -61,7 +62,8 @@ def load(path=LIB_PATH):
     L.nk_stream.argtypes = [P]
     L.nk_stream.restype = P
     for name in ("nk_synchronize", "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients",
-                 "nk_apply_operator", "nk_precondition", "nk_rate_factor_from_temperature", "nk_compute_velocity_profile"):
+                 "nk_apply_operator", "nk_precondition", "nk_rate_factor_from_temperature", "nk_compute_velocity_profile",
+                 "nk_thermal_diagnostics"):
         getattr(L, name).argtypes = [P]
     L.nk_solve.argtypes = [P, ct.POINTER(abi.Info)]
     L.nk_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
@@ -183,6 +185,9 @@ class HIPModel:
 
     def precondition(self):
         self._check(self._L.nk_precondition(self._h), "nk_precondition")
+
+    def thermal_diagnostics(self):
+        self._check(self._L.nk_thermal_diagnostics(self._h), "nk_thermal_diagnostics")
 
     def rate_factor_from_temperature(self):
         self._check(self._L.nk_rate_factor_from_temperature(self._h), "nk_rate_factor_from_temperature")

@@ -45,6 +45,7 @@ static const nko_loc LOCS[NK_FIELD_COUNT] = {
     [NK_FIELD_QV] = {C, F, 0},     [NK_FIELD_UBAR] = {F, C, 0},    [NK_FIELD_VBAR] = {C, F, 0},
     [NK_FIELD_LAMBDAU] = {F, C, 0},[NK_FIELD_LAMBDAV] = {C, F, 0}, [NK_FIELD_CBAR] = {C, C, 0},
     [NK_FIELD_ABAR] = {C, C, 0},   [NK_FIELD_USIGMABAR] = {F, C, 1}, [NK_FIELD_VSIGMABAR] = {C, F, 1},
+    [NK_FIELD_ENTHALPY] = {C, C, 1}, [NK_FIELD_WATER_FRACTION] = {C, C, 1},
 };
 
 /* ------------------------------------------------------------------------------------------------------------ */
@@ -1255,6 +1256,34 @@ EXPORT void nko_rate_factor_from_temperature(nko_model *m) {
                 A[o] = nko_arrhenius_rate_factor(Tp, w);
             }
     fill_field_halo(m, NK_FIELD_A);
+}
+
+/* update_thermal_diagnostics! (src/Thermodynamics/column_energy_solve.jl:85-117): _compute_thermal_diagnostics!
+ * (:113-117) then _compute_rate_factor! (:106-111) from the enthalpy field. */
+EXPORT void nko_thermal_diagnostics(nko_model *m) {
+    const og *g = &m->g;
+    const double *E = m->f[NK_FIELD_ENTHALPY], *H = m->f[NK_FIELD_H];
+    double *T = m->f[NK_FIELD_TEMPERATURE], *W = m->f[NK_FIELD_WATER_FRACTION], *A = m->f[NK_FIELD_A];
+    double rg = m->ph.ice_density * m->ph.gravitational_acceleration;
+    for (int k = 1; k <= m->Ns; k++)
+        for (int j = 1; j <= g->Ny; j++)
+            for (int i = 1; i <= g->Nx; i++) {
+                size_t o = (size_t)(k - 1) * g->plane + IDX(g, i, j);
+                double p = rg * AT(g, H, i, j) * (1.0 - m->zc[k]);   /* layer_pressure (enthalpy_conversions.jl:171-174) */
+                T[o] = e_temperature(E[o], p);
+                W[o] = e_water(E[o], p);
+                A[o] = nko_arrhenius_rate_factor(T[o] - pm_temperature(p) + E_T0, W[o]);
+            }
+    fill_field_halo(m, NK_FIELD_TEMPERATURE);
+    fill_field_halo(m, NK_FIELD_WATER_FRACTION);
+    fill_field_halo(m, NK_FIELD_A);
+}
+
+/* enthalpy (enthalpy_conversions.jl:159-162) and its inverses at pressure p, for tests */
+EXPORT void nko_enthalpy_conversions(double T, double w, double p, double *E, double *T_of_E, double *w_of_E) {
+    *E = e_enthalpy(T, w, p);
+    *T_of_E = e_temperature(*E, p);
+    *w_of_E = e_water(*E, p);
 }
 
 /* σ-grid metrics for tests */

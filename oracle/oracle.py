@@ -43,7 +43,8 @@ def lib():
         L.nko_get_field.argtypes = [P, I, dp, ct.c_int64, ct.c_int64]
         L.nko_get_classes.argtypes = [P, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64]
         for name in ["nko_update_geometry", "nko_compute_driving_stress", "nko_update_coefficients",
-                     "nko_apply_operator", "nko_precondition", "nko_rate_factor_from_temperature"]:
+                     "nko_apply_operator", "nko_precondition", "nko_rate_factor_from_temperature",
+                     "nko_thermal_diagnostics"]:
             getattr(L, name).argtypes = [P]
         L.nko_solve.argtypes = [P, ct.POINTER(abi.Info)]
         L.nko_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
@@ -53,6 +54,7 @@ def lib():
         L.nko_mg_transfer.argtypes = [P, I, dp, dp, dp, dp]
         L.nko_mg_transfer.restype = I
         L.nko_sigma_metrics.argtypes = [P, dp, dp, dp, dp]
+        L.nko_enthalpy_conversions.argtypes = [D, D, D, dp, dp, dp]
         for name, args in [("nko_ice_mask_value", [D, D, D]), ("nko_segment_grounded_fraction", [D, D]),
                            ("nko_triangle_grounded_fraction", [D, D, D]), ("nko_arrhenius_rate_factor", [D, D]),
                            ("nko_basal_drag", [D, D, D, I, D, D])]:
@@ -134,6 +136,9 @@ class OracleModel:
     def rate_factor_from_temperature(self):
         lib().nko_rate_factor_from_temperature(self._h)
 
+    def thermal_diagnostics(self):
+        lib().nko_thermal_diagnostics(self._h)
+
     def solve(self):
         info = abi.Info()
         lib().nko_solve(self._h, ct.byref(info))
@@ -188,6 +193,13 @@ def triangle_grounded_fraction(f1, f2, f3):
 
 def arrhenius_rate_factor(T, omega):
     return lib().nko_arrhenius_rate_factor(T, omega)
+
+
+def enthalpy_conversions(T, omega, p):
+    """(E = enthalpy(T, ω, p), temperature(E, p), water_fraction(E, p))"""
+    E, Tb, wb = ct.c_double(), ct.c_double(), ct.c_double()
+    lib().nko_enthalpy_conversions(T, omega, p, ct.byref(E), ct.byref(Tb), ct.byref(wb))
+    return E.value, Tb.value, wb.value
 
 
 def basal_drag(U, C, N, law, m, alpha2):

@@ -278,3 +278,26 @@ def test_graph_replays_match_eager_launches(precision):
         assert res[0][k][2] == res[1][k][2]
         assert np.array_equal(res[0][k][0], res[1][k][0]) and np.array_equal(res[0][k][1], res[1][k][1])
         assert np.array_equal(res[1][k][0], res[1][0][0])
+
+
+def test_thermal_diagnostics_parity():
+    """nk_thermal_diagnostics against the oracle on an enthalpy field with cold and temperate layers: T and ω bit for
+    bit (arithmetic, min and max only), A within 1e-13 (device exp/log)."""
+    say("thermal diagnostics")
+    rng = np.random.default_rng(11)
+    Ns = 10
+    g = grid(64, 48, Lx=64e3, Ly=48e3, Nsigma=Ns)
+    phys = abi.default_physics(rate_factor_depth=abi.RATE_FACTOR_3D)
+    h, o = pair(g, phys, abi.default_options())
+    H = 3000 * rng.random((48, 64))
+    E = 2009.0 * (220.0 + 55 * rng.random((Ns, 48, 64)) - 223.15)
+    E[rng.random(E.shape) < 0.3] += 0.03 * 3.34e5 * rng.random()
+    for m in (h, o):
+        m.set("H", H)
+        m.set("ENTHALPY", E)
+        m.thermal_diagnostics()
+    for name in ("TEMPERATURE", "WATER_FRACTION"):
+        assert ulps(h.get(name), o.get(name)) == 0, name
+    a, b = h.get("A"), o.get("A")
+    assert np.allclose(a, b, rtol=1e-13, atol=0)
+    assert np.any(o.get("WATER_FRACTION") > 0)

@@ -330,3 +330,57 @@ def test_rate_factor_from_cold_temperature_is_arrhenius():
         Tp = T[k, 0, 0] - (273.15 - 7.9e-8 * p) + 273.15
         assert math.isclose(A[k, 0, 0], O.arrhenius_rate_factor(Tp, 0.0), rel_tol=1e-14)
         assert Tp > T[k, 0, 0]
+
+
+# ---- update_thermal_diagnostics!: doctest enthalpy_conversions.jl:127-139, column_energy_solve.jl:85-117 ------------
+
+def test_enthalpy_conversions_doctest_and_temperate_branch():
+    # doctest: E = enthalpy(263.15, 0, 1e6) → (temperature, water_fraction) = (263.15, 0.0)
+    E, T, w = O.enthalpy_conversions(263.15, 0.0, 1e6)
+    assert (T, w) == (263.15, 0.0)
+    assert math.isclose(E, 2009.0 * (263.15 - 223.15), rel_tol=1e-15)
+    # temperate: T above Tₚₘ is capped, ω carries the latent heat; Tₚₘ = T₀ − β p
+    p = 5e6
+    Tpm = 273.15 - 7.9e-8 * p
+    E, T, w = O.enthalpy_conversions(280.0, 0.03, p)
+    assert T == Tpm
+    assert math.isclose(w, 0.03, rel_tol=1e-12)
+    assert math.isclose(E, 2009.0 * (Tpm - 223.15) + 0.03 * 3.34e5, rel_tol=1e-15)
+    # cold ice round trip keeps T (to rounding) and ω = 0
+    for T0 in (230.0, 250.0, 265.0):
+        E, T, w = O.enthalpy_conversions(T0, 0.0, p)
+        assert math.isclose(T, T0, rel_tol=1e-15) and w == 0.0
+
+
+def test_thermal_diagnostics_from_enthalpy():
+    """T, ω and A of every layer from an enthalpy field mixing cold and temperate ice: pointwise the conversions and
+    arrhenius_rate_factor(T − Tₚₘ + T₀, ω); for cold ice the same A as the temperature path (a cold round trip)."""
+    rng = np.random.default_rng(3)
+    Ns = 5
+    g = grid(4, 3, Lx=4e3, Ly=3e3, Nsigma=Ns)
+    m = model(g, masking=0, rate_factor_depth=abi.RATE_FACTOR_3D)
+    H = 500 + 1500 * rng.random((3, 4))
+    m.set("H", H)
+    _, zc, _, _ = m.sigma_metrics()
+    Ecold = 2009.0 * (230.0 + 40 * rng.random((Ns, 3, 4)) - 223.15)
+    temperate = rng.random((Ns, 3, 4)) < 0.4
+    E = np.where(temperate, 2009.0 * (273.15 - 223.15) + 0.02 * 3.34e5 * rng.random((Ns, 3, 4)), Ecold)
+    m.set("ENTHALPY", E)
+    m.thermal_diagnostics()
+    T, W, A = m.get("TEMPERATURE"), m.get("WATER_FRACTION"), m.get("A")
+    for k in range(Ns):
+        for j in range(3):
+            for i in range(4):
+                p = 918 * 9.81 * H[j, i] * (1 - zc[k])
+                Tpm = 273.15 - 7.9e-8 * p
+                assert T[k, j, i] == min(Tpm, 223.15 + E[k, j, i] / 2009.0)
+                assert W[k, j, i] == max(0.0, E[k, j, i] - 2009.0 * (Tpm - 223.15)) / 3.34e5
+                assert A[k, j, i] == O.arrhenius_rate_factor(T[k, j, i] - Tpm + 273.15, W[k, j, i])
+    assert np.any(W > 0) and np.any(W == 0)
+    cold_T = T.copy()
+    m2 = model(g, masking=0, rate_factor_depth=abi.RATE_FACTOR_3D)
+    m2.set("H", H)
+    m2.set("TEMPERATURE", np.where(temperate, 250.0, cold_T))
+    m2.rate_factor_from_temperature()
+    A2 = m2.get("A")
+    assert np.allclose(A2[~temperate], A[~temperate], rtol=1e-13)
