@@ -147,6 +147,46 @@ def serving_leg(n, rate_hz, slots, serial_sample):
                         "(FP32 Galerkin V-cycle), MG-PCG linear 1e-8, Picard 1e-6"}
 
 
+def transport_leg(N, steps):
+    """Thickness transport at N² on the device: `steps` split Runge-Kutta steps (cache, then three stages of tendency
+    and substep) with prescribed velocities, upwind face thickness and sources, no geometry update between stages, so
+    the time is the transport kernels' (each entry point synchronises). Algorithmic bytes per step: three tendencies
+    (u, v, H, a, m, φ, mᵍ read, G written: 8 planes) and three substeps (H⁻, G, Hᶜ read, H, Hᶜ written: 5 planes) and
+    the cache copy (2 planes), 8 B per cell each."""
+    L = N * 1e3
+    g = abi.default_grid(N, N, 1e3, 1e3, topology=(abi.BOUNDED, abi.PERIODIC), halo=4, Nsigma=2)
+    m = HIPModel(g, abi.default_physics(masking=0, grounding=abi.GROUNDING_SUBGRID), abi.default_options())
+    x = abi.node_coordinates(g, "H")[0]
+    xu = abi.node_coordinates(g, "U")[0]
+    m.set("H", 100 + 500 * np.exp(-((x - L / 4) / (L / 16)) ** 2))
+    m.set("BED", np.full_like(x, -300.0))
+    m.set("U", 1e-5 * np.sin(np.pi * xu / L))
+    m.set("SMB", np.full_like(x, 3e-9))
+    m.set("BASAL_MELT", 1e-9 * x / L)
+    m.update_geometry()
+    dt = 0.5 * m.cell_advection_timescale()
+
+    def step():
+        m.cache_thickness()
+        for k, beta in enumerate((3.0, 2.0, 1.0)):
+            m.compute_thickness_tendency(abi.ADVECTION_UPWIND1)
+            m.thickness_substep(dt / beta, k == 2)
+
+    step()
+    m.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    m.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    bytes_step = 8.0 * N * N * (3 * 8 + 3 * 5 + 2)
+    log(f"[bench] transport {N}x{N}: {ms:.3f} ms per RK3 step, {bytes_step / (ms * 1e-3) / 1e9:.0f} GB/s (algorithmic, wall)")
+    m.close()
+    return {"workload": f"thickness transport {N}x{N}, (Bounded, Periodic), UpwindBiased(order = 1), SplitRungeKutta3, "
+                        "prescribed velocities, sources, no geometry update", "ms_per_step": ms,
+            "algorithmic_bytes_per_step": bytes_step, "GB_s_wall": bytes_step / (ms * 1e-3) / 1e9, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,12 +210,17 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-serial", type=int, default=12, help="requests timed serially in FP64 for the baseline")
+    ap.add_argument("--transport", type=int, default=20, help="RK3 thickness steps timed in the transport leg (0: skip)")
+    ap.add_argument("--transport-only", action="store_true", help="run only the transport leg; prints its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the velocity solve of one grid is not "
               "sharded across GPUs in this build.", file=sys.stderr)
         sys.exit(2)
     N, K, Wm = args.size, args.steps, args.warmup
+    if args.transport_only:
+        print(json.dumps(transport_leg(N, max(args.transport, 1))), flush=True)
+        return
     os.environ.setdefault("NK_VERBOSE", "1")  # one library progress line per Picard iteration
     if args.stream_curve:
         import torch
@@ -288,6 +333,8 @@ def main():
         out["serving"] = serving_leg(args.serve, args.serve_rate, args.serve_slots, min(args.serve_serial, args.serve))
         out["served_p50_ms"] = out["serving"]["p50_ms"]
         out["served_p99_ms"] = out["serving"]["p99_ms"]
+    if args.transport > 0:
+        out["transport"] = transport_leg(N, args.transport)
     print(json.dumps(out), flush=True)
 
 

@@ -179,6 +179,12 @@ typedef enum nk_field_id {
     NK_FIELD_VSIGMABAR,    /* ∂J/∂vσ                      (Center, Face, σ)   */
     NK_FIELD_ENTHALPY,     /* enthalpy E (J/kg), input of nk_thermal_diagnostics (Center, Center, σ) */
     NK_FIELD_WATER_FRACTION, /* liquid water fraction ω    (Center, Center, σ)   */
+    NK_FIELD_SMB,          /* surface mass balance a (m/s) (Center, Center)    */
+    NK_FIELD_BASAL_MELT,   /* basal melt rate m (m/s)     (Center, Center)    */
+    NK_FIELD_GROUNDED_MELT,/* grounded melt mᵍ (m/s; zero unless coupled) (Center, Center) */
+    NK_FIELD_HPREV,        /* H⁻: thickness at the start of the time step (Center, Center) */
+    NK_FIELD_GH,           /* thickness tendency G        (Center, Center)    */
+    NK_FIELD_HCLIP,        /* clipped thickness Hᶜ (thickness added by the clip at zero) (Center, Center) */
     NK_FIELD_COUNT
 } nk_field_id;
 
@@ -276,6 +282,20 @@ nk_status nk_rate_factor_from_temperature(nk_solver *solver);
  * pressure (enthalpy_conversions.jl), then A = arrhenius_rate_factor(T − Tₚₘ + T₀, ω) on the σ grid
  * (NK_RATE_FACTOR_3D). */
 nk_status nk_thermal_diagnostics(nk_solver *solver);
+
+/* ---- Thickness transport: the step after the velocity solve in time_step! ---------------------------------------- */
+/* Advection schemes of the thickness flux (Oceananigans UpwindBiased(order = 1) and Centered(order = 2)). */
+enum { NK_ADVECTION_UPWIND1 = 0, NK_ADVECTION_CENTERED2 = 1 };
+/* compute_thickness_tendency! (src/Models/time_stepping.jl:34-63): GH = a − m·(1 − φ if grounding else 1) − mᵍ
+ * − ∇·(ū H) with the face thickness of the scheme; U, V are the advecting depth-averaged velocities. */
+nk_status nk_compute_thickness_tendency(nk_solver *solver, int32_t advection);
+/* _thickness_substep! (time_stepping.jl:65-72): H★ = HPREV + Δτ·GH; H = max(0, H★); in the final stage
+ * HCLIP += max(0, −H★). */
+nk_status nk_thickness_substep(nk_solver *solver, double dtau, int32_t final_stage);
+/* cache_current_fields! (time_stepping.jl:1-5): HPREV = H. */
+nk_status nk_cache_thickness(nk_solver *solver);
+/* cell_advection_timescale(model) (time_stepping.jl:78, Oceananigans): min over cells of 1/(|u|/Δx + |v|/Δy). */
+nk_status nk_cell_advection_timescale(nk_solver *solver, double *timescale);
 
 /* Fixed-point adjoint of the converged SSA Picard solve (the EnzymeRules reverse rule of picard_solve!,
  * ext/NunatakEnzymeExt.jl:175-240, Christianson 1994), hand-written. Inputs: the converged solution in (U, V) and the

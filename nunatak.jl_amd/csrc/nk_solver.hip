@@ -17,6 +17,7 @@
 #include "nk_physics.hpp"
 #include "nk_adjoint.hpp"
 #include "nk_galerkin.hpp"
+#include "nk_transport.hpp"
 #include "nk_internal.hpp"
 
 using namespace nk;
@@ -31,7 +32,7 @@ const LocInfo LOC[NK_FIELD_COUNT] = {
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0},
     {1, 0, 0}, {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0},
     {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2}, {1, 0, 1}, {0, 1, 1}, {0, 0, 1},
-    {0, 0, 1},
+    {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
 };
 static_assert(sizeof(LOC) / sizeof(LOC[0]) == NK_FIELD_COUNT, "one location per field id");
 
@@ -1563,6 +1564,54 @@ nk_status nk_update_velocities(nk_solver *s, int32_t geometry, nk_info *info) {
     if (st == NK_OK) st = picard(s, info);
     if (st == NK_OK) st = post_process(s);
     return t.done(st);
+}
+
+nk_status nk_compute_thickness_tendency(nk_solver *s, int32_t advection) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    if (advection != NK_ADVECTION_UPWIND1 && advection != NK_ADVECTION_CENTERED2) {
+        s->err = "thickness advection: NK_ADVECTION_UPWIND1 or NK_ADVECTION_CENTERED2";
+        return NK_INVALID_ARGUMENT;
+    }
+    double **f = s->f;
+    k_thickness_tendency<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(
+        s->g, advection, s->ph.grounding != NK_GROUNDING_NONE ? 1 : 0, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_H],
+        f[NK_FIELD_SMB], f[NK_FIELD_BASAL_MELT], f[NK_FIELD_PHI], f[NK_FIELD_GROUNDED_MELT], f[NK_FIELD_GH]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_thickness_substep(nk_solver *s, double dtau, int32_t final_stage) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    double **f = s->f;
+    k_thickness_substep<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(s->g, dtau, final_stage ? 1 : 0, f[NK_FIELD_HPREV],
+                                                                                   f[NK_FIELD_GH], f[NK_FIELD_H], f[NK_FIELD_HCLIP]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_cache_thickness(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    k_copy_plane<<<blocks_for(s->g.Nx + 1, s->g.Ny + 1), THREADS, 0, s->stream>>>(s->g, s->f[NK_FIELD_H], s->f[NK_FIELD_HPREV]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_cell_advection_timescale(nk_solver *s, double *timescale) {
+    if (!s || !timescale) return NK_INVALID_ARGUMENT;
+    const dim3 B = blocks_for(s->g.Nx, s->g.Ny);
+    const size_t nb = (size_t)B.x * B.y;
+    k_advection_timescale<<<B, THREADS, 0, s->stream>>>(s->g, s->f[NK_FIELD_U], s->f[NK_FIELD_V], s->partials);
+    CHK(launch_check(s));
+    std::vector<double> mins(nb);
+    HIPCHK(hipMemcpyAsync(mins.data(), s->partials, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double t = INFINITY;
+    for (double b : mins) t = std::fmin(t, b);
+    *timescale = t;
+    return NK_OK;
 }
 
 nk_status nk_thermal_diagnostics(nk_solver *s) {

@@ -63,7 +63,8 @@ end
 # field ids (nk_field_id)
 const U, V, H, BED, S, PHI, PHIU, PHIV, N, MU, TAUU, TAUV, ETAH, ETAHFF, BETAU, BETAV, F2, ETASIGMA,
       BETABU, BETABV, UB, VB, USIGMA, VSIGMA, A, C, TEMPERATURE, PU, PV, QU, QV,
-      UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR, USIGMABAR, VSIGMABAR, ENTHALPY, WATER_FRACTION = 0:40
+      UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR, USIGMABAR, VSIGMABAR, ENTHALPY, WATER_FRACTION,
+      SMB, BASAL_MELT, GROUNDED_MELT, HPREV, GH, HCLIP = 0:46
 
 control(c::Convergence) = NKControl(0, c.maxiter, 0, 0, c.reltol, c.abstol)
 control(c::FixedIterations) = NKControl(1, c.iterations, c.iterations, 0, 0.0, 0.0)
@@ -208,8 +209,25 @@ function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalan
     return info.picard_iterations
 end
 
+# One split Runge-Kutta stage of the thickness on the device (rk_substep!, src/Models/time_stepping.jl:15-72) for a
+# model whose velocities are current: the tendency with UpwindBiased(order = 1) (advection = 0) or Centered(order = 2)
+# (advection = 1), then H = max(0, H⁻ + Δτ G), the clip accumulated in the final stage.
+function hip_rk_substep!(model, solver::HIPStressBalanceSolver, Δτ, final_stage; advection = 0)
+    geometry, vel = model.geometry, model.velocities
+    set_field!(solver, NunatakHIP.H, geometry.H); set_field!(solver, HPREV, model.timestepper.Ψ⁻.H)
+    set_field!(solver, U, vel.u); set_field!(solver, V, vel.v); set_field!(solver, PHI, geometry.φ)
+    set_field!(solver, SMB, model.surface_mass_balance); set_field!(solver, BASAL_MELT, model.basal_melt_rate)
+    set_field!(solver, HCLIP, geometry.clipped_thickness)
+    check(ccall((:nk_compute_thickness_tendency, libnunatak_hip), Cint, (Ptr{Cvoid}, Int32), solver.handle, advection),
+          solver.handle, "nk_compute_thickness_tendency")
+    check(ccall((:nk_thickness_substep, libnunatak_hip), Cint, (Ptr{Cvoid}, Float64, Int32), solver.handle, Δτ, final_stage),
+          solver.handle, "nk_thickness_substep")
+    get_field!(geometry.H, solver, NunatakHIP.H); get_field!(geometry.clipped_thickness, solver, HCLIP)
+    return nothing
+end
+
 Base.summary(s::HIPStressBalanceSolver) = "HIPStressBalanceSolver(libnunatak_hip, iteration = $(s.iteration))"
 
-export HIPStressBalanceSolver, hip_velocity_adjoint!
+export HIPStressBalanceSolver, hip_velocity_adjoint!, hip_rk_substep!
 
 end # module

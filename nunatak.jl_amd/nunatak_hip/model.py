@@ -21,7 +21,8 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_fill_field", "nk_get_classes", "nk_field_device_pointer", "nk_stream", "nk_synchronize",
            "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients", "nk_apply_operator",
            "nk_precondition", "nk_solve", "nk_picard_solve", "nk_update_velocities", "nk_compute_velocity_profile",
-           "nk_thermal_diagnostics",
+           "nk_thermal_diagnostics", "nk_compute_thickness_tendency", "nk_thickness_substep", "nk_cache_thickness",
+           "nk_cell_advection_timescale",
            "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint", "nk_device_bytes",
            "nk_estimate_device_bytes", "nk_server_create", "nk_server_destroy", "nk_server_last_error",
            "nk_server_submit", "nk_server_poll", "nk_server_wait"]
@@ -63,13 +64,16 @@ def load(path=LIB_PATH):
     L.nk_stream.restype = P
     for name in ("nk_synchronize", "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients",
                  "nk_apply_operator", "nk_precondition", "nk_rate_factor_from_temperature", "nk_compute_velocity_profile",
-                 "nk_thermal_diagnostics"):
+                 "nk_thermal_diagnostics", "nk_cache_thickness"):
         getattr(L, name).argtypes = [P]
     L.nk_solve.argtypes = [P, ct.POINTER(abi.Info)]
     L.nk_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
     L.nk_update_velocities.argtypes = [P, I32, ct.POINTER(abi.Info)]
     L.nk_multigrid_levels.argtypes = [P, ct.POINTER(I32), ct.POINTER(I32), I32]
     L.nk_adjoint.argtypes = [P, ct.POINTER(abi.Control), ct.POINTER(abi.Info)]
+    L.nk_compute_thickness_tendency.argtypes = [P, I32]
+    L.nk_thickness_substep.argtypes = [P, D, I32]
+    L.nk_cell_advection_timescale.argtypes = [P, ct.POINTER(D)]
     L.nk_device_bytes.argtypes = [P, ct.POINTER(I64)]
     L.nk_estimate_device_bytes.argtypes = [ct.POINTER(abi.Grid), ct.POINTER(abi.Physics), ct.POINTER(abi.SolverOptions),
                                            ct.POINTER(I64)]
@@ -185,6 +189,20 @@ class HIPModel:
 
     def precondition(self):
         self._check(self._L.nk_precondition(self._h), "nk_precondition")
+
+    def compute_thickness_tendency(self, advection=abi.ADVECTION_UPWIND1):
+        self._check(self._L.nk_compute_thickness_tendency(self._h, advection), "nk_compute_thickness_tendency")
+
+    def thickness_substep(self, dtau, final_stage):
+        self._check(self._L.nk_thickness_substep(self._h, dtau, 1 if final_stage else 0), "nk_thickness_substep")
+
+    def cache_thickness(self):
+        self._check(self._L.nk_cache_thickness(self._h), "nk_cache_thickness")
+
+    def cell_advection_timescale(self):
+        t = ct.c_double()
+        self._check(self._L.nk_cell_advection_timescale(self._h, ct.byref(t)), "nk_cell_advection_timescale")
+        return t.value
 
     def thermal_diagnostics(self):
         self._check(self._L.nk_thermal_diagnostics(self._h), "nk_thermal_diagnostics")

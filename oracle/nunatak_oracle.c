@@ -46,6 +46,8 @@ static const nko_loc LOCS[NK_FIELD_COUNT] = {
     [NK_FIELD_LAMBDAU] = {F, C, 0},[NK_FIELD_LAMBDAV] = {C, F, 0}, [NK_FIELD_CBAR] = {C, C, 0},
     [NK_FIELD_ABAR] = {C, C, 0},   [NK_FIELD_USIGMABAR] = {F, C, 1}, [NK_FIELD_VSIGMABAR] = {C, F, 1},
     [NK_FIELD_ENTHALPY] = {C, C, 1}, [NK_FIELD_WATER_FRACTION] = {C, C, 1},
+    [NK_FIELD_SMB] = {C, C, 0},    [NK_FIELD_BASAL_MELT] = {C, C, 0}, [NK_FIELD_GROUNDED_MELT] = {C, C, 0},
+    [NK_FIELD_HPREV] = {C, C, 0},  [NK_FIELD_GH] = {C, C, 0},       [NK_FIELD_HCLIP] = {C, C, 0},
 };
 
 /* ------------------------------------------------------------------------------------------------------------ */
@@ -1284,6 +1286,67 @@ EXPORT void nko_enthalpy_conversions(double T, double w, double p, double *E, do
     *E = e_enthalpy(T, w, p);
     *T_of_E = e_temperature(*E, p);
     *w_of_E = e_water(*E, p);
+}
+
+/* ------------------------------------------------------------------------------------------------------------ */
+/* Thickness transport (src/Models/time_stepping.jl:1-72), flux form of Oceananigans' div_Uc (third-party: the      */
+/* Oceananigans 0.117 dependency is not in the reference tree; restated from its published finite-volume form)    */
+/* ------------------------------------------------------------------------------------------------------------ */
+
+/* face thickness: UpwindBiased(order = 1) takes the upwind cell (the right one unless the velocity is positive),
+ * Centered(order = 2) the mean of the two cells (halo cells hold the zero-gradient / periodic fill) */
+static double face_H(double left, double right, double vel, int adv) { return adv == 0 ? (vel > 0.0 ? left : right) : (left + right) / 2.0; }
+
+/* compute_thickness_tendency! / _compute_thickness_tendency! (time_stepping.jl:34-63) */
+EXPORT void nko_thickness_tendency(nko_model *m, int adv) {
+    const og *g = &m->g;
+    const double *u = m->f[NK_FIELD_U], *v = m->f[NK_FIELD_V], *H = m->f[NK_FIELD_H];
+    const double *a = m->f[NK_FIELD_SMB], *mm = m->f[NK_FIELD_BASAL_MELT], *phi = m->f[NK_FIELD_PHI], *mg = m->f[NK_FIELD_GROUNDED_MELT];
+    double *G = m->f[NK_FIELD_GH];
+    for (int j = 1; j <= g->Ny; j++)
+        for (int i = 1; i <= g->Nx; i++) {
+            double u0 = AT(g, u, i, j), u1 = AT(g, u, i + 1, j), v0 = AT(g, v, i, j), v1 = AT(g, v, i, j + 1);
+            double Fx0 = g->dy * u0 * face_H(AT(g, H, i - 1, j), AT(g, H, i, j), u0, adv);
+            double Fx1 = g->dy * u1 * face_H(AT(g, H, i, j), AT(g, H, i + 1, j), u1, adv);
+            double Fy0 = g->dx * v0 * face_H(AT(g, H, i, j - 1), AT(g, H, i, j), v0, adv);
+            double Fy1 = g->dx * v1 * face_H(AT(g, H, i, j), AT(g, H, i, j + 1), v1, adv);
+            double div = (1.0 / (g->dx * g->dy)) * ((Fx1 - Fx0) + (Fy1 - Fy0));
+            /* basal_melt (:51-52): (1 − φ) m with a grounding treatment; grounded_melt (:57-58): mᵍ */
+            double melt = m->ph.grounding != NK_GROUNDING_NONE ? (1.0 - AT(g, phi, i, j)) * AT(g, mm, i, j) : AT(g, mm, i, j);
+            double S = (AT(g, a, i, j) - melt) - AT(g, mg, i, j);
+            AT(g, G, i, j) = S - div;
+        }
+    fill_field_halo(m, NK_FIELD_GH);
+}
+
+/* _thickness_substep! (time_stepping.jl:65-72) */
+EXPORT void nko_thickness_substep(nko_model *m, double dtau, int final_stage) {
+    const og *g = &m->g;
+    const double *Hm = m->f[NK_FIELD_HPREV], *G = m->f[NK_FIELD_GH];
+    double *H = m->f[NK_FIELD_H], *Hc = m->f[NK_FIELD_HCLIP];
+    for (int j = 1; j <= g->Ny; j++)
+        for (int i = 1; i <= g->Nx; i++) {
+            double Hs = AT(g, Hm, i, j) + dtau * AT(g, G, i, j);
+            AT(g, H, i, j) = jmax(0.0, Hs);
+            AT(g, Hc, i, j) = AT(g, Hc, i, j) + (final_stage ? jmax(0.0, -Hs) : 0.0);
+        }
+    fill_field_halo(m, NK_FIELD_H);
+    fill_field_halo(m, NK_FIELD_HCLIP);
+}
+
+/* cache_current_fields! (time_stepping.jl:1-5) */
+EXPORT void nko_cache_thickness(nko_model *m) {
+    memcpy(m->f[NK_FIELD_HPREV], m->f[NK_FIELD_H], sizeof(double) * m->g.plane);
+}
+
+/* Oceananigans cell_advection_timescale: min over cells of 1/(|u|/Δx + |v|/Δy) */
+EXPORT double nko_cell_advection_timescale(const nko_model *m) {
+    const og *g = &m->g;
+    const double *u = m->f[NK_FIELD_U], *v = m->f[NK_FIELD_V];
+    double t = INFINITY;
+    for (int j = 1; j <= g->Ny; j++)
+        for (int i = 1; i <= g->Nx; i++) t = fmin(t, 1.0 / (fabs(AT(g, u, i, j)) / g->dx + fabs(AT(g, v, i, j)) / g->dy));
+    return t;
 }
 
 /* σ-grid metrics for tests */
