@@ -187,6 +187,56 @@ def transport_leg(N, steps):
             "algorithmic_bytes_per_step": bytes_step, "GB_s_wall": bytes_step / (ms * 1e-3) / 1e9, "steps": steps}
 
 
+def energy_leg(N, steps, Ns=10):
+    """Enthalpy step at N² × Nσ on the device: strain heating, then nk_energy_step (layer fluxes, basal state, column
+    system with explicit upwind transport, tridiagonal solve, drainage, basal water, thermal diagnostics), on a sheet
+    with flow, temperate layers and melt. Algorithmic bytes per step (8 B per value): strain heating reads u, v and
+    ησ and writes Φ (Nσ + 2 planes + Nσ); the assembly reads E, uσ, vσ, Φ (4Nσ planes) and 11 planes (H, Hⁿ, φ, m,
+    mᵍ, Hw, G, Tₛ, Tₒ, two drag pairs counted as one each) and writes a, b, c, f, W and the basal state (5Nσ + 5); the
+    solve reads a, b, c, f and the basal state, H, φ, Hw (4Nσ + 7) and writes E, t, T, ω, A, Hw, mᵍ, Hⁿ (5Nσ + 3)."""
+    rng = np.random.default_rng(9)
+    g = abi.default_grid(N, N, 1e3, 1e3, topology=(abi.BOUNDED, abi.PERIODIC), halo=4, Nsigma=Ns)
+    phys = abi.default_physics(masking=0, grounding=abi.GROUNDING_SUBGRID, rate_factor_depth=abi.RATE_FACTOR_3D)
+    m = HIPModel(g, phys, abi.default_options())
+    x = abi.node_coordinates(g, "H")[0]
+    H = 500 + 1500 * np.exp(-((x - N * 500.0) / (N * 300.0)) ** 2)
+    m.set("H", H)
+    m.set("HN", H * 0.999)
+    m.set("BED", np.full_like(x, -200.0))
+    Epm = 2009.0 * (273.15 - 223.15)
+    E = Epm - 2009.0 * 30 * rng.random((Ns, N, N))
+    E[rng.random(E.shape) < 0.2] = Epm + 0.02 * 3.34e5
+    m.set("ENTHALPY", E)
+    for name, val in (("GEOTHERMAL", 0.06), ("TSURF", 250.0), ("TOCEAN", 271.15), ("BASAL_MELT", 1e-9)):
+        m.set(name, np.full_like(x, val))
+    m.set("ETASIGMA", np.full((Ns, N, N), 1e13))
+    xu = abi.node_coordinates(g, "U")[0]
+    m.set("U", 1e-5 * np.sin(np.pi * xu / (N * 1e3)))
+    m.set("USIGMA", np.broadcast_to(1e-5 * np.sin(np.pi * xu / (N * 1e3)), (Ns,) + xu.shape).copy())
+    m.set("BETAU", np.full_like(xu, 1e8))
+    m.set("UB", 5e-6 * np.ones_like(xu))
+    m.update_geometry()
+
+    def step():
+        m.compute_strain_heating()
+        m.energy_step(3e7, False)
+
+    step()
+    m.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    m.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    planes = (2 * Ns + 2) + (4 * Ns + 11) + (5 * Ns + 5) + (4 * Ns + 7) + (5 * Ns + 3)
+    bytes_step = 8.0 * N * N * planes
+    log(f"[bench] energy {N}x{N}x{Ns}: {ms:.3f} ms per step, {bytes_step / (ms * 1e-3) / 1e9:.0f} GB/s (algorithmic, wall)")
+    m.close()
+    return {"workload": f"enthalpy step {N}x{N}, Nsigma={Ns}, (Bounded, Periodic), flow, temperate layers, melt",
+            "ms_per_step": ms, "algorithmic_bytes_per_step": bytes_step, "GB_s_wall": bytes_step / (ms * 1e-3) / 1e9,
+            "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,7 +261,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-serial", type=int, default=12, help="requests timed serially in FP64 for the baseline")
     ap.add_argument("--transport", type=int, default=20, help="RK3 thickness steps timed in the transport leg (0: skip)")
-    ap.add_argument("--transport-only", action="store_true", help="run only the transport leg; prints its JSON object")
+    ap.add_argument("--transport-only", action="store_true",
+                    help="run only the thickness-transport and enthalpy legs; prints their JSON objects")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the velocity solve of one grid is not "
@@ -219,7 +270,7 @@ def main():
         sys.exit(2)
     N, K, Wm = args.size, args.steps, args.warmup
     if args.transport_only:
-        print(json.dumps(transport_leg(N, max(args.transport, 1))), flush=True)
+        print(json.dumps({"transport": transport_leg(N, max(args.transport, 1)), "energy": energy_leg(N, 10)}), flush=True)
         return
     os.environ.setdefault("NK_VERBOSE", "1")  # one library progress line per Picard iteration
     if args.stream_curve:
@@ -335,6 +386,7 @@ def main():
         out["served_p99_ms"] = out["serving"]["p99_ms"]
     if args.transport > 0:
         out["transport"] = transport_leg(N, args.transport)
+        out["energy"] = energy_leg(N, 10)
     print(json.dumps(out), flush=True)
 
 

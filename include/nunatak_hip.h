@@ -181,10 +181,16 @@ typedef enum nk_field_id {
     NK_FIELD_WATER_FRACTION, /* liquid water fraction ω    (Center, Center, σ)   */
     NK_FIELD_SMB,          /* surface mass balance a (m/s) (Center, Center)    */
     NK_FIELD_BASAL_MELT,   /* basal melt rate m (m/s)     (Center, Center)    */
-    NK_FIELD_GROUNDED_MELT,/* grounded melt mᵍ (m/s; zero unless coupled) (Center, Center) */
+    NK_FIELD_GROUNDED_MELT,/* grounded melt mᵍ (m/s), written by nk_energy_step (Center, Center) */
     NK_FIELD_HPREV,        /* H⁻: thickness at the start of the time step (Center, Center) */
     NK_FIELD_GH,           /* thickness tendency G        (Center, Center)    */
     NK_FIELD_HCLIP,        /* clipped thickness Hᶜ (thickness added by the clip at zero) (Center, Center) */
+    NK_FIELD_HW,           /* basal water layer thickness Hw (m) (Center, Center) */
+    NK_FIELD_HN,           /* thickness at the last energy step Hⁿ (Center, Center) */
+    NK_FIELD_GEOTHERMAL,   /* geothermal heat flux G (W m⁻²) (Center, Center) */
+    NK_FIELD_TSURF,        /* surface temperature Tₛ (K)  (Center, Center)    */
+    NK_FIELD_TOCEAN,       /* ocean temperature under floating ice Tₒ (K) (Center, Center) */
+    NK_FIELD_STRAIN_HEATING, /* strain heating Φ (W m⁻³) (Center, Center, σ)  */
     NK_FIELD_COUNT
 } nk_field_id;
 
@@ -273,7 +279,7 @@ nk_status nk_update_velocities(nk_solver *solver, int32_t geometry, nk_info *inf
  * UB, VB and the layer velocities USIGMA, VSIGMA from U, V and the DIVA state) or extend_depth_average!
  * (src/Models/ice_sheet_model_methods.jl:167-180) for SSA. */
 nk_status nk_compute_velocity_profile(nk_solver *solver);
-/* Arrhenius rate factor A on the σ grid from TEMPERATURE via the enthalpy round trip of set!(model; T)
+/* set!(model; T = ...): ENTHALPY = enthalpy(T, 0, p) and the Arrhenius rate factor A on the σ grid from TEMPERATURE via the enthalpy round trip
  * (src/Models/ice_sheet_model_methods.jl:47-57, src/Thermodynamics/column_energy_solve.jl:105-117,
  * src/Rheologies/arrhenius_rate_factor.jl:102-110) with the reference's default parameters. */
 nk_status nk_rate_factor_from_temperature(nk_solver *solver);
@@ -287,8 +293,9 @@ nk_status nk_thermal_diagnostics(nk_solver *solver);
 /* Advection schemes of the thickness flux (Oceananigans UpwindBiased(order = 1) and Centered(order = 2)). */
 enum { NK_ADVECTION_UPWIND1 = 0, NK_ADVECTION_CENTERED2 = 1 };
 /* compute_thickness_tendency! (src/Models/time_stepping.jl:34-63): GH = a − m·(1 − φ if grounding else 1) − mᵍ
- * − ∇·(ū H) with the face thickness of the scheme; U, V are the advecting depth-averaged velocities. */
-nk_status nk_compute_thickness_tendency(nk_solver *solver, int32_t advection);
+ * − ∇·(ū H) with the face thickness of the scheme; U, V are the advecting depth-averaged velocities; mᵍ
+ * (GROUNDED_MELT) enters only with grounded_melt_coupling (coupled_grounded_melt, :54-55). */
+nk_status nk_compute_thickness_tendency(nk_solver *solver, int32_t advection, int32_t grounded_melt_coupling);
 /* _thickness_substep! (time_stepping.jl:65-72): H★ = HPREV + Δτ·GH; H = max(0, H★); in the final stage
  * HCLIP += max(0, −H★). */
 nk_status nk_thickness_substep(nk_solver *solver, double dtau, int32_t final_stage);
@@ -296,6 +303,19 @@ nk_status nk_thickness_substep(nk_solver *solver, double dtau, int32_t final_sta
 nk_status nk_cache_thickness(nk_solver *solver);
 /* cell_advection_timescale(model) (time_stepping.jl:78, Oceananigans): min over cells of 1/(|u|/Δx + |v|/Δy). */
 nk_status nk_cell_advection_timescale(nk_solver *solver, double *timescale);
+
+/* ---- Enthalpy step: the energy equation once per ice time step -------------------------------------------------- */
+/* compute_strain_heating! (src/Thermodynamics/strain_heating.jl): STRAIN_HEATING = 4 ησ ε̇ₑ² (SSA); DIVA adds the
+ * vertical shear τᵇ² w₂ₖ / Δζₖ / ησ. Reads U, V, ETASIGMA (and UB, VB, BETABU, BETABV for DIVA). */
+nk_status nk_compute_strain_heating(nk_solver *solver);
+/* advance_enthalpy! (src/Thermodynamics/column_energy_solve.jl:48-84) over dt with the reference's default
+ * EnthalpyParameters: layer volume fluxes from USIGMA, VSIGMA, H, HN, BASAL_MELT (and GROUNDED_MELT when
+ * grounded_melt_coupling), the basal state from ENTHALPY, HW, GEOTHERMAL and the basal drag (BETAU/BETAV with UB/VB
+ * under SSA, BETABU/BETABV under DIVA), the implicit column system with STRAIN_HEATING, TSURF and TOCEAN, the
+ * tridiagonal solve, drainage and the basal water (HW, GROUNDED_MELT = mᵍ), then update_thermal_diagnostics!
+ * (TEMPERATURE, WATER_FRACTION; A with NK_RATE_FACTOR_3D) and HN = H. Needs Nσ ≥ 1 and the σ grid of the handle;
+ * the first call allocates the column system's workspace (≈ (6 Nσ + 5) planes of FP64). */
+nk_status nk_energy_step(nk_solver *solver, double dt, int32_t grounded_melt_coupling);
 
 /* Fixed-point adjoint of the converged SSA Picard solve (the EnzymeRules reverse rule of picard_solve!,
  * ext/NunatakEnzymeExt.jl:175-240, Christianson 1994), hand-written. Inputs: the converged solution in (U, V) and the

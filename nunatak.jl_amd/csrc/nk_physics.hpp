@@ -338,7 +338,7 @@ __device__ __forceinline__ void thermal_state(double E, double pr, double &Tk, d
 
 // A from TEMPERATURE through the enthalpy round trip of set!(model; T) (ice_sheet_model_methods.jl:47-57)
 __global__ __launch_bounds__(NT) void k_rate_factor(KG g, Sig sg, int Ns, long plane, double rig, const double *__restrict__ T,
-                                                    const double *__restrict__ H, double *__restrict__ A) {
+                                                    const double *__restrict__ H, double *__restrict__ A, double *__restrict__ Eout) {
     NK_IJ
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
@@ -349,6 +349,7 @@ __global__ __launch_bounds__(NT) void k_rate_factor(KG g, Sig sg, int Ns, long p
         const double Tpm = C::T0 - C::be * pr;
         const double Tc = jmin(T[ok], Tpm);
         const double E = C::ci * (Tc - C::Tr) + 0.0 * C::L;
+        Eout[ok] = E;
         double Tk, w;
         thermal_state(E, pr, Tk, w, A[ok]);
     }

@@ -18,6 +18,7 @@
 #include "nk_adjoint.hpp"
 #include "nk_galerkin.hpp"
 #include "nk_transport.hpp"
+#include "nk_thermal.hpp"
 #include "nk_internal.hpp"
 
 using namespace nk;
@@ -32,7 +33,8 @@ const LocInfo LOC[NK_FIELD_COUNT] = {
     {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0},
     {1, 0, 0}, {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0},
     {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2}, {1, 0, 1}, {0, 1, 1}, {0, 0, 1},
-    {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
+    {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
+    {0, 0, 0}, {0, 0, 0}, {0, 0, 1},
 };
 static_assert(sizeof(LOC) / sizeof(LOC[0]) == NK_FIELD_COUNT, "one location per field id");
 
@@ -98,7 +100,10 @@ struct nk_solver {
     const double *fz_r64u = nullptr, *fz_r64v = nullptr;
     double *fz_z64u = nullptr, *fz_z64v = nullptr;
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
-    bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
+    bool fuse_pre = true;
+    // enthalpy step workspace, allocated by the first nk_energy_step: face volume fluxes (Nσ + 1 planes), the column
+    // system a, b, c, f and the solver's t (Nσ planes each), the basal state rᵇ, sᵇ, Dᵇ, qᵍ (4 planes)
+    double *en_W = nullptr, *en_a = nullptr, *en_b = nullptr, *en_c = nullptr, *en_f = nullptr, *en_t = nullptr, *en_bas = nullptr;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
     unsigned int *counter = nullptr;
@@ -1566,7 +1571,7 @@ nk_status nk_update_velocities(nk_solver *s, int32_t geometry, nk_info *info) {
     return t.done(st);
 }
 
-nk_status nk_compute_thickness_tendency(nk_solver *s, int32_t advection) {
+nk_status nk_compute_thickness_tendency(nk_solver *s, int32_t advection, int32_t grounded_melt_coupling) {
     if (!s) return NK_INVALID_ARGUMENT;
     if (advection != NK_ADVECTION_UPWIND1 && advection != NK_ADVECTION_CENTERED2) {
         s->err = "thickness advection: NK_ADVECTION_UPWIND1 or NK_ADVECTION_CENTERED2";
@@ -1574,8 +1579,8 @@ nk_status nk_compute_thickness_tendency(nk_solver *s, int32_t advection) {
     }
     double **f = s->f;
     k_thickness_tendency<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(
-        s->g, advection, s->ph.grounding != NK_GROUNDING_NONE ? 1 : 0, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_H],
-        f[NK_FIELD_SMB], f[NK_FIELD_BASAL_MELT], f[NK_FIELD_PHI], f[NK_FIELD_GROUNDED_MELT], f[NK_FIELD_GH]);
+        s->g, advection, s->ph.grounding != NK_GROUNDING_NONE ? 1 : 0, grounded_melt_coupling ? 1 : 0, f[NK_FIELD_U], f[NK_FIELD_V],
+        f[NK_FIELD_H], f[NK_FIELD_SMB], f[NK_FIELD_BASAL_MELT], f[NK_FIELD_PHI], f[NK_FIELD_GROUNDED_MELT], f[NK_FIELD_GH]);
     CHK(launch_check(s));
     HIPCHK(hipStreamSynchronize(s->stream));
     return NK_OK;
@@ -1614,6 +1619,58 @@ nk_status nk_cell_advection_timescale(nk_solver *s, double *timescale) {
     return NK_OK;
 }
 
+nk_status nk_compute_strain_heating(nk_solver *s) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    double **f = s->f;
+    const bool diva = s->ph.stress_balance == NK_DIVA;
+    k_strain_heating<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(
+        s->g, sig_tables(s), s->Ns, s->plane, s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, diva ? 1 : 0, f[NK_FIELD_U], f[NK_FIELD_V],
+        f[NK_FIELD_ETASIGMA], f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_STRAIN_HEATING]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_energy_step(nk_solver *s, double dt, int32_t grounded_melt_coupling) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    if (!(dt > 0.0) || s->Ns < 1) { s->err = "nk_energy_step: dt must be positive and the grid needs σ layers"; return NK_INVALID_ARGUMENT; }
+    if (!s->en_W) {
+        std::lock_guard<std::mutex> lk(g_device_mu);
+        nk_status st;
+        if ((st = alloc_plane(s, &s->en_W, s->g, s->Ns + 1)) != NK_OK) return st;
+        double **w[] = {&s->en_a, &s->en_b, &s->en_c, &s->en_f, &s->en_t};
+        for (auto p : w) if ((st = alloc_plane(s, p, s->g, s->Ns)) != NK_OK) return st;
+        if ((st = alloc_plane(s, &s->en_bas, s->g, 4)) != NK_OK) return st;
+    }
+    double **f = s->f;
+    const bool diva = s->ph.stress_balance == NK_DIVA;
+    EnergyArgs e;
+    e.rho = s->ph.ice_density;
+    e.grav = s->ph.gravitational_acceleration;
+    e.dt = dt;
+    e.grounding = s->ph.grounding != NK_GROUNDING_NONE ? 1 : 0;
+    e.coupled = grounded_melt_coupling ? 1 : 0;
+    e.Ns = s->Ns;
+    e.a3d = s->ph.rate_factor_depth == NK_RATE_FACTOR_3D ? 1 : 0;
+    e.plane = s->plane;
+    e.H = f[NK_FIELD_H]; e.Hn = f[NK_FIELD_HN]; e.us = f[NK_FIELD_USIGMA]; e.vs = f[NK_FIELD_VSIGMA];
+    e.ub = f[NK_FIELD_UB]; e.vb = f[NK_FIELD_VB];
+    e.bu = diva ? f[NK_FIELD_BETABU] : f[NK_FIELD_BETAU];
+    e.bv = diva ? f[NK_FIELD_BETABV] : f[NK_FIELD_BETAV];
+    e.phi = f[NK_FIELD_PHI]; e.m = f[NK_FIELD_BASAL_MELT]; e.G = f[NK_FIELD_GEOTHERMAL]; e.Ts = f[NK_FIELD_TSURF];
+    e.To = f[NK_FIELD_TOCEAN]; e.Phi = f[NK_FIELD_STRAIN_HEATING];
+    e.E = f[NK_FIELD_ENTHALPY]; e.Hw = f[NK_FIELD_HW]; e.mg = f[NK_FIELD_GROUNDED_MELT]; e.T = f[NK_FIELD_TEMPERATURE];
+    e.W = f[NK_FIELD_WATER_FRACTION]; e.A = f[NK_FIELD_A];
+    e.W3 = s->en_W; e.a = s->en_a; e.b = s->en_b; e.c = s->en_c; e.f = s->en_f; e.t = s->en_t; e.bas = s->en_bas;
+    const dim3 B = blocks_for(s->g.Nx, s->g.Ny);
+    k_energy_assemble<<<B, THREADS, 0, s->stream>>>(s->g, sig_tables(s), e);
+    CHK(launch_check(s));
+    k_energy_solve<<<B, THREADS, 0, s->stream>>>(s->g, sig_tables(s), e, f[NK_FIELD_HN]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
 nk_status nk_thermal_diagnostics(nk_solver *s) {
     if (!s) return NK_INVALID_ARGUMENT;
     if (s->ph.rate_factor_depth != NK_RATE_FACTOR_3D) { s->err = "the rate factor must be on the σ grid (NK_RATE_FACTOR_3D)"; return NK_INVALID_ARGUMENT; }
@@ -1630,7 +1687,8 @@ nk_status nk_rate_factor_from_temperature(nk_solver *s) {
     if (s->ph.rate_factor_depth != NK_RATE_FACTOR_3D) { s->err = "the rate factor must be on the σ grid (NK_RATE_FACTOR_3D)"; return NK_INVALID_ARGUMENT; }
     k_rate_factor<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(s->g, sig_tables(s), s->Ns, s->plane,
                                                                            s->ph.ice_density * s->ph.gravitational_acceleration,
-                                                                           s->f[NK_FIELD_TEMPERATURE], s->f[NK_FIELD_H], s->f[NK_FIELD_A]);
+                                                                           s->f[NK_FIELD_TEMPERATURE], s->f[NK_FIELD_H], s->f[NK_FIELD_A],
+                                                                           s->f[NK_FIELD_ENTHALPY]);
     CHK(launch_check(s));
     HIPCHK(hipStreamSynchronize(s->stream));
     return NK_OK;

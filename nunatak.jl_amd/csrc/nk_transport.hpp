@@ -20,8 +20,8 @@ __device__ __forceinline__ double face_thickness(const KG &g, const double *H, i
 }
 
 // compute_thickness_tendency! (time_stepping.jl:34-63): G = (a − melt·m) − mᵍ − ∇·(ū H); melt = 1 − φ with a
-// grounding treatment, else 1
-__global__ __launch_bounds__(NT) void k_thickness_tendency(KG g, int adv, int grounding, const double *__restrict__ u,
+// grounding treatment, else 1; mᵍ only when coupled
+__global__ __launch_bounds__(NT) void k_thickness_tendency(KG g, int adv, int grounding, int coupled, const double *__restrict__ u,
                                                            const double *__restrict__ v, const double *__restrict__ H,
                                                            const double *__restrict__ a, const double *__restrict__ m,
                                                            const double *__restrict__ phi, const double *__restrict__ mg,
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(NT) void k_thickness_tendency(KG g, int adv, int gr
     const double Fy1 = g.dx * v1 * face_thickness(g, H, i, j + 1, 1, v1, adv);
     const double div = (1.0 / (g.dx * g.dy)) * ((Fx1 - Fx0) + (Fy1 - Fy0));
     const double melt = grounding ? (1.0 - phi[o]) * m[o] : m[o];
-    const double S = (a[o] - melt) - mg[o];
+    const double S = (a[o] - melt) - (coupled ? mg[o] : 0.0);
     G[o] = S - div;
 }
 

@@ -64,7 +64,7 @@ end
 const U, V, H, BED, S, PHI, PHIU, PHIV, N, MU, TAUU, TAUV, ETAH, ETAHFF, BETAU, BETAV, F2, ETASIGMA,
       BETABU, BETABV, UB, VB, USIGMA, VSIGMA, A, C, TEMPERATURE, PU, PV, QU, QV,
       UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR, USIGMABAR, VSIGMABAR, ENTHALPY, WATER_FRACTION,
-      SMB, BASAL_MELT, GROUNDED_MELT, HPREV, GH, HCLIP = 0:46
+      SMB, BASAL_MELT, GROUNDED_MELT, HPREV, GH, HCLIP, HW, HN, GEOTHERMAL, TSURF, TOCEAN, STRAIN_HEATING = 0:52
 
 control(c::Convergence) = NKControl(0, c.maxiter, 0, 0, c.reltol, c.abstol)
 control(c::FixedIterations) = NKControl(1, c.iterations, c.iterations, 0, 0.0, 0.0)
@@ -218,7 +218,9 @@ function hip_rk_substep!(model, solver::HIPStressBalanceSolver, Δτ, final_stag
     set_field!(solver, U, vel.u); set_field!(solver, V, vel.v); set_field!(solver, PHI, geometry.φ)
     set_field!(solver, SMB, model.surface_mass_balance); set_field!(solver, BASAL_MELT, model.basal_melt_rate)
     set_field!(solver, HCLIP, geometry.clipped_thickness)
-    check(ccall((:nk_compute_thickness_tendency, libnunatak_hip), Cint, (Ptr{Cvoid}, Int32), solver.handle, advection),
+    coupled = model.thermodynamics !== nothing && model.thermodynamics.grounded_melt_coupling
+    coupled && set_field!(solver, GROUNDED_MELT, model.thermodynamics.state.mᵍ)
+    check(ccall((:nk_compute_thickness_tendency, libnunatak_hip), Cint, (Ptr{Cvoid}, Int32, Int32), solver.handle, advection, coupled),
           solver.handle, "nk_compute_thickness_tendency")
     check(ccall((:nk_thickness_substep, libnunatak_hip), Cint, (Ptr{Cvoid}, Float64, Int32), solver.handle, Δτ, final_stage),
           solver.handle, "nk_thickness_substep")

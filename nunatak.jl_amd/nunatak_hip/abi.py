@@ -27,7 +27,8 @@ PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE = 0, 1, 2, 3
 FIELDS = ["U", "V", "H", "BED", "S", "PHI", "PHIU", "PHIV", "N", "MU", "TAUU", "TAUV", "ETAH", "ETAHFF",
           "BETAU", "BETAV", "F2", "ETASIGMA", "BETABU", "BETABV", "UB", "VB", "USIGMA", "VSIGMA", "A", "C",
           "TEMPERATURE", "PU", "PV", "QU", "QV", "UBAR", "VBAR", "LAMBDAU", "LAMBDAV", "CBAR", "ABAR", "USIGMABAR",
-          "VSIGMABAR", "ENTHALPY", "WATER_FRACTION", "SMB", "BASAL_MELT", "GROUNDED_MELT", "HPREV", "GH", "HCLIP"]
+          "VSIGMABAR", "ENTHALPY", "WATER_FRACTION", "SMB", "BASAL_MELT", "GROUNDED_MELT", "HPREV", "GH", "HCLIP",
+          "HW", "HN", "GEOTHERMAL", "TSURF", "TOCEAN", "STRAIN_HEATING"]
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 
 # (x location, y location, on the σ grid); "f" = Face, "c" = Center.
@@ -45,6 +46,8 @@ LOCATIONS = {
     "USIGMABAR": ("f", "c", True), "VSIGMABAR": ("c", "f", True), "ENTHALPY": ("c", "c", True),
     "WATER_FRACTION": ("c", "c", True), "SMB": ("c", "c", False), "BASAL_MELT": ("c", "c", False),
     "GROUNDED_MELT": ("c", "c", False), "HPREV": ("c", "c", False), "GH": ("c", "c", False), "HCLIP": ("c", "c", False),
+    "HW": ("c", "c", False), "HN": ("c", "c", False), "GEOTHERMAL": ("c", "c", False), "TSURF": ("c", "c", False),
+    "TOCEAN": ("c", "c", False), "STRAIN_HEATING": ("c", "c", True),
 }
 ADVECTION_UPWIND1, ADVECTION_CENTERED2 = 0, 1
 

@@ -22,7 +22,7 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients", "nk_apply_operator",
            "nk_precondition", "nk_solve", "nk_picard_solve", "nk_update_velocities", "nk_compute_velocity_profile",
            "nk_thermal_diagnostics", "nk_compute_thickness_tendency", "nk_thickness_substep", "nk_cache_thickness",
-           "nk_cell_advection_timescale",
+           "nk_cell_advection_timescale", "nk_compute_strain_heating", "nk_energy_step",
            "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint", "nk_device_bytes",
            "nk_estimate_device_bytes", "nk_server_create", "nk_server_destroy", "nk_server_last_error",
            "nk_server_submit", "nk_server_poll", "nk_server_wait"]
@@ -64,16 +64,17 @@ def load(path=LIB_PATH):
     L.nk_stream.restype = P
     for name in ("nk_synchronize", "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients",
                  "nk_apply_operator", "nk_precondition", "nk_rate_factor_from_temperature", "nk_compute_velocity_profile",
-                 "nk_thermal_diagnostics", "nk_cache_thickness"):
+                 "nk_thermal_diagnostics", "nk_cache_thickness", "nk_compute_strain_heating"):
         getattr(L, name).argtypes = [P]
     L.nk_solve.argtypes = [P, ct.POINTER(abi.Info)]
     L.nk_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
     L.nk_update_velocities.argtypes = [P, I32, ct.POINTER(abi.Info)]
     L.nk_multigrid_levels.argtypes = [P, ct.POINTER(I32), ct.POINTER(I32), I32]
     L.nk_adjoint.argtypes = [P, ct.POINTER(abi.Control), ct.POINTER(abi.Info)]
-    L.nk_compute_thickness_tendency.argtypes = [P, I32]
+    L.nk_compute_thickness_tendency.argtypes = [P, I32, I32]
     L.nk_thickness_substep.argtypes = [P, D, I32]
     L.nk_cell_advection_timescale.argtypes = [P, ct.POINTER(D)]
+    L.nk_energy_step.argtypes = [P, D, I32]
     L.nk_device_bytes.argtypes = [P, ct.POINTER(I64)]
     L.nk_estimate_device_bytes.argtypes = [ct.POINTER(abi.Grid), ct.POINTER(abi.Physics), ct.POINTER(abi.SolverOptions),
                                            ct.POINTER(I64)]
@@ -190,8 +191,9 @@ class HIPModel:
     def precondition(self):
         self._check(self._L.nk_precondition(self._h), "nk_precondition")
 
-    def compute_thickness_tendency(self, advection=abi.ADVECTION_UPWIND1):
-        self._check(self._L.nk_compute_thickness_tendency(self._h, advection), "nk_compute_thickness_tendency")
+    def compute_thickness_tendency(self, advection=abi.ADVECTION_UPWIND1, grounded_melt_coupling=False):
+        self._check(self._L.nk_compute_thickness_tendency(self._h, advection, 1 if grounded_melt_coupling else 0),
+                    "nk_compute_thickness_tendency")
 
     def thickness_substep(self, dtau, final_stage):
         self._check(self._L.nk_thickness_substep(self._h, dtau, 1 if final_stage else 0), "nk_thickness_substep")
@@ -203,6 +205,12 @@ class HIPModel:
         t = ct.c_double()
         self._check(self._L.nk_cell_advection_timescale(self._h, ct.byref(t)), "nk_cell_advection_timescale")
         return t.value
+
+    def compute_strain_heating(self):
+        self._check(self._L.nk_compute_strain_heating(self._h), "nk_compute_strain_heating")
+
+    def energy_step(self, dt, grounded_melt_coupling=False):
+        self._check(self._L.nk_energy_step(self._h, dt, 1 if grounded_melt_coupling else 0), "nk_energy_step")
 
     def thermal_diagnostics(self):
         self._check(self._L.nk_thermal_diagnostics(self._h), "nk_thermal_diagnostics")

@@ -48,6 +48,8 @@ static const nko_loc LOCS[NK_FIELD_COUNT] = {
     [NK_FIELD_ENTHALPY] = {C, C, 1}, [NK_FIELD_WATER_FRACTION] = {C, C, 1},
     [NK_FIELD_SMB] = {C, C, 0},    [NK_FIELD_BASAL_MELT] = {C, C, 0}, [NK_FIELD_GROUNDED_MELT] = {C, C, 0},
     [NK_FIELD_HPREV] = {C, C, 0},  [NK_FIELD_GH] = {C, C, 0},       [NK_FIELD_HCLIP] = {C, C, 0},
+    [NK_FIELD_HW] = {C, C, 0},     [NK_FIELD_HN] = {C, C, 0},       [NK_FIELD_GEOTHERMAL] = {C, C, 0},
+    [NK_FIELD_TSURF] = {C, C, 0},  [NK_FIELD_TOCEAN] = {C, C, 0},   [NK_FIELD_STRAIN_HEATING] = {C, C, 1},
 };
 
 /* ------------------------------------------------------------------------------------------------------------ */
@@ -1253,11 +1255,13 @@ EXPORT void nko_rate_factor_from_temperature(nko_model *m) {
                 size_t o = (size_t)(k - 1) * g->plane + IDX(g, i, j);
                 double p = rg * AT(g, H, i, j) * (1.0 - m->zc[k]);   /* layer_pressure (enthalpy_conversions.jl:171-174) */
                 double E = e_enthalpy(T[o], 0.0, p);
+                m->f[NK_FIELD_ENTHALPY][o] = E;
                 double Tk = e_temperature(E, p), w = e_water(E, p);
                 double Tp = Tk - pm_temperature(p) + E_T0;
                 A[o] = nko_arrhenius_rate_factor(Tp, w);
             }
     fill_field_halo(m, NK_FIELD_A);
+    fill_field_halo(m, NK_FIELD_ENTHALPY);
 }
 
 /* update_thermal_diagnostics! (src/Thermodynamics/column_energy_solve.jl:85-117): _compute_thermal_diagnostics!
@@ -1298,7 +1302,7 @@ EXPORT void nko_enthalpy_conversions(double T, double w, double p, double *E, do
 static double face_H(double left, double right, double vel, int adv) { return adv == 0 ? (vel > 0.0 ? left : right) : (left + right) / 2.0; }
 
 /* compute_thickness_tendency! / _compute_thickness_tendency! (time_stepping.jl:34-63) */
-EXPORT void nko_thickness_tendency(nko_model *m, int adv) {
+EXPORT void nko_thickness_tendency(nko_model *m, int adv, int coupled) {
     const og *g = &m->g;
     const double *u = m->f[NK_FIELD_U], *v = m->f[NK_FIELD_V], *H = m->f[NK_FIELD_H];
     const double *a = m->f[NK_FIELD_SMB], *mm = m->f[NK_FIELD_BASAL_MELT], *phi = m->f[NK_FIELD_PHI], *mg = m->f[NK_FIELD_GROUNDED_MELT];
@@ -1313,7 +1317,7 @@ EXPORT void nko_thickness_tendency(nko_model *m, int adv) {
             double div = (1.0 / (g->dx * g->dy)) * ((Fx1 - Fx0) + (Fy1 - Fy0));
             /* basal_melt (:51-52): (1 − φ) m with a grounding treatment; grounded_melt (:57-58): mᵍ */
             double melt = m->ph.grounding != NK_GROUNDING_NONE ? (1.0 - AT(g, phi, i, j)) * AT(g, mm, i, j) : AT(g, mm, i, j);
-            double S = (AT(g, a, i, j) - melt) - AT(g, mg, i, j);
+            double S = (AT(g, a, i, j) - melt) - (coupled ? AT(g, mg, i, j) : 0.0);
             AT(g, G, i, j) = S - div;
         }
     fill_field_halo(m, NK_FIELD_GH);
@@ -1347,6 +1351,199 @@ EXPORT double nko_cell_advection_timescale(const nko_model *m) {
     for (int j = 1; j <= g->Ny; j++)
         for (int i = 1; i <= g->Nx; i++) t = fmin(t, 1.0 / (fabs(AT(g, u, i, j)) / g->dx + fabs(AT(g, v, i, j)) / g->dy));
     return t;
+}
+
+/* ------------------------------------------------------------------------------------------------------------ */
+/* Enthalpy step (src/Thermodynamics/column_energy_solve.jl, layer_transport.jl, strain_heating.jl)               */
+/* ------------------------------------------------------------------------------------------------------------ */
+
+/* EnthalpyParameters defaults not above (enthalpy_thermodynamics.jl:151-166) */
+static const double E_K = 2.1, E_RATIO = 1e-3, E_DE = 1.0, E_HWB = 1e-3, E_HMIN = 1.0, E_WD = 0.01, E_RMAX = 1.58443823077064e-9;
+
+static double unit_ramp(double x, double w) { return jclamp(x / w, 0.0, 1.0); }                 /* enthalpy_conversions.jl:177 */
+static double col_thickness(double H) { return jmax(H, E_HMIN); }                              /* :166 */
+/* layer_diffusivity (:180-188) at enthalpy E and pressure p */
+static double layer_diffusivity(double E, double p) {
+    double w = unit_ramp(E - pm_enthalpy(p) + E_DE, E_DE);
+    double Kc = E_K / E_CI;
+    return Kc * (1.0 - w + w * E_RATIO);
+}
+static double upwind_value(double u, double cm, double cp) { return u > 0.0 ? cm : cp; }        /* layer_transport.jl:5 */
+
+/* compute_strain_heating! for SSA and DIVA (strain_heating.jl:8-56) */
+EXPORT void nko_compute_strain_heating(nko_model *m) {
+    const og *g = &m->g;
+    const double *u = m->f[NK_FIELD_U], *v = m->f[NK_FIELD_V], *es = m->f[NK_FIELD_ETASIGMA];
+    const double *ub = m->f[NK_FIELD_UB], *vb = m->f[NK_FIELD_VB], *bbu = m->f[NK_FIELD_BETABU], *bbv = m->f[NK_FIELD_BETABV];
+    double *Phi = m->f[NK_FIELD_STRAIN_HEATING];
+    int diva = m->ph.stress_balance == NK_DIVA;
+    for (int k = 1; k <= m->Ns; k++)
+        for (int j = 1; j <= g->Ny; j++)
+            for (int i = 1; i <= g->Nx; i++) {
+                size_t o = (size_t)(k - 1) * g->plane + IDX(g, i, j);
+                double eta = es[o];
+                double e2 = eff_strain2(g, i, j, u, v, m->ph.walls);
+                double val = 4.0 * eta * e2;   /* _compute_membrane_heating! (:41-44) */
+                if (diva) {                    /* _compute_shear_heating! (:48-56): ℑx, ℑy of (β u)² */
+                    double s0 = AT(g, bbu, i, j) * AT(g, ub, i, j), s1 = AT(g, bbu, i + 1, j) * AT(g, ub, i + 1, j);
+                    double t0 = AT(g, bbv, i, j) * AT(g, vb, i, j), t1 = AT(g, bbv, i, j + 1) * AT(g, vb, i, j + 1);
+                    double tb2 = (s0 * s0 + s1 * s1) / 2.0 + (t0 * t0 + t1 * t1) / 2.0;
+                    double shear = tb2 * m->w2[k] / m->dz[k] / (eta == 0.0 ? INFINITY : eta);
+                    val = val + shear;
+                }
+                Phi[o] = val;
+            }
+    fill_field_halo(m, NK_FIELD_STRAIN_HEATING);
+}
+
+/* advance_enthalpy! (column_energy_solve.jl:48-84) with update_thermal_diagnostics! and Hⁿ = H */
+EXPORT void nko_energy_step(nko_model *m, double dt, int coupled) {
+    const og *g = &m->g;
+    const int Ns = m->Ns;
+    const double rho = m->ph.ice_density, rg = m->ph.ice_density * m->ph.gravitational_acceleration, Az = g->dx * g->dy;
+    const int grounding = m->ph.grounding != NK_GROUNDING_NONE, diva = m->ph.stress_balance == NK_DIVA;
+    const double *H = m->f[NK_FIELD_H], *Hn = m->f[NK_FIELD_HN], *phi = m->f[NK_FIELD_PHI], *mm = m->f[NK_FIELD_BASAL_MELT];
+    const double *G = m->f[NK_FIELD_GEOTHERMAL], *Ts = m->f[NK_FIELD_TSURF], *To = m->f[NK_FIELD_TOCEAN], *Phi = m->f[NK_FIELD_STRAIN_HEATING];
+    const double *us = m->f[NK_FIELD_USIGMA], *vs = m->f[NK_FIELD_VSIGMA], *ub = m->f[NK_FIELD_UB], *vb = m->f[NK_FIELD_VB];
+    const double *bu = diva ? m->f[NK_FIELD_BETABU] : m->f[NK_FIELD_BETAU], *bv = diva ? m->f[NK_FIELD_BETABV] : m->f[NK_FIELD_BETAV];
+    double *E = m->f[NK_FIELD_ENTHALPY], *Hw = m->f[NK_FIELD_HW], *mg = m->f[NK_FIELD_GROUNDED_MELT];
+    fill_field_halo(m, NK_FIELD_ENTHALPY);
+    double *Enew = malloc(sizeof(double) * g->plane * (size_t)Ns);
+    memcpy(Enew, E, sizeof(double) * g->plane * (size_t)Ns);
+    double *W = malloc(sizeof(double) * (size_t)(Ns + 2)), *a = malloc(sizeof(double) * (size_t)(Ns + 1)), *b = malloc(sizeof(double) * (size_t)(Ns + 1));
+    double *c = malloc(sizeof(double) * (size_t)(Ns + 1)), *f = malloc(sizeof(double) * (size_t)(Ns + 1)), *t = calloc((size_t)(Ns + 1), sizeof(double));
+    double *x = malloc(sizeof(double) * (size_t)(Ns + 1));
+#define EK(i, j, k) E[(size_t)((k) - 1) * g->plane + IDX(g, i, j)]
+#define LP(i, j, k) (rg * AT(g, H, i, j) * (1.0 - m->zc[k]))   /* layer_pressure (enthalpy_conversions.jl:171-174) */
+    for (int j = 1; j <= g->Ny; j++)
+        for (int i = 1; i <= g->Nx; i++) {
+            double Hc = AT(g, H, i, j), Ht = col_thickness(Hc), Htn = col_thickness(AT(g, Hn, i, j));
+            int mu = Hc > E_HMIN;                                                   /* column_has_ice (:168) */
+            double phg = grounding ? AT(g, phi, i, j) : 1.0;                        /* grounded_part (:119-120) */
+            /* _compute_layer_volume_fluxes! (layer_transport.jl:34-44) */
+            double dHdt = (Ht - Htn) / dt;
+            double bm = grounding ? (1.0 - AT(g, phi, i, j)) * AT(g, mm, i, j) : AT(g, mm, i, j);
+            W[1] = -bm - (coupled ? AT(g, mg, i, j) : 0.0);
+            for (int k = 1; k <= Ns; k++) {
+                const double *uk = us + (size_t)(k - 1) * g->plane, *vk = vs + (size_t)(k - 1) * g->plane;
+                double xv[2], yv[2];
+                for (int s = 0; s < 2; s++) {   /* x_volume_flux, y_volume_flux (:8-16) with wall faces carrying none */
+                    double u = periph(g, i + s, j, F, C) ? 0.0 : AT(g, uk, i + s, j);
+                    xv[s] = g->dy * u * upwind_value(u, AT(g, Hn, i + s - 1, j), AT(g, Hn, i + s, j));
+                    double v = periph(g, i, j + s, C, F) ? 0.0 : AT(g, vk, i, j + s);
+                    yv[s] = g->dx * v * upwind_value(v, AT(g, Hn, i, j + s - 1), AT(g, Hn, i, j + s));
+                }
+                double vfd = (xv[1] - xv[0] + yv[1] - yv[0]) / Az;             /* volume_flux_divergence (:20-23) */
+                W[k + 1] = W[k] - m->dz[k] * (dHdt + vfd);
+            }
+            /* _compute_basal_state! (column_energy_solve.jl:130-147) */
+            double pb = rg * Hc;                                                    /* basal_pressure (:176) */
+            double D = layer_diffusivity(EK(i, j, 1), LP(i, j, 1)) / (Ht * m->zc[1]);
+            double F_ = (AT(g, bu, i, j) * (AT(g, ub, i, j) * AT(g, ub, i, j)) + AT(g, bu, i + 1, j) * (AT(g, ub, i + 1, j) * AT(g, ub, i + 1, j)) +
+                         AT(g, bv, i, j) * (AT(g, vb, i, j) * AT(g, vb, i, j)) + AT(g, bv, i, j + 1) * (AT(g, vb, i, j + 1) * AT(g, vb, i, j + 1))) / 2.0;
+            double Epmb = pm_enthalpy(pb);
+            double Eb = EK(i, j, 1) + (AT(g, G, i, j) + F_) / D;
+            double r = jmax(unit_ramp(Eb - Epmb, E_DE), unit_ramp(AT(g, Hw, i, j), E_HWB));
+            double sb = unit_ramp(EK(i, j, 1) - pm_enthalpy(LP(i, j, 1)) + E_DE, E_DE);
+            double qg = phg * AT(g, G, i, j) + F_;
+            /* _compute_column_system! (:185-226) */
+            double Es = e_enthalpy(AT(g, Ts, i, j), 0.0, 0.0);                      /* surface_enthalpy (:164) */
+            double chi = phg * r * (1.0 - sb);
+            double Eo = e_enthalpy(AT(g, To, i, j), 0.0, pb);
+            double Fb = rho * W[1];
+            double Lam = D * (chi + 1.0 - phg) + jmax(-Fb, 0.0);
+            double S = (1.0 - r) * qg + D * (chi * Epmb + (1.0 - phg) * Eo) + jmax(Fb, 0.0) * Epmb;
+            double Fs = rho * W[Ns + 1];
+            double Ks = layer_diffusivity(EK(i, j, Ns), LP(i, j, Ns)) / (Ht * (1.0 - m->zc[Ns]));   /* surface_conductance */
+            double Ds = Ks + jmax(Fs, 0.0), Ss = (Ks + jmax(-Fs, 0.0)) * Es;
+            for (int k = 1; k <= Ns; k++) {
+                /* face_couplings at the faces k (below layer k) and k + 1 (above), interior faces only */
+                double low[2] = {0.0, 0.0}, upp[2] = {0.0, 0.0};
+                for (int s = 0; s < 2; s++) {
+                    int kf = k + s;
+                    if (kf < 2 || kf > Ns) continue;
+                    double zf = m->zf[kf], zm = m->zc[kf - 1], zp = m->zc[kf];
+                    double th = (zf - zm) / (zp - zm);
+                    double Fk = rho * W[kf];
+                    double Km = layer_diffusivity(EK(i, j, kf - 1), LP(i, j, kf - 1)), Kp = layer_diffusivity(EK(i, j, kf), LP(i, j, kf));
+                    double cond = 1.0 / (Ht * ((zf - zm) / Km + (zp - zf) / Kp));    /* face_conductance (:150-158) */
+                    double Dk = jmax(jmax(cond, jmax(Fk, 0.0) * th), jmax(-Fk, 0.0) * (1.0 - th));
+                    low[s] = -Fk * (1.0 - th) - Dk;
+                    upp[s] = Fk * th - Dk;
+                }
+                double dm = k == 1 ? Lam : -upp[0];
+                double dp = k == Ns ? Ds : -low[1];
+                double forcing = (k == 1 ? S : 0.0) + (k == Ns ? Ss : 0.0);
+                /* enthalpy_flux_divergence (layer_transport.jl:25-28) with Hⁿ and the old enthalpy */
+                const double *uk = us + (size_t)(k - 1) * g->plane, *vk = vs + (size_t)(k - 1) * g->plane;
+                double xe[2], ye[2];
+                for (int s = 0; s < 2; s++) {
+                    double uraw = AT(g, uk, i + s, j), u = periph(g, i + s, j, F, C) ? 0.0 : uraw;
+                    xe[s] = g->dy * u * upwind_value(u, AT(g, Hn, i + s - 1, j), AT(g, Hn, i + s, j)) * upwind_value(uraw, EK(i + s - 1, j, k), EK(i + s, j, k));
+                    double vraw = AT(g, vk, i, j + s), v = periph(g, i, j + s, C, F) ? 0.0 : vraw;
+                    ye[s] = g->dx * v * upwind_value(v, AT(g, Hn, i, j + s - 1), AT(g, Hn, i, j + s)) * upwind_value(vraw, EK(i, j + s - 1, k), EK(i, j + s, k));
+                }
+                double efd = (xe[1] - xe[0] + ye[1] - ye[0]) / Az;
+                double transport = rho * m->dz[k] * efd;
+                a[k] = mu ? low[1] : 0.0;
+                c[k] = mu ? upp[1] : 0.0;
+                b[k] = mu ? rho * Ht * m->dz[k] / dt + dm + dp : 1.0;
+                f[k] = mu ? rho * Htn * m->dz[k] * EK(i, j, k) / dt - transport + Ht * m->dz[k] * Phi[(size_t)(k - 1) * g->plane + IDX(g, i, j)] + forcing : Es;
+            }
+            /* BatchedTridiagonalSolver (Oceananigans): a[k] couples x[k] into row k + 1, c[k] x[k + 1] into row k */
+            for (int k = 1; k <= Ns; k++) x[k] = EK(i, j, k);
+            double beta = b[1];
+            x[1] = f[1] / beta;
+            for (int k = 2; k <= Ns; k++) {
+                t[k] = c[k - 1] / beta;
+                beta = b[k] - a[k - 1] * t[k];
+                if (!(fabs(beta) > 10.0 * 2.220446049250313e-16)) break;
+                x[k] = (f[k] - a[k - 1] * x[k - 1]) / beta;
+            }
+            for (int k = Ns - 1; k >= 1; k--) x[k] = x[k] - t[k + 1] * x[k + 1];
+            /* _update_basal_water! (:238-267), no till */
+            double q = D * (Epmb - x[1]);
+            double melt = r * (qg - phg * (1.0 - sb) * q) / (rho * E_L);
+            double drained = 0.0;
+            for (int k = 1; k <= Ns; k++) {
+                double w = e_water(x[k], LP(i, j, k));
+                double xr = w / E_WD;   /* drainage_rate (:230-234) */
+                double shape = jclamp(xr - 1.0, 0.0, 1.0) / 10.0 + 9.0 * jclamp(xr - 2.0, 0.0, 1.0) / 10.0;
+                double dw = jmin(dt * (E_RMAX * shape), jmax(w - E_WD, 0.0));
+                dw = mu ? dw : 0.0;
+                x[k] = x[k] - E_L * dw;
+                drained = drained + Ht * m->dz[k] * dw;
+            }
+            double Hw0 = AT(g, Hw, i, j);
+            double Hstar = Hw0 + dt * (mu ? melt : 0.0) + drained;
+            double Hplus = jmax(0.0, Hstar), missing = Hplus - Hstar;
+            AT(g, mg, i, j) = (Hplus - Hw0) / dt;
+            AT(g, Hw, i, j) = Hplus;
+            x[1] = x[1] - E_L * missing / (Ht * m->dz[1]);
+            for (int k = 1; k <= Ns; k++) Enew[(size_t)(k - 1) * g->plane + IDX(g, i, j)] = x[k];
+        }
+#undef EK
+#undef LP
+    memcpy(E, Enew, sizeof(double) * g->plane * (size_t)Ns);
+    free(Enew); free(W); free(a); free(b); free(c); free(f); free(t); free(x);
+    fill_field_halo(m, NK_FIELD_ENTHALPY);
+    fill_field_halo(m, NK_FIELD_HW);
+    fill_field_halo(m, NK_FIELD_GROUNDED_MELT);
+    /* update_thermal_diagnostics!: T and ω, A when the rate factor is on the σ grid */
+    double *T = m->f[NK_FIELD_TEMPERATURE], *Wf = m->f[NK_FIELD_WATER_FRACTION], *A = m->f[NK_FIELD_A];
+    for (int k = 1; k <= Ns; k++)
+        for (int j = 1; j <= g->Ny; j++)
+            for (int i = 1; i <= g->Nx; i++) {
+                size_t o = (size_t)(k - 1) * g->plane + IDX(g, i, j);
+                double p = rg * AT(g, H, i, j) * (1.0 - m->zc[k]);
+                T[o] = e_temperature(E[o], p);
+                Wf[o] = e_water(E[o], p);
+                if (m->ph.rate_factor_depth == NK_RATE_FACTOR_3D) A[o] = nko_arrhenius_rate_factor(T[o] - pm_temperature(p) + E_T0, Wf[o]);
+            }
+    fill_field_halo(m, NK_FIELD_TEMPERATURE);
+    fill_field_halo(m, NK_FIELD_WATER_FRACTION);
+    if (m->ph.rate_factor_depth == NK_RATE_FACTOR_3D) fill_field_halo(m, NK_FIELD_A);
+    memcpy(m->f[NK_FIELD_HN], m->f[NK_FIELD_H], sizeof(double) * g->plane);   /* parent(Hⁿ) .= parent(H) */
 }
 
 /* σ-grid metrics for tests */

@@ -44,7 +44,7 @@ def lib():
         L.nko_get_classes.argtypes = [P, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64]
         for name in ["nko_update_geometry", "nko_compute_driving_stress", "nko_update_coefficients",
                      "nko_apply_operator", "nko_precondition", "nko_rate_factor_from_temperature",
-                     "nko_thermal_diagnostics", "nko_cache_thickness"]:
+                     "nko_thermal_diagnostics", "nko_cache_thickness", "nko_compute_strain_heating"]:
             getattr(L, name).argtypes = [P]
         L.nko_solve.argtypes = [P, ct.POINTER(abi.Info)]
         L.nko_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
@@ -55,10 +55,11 @@ def lib():
         L.nko_mg_transfer.restype = I
         L.nko_sigma_metrics.argtypes = [P, dp, dp, dp, dp]
         L.nko_enthalpy_conversions.argtypes = [D, D, D, dp, dp, dp]
-        L.nko_thickness_tendency.argtypes = [P, I]
+        L.nko_thickness_tendency.argtypes = [P, I, I]
         L.nko_thickness_substep.argtypes = [P, D, I]
         L.nko_cell_advection_timescale.argtypes = [P]
         L.nko_cell_advection_timescale.restype = D
+        L.nko_energy_step.argtypes = [P, D, I]
         for name, args in [("nko_ice_mask_value", [D, D, D]), ("nko_segment_grounded_fraction", [D, D]),
                            ("nko_triangle_grounded_fraction", [D, D, D]), ("nko_arrhenius_rate_factor", [D, D]),
                            ("nko_basal_drag", [D, D, D, I, D, D])]:
@@ -143,8 +144,8 @@ class OracleModel:
     def thermal_diagnostics(self):
         lib().nko_thermal_diagnostics(self._h)
 
-    def compute_thickness_tendency(self, advection=0):
-        lib().nko_thickness_tendency(self._h, advection)
+    def compute_thickness_tendency(self, advection=0, grounded_melt_coupling=False):
+        lib().nko_thickness_tendency(self._h, advection, 1 if grounded_melt_coupling else 0)
 
     def thickness_substep(self, dtau, final_stage):
         lib().nko_thickness_substep(self._h, dtau, 1 if final_stage else 0)
@@ -154,6 +155,12 @@ class OracleModel:
 
     def cell_advection_timescale(self):
         return lib().nko_cell_advection_timescale(self._h)
+
+    def compute_strain_heating(self):
+        lib().nko_compute_strain_heating(self._h)
+
+    def energy_step(self, dt, grounded_melt_coupling=False):
+        lib().nko_energy_step(self._h, dt, 1 if grounded_melt_coupling else 0)
 
     def solve(self):
         info = abi.Info()
