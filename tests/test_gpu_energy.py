@@ -97,3 +97,22 @@ def test_column_energy_budget_on_the_device():
         transport.time_step(m, dt, thermodynamics=TE.TH, first=(k == 0))
         q = TE.K / TE.CI * (TE.col(m, "ENTHALPY")[-1] - TE.CI * (243.15 - TE.TR)) / (H0 * (1 - z[-1]))
         assert abs(energy() - e0 - dt * (G + F - q)) < 1e-11 * abs(e0)
+
+
+@pytest.mark.parametrize("stress_balance", [abi.SSA, abi.DIVA])
+def test_friction_and_strain_heat_account_for_the_work_of_the_driving_stress_on_the_device(stress_balance):
+    """test/test_thermodynamics.jl:392-398 through the device: velocity solves in every stage, strain heating and the
+    energy step; frictional plus strain heat against the driving power, bar 1e-8."""
+    say(f"energy: dissipation {stress_balance}")
+    assert TE.dissipation_mismatch(stress_balance, cls=HIPModel) < 1e-8
+
+
+def test_uniform_enthalpy_stays_uniform_on_the_device():
+    say("energy: uniform enthalpy patch")
+    m, g, z = TE.patch_model((P, P), cls=HIPModel)
+    E0 = m.get("ENTHALPY")[0, 0, 0]
+    H0 = m.get("H").copy()
+    for k in range(20):
+        transport.time_step(m, 1e9, thermodynamics=TE.TH, first=(k == 0))
+    assert np.max(np.abs(m.get("H") - H0)) > 10
+    assert np.max(np.abs(m.get("ENTHALPY") - E0)) < 1e-9 * E0

@@ -164,3 +164,174 @@ def test_ice_free_columns_hold_the_surface_enthalpy():
     steps(m, 1e9, 1, first=True)
     assert np.allclose(col(m, "TEMPERATURE"), Ts, rtol=1e-14)
     assert m.get("HW")[0, 0] == 0
+
+
+# ---- further reference column and patch tests (test/test_thermodynamics.jl:293-470) ------------------------------
+
+def test_thickening_or_thinning_column_follows_the_heat_equation_in_fixed_height():
+    H0, G, Tb, Bamp = 1000.0, 0.04, 240.0, 5.0
+    lam, t_end = math.pi / 1000, 2e10
+    for a0 in (1e-8, -1e-8):
+        errors = []
+        for Ns in (10, 20, 40):
+            m, g = column_model(H0=H0, Ns=Ns, G=G)
+            m.set("SMB", np.full((1, 1), a0))
+            kappa = K / (910.0 * CI)
+
+            def exact(z, t):
+                return Tb - G * z / K + Bamp * math.exp(-kappa * lam ** 2 * t) * np.cos(lam * z)
+
+            set_temperature(m, exact(centers(m) * H0, 0.0))
+            Nt = 4 * Ns ** 2
+            dt = t_end / Nt
+            for n in range(1, Nt + 1):
+                m.set("TSURF", np.full((1, 1), exact(H0 + a0 * n * dt, n * dt)))
+                transport.time_step(m, dt, thermodynamics=TH, first=(n == 1))
+            H = m.get("H")[0, 0]
+            assert math.isclose(H, H0 + a0 * t_end, rel_tol=1e-12)
+            errors.append(float(np.max(np.abs(col(m, "TEMPERATURE") - exact(centers(m) * H, t_end)))))
+        assert errors[2] < 0.03
+        assert errors[0] / errors[1] > 2.5 and errors[1] / errors[2] > 2.5
+
+
+def patch_model(topology, T=250.0, Ts=250.0, G=0.0, cls=O.OracleModel):
+    """patch_model of test/test_thermodynamics.jl:44-58 (thickness advected upwind here): 8 × 8, variable thickness,
+    flow shearing with depth, accumulation and melt."""
+    L = 8e4
+    g = grid(8, 8, Lx=L, Ly=L, topology=topology, Nsigma=10, bed_refinement=2.0)
+    phys = abi.default_physics(masking=0, grounding=abi.GROUNDING_NONE, ice_density=910.0, rate_factor_depth=abi.RATE_FACTOR_3D)
+    m = cls(g, phys, abi.default_options())
+    from helpers import field_values
+    ubar = lambda x, y: 1e-6 * (1 + np.sin(2 * np.pi * y / L) / 2)  # noqa: E731
+    vbar = lambda x, y: 5e-7 * np.cos(2 * np.pi * x / L)  # noqa: E731
+    z = np.array(O.OracleModel(g, phys, abi.default_options()).sigma_metrics()[1])
+    H = field_values(g, "H", lambda x, y: 1000 + 200 * np.sin(2 * np.pi * x / L) * np.cos(2 * np.pi * y / L))
+    m.set("H", H)
+    m.set("HN", H)
+    m.set("BED", field_values(g, "BED", -5000.0))
+    m.set("U", field_values(g, "U", ubar))
+    m.set("V", field_values(g, "V", vbar))
+    m.set("USIGMA", np.stack([field_values(g, "U", ubar) * (0.5 + zk) for zk in z]))
+    m.set("VSIGMA", np.stack([field_values(g, "V", vbar) * (0.5 + zk) for zk in z]))
+    m.set("SMB", field_values(g, "SMB", lambda x, y: 1e-8 * (1 + np.cos(2 * np.pi * x / L))))
+    m.set("BASAL_MELT", field_values(g, "BASAL_MELT", 2e-9))
+    m.set("GEOTHERMAL", field_values(g, "H", G))
+    m.set("TSURF", field_values(g, "H", Ts))
+    m.set("TOCEAN", field_values(g, "H", 271.15))
+    Tf = T(g, z) if callable(T) else np.full((10, 8, 8), T)
+    m.set("TEMPERATURE", Tf)
+    m.rate_factor_from_temperature()
+    m.update_geometry()
+    return m, g, z
+
+
+def test_uniform_enthalpy_stays_uniform_under_variable_thickness_shear_accumulation_and_melt():
+    m, g, z = patch_model((P, P))
+    E0 = m.get("ENTHALPY")[0, 0, 0]
+    H0 = m.get("H").copy()
+    for k in range(20):
+        transport.time_step(m, 1e9, thermodynamics=TH, first=(k == 0))
+    assert np.max(np.abs(m.get("H") - H0)) > 10
+    assert np.max(np.abs(m.get("ENTHALPY") - E0)) < 1e-9 * E0
+
+
+def test_strong_accumulation_over_a_sharp_front_creates_no_new_extrema():
+    m, g = column_model(H0=1000.0, Ns=10, G=0.0, Ts=250.0)
+    m.set("SMB", np.full((1, 1), 2e-7))
+    set_temperature(m, np.where(centers(m) < 0.5, 230.0, 260.0))
+    for k in range(20):
+        transport.time_step(m, 1e8, thermodynamics=TH, first=(k == 0))
+        T = col(m, "TEMPERATURE")
+        assert T.min() >= 230 - 1e-8 and T.max() <= 260 + 1e-8
+
+
+def test_temperate_layers_drain_towards_the_drainage_water_fraction():
+    H0, wd, Rmax = 1000.0, 0.01, 1.58443823077064e-9
+    m, g = column_model(H0=H0, G=0.0, Ts=263.15, Ns=10)
+    m.set("ENTHALPY", np.full((10, 1, 1), CI * (T0 - TR) + 5 * wd * L))
+    m.thermal_diagnostics()
+    assert col(m, "WATER_FRACTION").min() > 3 * wd
+    transport.time_step(m, 1.0, thermodynamics=TH, first=True)
+    assert math.isclose(m.get("HW")[0, 0], H0 * Rmax * 1.0, rel_tol=1e-8)
+    for _ in range(100):
+        transport.time_step(m, 1e9, thermodynamics=TH)
+    assert col(m, "WATER_FRACTION").max() <= wd + 1e-12
+    assert m.get("HW")[0, 0] > 0.1 * H0 * wd
+
+
+def test_coupled_grounded_melt_moves_mass_between_the_ice_and_the_basal_water():
+    H0, G, Ts = 1000.0, 0.1, 263.15
+    coupled, _ = column_model(H0=H0, G=G, Ts=Ts, Ns=20)
+    fixed, _ = column_model(H0=H0, G=G, Ts=Ts, Ns=20)
+    TC = transport.Thermodynamics(grounded_melt_coupling=True)
+    for mm in (coupled, fixed):
+        set_temperature(mm, Ts + 10 * (1 - centers(mm)))
+    dt = 1e10
+    for k in range(30):
+        transport.time_step(coupled, dt, thermodynamics=TC, first=(k == 0))
+        transport.time_step(fixed, dt, thermodynamics=TH, first=(k == 0))
+        total = coupled.get("H")[0, 0] + coupled.get("HW")[0, 0] - dt * coupled.get("GROUNDED_MELT")[0, 0]
+        assert math.isclose(total, H0, rel_tol=1e-12)
+        assert fixed.get("H")[0, 0] == H0
+    assert coupled.get("GROUNDED_MELT")[0, 0] > 0
+    assert coupled.get("H")[0, 0] < H0
+
+
+def test_a_cold_base_leaves_the_thickness_as_a_model_without_thermodynamics():
+    from helpers import B, field_values
+    g = grid(8, 1, Lx=8e3, Ly=1e3, topology=(B, P), Nsigma=10)
+    phys = abi.default_physics(masking=0, grounding=abi.GROUNDING_NONE, rate_factor_depth=abi.RATE_FACTOR_3D)
+    models = []
+    for thermal in (False, True):
+        m = O.OracleModel(g, phys, abi.default_options())
+        m.set("H", field_values(g, "H", lambda x, y: 100 + x / 100))
+        m.set("U", field_values(g, "U", lambda x, y: np.where((x > 0) & (x < 8e3), 1e-6, 0.0)))
+        m.set("BED", field_values(g, "BED", 0.0))
+        if thermal:
+            m.set("HN", m.get("H"))
+            m.set("GEOTHERMAL", field_values(g, "H", 0.05))
+            m.set("TSURF", field_values(g, "H", 243.15))
+            m.set("TOCEAN", field_values(g, "H", 271.15))
+            m.set("ENTHALPY", np.full((10, 1, 8), CI * (243.15 - TR)))
+        for k in range(5):
+            transport.time_step(m, 1e7, thermodynamics=transport.Thermodynamics(True) if thermal else None, first=(k == 0))
+        models.append(m)
+    assert np.array_equal(models[0].get("H"), models[1].get("H"))
+
+
+def dissipation_mismatch(stress_balance, cls=O.OracleModel):
+    """dissipation_mismatch of test/test_thermodynamics.jl:60-96: one time step of a 16 × 16 doubly periodic sheet
+    with the velocity solve in every stage; the frictional heat qᵍ plus the column strain heat H Σ Δζ Φ against the
+    power of the driving stress Σ τ·u."""
+    from helpers import field_values
+    Nx, L = 16, 3.2e5
+    g = grid(Nx, Nx, Lx=L, Ly=L, topology=(P, P), Nsigma=10)
+    phys = abi.default_physics(masking=0, grounding=abi.GROUNDING_NONE, ice_density=910.0, stress_balance=stress_balance,
+                               friction=abi.FRICTION_WEERTMAN, friction_exponent=1.0, strain_rate_regularization=1e-16,
+                               rate_factor_depth=abi.RATE_FACTOR_2D)
+    opts = abi.default_options(linear=abi.convergence(1e-12, 0.0, 10_000), picard=abi.convergence(1e-11, 0.0, 500))
+    m = cls(g, phys, opts)
+    m.set("H", field_values(g, "H", lambda x, y: 1000 + 100 * np.sin(2 * np.pi * x / L) * np.cos(2 * np.pi * y / L)))
+    m.set("HN", m.get("H"))
+    m.set("BED", field_values(g, "BED", 0.0))
+    m.set("A", field_values(g, "H", 1e-24))
+    m.set("C", field_values(g, "H", 1e10))
+    m.set("TSURF", field_values(g, "H", 243.15))
+    m.set("TOCEAN", field_values(g, "H", 271.15))
+    m.set("ENTHALPY", np.full((10, Nx, Nx), CI * (243.15 - TR)))
+    transport.time_step(m, 1e6, solve_velocities=True, first=True, thermodynamics=transport.Thermodynamics(True))
+    zf = np.array(O.OracleModel(g, phys, opts).sigma_metrics()[0])
+    strain = np.tensordot(np.diff(zf), m.get("STRAIN_HEATING"), axes=1)
+    diva = stress_balance == abi.DIVA
+    bu, bv = m.get("BETABU" if diva else "BETAU"), m.get("BETABV" if diva else "BETAV")
+    ub, vb = m.get("UB"), m.get("VB")
+    fu, fv = bu * ub * ub, bv * vb * vb
+    F = (fu + np.roll(fu, -1, axis=1) + fv + np.roll(fv, -1, axis=0)) / 2
+    heat = (L / Nx) ** 2 * float(np.sum(F + m.get("H") * strain))
+    power = float(np.sum(m.get("TAUU") * m.get("U")) + np.sum(m.get("TAUV") * m.get("V")))
+    return abs(heat - power) / power
+
+
+@pytest.mark.parametrize("stress_balance", [abi.SSA, abi.DIVA])
+def test_friction_and_strain_heat_account_for_the_work_of_the_driving_stress(stress_balance):
+    assert dissipation_mismatch(stress_balance) < 1e-8
