@@ -314,8 +314,23 @@ nk_status nk_compute_strain_heating(nk_solver *solver);
  * under SSA, BETABU/BETABV under DIVA), the implicit column system with STRAIN_HEATING, TSURF and TOCEAN, the
  * tridiagonal solve, drainage and the basal water (HW, GROUNDED_MELT = mᵍ), then update_thermal_diagnostics!
  * (TEMPERATURE, WATER_FRACTION; A with NK_RATE_FACTOR_3D) and HN = H. Needs Nσ ≥ 1 and the σ grid of the handle;
- * the first call allocates the column system's workspace (≈ (6 Nσ + 5) planes of FP64). */
-nk_status nk_energy_step(nk_solver *solver, double dt, int32_t grounded_melt_coupling);
+ * the first call allocates the column system's workspace (≈ (6 Nσ + 5) planes of FP64). With a till law, the basal
+ * water then drains and is capped (drain_till); till == NULL is a bed connected to the ocean (no till). */
+typedef struct nk_till {
+    /* TillEffectivePressure (src/Thermodynamics/till_effective_pressure.jl:16-61); nk_default_till gives its defaults */
+    double reference_effective_pressure;   /* N₀ = 1000 Pa */
+    double overburden_fraction;            /* δ = 0.02 */
+    double reference_void_ratio;           /* e₀ = 0.69 */
+    double compressibility;                /* Cc = 0.12 */
+    double maximum_water_thickness;        /* Wᵐᵃˣ = 2 m */
+    double drainage_rate;                  /* 1 mm per year, in m/s */
+} nk_till;
+void nk_default_till(nk_till *till);
+nk_status nk_energy_step(nk_solver *solver, double dt, int32_t grounded_melt_coupling, const nk_till *till);
+/* compute_effective_pressure! with a till law (src/Groundings/subgrid_grounded_fraction.jl:168-183, effective_pressure
+ * of till_effective_pressure.jl:66-73): N = min(N₀ (δ P̃/N₀)ˢ 10^((e₀/Cc)(1 − s)), Nᵒ), s = clamp(HW/Wᵐᵃˣ, 0, 1),
+ * P₀ = ρᵢ g H, P̃ = P₀ (N₀/δ where P₀ = 0), Nᵒ = max(P₀ + ρw g min(BED, 0), 0). Replaces the N of update_geometry. */
+nk_status nk_till_effective_pressure(nk_solver *solver, const nk_till *till);
 
 /* Fixed-point adjoint of the converged SSA Picard solve (the EnzymeRules reverse rule of picard_solve!,
  * ext/NunatakEnzymeExt.jl:175-240, Christianson 1994), hand-written. Inputs: the converged solution in (U, V) and the

@@ -1072,6 +1072,16 @@ void nk_default_physics(nk_physics *p) {
     p->walls = NK_FREE_SLIP; p->stress_balance = NK_SSA;
 }
 
+void nk_default_till(nk_till *t) {
+    if (!t) return;
+    t->reference_effective_pressure = 1000.0;
+    t->overburden_fraction = 0.02;
+    t->reference_void_ratio = 0.69;
+    t->compressibility = 0.12;
+    t->maximum_water_thickness = 2.0;
+    t->drainage_rate = 1e-3 / (365.25 * 86400.0);
+}
+
 void nk_default_options(nk_solver_options *o) {
     std::memset(o, 0, sizeof(*o));
     o->linear.kind = NK_CONVERGENCE; o->linear.reltol = 1e-8; o->linear.maxiter = 10000;
@@ -1631,7 +1641,18 @@ nk_status nk_compute_strain_heating(nk_solver *s) {
     return NK_OK;
 }
 
-nk_status nk_energy_step(nk_solver *s, double dt, int32_t grounded_melt_coupling) {
+nk_status nk_till_effective_pressure(nk_solver *s, const nk_till *till) {
+    if (!s || !till) return NK_INVALID_ARGUMENT;
+    double **f = s->f;
+    const PhysK p = physk(s);
+    k_till_pressure<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(s->g, p.rig, p.rwg, *till, f[NK_FIELD_H], f[NK_FIELD_BED],
+                                                                             f[NK_FIELD_HW], f[NK_FIELD_N]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_energy_step(nk_solver *s, double dt, int32_t grounded_melt_coupling, const nk_till *till) {
     if (!s) return NK_INVALID_ARGUMENT;
     if (!(dt > 0.0) || s->Ns < 1) { s->err = "nk_energy_step: dt must be positive and the grid needs σ layers"; return NK_INVALID_ARGUMENT; }
     if (!s->en_W) {
@@ -1662,6 +1683,9 @@ nk_status nk_energy_step(nk_solver *s, double dt, int32_t grounded_melt_coupling
     e.E = f[NK_FIELD_ENTHALPY]; e.Hw = f[NK_FIELD_HW]; e.mg = f[NK_FIELD_GROUNDED_MELT]; e.T = f[NK_FIELD_TEMPERATURE];
     e.W = f[NK_FIELD_WATER_FRACTION]; e.A = f[NK_FIELD_A];
     e.W3 = s->en_W; e.a = s->en_a; e.b = s->en_b; e.c = s->en_c; e.f = s->en_f; e.t = s->en_t; e.bas = s->en_bas;
+    e.till = till ? 1 : 0;
+    e.till_rate = till ? till->drainage_rate : 0.0;
+    e.till_wmax = till ? till->maximum_water_thickness : 0.0;
     const dim3 B = blocks_for(s->g.Nx, s->g.Ny);
     k_energy_assemble<<<B, THREADS, 0, s->stream>>>(s->g, sig_tables(s), e);
     CHK(launch_check(s));

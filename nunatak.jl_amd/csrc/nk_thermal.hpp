@@ -6,6 +6,7 @@
 #pragma once
 #include "nk_device.hpp"
 #include "nk_physics.hpp"
+#include "../../include/nunatak_hip.h"
 
 namespace nk {
 
@@ -17,7 +18,8 @@ struct EP {
 
 struct EnergyArgs {
     double rho, grav, dt;
-    int grounding, coupled, Ns, a3d;
+    int grounding, coupled, Ns, a3d, till;
+    double till_rate, till_wmax;   // drain_till (till_effective_pressure.jl:77-78) when till
     long plane;
     const double *H, *Hn, *us, *vs, *ub, *vb, *bu, *bv, *phi, *m, *G, *Ts, *To, *Phi;
     double *E, *Hw, *mg, *T, *W, *A, *W3, *a, *b, *c, *f, *t, *bas;   // W3: Ns + 1 face planes; bas: rᵇ, sᵇ, Dᵇ, qᵍ
@@ -163,6 +165,22 @@ __global__ __launch_bounds__(NT) void k_energy_assemble(KG g, Sig sg, EnergyArgs
     }
 }
 
+// N with a saturated-till law (till_effective_pressure.jl:66-73, subgrid_grounded_fraction.jl:176-183)
+__global__ __launch_bounds__(NT) void k_till_pressure(KG g, double rig, double rwg, nk_till t, const double *__restrict__ H,
+                                                      const double *__restrict__ b, const double *__restrict__ Hw,
+                                                      double *__restrict__ N) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    const double P0 = rig * H[o];
+    const double No = jmax(P0 + rwg * jmin(b[o], 0.0), 0.0);
+    const double N0 = t.reference_effective_pressure, d = t.overburden_fraction;
+    const double s = jclamp(Hw[o] / t.maximum_water_thickness, 0.0, 1.0);
+    const double Pt = P0 > 0.0 ? P0 : N0 / d;
+    const double Nt = N0 * pow(d * Pt / N0, s) * pow(10.0, t.reference_void_ratio / t.compressibility * (1.0 - s));
+    N[o] = jmin(Nt, No);
+}
+
 // drainage_rate (column_energy_solve.jl:230-234)
 __device__ __forceinline__ double drainage_rate(double w) {
     const double x = w / EP::wd;
@@ -216,7 +234,7 @@ __global__ __launch_bounds__(NT) void k_energy_solve(KG g, Sig sg, EnergyArgs e,
     const double Hplus = jmax(0.0, Hstar);
     const double missing = Hplus - Hstar;
     e.mg[o] = (Hplus - Hw0) / e.dt;
-    e.Hw[o] = Hplus;
+    e.Hw[o] = e.till ? jclamp(Hplus - e.dt * e.till_rate, 0.0, e.till_wmax) : Hplus;
     e.E[o] = e.E[o] - L * missing / (Ht * sg.dz[0]);
     // thermal diagnostics
     for (int k = 0; k < Ns; k++) {

@@ -93,6 +93,20 @@ class Info(ct.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if not name.startswith("pad")}
 
 
+class Till(ct.Structure):
+    _fields_ = [("reference_effective_pressure", ct.c_double), ("overburden_fraction", ct.c_double),
+                ("reference_void_ratio", ct.c_double), ("compressibility", ct.c_double),
+                ("maximum_water_thickness", ct.c_double), ("drainage_rate", ct.c_double)]
+
+
+def default_till(**kw):
+    """TillEffectivePressure defaults (src/Thermodynamics/till_effective_pressure.jl:47-61), overridable by keyword."""
+    t = Till(1000.0, 0.02, 0.69, 0.12, 2.0, 1e-3 / (365.25 * 86400.0))
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
 class ServerConfig(ct.Structure):
     _fields_ = [("slots", ct.c_int32), ("pad", ct.c_int32), ("hbm_budget_bytes", ct.c_int64)]
 

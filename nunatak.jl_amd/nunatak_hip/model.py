@@ -22,7 +22,8 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_update_geometry", "nk_compute_driving_stress", "nk_update_coefficients", "nk_apply_operator",
            "nk_precondition", "nk_solve", "nk_picard_solve", "nk_update_velocities", "nk_compute_velocity_profile",
            "nk_thermal_diagnostics", "nk_compute_thickness_tendency", "nk_thickness_substep", "nk_cache_thickness",
-           "nk_cell_advection_timescale", "nk_compute_strain_heating", "nk_energy_step",
+           "nk_cell_advection_timescale", "nk_compute_strain_heating", "nk_energy_step", "nk_default_till",
+           "nk_till_effective_pressure",
            "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint", "nk_device_bytes",
            "nk_estimate_device_bytes", "nk_server_create", "nk_server_destroy", "nk_server_last_error",
            "nk_server_submit", "nk_server_poll", "nk_server_wait"]
@@ -74,7 +75,10 @@ def load(path=LIB_PATH):
     L.nk_compute_thickness_tendency.argtypes = [P, I32, I32]
     L.nk_thickness_substep.argtypes = [P, D, I32]
     L.nk_cell_advection_timescale.argtypes = [P, ct.POINTER(D)]
-    L.nk_energy_step.argtypes = [P, D, I32]
+    L.nk_energy_step.argtypes = [P, D, I32, ct.POINTER(abi.Till)]
+    L.nk_till_effective_pressure.argtypes = [P, ct.POINTER(abi.Till)]
+    L.nk_default_till.argtypes = [ct.POINTER(abi.Till)]
+    L.nk_default_till.restype = None
     L.nk_device_bytes.argtypes = [P, ct.POINTER(I64)]
     L.nk_estimate_device_bytes.argtypes = [ct.POINTER(abi.Grid), ct.POINTER(abi.Physics), ct.POINTER(abi.SolverOptions),
                                            ct.POINTER(I64)]
@@ -209,8 +213,12 @@ class HIPModel:
     def compute_strain_heating(self):
         self._check(self._L.nk_compute_strain_heating(self._h), "nk_compute_strain_heating")
 
-    def energy_step(self, dt, grounded_melt_coupling=False):
-        self._check(self._L.nk_energy_step(self._h, dt, 1 if grounded_melt_coupling else 0), "nk_energy_step")
+    def energy_step(self, dt, grounded_melt_coupling=False, till=None):
+        self._check(self._L.nk_energy_step(self._h, dt, 1 if grounded_melt_coupling else 0,
+                                           ct.byref(till) if till is not None else None), "nk_energy_step")
+
+    def till_effective_pressure(self, till):
+        self._check(self._L.nk_till_effective_pressure(self._h, ct.byref(till)), "nk_till_effective_pressure")
 
     def thermal_diagnostics(self):
         self._check(self._L.nk_thermal_diagnostics(self._h), "nk_thermal_diagnostics")

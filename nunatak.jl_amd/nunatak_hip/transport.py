@@ -17,22 +17,29 @@ SPLIT_RK3 = (3.0, 2.0, 1.0)
 
 class Thermodynamics:
     """EnthalpyThermodynamics switches of the step: the energy is advanced in the final stage of every time step
-    (schedule IterationInterval(1)); `grounded_melt_coupling` takes the basal water's change from the thickness."""
+    (schedule IterationInterval(1)); `grounded_melt_coupling` takes the basal water's change from the thickness;
+    `till` (an abi.Till) sets N from the basal water and drains the till (TillEffectivePressure)."""
 
-    def __init__(self, grounded_melt_coupling=False):
+    def __init__(self, grounded_melt_coupling=False, till=None):
         self.grounded_melt_coupling = grounded_melt_coupling
+        self.till = till
 
 
 def update_state(model, solve_velocities, thermodynamics=None, energy_dt=None):
     """update_state! (src/Models/ice_sheet_model_methods.jl:80-86): geometry, then the velocities if solved, then in
     the final stage of a step the energy (strain heating, then advance_enthalpy! over the step)."""
-    if solve_velocities:
+    till = thermodynamics.till if thermodynamics is not None else None
+    if solve_velocities and till is None:
         model.update_velocities(geometry=True)
     else:
         model.update_geometry()
+        if till is not None:
+            model.till_effective_pressure(till)
+        if solve_velocities:
+            model.update_velocities(geometry=False)
     if thermodynamics is not None and energy_dt:
         model.compute_strain_heating()
-        model.energy_step(energy_dt, thermodynamics.grounded_melt_coupling)
+        model.energy_step(energy_dt, thermodynamics.grounded_melt_coupling, till)
 
 
 def time_step(model, dt, advection=abi.ADVECTION_UPWIND1, solve_velocities=False, first=False, thermodynamics=None):

@@ -1397,7 +1397,27 @@ EXPORT void nko_compute_strain_heating(nko_model *m) {
 }
 
 /* advance_enthalpy! (column_energy_solve.jl:48-84) with update_thermal_diagnostics! and Hⁿ = H */
-EXPORT void nko_energy_step(nko_model *m, double dt, int coupled) {
+/* effective_pressure with TillEffectivePressure (till_effective_pressure.jl:66-73) in _compute_effective_pressure!
+ * (subgrid_grounded_fraction.jl:176-183) */
+EXPORT void nko_till_effective_pressure(nko_model *m, const nk_till *t) {
+    const og *g = &m->g;
+    const double *H = m->f[NK_FIELD_H], *b = m->f[NK_FIELD_BED], *Hw = m->f[NK_FIELD_HW];
+    double *N = m->f[NK_FIELD_N];
+    double rig = m->ph.ice_density * m->ph.gravitational_acceleration, rwg = m->ph.ocean_density * m->ph.gravitational_acceleration;
+    for (int j = 1; j <= g->Ny; j++)
+        for (int i = 1; i <= g->Nx; i++) {
+            double P0 = rig * AT(g, H, i, j);
+            double No = jmax(P0 + rwg * jmin(AT(g, b, i, j), 0.0), 0.0);
+            double N0 = t->reference_effective_pressure, d = t->overburden_fraction;
+            double s = jclamp(AT(g, Hw, i, j) / t->maximum_water_thickness, 0.0, 1.0);
+            double Pt = P0 > 0.0 ? P0 : N0 / d;
+            double Nt = N0 * pow(d * Pt / N0, s) * pow(10.0, t->reference_void_ratio / t->compressibility * (1.0 - s));
+            AT(g, N, i, j) = jmin(Nt, No);
+        }
+    fill_field_halo(m, NK_FIELD_N);
+}
+
+EXPORT void nko_energy_step(nko_model *m, double dt, int coupled, const nk_till *till) {
     const og *g = &m->g;
     const int Ns = m->Ns;
     const double rho = m->ph.ice_density, rg = m->ph.ice_density * m->ph.gravitational_acceleration, Az = g->dx * g->dy;
@@ -1518,7 +1538,8 @@ EXPORT void nko_energy_step(nko_model *m, double dt, int coupled) {
             double Hstar = Hw0 + dt * (mu ? melt : 0.0) + drained;
             double Hplus = jmax(0.0, Hstar), missing = Hplus - Hstar;
             AT(g, mg, i, j) = (Hplus - Hw0) / dt;
-            AT(g, Hw, i, j) = Hplus;
+            /* drain_till (till_effective_pressure.jl:77-78) */
+            AT(g, Hw, i, j) = till ? jclamp(Hplus - dt * till->drainage_rate, 0.0, till->maximum_water_thickness) : Hplus;
             x[1] = x[1] - E_L * missing / (Ht * m->dz[1]);
             for (int k = 1; k <= Ns; k++) Enew[(size_t)(k - 1) * g->plane + IDX(g, i, j)] = x[k];
         }

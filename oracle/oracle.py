@@ -59,7 +59,8 @@ def lib():
         L.nko_thickness_substep.argtypes = [P, D, I]
         L.nko_cell_advection_timescale.argtypes = [P]
         L.nko_cell_advection_timescale.restype = D
-        L.nko_energy_step.argtypes = [P, D, I]
+        L.nko_energy_step.argtypes = [P, D, I, ct.c_void_p]
+        L.nko_till_effective_pressure.argtypes = [P, ct.c_void_p]
         for name, args in [("nko_ice_mask_value", [D, D, D]), ("nko_segment_grounded_fraction", [D, D]),
                            ("nko_triangle_grounded_fraction", [D, D, D]), ("nko_arrhenius_rate_factor", [D, D]),
                            ("nko_basal_drag", [D, D, D, I, D, D])]:
@@ -159,8 +160,11 @@ class OracleModel:
     def compute_strain_heating(self):
         lib().nko_compute_strain_heating(self._h)
 
-    def energy_step(self, dt, grounded_melt_coupling=False):
-        lib().nko_energy_step(self._h, dt, 1 if grounded_melt_coupling else 0)
+    def energy_step(self, dt, grounded_melt_coupling=False, till=None):
+        lib().nko_energy_step(self._h, dt, 1 if grounded_melt_coupling else 0, ct.byref(till) if till is not None else None)
+
+    def till_effective_pressure(self, till):
+        lib().nko_till_effective_pressure(self._h, ct.byref(till))
 
     def solve(self):
         info = abi.Info()

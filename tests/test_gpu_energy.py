@@ -116,3 +116,21 @@ def test_uniform_enthalpy_stays_uniform_on_the_device():
         transport.time_step(m, 1e9, thermodynamics=TE.TH, first=(k == 0))
     assert np.max(np.abs(m.get("H") - H0)) > 10
     assert np.max(np.abs(m.get("ENTHALPY") - E0)) < 1e-9 * E0
+
+
+def test_till_pressure_and_drainage_on_the_device():
+    """test/test_till_effective_pressure.jl:54-80 through the device, and nk_till_effective_pressure against the
+    oracle on a random sheet (device pow: 1e-13)."""
+    say("energy: till")
+    TE.test_the_till_drains_and_is_capped_and_the_model_sees_the_till_pressure(cls=HIPModel)
+    rng = np.random.default_rng(4)
+    g = grid(19, 13, Lx=19e3, Ly=13e3, topology=(B, P), Nsigma=2)
+    phys = abi.default_physics(masking=0)
+    h, o = HIPModel(g, phys, abi.default_options()), O.OracleModel(g, phys, abi.default_options())
+    f = {"H": 3000 * rng.random((13, 19)) * (rng.random((13, 19)) > 0.1), "BED": -1500 + 2000 * rng.random((13, 19)),
+         "HW": 3 * rng.random((13, 19))}
+    for m in (h, o):
+        for k, v in f.items():
+            m.set(k, v)
+        m.till_effective_pressure(abi.default_till())
+    assert np.allclose(h.get("N"), o.get("N"), rtol=1e-13, atol=0)

@@ -335,3 +335,76 @@ def dissipation_mismatch(stress_balance, cls=O.OracleModel):
 @pytest.mark.parametrize("stress_balance", [abi.SSA, abi.DIVA])
 def test_friction_and_strain_heat_account_for_the_work_of_the_driving_stress(stress_balance):
     assert dissipation_mismatch(stress_balance) < 1e-8
+
+
+# ---- till effective pressure (test/test_till_effective_pressure.jl) ------------------------------------------------
+
+def till_formula(t, P0, Hw):
+    s = min(max(Hw / t.maximum_water_thickness, 0.0), 1.0)
+    Nt = t.reference_effective_pressure * (t.overburden_fraction * P0 / t.reference_effective_pressure) ** s * \
+        10 ** (t.reference_void_ratio / t.compressibility * (1 - s))
+    return min(Nt, P0)
+
+
+def test_dry_partly_wet_saturated_and_floating_till():
+    from helpers import B, field_values
+    g = grid(5, 1, Lx=5.0, Ly=1.0, topology=(B, P), Nsigma=2)
+    m = O.OracleModel(g, abi.default_physics(masking=0), abi.default_options())
+    t = abi.default_till()
+    m.set("H", np.full((1, 5), 1000.0))
+    m.set("BED", field_values(g, "BED", lambda x, y: np.where(x < 4, 0.0, -2000.0)))
+    m.set("HW", np.array([[0.0, 1.5, 2.0, 3.0, 1.0]]))
+    m.till_effective_pressure(t)
+    P0 = 918.0 * 9.81 * 1000.0
+    N = m.get("N")[0]
+    assert math.isclose(N[0], P0, rel_tol=1e-14)
+    assert math.isclose(N[1], till_formula(t, P0, 1.5), rel_tol=1e-13) and N[1] < P0
+    assert math.isclose(N[2], t.overburden_fraction * P0, rel_tol=1e-13)
+    assert math.isclose(N[3], t.overburden_fraction * P0, rel_tol=1e-13)
+    assert N[4] == 0
+    assert N[0] > N[1] > N[2]
+
+
+def test_ice_free_cells_have_zero_effective_pressure():
+    from helpers import B
+    g = grid(2, 1, Lx=2.0, Ly=1.0, topology=(B, P), Nsigma=2)
+    m = O.OracleModel(g, abi.default_physics(masking=0), abi.default_options())
+    m.set("H", np.array([[0.0, 100.0]]))
+    m.set("HW", np.ones((1, 2)))
+    m.till_effective_pressure(abi.default_till())
+    N = m.get("N")[0]
+    assert N[0] == 0 and N[1] > 0
+
+
+def till_column_model(till, cls=O.OracleModel):
+    g = grid(2, 1, Lx=2e3, Ly=1e3, topology=(P, P), Nsigma=10)
+    phys = abi.default_physics(masking=0, ice_density=910.0, rate_factor_depth=abi.RATE_FACTOR_3D)
+    m = cls(g, phys, abi.default_options())
+    m.set("H", np.full((1, 2), 1000.0))
+    m.set("HN", np.full((1, 2), 1000.0))
+    m.set("TSURF", np.full((1, 2), 243.15))
+    m.set("TOCEAN", np.full((1, 2), 271.15))
+    m.set("ENTHALPY", np.full((10, 1, 2), CI * (243.15 - TR)))
+    m.set("HW", np.array([[1.5, 3.0]]))
+    return m, transport.Thermodynamics(grounded_melt_coupling=True, till=till)
+
+
+def test_the_till_drains_and_is_capped_and_the_model_sees_the_till_pressure(cls=O.OracleModel):
+    dt = 365.25 * 86400
+    law = abi.default_till()
+    ref, tr = till_column_model(None, cls)
+    till, tt = till_column_model(law, cls)
+    transport.time_step(ref, dt, thermodynamics=tr, first=True)
+    transport.time_step(till, dt, thermodynamics=tt, first=True)
+    Hwr, Hwt = ref.get("HW")[0], till.get("HW")[0]
+    assert math.isclose(Hwt[0], Hwr[0] - dt * law.drainage_rate, rel_tol=1e-12)
+    assert Hwt[1] == law.maximum_water_thickness
+    assert Hwr[1] > law.maximum_water_thickness
+    assert np.array_equal(till.get("GROUNDED_MELT"), ref.get("GROUNDED_MELT"))
+    P0 = 910.0 * 9.81 * till.get("H")[0]
+    N = till.get("N")[0]
+    # the geometry, and so N, is diagnosed before the energy step updates the water
+    assert np.allclose(N, [till_formula(law, P0[0], 1.5), till_formula(law, P0[1], 3.0)], rtol=1e-13)
+    transport.update_state(till, False, tt)
+    assert np.allclose(till.get("N")[0], [till_formula(law, P0[k], Hwt[k]) for k in range(2)], rtol=1e-13)
+    assert np.allclose(ref.get("N")[0], P0, rtol=1e-14)
