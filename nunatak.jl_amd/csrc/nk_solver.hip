@@ -100,10 +100,10 @@ struct nk_solver {
     const double *fz_r64u = nullptr, *fz_r64v = nullptr;
     double *fz_z64u = nullptr, *fz_z64v = nullptr;
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
-    bool fuse_pre = true;
+    bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     // enthalpy step workspace, allocated by the first nk_energy_step: face volume fluxes (Nσ + 1 planes), the column
     // system a, b, c, f and the solver's t (Nσ planes each), the basal state rᵇ, sᵇ, Dᵇ, qᵍ (4 planes)
-    double *en_W = nullptr, *en_a = nullptr, *en_b = nullptr, *en_c = nullptr, *en_f = nullptr, *en_t = nullptr, *en_bas = nullptr;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
+    double *en_W = nullptr, *en_a = nullptr, *en_b = nullptr, *en_c = nullptr, *en_f = nullptr, *en_t = nullptr, *en_bas = nullptr;
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
     unsigned int *counter = nullptr;
@@ -636,10 +636,10 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
         return NK_OK;
     }
     CHK(smooth(cur, oth, s->op.smoothing_sweeps, true));
-    NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
-    CHK(launch_check(s));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
+    NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+    CHK(launch_check(s));
     NK_LAUNCH_TOPO(C.g, k_restrict, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, L.g, C.cx, C.cy, C.act, L.act, V.qu, V.qv, CV.bu, CV.bv, gate);
     CHK(launch_check(s));
     Pair<T> cres;
