@@ -127,19 +127,22 @@ template <typename T, typename FV> __device__ __forceinline__ T iv_f(const KG &g
     return vact(g, act, i, j) ? val : T(0);
 }
 
-template <typename T, typename FU, typename FV>
-__device__ __forceinline__ void ssa_apply_f(const KG &g, int i, int j, FU fu, FV fv, const Coef<T> &c, T wf, T &qu, T &qv) {
+// The operator's arithmetic with every input behind an accessor: U(a, b), V(a, b) the masked iterate (as iu, iv
+// give it), E(a, b) η̄H at a centre (as ldc), FF(a, b) η̄Hᶠᶠ at a corner (as ldff); pu0, pv0, bu0, bv0 and f0 the
+// unmasked iterate, drag and activity byte at (i, j). One body for the global-memory and the LDS-tiled kernels.
+template <typename T, typename FU, typename FV, typename FE, typename FF>
+__device__ __forceinline__ void ssa_core(const KG &g, int i, int j, FU U, FV V, FE E, FF Ff, T pu0, T pv0, T bu0, T bv0, uint8_t f0,
+                                         T wf, T &qu, T &qv) {
     const T dx = (T)g.dx, dy = (T)g.dy, Az = (T)(g.dx * g.dy);
-    const uint8_t *a = c.act;
-    const T u00 = iu_f<T>(g, a, fu, i, j), u10 = iu_f<T>(g, a, fu, i + 1, j), um0 = iu_f<T>(g, a, fu, i - 1, j);
-    const T u0m = iu_f<T>(g, a, fu, i, j - 1), u0p = iu_f<T>(g, a, fu, i, j + 1), u1m = iu_f<T>(g, a, fu, i + 1, j - 1);
-    const T v00 = iv_f<T>(g, a, fv, i, j), v01 = iv_f<T>(g, a, fv, i, j + 1), vm0 = iv_f<T>(g, a, fv, i - 1, j);
-    const T vm1 = iv_f<T>(g, a, fv, i - 1, j + 1), v0m = iv_f<T>(g, a, fv, i, j - 1), v10 = iv_f<T>(g, a, fv, i + 1, j);
+    const T u00 = U(i, j), u10 = U(i + 1, j), um0 = U(i - 1, j);
+    const T u0m = U(i, j - 1), u0p = U(i, j + 1), u1m = U(i + 1, j - 1);
+    const T v00 = V(i, j), v01 = V(i, j + 1), vm0 = V(i - 1, j);
+    const T vm1 = V(i - 1, j + 1), v0m = V(i, j - 1), v10 = V(i + 1, j);
     // strain rates at the three centres
     const T exx_c = qd(u10 - u00, dx), eyy_c = qd(v01 - v00, dy);      // (i, j)
     const T exx_w = qd(u00 - um0, dx), eyy_w = qd(vm1 - vm0, dy);      // (i-1, j)
     const T exx_s = qd(u1m - u0m, dx), eyy_s = qd(v00 - v0m, dy);      // (i, j-1)
-    const T e_c = ldc(g, c.eta, i, j), e_w = ldc(g, c.eta, i - 1, j), e_s = ldc(g, c.eta, i, j - 1);
+    const T e_c = E(i, j), e_w = E(i - 1, j), e_s = E(i, j - 1);
     const T txx_c = T(2) * e_c * (T(2) * exx_c + eyy_c), txx_w = T(2) * e_w * (T(2) * exx_w + eyy_w);
     const T tyy_c = T(2) * e_c * (T(2) * eyy_c + exx_c), tyy_s = T(2) * e_s * (T(2) * eyy_s + exx_s);
     // shear strain at the three corners
@@ -149,18 +152,24 @@ __device__ __forceinline__ void ssa_apply_f(const KG &g, int i, int j, FU fu, FV
     exy_00 = periph_ff(g, i, j) ? wf * exy_00 : exy_00;
     exy_01 = periph_ff(g, i, j + 1) ? wf * exy_01 : exy_01;
     exy_10 = periph_ff(g, i + 1, j) ? wf * exy_10 : exy_10;
-    const T txy_00 = T(2) * ldff(g, c.etaff, i, j) * exy_00;
-    const T txy_01 = T(2) * ldff(g, c.etaff, i, j + 1) * exy_01;
-    const T txy_10 = T(2) * ldff(g, c.etaff, i + 1, j) * exy_10;
-    const long o = off(g, i, j);
-    const T pu0 = fu(o), pv0 = fv(o);
+    const T txy_00 = T(2) * Ff(i, j) * exy_00;
+    const T txy_01 = T(2) * Ff(i, j + 1) * exy_01;
+    const T txy_10 = T(2) * Ff(i + 1, j) * exy_10;
     const T divu = (dy * txx_c - dy * txx_w) + (dx * txy_01 - dx * txy_00);
     const T divv = (dx * tyy_c - dx * tyy_s) + (dy * txy_10 - dy * txy_00);
-    const T qu_ = Az * c.bu[o] * pu0 - divu;
-    const T qv_ = Az * c.bv[o] * pv0 - divv;
-    const uint8_t f = a[o];
-    qu = (f & 1) ? qu_ : pu0;
-    qv = (f & 2) ? qv_ : pv0;
+    const T qu_ = Az * bu0 * pu0 - divu;
+    const T qv_ = Az * bv0 * pv0 - divv;
+    qu = (f0 & 1) ? qu_ : pu0;
+    qv = (f0 & 2) ? qv_ : pv0;
+}
+
+template <typename T, typename FU, typename FV>
+__device__ __forceinline__ void ssa_apply_f(const KG &g, int i, int j, FU fu, FV fv, const Coef<T> &c, T wf, T &qu, T &qv) {
+    const uint8_t *a = c.act;
+    const long o = off(g, i, j);
+    ssa_core<T>(g, i, j, [&](int p, int q) { return iu_f<T>(g, a, fu, p, q); }, [&](int p, int q) { return iv_f<T>(g, a, fv, p, q); },
+                [&](int p, int q) { return ldc(g, c.eta, p, q); }, [&](int p, int q) { return ldff(g, c.etaff, p, q); }, fu(o), fv(o),
+                c.bu[o], c.bv[o], a[o], wf, qu, qv);
 }
 
 template <typename T>
@@ -187,17 +196,32 @@ template <typename T> __device__ __forceinline__ T y_diag(const KG &g, const Coe
     return (!(c.act[off(g, i, j)] & 2) || d == T(0)) ? T(1) : d;
 }
 // ℓ1 row sums (src/StressBalances/multigrid_preconditioner.jl:195-211)
+// with accessors E (η̄H at centres) and Ff (η̄Hᶠᶠ at corners), drag and activity of the row's point
+template <typename T, typename FE, typename FF>
+__device__ __forceinline__ T x_l1_f(const KG &g, int i, int j, T wf, FE E, FF Ff, T bu0, uint8_t f0) {
+    auto nd = [&](int p, int q) { return qd(T(4) * (T)g.dy * E(p, q), (T)g.dx); };
+    auto sd = [&](int p, int q) { return qd(corner_factor(g, p, q, wf) * (T)g.dx * Ff(p, q), (T)g.dy); };
+    T s = (T)(g.dx * g.dy) * bu0 + T(2) * (nd(i, j) + nd(i - 1, j)) + T(4) * (E(i, j) + E(i - 1, j)) + T(2) * (sd(i, j) + sd(i, j + 1)) +
+          T(2) * (corner_factor(g, i, j, wf) * Ff(i, j) + corner_factor(g, i, j + 1, wf) * Ff(i, j + 1));
+    return (!(f0 & 1) || s == T(0)) ? T(1) : s;
+}
+template <typename T, typename FE, typename FF>
+__device__ __forceinline__ T y_l1_f(const KG &g, int i, int j, T wf, FE E, FF Ff, T bv0, uint8_t f0) {
+    auto nd = [&](int p, int q) { return qd(T(4) * (T)g.dx * E(p, q), (T)g.dy); };
+    auto sd = [&](int p, int q) { return qd(corner_factor(g, p, q, wf) * (T)g.dy * Ff(p, q), (T)g.dx); };
+    T s = (T)(g.dx * g.dy) * bv0 + T(2) * (nd(i, j) + nd(i, j - 1)) + T(4) * (E(i, j) + E(i, j - 1)) + T(2) * (sd(i, j) + sd(i + 1, j)) +
+          T(2) * (corner_factor(g, i, j, wf) * Ff(i, j) + corner_factor(g, i + 1, j, wf) * Ff(i + 1, j));
+    return (!(f0 & 2) || s == T(0)) ? T(1) : s;
+}
 template <typename T> __device__ __forceinline__ T x_l1(const KG &g, const Coef<T> &c, int i, int j, T wf) {
-    T s = (T)(g.dx * g.dy) * c.bu[off(g, i, j)] + T(2) * (x_nd(g, c.eta, i, j) + x_nd(g, c.eta, i - 1, j)) +
-          T(4) * (ldc(g, c.eta, i, j) + ldc(g, c.eta, i - 1, j)) + T(2) * (x_sd(g, c.etaff, i, j, wf) + x_sd(g, c.etaff, i, j + 1, wf)) +
-          T(2) * (corner_factor(g, i, j, wf) * ldff(g, c.etaff, i, j) + corner_factor(g, i, j + 1, wf) * ldff(g, c.etaff, i, j + 1));
-    return (!(c.act[off(g, i, j)] & 1) || s == T(0)) ? T(1) : s;
+    const long o = off(g, i, j);
+    return x_l1_f<T>(g, i, j, wf, [&](int p, int q) { return ldc(g, c.eta, p, q); }, [&](int p, int q) { return ldff(g, c.etaff, p, q); },
+                     c.bu[o], c.act[o]);
 }
 template <typename T> __device__ __forceinline__ T y_l1(const KG &g, const Coef<T> &c, int i, int j, T wf) {
-    T s = (T)(g.dx * g.dy) * c.bv[off(g, i, j)] + T(2) * (y_nd(g, c.eta, i, j) + y_nd(g, c.eta, i, j - 1)) +
-          T(4) * (ldc(g, c.eta, i, j) + ldc(g, c.eta, i, j - 1)) + T(2) * (y_sd(g, c.etaff, i, j, wf) + y_sd(g, c.etaff, i + 1, j, wf)) +
-          T(2) * (corner_factor(g, i, j, wf) * ldff(g, c.etaff, i, j) + corner_factor(g, i + 1, j, wf) * ldff(g, c.etaff, i + 1, j));
-    return (!(c.act[off(g, i, j)] & 2) || s == T(0)) ? T(1) : s;
+    const long o = off(g, i, j);
+    return y_l1_f<T>(g, i, j, wf, [&](int p, int q) { return ldc(g, c.eta, p, q); }, [&](int p, int q) { return ldff(g, c.etaff, p, q); },
+                     c.bv[o], c.act[o]);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -590,6 +614,59 @@ __global__ __launch_bounds__(NT) void k_sweep_z64(KG g, const T *__restrict__ xu
     sweep_body<T, true>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, 0, red, 1, r64u, r64v, z64u, z64v);
 }
 
+// The first two ℓ1-Jacobi sweeps from x = 0 at the finest level of the FP32 V-cycle, LDS-tiled: x₁ = 0 + b·rcp(ℓ1) is
+// pointwise, so a block's 64 × 8 points read x₁ (masked as the operator sees it) on a 66 × 10 tile, η̄H on 65 × 9 and
+// η̄Hᶠᶠ on 65 × 9 from LDS; each tile element's clamped or wrapped index, activity test and x₁ are formed once instead
+// of once per use. The second sweep is x₂ = x₁ + (b − A x₁)·rcp(ℓ1) with the arithmetic of ssa_core, so the result is
+// that of k_sweep_zero followed by k_sweep, bit for bit.
+constexpr int TTW = BX + 2, TTH = BY * SROWS + 2;
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_pre_t(KG g, Coef<T> c, T wf, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                    const T *__restrict__ rlu, const T *__restrict__ rlv, T *__restrict__ ou,
+                                                    T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    __shared__ T su[TTH][TTW], sv[TTH][TTW], se[TTH - 1][TTW - 1], sf[TTH - 1][TTW - 1];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int i0 = bx_ * BX + 1, j0 = by_ * (BY * SROWS) + 1;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    // tile indices beyond the last row or column any point uses are clamped, so every address stays in range
+    auto clx = [&](int i) { return i < 0 ? 0 : (i > g.Nx + 1 ? g.Nx + 1 : i); };
+    auto cly = [&](int j) { return j < 0 ? 0 : (j > g.Ny + 1 ? g.Ny + 1 : j); };
+    auto x1u = [&](long q) { return T(0) + bu[q] * rlu[q]; };   // as k_sweep_zero forms it
+    auto x1v = [&](long q) { return T(0) + bv[q] * rlv[q]; };
+    for (int e = tid; e < TTH * TTW; e += NT) {
+        const int tj = e / TTW, ti = e - tj * TTW;
+        const int i = clx(i0 - 1 + ti), j = cly(j0 - 1 + tj);
+        su[tj][ti] = iu_f<T>(g, c.act, x1u, i, j);
+        sv[tj][ti] = iv_f<T>(g, c.act, x1v, i, j);
+    }
+    for (int e = tid; e < (TTH - 1) * (TTW - 1); e += NT) {
+        const int tj = e / (TTW - 1), ti = e - tj * (TTW - 1);
+        se[tj][ti] = ldc(g, c.eta, clx(i0 - 1 + ti), cly(j0 - 1 + tj));
+        sf[tj][ti] = ldff(g, c.etaff, clx(i0 + ti), cly(j0 + tj));
+    }
+    __syncthreads();
+    auto U = [&](int p, int q) { return su[q - j0 + 1][p - i0 + 1]; };
+    auto V = [&](int p, int q) { return sv[q - j0 + 1][p - i0 + 1]; };
+    auto E = [&](int p, int q) { return se[q - j0 + 1][p - i0 + 1]; };
+    auto Ff = [&](int p, int q) { return sf[q - j0][p - i0]; };
+    const int i = i0 + threadIdx.x, jb = j0 + threadIdx.y * SROWS;
+#pragma unroll
+    for (int r = 0; r < SROWS; r++) {
+        const int j = jb + r;
+        if (i <= g.Nx && j <= g.Ny) {
+            const long o = off(g, i, j);
+            const T pu0 = x1u(o), pv0 = x1v(o);
+            T qa, qb;
+            ssa_core<T>(g, i, j, U, V, E, Ff, pu0, pv0, c.bu[o], c.bv[o], c.act[o], wf, qa, qb);
+            ou[o] = pu0 + (bu[o] - qa) * rlu[o];
+            ov[o] = pv0 + (bv[o] - qb) * rlv[o];
+        }
+    }
+}
+
 // 1/ℓ1 of every row at the finest level (FP32 V-cycle of the mixed form): qd(1, ℓ1) = rcp(ℓ1), the factor every FP32
 // sweep applies, stored so the fused first sweeps can form x₁ = b/ℓ1 at a row's neighbours.
 template <typename T, int TX, int TY>
@@ -600,30 +677,6 @@ __global__ __launch_bounds__(NT) void k_l1_recip(KG g, Coef<T> c, T wf, T *__res
     const long o = off(g, i, j);
     ru[o] = qd(T(1), x_l1(g, c, i, j, wf));
     rv[o] = qd(T(1), y_l1(g, c, i, j, wf));
-}
-
-// The first two ℓ1-Jacobi sweeps from x = 0 in one pass (FP32): x₁ = b·rcp(ℓ1) is pointwise, so the second sweep
-// x₂ = x₁ + (b − A x₁)·rcp(ℓ1) reads x₁ at its neighbours as b·(1/ℓ1) there. Bit for bit the two separate FP32 sweeps.
-template <typename T, int TX, int TY>
-__global__ __launch_bounds__(NT) void k_sweep_pre(KG g, Coef<T> c, T wf, const T *__restrict__ bu, const T *__restrict__ bv,
-                                                  const T *__restrict__ rlu, const T *__restrict__ rlv, T *__restrict__ ou,
-                                                  T *__restrict__ ov, const Scal *gate) {
-    g.tx = TX; g.ty = TY;
-    if (solve_done(gate)) return;
-    NK_IJ2
-    auto x1u = [&](long q) { return T(0) + bu[q] * rlu[q]; };   // as k_sweep_zero forms it
-    auto x1v = [&](long q) { return T(0) + bv[q] * rlv[q]; };
-#pragma unroll
-    for (int r = 0; r < SROWS; r++) {
-        const int j = j0 + r;
-        if (i <= g.Nx && j <= g.Ny) {
-            const long o = off(g, i, j);
-            T qa, qb;
-            ssa_apply_f<T>(g, i, j, x1u, x1v, c, wf, qa, qb);
-            ou[o] = x1u(o) + (bu[o] - qa) * rlu[o];
-            ov[o] = x1v(o) + (bv[o] - qb) * rlv[o];
-        }
-    }
 }
 
 // First ℓ1-Jacobi sweep from x = 0: A·0 = 0, so x = 0 + (b − 0)/s exactly as the reference evaluates it.

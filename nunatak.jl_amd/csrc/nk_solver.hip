@@ -416,7 +416,7 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     const dim3 B2 = blocks_for_rows(L.g.Nx, L.g.Ny, SROWS);
     int k0 = 0;
     if (sizeof(T) == 4 && l == 0 && s->rl1u32 && s->fuse_pre && sweeps >= 2 && !final_dot) {   // first two FP32 sweeps in one pass
-        NK_LAUNCH_TOPO(L.g, k_sweep_pre, T, B2, THREADS, 0, s->stream, L.g, c, wf, bu, bv, (const T *)s->rl1u32, (const T *)s->rl1v32,
+        NK_LAUNCH_TOPO(L.g, k_sweep_pre_t, T, B2, THREADS, 0, s->stream, L.g, c, wf, bu, bv, (const T *)s->rl1u32, (const T *)s->rl1v32,
                        oth.u, oth.v, gate);
         CHK(launch_check(s));
         std::swap(cur, oth);

@@ -120,9 +120,9 @@ def test_mixed_galerkin_on_the_marine_sheet_matches_fp64():
 @pytest.mark.parametrize("precision,coarse", [(abi.PRECISION_MIXED, abi.COARSE_GALERKIN), (abi.PRECISION_FP64, abi.COARSE_GALERKIN),
                                               (abi.PRECISION_MIXED, abi.COARSE_REDISCRETIZED)])
 def test_fused_first_sweeps_equal_separate_sweeps(monkeypatch, precision, coarse):
-    """The first two smoothing sweeps from zero run as one kernel (k_sweep_pre at the finest FP32 level, k_g_sweep_pre on
-    Galerkin levels); with NK_NO_PRE_FUSE=1 at creation they run as two. The solutions and iteration counts must be
-    identical bit for bit."""
+    """The first two smoothing sweeps from zero run as one kernel (the LDS-tiled k_sweep_pre_t at the finest FP32 level,
+    k_g_sweep_pre on Galerkin levels); with NK_NO_PRE_FUSE=1 at creation they run as two. The solutions and iteration
+    counts must be identical bit for bit."""
     out = []
     for fuse in (True, False):
         if fuse:
