@@ -37,6 +37,16 @@ __device__ __forceinline__ int wrapi(int i, int N) {
 }
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 __device__ __forceinline__ long off(const KG &g, int i, int j) { return (long)(j - 1) * g.pitch + (i - 1); }
+// The same offset in 32 bits, and loads and stores through a 32-bit byte offset from a plane's base: the address is
+// base + zext(offset), which compiles to the plane's scalar base plus one 32-bit vector offset per access instead of
+// 64-bit address arithmetic per load. nk_create rejects grids whose FP64 plane reaches 4 GiB.
+__device__ __forceinline__ unsigned uoff(const KG &g, int i, int j) { return (unsigned)((j - 1) * g.pitch + (i - 1)); }
+template <typename T> __device__ __forceinline__ T ld32(const T *p, unsigned e) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + e * (unsigned)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ void st32(T *p, unsigned e, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + e * (unsigned)sizeof(T)) = v;
+}
 // index maps by location (Center / Face) along x and y
 __device__ __forceinline__ int cx(const KG &g, int i) { return g.tx ? clampi(i, 1, g.Nx) : wrapi(i, g.Nx); }
 __device__ __forceinline__ int fx(const KG &g, int i) { return g.tx ? clampi(i, 1, g.Nx + 1) : wrapi(i, g.Nx); }
@@ -71,36 +81,36 @@ __device__ __forceinline__ bool uact(const KG &g, const uint8_t *act, int i, int
     const bool in = (!g.tx || (i >= 1 && i <= g.Nx + 1)) && (!g.ty || (j >= 1 && j <= g.Ny));
     const int ii = g.tx ? clampi(i, 1, g.Nx + 1) : wrapi(i, g.Nx);
     const int jj = g.ty ? clampi(j, 1, g.Ny) : wrapi(j, g.Ny);
-    return in & (bool)(act[off(g, ii, jj)] & 1);
+    return in & (bool)(ld32(act, uoff(g, ii, jj)) & 1);
 }
 __device__ __forceinline__ bool vact(const KG &g, const uint8_t *act, int i, int j) {
     const bool in = (!g.tx || (i >= 1 && i <= g.Nx)) && (!g.ty || (j >= 1 && j <= g.Ny + 1));
     const int ii = g.tx ? clampi(i, 1, g.Nx) : wrapi(i, g.Nx);
     const int jj = g.ty ? clampi(j, 1, g.Ny + 1) : wrapi(j, g.Ny);
-    return in & (bool)((act[off(g, ii, jj)] >> 1) & 1);
+    return in & (bool)((ld32(act, uoff(g, ii, jj)) >> 1) & 1);
 }
 // interior_u / interior_v with the stiffness mask (ssa_operator.jl:94-95)
 template <typename T> __device__ __forceinline__ T iu(const KG &g, const uint8_t *act, const T *u, int i, int j) {
-    const T val = u[off(g, fx(g, i), cy(g, j))];
+    const T val = ld32(u, uoff(g, fx(g, i), cy(g, j)));
     return uact(g, act, i, j) ? val : T(0);
 }
 template <typename T> __device__ __forceinline__ T iv(const KG &g, const uint8_t *act, const T *v, int i, int j) {
-    const T val = v[off(g, cx(g, i), fy(g, j))];
+    const T val = ld32(v, uoff(g, cx(g, i), fy(g, j)));
     return vact(g, act, i, j) ? val : T(0);
 }
 // interior_u / interior_v without a stiffness (walls only), as the viscosity sees the velocity
 template <typename T> __device__ __forceinline__ T iu_w(const KG &g, const T *u, int i, int j) {
-    const T val = u[off(g, fx(g, i), cy(g, j))];
+    const T val = ld32(u, uoff(g, fx(g, i), cy(g, j)));
     return periph_fc(g, i, j) ? T(0) : val;
 }
 template <typename T> __device__ __forceinline__ T iv_w(const KG &g, const T *v, int i, int j) {
-    const T val = v[off(g, cx(g, i), fy(g, j))];
+    const T val = ld32(v, uoff(g, cx(g, i), fy(g, j)));
     return periph_cf(g, i, j) ? T(0) : val;
 }
-template <typename T> __device__ __forceinline__ T ldc(const KG &g, const T *f, int i, int j) { return f[off(g, cx(g, i), cy(g, j))]; }
-template <typename T> __device__ __forceinline__ T ldfc(const KG &g, const T *f, int i, int j) { return f[off(g, fx(g, i), cy(g, j))]; }
-template <typename T> __device__ __forceinline__ T ldcf(const KG &g, const T *f, int i, int j) { return f[off(g, cx(g, i), fy(g, j))]; }
-template <typename T> __device__ __forceinline__ T ldff(const KG &g, const T *f, int i, int j) { return f[off(g, fx(g, i), fy(g, j))]; }
+template <typename T> __device__ __forceinline__ T ldc(const KG &g, const T *f, int i, int j) { return ld32(f, uoff(g, cx(g, i), cy(g, j))); }
+template <typename T> __device__ __forceinline__ T ldfc(const KG &g, const T *f, int i, int j) { return ld32(f, uoff(g, fx(g, i), cy(g, j))); }
+template <typename T> __device__ __forceinline__ T ldcf(const KG &g, const T *f, int i, int j) { return ld32(f, uoff(g, cx(g, i), fy(g, j))); }
+template <typename T> __device__ __forceinline__ T ldff(const KG &g, const T *f, int i, int j) { return ld32(f, uoff(g, fx(g, i), fy(g, j))); }
 
 // Division in the operator and smoother: IEEE a / d in FP64 (the reference's operation order, bit for bit); in the
 // FP32 multigrid preconditioner a · rcp(d) with the hardware reciprocal (≤ 1 ulp), which the compiler computes once
@@ -119,11 +129,11 @@ template <typename T> struct Coef {
 // ---------------------------------------------------------------------------------------------------------------
 // The iterate is read through accessors fu(offset), fv(offset) (plain arrays, or values formed on the fly).
 template <typename T, typename FU> __device__ __forceinline__ T iu_f(const KG &g, const uint8_t *act, FU fu, int i, int j) {
-    const T val = fu(off(g, fx(g, i), cy(g, j)));
+    const T val = fu(uoff(g, fx(g, i), cy(g, j)));
     return uact(g, act, i, j) ? val : T(0);
 }
 template <typename T, typename FV> __device__ __forceinline__ T iv_f(const KG &g, const uint8_t *act, FV fv, int i, int j) {
-    const T val = fv(off(g, cx(g, i), fy(g, j)));
+    const T val = fv(uoff(g, cx(g, i), fy(g, j)));
     return vact(g, act, i, j) ? val : T(0);
 }
 
@@ -166,16 +176,16 @@ __device__ __forceinline__ void ssa_core(const KG &g, int i, int j, FU U, FV V, 
 template <typename T, typename FU, typename FV>
 __device__ __forceinline__ void ssa_apply_f(const KG &g, int i, int j, FU fu, FV fv, const Coef<T> &c, T wf, T &qu, T &qv) {
     const uint8_t *a = c.act;
-    const long o = off(g, i, j);
+    const unsigned o = uoff(g, i, j);
     ssa_core<T>(g, i, j, [&](int p, int q) { return iu_f<T>(g, a, fu, p, q); }, [&](int p, int q) { return iv_f<T>(g, a, fv, p, q); },
                 [&](int p, int q) { return ldc(g, c.eta, p, q); }, [&](int p, int q) { return ldff(g, c.etaff, p, q); }, fu(o), fv(o),
-                c.bu[o], c.bv[o], a[o], wf, qu, qv);
+                ld32(c.bu, o), ld32(c.bv, o), ld32(a, o), wf, qu, qv);
 }
 
 template <typename T>
 __device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu, const T *pv, const Coef<T> &c, T wf,
                                           T &qu, T &qv) {
-    ssa_apply_f<T>(g, i, j, [pu](long q) { return pu[q]; }, [pv](long q) { return pv[q]; }, c, wf, qu, qv);
+    ssa_apply_f<T>(g, i, j, [pu](unsigned q) { return ld32(pu, q); }, [pv](unsigned q) { return ld32(pv, q); }, c, wf, qu, qv);
 }
 
 // Operator diagonal (src/StressBalances/diagonal_preconditioner.jl:20-44)
@@ -214,14 +224,14 @@ __device__ __forceinline__ T y_l1_f(const KG &g, int i, int j, T wf, FE E, FF Ff
     return (!(f0 & 2) || s == T(0)) ? T(1) : s;
 }
 template <typename T> __device__ __forceinline__ T x_l1(const KG &g, const Coef<T> &c, int i, int j, T wf) {
-    const long o = off(g, i, j);
+    const unsigned o = uoff(g, i, j);
     return x_l1_f<T>(g, i, j, wf, [&](int p, int q) { return ldc(g, c.eta, p, q); }, [&](int p, int q) { return ldff(g, c.etaff, p, q); },
-                     c.bu[o], c.act[o]);
+                     ld32(c.bu, o), ld32(c.act, o));
 }
 template <typename T> __device__ __forceinline__ T y_l1(const KG &g, const Coef<T> &c, int i, int j, T wf) {
-    const long o = off(g, i, j);
+    const unsigned o = uoff(g, i, j);
     return y_l1_f<T>(g, i, j, wf, [&](int p, int q) { return ldc(g, c.eta, p, q); }, [&](int p, int q) { return ldff(g, c.etaff, p, q); },
-                     c.bv[o], c.act[o]);
+                     ld32(c.bv, o), ld32(c.act, o));
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -552,54 +562,59 @@ __global__ __launch_bounds__(NT) void k_picard_change(KG g, double *__restrict__
 // ---------------------------------------------------------------------------------------------------------------
 
 // Fused ℓ1-Jacobi sweep x_out = x + (b − A x) / s (src/StressBalances/multigrid_preconditioner.jl:165-219).
-// from_zero: x = 0, so A x = 0 and x_out = 0 + (b − 0) / s exactly as the reference computes it.
-// With nred: Σ b·x_out → finisher (the r·z of the PCG at the finest level). FZ (the last sweep of the mixed form's
-// FP32 V-cycle, k_sweep_z64): also z64 = (double) x_out, and the sum is Σ r64·z64 with the FP64 residual r64.
-template <typename T, bool FZ>
+// One ℓ1-Jacobi sweep x_out = x + (b − A x)/ℓ1 (multigrid_preconditioner.jl:195-219), two rows per thread.
+// DOT: Σ b·x_out → finisher (the r·z of the PCG at the finest level). FZ (the last sweep of the mixed form's FP32
+// V-cycle): also z64 = (double) x_out, and the sum is Σ r64·z64 with the FP64 residual r64.
+template <typename T, bool FZ, bool DOT>
 __device__ __forceinline__ void sweep_body(KG g, int i, int j0, const T *__restrict__ xu, const T *__restrict__ xv, const Coef<T> &c,
                                            T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
-                                           T *__restrict__ ov, int from_zero, const Red &red, int nred,
-                                           const double *__restrict__ r64u, const double *__restrict__ r64v,
-                                           double *__restrict__ z64u, double *__restrict__ z64v) {
+                                           T *__restrict__ ov, const Red &red, const double *__restrict__ r64u,
+                                           const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v) {
     double acc[1] = {0.0};
 #pragma unroll
     for (int r = 0; r < SROWS; r++) {
         const int j = j0 + r;
         if (i <= g.Nx && j <= g.Ny) {
-            const long o = off(g, i, j);
-            T qa, qb, x_u, x_v;
-            if (from_zero) {
-                qa = T(0); qb = T(0); x_u = T(0); x_v = T(0);
-            } else {
-                ssa_apply(g, i, j, xu, xv, c, wf, qa, qb);
-                x_u = xu[o];
-                x_v = xv[o];
-            }
-            const T nu = x_u + qd(bu[o] - qa, x_l1(g, c, i, j, wf));
-            const T nv = x_v + qd(bv[o] - qb, y_l1(g, c, i, j, wf));
-            ou[o] = nu;
-            ov[o] = nv;
+            const unsigned o = uoff(g, i, j);
+            T qa, qb;
+            ssa_apply(g, i, j, xu, xv, c, wf, qa, qb);
+            const T x_u = ld32(xu, o), x_v = ld32(xv, o);
+            const T b_u = ld32(bu, o), b_v = ld32(bv, o);
+            const T nu = x_u + qd(b_u - qa, x_l1(g, c, i, j, wf));
+            const T nv = x_v + qd(b_v - qb, y_l1(g, c, i, j, wf));
+            st32(ou, o, nu);
+            st32(ov, o, nv);
             if (FZ) {
                 const double a = (double)nu, b = (double)nv;
-                z64u[o] = a;
-                z64v[o] = b;
-                acc[0] += r64u[o] * a + r64v[o] * b;
-            } else if (nred) {
-                acc[0] += (double)bu[o] * (double)nu + (double)bv[o] * (double)nv;
+                st32(z64u, o, a);
+                st32(z64v, o, b);
+                acc[0] += ld32(r64u, o) * a + ld32(r64v, o) * b;
+            } else if (DOT) {
+                acc[0] += (double)b_u * (double)nu + (double)b_v * (double)nv;
             }
         }
     }
-    if (FZ || nred) reduce_finish<1>(acc, red);
+    if (FZ || DOT) reduce_finish<1>(acc, red);
 }
 
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_sweep(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c, T wf,
                                               const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
-                                              T *__restrict__ ov, int from_zero, Red red, int nred, const Scal *gate) {
+                                              T *__restrict__ ov, const Scal *gate) {
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ2
-    sweep_body<T, false>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, from_zero, red, nred, nullptr, nullptr, nullptr, nullptr);
+    sweep_body<T, false, false>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, Red{}, nullptr, nullptr, nullptr, nullptr);
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_dot(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c, T wf,
+                                                  const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                                  T *__restrict__ ov, Red red, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ2
+    sweep_body<T, false, true>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, red, nullptr, nullptr, nullptr, nullptr);
 }
 
 template <typename T, int TX, int TY>
@@ -611,7 +626,7 @@ __global__ __launch_bounds__(NT) void k_sweep_z64(KG g, const T *__restrict__ xu
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ2
-    sweep_body<T, true>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, 0, red, 1, r64u, r64v, z64u, z64v);
+    sweep_body<T, true, false>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
 }
 
 // The first two ℓ1-Jacobi sweeps from x = 0 at the finest level of the FP32 V-cycle, LDS-tiled: x₁ = 0 + b·rcp(ℓ1) is
@@ -634,8 +649,8 @@ __global__ __launch_bounds__(NT) void k_sweep_pre_t(KG g, Coef<T> c, T wf, const
     // tile indices beyond the last row or column any point uses are clamped, so every address stays in range
     auto clx = [&](int i) { return i < 0 ? 0 : (i > g.Nx + 1 ? g.Nx + 1 : i); };
     auto cly = [&](int j) { return j < 0 ? 0 : (j > g.Ny + 1 ? g.Ny + 1 : j); };
-    auto x1u = [&](long q) { return T(0) + bu[q] * rlu[q]; };   // as k_sweep_zero forms it
-    auto x1v = [&](long q) { return T(0) + bv[q] * rlv[q]; };
+    auto x1u = [&](unsigned q) { return T(0) + ld32(bu, q) * ld32(rlu, q); };   // as k_sweep_zero forms it
+    auto x1v = [&](unsigned q) { return T(0) + ld32(bv, q) * ld32(rlv, q); };
     for (int e = tid; e < TTH * TTW; e += NT) {
         const int tj = e / TTW, ti = e - tj * TTW;
         const int i = clx(i0 - 1 + ti), j = cly(j0 - 1 + tj);
@@ -657,12 +672,12 @@ __global__ __launch_bounds__(NT) void k_sweep_pre_t(KG g, Coef<T> c, T wf, const
     for (int r = 0; r < SROWS; r++) {
         const int j = jb + r;
         if (i <= g.Nx && j <= g.Ny) {
-            const long o = off(g, i, j);
+            const unsigned o = uoff(g, i, j);
             const T pu0 = x1u(o), pv0 = x1v(o);
             T qa, qb;
-            ssa_core<T>(g, i, j, U, V, E, Ff, pu0, pv0, c.bu[o], c.bv[o], c.act[o], wf, qa, qb);
-            ou[o] = pu0 + (bu[o] - qa) * rlu[o];
-            ov[o] = pv0 + (bv[o] - qb) * rlv[o];
+            ssa_core<T>(g, i, j, U, V, E, Ff, pu0, pv0, ld32(c.bu, o), ld32(c.bv, o), ld32(c.act, o), wf, qa, qb);
+            st32(ou, o, pu0 + (ld32(bu, o) - qa) * ld32(rlu, o));
+            st32(ov, o, pv0 + (ld32(bv, o) - qb) * ld32(rlv, o));
         }
     }
 }

@@ -429,8 +429,12 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
                            last ? 1 : 0, gate);
         } else {
             ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-            NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu,
-                              bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+            if (last)
+                NK_LAUNCH_TOPO_EV(L.g, k_sweep_dot, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
+                                  wf, bu, bv, oth.u, oth.v, red(s, FIN_BETA), gate);
+            else
+                NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
+                                  wf, bu, bv, oth.u, oth.v, gate);
         }
         CHK(launch_check(s));
         if (last) CHK(finish_red(s, B2, 1, FIN_BETA, gate));
@@ -469,8 +473,12 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
                                    s->fz_z64u, s->fz_z64v);
                 } else {
                     ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-                    NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
-                                      wf, bu, bv, oth.u, oth.v, 0, red(s, FIN_BETA), last ? 1 : 0, gate);
+                    if (last)
+                        NK_LAUNCH_TOPO_EV(L.g, k_sweep_dot, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u,
+                                          (const T *)cur.v, c, wf, bu, bv, oth.u, oth.v, red(s, FIN_BETA), gate);
+                    else
+                        NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u,
+                                          (const T *)cur.v, c, wf, bu, bv, oth.u, oth.v, gate);
                 }
             }
             CHK(launch_check(s));
@@ -1012,6 +1020,10 @@ nk_status validate(const nk_grid *g, const nk_physics *ph, const nk_solver_optio
     if (!g || !ph || !op) { err = "null argument"; return NK_INVALID_ARGUMENT; }
     if (g->Nx < 1 || g->Ny < 1 || g->Nsigma < 1 || g->Nsigma > 256) { err = "grid sizes must be positive (Nσ ≤ 256)"; return NK_INVALID_ARGUMENT; }
     if (!(g->dx > 0) || !(g->dy > 0)) { err = "grid spacing must be positive"; return NK_INVALID_ARGUMENT; }
+    {   // the stencil kernels address a plane with 32-bit byte offsets
+        const double plane_bytes = 8.0 * (double)(g->Ny + 1) * (double)(((g->Nx + 1) + 63) / 64 * 64);
+        if (plane_bytes >= 4294967296.0) { err = "grid too large: one FP64 plane must stay below 4 GiB"; return NK_INVALID_ARGUMENT; }
+    }
     if (ph->masking && !(ph->mask_transition_thickness > 0)) {
         err = "transition_thickness must be positive";  // SmoothIceMask (src/Models/smooth_ice_mask.jl:42)
         return NK_INVALID_ARGUMENT;
