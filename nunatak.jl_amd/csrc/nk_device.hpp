@@ -140,37 +140,46 @@ template <typename T, typename FV> __device__ __forceinline__ T iv_f(const KG &g
 // The operator's arithmetic with every input behind an accessor: U(a, b), V(a, b) the masked iterate (as iu, iv
 // give it), E(a, b) η̄H at a centre (as ldc), FF(a, b) η̄Hᶠᶠ at a corner (as ldff); pu0, pv0, bu0, bv0 and f0 the
 // unmasked iterate, drag and activity byte at (i, j). One body for the global-memory and the LDS-tiled kernels.
-template <typename T, typename FU, typename FV, typename FE, typename FF>
-__device__ __forceinline__ void ssa_core(const KG &g, int i, int j, FU U, FV V, FE E, FF Ff, T pu0, T pv0, T bu0, T bv0, uint8_t f0,
-                                         T wf, T &qu, T &qv) {
+// The operator's arithmetic on values: the masked iterate at the twelve stencil points (u at (i, j), (i+1, j),
+// (i−1, j), (i, j−1), (i, j+1), (i+1, j−1); v at (i, j), (i, j+1), (i−1, j), (i−1, j+1), (i, j−1), (i+1, j)), η̄H at
+// the centres (i, j), (i−1, j), (i, j−1), η̄Hᶠᶠ and the peripheral test at the corners (i, j), (i, j+1), (i+1, j),
+// the unmasked iterate and drag at (i, j) and the row activities au, av.
+template <typename T>
+__device__ __forceinline__ void ssa_arith(const KG &g, T u00, T u10, T um0, T u0m, T u0p, T u1m, T v00, T v01, T vm0, T vm1, T v0m,
+                                          T v10, T e_c, T e_w, T e_s, T f00, T f01, T f10, bool p00, bool p01, bool p10, T pu0,
+                                          T pv0, T bu0, T bv0, bool au, bool av, T wf, T &qu, T &qv) {
     const T dx = (T)g.dx, dy = (T)g.dy, Az = (T)(g.dx * g.dy);
-    const T u00 = U(i, j), u10 = U(i + 1, j), um0 = U(i - 1, j);
-    const T u0m = U(i, j - 1), u0p = U(i, j + 1), u1m = U(i + 1, j - 1);
-    const T v00 = V(i, j), v01 = V(i, j + 1), vm0 = V(i - 1, j);
-    const T vm1 = V(i - 1, j + 1), v0m = V(i, j - 1), v10 = V(i + 1, j);
     // strain rates at the three centres
     const T exx_c = qd(u10 - u00, dx), eyy_c = qd(v01 - v00, dy);      // (i, j)
     const T exx_w = qd(u00 - um0, dx), eyy_w = qd(vm1 - vm0, dy);      // (i-1, j)
     const T exx_s = qd(u1m - u0m, dx), eyy_s = qd(v00 - v0m, dy);      // (i, j-1)
-    const T e_c = E(i, j), e_w = E(i - 1, j), e_s = E(i, j - 1);
     const T txx_c = T(2) * e_c * (T(2) * exx_c + eyy_c), txx_w = T(2) * e_w * (T(2) * exx_w + eyy_w);
     const T tyy_c = T(2) * e_c * (T(2) * eyy_c + exx_c), tyy_s = T(2) * e_s * (T(2) * eyy_s + exx_s);
     // shear strain at the three corners
     T exy_00 = (qd(u00 - u0m, dy) + qd(v00 - vm0, dx)) / T(2);
     T exy_01 = (qd(u0p - u00, dy) + qd(v01 - vm1, dx)) / T(2);
     T exy_10 = (qd(u10 - u1m, dy) + qd(v10 - v00, dx)) / T(2);
-    exy_00 = periph_ff(g, i, j) ? wf * exy_00 : exy_00;
-    exy_01 = periph_ff(g, i, j + 1) ? wf * exy_01 : exy_01;
-    exy_10 = periph_ff(g, i + 1, j) ? wf * exy_10 : exy_10;
-    const T txy_00 = T(2) * Ff(i, j) * exy_00;
-    const T txy_01 = T(2) * Ff(i, j + 1) * exy_01;
-    const T txy_10 = T(2) * Ff(i + 1, j) * exy_10;
+    exy_00 = p00 ? wf * exy_00 : exy_00;
+    exy_01 = p01 ? wf * exy_01 : exy_01;
+    exy_10 = p10 ? wf * exy_10 : exy_10;
+    const T txy_00 = T(2) * f00 * exy_00;
+    const T txy_01 = T(2) * f01 * exy_01;
+    const T txy_10 = T(2) * f10 * exy_10;
     const T divu = (dy * txx_c - dy * txx_w) + (dx * txy_01 - dx * txy_00);
     const T divv = (dx * tyy_c - dx * tyy_s) + (dy * txy_10 - dy * txy_00);
     const T qu_ = Az * bu0 * pu0 - divu;
     const T qv_ = Az * bv0 * pv0 - divv;
-    qu = (f0 & 1) ? qu_ : pu0;
-    qv = (f0 & 2) ? qv_ : pv0;
+    qu = au ? qu_ : pu0;
+    qv = av ? qv_ : pv0;
+}
+
+template <typename T, typename FU, typename FV, typename FE, typename FF>
+__device__ __forceinline__ void ssa_core(const KG &g, int i, int j, FU U, FV V, FE E, FF Ff, T pu0, T pv0, T bu0, T bv0, uint8_t f0,
+                                         T wf, T &qu, T &qv) {
+    ssa_arith<T>(g, U(i, j), U(i + 1, j), U(i - 1, j), U(i, j - 1), U(i, j + 1), U(i + 1, j - 1), V(i, j), V(i, j + 1), V(i - 1, j),
+                 V(i - 1, j + 1), V(i, j - 1), V(i + 1, j), E(i, j), E(i - 1, j), E(i, j - 1), Ff(i, j), Ff(i, j + 1), Ff(i + 1, j),
+                 periph_ff(g, i, j), periph_ff(g, i, j + 1), periph_ff(g, i + 1, j), pu0, pv0, bu0, bv0, (f0 & 1) != 0, (f0 & 2) != 0,
+                 wf, qu, qv);
 }
 
 template <typename T, typename FU, typename FV>
@@ -207,21 +216,33 @@ template <typename T> __device__ __forceinline__ T y_diag(const KG &g, const Coe
 }
 // ℓ1 row sums (src/StressBalances/multigrid_preconditioner.jl:195-211)
 // with accessors E (η̄H at centres) and Ff (η̄Hᶠᶠ at corners), drag and activity of the row's point
+// On values: η̄H at the row's two centres (e0 at (i, j)), η̄Hᶠᶠ and the wall factor at its two corners (f0v, c0 at
+// (i, j)), its drag and activity.
+template <typename T>
+__device__ __forceinline__ T x_l1_vals(const KG &g, T e0, T e1, T f0v, T f1v, T c0, T c1, T bu0, bool active) {
+    auto nd = [&](T e) { return qd(T(4) * (T)g.dy * e, (T)g.dx); };
+    auto sd = [&](T cf, T f) { return qd(cf * (T)g.dx * f, (T)g.dy); };
+    T s = (T)(g.dx * g.dy) * bu0 + T(2) * (nd(e0) + nd(e1)) + T(4) * (e0 + e1) + T(2) * (sd(c0, f0v) + sd(c1, f1v)) +
+          T(2) * (c0 * f0v + c1 * f1v);
+    return (!active || s == T(0)) ? T(1) : s;
+}
+template <typename T>
+__device__ __forceinline__ T y_l1_vals(const KG &g, T e0, T e1, T f0v, T f1v, T c0, T c1, T bv0, bool active) {
+    auto nd = [&](T e) { return qd(T(4) * (T)g.dx * e, (T)g.dy); };
+    auto sd = [&](T cf, T f) { return qd(cf * (T)g.dy * f, (T)g.dx); };
+    T s = (T)(g.dx * g.dy) * bv0 + T(2) * (nd(e0) + nd(e1)) + T(4) * (e0 + e1) + T(2) * (sd(c0, f0v) + sd(c1, f1v)) +
+          T(2) * (c0 * f0v + c1 * f1v);
+    return (!active || s == T(0)) ? T(1) : s;
+}
 template <typename T, typename FE, typename FF>
 __device__ __forceinline__ T x_l1_f(const KG &g, int i, int j, T wf, FE E, FF Ff, T bu0, uint8_t f0) {
-    auto nd = [&](int p, int q) { return qd(T(4) * (T)g.dy * E(p, q), (T)g.dx); };
-    auto sd = [&](int p, int q) { return qd(corner_factor(g, p, q, wf) * (T)g.dx * Ff(p, q), (T)g.dy); };
-    T s = (T)(g.dx * g.dy) * bu0 + T(2) * (nd(i, j) + nd(i - 1, j)) + T(4) * (E(i, j) + E(i - 1, j)) + T(2) * (sd(i, j) + sd(i, j + 1)) +
-          T(2) * (corner_factor(g, i, j, wf) * Ff(i, j) + corner_factor(g, i, j + 1, wf) * Ff(i, j + 1));
-    return (!(f0 & 1) || s == T(0)) ? T(1) : s;
+    return x_l1_vals<T>(g, E(i, j), E(i - 1, j), Ff(i, j), Ff(i, j + 1), corner_factor(g, i, j, wf), corner_factor(g, i, j + 1, wf), bu0,
+                        (f0 & 1) != 0);
 }
 template <typename T, typename FE, typename FF>
 __device__ __forceinline__ T y_l1_f(const KG &g, int i, int j, T wf, FE E, FF Ff, T bv0, uint8_t f0) {
-    auto nd = [&](int p, int q) { return qd(T(4) * (T)g.dx * E(p, q), (T)g.dy); };
-    auto sd = [&](int p, int q) { return qd(corner_factor(g, p, q, wf) * (T)g.dy * Ff(p, q), (T)g.dx); };
-    T s = (T)(g.dx * g.dy) * bv0 + T(2) * (nd(i, j) + nd(i, j - 1)) + T(4) * (E(i, j) + E(i, j - 1)) + T(2) * (sd(i, j) + sd(i + 1, j)) +
-          T(2) * (corner_factor(g, i, j, wf) * Ff(i, j) + corner_factor(g, i + 1, j, wf) * Ff(i + 1, j));
-    return (!(f0 & 2) || s == T(0)) ? T(1) : s;
+    return y_l1_vals<T>(g, E(i, j), E(i, j - 1), Ff(i, j), Ff(i + 1, j), corner_factor(g, i, j, wf), corner_factor(g, i + 1, j, wf), bv0,
+                        (f0 & 2) != 0);
 }
 template <typename T> __device__ __forceinline__ T x_l1(const KG &g, const Coef<T> &c, int i, int j, T wf) {
     const unsigned o = uoff(g, i, j);
@@ -627,6 +648,111 @@ __global__ __launch_bounds__(NT) void k_sweep_z64(KG g, const T *__restrict__ xu
     if (solve_done(gate)) return;
     NK_IJ2
     sweep_body<T, true, false>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
+}
+
+// Neighbour masks of the finest FP32 level: one 16-bit word per point holding every activity and peripheral test the
+// operator at (i, j) makes, so a sweep loads one word instead of twelve flag bytes and their range tests.
+//   bits 0-5:  uact at (i, j), (i+1, j), (i−1, j), (i, j−1), (i, j+1), (i+1, j−1)
+//   bits 6-11: vact at (i, j), (i, j+1), (i−1, j), (i−1, j+1), (i, j−1), (i+1, j)
+//   bits 12-14: periph_ff at (i, j), (i, j+1), (i+1, j)
+// (the order of ssa_arith's arguments). Rebuilt with the activity flags, once per coefficient update.
+template <int TX, int TY>
+__global__ __launch_bounds__(NT) void k_nmask(KG g, const uint8_t *__restrict__ act, uint16_t *__restrict__ nm) {
+    g.tx = TX; g.ty = TY;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    unsigned m = 0;
+    m |= (unsigned)uact(g, act, i, j) << 0;
+    m |= (unsigned)uact(g, act, i + 1, j) << 1;
+    m |= (unsigned)uact(g, act, i - 1, j) << 2;
+    m |= (unsigned)uact(g, act, i, j - 1) << 3;
+    m |= (unsigned)uact(g, act, i, j + 1) << 4;
+    m |= (unsigned)uact(g, act, i + 1, j - 1) << 5;
+    m |= (unsigned)vact(g, act, i, j) << 6;
+    m |= (unsigned)vact(g, act, i, j + 1) << 7;
+    m |= (unsigned)vact(g, act, i - 1, j) << 8;
+    m |= (unsigned)vact(g, act, i - 1, j + 1) << 9;
+    m |= (unsigned)vact(g, act, i, j - 1) << 10;
+    m |= (unsigned)vact(g, act, i + 1, j) << 11;
+    m |= (unsigned)periph_ff(g, i, j) << 12;
+    m |= (unsigned)periph_ff(g, i, j + 1) << 13;
+    m |= (unsigned)periph_ff(g, i + 1, j) << 14;
+    st32(nm, uoff(g, i, j), (uint16_t)m);
+}
+
+// The finest-level FP32 sweep with the neighbour masks: the same values as sweep_body — the masked iterate, the
+// peripheral tests and the row activities from the word, the ℓ1 sums from the coefficients the operator loaded — so
+// the result equals k_sweep's bit for bit, with one 2-byte load in place of twelve flag bytes and their range tests
+// (≈ 20 % fewer vector instructions). FZ as in sweep_body.
+// x where bit b of m is set, else +0: a one-bit signed field extract (0 or all ones) ANDed with x's bits, the same
+// value as (bit ? x : 0) in two instructions instead of a test, a compare and a select.
+__device__ __forceinline__ float keep_if(unsigned m, int b, float x) {
+    return __uint_as_float(__float_as_uint(x) & (unsigned)__builtin_amdgcn_sbfe((int)m, b, 1));
+}
+
+template <typename T, bool FZ>
+__device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__restrict__ xu, const T *__restrict__ xv, const Coef<T> &c,
+                                             const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                             T *__restrict__ ov, const Red &red, const double *__restrict__ r64u,
+                                             const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v) {
+    double acc[1] = {0.0};
+    // x indices of faces and centres beside i, y indices beside the row (clamped or wrapped: always in range)
+    const int fip = fx(g, i + 1), fim = fx(g, i - 1), cip = cx(g, i + 1), cim = cx(g, i - 1);
+#pragma unroll
+    for (int r = 0; r < SROWS; r++) {
+        const int j = j0 + r;
+        if (i <= g.Nx && j <= g.Ny) {
+            const int cjm = cy(g, j - 1), cjp = cy(g, j + 1), fjm = fy(g, j - 1), fjp = fy(g, j + 1);
+            const unsigned o = uoff(g, i, j);
+            const unsigned m = ld32(nm, o);
+            auto mk = [m](int b, T v) { return keep_if(m, b, v); };
+            const T x_u = ld32(xu, o), x_v = ld32(xv, o);
+            const T e_c = ld32(c.eta, o), e_w = ld32(c.eta, uoff(g, cim, j)), e_s = ld32(c.eta, uoff(g, i, cjm));
+            const T f00 = ld32(c.etaff, o), f01 = ld32(c.etaff, uoff(g, i, fjp)), f10 = ld32(c.etaff, uoff(g, fip, j));
+            const T bu0 = ld32(c.bu, o), bv0 = ld32(c.bv, o);
+            const bool p00 = (m >> 12) & 1u, p01 = (m >> 13) & 1u, p10 = (m >> 14) & 1u, au = m & 1u, av = (m >> 6) & 1u;
+            T qa, qb;
+            ssa_arith<T>(g, mk(0, x_u), mk(1, ld32(xu, uoff(g, fip, j))), mk(2, ld32(xu, uoff(g, fim, j))),
+                         mk(3, ld32(xu, uoff(g, i, cjm))), mk(4, ld32(xu, uoff(g, i, cjp))), mk(5, ld32(xu, uoff(g, fip, cjm))),
+                         mk(6, x_v), mk(7, ld32(xv, uoff(g, i, fjp))), mk(8, ld32(xv, uoff(g, cim, j))),
+                         mk(9, ld32(xv, uoff(g, cim, fjp))), mk(10, ld32(xv, uoff(g, i, fjm))), mk(11, ld32(xv, uoff(g, cip, j))),
+                         e_c, e_w, e_s, f00, f01, f10, p00, p01, p10, x_u, x_v, bu0, bv0, au, av, wf, qa, qb);
+            const T c00 = p00 ? wf : T(1), c01 = p01 ? wf : T(1), c10 = p10 ? wf : T(1);   // corner_factor
+            const T nu = x_u + qd(ld32(bu, o) - qa, x_l1_vals<T>(g, e_c, e_w, f00, f01, c00, c01, bu0, au));
+            const T nv = x_v + qd(ld32(bv, o) - qb, y_l1_vals<T>(g, e_c, e_s, f00, f10, c00, c10, bv0, av));
+            st32(ou, o, nu);
+            st32(ov, o, nv);
+            if (FZ) {
+                const double a = (double)nu, b = (double)nv;
+                st32(z64u, o, a);
+                st32(z64v, o, b);
+                acc[0] += ld32(r64u, o) * a + ld32(r64v, o) * b;
+            }
+        }
+    }
+    if (FZ) reduce_finish<1>(acc, red);
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_m(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c,
+                                                const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                                T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ2
+    sweep_m_body<T, false>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, Red{}, nullptr, nullptr, nullptr, nullptr);
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_m_z64(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c,
+                                                    const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                    T *__restrict__ ou, T *__restrict__ ov, Red red, const Scal *gate,
+                                                    const double *__restrict__ r64u, const double *__restrict__ r64v,
+                                                    double *__restrict__ z64u, double *__restrict__ z64v) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ2
+    sweep_m_body<T, true>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
 }
 
 // The first two ℓ1-Jacobi sweeps from x = 0 at the finest level of the FP32 V-cycle, LDS-tiled: x₁ = 0 + b·rcp(ℓ1) is

@@ -101,6 +101,8 @@ struct nk_solver {
     double *fz_z64u = nullptr, *fz_z64v = nullptr;
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
+    uint16_t *nm16 = nullptr;   // mixed multigrid: neighbour masks of the finest points (k_nmask)
+    bool use_nmask = true;      // finest FP32 sweeps read nm16 (NK_NO_NMASK=1 at creation: k_sweep, k_sweep_z64)
     // enthalpy step workspace, allocated by the first nk_energy_step: face volume fluxes (Nσ + 1 planes), the column
     // system a, b, c, f and the solver's t (Nσ planes each), the basal state rᵇ, sᵇ, Dᵇ, qᵍ (4 planes)
     double *en_W = nullptr, *en_a = nullptr, *en_b = nullptr, *en_c = nullptr, *en_f = nullptr, *en_t = nullptr, *en_bas = nullptr;
@@ -183,6 +185,17 @@ const dim3 THREADS(BX, BY);
         } else {                                                                                     \
             if ((G).ty) KERNEL<T, 0, 1><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                   \
             else KERNEL<T, 0, 0><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                          \
+        }                                                                                            \
+    } while (0)
+
+#define NK_LAUNCH_TOPO2(G, KERNEL, GRID, BLOCK, SHM, STREAM, ...)                                    \
+    do {                                                                                             \
+        if ((G).tx) {                                                                                \
+            if ((G).ty) KERNEL<1, 1><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                      \
+            else KERNEL<1, 0><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                             \
+        } else {                                                                                     \
+            if ((G).ty) KERNEL<0, 1><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                      \
+            else KERNEL<0, 0><<<GRID, BLOCK, SHM, STREAM>>>(__VA_ARGS__);                             \
         }                                                                                            \
     } while (0)
 
@@ -371,6 +384,10 @@ nk_status update_preconditioner(nk_solver *s) {
             NK_LAUNCH_TOPO(L0.g, k_l1_recip, float, blocks_for(L0.g.Nx, L0.g.Ny), THREADS, 0, s->stream, L0.g, coef<float>(L0), wf,
                            s->rl1u32, s->rl1v32);
             CHK(launch_check(s));
+            if (s->nm16) {
+                NK_LAUNCH_TOPO2(L0.g, k_nmask, blocks_for(L0.g.Nx, L0.g.Ny), THREADS, 0, s->stream, L0.g, L0.act, s->nm16);
+                CHK(launch_check(s));
+            }
         }
     }
     if (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID && s->op.coarse_operator == NK_COARSE_GALERKIN && s->lev.size() > 1) {
@@ -383,6 +400,32 @@ nk_status update_preconditioner(nk_solver *s) {
 // ---- multigrid V-cycle ------------------------------------------------------------------------------------------
 
 template <typename T> struct Pair { T *u, *v; };
+
+// The finest FP32 sweeps through the neighbour masks (k_sweep_m, k_sweep_m_z64) where the handle has them: level 0 of
+// the mixed form's multigrid V-cycle. Returns false (nothing launched) elsewhere.
+template <typename T>
+bool launch_sweep_m(nk_solver *s, int l, const KG &g, dim3 B2, const ProfScope &ps, Pair<T> cur, Pair<T> oth, const Coef<T> &c, T wf,
+                    const T *bu, const T *bv, const Scal *gate) {
+    if constexpr (sizeof(T) == 4) {
+        if (l != 0 || !s->nm16 || !s->use_nmask || !s->rl1u32) return false;   // rl1u32: the masks are built beside 1/ℓ1
+        NK_LAUNCH_TOPO_EV(g, k_sweep_m, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, g, (const T *)cur.u, (const T *)cur.v, c,
+                          (const uint16_t *)s->nm16, wf, bu, bv, oth.u, oth.v, gate);
+        return true;
+    }
+    return false;
+}
+template <typename T>
+bool launch_sweep_m_z64(nk_solver *s, int l, const KG &g, dim3 B2, Pair<T> cur, Pair<T> oth, const Coef<T> &c, T wf, const T *bu,
+                        const T *bv, const Scal *gate) {
+    if constexpr (sizeof(T) == 4) {
+        if (l != 0 || !s->nm16 || !s->use_nmask || !s->rl1u32) return false;   // rl1u32: the masks are built beside 1/ℓ1
+        NK_LAUNCH_TOPO(g, k_sweep_m_z64, T, B2, THREADS, 0, s->stream, g, (const T *)cur.u, (const T *)cur.v, c,
+                       (const uint16_t *)s->nm16, wf, bu, bv, oth.u, oth.v,
+                       red(s, FIN_BETA), gate, (const double *)s->fz_r64u, (const double *)s->fz_r64v, s->fz_z64u, s->fz_z64v);
+        return true;
+    }
+    return false;
+}
 
 // smooth! from a zero initial guess (fill!(x, 0) then `sweeps` sweeps); result in x0 or x1 (returned)
 template <typename T>
@@ -432,7 +475,7 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
             if (last)
                 NK_LAUNCH_TOPO_EV(L.g, k_sweep_dot, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
                                   wf, bu, bv, oth.u, oth.v, red(s, FIN_BETA), gate);
-            else
+            else if (!launch_sweep_m(s, l, L.g, B2, ps, cur, oth, c, wf, bu, bv, gate))
                 NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
                                   wf, bu, bv, oth.u, oth.v, gate);
         }
@@ -468,15 +511,16 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
             {
                 const bool fz = last && l == 0 && sizeof(T) == 4 && s->fz_z64u;   // mixed: z64 and r64·z64 here
                 if (fz) {
-                    NK_LAUNCH_TOPO(L.g, k_sweep_z64, T, B2, THREADS, 0, s->stream, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu, bv,
-                                   oth.u, oth.v, red(s, FIN_BETA), gate, (const double *)s->fz_r64u, (const double *)s->fz_r64v,
-                                   s->fz_z64u, s->fz_z64v);
+                    if (!launch_sweep_m_z64(s, l, L.g, B2, cur, oth, c, wf, bu, bv, gate))
+                        NK_LAUNCH_TOPO(L.g, k_sweep_z64, T, B2, THREADS, 0, s->stream, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu,
+                                       bv, oth.u, oth.v, red(s, FIN_BETA), gate, (const double *)s->fz_r64u, (const double *)s->fz_r64v,
+                                       s->fz_z64u, s->fz_z64v);
                 } else {
                     ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
                     if (last)
                         NK_LAUNCH_TOPO_EV(L.g, k_sweep_dot, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u,
                                           (const T *)cur.v, c, wf, bu, bv, oth.u, oth.v, red(s, FIN_BETA), gate);
-                    else
+                    else if (!launch_sweep_m(s, l, L.g, B2, ps, cur, oth, c, wf, bu, bv, gate))
                         NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u,
                                           (const T *)cur.v, c, wf, bu, bv, oth.u, oth.v, gate);
                 }
@@ -1212,6 +1256,7 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
                              &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv, &s->rl1u32, &s->rl1v32};
             for (auto p : w32) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
+            if ((st = alloc_plane(s, &s->nm16, s->g)) != NK_OK) return fail(st);
         }
     }
     // coarse tail: the longest run of coarsest levels whose LDS image fits the budget (point Jacobi only)
@@ -1239,6 +1284,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) s->tail_start[0] = s->tail_start[1] = -1;
         const char *nf = std::getenv("NK_NO_PRE_FUSE");
         s->fuse_pre = !(nf && nf[0] == '1');
+        const char *nn = std::getenv("NK_NO_NMASK");
+        s->use_nmask = !(nn && nn[0] == '1');
         const char *nt = std::getenv("NK_NO_TAIL");
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
         // the LDS tail runs the rediscretized operator: levels with Galerkin operators have a tail of their own
@@ -1319,7 +1366,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
         }
     }
     b += (24 + 16) * sizeof(double) * p0 + 3 * Ns * sizeof(double) * p0;
-    if (mixed) b += 14 * sizeof(float) * p0;
+    if (mixed) b += 14 * sizeof(float) * p0 + sizeof(uint16_t) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
     if (o->coarse_operator == NK_COARSE_GALERKIN) {
