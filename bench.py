@@ -6,10 +6,12 @@ multigrid-preconditioned CG, velocity-profile reconstruction) of the 1024 × 102
 `value` is the mixed-precision form's solves/s; the FP64 form's solves/s and the mixed result's relative L2 error
 against it are reported beside it.
 
-Roofline: the dominant kernel of the mixed solve is the fused FP32 ℓ1-Jacobi sweep on the finest grid. Its
-launches are bracketed with HIP events on the solver's own stream (nk_solver_options.profile_kernel) during one
-extra solve; algorithmic bytes per launch = Nx·Ny·(10·4 + 1) (read x, b: 4 arrays; η̄H, η̄Hᶠᶠ, βᵘ, βᵛ: 4 arrays;
-activity flags: 1 byte; write x: 2 arrays).
+Roofline: the three HBM-bound operator/smoother kernels of the headline solve — the finest-grid l1-Jacobi sweep
+(k_sweep_m), the PCG operator (k_apply_m<double>) and the Galerkin stencil sweep on the first coarse level (k_g_sweep)
+— each timed with HIP events on the solver's own stream (nk_solver_options.profile_kernel) during one extra solve;
+`roofline` is the one with the most device time per solve, the others are listed under roofline.kernels with their
+algorithmic bytes per launch. (The LDS-resident coarse tail, k_g_tail, is latency-bound on one workgroup and has no
+HBM roofline.)
 
 CPU baseline: the CPU oracle (plain C, one thread) timed on a bounded sample of the same workload — one DIVA
 coefficient update and three MG-PCG iterations at 1024² — and scaled to one solve with the FP64 GPU run's Picard
@@ -351,31 +353,49 @@ def main():
         out["mixed_rel_l2_err_vs_fp64"] = err
         out["mixed_speedup_vs_fp64"] = out["value"] / out["fp64_solves_per_s"]
 
-    # roofline of the dominant kernel: one more solve with event brackets on the FP32 (or FP64) sweep
+    # roofline: one more solve per kernel with HIP-event brackets on the solver's stream, for the three HBM-bound
+    # operator/smoother kernels of the headline solve; the one with the largest device time per solve is `roofline`
     if args.no_roofline:
         print(json.dumps(out), flush=True)
         return
     m, setup = models[main_label]
     opts = setup[2]
-    opts.profile_kernel = abi.PROFILE_SWEEP
-    opts.check_interval = 1
+    n0 = N * N
+    n1 = (N // 2) * (N // 2)   # first coarse level (halved in both directions at these sizes)
+    T = 4 if main_label == "mixed" else 8
+    kinds = [("sweep", abi.PROFILE_SWEEP, f"k_sweep_m<{'float' if T == 4 else 'double'}> (finest-grid l1-Jacobi sweep)",
+              n0 * (10 * T + 2), f"{N}^2 x (10 planes x {T} B: x, b, eta, etaff, beta_u, beta_v read, x written; 2 B mask)"),
+             ("operator", abi.PROFILE_OPERATOR, "k_apply_m<double> (PCG operator q = A p with p.q)",
+              n0 * (8 * 8 + 2), f"{N}^2 x (8 planes x 8 B: p, eta, etaff, beta_u, beta_v read, q written; 2 B mask)")]
+    if main_label == "mixed" and args.coarse == "galerkin":
+        kinds.append(("galerkin_sweep", abi.PROFILE_GALERKIN_SWEEP, f"k_g_sweep<float> (Galerkin stencil sweep, {N // 2}^2 level)",
+                      n1 * 70 * 4, f"{N // 2}^2 x 70 planes x 4 B (62 couplings, 2 x 1/l1, x, b read; x written)"))
+    kernels = []
+    for name, kind, label, nbytes, formula in kinds:
+        opts.profile_kernel = kind
+        opts.check_interval = 1
+        m.set_options(opts)
+        reset_state(m)
+        pinfo = m.update_velocities(geometry=True)
+        launches = max(pinfo["profiled_launches"], 1)
+        dur_s = pinfo["profiled_ms"] / launches / 1e3
+        achieved = nbytes / dur_s / 1e9 if dur_s > 0 else 0.0
+        log(f"[bench] profiled {name}: {launches} launches, avg {dur_s * 1e6:.2f} us, {achieved:.0f} GB/s "
+            f"({achieved / HBM_PEAK_GBS:.2f} of peak)")
+        kernels.append({"name": name, "kernel": label, "launches": launches, "avg_launch_us": dur_s * 1e6,
+                        "ms_per_solve": pinfo["profiled_ms"], "algorithmic_bytes_per_launch": nbytes, "bytes_formula": formula,
+                        "achieved": achieved, "frac": achieved / HBM_PEAK_GBS})
+    opts.profile_kernel = abi.PROFILE_NONE
     m.set_options(opts)
-    reset_state(m)
-    pinfo = m.update_velocities(geometry=True)
-    launches = max(pinfo["profiled_launches"], 1)
-    dur_s = pinfo["profiled_ms"] / launches / 1e3
-    T = pinfo["profiled_precision"] or 8
-    bytes_per_launch = N * N * (10 * T + 1)
-    achieved = bytes_per_launch / dur_s / 1e9 if dur_s > 0 else 0.0
-    log(f"[bench] profiled sweep: {launches} launches, avg {dur_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+    top = max(kernels, key=lambda k: k["ms_per_solve"])
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_k_sweep_f32_1024.json")
-    if T == 4 and N == 1024 and os.path.exists(tfile):   # measured by rocprofv3 --pmc runs of --traffic-probe
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{top['name']}_{N}.json")
+    if os.path.exists(tfile):   # measured by rocprofv3 --pmc runs of --traffic-probe, calibrated on a 1 GiB copy
         traffic = json.load(open(tfile))["traffic_bytes_per_launch"]
-    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                       "kernel": f"k_sweep<{'float' if T == 4 else 'double'}> (finest grid)",
-                       "avg_launch_us": dur_s * 1e6, "launches": launches, "algorithmic_bytes_per_launch": bytes_per_launch}
+    out["roofline"] = {"bound": "hbm", "achieved": top["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": top["frac"],
+                       "traffic": traffic, "kernel": top["kernel"], "avg_launch_us": top["avg_launch_us"],
+                       "launches": top["launches"], "algorithmic_bytes_per_launch": top["algorithmic_bytes_per_launch"],
+                       "kernels": kernels}
     if not args.no_cpu_baseline and "fp64" in results:
         out["cpu_baseline"] = cpu_baseline(models["fp64"][1], results["fp64"]["infos"][-1])
     if args.serve > 0:

@@ -120,20 +120,22 @@ typedef struct nk_solver_options {
                                          stopping rule) preconditioned by an FP32 multigrid V-cycle (multigrid only) */
     int32_t coarse_operator;          /* NK_COARSE_REDISCRETIZED; NK_COARSE_GALERKIN (multigrid, point Jacobi) */
     int32_t check_interval;           /* PCG iterations launched between host checks of the on-device stop flag */
-    int32_t profile_kernel;           /* NK_PROFILE_*: bracket every finest-level launch of that kernel with HIP events
+    int32_t profile_kernel;           /* NK_PROFILE_*: bracket every launch of that kernel with HIP events
                                          on the handle's stream and report the summed device time in nk_info */
     int32_t graphs;                   /* 1: capture the launches of check_interval PCG iterations in a hipGraph once
                                          and replay it (not while profiling); 0: launch kernel by kernel */
     int32_t pad;
 } nk_solver_options;
 
-/* Kernels that nk_solver_options.profile_kernel can time (finest grid only). */
+/* Kernels that nk_solver_options.profile_kernel can time (finest grid unless stated). */
 enum {
     NK_PROFILE_NONE = 0,
     NK_PROFILE_OPERATOR = 1,   /* q = A p of the PCG iteration */
     NK_PROFILE_SWEEP = 2,      /* fused ℓ1-Jacobi sweep x + (b − A x)/s (not the first sweep from zero, nor the mixed
                                   form's last one, which also writes the FP64 z and r·z) */
-    NK_PROFILE_UPDATE = 3      /* PCG vector update x += αp, r −= αq (+ Jacobi z = r/d) */
+    NK_PROFILE_UPDATE = 3,     /* PCG vector update x += αp, r −= αq (+ Jacobi z = r/d) */
+    NK_PROFILE_GALERKIN_SWEEP = 4   /* ℓ1-Jacobi sweep with the Galerkin stencil on the first coarse level (not the sweeps
+                                       from zero); its launches count that level's points */
 };
 
 /* Field ids. Location (x, y[, σ]) in brackets; Face on a Bounded direction has N + 1 points. */

@@ -25,6 +25,7 @@ constexpr int MAXRED = 4;  // values reduced by one kernel at most
 struct KG {
     int Nx, Ny, tx, ty, pitch;
     double dx, dy;
+    double rdx, rdy;   // 1/dx and 1/dy, IEEE correctly rounded (host division)
 };
 
 // Periodic index map onto 1..N for |offset| ≤ 2N, branch-free (selects only).
@@ -117,6 +118,19 @@ template <typename T> __device__ __forceinline__ T ldff(const KG &g, const T *f,
 // per divisor instead of expanding every quotient into a denormal-safe division sequence.
 __device__ __forceinline__ double qd(double a, double d) { return a / d; }
 __device__ __forceinline__ float qd(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
+// Division by a grid spacing d with r = RN(1/d) known. FP64: q = a·r, e = q·d − a (exact in one FMA), q − e·r (one FMA).
+// With r the correctly rounded reciprocal and no over- or underflow, Markstein's correction step returns the
+// correctly rounded quotient, so the result is a / d bit for bit (signed zeros included: for a = ±0, e = +0 and
+// q − e·r = q), in three instructions instead of a full division's ten. FP32: a · rcp(d), as qd(float, float).
+struct Dv { double d, r; };
+__device__ __forceinline__ double qd(double a, Dv v) {
+    const double q = a * v.r;
+    const double e = __builtin_fma(q, v.d, -a);
+    return __builtin_fma(-e, v.r, q);
+}
+__device__ __forceinline__ float qd(float a, Dv v) { return a * __builtin_amdgcn_rcpf((float)v.d); }
+__device__ __forceinline__ Dv dvx(const KG &g) { return Dv{g.dx, g.rdx}; }
+__device__ __forceinline__ Dv dvy(const KG &g) { return Dv{g.dy, g.rdy}; }
 
 template <typename T> struct Coef {
     const T *eta, *etaff, *bu, *bv;
@@ -149,16 +163,17 @@ __device__ __forceinline__ void ssa_arith(const KG &g, T u00, T u10, T um0, T u0
                                           T v10, T e_c, T e_w, T e_s, T f00, T f01, T f10, bool p00, bool p01, bool p10, T pu0,
                                           T pv0, T bu0, T bv0, bool au, bool av, T wf, T &qu, T &qv) {
     const T dx = (T)g.dx, dy = (T)g.dy, Az = (T)(g.dx * g.dy);
+    const Dv DX = dvx(g), DY = dvy(g);
     // strain rates at the three centres
-    const T exx_c = qd(u10 - u00, dx), eyy_c = qd(v01 - v00, dy);      // (i, j)
-    const T exx_w = qd(u00 - um0, dx), eyy_w = qd(vm1 - vm0, dy);      // (i-1, j)
-    const T exx_s = qd(u1m - u0m, dx), eyy_s = qd(v00 - v0m, dy);      // (i, j-1)
+    const T exx_c = qd(u10 - u00, DX), eyy_c = qd(v01 - v00, DY);      // (i, j)
+    const T exx_w = qd(u00 - um0, DX), eyy_w = qd(vm1 - vm0, DY);      // (i-1, j)
+    const T exx_s = qd(u1m - u0m, DX), eyy_s = qd(v00 - v0m, DY);      // (i, j-1)
     const T txx_c = T(2) * e_c * (T(2) * exx_c + eyy_c), txx_w = T(2) * e_w * (T(2) * exx_w + eyy_w);
     const T tyy_c = T(2) * e_c * (T(2) * eyy_c + exx_c), tyy_s = T(2) * e_s * (T(2) * eyy_s + exx_s);
     // shear strain at the three corners
-    T exy_00 = (qd(u00 - u0m, dy) + qd(v00 - vm0, dx)) / T(2);
-    T exy_01 = (qd(u0p - u00, dy) + qd(v01 - vm1, dx)) / T(2);
-    T exy_10 = (qd(u10 - u1m, dy) + qd(v10 - v00, dx)) / T(2);
+    T exy_00 = (qd(u00 - u0m, DY) + qd(v00 - vm0, DX)) / T(2);
+    T exy_01 = (qd(u0p - u00, DY) + qd(v01 - vm1, DX)) / T(2);
+    T exy_10 = (qd(u10 - u1m, DY) + qd(v10 - v00, DX)) / T(2);
     exy_00 = p00 ? wf * exy_00 : exy_00;
     exy_01 = p01 ? wf * exy_01 : exy_01;
     exy_10 = p10 ? wf * exy_10 : exy_10;
@@ -199,10 +214,10 @@ __device__ __forceinline__ void ssa_apply(const KG &g, int i, int j, const T *pu
 
 // Operator diagonal (src/StressBalances/diagonal_preconditioner.jl:20-44)
 template <typename T> __device__ __forceinline__ T corner_factor(const KG &g, int i, int j, T wf) { return periph_ff(g, i, j) ? wf : T(1); }
-template <typename T> __device__ __forceinline__ T x_nd(const KG &g, const T *eta, int i, int j) { return qd(T(4) * (T)g.dy * ldc(g, eta, i, j), (T)g.dx); }
-template <typename T> __device__ __forceinline__ T y_nd(const KG &g, const T *eta, int i, int j) { return qd(T(4) * (T)g.dx * ldc(g, eta, i, j), (T)g.dy); }
-template <typename T> __device__ __forceinline__ T x_sd(const KG &g, const T *ff, int i, int j, T wf) { return qd(corner_factor(g, i, j, wf) * (T)g.dx * ldff(g, ff, i, j), (T)g.dy); }
-template <typename T> __device__ __forceinline__ T y_sd(const KG &g, const T *ff, int i, int j, T wf) { return qd(corner_factor(g, i, j, wf) * (T)g.dy * ldff(g, ff, i, j), (T)g.dx); }
+template <typename T> __device__ __forceinline__ T x_nd(const KG &g, const T *eta, int i, int j) { return qd(T(4) * (T)g.dy * ldc(g, eta, i, j), dvx(g)); }
+template <typename T> __device__ __forceinline__ T y_nd(const KG &g, const T *eta, int i, int j) { return qd(T(4) * (T)g.dx * ldc(g, eta, i, j), dvy(g)); }
+template <typename T> __device__ __forceinline__ T x_sd(const KG &g, const T *ff, int i, int j, T wf) { return qd(corner_factor(g, i, j, wf) * (T)g.dx * ldff(g, ff, i, j), dvy(g)); }
+template <typename T> __device__ __forceinline__ T y_sd(const KG &g, const T *ff, int i, int j, T wf) { return qd(corner_factor(g, i, j, wf) * (T)g.dy * ldff(g, ff, i, j), dvx(g)); }
 
 template <typename T> __device__ __forceinline__ T x_diag(const KG &g, const Coef<T> &c, int i, int j, T wf) {
     T d = (T)(g.dx * g.dy) * c.bu[off(g, i, j)] + x_nd(g, c.eta, i, j) + x_nd(g, c.eta, i - 1, j) + x_sd(g, c.etaff, i, j, wf) +
@@ -220,16 +235,16 @@ template <typename T> __device__ __forceinline__ T y_diag(const KG &g, const Coe
 // (i, j)), its drag and activity.
 template <typename T>
 __device__ __forceinline__ T x_l1_vals(const KG &g, T e0, T e1, T f0v, T f1v, T c0, T c1, T bu0, bool active) {
-    auto nd = [&](T e) { return qd(T(4) * (T)g.dy * e, (T)g.dx); };
-    auto sd = [&](T cf, T f) { return qd(cf * (T)g.dx * f, (T)g.dy); };
+    auto nd = [&](T e) { return qd(T(4) * (T)g.dy * e, dvx(g)); };
+    auto sd = [&](T cf, T f) { return qd(cf * (T)g.dx * f, dvy(g)); };
     T s = (T)(g.dx * g.dy) * bu0 + T(2) * (nd(e0) + nd(e1)) + T(4) * (e0 + e1) + T(2) * (sd(c0, f0v) + sd(c1, f1v)) +
           T(2) * (c0 * f0v + c1 * f1v);
     return (!active || s == T(0)) ? T(1) : s;
 }
 template <typename T>
 __device__ __forceinline__ T y_l1_vals(const KG &g, T e0, T e1, T f0v, T f1v, T c0, T c1, T bv0, bool active) {
-    auto nd = [&](T e) { return qd(T(4) * (T)g.dx * e, (T)g.dy); };
-    auto sd = [&](T cf, T f) { return qd(cf * (T)g.dy * f, (T)g.dx); };
+    auto nd = [&](T e) { return qd(T(4) * (T)g.dx * e, dvy(g)); };
+    auto sd = [&](T cf, T f) { return qd(cf * (T)g.dy * f, dvx(g)); };
     T s = (T)(g.dx * g.dy) * bv0 + T(2) * (nd(e0) + nd(e1)) + T(4) * (e0 + e1) + T(2) * (sd(c0, f0v) + sd(c1, f1v)) +
           T(2) * (c0 * f0v + c1 * f1v);
     return (!active || s == T(0)) ? T(1) : s;
@@ -680,46 +695,86 @@ __global__ __launch_bounds__(NT) void k_nmask(KG g, const uint8_t *__restrict__ 
     st32(nm, uoff(g, i, j), (uint16_t)m);
 }
 
-// The finest-level FP32 sweep with the neighbour masks: the same values as sweep_body — the masked iterate, the
-// peripheral tests and the row activities from the word, the ℓ1 sums from the coefficients the operator loaded — so
-// the result equals k_sweep's bit for bit, with one 2-byte load in place of twelve flag bytes and their range tests
-// (≈ 20 % fewer vector instructions). FZ as in sweep_body.
 // x where bit b of m is set, else +0: a one-bit signed field extract (0 or all ones) ANDed with x's bits, the same
-// value as (bit ? x : 0) in two instructions instead of a test, a compare and a select.
+// value as (bit ? x : 0) without a compare and a select per value.
 __device__ __forceinline__ float keep_if(unsigned m, int b, float x) {
     return __uint_as_float(__float_as_uint(x) & (unsigned)__builtin_amdgcn_sbfe((int)m, b, 1));
 }
+__device__ __forceinline__ double keep_if(unsigned m, int b, double x) {   // the builtin returns unsigned: widen as int
+    return __longlong_as_double(__double_as_longlong(x) & (long long)(int)__builtin_amdgcn_sbfe((int)m, b, 1));
+}
 
-template <typename T, bool FZ>
+// The operator at (i, j) through the neighbour-mask word m of the point (offset o): the same values as ssa_apply —
+// the masked iterate, the peripheral corner tests and the row activities from the word — so q is ssa_apply's bit for
+// bit, with one 2-byte load in place of twelve flag bytes and their range tests. The centre and corner coefficients,
+// the drag and the unmasked iterate at (i, j) are returned for a caller that needs them (the ℓ1 sums of a sweep).
+template <typename T> struct MPoint {
+    T x_u, x_v, e_c, e_w, e_s, f00, f01, f10, bu0, bv0;
+    bool p00, p01, p10, au, av;
+};
+template <typename T>
+__device__ __forceinline__ MPoint<T> ssa_apply_m(const KG &g, int i, int j, unsigned o, unsigned m, const T *__restrict__ xu,
+                                                 const T *__restrict__ xv, const Coef<T> &c, T wf, T &qa, T &qb) {
+    // x indices of faces and centres beside i, y indices beside j (clamped or wrapped: always in range)
+    const int fip = fx(g, i + 1), fim = fx(g, i - 1), cip = cx(g, i + 1), cim = cx(g, i - 1);
+    const int cjm = cy(g, j - 1), cjp = cy(g, j + 1), fjm = fy(g, j - 1), fjp = fy(g, j + 1);
+    auto mk = [m](int b, T v) { return keep_if(m, b, v); };
+    MPoint<T> P;
+    P.x_u = ld32(xu, o); P.x_v = ld32(xv, o);
+    P.e_c = ld32(c.eta, o); P.e_w = ld32(c.eta, uoff(g, cim, j)); P.e_s = ld32(c.eta, uoff(g, i, cjm));
+    P.f00 = ld32(c.etaff, o); P.f01 = ld32(c.etaff, uoff(g, i, fjp)); P.f10 = ld32(c.etaff, uoff(g, fip, j));
+    P.bu0 = ld32(c.bu, o); P.bv0 = ld32(c.bv, o);
+    P.p00 = (m >> 12) & 1u; P.p01 = (m >> 13) & 1u; P.p10 = (m >> 14) & 1u; P.au = m & 1u; P.av = (m >> 6) & 1u;
+    ssa_arith<T>(g, mk(0, P.x_u), mk(1, ld32(xu, uoff(g, fip, j))), mk(2, ld32(xu, uoff(g, fim, j))), mk(3, ld32(xu, uoff(g, i, cjm))),
+                 mk(4, ld32(xu, uoff(g, i, cjp))), mk(5, ld32(xu, uoff(g, fip, cjm))), mk(6, P.x_v), mk(7, ld32(xv, uoff(g, i, fjp))),
+                 mk(8, ld32(xv, uoff(g, cim, j))), mk(9, ld32(xv, uoff(g, cim, fjp))), mk(10, ld32(xv, uoff(g, i, fjm))),
+                 mk(11, ld32(xv, uoff(g, cip, j))), P.e_c, P.e_w, P.e_s, P.f00, P.f01, P.f10, P.p00, P.p01, P.p10, P.x_u, P.x_v,
+                 P.bu0, P.bv0, P.au, P.av, wf, qa, qb);
+    return P;
+}
+
+// q = A p (+ p·q), or r = b − A p, through the neighbour masks: k_apply's results bit for bit (finest level).
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_apply_m(KG g, const T *__restrict__ pu, const T *__restrict__ pv, Coef<T> c,
+                                                const uint16_t *__restrict__ nm, T wf, T *__restrict__ qu, T *__restrict__ qv,
+                                                const T *__restrict__ bu, const T *__restrict__ bv, Red red, int nred, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    double acc[1] = {0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const unsigned o = uoff(g, i, j);
+        T a, b;
+        const MPoint<T> P = ssa_apply_m<T>(g, i, j, o, ld32(nm, o), pu, pv, c, wf, a, b);
+        if (bu) {
+            a = T(1) * ld32(bu, o) + T(-1) * a;
+            b = T(1) * ld32(bv, o) + T(-1) * b;
+        }
+        st32(qu, o, a);
+        st32(qv, o, b);
+        if (nred) acc[0] = (double)P.x_u * (double)a + (double)P.x_v * (double)b;
+    }
+    if (nred) reduce_finish<1>(acc, red);
+}
+
+// The finest-level sweep through the neighbour masks: sweep_body's results bit for bit, the ℓ1 sums formed from the
+// coefficients the operator loaded. FZ as in sweep_body.
+template <typename T, bool FZ, bool DOT>
 __device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__restrict__ xu, const T *__restrict__ xv, const Coef<T> &c,
                                              const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
                                              T *__restrict__ ov, const Red &red, const double *__restrict__ r64u,
                                              const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v) {
     double acc[1] = {0.0};
-    // x indices of faces and centres beside i, y indices beside the row (clamped or wrapped: always in range)
-    const int fip = fx(g, i + 1), fim = fx(g, i - 1), cip = cx(g, i + 1), cim = cx(g, i - 1);
 #pragma unroll
     for (int r = 0; r < SROWS; r++) {
         const int j = j0 + r;
         if (i <= g.Nx && j <= g.Ny) {
-            const int cjm = cy(g, j - 1), cjp = cy(g, j + 1), fjm = fy(g, j - 1), fjp = fy(g, j + 1);
             const unsigned o = uoff(g, i, j);
-            const unsigned m = ld32(nm, o);
-            auto mk = [m](int b, T v) { return keep_if(m, b, v); };
-            const T x_u = ld32(xu, o), x_v = ld32(xv, o);
-            const T e_c = ld32(c.eta, o), e_w = ld32(c.eta, uoff(g, cim, j)), e_s = ld32(c.eta, uoff(g, i, cjm));
-            const T f00 = ld32(c.etaff, o), f01 = ld32(c.etaff, uoff(g, i, fjp)), f10 = ld32(c.etaff, uoff(g, fip, j));
-            const T bu0 = ld32(c.bu, o), bv0 = ld32(c.bv, o);
-            const bool p00 = (m >> 12) & 1u, p01 = (m >> 13) & 1u, p10 = (m >> 14) & 1u, au = m & 1u, av = (m >> 6) & 1u;
             T qa, qb;
-            ssa_arith<T>(g, mk(0, x_u), mk(1, ld32(xu, uoff(g, fip, j))), mk(2, ld32(xu, uoff(g, fim, j))),
-                         mk(3, ld32(xu, uoff(g, i, cjm))), mk(4, ld32(xu, uoff(g, i, cjp))), mk(5, ld32(xu, uoff(g, fip, cjm))),
-                         mk(6, x_v), mk(7, ld32(xv, uoff(g, i, fjp))), mk(8, ld32(xv, uoff(g, cim, j))),
-                         mk(9, ld32(xv, uoff(g, cim, fjp))), mk(10, ld32(xv, uoff(g, i, fjm))), mk(11, ld32(xv, uoff(g, cip, j))),
-                         e_c, e_w, e_s, f00, f01, f10, p00, p01, p10, x_u, x_v, bu0, bv0, au, av, wf, qa, qb);
-            const T c00 = p00 ? wf : T(1), c01 = p01 ? wf : T(1), c10 = p10 ? wf : T(1);   // corner_factor
-            const T nu = x_u + qd(ld32(bu, o) - qa, x_l1_vals<T>(g, e_c, e_w, f00, f01, c00, c01, bu0, au));
-            const T nv = x_v + qd(ld32(bv, o) - qb, y_l1_vals<T>(g, e_c, e_s, f00, f10, c00, c10, bv0, av));
+            const MPoint<T> P = ssa_apply_m<T>(g, i, j, o, ld32(nm, o), xu, xv, c, wf, qa, qb);
+            const T c00 = P.p00 ? wf : T(1), c01 = P.p01 ? wf : T(1), c10 = P.p10 ? wf : T(1);   // corner_factor
+            const T nu = P.x_u + qd(ld32(bu, o) - qa, x_l1_vals<T>(g, P.e_c, P.e_w, P.f00, P.f01, c00, c01, P.bu0, P.au));
+            const T nv = P.x_v + qd(ld32(bv, o) - qb, y_l1_vals<T>(g, P.e_c, P.e_s, P.f00, P.f10, c00, c10, P.bv0, P.av));
             st32(ou, o, nu);
             st32(ov, o, nv);
             if (FZ) {
@@ -727,10 +782,12 @@ __device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__res
                 st32(z64u, o, a);
                 st32(z64v, o, b);
                 acc[0] += ld32(r64u, o) * a + ld32(r64v, o) * b;
+            } else if (DOT) {
+                acc[0] += (double)ld32(bu, o) * (double)nu + (double)ld32(bv, o) * (double)nv;
             }
         }
     }
-    if (FZ) reduce_finish<1>(acc, red);
+    if (FZ || DOT) reduce_finish<1>(acc, red);
 }
 
 template <typename T, int TX, int TY>
@@ -740,7 +797,18 @@ __global__ __launch_bounds__(NT) void k_sweep_m(KG g, const T *__restrict__ xu, 
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ2
-    sweep_m_body<T, false>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, Red{}, nullptr, nullptr, nullptr, nullptr);
+    sweep_m_body<T, false, false>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, Red{}, nullptr, nullptr, nullptr, nullptr);
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_m_dot(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c,
+                                                    const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu,
+                                                    const T *__restrict__ bv, T *__restrict__ ou, T *__restrict__ ov, Red red,
+                                                    const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ2
+    sweep_m_body<T, false, true>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, nullptr, nullptr, nullptr, nullptr);
 }
 
 template <typename T, int TX, int TY>
@@ -752,7 +820,7 @@ __global__ __launch_bounds__(NT) void k_sweep_m_z64(KG g, const T *__restrict__ 
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ2
-    sweep_m_body<T, true>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
+    sweep_m_body<T, true, false>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
 }
 
 // The first two ℓ1-Jacobi sweeps from x = 0 at the finest level of the FP32 V-cycle, LDS-tiled: x₁ = 0 + b·rcp(ℓ1) is

@@ -101,8 +101,9 @@ struct nk_solver {
     double *fz_z64u = nullptr, *fz_z64v = nullptr;
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
-    uint16_t *nm16 = nullptr;   // mixed multigrid: neighbour masks of the finest points (k_nmask)
-    bool use_nmask = true;      // finest FP32 sweeps read nm16 (NK_NO_NMASK=1 at creation: k_sweep, k_sweep_z64)
+    uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
+    bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
+                                // flag-byte kernels k_apply, k_sweep, k_sweep_dot, k_sweep_z64)
     // enthalpy step workspace, allocated by the first nk_energy_step: face volume fluxes (Nσ + 1 planes), the column
     // system a, b, c, f and the solver's t (Nσ planes each), the basal state rᵇ, sᵇ, Dᵇ, qᵍ (4 planes)
     double *en_W = nullptr, *en_a = nullptr, *en_b = nullptr, *en_c = nullptr, *en_f = nullptr, *en_t = nullptr, *en_bas = nullptr;
@@ -155,6 +156,7 @@ std::mutex g_device_mu;
 KG make_kg(int Nx, int Ny, int tx, int ty, double dx, double dy) {
     KG g;
     g.Nx = Nx; g.Ny = Ny; g.tx = tx; g.ty = ty; g.dx = dx; g.dy = dy;
+    g.rdx = 1.0 / dx; g.rdy = 1.0 / dy;   // correctly rounded: the device forms a / dx from them (qd with Dv)
     g.pitch = ((Nx + 1) + 63) / 64 * 64;
     return g;
 }
@@ -279,7 +281,8 @@ struct ProfScope {
     nk_solver *s;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ProfScope(nk_solver *s_, int kind, int level, int precision_bytes) : s(s_) {
-        if (s->op.profile_kernel != kind || kind == NK_PROFILE_NONE || level != 0) return;
+        if (s->op.profile_kernel != kind || kind == NK_PROFILE_NONE) return;
+        if (level != (kind == NK_PROFILE_GALERKIN_SWEEP ? 1 : 0)) return;
         if (s->prof_used + 2 > s->prof_events.size()) {
             for (int k = 0; k < 256; k++) {
                 hipEvent_t e;
@@ -291,7 +294,7 @@ struct ProfScope {
         e1 = s->prof_events[s->prof_used + 1];
         s->prof_used += 2;
         s->prof_launches += 1;
-        s->prof_points += (long)s->g.Nx * s->g.Ny;
+        s->prof_points += (long)s->lev[level].g.Nx * s->lev[level].g.Ny;
         s->prof_precision = precision_bytes;
     }
 };
@@ -323,7 +326,25 @@ nk_status finish_red(nk_solver *s, dim3 B, int nv, int fin, const Scal *gate) {
 nk_status corners_flags(nk_solver *s, int l, bool write_corners) {
     Level &L = s->lev[l];
     k_corners_flags<<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, L.eta, write_corners ? L.etaff : nullptr, L.act);
-    return launch_check(s);
+    CHK(launch_check(s));
+    if (l == 0 && s->nm16) {
+        NK_LAUNCH_TOPO2(L.g, k_nmask, blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream, L.g, L.act, s->nm16);
+        CHK(launch_check(s));
+    }
+    return NK_OK;
+}
+
+// q = A p, or r = b − A p with b, at the finest level: through the neighbour masks (k_apply_m) unless NK_NO_NMASK.
+template <typename T>
+void launch_apply0(nk_solver *s, const Coef<T> &c, T wf, const T *pu, const T *pv, T *qu, T *qv, const T *bu, const T *bv, Red r,
+                   int nred, const Scal *gate, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const KG &g = s->lev[0].g;
+    const dim3 B = blocks_for(g.Nx, g.Ny);
+    if (s->nm16 && s->use_nmask)
+        NK_LAUNCH_TOPO_EV(g, k_apply_m, T, B, THREADS, 0, s->stream, e0, e1, g, pu, pv, c, (const uint16_t *)s->nm16, wf, qu, qv, bu, bv,
+                          r, nred, gate);
+    else
+        NK_LAUNCH_TOPO_EV(g, k_apply, T, B, THREADS, 0, s->stream, e0, e1, g, pu, pv, c, wf, qu, qv, bu, bv, r, nred, gate);
 }
 
 // update_preconditioner!: coarse coefficients of every level (multigrid) and FP32 copies (mixed precision)
@@ -384,10 +405,6 @@ nk_status update_preconditioner(nk_solver *s) {
             NK_LAUNCH_TOPO(L0.g, k_l1_recip, float, blocks_for(L0.g.Nx, L0.g.Ny), THREADS, 0, s->stream, L0.g, coef<float>(L0), wf,
                            s->rl1u32, s->rl1v32);
             CHK(launch_check(s));
-            if (s->nm16) {
-                NK_LAUNCH_TOPO2(L0.g, k_nmask, blocks_for(L0.g.Nx, L0.g.Ny), THREADS, 0, s->stream, L0.g, L0.act, s->nm16);
-                CHK(launch_check(s));
-            }
         }
     }
     if (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID && s->op.coarse_operator == NK_COARSE_GALERKIN && s->lev.size() > 1) {
@@ -401,24 +418,25 @@ nk_status update_preconditioner(nk_solver *s) {
 
 template <typename T> struct Pair { T *u, *v; };
 
-// The finest FP32 sweeps through the neighbour masks (k_sweep_m, k_sweep_m_z64) where the handle has them: level 0 of
-// the mixed form's multigrid V-cycle. Returns false (nothing launched) elsewhere.
+// The finest-level sweeps through the neighbour masks (k_sweep_m, k_sweep_m_dot; k_sweep_m_z64 in the mixed form's FP32
+// V-cycle) unless NK_NO_NMASK. Returns false (nothing launched) on coarser levels.
 template <typename T>
 bool launch_sweep_m(nk_solver *s, int l, const KG &g, dim3 B2, const ProfScope &ps, Pair<T> cur, Pair<T> oth, const Coef<T> &c, T wf,
-                    const T *bu, const T *bv, const Scal *gate) {
-    if constexpr (sizeof(T) == 4) {
-        if (l != 0 || !s->nm16 || !s->use_nmask || !s->rl1u32) return false;   // rl1u32: the masks are built beside 1/ℓ1
+                    const T *bu, const T *bv, const Scal *gate, bool dot) {
+    if (l != 0 || !s->nm16 || !s->use_nmask) return false;
+    if (dot)
+        NK_LAUNCH_TOPO_EV(g, k_sweep_m_dot, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, g, (const T *)cur.u, (const T *)cur.v, c,
+                          (const uint16_t *)s->nm16, wf, bu, bv, oth.u, oth.v, red(s, FIN_BETA), gate);
+    else
         NK_LAUNCH_TOPO_EV(g, k_sweep_m, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, g, (const T *)cur.u, (const T *)cur.v, c,
                           (const uint16_t *)s->nm16, wf, bu, bv, oth.u, oth.v, gate);
-        return true;
-    }
-    return false;
+    return true;
 }
 template <typename T>
 bool launch_sweep_m_z64(nk_solver *s, int l, const KG &g, dim3 B2, Pair<T> cur, Pair<T> oth, const Coef<T> &c, T wf, const T *bu,
                         const T *bv, const Scal *gate) {
     if constexpr (sizeof(T) == 4) {
-        if (l != 0 || !s->nm16 || !s->use_nmask || !s->rl1u32) return false;   // rl1u32: the masks are built beside 1/ℓ1
+        if (l != 0 || !s->nm16 || !s->use_nmask) return false;
         NK_LAUNCH_TOPO(g, k_sweep_m_z64, T, B2, THREADS, 0, s->stream, g, (const T *)cur.u, (const T *)cur.v, c,
                        (const uint16_t *)s->nm16, wf, bu, bv, oth.u, oth.v,
                        red(s, FIN_BETA), gate, (const double *)s->fz_r64u, (const double *)s->fz_r64v, s->fz_z64u, s->fz_z64v);
@@ -472,10 +490,11 @@ nk_status smooth_from_zero(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
                            last ? 1 : 0, gate);
         } else {
             ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-            if (last)
+            if (launch_sweep_m(s, l, L.g, B2, ps, cur, oth, c, wf, bu, bv, gate, last)) {
+            } else if (last)
                 NK_LAUNCH_TOPO_EV(L.g, k_sweep_dot, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
                                   wf, bu, bv, oth.u, oth.v, red(s, FIN_BETA), gate);
-            else if (!launch_sweep_m(s, l, L.g, B2, ps, cur, oth, c, wf, bu, bv, gate))
+            else
                 NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u, (const T *)cur.v, c,
                                   wf, bu, bv, oth.u, oth.v, gate);
         }
@@ -517,10 +536,11 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
                                        s->fz_z64u, s->fz_z64v);
                 } else {
                     ProfScope ps(s, NK_PROFILE_SWEEP, l, (int)sizeof(T));
-                    if (last)
+                    if (launch_sweep_m(s, l, L.g, B2, ps, cur, oth, c, wf, bu, bv, gate, last)) {
+                    } else if (last)
                         NK_LAUNCH_TOPO_EV(L.g, k_sweep_dot, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u,
                                           (const T *)cur.v, c, wf, bu, bv, oth.u, oth.v, red(s, FIN_BETA), gate);
-                    else if (!launch_sweep_m(s, l, L.g, B2, ps, cur, oth, c, wf, bu, bv, gate))
+                    else
                         NK_LAUNCH_TOPO_EV(L.g, k_sweep, T, B2, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, (const T *)cur.u,
                                           (const T *)cur.v, c, wf, bu, bv, oth.u, oth.v, gate);
                 }
@@ -669,7 +689,9 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
             if (from_zero && k == 0) {
                 NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, cur.u, cur.v, 1, gate);
             } else {
-                NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, oth.u, oth.v, 0, gate);
+                ProfScope ps(s, NK_PROFILE_GALERKIN_SWEEP, l, (int)sizeof(T));
+                NK_LAUNCH_TOPO_EV(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, cur.u, cur.v, bu, bv, oth.u, oth.v, 0,
+                                  gate);
                 std::swap(cur, oth);
             }
             CHK(launch_check(s));
@@ -729,7 +751,8 @@ nk_status v_cycle(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T *bu, cons
     Coef<T> c = coef<T>(L);
     const T wf = (T)(s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0);
     T *qu = (l == 0) ? work<T>(s).qu : V.qu, *qv = (l == 0) ? work<T>(s).qv : V.qv;
-    NK_LAUNCH_TOPO(L.g, k_apply, T, blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, qu, qv, bu, bv, red(s, FIN_NONE), 0, gate);
+    if (l == 0) launch_apply0<T>(s, c, wf, cur.u, cur.v, qu, qv, bu, bv, red(s, FIN_NONE), 0, gate);
+    else NK_LAUNCH_TOPO(L.g, k_apply, T, blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream, L.g, cur.u, cur.v, c, wf, qu, qv, bu, bv, red(s, FIN_NONE), 0, gate);
     CHK(launch_check(s));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
@@ -804,7 +827,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
     const int pre = s->op.preconditioner;
     CHK(write_scalars(s, ctl));
     if (!r_ready) {
-        NK_LAUNCH_TOPO(g, k_apply, T, B, THREADS, 0, s->stream, g, xu, xv, c, wf, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
+        launch_apply0<T>(s, c, wf, xu, xv, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
         CHK(launch_check(s));
     }
     Pair<T> z{W.zu, W.zv};
@@ -831,8 +854,8 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         for (int k = 0; k < chunk; k++) {
             {
                 ProfScope ps(s, NK_PROFILE_OPERATOR, 0, (int)sizeof(T));
-                NK_LAUNCH_TOPO_EV(g, k_apply, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, g, (const T *)W.pu, (const T *)W.pv, c, wf, W.qu, W.qv,
-                                  (const T *)nullptr, (const T *)nullptr, red(s, FIN_ALPHA), 1, (const Scal *)s->S);
+                launch_apply0<T>(s, c, wf, (const T *)W.pu, (const T *)W.pv, W.qu, W.qv, (const T *)nullptr, (const T *)nullptr,
+                                 red(s, FIN_ALPHA), 1, (const Scal *)s->S, ps.e0, ps.e1);
             }
             CHK(launch_check(s));
             CHK(finish_red(s, B, 1, FIN_ALPHA, s->S));
@@ -1252,11 +1275,13 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         for (auto p : w) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
         double **wσ[] = {&s->st_es, &s->out_es, &s->cs_es};
         for (auto p : wσ) if ((st = alloc_plane(s, p, s->g, s->Ns)) != NK_OK) return fail(st);
+        if ((st = alloc_plane(s, &s->nm16, s->g)) != NK_OK) return fail(st);
+        const char *nn = std::getenv("NK_NO_NMASK");
+        s->use_nmask = !(nn && nn[0] == '1');
         if (options->precision == NK_PRECISION_MIXED) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
                              &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv, &s->rl1u32, &s->rl1v32};
             for (auto p : w32) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
-            if ((st = alloc_plane(s, &s->nm16, s->g)) != NK_OK) return fail(st);
         }
     }
     // coarse tail: the longest run of coarsest levels whose LDS image fits the budget (point Jacobi only)
@@ -1284,8 +1309,6 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) s->tail_start[0] = s->tail_start[1] = -1;
         const char *nf = std::getenv("NK_NO_PRE_FUSE");
         s->fuse_pre = !(nf && nf[0] == '1');
-        const char *nn = std::getenv("NK_NO_NMASK");
-        s->use_nmask = !(nn && nn[0] == '1');
         const char *nt = std::getenv("NK_NO_TAIL");
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
         // the LDS tail runs the rediscretized operator: levels with Galerkin operators have a tail of their own
@@ -1366,7 +1389,8 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
         }
     }
     b += (24 + 16) * sizeof(double) * p0 + 3 * Ns * sizeof(double) * p0;
-    if (mixed) b += 14 * sizeof(float) * p0 + sizeof(uint16_t) * p0;
+    b += sizeof(uint16_t) * p0;                                               // neighbour masks of the finest level
+    if (mixed) b += 14 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
     if (o->coarse_operator == NK_COARSE_GALERKIN) {
@@ -1566,9 +1590,8 @@ nk_status nk_apply_operator(nk_solver *s) {
     if (!s) return NK_INVALID_ARGUMENT;
     CHK(corners_flags(s, 0, false));
     Level &L = s->lev[0];
-    NK_LAUNCH_TOPO(s->g, k_apply, double, blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream,
-        s->g, s->f[NK_FIELD_PU], s->f[NK_FIELD_PV], coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, s->f[NK_FIELD_QU],
-        s->f[NK_FIELD_QV], nullptr, nullptr, red(s, FIN_NONE), 0, nullptr);
+    launch_apply0<double>(s, coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, s->f[NK_FIELD_PU], s->f[NK_FIELD_PV],
+                          s->f[NK_FIELD_QU], s->f[NK_FIELD_QV], nullptr, nullptr, red(s, FIN_NONE), 0, nullptr);
     CHK(launch_check(s));
     HIPCHK(hipStreamSynchronize(s->stream));
     return NK_OK;

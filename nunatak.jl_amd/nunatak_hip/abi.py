@@ -22,7 +22,7 @@ COARSE_REDISCRETIZED, COARSE_GALERKIN = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 CLASS_WALL, CLASS_ICE_FREE, CLASS_FRONT, CLASS_GROUNDING = 1, 2, 4, 8
-PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE = 0, 1, 2, 3
+PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE, PROFILE_GALERKIN_SWEEP = 0, 1, 2, 3, 4
 
 FIELDS = ["U", "V", "H", "BED", "S", "PHI", "PHIU", "PHIV", "N", "MU", "TAUU", "TAUV", "ETAH", "ETAHFF",
           "BETAU", "BETAV", "F2", "ETASIGMA", "BETABU", "BETABV", "UB", "VB", "USIGMA", "VSIGMA", "A", "C",

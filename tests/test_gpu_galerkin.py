@@ -139,24 +139,32 @@ def test_fused_first_sweeps_equal_separate_sweeps(monkeypatch, precision, coarse
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("topology,coarse,walls", [((B, B), abi.COARSE_GALERKIN, abi.NO_SLIP), ((P, B), abi.COARSE_GALERKIN, abi.NO_SLIP),
-                                                   ((B, P), abi.COARSE_GALERKIN, abi.NO_SLIP), ((P, P), abi.COARSE_GALERKIN, abi.NO_SLIP),
-                                                   ((B, B), abi.COARSE_GALERKIN, abi.FREE_SLIP),
-                                                   ((B, B), abi.COARSE_REDISCRETIZED, abi.NO_SLIP)])
-def test_masked_finest_sweeps_equal_plain_sweeps(monkeypatch, topology, coarse, walls):
-    """The finest FP32 sweeps of the mixed form read a 16-bit neighbour mask per point and the stored 1/ℓ1 of every row
-    (k_sweep_m, k_sweep_m_z64); with NK_NO_NMASK=1 at creation they form the activity tests and ℓ1 sums in place
-    (k_sweep, k_sweep_z64). Solutions and iteration counts must be identical bit for bit, on every topology and with
-    both wall conditions (the peripheral corner bits)."""
+MIX, F64 = abi.PRECISION_MIXED, abi.PRECISION_FP64
+
+
+@pytest.mark.parametrize("topology,coarse,walls,precision", [((B, B), abi.COARSE_GALERKIN, abi.NO_SLIP, MIX),
+                                                             ((P, B), abi.COARSE_GALERKIN, abi.NO_SLIP, MIX),
+                                                             ((B, P), abi.COARSE_GALERKIN, abi.NO_SLIP, MIX),
+                                                             ((P, P), abi.COARSE_GALERKIN, abi.NO_SLIP, MIX),
+                                                             ((B, B), abi.COARSE_GALERKIN, abi.FREE_SLIP, MIX),
+                                                             ((B, B), abi.COARSE_REDISCRETIZED, abi.NO_SLIP, MIX),
+                                                             ((B, B), abi.COARSE_REDISCRETIZED, abi.NO_SLIP, F64),
+                                                             ((P, B), abi.COARSE_REDISCRETIZED, abi.FREE_SLIP, F64)])
+def test_masked_finest_sweeps_equal_plain_sweeps(monkeypatch, topology, coarse, walls, precision):
+    """The finest-level operator applies and sweeps read a 16-bit neighbour mask per point (k_apply_m, k_sweep_m,
+    k_sweep_m_dot, k_sweep_m_z64); with NK_NO_NMASK=1 at creation they test the activity flag bytes in place (k_apply,
+    k_sweep, k_sweep_dot, k_sweep_z64). Solutions and iteration counts must be identical bit for bit, on every topology,
+    with both wall conditions (the peripheral corner bits) and in both precisions."""
     out = []
     for use in (True, False):
         if use:
             monkeypatch.delenv("NK_NO_NMASK", raising=False)
         else:
             monkeypatch.setenv("NK_NO_NMASK", "1")
-        m, info = channel(96, coarse, precision=abi.PRECISION_MIXED, picard=2, linear=1e-8, topology=topology, walls=walls)
+        m, info = channel(96, coarse, precision=precision, picard=2, linear=1e-8, topology=topology, walls=walls)
         out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
         m.close()
-    say(f"[galerkin] masked vs plain finest sweeps ({topology}, coarse {coarse}, walls {walls}): CG {out[0][2]} / {out[1][2]}")
+    say(f"[galerkin] masked vs plain finest kernels ({topology}, coarse {coarse}, walls {walls}, precision {precision}): "
+        f"CG {out[0][2]} / {out[1][2]}")
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
