@@ -372,8 +372,7 @@ def main():
                       n1 * 70 * 4, f"{N // 2}^2 x 70 planes x 4 B (62 couplings, 2 x 1/l1, x, b read; x written)"))
     kernels = []
     for name, kind, label, nbytes, formula in kinds:
-        opts.profile_kernel = kind
-        opts.check_interval = 1
+        opts.profile_kernel = kind   # same check interval as the timed solves; eager launches (no graph) while profiling
         m.set_options(opts)
         reset_state(m)
         pinfo = m.update_velocities(geometry=True)

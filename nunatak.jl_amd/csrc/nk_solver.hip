@@ -299,6 +299,24 @@ struct ProfScope {
     }
 };
 
+// Profiling with chunks of PCG iterations: the launches of iterations after the one that stopped the solve return at
+// once (stop flag), so their event pairs are dropped before the sum. A chunk records the same number of pairs per
+// iteration; `ran` iterations counted by the device ran their operator, and all but the last one (when it stopped
+// the solve) ran their V-cycle.
+void prof_trim_chunk(nk_solver *s, int chunk, int ran, bool stopped) {
+    const size_t pairs = s->prof_used / 2;
+    if (s->op.profile_kernel == NK_PROFILE_NONE || pairs == 0 || pairs % (size_t)chunk != 0) return;
+    const size_t per = pairs / (size_t)chunk;
+    const int real = s->op.profile_kernel == NK_PROFILE_OPERATOR ? ran : ran - (stopped ? 1 : 0);
+    const size_t keep = per * (size_t)(real < 0 ? 0 : (real > chunk ? chunk : real));
+    const long dropped = (long)(pairs - keep);
+    if (dropped <= 0) return;
+    const long per_launch_points = s->prof_launches > 0 ? s->prof_points / s->prof_launches : 0;
+    s->prof_launches -= dropped;
+    s->prof_points -= dropped * per_launch_points;
+    s->prof_used = 2 * keep;
+}
+
 // Sum the recorded event pairs (after the stream has been synchronised).
 void prof_harvest(nk_solver *s) {
     for (size_t k = 0; k + 1 < s->prof_used; k += 2) {
@@ -909,9 +927,14 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         s->chunk_graph_key = key;
         s->chunk_graph_x = xu;
     }
+    int it0 = -1;
     for (;;) {
-        CHK(read_scalars(s));
+        HIPCHK(hipMemcpyAsync(s->S_host, s->S, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (it0 >= 0) prof_trim_chunk(s, chunk, s->S_host->it - it0, s->S_host->done != 0);
+        prof_harvest(s);
         if (s->S_host->done) break;
+        it0 = s->S_host->it;
         if (use_graph) HIPCHK(hipGraphLaunch(s->chunk_graph, s->stream));
         else CHK(launch_chunk());
     }
