@@ -7,7 +7,7 @@ multigrid-preconditioned CG, velocity-profile reconstruction) of the 1024 × 102
 against it are reported beside it.
 
 Roofline: the three HBM-bound operator/smoother kernels of the headline solve — the finest-grid l1-Jacobi sweep
-(k_sweep_m), the PCG operator (k_apply_m<double>) and the Galerkin stencil sweep on the first coarse level (k_g_sweep)
+(k_sweep_m), the PCG operator (k_apply_m<double>) and the Galerkin stencil sweep on the first coarse level (k_gp_sweep)
 — each timed with HIP events on the solver's own stream (nk_solver_options.profile_kernel) during one extra solve;
 `roofline` is the one with the most device time per solve, the others are listed under roofline.kernels with their
 algorithmic bytes per launch. (The LDS-resident coarse tail, k_g_tail, is latency-bound on one workgroup and has no
@@ -368,8 +368,9 @@ def main():
              ("operator", abi.PROFILE_OPERATOR, "k_apply_m<double> (PCG operator q = A p with p.q)",
               n0 * (8 * 8 + 2), f"{N}^2 x (8 planes x 8 B: p, eta, etaff, beta_u, beta_v read, q written; 2 B mask)")]
     if main_label == "mixed" and args.coarse == "galerkin":
-        kinds.append(("galerkin_sweep", abi.PROFILE_GALERKIN_SWEEP, f"k_g_sweep<float> (Galerkin stencil sweep, {N // 2}^2 level)",
-                      n1 * 70 * 4, f"{N // 2}^2 x 70 planes x 4 B (62 couplings, 2 x 1/l1, x, b read; x written)"))
+        kinds.append(("galerkin_sweep", abi.PROFILE_GALERKIN_SWEEP,
+                      f"k_gp_sweep<float> (Galerkin stencil sweep, symmetric packed planes, {N // 2}^2 level)",
+                      n1 * 40 * 4, f"{N // 2}^2 x 40 planes x 4 B (32 packed couplings, 2 x 1/l1, x, b read; x written)"))
     kernels = []
     for name, kind, label, nbytes, formula in kinds:
         opts.profile_kernel = kind   # same check interval as the timed solves; eager launches (no graph) while profiling

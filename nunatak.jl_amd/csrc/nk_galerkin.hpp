@@ -70,6 +70,96 @@ __device__ __forceinline__ T g_row(const KG &g, const T *__restrict__ st, long p
     return q;
 }
 
+// Symmetric packed copy of a level's stencil, read by the launched sweeps and residuals. The Galerkin operator is
+// symmetric, so each coupling pair is kept once: a u row keeps the u columns at or after its own point in (dy, dx)
+// order (8 of 15) and all 16 v columns; a v row keeps 8 of its 15 v columns and no u column. A coupling left out is
+// read from the column point's row, A(p, q) = A(q, p), so the operator evaluated from the packed planes is exactly
+// symmetric (the two rows of a pair of the Pᵀ A P product can differ in the last bit). 32 planes instead of 62. The
+// planes carry a GP_HALO-point zero border, so on a Bounded side a coupling to a point outside the level reads 0, as
+// the full stencil's does; on a Periodic side the column point's row is found by wrapping.
+constexpr int GP_PLANES = 32;
+constexpr int GP_HALO = 2;
+__host__ __device__ constexpr bool gp_stored(int dx, int dy) { return dy > 0 || (dy == 0 && dx >= 0); }
+// slot of a kept offset among block b's kept offsets, in gidx order (dx outer, dy inner)
+__host__ __device__ constexpr int gp_rank(const GBlk &b, int dx, int dy) {
+    int r = 0;
+    for (int a = b.x0; a <= b.x1; a++)
+        for (int c = b.y0; c <= b.y1; c++) {
+            if (a == dx && c == dy) return r;
+            if (gp_stored(a, c)) r++;
+        }
+    return r;
+}
+// slots: u row's u columns [0, 8), u row's v columns [8, 24), v row's v columns [24, 32)
+__host__ __device__ constexpr int gp_uu(int dx, int dy) { return gp_rank(gblk(0, 0), dx, dy); }
+__host__ __device__ constexpr int gp_uv(int dx, int dy) { return 8 + gidx(gblk(0, 1), dx, dy) - gblk(0, 1).base; }
+__host__ __device__ constexpr int gp_vv(int dx, int dy) { return 24 + gp_rank(gblk(1, 1), dx, dy); }
+__host__ __device__ inline int gp_pitch(const KG &g) { return ((g.Nx + 1 + 2 * GP_HALO) + 63) / 64 * 64; }
+__host__ __device__ inline long gp_plane(const KG &g) { return (long)gp_pitch(g) * (long)(g.Ny + 1 + 2 * GP_HALO); }
+__device__ __forceinline__ long gp_off(const KG &g, int i, int j) { return (long)(j - 1 + GP_HALO) * gp_pitch(g) + (i - 1 + GP_HALO); }
+// the row point of a column (Bounded: as is, the border reads 0; Periodic: wrapped)
+__device__ __forceinline__ long gp_nb(const KG &g, int i, int j) { return gp_off(g, g.tx ? i : wrapi(i, g.Nx), g.ty ? j : wrapi(j, g.Ny)); }
+// levels that get a packed copy (shape only, so the device-memory estimate can mirror it): no offset aliases there
+__host__ __device__ inline bool wants_gpack(const KG &g, int level) { return level >= 1 && g.Nx >= 16 && g.Ny >= 16; }
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_g_pack(KG g, const T *__restrict__ st, T *__restrict__ pk) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j), P = gplane(g), q = gp_off(g, i, j), PP = gp_plane(g);
+    constexpr GBlk buu = gblk(0, 0), buv = gblk(0, 1), bvv = gblk(1, 1);
+#pragma unroll
+    for (int dx = buu.x0; dx <= buu.x1; dx++)
+#pragma unroll
+        for (int dy = buu.y0; dy <= buu.y1; dy++)
+            if (gp_stored(dx, dy)) pk[gp_uu(dx, dy) * PP + q] = st[gidx(buu, dx, dy) * P + o];
+#pragma unroll
+    for (int dx = buv.x0; dx <= buv.x1; dx++)
+#pragma unroll
+        for (int dy = buv.y0; dy <= buv.y1; dy++) pk[gp_uv(dx, dy) * PP + q] = st[gidx(buv, dx, dy) * P + o];
+#pragma unroll
+    for (int dx = bvv.x0; dx <= bvv.x1; dx++)
+#pragma unroll
+        for (int dy = bvv.y0; dy <= bvv.y1; dy++)
+            if (gp_stored(dx, dy)) pk[gp_vv(dx, dy) * PP + q] = st[gidx(bvv, dx, dy) * P + o];
+}
+
+// Row RC of A x at (i, j) from the packed planes; x through accessors fu(offset), fv(offset) as in g_row_f.
+template <typename T, int RC, typename FU, typename FV>
+__device__ __forceinline__ T gp_row_f(const KG &g, const T *__restrict__ pk, int i, int j, FU fu, FV fv) {
+    const long PP = gp_plane(g), p = gp_off(g, i, j);
+    T q = T(0);
+    if constexpr (RC == 0) {
+        constexpr GBlk bu = gblk(0, 0), bv = gblk(0, 1);
+#pragma unroll
+        for (int dx = bu.x0; dx <= bu.x1; dx++)
+#pragma unroll
+            for (int dy = bu.y0; dy <= bu.y1; dy++) {
+                const T c = gp_stored(dx, dy) ? pk[gp_uu(dx, dy) * PP + p] : pk[gp_uu(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)];
+                q += c * fu(off(g, fx(g, i + dx), cy(g, j + dy)));
+            }
+#pragma unroll
+        for (int dx = bv.x0; dx <= bv.x1; dx++)
+#pragma unroll
+            for (int dy = bv.y0; dy <= bv.y1; dy++) q += pk[gp_uv(dx, dy) * PP + p] * fv(off(g, cx(g, i + dx), fy(g, j + dy)));
+    } else {
+        constexpr GBlk bu = gblk(1, 0), bv = gblk(1, 1);
+#pragma unroll
+        for (int dx = bu.x0; dx <= bu.x1; dx++)
+#pragma unroll
+            for (int dy = bu.y0; dy <= bu.y1; dy++)   // v row to u column = the u row's coupling to this v point
+                q += pk[gp_uv(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)] * fu(off(g, fx(g, i + dx), cy(g, j + dy)));
+#pragma unroll
+        for (int dx = bv.x0; dx <= bv.x1; dx++)
+#pragma unroll
+            for (int dy = bv.y0; dy <= bv.y1; dy++) {
+                const T c = gp_stored(dx, dy) ? pk[gp_vv(dx, dy) * PP + p] : pk[gp_vv(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)];
+                q += c * fv(off(g, cx(g, i + dx), fy(g, j + dy)));
+            }
+    }
+    return q;
+}
+
 // The finest level's operator as a stencil (the ssa_operator! couplings expanded, same masks as ssa_apply).
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_g_assemble(KG g, Coef<T> c, T wf, T *__restrict__ st) {
@@ -314,6 +404,58 @@ __global__ __launch_bounds__(NT) void k_g_residual(KG g, const T *__restrict__ s
     const long o = off(g, i, j), P = gplane(g);
     ru[o] = bu[o] - g_row<T, 0>(g, st, P, o, i, j, xu, xv);
     rv[o] = bv[o] - g_row<T, 1>(g, st, P, o, i, j, xu, xv);
+}
+
+// The launched sweeps and residual with the symmetric packed planes pk (1/ℓ1 still from st's last two planes).
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_sweep(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ xu,
+                                                 const T *__restrict__ xv, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                 T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j), P = gplane(g);
+    const T su = st[GS_PLANES * P + o], sv = st[(GS_PLANES + 1) * P + o];
+    auto fu = [&](long q) { return xu[q]; };
+    auto fv = [&](long q) { return xv[q]; };
+    const T qu = gp_row_f<T, 0>(g, pk, i, j, fu, fv);
+    const T qv = gp_row_f<T, 1>(g, pk, i, j, fu, fv);
+    ou[o] = xu[o] + (bu[o] - qu) * su;
+    ov[o] = xv[o] + (bv[o] - qv) * sv;
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_sweep_pre(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ bu,
+                                                     const T *__restrict__ bv, T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j), P = gplane(g);
+    const T *su_p = st + GS_PLANES * P, *sv_p = st + (GS_PLANES + 1) * P;
+    const T su = su_p[o], sv = sv_p[o];
+    auto x1u = [&](long q) { return bu[q] * su_p[q]; };
+    auto x1v = [&](long q) { return bv[q] * sv_p[q]; };
+    const T qu = gp_row_f<T, 0>(g, pk, i, j, x1u, x1v);
+    const T qv = gp_row_f<T, 1>(g, pk, i, j, x1u, x1v);
+    ou[o] = bu[o] * su + (bu[o] - qu) * su;
+    ov[o] = bv[o] * sv + (bv[o] - qv) * sv;
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_residual(KG g, const T *__restrict__ pk, const T *__restrict__ xu, const T *__restrict__ xv,
+                                                    const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ru,
+                                                    T *__restrict__ rv, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    auto fu = [&](long q) { return xu[q]; };
+    auto fv = [&](long q) { return xv[q]; };
+    ru[o] = bu[o] - gp_row_f<T, 0>(g, pk, i, j, fu, fv);
+    rv[o] = bv[o] - gp_row_f<T, 1>(g, pk, i, j, fu, fv);
 }
 
 // ---------------------------------------------------------------------------------------------------------------

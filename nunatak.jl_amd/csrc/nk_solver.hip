@@ -54,6 +54,7 @@ struct Level {
     double *gs64 = nullptr;  // Galerkin stencil + 1/ℓ1 planes (GS_ALLOC) at the V-cycle's precision
     float *gs32 = nullptr;
     void *gcst = nullptr;    // compact copy of the Galerkin stencil (k_g_compact) for levels the LDS tail may hold
+    void *gpk = nullptr;     // symmetric packed copy of the Galerkin stencil (k_g_pack) for the launched sweeps
 };
 
 // Levels small enough for the Galerkin LDS tail keep a compact copy of their stencil (shape-only rule, so the
@@ -102,6 +103,7 @@ struct nk_solver {
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
+    bool use_gpack = true;      // launched Galerkin sweeps read the packed stencil (NK_NO_GPACK=1 at creation: full)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
                                 // flag-byte kernels k_apply, k_sweep, k_sweep_dot, k_sweep_z64)
     // enthalpy step workspace, allocated by the first nk_energy_step: face volume fluxes (Nσ + 1 planes), the column
@@ -383,6 +385,12 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
     if (s->cinv) {
         Level &C = s->lev.back();
         NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, 0, s->stream, C.g, gs<T>(C), (T *)s->cinv);
+        CHK(launch_check(s));
+    }
+    for (size_t l = 1; l < s->lev.size(); l++) {
+        Level &L = s->lev[l];
+        if (!L.gpk) continue;
+        k_g_pack<T><<<blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream>>>(L.g, gs<T>(L), (T *)L.gpk);
         CHK(launch_check(s));
     }
     for (size_t l = 1; l < s->lev.size(); l++) {
@@ -690,6 +698,7 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     Level &L = s->lev[l];
     Vecs<T> &V = vecs<T>(L);
     const T *st = gs<T>(L);
+    const T *pk = (s->use_gpack && L.gpk) ? (const T *)L.gpk : nullptr;
     const dim3 B = blocks_for(L.g.Nx, L.g.Ny);
     auto smooth = [&](Pair<T> &cur, Pair<T> &oth, int sweeps, bool from_zero) -> nk_status {
         if (from_zero && sweeps == 0) {
@@ -698,7 +707,8 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
         }
         int k0 = 0;
         if (from_zero && sweeps >= 2 && s->fuse_pre) {   // the first two sweeps in one pass
-            NK_LAUNCH_TOPO(L.g, k_g_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, bu, bv, oth.u, oth.v, gate);
+            if (pk) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
+            else NK_LAUNCH_TOPO(L.g, k_g_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, bu, bv, oth.u, oth.v, gate);
             CHK(launch_check(s));
             std::swap(cur, oth);
             k0 = 2;
@@ -708,8 +718,12 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
                 NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, cur.u, cur.v, 1, gate);
             } else {
                 ProfScope ps(s, NK_PROFILE_GALERKIN_SWEEP, l, (int)sizeof(T));
-                NK_LAUNCH_TOPO_EV(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, cur.u, cur.v, bu, bv, oth.u, oth.v, 0,
-                                  gate);
+                if (pk)
+                    NK_LAUNCH_TOPO_EV(L.g, k_gp_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, pk, cur.u, cur.v, bu, bv, oth.u,
+                                      oth.v, gate);
+                else
+                    NK_LAUNCH_TOPO_EV(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, cur.u, cur.v, bu, bv, oth.u, oth.v,
+                                      0, gate);
                 std::swap(cur, oth);
             }
             CHK(launch_check(s));
@@ -730,7 +744,8 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     CHK(smooth(cur, oth, s->op.smoothing_sweeps, true));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
-    NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+    if (pk) NK_LAUNCH_TOPO(L.g, k_gp_residual, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+    else NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
     CHK(launch_check(s));
     NK_LAUNCH_TOPO(C.g, k_restrict, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, L.g, C.cx, C.cy, C.act, L.act, V.qu, V.qv, CV.bu, CV.bv, gate);
     CHK(launch_check(s));
@@ -1283,6 +1298,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
                 else st = alloc_plane(s, &L.gs64, L.g, GS_ALLOC);
                 if (st != NK_OK) return fail(st);
                 const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
+                if (wants_gpack(L.g, (int)l) && (st = alloc_bytes(s, &L.gpk, GP_PLANES * es * (size_t)gp_plane(L.g))) != NK_OK)
+                    return fail(st);
                 if (wants_gcst(L.g, (int)l, es) &&
                     (st = alloc_bytes(s, &L.gcst, (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1) * GS_ALLOC * es)) != NK_OK)
                     return fail(st);
@@ -1301,6 +1318,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if ((st = alloc_plane(s, &s->nm16, s->g)) != NK_OK) return fail(st);
         const char *nn = std::getenv("NK_NO_NMASK");
         s->use_nmask = !(nn && nn[0] == '1');
+        const char *np = std::getenv("NK_NO_GPACK");
+        s->use_gpack = !(np && np[0] == '1');
         if (options->precision == NK_PRECISION_MIXED) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
                              &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv, &s->rl1u32, &s->rl1v32};
@@ -1409,6 +1428,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
             const size_t es = mixed ? sizeof(float) : sizeof(double);
             b += GS_ALLOC * es * p;
             if (wants_gcst(lev[l], (int)l, es)) b += (size_t)(lev[l].Nx + 1) * (size_t)(lev[l].Ny + 1) * GS_ALLOC * es;
+            if (wants_gpack(lev[l], (int)l)) b += GP_PLANES * es * (size_t)gp_plane(lev[l]);
         }
     }
     b += (24 + 16) * sizeof(double) * p0 + 3 * Ns * sizeof(double) * p0;
