@@ -79,3 +79,23 @@ def test_create_rejects_bad_arguments_without_touching_a_device():
     p = abi.default_physics(walls=7)
     st = L.nk_create(ct.byref(g), ct.byref(p), ct.byref(abi.default_options()), ct.byref(h))
     assert st == abi.NK_INVALID_ARGUMENT
+
+
+def test_enum_mirrors_match_header():
+    """Every NK_* enumerator include/nunatak_hip.h gives a value has the same value in the Python ABI mirror (named with
+    or without the NK_ prefix), and every mirror of a header enum family is present."""
+    text = open(HEADER).read()
+    pairs = re.findall(r"\bNK_([A-Z0-9_]+)\s*=\s*(-?[0-9]+)", text)
+    assert len(pairs) >= 40
+    checked = 0
+    for name, value in pairs:
+        for attr in (name, "NK_" + name):
+            if hasattr(abi, attr):
+                assert getattr(abi, attr) == int(value), name
+                checked += 1
+                break
+    assert checked >= 35
+    for family in ("PROFILE_", "PRECONDITIONER_", "FRICTION_", "COARSE_", "CLASS_"):
+        for name, _ in pairs:
+            if name.startswith(family):
+                assert hasattr(abi, name), name
