@@ -733,6 +733,68 @@ __device__ __forceinline__ MPoint<T> ssa_apply_m(const KG &g, int i, int j, unsi
     return P;
 }
 
+constexpr int ATW = BX + 2, ATH = BY + 2;   // LDS tile of the iterate for a 64 × 4 block (k_apply_mt)
+// ssa_apply_m with the iterate from LDS tiles tu, tv whose element [1][1] is point (i0, j0) (see k_apply_mt).
+template <typename T>
+__device__ __forceinline__ MPoint<T> ssa_apply_mtile(const KG &g, int i, int j, unsigned o, unsigned m, const T (*tu)[ATW],
+                                                     const T (*tv)[ATW], int i0, int j0, const Coef<T> &c, T wf, T &qa, T &qb) {
+    const int fip = fx(g, i + 1), cim = cx(g, i - 1), cjm = cy(g, j - 1), fjp = fy(g, j + 1);
+    auto mk = [m](int b, T v) { return keep_if(m, b, v); };
+    auto U = [&](int di, int dj) { return tu[j - j0 + 1 + dj][i - i0 + 1 + di]; };
+    auto V = [&](int di, int dj) { return tv[j - j0 + 1 + dj][i - i0 + 1 + di]; };
+    MPoint<T> P;
+    P.x_u = U(0, 0); P.x_v = V(0, 0);
+    P.e_c = ld32(c.eta, o); P.e_w = ld32(c.eta, uoff(g, cim, j)); P.e_s = ld32(c.eta, uoff(g, i, cjm));
+    P.f00 = ld32(c.etaff, o); P.f01 = ld32(c.etaff, uoff(g, i, fjp)); P.f10 = ld32(c.etaff, uoff(g, fip, j));
+    P.bu0 = ld32(c.bu, o); P.bv0 = ld32(c.bv, o);
+    P.p00 = (m >> 12) & 1u; P.p01 = (m >> 13) & 1u; P.p10 = (m >> 14) & 1u; P.au = m & 1u; P.av = (m >> 6) & 1u;
+    ssa_arith<T>(g, mk(0, P.x_u), mk(1, U(1, 0)), mk(2, U(-1, 0)), mk(3, U(0, -1)), mk(4, U(0, 1)), mk(5, U(1, -1)), mk(6, P.x_v),
+                 mk(7, V(0, 1)), mk(8, V(-1, 0)), mk(9, V(-1, 1)), mk(10, V(0, -1)), mk(11, V(1, 0)), P.e_c, P.e_w, P.e_s, P.f00,
+                 P.f01, P.f10, P.p00, P.p01, P.p10, P.x_u, P.x_v, P.bu0, P.bv0, P.au, P.av, wf, qa, qb);
+    return P;
+}
+
+// k_apply_m with the iterate staged through LDS: a 64 × 4 block reads p on a 66 × 6 tile per component once (each
+// element at its clamped or wrapped index, as the stencil would read it), then every stencil value comes from LDS.
+// Same values, so the results are k_apply_m's bit for bit.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_apply_mt(KG g, const T *__restrict__ pu, const T *__restrict__ pv, Coef<T> c,
+                                                 const uint16_t *__restrict__ nm, T wf, T *__restrict__ qu, T *__restrict__ qv,
+                                                 const T *__restrict__ bu, const T *__restrict__ bv, Red red, int nred, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    __shared__ T tu[ATH][ATW], tv[ATH][ATW];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int i0 = bx_ * BX + 1, j0 = by_ * BY + 1;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    // tile coordinates beyond the last row or column any point uses are clamped, so every address stays in range
+    auto clx = [&](int i) { return i < 0 ? 0 : (i > g.Nx + 1 ? g.Nx + 1 : i); };
+    auto cly = [&](int j) { return j < 0 ? 0 : (j > g.Ny + 1 ? g.Ny + 1 : j); };
+    for (int e = tid; e < ATH * ATW; e += NT) {
+        const int tj = e / ATW, ti = e - tj * ATW;
+        const int i = clx(i0 - 1 + ti), j = cly(j0 - 1 + tj);
+        tu[tj][ti] = ld32(pu, uoff(g, fx(g, i), cy(g, j)));
+        tv[tj][ti] = ld32(pv, uoff(g, cx(g, i), fy(g, j)));
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    double acc[1] = {0.0};
+    if (i <= g.Nx && j <= g.Ny) {
+        const unsigned o = uoff(g, i, j);
+        T a, b;
+        const MPoint<T> P = ssa_apply_mtile<T>(g, i, j, o, ld32(nm, o), tu, tv, i0, j0, c, wf, a, b);
+        if (bu) {
+            a = T(1) * ld32(bu, o) + T(-1) * a;
+            b = T(1) * ld32(bv, o) + T(-1) * b;
+        }
+        st32(qu, o, a);
+        st32(qv, o, b);
+        if (nred) acc[0] = (double)P.x_u * (double)a + (double)P.x_v * (double)b;
+    }
+    if (nred) reduce_finish<1>(acc, red);
+}
+
 // q = A p (+ p·q), or r = b − A p, through the neighbour masks: k_apply's results bit for bit (finest level).
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_apply_m(KG g, const T *__restrict__ pu, const T *__restrict__ pv, Coef<T> c,

@@ -103,6 +103,7 @@ struct nk_solver {
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
+    bool apply_tile = true;     // finest operator applies stage the iterate through LDS (NK_NO_ATILE=1: k_apply_m)
     bool use_gpack = true;      // launched Galerkin sweeps read the packed stencil (NK_NO_GPACK=1 at creation: full)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
                                 // flag-byte kernels k_apply, k_sweep, k_sweep_dot, k_sweep_z64)
@@ -360,7 +361,10 @@ void launch_apply0(nk_solver *s, const Coef<T> &c, T wf, const T *pu, const T *p
                    int nred, const Scal *gate, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const KG &g = s->lev[0].g;
     const dim3 B = blocks_for(g.Nx, g.Ny);
-    if (s->nm16 && s->use_nmask)
+    if (s->nm16 && s->use_nmask && s->apply_tile)
+        NK_LAUNCH_TOPO_EV(g, k_apply_mt, T, B, THREADS, 0, s->stream, e0, e1, g, pu, pv, c, (const uint16_t *)s->nm16, wf, qu, qv, bu, bv,
+                          r, nred, gate);
+    else if (s->nm16 && s->use_nmask)
         NK_LAUNCH_TOPO_EV(g, k_apply_m, T, B, THREADS, 0, s->stream, e0, e1, g, pu, pv, c, (const uint16_t *)s->nm16, wf, qu, qv, bu, bv,
                           r, nred, gate);
     else
@@ -1318,6 +1322,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if ((st = alloc_plane(s, &s->nm16, s->g)) != NK_OK) return fail(st);
         const char *nn = std::getenv("NK_NO_NMASK");
         s->use_nmask = !(nn && nn[0] == '1');
+        const char *na = std::getenv("NK_NO_ATILE");
+        s->apply_tile = !(na && na[0] == '1');
         const char *np = std::getenv("NK_NO_GPACK");
         s->use_gpack = !(np && np[0] == '1');
         if (options->precision == NK_PRECISION_MIXED) {
