@@ -820,12 +820,14 @@ __global__ __launch_bounds__(NT) void k_apply_m(KG g, const T *__restrict__ pu, 
 }
 
 // The finest-level sweep through the neighbour masks: sweep_body's results bit for bit, the ℓ1 sums formed from the
-// coefficients the operator loaded. FZ as in sweep_body.
-template <typename T, bool FZ, bool DOT>
+// coefficients the operator loaded. FZ as in sweep_body. With tu, tv (TILE) the iterate comes from the block's LDS
+// tiles, element [1][1] being point (it0, jt0) (load_xtile).
+template <typename T, bool FZ, bool DOT, bool TILE = false>
 __device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__restrict__ xu, const T *__restrict__ xv, const Coef<T> &c,
                                              const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
                                              T *__restrict__ ov, const Red &red, const double *__restrict__ r64u,
-                                             const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v) {
+                                             const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v,
+                                             const T (*tu)[ATW] = nullptr, const T (*tv)[ATW] = nullptr, int it0 = 0, int jt0 = 0) {
     double acc[1] = {0.0};
 #pragma unroll
     for (int r = 0; r < SROWS; r++) {
@@ -833,7 +835,8 @@ __device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__res
         if (i <= g.Nx && j <= g.Ny) {
             const unsigned o = uoff(g, i, j);
             T qa, qb;
-            const MPoint<T> P = ssa_apply_m<T>(g, i, j, o, ld32(nm, o), xu, xv, c, wf, qa, qb);
+            const MPoint<T> P = TILE ? ssa_apply_mtile<T>(g, i, j, o, ld32(nm, o), tu, tv, it0, jt0, c, wf, qa, qb)
+                                     : ssa_apply_m<T>(g, i, j, o, ld32(nm, o), xu, xv, c, wf, qa, qb);
             const T c00 = P.p00 ? wf : T(1), c01 = P.p01 ? wf : T(1), c10 = P.p10 ? wf : T(1);   // corner_factor
             const T nu = P.x_u + qd(ld32(bu, o) - qa, x_l1_vals<T>(g, P.e_c, P.e_w, P.f00, P.f01, c00, c01, P.bu0, P.au));
             const T nv = P.x_v + qd(ld32(bv, o) - qb, y_l1_vals<T>(g, P.e_c, P.e_s, P.f00, P.f10, c00, c10, P.bv0, P.av));
@@ -883,6 +886,54 @@ __global__ __launch_bounds__(NT) void k_sweep_m_z64(KG g, const T *__restrict__ 
     if (solve_done(gate)) return;
     NK_IJ2
     sweep_m_body<T, true, false>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
+}
+
+// The iterate of a 64 × (4·SROWS) block on a 66 × (4·SROWS + 2) LDS tile per component, each element read once at its
+// clamped or wrapped index as the stencil reads it (the clamp keeps idle tile elements in range).
+constexpr int STH = BY * SROWS + 2;
+template <typename T>
+__device__ __forceinline__ void load_xtile(const KG &g, const T *__restrict__ xu, const T *__restrict__ xv, T (*tu)[ATW], T (*tv)[ATW],
+                                           int i0, int j0) {
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    for (int e = tid; e < STH * ATW; e += NT) {
+        const int tj = e / ATW, ti = e - tj * ATW;
+        const int i = clampi(i0 - 1 + ti, 0, g.Nx + 1), j = clampi(j0 - 1 + tj, 0, g.Ny + 1);
+        tu[tj][ti] = ld32(xu, uoff(g, fx(g, i), cy(g, j)));
+        tv[tj][ti] = ld32(xv, uoff(g, cx(g, i), fy(g, j)));
+    }
+}
+
+// k_sweep_m and k_sweep_m_dot with the iterate from LDS tiles: the same values, results bit for bit.
+template <typename T, int TX, int TY, bool FZ, bool DOT>
+__device__ __forceinline__ void sweep_mt(KG g, const T *__restrict__ xu, const T *__restrict__ xv, const Coef<T> &c,
+                                         const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv,
+                                         T *__restrict__ ou, T *__restrict__ ov, const Red &red, const double *__restrict__ r64u,
+                                         const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v) {
+    __shared__ T tu[STH][ATW], tv[STH][ATW];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int i0 = bx_ * BX + 1, jt0 = by_ * (BY * SROWS) + 1;
+    load_xtile<T>(g, xu, xv, tu, tv, i0, jt0);
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j0 = jt0 + threadIdx.y * SROWS;
+    sweep_m_body<T, FZ, DOT, true>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v, tu, tv, i0, jt0);
+}
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_mt(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c,
+                                                 const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                 T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    sweep_mt<T, TX, TY, false, false>(g, xu, xv, c, nm, wf, bu, bv, ou, ov, Red{}, nullptr, nullptr, nullptr, nullptr);
+}
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_sweep_mt_dot(KG g, const T *__restrict__ xu, const T *__restrict__ xv, Coef<T> c,
+                                                     const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu,
+                                                     const T *__restrict__ bv, T *__restrict__ ou, T *__restrict__ ov, Red red,
+                                                     const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    sweep_mt<T, TX, TY, false, true>(g, xu, xv, c, nm, wf, bu, bv, ou, ov, red, nullptr, nullptr, nullptr, nullptr);
 }
 
 // The first two ℓ1-Jacobi sweeps from x = 0 at the finest level of the FP32 V-cycle, LDS-tiled: x₁ = 0 + b·rcp(ℓ1) is
