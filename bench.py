@@ -7,7 +7,8 @@ multigrid-preconditioned CG, velocity-profile reconstruction) of the 1024 × 102
 against it are reported beside it.
 
 Roofline: the three HBM-bound operator/smoother kernels of the headline solve — the finest-grid l1-Jacobi sweep
-(k_sweep_m), the PCG operator (k_apply_m<double>) and the Galerkin stencil sweep on the first coarse level (k_gp_sweep)
+(k_sweep_m), the PCG operator (k_apply_mt<double>) and the Galerkin stencil sweep on the first coarse level
+(k_gp_sweep_t)
 — each timed with HIP events on the solver's own stream (nk_solver_options.profile_kernel) during one extra solve;
 `roofline` is the one with the most device time per solve, the others are listed under roofline.kernels with their
 algorithmic bytes per launch. (The LDS-resident coarse tail, k_g_tail, is latency-bound on one workgroup and has no
@@ -365,11 +366,11 @@ def main():
     T = 4 if main_label == "mixed" else 8
     kinds = [("sweep", abi.PROFILE_SWEEP, f"k_sweep_m<{'float' if T == 4 else 'double'}> (finest-grid l1-Jacobi sweep)",
               n0 * (10 * T + 2), f"{N}^2 x (10 planes x {T} B: x, b, eta, etaff, beta_u, beta_v read, x written; 2 B mask)"),
-             ("operator", abi.PROFILE_OPERATOR, "k_apply_m<double> (PCG operator q = A p with p.q)",
+             ("operator", abi.PROFILE_OPERATOR, "k_apply_mt<double> (PCG operator q = A p with p.q, p through LDS)",
               n0 * (8 * 8 + 2), f"{N}^2 x (8 planes x 8 B: p, eta, etaff, beta_u, beta_v read, q written; 2 B mask)")]
     if main_label == "mixed" and args.coarse == "galerkin":
         kinds.append(("galerkin_sweep", abi.PROFILE_GALERKIN_SWEEP,
-                      f"k_gp_sweep<float> (Galerkin stencil sweep, symmetric packed planes, {N // 2}^2 level)",
+                      f"k_gp_sweep_t<float> (Galerkin stencil sweep, symmetric packed planes, x through LDS, {N // 2}^2 level)",
                       n1 * 40 * 4, f"{N // 2}^2 x 40 planes x 4 B (32 packed couplings, 2 x 1/l1, x, b read; x written)"))
     kernels = []
     for name, kind, label, nbytes, formula in kinds:

@@ -124,9 +124,10 @@ __global__ __launch_bounds__(NT) void k_g_pack(KG g, const T *__restrict__ st, T
             if (gp_stored(dx, dy)) pk[gp_vv(dx, dy) * PP + q] = st[gidx(bvv, dx, dy) * P + o];
 }
 
-// Row RC of A x at (i, j) from the packed planes; x through accessors fu(offset), fv(offset) as in g_row_f.
+// Row RC of A x at (i, j) from the packed planes; x through accessors XU(dx, dy), XV(dx, dy) of the column at offset
+// (dx, dy) from the row point.
 template <typename T, int RC, typename FU, typename FV>
-__device__ __forceinline__ T gp_row_f(const KG &g, const T *__restrict__ pk, int i, int j, FU fu, FV fv) {
+__device__ __forceinline__ T gp_row_d(const KG &g, const T *__restrict__ pk, int i, int j, FU XU, FV XV) {
     const long PP = gp_plane(g), p = gp_off(g, i, j);
     T q = T(0);
     if constexpr (RC == 0) {
@@ -136,28 +137,34 @@ __device__ __forceinline__ T gp_row_f(const KG &g, const T *__restrict__ pk, int
 #pragma unroll
             for (int dy = bu.y0; dy <= bu.y1; dy++) {
                 const T c = gp_stored(dx, dy) ? pk[gp_uu(dx, dy) * PP + p] : pk[gp_uu(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)];
-                q += c * fu(off(g, fx(g, i + dx), cy(g, j + dy)));
+                q += c * XU(dx, dy);
             }
 #pragma unroll
         for (int dx = bv.x0; dx <= bv.x1; dx++)
 #pragma unroll
-            for (int dy = bv.y0; dy <= bv.y1; dy++) q += pk[gp_uv(dx, dy) * PP + p] * fv(off(g, cx(g, i + dx), fy(g, j + dy)));
+            for (int dy = bv.y0; dy <= bv.y1; dy++) q += pk[gp_uv(dx, dy) * PP + p] * XV(dx, dy);
     } else {
         constexpr GBlk bu = gblk(1, 0), bv = gblk(1, 1);
 #pragma unroll
         for (int dx = bu.x0; dx <= bu.x1; dx++)
 #pragma unroll
             for (int dy = bu.y0; dy <= bu.y1; dy++)   // v row to u column = the u row's coupling to this v point
-                q += pk[gp_uv(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)] * fu(off(g, fx(g, i + dx), cy(g, j + dy)));
+                q += pk[gp_uv(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)] * XU(dx, dy);
 #pragma unroll
         for (int dx = bv.x0; dx <= bv.x1; dx++)
 #pragma unroll
             for (int dy = bv.y0; dy <= bv.y1; dy++) {
                 const T c = gp_stored(dx, dy) ? pk[gp_vv(dx, dy) * PP + p] : pk[gp_vv(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)];
-                q += c * fv(off(g, cx(g, i + dx), fy(g, j + dy)));
+                q += c * XV(dx, dy);
             }
     }
     return q;
+}
+// x through accessors fu(offset), fv(offset) as in g_row_f
+template <typename T, int RC, typename FU, typename FV>
+__device__ __forceinline__ T gp_row_f(const KG &g, const T *__restrict__ pk, int i, int j, FU fu, FV fv) {
+    return gp_row_d<T, RC>(g, pk, i, j, [&](int dx, int dy) { return fu(off(g, fx(g, i + dx), cy(g, j + dy))); },
+                           [&](int dx, int dy) { return fv(off(g, cx(g, i + dx), fy(g, j + dy))); });
 }
 
 // The finest level's operator as a stencil (the ssa_operator! couplings expanded, same masks as ssa_apply).
@@ -456,6 +463,65 @@ __global__ __launch_bounds__(NT) void k_gp_residual(KG g, const T *__restrict__ 
     auto fv = [&](long q) { return xv[q]; };
     ru[o] = bu[o] - gp_row_f<T, 0>(g, pk, i, j, fu, fv);
     rv[o] = bv[o] - gp_row_f<T, 1>(g, pk, i, j, fu, fv);
+}
+
+// The packed sweep and residual with x staged through LDS: a 64 × 4 block reads x on a 68 × 8 tile per component (the
+// stencil reaches two points each way), each element once at its clamped or wrapped index as gp_row_f reads it. Same
+// values and order, so the results are k_gp_sweep's and k_gp_residual's bit for bit.
+constexpr int GTW = BX + 4, GTH = BY + 4;
+template <typename T>
+__device__ __forceinline__ void load_gtile(const KG &g, const T *__restrict__ xu, const T *__restrict__ xv, T (*tu)[GTW], T (*tv)[GTW],
+                                           int i0, int j0) {
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    for (int e = tid; e < GTH * GTW; e += NT) {
+        const int tj = e / GTW, ti = e - tj * GTW;
+        // every column a real point reads lies in [−1, N + 2]; beyond it the clamp only keeps idle elements in range
+        const int i = clampi(i0 - 2 + ti, -1, g.Nx + 2), j = clampi(j0 - 2 + tj, -1, g.Ny + 2);
+        tu[tj][ti] = xu[off(g, fx(g, i), cy(g, j))];
+        tv[tj][ti] = xv[off(g, cx(g, i), fy(g, j))];
+    }
+}
+template <typename T, int TX, int TY, bool RES>
+__device__ __forceinline__ void gp_pass_t(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ xu,
+                                          const T *__restrict__ xv, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                          T *__restrict__ ov) {
+    __shared__ T tu[GTH][GTW], tv[GTH][GTW];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int i0 = bx_ * BX + 1, j0 = by_ * BY + 1;
+    load_gtile<T>(g, xu, xv, tu, tv, i0, j0);
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    if (i > g.Nx || j > g.Ny) return;
+    auto XU = [&](int dx, int dy) { return tu[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    auto XV = [&](int dx, int dy) { return tv[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    const long o = off(g, i, j), P = gplane(g);
+    const T qu = gp_row_d<T, 0>(g, pk, i, j, XU, XV);
+    const T qv = gp_row_d<T, 1>(g, pk, i, j, XU, XV);
+    if (RES) {
+        ou[o] = bu[o] - qu;
+        ov[o] = bv[o] - qv;
+    } else {
+        const T su = st[GS_PLANES * P + o], sv = st[(GS_PLANES + 1) * P + o];
+        ou[o] = XU(0, 0) + (bu[o] - qu) * su;
+        ov[o] = XV(0, 0) + (bv[o] - qv) * sv;
+    }
+}
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_sweep_t(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ xu,
+                                                   const T *__restrict__ xv, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                   T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    gp_pass_t<T, TX, TY, false>(g, st, pk, xu, xv, bu, bv, ou, ov);
+}
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_residual_t(KG g, const T *__restrict__ pk, const T *__restrict__ xu, const T *__restrict__ xv,
+                                                      const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ru,
+                                                      T *__restrict__ rv, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    gp_pass_t<T, TX, TY, true>(g, nullptr, pk, xu, xv, bu, bv, ru, rv);
 }
 
 // ---------------------------------------------------------------------------------------------------------------

@@ -103,8 +103,9 @@ struct nk_solver {
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
-    bool apply_tile = true;     // FP64 finest operator applies and sweeps stage the iterate through LDS (NK_NO_ATILE=1:
-                                // k_apply_m, k_sweep_m, k_sweep_m_dot)
+    bool apply_tile = true;     // FP64 finest operator applies and sweeps, and the packed Galerkin sweeps and residuals,
+                                // stage the iterate through LDS (NK_NO_ATILE=1: k_apply_m, k_sweep_m, k_sweep_m_dot,
+                                // k_gp_sweep, k_gp_residual)
     bool use_gpack = true;      // launched Galerkin sweeps read the packed stencil (NK_NO_GPACK=1 at creation: full)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
                                 // flag-byte kernels k_apply, k_sweep, k_sweep_dot, k_sweep_z64)
@@ -730,7 +731,10 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
                 NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, cur.u, cur.v, 1, gate);
             } else {
                 ProfScope ps(s, NK_PROFILE_GALERKIN_SWEEP, l, (int)sizeof(T));
-                if (pk)
+                if (pk && s->apply_tile)
+                    NK_LAUNCH_TOPO_EV(L.g, k_gp_sweep_t, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, pk, cur.u, cur.v, bu, bv,
+                                      oth.u, oth.v, gate);
+                else if (pk)
                     NK_LAUNCH_TOPO_EV(L.g, k_gp_sweep, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, pk, cur.u, cur.v, bu, bv, oth.u,
                                       oth.v, gate);
                 else
@@ -756,7 +760,9 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     CHK(smooth(cur, oth, s->op.smoothing_sweeps, true));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
-    if (pk) NK_LAUNCH_TOPO(L.g, k_gp_residual, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+    if (pk && s->apply_tile)
+        NK_LAUNCH_TOPO(L.g, k_gp_residual_t, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+    else if (pk) NK_LAUNCH_TOPO(L.g, k_gp_residual, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
     else NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
     CHK(launch_check(s));
     NK_LAUNCH_TOPO(C.g, k_restrict, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, L.g, C.cx, C.cy, C.act, L.act, V.qu, V.qv, CV.bu, CV.bv, gate);

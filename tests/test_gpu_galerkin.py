@@ -200,3 +200,24 @@ def test_packed_stencil_sweeps_match_full_stencil(monkeypatch, topology, precisi
     assert e < 1e-9
     assert abs(out[0][1] - out[1][1]) <= max(2, out[1][1] // 20)
     assert r0 < 1e-9 and r1 < 1e-9
+
+
+@pytest.mark.parametrize("precision,coarse,topology", [(MIX, abi.COARSE_GALERKIN, (B, B)), (MIX, abi.COARSE_GALERKIN, (P, P)),
+                                                       (F64, abi.COARSE_GALERKIN, (B, P)), (F64, abi.COARSE_REDISCRETIZED, (P, B))])
+def test_lds_tiled_kernels_equal_global_reads(monkeypatch, precision, coarse, topology):
+    """The FP64 finest operator applies and sweeps and the packed Galerkin sweeps and residuals stage the iterate
+    through LDS tiles (k_apply_mt, k_sweep_mt, k_sweep_mt_dot, k_gp_sweep_t, k_gp_residual_t); with NK_NO_ATILE=1 at
+    creation they read it from global memory. The same values in the same order, so solutions and iteration counts
+    must be identical bit for bit."""
+    out = []
+    for tile in (True, False):
+        if tile:
+            monkeypatch.delenv("NK_NO_ATILE", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_ATILE", "1")
+        m, info = channel(128, coarse, precision=precision, picard=2, linear=1e-8, topology=topology)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    say(f"[galerkin] LDS tiles vs global reads (precision {precision}, coarse {coarse}, {topology}): CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
