@@ -524,6 +524,40 @@ __global__ __launch_bounds__(NT) void k_gp_residual_t(KG g, const T *__restrict_
     gp_pass_t<T, TX, TY, true>(g, nullptr, pk, xu, xv, bu, bv, ru, rv);
 }
 
+// k_gp_sweep_pre with x₁ = b·(1/ℓ1) formed once per element of a 68 × 8 LDS tile instead of at every stencil read (two
+// loads and a product each); the same products, so the result is k_gp_sweep_pre's bit for bit.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_sweep_pre_t(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ bu,
+                                                       const T *__restrict__ bv, T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    __shared__ T tu[GTH][GTW], tv[GTH][GTW];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int i0 = bx_ * BX + 1, j0 = by_ * BY + 1;
+    const long P = gplane(g);
+    const T *su_p = st + GS_PLANES * P, *sv_p = st + (GS_PLANES + 1) * P;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    for (int e = tid; e < GTH * GTW; e += NT) {
+        const int tj = e / GTW, ti = e - tj * GTW;
+        const int i = clampi(i0 - 2 + ti, -1, g.Nx + 2), j = clampi(j0 - 2 + tj, -1, g.Ny + 2);
+        const long qu = off(g, fx(g, i), cy(g, j)), qv = off(g, cx(g, i), fy(g, j));
+        tu[tj][ti] = bu[qu] * su_p[qu];
+        tv[tj][ti] = bv[qv] * sv_p[qv];
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    if (i > g.Nx || j > g.Ny) return;
+    auto XU = [&](int dx, int dy) { return tu[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    auto XV = [&](int dx, int dy) { return tv[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    const long o = off(g, i, j);
+    const T su = su_p[o], sv = sv_p[o];
+    const T qu = gp_row_d<T, 0>(g, pk, i, j, XU, XV);
+    const T qv = gp_row_d<T, 1>(g, pk, i, j, XU, XV);
+    ou[o] = bu[o] * su + (bu[o] - qu) * su;
+    ov[o] = bv[o] * sv + (bv[o] - qv) * sv;
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Coarsest level: the Galerkin operator as a dense matrix, inverted once per build (Gauss-Jordan without pivoting:
 // the operator is symmetric positive semidefinite, an inactive row is an identity row). A pivot that falls below

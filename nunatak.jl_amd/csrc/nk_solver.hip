@@ -105,7 +105,7 @@ struct nk_solver {
     uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
     bool apply_tile = true;     // FP64 finest operator applies and sweeps, and the packed Galerkin sweeps and residuals,
                                 // stage the iterate through LDS (NK_NO_ATILE=1: k_apply_m, k_sweep_m, k_sweep_m_dot,
-                                // k_gp_sweep, k_gp_residual)
+                                // k_gp_sweep, k_gp_sweep_pre, k_gp_residual)
     bool use_gpack = true;      // launched Galerkin sweeps read the packed stencil (NK_NO_GPACK=1 at creation: full)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
                                 // flag-byte kernels k_apply, k_sweep, k_sweep_dot, k_sweep_z64)
@@ -720,7 +720,8 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
         }
         int k0 = 0;
         if (from_zero && sweeps >= 2 && s->fuse_pre) {   // the first two sweeps in one pass
-            if (pk) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
+            if (pk && s->apply_tile) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre_t, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
+            else if (pk) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
             else NK_LAUNCH_TOPO(L.g, k_g_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, bu, bv, oth.u, oth.v, gate);
             CHK(launch_check(s));
             std::swap(cur, oth);

@@ -206,8 +206,8 @@ def test_packed_stencil_sweeps_match_full_stencil(monkeypatch, topology, precisi
                                                        (F64, abi.COARSE_GALERKIN, (B, P)), (F64, abi.COARSE_REDISCRETIZED, (P, B))])
 def test_lds_tiled_kernels_equal_global_reads(monkeypatch, precision, coarse, topology):
     """The FP64 finest operator applies and sweeps and the packed Galerkin sweeps and residuals stage the iterate
-    through LDS tiles (k_apply_mt, k_sweep_mt, k_sweep_mt_dot, k_gp_sweep_t, k_gp_residual_t); with NK_NO_ATILE=1 at
-    creation they read it from global memory. The same values in the same order, so solutions and iteration counts
+    through LDS tiles (k_apply_mt, k_sweep_mt, k_sweep_mt_dot, k_gp_sweep_t, k_gp_sweep_pre_t, k_gp_residual_t);
+    with NK_NO_ATILE=1 at creation they read it from global memory. The same values in the same order, so solutions and iteration counts
     must be identical bit for bit."""
     out = []
     for tile in (True, False):
