@@ -561,6 +561,21 @@ __global__ __launch_bounds__(NT) void k_update_p(KG g, const T *__restrict__ zu,
     }
 }
 
+// k_update_p with the V-cycle's FP32 result: p = z + β p with z = (double) z32, the value k_sweep_m_z64 would have
+// stored as z64, so p is k_update_p's bit for bit without the FP64 copy of z being written and read.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_update_p32(KG g, const float *__restrict__ zu, const float *__restrict__ zv, T *__restrict__ pu,
+                                                   T *__restrict__ pv, const Scal *S) {
+    if (solve_done(S)) return;
+    NK_IJ
+    if (i <= g.Nx && j <= g.Ny) {
+        const long o = off(g, i, j);
+        const T be = (T)S->beta;
+        pu[o] = T(1) * (T)zu[o] + be * pu[o];
+        pv[o] = T(1) * (T)zv[o] + be * pv[o];
+    }
+}
+
 // Σ over :xyz of a·b (+ c·d) → finisher
 template <typename T>
 __global__ __launch_bounds__(NT) void k_dot(KG g, const T *__restrict__ au, const T *__restrict__ av, const T *__restrict__ bu,
@@ -622,8 +637,10 @@ __device__ __forceinline__ void sweep_body(KG g, int i, int j0, const T *__restr
             st32(ov, o, nv);
             if (FZ) {
                 const double a = (double)nu, b = (double)nv;
-                st32(z64u, o, a);
-                st32(z64v, o, b);
+                if (z64u) {   // null: the PCG reads the FP32 z (k_update_p32), z64 = (double) z32 is not stored
+                    st32(z64u, o, a);
+                    st32(z64v, o, b);
+                }
                 acc[0] += ld32(r64u, o) * a + ld32(r64v, o) * b;
             } else if (DOT) {
                 acc[0] += (double)b_u * (double)nu + (double)b_v * (double)nv;
@@ -844,8 +861,10 @@ __device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__res
             st32(ov, o, nv);
             if (FZ) {
                 const double a = (double)nu, b = (double)nv;
-                st32(z64u, o, a);
-                st32(z64v, o, b);
+                if (z64u) {   // null: the PCG reads the FP32 z (k_update_p32), z64 = (double) z32 is not stored
+                    st32(z64u, o, a);
+                    st32(z64v, o, b);
+                }
                 acc[0] += ld32(r64u, o) * a + ld32(r64v, o) * b;
             } else if (DOT) {
                 acc[0] += (double)ld32(bu, o) * (double)nu + (double)ld32(bv, o) * (double)nv;

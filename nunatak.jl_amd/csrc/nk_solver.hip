@@ -103,6 +103,7 @@ struct nk_solver {
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
+    bool use_z32 = true;        // mixed: the direction update reads the FP32 z (NK_NO_Z32=1: FP64 z via z64)
     bool apply_tile = true;     // FP64 finest operator applies and sweeps, and the packed Galerkin sweeps and residuals,
                                 // stage the iterate through LDS (NK_NO_ATILE=1: k_apply_m, k_sweep_m, k_sweep_m_dot,
                                 // k_gp_sweep, k_gp_sweep_pre, k_gp_residual)
@@ -567,7 +568,7 @@ nk_status smooth_from(nk_solver *s, int l, Pair<T> cur, Pair<T> oth, const T *bu
         for (int k = 0; k < sweeps; k++) {
             const bool last = (k == sweeps - 1) && final_dot;
             {
-                const bool fz = last && l == 0 && sizeof(T) == 4 && s->fz_z64u;   // mixed: z64 and r64·z64 here
+                const bool fz = last && l == 0 && sizeof(T) == 4 && s->fz_r64u;   // mixed: r64·z64 (and z64) here
                 if (fz) {
                     if (!launch_sweep_m_z64(s, l, L.g, B2, cur, oth, c, wf, bu, bv, gate))
                         NK_LAUNCH_TOPO(L.g, k_sweep_z64, T, B2, THREADS, 0, s->stream, L.g, (const T *)cur.u, (const T *)cur.v, c, wf, bu,
@@ -842,17 +843,22 @@ nk_status write_scalars(nk_solver *s, const nk_control &ctl) {
 // Mixed-precision preconditioner: z = V₃₂(r) with r cast to FP32 and z promoted back to FP64, then the PCG dot(s)
 // r·z (and r·r at the start) in FP64 → finisher `fin`. Inside the PCG loop (FIN_BETA) the FP32 copy of r comes from
 // k_update_xr, and with point Jacobi the V-cycle's last finest-level sweep writes z64 and sums r·z itself.
-nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate) {
+// With zout (the PCG's direction update reads the FP32 z itself) the last sweep does not store z64 and *zout names the
+// FP32 result; *zout stays null where the result is only in the FP64 z.
+nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate, Pair<float> *zout = nullptr) {
     const KG &g = s->g;
     const dim3 B = blocks_for(g.Nx, g.Ny);
     Work<double> &W = s->w64;
     Work<float> &V = s->w32;
     if (fin == FIN_BETA && s->op.smoother == NK_POINT_JACOBI && s->op.smoothing_sweeps > 0 && s->lev.size() >= 2 &&
         s->tail_start[1] != 0) {
-        s->fz_r64u = W.ru; s->fz_r64v = W.rv; s->fz_z64u = W.zu; s->fz_z64v = W.zv;
+        s->fz_r64u = W.ru; s->fz_r64v = W.rv;
+        s->fz_z64u = zout ? nullptr : W.zu;
+        s->fz_z64v = zout ? nullptr : W.zv;
         Pair<float> z32;
         const nk_status st = v_cycle<float>(s, 0, Pair<float>{V.zu, V.zv}, Pair<float>{V.z2u, V.z2v}, V.ru, V.rv, true, gate, &z32);
         s->fz_r64u = s->fz_r64v = nullptr; s->fz_z64u = s->fz_z64v = nullptr;
+        if (zout) *zout = z32;
         return st;
     }
     if (fin != FIN_BETA) {
@@ -921,9 +927,11 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
                 CHK(launch_check(s));
                 CHK(finish_red(s, B, 1, FIN_RES, s->S));
                 Pair<T> zz{W.zu, W.zv};
-                if (mixed) CHK(precondition_fp32(s, FIN_BETA, s->S));
+                Pair<float> z32{nullptr, nullptr};
+                if (mixed) CHK(precondition_fp32(s, FIN_BETA, s->S, s->use_z32 ? &z32 : nullptr));
                 else CHK(v_cycle<T>(s, 0, Pair<T>{W.zu, W.zv}, Pair<T>{W.z2u, W.z2v}, W.ru, W.rv, true, s->S, &zz));
-                k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, zz.u, zz.v, W.pu, W.pv, 0, s->S);
+                if (z32.u) k_update_p32<T><<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.pu, W.pv, s->S);
+                else k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, zz.u, zz.v, W.pu, W.pv, 0, s->S);
                 CHK(launch_check(s));
             } else {
                 {
@@ -1339,6 +1347,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->use_nmask = !(nn && nn[0] == '1');
         const char *na = std::getenv("NK_NO_ATILE");
         s->apply_tile = !(na && na[0] == '1');
+        const char *nz = std::getenv("NK_NO_Z32");
+        s->use_z32 = !(nz && nz[0] == '1');
         const char *np = std::getenv("NK_NO_GPACK");
         s->use_gpack = !(np && np[0] == '1');
         if (options->precision == NK_PRECISION_MIXED) {

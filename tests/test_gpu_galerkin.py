@@ -221,3 +221,22 @@ def test_lds_tiled_kernels_equal_global_reads(monkeypatch, precision, coarse, to
     say(f"[galerkin] LDS tiles vs global reads (precision {precision}, coarse {coarse}, {topology}): CG {out[0][2]} / {out[1][2]}")
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("coarse,topology", [(abi.COARSE_GALERKIN, (B, B)), (abi.COARSE_GALERKIN, (P, B)), (abi.COARSE_REDISCRETIZED, (B, B))])
+def test_direction_update_from_fp32_z_equals_fp64_copy(monkeypatch, coarse, topology):
+    """In the mixed form the last finest sweep no longer stores z64 = (double) z32 and the direction update reads the
+    FP32 z (k_update_p32); with NK_NO_Z32=1 at creation the sweep stores z64 and k_update_p reads it. The same values,
+    so solutions and iteration counts must be identical bit for bit."""
+    out = []
+    for z32 in (True, False):
+        if z32:
+            monkeypatch.delenv("NK_NO_Z32", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_Z32", "1")
+        m, info = channel(96, coarse, precision=MIX, picard=2, linear=1e-8, topology=topology)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    say(f"[galerkin] FP32 z vs FP64 copy in the direction update (coarse {coarse}, {topology}): CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
