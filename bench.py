@@ -2,7 +2,7 @@
 
 A step is one full velocity solve (update_velocities!: driving stress, Picard with DIVA coefficient updates and
 multigrid-preconditioned CG, velocity-profile reconstruction) of the 1024 × 1024 floating-channel workload
-(config 3 of the benchmark set), from u = v = 0 and zero DIVA state, with every input already in HBM.
+(the DIVA shelf, 512 km square at dx = 500 m), from u = v = 0 and zero DIVA state, with every input already in HBM.
 `value` is the mixed-precision form's solves/s; the FP64 form's solves/s and the mixed result's relative L2 error
 against it are reported beside it.
 
@@ -18,7 +18,7 @@ CPU baseline: the CPU oracle (plain C, one thread) timed on a bounded sample of 
 coefficient update and three MG-PCG iterations at 1024² — and scaled to one solve with the FP64 GPU run's Picard
 and CG counts.
 
-Serving (config 5, bounded sample): --serve requests of the config-5 stream replayed at their Poisson arrival times
+Serving (bounded sample): --serve requests of the seeded serving stream (sides 64–1024) replayed at their Poisson arrival times
 through nk_server (--serve-slots workers); p50/p99 of finish − arrival and solves/s over the run, beside serial FP64
 GPU solves/s of the first --serve-serial requests.
 """
@@ -146,7 +146,7 @@ def serving_leg(n, rate_hz, slots, serial_sample):
     return {"requests": n, "rate_hz": rate_hz, "slots": slots, "sides_min_max": [min(sides), max(sides)],
             "p50_ms": out["p50_ms"], "p99_ms": out["p99_ms"], "solves_per_s": out["solves_per_s"],
             "serial_fp64_solves_per_s": serial, "serial_sample": serial_sample,
-            "workload": "config5 stream (seed 5): SSA grounded/floating channel per request, mixed precision "
+            "workload": "serving stream (seed 5): SSA grounded/floating channel per request, mixed precision "
                         "(FP32 Galerkin V-cycle), MG-PCG linear 1e-8, Picard 1e-6"}
 
 
@@ -325,7 +325,7 @@ def main():
         results[label] = {"wall": wall, "infos": infos, "u": m.get("U"), "v": m.get("V")}
 
     out = {"metric": METRIC, "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": Wm, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "data": "synthetic (seeded numpy PCG64 generator of config 3)"}
+           "scaling": "weak", "vs_baseline": None, "data": "synthetic (seeded numpy PCG64 generator of the DIVA shelf)"}
     if "fp64" in results and "mixed" in results:
         a = np.concatenate([results["mixed"]["u"].ravel(), results["mixed"]["v"].ravel()])
         b = np.concatenate([results["fp64"]["u"].ravel(), results["fp64"]["v"].ravel()])
@@ -338,7 +338,7 @@ def main():
     out["ms_per_step"] = 1e3 * r["wall"] / K
     out["dtype"] = "f32+f64" if main_label == "mixed" else "f64"
     info_last = r["infos"][-1]
-    out["config"] = {"workload": f"config3 DIVA floating channel {N}x{N} dx={512 / N:g} km, Nsigma=10, Arrhenius A, "
+    out["config"] = {"workload": f"DIVA floating channel {N}x{N} dx={512 / N:g} km, Nsigma=10, Arrhenius A, "
                                  f"MG-PCG (PointJacobi V(2,2)), Picard 1e-6, linear 1e-8; mixed: FP32 V-cycle with "
                                  f"{args.coarse} coarse operators",
                      "grid": [N, N], "precision": main_label,

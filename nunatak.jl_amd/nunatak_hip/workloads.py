@@ -87,7 +87,7 @@ def marine_sheet(Nx=640, Ny=80, dx=1e3, seed=2, preconditioner=abi.PRECONDITIONE
 
 def diva_shelf(N=1024, dx=500.0, seed=3, Nsigma=10, precision=abi.PRECISION_FP64,
                preconditioner=abi.PRECONDITIONER_MULTIGRID, linear_reltol=1e-8, picard_reltol=1e-6):
-    """Config 3: DIVA on config 1's geometry rescaled to 512 km × 512 km (N × N cells at dx): N − 1 ice columns and
+    """DIVA floating shelf: the floating channel's geometry rescaled to 512 km × 512 km (N × N cells at dx): N − 1 ice columns and
     one open-ocean column, H = 1000 − 8·(x/km)·(64 km / 512 km) + box3(N(0, 10 m)), bed −2000, no-slip walls;
     Arrhenius A on a 10-level σ grid from T(ζ) = T_s·ζ (°C), T_s ~ U[−30, 0] °C per column; regularized Coulomb
     (m = 1/3, α² = 0.5, C = 3.16e6, u₀ = 300 m/yr; the shelf is afloat so β ≡ 0)."""
@@ -113,7 +113,7 @@ def diva_shelf(N=1024, dx=500.0, seed=3, Nsigma=10, precision=abi.PRECISION_FP64
 
 def friction_inversion(N=512, dx=1e3, seed=4, A=1e-25, preconditioner=abi.PRECONDITIONER_MULTIGRID,
                        linear_reltol=1e-12, picard_reltol=1e-10):
-    """Config 4: gradient of J = ½Σ(|u| − u_obs)² with respect to log C on config 1's geometry at N × N, dx (512 km at
+    """Friction inversion: gradient of J = ½Σ(|u| − u_obs)² with respect to log C on the floating channel's geometry at N × N, dx (512 km at
     the defaults): N − 1 ice columns and one open-ocean column, H = 1000 − 8·(x/km)·(64 km / L) + box3(N(0, 10 m)),
     no-slip walls, ρᵢ = 917, ρw = 1028, Glen n = 3 with constant A, SSA, Weertman m = 1/3.
 
@@ -139,7 +139,7 @@ def friction_inversion(N=512, dx=1e3, seed=4, A=1e-25, preconditioner=abi.PRECON
 
 
 def serving_request(N, seed, dx=1e3, precision=abi.PRECISION_MIXED, linear_reltol=1e-8, picard_reltol=1e-6):
-    """One request of the serving workload: config 1's geometry at N × N, dx (N − 1 ice columns and one open-ocean
+    """One request of the serving workload: the floating channel's geometry at N × N, dx (N − 1 ice columns and one open-ocean
     column, H = 1000 − 8·(x/km)·(64 km / L) + box3(N(0, 10 m)), no-slip walls) on the friction-inversion bed
     B = −300 − 1400·x/L (grounded upstream, afloat downstream) with its own seeded friction
     C = 3.16e6·exp(GRF(σ = 0.2, ℓ = 20 km)); SSA, Glen A = 1e-25, Weertman m = 1/3, MG-PCG at the given precision

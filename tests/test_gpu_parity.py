@@ -202,8 +202,8 @@ def run_pair(g, phys, opts, fields):
     return h, o, ih, io
 
 
-def test_config1_floating_channel_parity():
-    say("config 1: floating channel 65x32 SSA Jacobi")
+def test_floating_channel_65x32_parity():
+    say("floating channel 65x32 SSA Jacobi")
     h, o, ih, io = run_pair(*W.floating_channel())
     err = rel_l2([h.get("U"), h.get("V")], [o.get("U"), o.get("V")])
     say(f"  picard {ih['picard_iterations']} vs {io['picard_iterations']}, cg {ih['total_linear_iterations']} vs "
@@ -216,8 +216,8 @@ def test_config1_floating_channel_parity():
 
 
 @pytest.mark.parametrize("precond", [abi.PRECONDITIONER_DIAGONAL, abi.PRECONDITIONER_MULTIGRID])
-def test_config2_marine_sheet_parity(precond):
-    say(f"config 2 (160x20 at 4 km): marine sheet precond={precond}")
+def test_marine_sheet_160x20_parity(precond):
+    say(f"marine sheet 160x20 at 4 km, precond={precond}")
     g, phys, opts, f = W.marine_sheet(Nx=160, Ny=20, dx=4e3, preconditioner=precond)
     h, o, ih, io = run_pair(g, phys, opts, f)
     err = rel_l2([h.get("U"), h.get("V")], [o.get("U"), o.get("V")])
@@ -227,8 +227,8 @@ def test_config2_marine_sheet_parity(precond):
     assert err <= 1e-9
 
 
-def test_config3_diva_parity_small():
-    say("config 3 (64x64 DIVA, Arrhenius A, MG)")
+def test_diva_floating_channel_64x64_parity():
+    say("DIVA floating channel 64x64, Arrhenius A, MG")
     g, phys, opts, f = W.diva_shelf(N=64, dx=8e3)
     h, o, ih, io = run_pair(g, phys, opts, f)
     assert np.allclose(h.get("A"), o.get("A"), rtol=1e-14, atol=0)
@@ -241,7 +241,7 @@ def test_config3_diva_parity_small():
 
 
 def test_mixed_precision_matches_fp64():
-    say("mixed vs FP64 on config 3 at 128x128")
+    say("mixed vs FP64 on the DIVA shelf at 128x128")
     out = {}
     for prec in (abi.PRECISION_FP64, abi.PRECISION_MIXED):
         g, phys, opts, f = W.diva_shelf(N=128, dx=4e3, precision=prec)

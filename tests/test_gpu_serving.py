@@ -90,7 +90,7 @@ def test_request_larger_than_budget_fails_cleanly():
 
 @pytest.mark.parametrize("kind", ["diva", "fp64"])
 def test_other_request_kinds_same_alone_and_in_the_mix(kind):
-    """DIVA requests (config 3's generator, 3D Arrhenius A from temperature) and FP64 requests through the server."""
+    """DIVA requests (the DIVA shelf's generator, 3D Arrhenius A from temperature) and FP64 requests through the server."""
     reqs = []
     for k, n in enumerate([64, 80, 64, 96]):
         if kind == "diva":
