@@ -13,6 +13,7 @@
 // updates the PCG scalars on the device, so no host round trip is needed inside an iteration.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cfloat>
 #include <stdint.h>
 
 namespace nk {
@@ -279,6 +280,11 @@ struct Scal {
     int maxiter, fixed, it, done;
     double out[MAXRED];
     double picard_D, picard_X;
+    // Mixed precision: the FP32 preconditioner sees r·zs and its z is scaled back by zi = 1/zs. zs = 2^−⌊log₂ ρ⌋ with ρ the
+    // residual norm before the iteration's update, so r32 is O(1) however small the FP64 residual gets (a warm-started
+    // solve to 1e-14 relative takes ‖r‖ to 1e-27 and below, where z32 = r32/ℓ1 would fall into FP32 denormals). A power of
+    // two scales every FP32 operation exactly: without underflow, results are the unscaled ones bit for bit.
+    double zs, zi;
 };
 
 enum Fin : int {
@@ -293,7 +299,15 @@ enum Fin : int {
 
 __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
     switch (fin) {
-    case FIN_ALPHA: S->pq = t[0]; S->alpha = t[0] == 0.0 ? 0.0 : S->rho / t[0]; break;
+    case FIN_ALPHA: {
+        S->pq = t[0];
+        S->alpha = t[0] == 0.0 ? 0.0 : S->rho / t[0];
+        const int e = (S->res > 0.0 && S->res <= DBL_MAX) ? ilogb(S->res) : 0;
+        const int k = e < -1000 ? 1000 : (e > 1000 ? -1000 : -e);
+        S->zs = ldexp(1.0, k);
+        S->zi = ldexp(1.0, -k);
+        break;
+    }
     case FIN_BETA_RES: {
         double rz = t[0];
         S->beta = S->rho == 0.0 ? 0.0 : rz / S->rho;
@@ -516,7 +530,7 @@ __global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int p
 }
 
 // Multigrid PCG: x += α p; r += (−α) q; r·r → res, it, done.
-// With r32u: also r32 = (float) r, the FP32 preconditioner's right-hand side (mixed precision).
+// With r32u: also r32 = (float) (r·zs), the FP32 preconditioner's right-hand side (mixed precision; zs set with α).
 template <typename T>
 __global__ __launch_bounds__(NT) void k_update_xr(KG g, T *__restrict__ xu, T *__restrict__ xv, const T *__restrict__ pu,
                                                   const T *__restrict__ pv, const T *__restrict__ qu, const T *__restrict__ qv,
@@ -534,8 +548,9 @@ __global__ __launch_bounds__(NT) void k_update_xr(KG g, T *__restrict__ xu, T *_
         ru[o] = a;
         rv[o] = b;
         if (r32u) {
-            r32u[o] = (float)a;
-            r32v[o] = (float)b;
+            const double zs = S->zs;
+            r32u[o] = (float)((double)a * zs);
+            r32v[o] = (float)((double)b * zs);
         }
         acc[0] = (double)a * (double)a + (double)b * (double)b;
     }
@@ -561,7 +576,7 @@ __global__ __launch_bounds__(NT) void k_update_p(KG g, const T *__restrict__ zu,
     }
 }
 
-// k_update_p with the V-cycle's FP32 result: p = z + β p with z = (double) z32, the value k_sweep_m_z64 would have
+// k_update_p with the V-cycle's FP32 result: p = z + β p with z = (double) z32 · zi, the value k_sweep_m_z64 would have
 // stored as z64, so p is k_update_p's bit for bit without the FP64 copy of z being written and read.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_update_p32(KG g, const float *__restrict__ zu, const float *__restrict__ zv, T *__restrict__ pu,
@@ -570,9 +585,9 @@ __global__ __launch_bounds__(NT) void k_update_p32(KG g, const float *__restrict
     NK_IJ
     if (i <= g.Nx && j <= g.Ny) {
         const long o = off(g, i, j);
-        const T be = (T)S->beta;
-        pu[o] = T(1) * (T)zu[o] + be * pu[o];
-        pv[o] = T(1) * (T)zv[o] + be * pv[o];
+        const T be = (T)S->beta, zi = (T)S->zi;
+        pu[o] = T(1) * ((T)zu[o] * zi) + be * pu[o];
+        pv[o] = T(1) * ((T)zv[o] * zi) + be * pv[o];
     }
 }
 
@@ -615,12 +630,14 @@ __global__ __launch_bounds__(NT) void k_picard_change(KG g, double *__restrict__
 // Fused ℓ1-Jacobi sweep x_out = x + (b − A x) / s (src/StressBalances/multigrid_preconditioner.jl:165-219).
 // One ℓ1-Jacobi sweep x_out = x + (b − A x)/ℓ1 (multigrid_preconditioner.jl:195-219), two rows per thread.
 // DOT: Σ b·x_out → finisher (the r·z of the PCG at the finest level). FZ (the last sweep of the mixed form's FP32
-// V-cycle): also z64 = (double) x_out, and the sum is Σ r64·z64 with the FP64 residual r64.
+// V-cycle): also z64 = (double) x_out · zi (zi undoes the power-of-two scale of the FP32 right-hand side), and the sum
+// is Σ r64·z64 with the FP64 residual r64.
 template <typename T, bool FZ, bool DOT>
 __device__ __forceinline__ void sweep_body(KG g, int i, int j0, const T *__restrict__ xu, const T *__restrict__ xv, const Coef<T> &c,
                                            T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
                                            T *__restrict__ ov, const Red &red, const double *__restrict__ r64u,
-                                           const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v) {
+                                           const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v,
+                                           double zi = 1.0) {
     double acc[1] = {0.0};
 #pragma unroll
     for (int r = 0; r < SROWS; r++) {
@@ -636,7 +653,7 @@ __device__ __forceinline__ void sweep_body(KG g, int i, int j0, const T *__restr
             st32(ou, o, nu);
             st32(ov, o, nv);
             if (FZ) {
-                const double a = (double)nu, b = (double)nv;
+                const double a = (double)nu * zi, b = (double)nv * zi;
                 if (z64u) {   // null: the PCG reads the FP32 z (k_update_p32), z64 = (double) z32 is not stored
                     st32(z64u, o, a);
                     st32(z64v, o, b);
@@ -679,7 +696,7 @@ __global__ __launch_bounds__(NT) void k_sweep_z64(KG g, const T *__restrict__ xu
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ2
-    sweep_body<T, true, false>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
+    sweep_body<T, true, false>(g, i, j0, xu, xv, c, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v, gate ? gate->zi : 1.0);
 }
 
 // Neighbour masks of the finest FP32 level: one 16-bit word per point holding every activity and peripheral test the
@@ -844,7 +861,8 @@ __device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__res
                                              const uint16_t *__restrict__ nm, T wf, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
                                              T *__restrict__ ov, const Red &red, const double *__restrict__ r64u,
                                              const double *__restrict__ r64v, double *__restrict__ z64u, double *__restrict__ z64v,
-                                             const T (*tu)[ATW] = nullptr, const T (*tv)[ATW] = nullptr, int it0 = 0, int jt0 = 0) {
+                                             const T (*tu)[ATW] = nullptr, const T (*tv)[ATW] = nullptr, int it0 = 0, int jt0 = 0,
+                                             double zi = 1.0) {
     double acc[1] = {0.0};
 #pragma unroll
     for (int r = 0; r < SROWS; r++) {
@@ -860,7 +878,7 @@ __device__ __forceinline__ void sweep_m_body(KG g, int i, int j0, const T *__res
             st32(ou, o, nu);
             st32(ov, o, nv);
             if (FZ) {
-                const double a = (double)nu, b = (double)nv;
+                const double a = (double)nu * zi, b = (double)nv * zi;
                 if (z64u) {   // null: the PCG reads the FP32 z (k_update_p32), z64 = (double) z32 is not stored
                     st32(z64u, o, a);
                     st32(z64v, o, b);
@@ -904,7 +922,8 @@ __global__ __launch_bounds__(NT) void k_sweep_m_z64(KG g, const T *__restrict__ 
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     NK_IJ2
-    sweep_m_body<T, true, false>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v);
+    sweep_m_body<T, true, false>(g, i, j0, xu, xv, c, nm, wf, bu, bv, ou, ov, red, r64u, r64v, z64u, z64v, nullptr, nullptr, 0, 0,
+                                 gate ? gate->zi : 1.0);
 }
 
 // The iterate of a 64 × (4·SROWS) block on a 66 × (4·SROWS + 2) LDS tile per component, each element read once at its
@@ -1275,16 +1294,18 @@ __global__ __launch_bounds__(NT) void k_demote_r(KG g, const double *__restrict_
     r32v[o] = (float)rv[o];
 }
 
-// Mixed precision preconditioning: z64 = (double) z32 and (r·z[, r·r]) → finisher
+// Mixed precision preconditioning: z64 = (double) z32 · zi and (r·z[, r·r]) → finisher. zi = scale->zi where the FP32
+// right-hand side carried the PCG's power-of-two scale (k_update_xr), 1 where scale is null (k_demote_r).
 __global__ __launch_bounds__(NT) void k_promote_dot(KG g, const float *__restrict__ z32u, const float *__restrict__ z32v,
                                                     double *__restrict__ zu, double *__restrict__ zv, const double *__restrict__ ru,
-                                                    const double *__restrict__ rv, Red red, const Scal *gate) {
+                                                    const double *__restrict__ rv, Red red, const Scal *gate, const Scal *scale) {
     if (solve_done(gate)) return;
     NK_IJ
     double acc[2] = {0.0, 0.0};
     if (i <= g.Nx && j <= g.Ny) {
         const long o = off(g, i, j);
-        const double a = (double)z32u[o], b = (double)z32v[o];
+        const double zi = scale ? scale->zi : 1.0;
+        const double a = (double)z32u[o] * zi, b = (double)z32v[o] * zi;
         zu[o] = a;
         zv[o] = b;
         acc[0] = ru[o] * a + rv[o] * b;

@@ -835,6 +835,7 @@ nk_status write_scalars(nk_solver *s, const nk_control &ctl) {
     h.abstol = ctl.abstol;
     h.fixed = ctl.kind == NK_FIXED_ITERATIONS;
     h.maxiter = h.fixed ? ctl.iterations : ctl.maxiter;
+    h.zs = h.zi = 1.0;
     *s->S_host = h;
     HIPCHK(hipMemcpyAsync(s->S, s->S_host, sizeof(Scal), hipMemcpyHostToDevice, s->stream));
     return NK_OK;
@@ -867,7 +868,8 @@ nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate, Pair<float>
     }
     Pair<float> z32;
     CHK(v_cycle<float>(s, 0, Pair<float>{V.zu, V.zv}, Pair<float>{V.z2u, V.z2v}, V.ru, V.rv, false, gate, &z32));
-    k_promote_dot<<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.zu, W.zv, W.ru, W.rv, red(s, fin), gate);
+    k_promote_dot<<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.zu, W.zv, W.ru, W.rv, red(s, fin), gate,
+                                                 fin == FIN_BETA ? (const Scal *)s->S : nullptr);
     CHK(launch_check(s));
     return finish_red(s, B, 2, fin, gate);
 }

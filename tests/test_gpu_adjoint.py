@@ -120,10 +120,14 @@ def test_mixed_galerkin_adjoint_matches_fp64():
     m2.set("BED", 0.0)
     m2.set("A", 1e-24)
     m2.set("C", field_values(g2, "C", lambda x, y: 3e6 * (1 + x / L)))
-    adjoint_of_cost(m2, g2)
+    _, info = adjoint_of_cost(m2, g2)
     err = np.max(np.abs(m2.get("CBAR") - ref)) / np.max(np.abs(ref))
-    say(f"[adjoint] mixed Galerkin vs FP64 diagonal: {err:.3e}")
+    per_solve = info["total_linear_iterations"] / max(info["picard_iterations"], 1)
+    say(f"[adjoint] mixed Galerkin vs FP64 diagonal: {err:.3e}, {per_solve:.1f} CG iterations per adjoint solve")
     assert err < 1e-8
+    # the warm-started adjoint solves reach residuals near 1e-27; the FP32 V-cycle works on a power-of-two rescaled
+    # residual, so they converge like any other solve instead of stalling in FP32 denormals
+    assert per_solve < 100
 
 
 def test_multigrid_and_diagonal_adjoints_agree():
