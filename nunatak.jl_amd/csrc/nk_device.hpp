@@ -1176,6 +1176,26 @@ __global__ __launch_bounds__(NT) void k_restrict(KG cg, KG fg, int hx, int hy, c
     bv[o] = (cact[o] & 2) ? sv : T(0);
 }
 
+// (P e)_u and (P e)_v at fine point (i, j): bilinear weights of the parents, inactive coarse points read as zero.
+template <typename T>
+__device__ __forceinline__ T prolong_u(const KG &fg, const KG &cg, int hx, int hy, const uint8_t *cact, const T *cu, int i, int j) {
+    int I1, I2, J1, J2;
+    T wx1, wx2, wy1, wy2;
+    parents<T>(i, cg.Nx, fg.tx, 1, hx, I1, I2, wx1, wx2);
+    parents<T>(j, cg.Ny, fg.ty, 0, hy, J1, J2, wy1, wy2);
+    return wx1 * (wy1 * iu(cg, cact, cu, I1, J1) + wy2 * iu(cg, cact, cu, I1, J2)) +
+           wx2 * (wy1 * iu(cg, cact, cu, I2, J1) + wy2 * iu(cg, cact, cu, I2, J2));
+}
+template <typename T>
+__device__ __forceinline__ T prolong_v(const KG &fg, const KG &cg, int hx, int hy, const uint8_t *cact, const T *cv, int i, int j) {
+    int I1, I2, J1, J2;
+    T wx1, wx2, wy1, wy2;
+    parents<T>(i, cg.Nx, fg.tx, 0, hx, I1, I2, wx1, wx2);
+    parents<T>(j, cg.Ny, fg.ty, 1, hy, J1, J2, wy1, wy2);
+    return wx1 * (wy1 * iv(cg, cact, cv, I1, J1) + wy2 * iv(cg, cact, cv, I1, J2)) +
+           wx2 * (wy1 * iv(cg, cact, cv, I2, J1) + wy2 * iv(cg, cact, cv, I2, J2));
+}
+
 // _prolongate! (multigrid_preconditioner.jl:354-360): fine x += P (coarse x with inactive points zeroed)
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_prolongate(KG fg, KG cg, int hx, int hy, const uint8_t *__restrict__ fact,
@@ -1188,16 +1208,8 @@ __global__ __launch_bounds__(NT) void k_prolongate(KG fg, KG cg, int hx, int hy,
     NK_IJ
     if (i > fg.Nx || j > fg.Ny) return;
     const long o = off(fg, i, j);
-    int I1, I2, J1, J2;
-    T wx1, wx2, wy1, wy2;
-    parents<T>(i, cg.Nx, fg.tx, 1, hx, I1, I2, wx1, wx2);
-    parents<T>(j, cg.Ny, fg.ty, 0, hy, J1, J2, wy1, wy2);
-    const T pu = wx1 * (wy1 * iu(cg, cact, cu, I1, J1) + wy2 * iu(cg, cact, cu, I1, J2)) +
-                 wx2 * (wy1 * iu(cg, cact, cu, I2, J1) + wy2 * iu(cg, cact, cu, I2, J2));
-    parents<T>(i, cg.Nx, fg.tx, 0, hx, I1, I2, wx1, wx2);
-    parents<T>(j, cg.Ny, fg.ty, 1, hy, J1, J2, wy1, wy2);
-    const T pv = wx1 * (wy1 * iv(cg, cact, cv, I1, J1) + wy2 * iv(cg, cact, cv, I1, J2)) +
-                 wx2 * (wy1 * iv(cg, cact, cv, I2, J1) + wy2 * iv(cg, cact, cv, I2, J2));
+    const T pu = prolong_u<T>(fg, cg, hx, hy, cact, cu, i, j);
+    const T pv = prolong_v<T>(fg, cg, hx, hy, cact, cv, i, j);
     xu[o] += (fact[o] & 1) ? pu : T(0);
     xv[o] += (fact[o] & 2) ? pv : T(0);
 }

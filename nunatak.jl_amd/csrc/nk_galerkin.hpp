@@ -469,27 +469,48 @@ __global__ __launch_bounds__(NT) void k_gp_residual(KG g, const T *__restrict__ 
 // stencil reaches two points each way), each element once at its clamped or wrapped index as gp_row_f reads it. Same
 // values and order, so the results are k_gp_sweep's and k_gp_residual's bit for bit.
 constexpr int GTW = BX + 4, GTH = BY + 4;
-template <typename T>
+
+// The coarse correction of k_prolongate (coarse grid, refinement flags, activity flags, coarse e): with it the tile
+// load forms x + P e itself.
+template <typename T> struct ProSrc {
+    KG cg;
+    int hx, hy;
+    const uint8_t *fact, *cact;
+    const T *cu, *cv;
+};
+template <typename T, bool PRO = false>
 __device__ __forceinline__ void load_gtile(const KG &g, const T *__restrict__ xu, const T *__restrict__ xv, T (*tu)[GTW], T (*tv)[GTW],
-                                           int i0, int j0) {
+                                           int i0, int j0, const ProSrc<T> *pr = nullptr) {
     const int tid = threadIdx.y * BX + threadIdx.x;
     for (int e = tid; e < GTH * GTW; e += NT) {
         const int tj = e / GTW, ti = e - tj * GTW;
         // every column a real point reads lies in [−1, N + 2]; beyond it the clamp only keeps idle elements in range
         const int i = clampi(i0 - 2 + ti, -1, g.Nx + 2), j = clampi(j0 - 2 + tj, -1, g.Ny + 2);
-        tu[tj][ti] = xu[off(g, fx(g, i), cy(g, j))];
-        tv[tj][ti] = xv[off(g, cx(g, i), fy(g, j))];
+        const int ui = fx(g, i), uj = cy(g, j), vi = cx(g, i), vj = fy(g, j);
+        T a = xu[off(g, ui, uj)], b = xv[off(g, vi, vj)];
+        if (PRO) {   // k_prolongate's update of the points 1..Nx × 1..Ny, with its operands and order
+            if (ui >= 1 && ui <= g.Nx && uj >= 1 && uj <= g.Ny) {
+                const T p = prolong_u<T>(g, pr->cg, pr->hx, pr->hy, pr->cact, pr->cu, ui, uj);
+                a += (pr->fact[off(g, ui, uj)] & 1) ? p : T(0);
+            }
+            if (vi >= 1 && vi <= g.Nx && vj >= 1 && vj <= g.Ny) {
+                const T p = prolong_v<T>(g, pr->cg, pr->hx, pr->hy, pr->cact, pr->cv, vi, vj);
+                b += (pr->fact[off(g, vi, vj)] & 2) ? p : T(0);
+            }
+        }
+        tu[tj][ti] = a;
+        tv[tj][ti] = b;
     }
 }
-template <typename T, int TX, int TY, bool RES>
+template <typename T, int TX, int TY, bool RES, bool PRO = false>
 __device__ __forceinline__ void gp_pass_t(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ xu,
                                           const T *__restrict__ xv, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
-                                          T *__restrict__ ov) {
+                                          T *__restrict__ ov, const ProSrc<T> *pr = nullptr) {
     __shared__ T tu[GTH][GTW], tv[GTH][GTW];
     int bx_, by_;
     xcd_tile(bx_, by_);
     const int i0 = bx_ * BX + 1, j0 = by_ * BY + 1;
-    load_gtile<T>(g, xu, xv, tu, tv, i0, j0);
+    load_gtile<T, PRO>(g, xu, xv, tu, tv, i0, j0, pr);
     __syncthreads();
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     if (i > g.Nx || j > g.Ny) return;
@@ -514,6 +535,19 @@ __global__ __launch_bounds__(NT) void k_gp_sweep_t(KG g, const T *__restrict__ s
     g.tx = TX; g.ty = TY;
     if (solve_done(gate)) return;
     gp_pass_t<T, TX, TY, false>(g, st, pk, xu, xv, bu, bv, ou, ov);
+}
+// The first post-smoothing sweep with the prolongation folded into the tile load: every tile element k_prolongate
+// would update gets x + P e with k_prolongate's arithmetic (the same helpers, no contraction), so the result is
+// k_prolongate followed by k_gp_sweep_t, bit for bit, without the corrected x being stored and loaded again. The
+// correction is formed per tile element, about twice per point; the coarse e and flags come from L2.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_sweep_pt(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ xu,
+                                                    const T *__restrict__ xv, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                    T *__restrict__ ou, T *__restrict__ ov, ProSrc<T> pr, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    pr.cg.tx = TX; pr.cg.ty = TY;
+    if (solve_done(gate)) return;
+    gp_pass_t<T, TX, TY, false, true>(g, st, pk, xu, xv, bu, bv, ou, ov, &pr);
 }
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_gp_residual_t(KG g, const T *__restrict__ pk, const T *__restrict__ xu, const T *__restrict__ xv,

@@ -242,6 +242,27 @@ def test_direction_update_from_fp32_z_equals_fp64_copy(monkeypatch, coarse, topo
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("precision,topology,N,Ny", [(MIX, (B, B), 128, None), (MIX, (P, P), 96, None), (MIX, (B, P), 160, 96),
+                                                     (F64, (P, B), 128, None)])
+def test_prolongation_inside_first_post_sweep_equals_separate_kernels(monkeypatch, precision, topology, N, Ny):
+    """On packed Galerkin levels the first post-smoothing sweep forms x + P e while loading its LDS tile
+    (k_gp_sweep_pt); with NK_NO_PRO_FUSE=1 at creation k_prolongate writes x first and k_gp_sweep_t reads it. The same
+    values in the same order, so solutions and iteration counts must be identical bit for bit."""
+    out = []
+    for fused in (True, False):
+        if fused:
+            monkeypatch.delenv("NK_NO_PRO_FUSE", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_PRO_FUSE", "1")
+        m, info = channel(N, abi.COARSE_GALERKIN, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    say(f"[galerkin] prolongation in the first post-sweep vs k_prolongate (precision {precision}, {topology}, {N}x{Ny or N}): "
+        f"CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("coarse", [abi.COARSE_GALERKIN, abi.COARSE_REDISCRETIZED])
 def test_mixed_linear_solve_scales_exactly_with_a_tiny_right_hand_side(coarse):
     """The mixed PCG hands the FP32 V-cycle r·zs with zs = 2^−⌊log₂‖r‖⌋ and scales its z back, so a right-hand side
