@@ -281,7 +281,8 @@ struct Scal {
     double out[MAXRED];
     double picard_D, picard_X;
     // Mixed precision: the FP32 preconditioner sees r·zs and its z is scaled back by zi = 1/zs. zs = 2^−⌊log₂ ρ⌋ with ρ the
-    // residual norm before the iteration's update, so r32 is O(1) however small the FP64 residual gets (a warm-started
+    // residual norm before the iteration's update (‖r₀‖ for a solve's first preconditioning), so r32 is O(1) however
+    // small the FP64 residual gets (a warm-started
     // solve to 1e-14 relative takes ‖r‖ to 1e-27 and below, where z32 = r32/ℓ1 would fall into FP32 denormals). A power of
     // two scales every FP32 operation exactly: without underflow, results are the unscaled ones bit for bit.
     double zs, zi;
@@ -295,19 +296,25 @@ enum Fin : int {
     FIN_RES = 4,      // r·r → res, it, done (multigrid path: β follows after the V-cycle)
     FIN_BETA = 5,     // r·z → β, ρ
     FIN_PICARD = 6,   // (Δ², X²) → picard_D, picard_X
+    FIN_SCALE = 7,    // r·r → zs, zi (the mixed form's first FP32 right-hand side)
 };
+
+// zs = 2^−⌊log₂ n⌋ and zi = 1/zs for a norm n (1 for n = 0 or not finite; the exponent kept within ±1000)
+__device__ __forceinline__ void set_scale(Scal *S, double n) {
+    const int e = (n > 0.0 && n <= DBL_MAX) ? ilogb(n) : 0;
+    const int k = e < -1000 ? 1000 : (e > 1000 ? -1000 : -e);
+    S->zs = ldexp(1.0, k);
+    S->zi = ldexp(1.0, -k);
+}
 
 __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
     switch (fin) {
-    case FIN_ALPHA: {
+    case FIN_ALPHA:
         S->pq = t[0];
         S->alpha = t[0] == 0.0 ? 0.0 : S->rho / t[0];
-        const int e = (S->res > 0.0 && S->res <= DBL_MAX) ? ilogb(S->res) : 0;
-        const int k = e < -1000 ? 1000 : (e > 1000 ? -1000 : -e);
-        S->zs = ldexp(1.0, k);
-        S->zi = ldexp(1.0, -k);
+        set_scale(S, S->res);
         break;
-    }
+    case FIN_SCALE: set_scale(S, sqrt(t[0])); break;
     case FIN_BETA_RES: {
         double rz = t[0];
         S->beta = S->rho == 0.0 ? 0.0 : rz / S->rho;
@@ -1295,19 +1302,21 @@ __global__ __launch_bounds__(NT) void k_demote(KG g, const double *__restrict__ 
     dv[o] = 0.0f;
 }
 
-// Mixed precision preconditioning: r32 = (float) r64 over :xyz (gated like the PCG kernels)
+// Mixed precision preconditioning: r32 = (float) (r64 · scale->zs) over :xyz (gated like the PCG kernels)
 __global__ __launch_bounds__(NT) void k_demote_r(KG g, const double *__restrict__ ru, const double *__restrict__ rv,
-                                                 float *__restrict__ r32u, float *__restrict__ r32v, const Scal *gate) {
+                                                 float *__restrict__ r32u, float *__restrict__ r32v, const Scal *gate,
+                                                 const Scal *scale) {
     if (solve_done(gate)) return;
     NK_IJ
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
-    r32u[o] = (float)ru[o];
-    r32v[o] = (float)rv[o];
+    const double zs = scale->zs;
+    r32u[o] = (float)(ru[o] * zs);
+    r32v[o] = (float)(rv[o] * zs);
 }
 
-// Mixed precision preconditioning: z64 = (double) z32 · zi and (r·z[, r·r]) → finisher. zi = scale->zi where the FP32
-// right-hand side carried the PCG's power-of-two scale (k_update_xr), 1 where scale is null (k_demote_r).
+// Mixed precision preconditioning: z64 = (double) z32 · zi and (r·z[, r·r]) → finisher, zi = scale->zi undoing the
+// power-of-two scale the FP32 right-hand side carried (k_update_xr, k_demote_r).
 __global__ __launch_bounds__(NT) void k_promote_dot(KG g, const float *__restrict__ z32u, const float *__restrict__ z32v,
                                                     double *__restrict__ zu, double *__restrict__ zv, const double *__restrict__ ru,
                                                     const double *__restrict__ rv, Red red, const Scal *gate, const Scal *scale) {
@@ -1316,7 +1325,7 @@ __global__ __launch_bounds__(NT) void k_promote_dot(KG g, const float *__restric
     double acc[2] = {0.0, 0.0};
     if (i <= g.Nx && j <= g.Ny) {
         const long o = off(g, i, j);
-        const double zi = scale ? scale->zi : 1.0;
+        const double zi = scale->zi;
         const double a = (double)z32u[o] * zi, b = (double)z32v[o] * zi;
         zu[o] = a;
         zv[o] = b;

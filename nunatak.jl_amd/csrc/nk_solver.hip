@@ -872,14 +872,16 @@ nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate, Pair<float>
         if (zout) *zout = z32;
         return st;
     }
-    if (fin != FIN_BETA) {
-        k_demote_r<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, V.ru, V.rv, gate);
+    if (fin != FIN_BETA) {   // outside the loop: the scale from ‖r‖ first (inside, k_update_xr applied the α-time scale)
+        k_dot<double><<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, W.ru, W.rv, nullptr, nullptr, red(s, FIN_SCALE), gate);
+        CHK(launch_check(s));
+        CHK(finish_red(s, B, 2, FIN_SCALE, gate));
+        k_demote_r<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, V.ru, V.rv, gate, (const Scal *)s->S);
         CHK(launch_check(s));
     }
     Pair<float> z32;
     CHK(v_cycle<float>(s, 0, Pair<float>{V.zu, V.zv}, Pair<float>{V.z2u, V.z2v}, V.ru, V.rv, false, gate, &z32));
-    k_promote_dot<<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.zu, W.zv, W.ru, W.rv, red(s, fin), gate,
-                                                 fin == FIN_BETA ? (const Scal *)s->S : nullptr);
+    k_promote_dot<<<B, THREADS, 0, s->stream>>>(g, z32.u, z32.v, W.zu, W.zv, W.ru, W.rv, red(s, fin), gate, (const Scal *)s->S);
     CHK(launch_check(s));
     return finish_red(s, B, 2, fin, gate);
 }

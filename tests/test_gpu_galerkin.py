@@ -265,15 +265,16 @@ def test_prolongation_inside_first_post_sweep_equals_separate_kernels(monkeypatc
 
 @pytest.mark.parametrize("coarse", [abi.COARSE_GALERKIN, abi.COARSE_REDISCRETIZED])
 def test_mixed_linear_solve_scales_exactly_with_a_tiny_right_hand_side(coarse):
-    """The mixed PCG hands the FP32 V-cycle r·zs with zs = 2^−⌊log₂‖r‖⌋ and scales its z back, so a right-hand side
-    multiplied by 2^−60 gives the same iteration count and a solution multiplied by 2^−60 bit for bit (every FP64 and
-    FP32 operation commutes with a power of two while nothing underflows). Without the scale, the FP32 z of such a
-    solve near 1e-12 relative falls into FP32 denormals and the PCG stalls: the warm-started adjoint solves of the ridge
-    model need ~3600 CG iterations each instead of 13."""
+    """The mixed PCG hands the FP32 V-cycle r·zs with zs = 2^−⌊log₂‖r‖⌋ (‖r₀‖ for the first preconditioning, the
+    pre-update ‖r‖ inside the loop) and scales its z back, so a right-hand side multiplied by 2^−200, far below FP32's
+    range, gives the same iteration count and a solution multiplied by 2^−200 bit for bit (every FP64 and FP32
+    operation commutes with a power of two while nothing underflows). Without the scale the FP32 V-cycle would see
+    zeros here; at ordinary magnitudes its z falls into FP32 denormals near 1e-12 relative and the PCG stalls: the
+    warm-started adjoint solves of the ridge model need ~3600 CG iterations each instead of 13."""
     m, _ = channel(96, coarse, precision=MIX, picard=1, linear=1e-12)
     tu, tv = m.get("TAUU"), m.get("TAUV")
     sols = []
-    for scale in (1.0, 2.0 ** -60):
+    for scale in (1.0, 2.0 ** -200):
         m.set("TAUU", tu * scale)
         m.set("TAUV", tv * scale)
         m.set("U", 0.0)
@@ -282,6 +283,6 @@ def test_mixed_linear_solve_scales_exactly_with_a_tiny_right_hand_side(coarse):
         sols.append((m.get("U"), m.get("V"), info["linear_iterations"]))
     m.close()
     (u1, v1, n1), (u2, v2, n2) = sols
-    say(f"[galerkin] mixed solve with rhs x 2^-60 (coarse {coarse}): CG {n1} / {n2}")
+    say(f"[galerkin] mixed solve with rhs x 2^-200 (coarse {coarse}): CG {n1} / {n2}")
     assert n1 == n2 and n1 > 0
-    assert np.array_equal(u2, u1 * 2.0 ** -60) and np.array_equal(v2, v1 * 2.0 ** -60)
+    assert np.array_equal(u2, u1 * 2.0 ** -200) and np.array_equal(v2, v1 * 2.0 ** -200)
