@@ -60,6 +60,10 @@ struct Level {
 // Levels small enough for the Galerkin LDS tail keep a compact copy of their stencil (shape-only rule, so the
 // device-memory estimate can mirror it).
 constexpr size_t GTAIL_BUDGET = 150 * 1024;
+// Points the Galerkin LDS tail takes on by default beyond the two coarsest levels: 16² and below. From 32² the one
+// workgroup spends longer on that level than four launched kernels do (1024² mixed solve, same-box A/B: 1390.6 ms with
+// the tail from 32², 1374.2 ms from 16², 1515.8 ms from 8²).
+constexpr size_t GTAIL_POINTS = 640;
 bool wants_gcst(const KG &g, int l, size_t es) {
     return l >= 1 && (size_t)(g.Nx + 1) * (size_t)(g.Ny + 1) * (8 * es + 1) <= GTAIL_BUDGET;
 }
@@ -1406,13 +1410,16 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
             const char *gb = std::getenv("NK_GTAIL_BYTES");   // tuning: LDS bytes of vectors the tail may take
             const size_t gbudget = gb ? (size_t)std::strtoul(gb, nullptr, 10) : GTAIL_BUDGET;
-            size_t bytes = 0;
+            size_t bytes = 0, pts = 0;
             int start = -1;
-            for (int l = (int)s->lev.size() - 1; l >= 1; l--) {
+            const int nl = (int)s->lev.size();
+            for (int l = nl - 1; l >= 1; l--) {
                 const Level &L = s->lev[l];
-                const size_t b = (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1) * (8 * es + 1);
-                if (bytes + b > gbudget || bytes + b > GTAIL_BUDGET || !L.gcst || (int)s->lev.size() - l > GTAIL_MAX) break;
+                const size_t p = (size_t)(L.g.Nx + 1) * (size_t)(L.g.Ny + 1), b = p * (8 * es + 1);
+                const bool wanted = gb ? bytes + b <= gbudget : (nl - l <= 2 || pts + p <= GTAIL_POINTS);
+                if (!wanted || bytes + b > GTAIL_BUDGET || !L.gcst || nl - l > GTAIL_MAX) break;
                 bytes += b;
+                pts += p;
                 start = l;
             }
             const void *gt[] = {(const void *)k_g_tail<double, 0, 0>, (const void *)k_g_tail<double, 0, 1>,
