@@ -107,6 +107,7 @@ struct nk_solver {
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     bool fuse_pro = true;   // packed Galerkin levels: prolongation inside the first post-sweep (NK_NO_PRO_FUSE=1: k_prolongate)
+    long pro_fuse_points = 256L * 256;   // ... on levels of at most this many points (NK_PRO_FUSE_POINTS: tuning)
     uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
     bool use_z32 = true;        // mixed: the direction update reads the FP32 z (NK_NO_Z32=1: FP64 z via z64)
     bool apply_tile = true;     // FP64 finest operator applies and sweeps, and the packed Galerkin sweeps and residuals,
@@ -776,8 +777,8 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     CHK(launch_check(s));
     Pair<T> cres;
     CHK(v_cycle<T>(s, l + 1, Pair<T>{CV.x0u, CV.x0v}, Pair<T>{CV.x1u, CV.x1v}, CV.bu, CV.bv, false, gate, &cres));
-    // prolongation inside the first post-sweep on the launch-latency-bound levels (≤ 256 × 256 points)
-    if (pk && s->apply_tile && s->fuse_pro && s->op.smoothing_sweeps >= 1 && (long)L.g.Nx * L.g.Ny <= 256L * 256) {
+    // prolongation inside the first post-sweep on the launch-latency-bound levels (≤ 256 × 256 points by default)
+    if (pk && s->apply_tile && s->fuse_pro && s->op.smoothing_sweeps >= 1 && (long)L.g.Nx * L.g.Ny <= s->pro_fuse_points) {
         const ProSrc<T> pr{C.g, C.cx, C.cy, L.act, C.act, cres.u, cres.v};
         NK_LAUNCH_TOPO(L.g, k_gp_sweep_pt, T, B, THREADS, 0, s->stream, L.g, st, pk, cur.u, cur.v, bu, bv, oth.u, oth.v, pr, gate);
         CHK(launch_check(s));
@@ -1402,6 +1403,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->fuse_pre = !(nf && nf[0] == '1');
         const char *npro = std::getenv("NK_NO_PRO_FUSE");
         s->fuse_pro = !(npro && npro[0] == '1');
+        const char *nproc = std::getenv("NK_PRO_FUSE_POINTS");
+        if (nproc) s->pro_fuse_points = std::strtol(nproc, nullptr, 10);
         const char *nt = std::getenv("NK_NO_TAIL");
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
         // the LDS tail runs the rediscretized operator: levels with Galerkin operators have a tail of their own
