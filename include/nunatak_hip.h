@@ -124,7 +124,8 @@ typedef struct nk_solver_options {
                                          on the handle's stream and report the summed device time in nk_info */
     int32_t graphs;                   /* 1: capture the launches of check_interval PCG iterations in a hipGraph once
                                          and replay it (not while profiling); 0: launch kernel by kernel */
-    int32_t pad;
+    int32_t report_not_converged;     /* 0 (the reference: silent); 1: nk_solve returns NK_NOT_CONVERGED when PCG stops at
+                                         maxiter above its tolerance, the Picard entry points when Picard does */
 } nk_solver_options;
 
 /* Kernels that nk_solver_options.profile_kernel can time (finest grid unless stated). */

@@ -278,6 +278,8 @@ struct Scal {
     double rho, alpha, beta, pq, rr, rz, res, r0, tol;
     double reltol, abstol;
     int maxiter, fixed, it, done;
+    int nonfinite;    // set with done when ‖r‖ is NaN or Inf (a NaN fails every comparison and would stop as converged)
+    int done_by_tol;  // Convergence: the last stop met ‖r‖ ≤ tol
     double out[MAXRED];
     double picard_D, picard_X;
     // Mixed precision: the FP32 preconditioner sees r·zs and its z is scaled back by zi = 1/zs. zs = 2^−⌊log₂ ρ⌋ with ρ the
@@ -307,6 +309,16 @@ __device__ __forceinline__ void set_scale(Scal *S, double n) {
     S->zi = ldexp(1.0, -k);
 }
 
+// The stopping rule of iterate_conjugate_gradient! (src/Solvers/conjugate_gradient.jl:178-200): FixedIterations stops after
+// maxiter steps, Convergence when ‖r‖ ≤ tol or at maxiter. A residual norm that is not finite stops the solve and is
+// flagged, whatever the control.
+__device__ __forceinline__ void set_done(Scal *S) {
+    const bool bad = !(S->res <= DBL_MAX);
+    S->nonfinite |= bad ? 1 : 0;
+    S->done_by_tol = !S->fixed && S->res <= S->tol;
+    S->done = bad || (S->fixed ? (S->it >= S->maxiter) : !(S->res > S->tol && S->it < S->maxiter));
+}
+
 __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
     switch (fin) {
     case FIN_ALPHA:
@@ -322,7 +334,7 @@ __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
         S->rr = t[1];
         S->res = sqrt(t[1]);
         S->it += 1;
-        S->done = S->fixed ? (S->it >= S->maxiter) : !(S->res > S->tol && S->it < S->maxiter);
+        set_done(S);
         break;
     }
     case FIN_INIT: {
@@ -333,14 +345,15 @@ __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
         S->tol = jmax(a, b);
         S->res = S->r0;
         S->it = 0;
-        S->done = S->fixed ? (S->maxiter <= 0) : !(S->res > S->tol && 0 < S->maxiter);
+        S->nonfinite = 0;
+        set_done(S);
         break;
     }
     case FIN_RES:
         S->rr = t[0];
         S->res = sqrt(t[0]);
         S->it += 1;
-        S->done = S->fixed ? (S->it >= S->maxiter) : !(S->res > S->tol && S->it < S->maxiter);
+        set_done(S);
         break;
     case FIN_BETA: S->beta = S->rho == 0.0 ? 0.0 : t[0] / S->rho; S->rho = t[0]; break;
     case FIN_PICARD: S->picard_D = sqrt(t[0]); S->picard_X = sqrt(t[1]); break;
@@ -351,7 +364,6 @@ __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
 
 struct Red {
     double *partials;       // MAXRED * capacity slots
-    unsigned int *counter;  // zero between kernels
     Scal *S;
     int fin;
 };

@@ -80,6 +80,7 @@ struct nk_solver {
     nk_grid grid{};
     nk_physics ph{};
     nk_solver_options op{};
+    nk_solver_options op0{};   // options at nk_create: they fix the hierarchy, the precision and the tails
     KG g{};
     long plane = 0;  // elements per 2D plane
     int Ns = 0;
@@ -121,7 +122,6 @@ struct nk_solver {
     double *en_W = nullptr, *en_a = nullptr, *en_b = nullptr, *en_c = nullptr, *en_f = nullptr, *en_t = nullptr, *en_bas = nullptr;
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
-    unsigned int *counter = nullptr;
     Scal *S = nullptr;
     Scal *S_host = nullptr;
     std::vector<void *> allocs;
@@ -271,7 +271,7 @@ Sig sig_tables(const nk_solver *s) {
     return t;
 }
 
-Red red(nk_solver *s, int fin) { Red r; r.partials = s->partials; r.counter = s->counter; r.S = s->S; r.fin = fin; return r; }
+Red red(nk_solver *s, int fin) { Red r; r.partials = s->partials; r.S = s->S; r.fin = fin; return r; }
 
 template <typename T> Coef<T> coef(const Level &L);
 template <> Coef<double> coef<double>(const Level &L) { return Coef<double>{L.eta, L.etaff, L.bu, L.bv, L.act}; }
@@ -891,7 +891,7 @@ nk_status precondition_fp32(nk_solver *s, int fin, const Scal *gate, Pair<float>
     return finish_red(s, B, 2, fin, gate);
 }
 
-// conjugate_gradient! (src/Solvers/conjugate_gradient.jl:299-364) at precision T on the finest grid. When
+// conjugate_gradient! (src/Solvers/conjugate_gradient.jl:158-223) at precision T on the finest grid. When
 // r_ready, the residual r of the initial guess is already in the workspace (the mixed inner solve from zero).
 template <typename T>
 nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_control &ctl, bool r_ready, int *iters, double *res) {
@@ -1001,6 +1001,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
     }
     *iters = s->S_host->it;
     *res = s->S_host->res;
+    if (s->S_host->nonfinite) { s->err = "the linear solve's residual norm is not finite (NaN or Inf in the inputs?)"; return NK_NONFINITE; }
     return NK_OK;
 }
 
@@ -1112,6 +1113,7 @@ nk_status picard(nk_solver *s, nk_info *info) {
             CHK(read_scalars(s));
             D = s->S_host->picard_D;
             X = s->S_host->picard_X;
+            if (!std::isfinite(D) || !std::isfinite(X)) { s->err = "the Picard change or velocity norm is not finite"; return NK_NONFINITE; }
         }
         it++;
         if (s->verbose) {
@@ -1124,6 +1126,10 @@ nk_status picard(nk_solver *s, nk_info *info) {
         info->picard_iterations = it;
         info->last_picard_change = D;
         info->last_picard_norm = X;
+    }
+    if (s->op.report_not_converged && !fixed && D > jmax_h(ctl.reltol * X, ctl.abstol)) {
+        s->err = "Picard iteration stopped at maxiter before meeting its tolerance";
+        return NK_NOT_CONVERGED;
     }
     return NK_OK;
 }
@@ -1166,6 +1172,13 @@ struct Timer {
         return st;
     }
 };
+
+void drop_chunk_graph(nk_solver *s) {
+    if (s->chunk_graph) { (void)hipStreamSynchronize(s->stream); (void)hipGraphExecDestroy(s->chunk_graph); }
+    s->chunk_graph = nullptr;
+    s->chunk_graph_key = -1;
+    s->chunk_graph_x = nullptr;
+}
 
 nk_status validate(const nk_grid *g, const nk_physics *ph, const nk_solver_options *op, std::string &err) {
     if (!g || !ph || !op) { err = "null argument"; return NK_INVALID_ARGUMENT; }
@@ -1267,7 +1280,7 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
     }
     std::lock_guard<std::mutex> lk(g_device_mu);
     nk_solver *s = new nk_solver();
-    s->grid = *grid; s->ph = *physics; s->op = *options;
+    s->grid = *grid; s->ph = *physics; s->op = *options; s->op0 = *options;
     {
         const char *v = std::getenv("NK_VERBOSE");
         s->verbose = v && v[0] && v[0] != '0';
@@ -1437,7 +1450,6 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
     {
         long nb = (long)((s->g.Nx + BX - 1) / BX) * (long)((s->g.Ny + BY - 1) / BY);
         if ((st = alloc_bytes(s, (void **)&s->partials, sizeof(double) * MAXRED * (size_t)nb)) != NK_OK) return fail(st);
-        if ((st = alloc_bytes(s, (void **)&s->counter, 64)) != NK_OK) return fail(st);
         if (options->coarse_operator == NK_COARSE_GALERKIN && (st = alloc_bytes(s, (void **)&s->gerr, 64)) != NK_OK) return fail(st);
         if (options->coarse_operator == NK_COARSE_GALERKIN && s->lev.size() > 1) {
             const KG &gc = s->lev.back().g;
@@ -1492,7 +1504,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
     b += sizeof(uint16_t) * p0;                                               // neighbour masks of the finest level
     if (mixed) b += 14 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
-    b += sizeof(double) * MAXRED * nb + 64 + sizeof(Scal);
+    b += sizeof(double) * MAXRED * nb + sizeof(Scal);
     if (o->coarse_operator == NK_COARSE_GALERKIN) {
         b += 64;
         const size_t n = 2 * (size_t)lev.back().Nx * (size_t)lev.back().Ny;
@@ -1540,6 +1552,8 @@ nk_status nk_set_physics(nk_solver *s, const nk_physics *physics) {
     if (!s || !physics) return NK_INVALID_ARGUMENT;
     nk_status st = validate(&s->grid, physics, &s->op, s->err);
     if (st != NK_OK) return st;
+    // The captured PCG chunk holds the wall factor in its kernel arguments: a change of physics discards it.
+    if (std::memcmp(&s->ph, physics, sizeof(nk_physics)) != 0) drop_chunk_graph(s);
     s->ph = *physics;
     return NK_OK;
 }
@@ -1548,16 +1562,20 @@ nk_status nk_set_options(nk_solver *s, const nk_solver_options *o) {
     if (!s || !o) return NK_INVALID_ARGUMENT;
     nk_status st = validate(&s->grid, &s->ph, o, s->err);
     if (st != NK_OK) return st;
-    if (o->preconditioner != s->op.preconditioner || o->precision != s->op.precision || o->minimum_coarse_size != s->op.minimum_coarse_size) {
-        if ((o->preconditioner == NK_PRECONDITIONER_MULTIGRID) != (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID) ||
-            o->precision != s->op.precision || o->minimum_coarse_size != s->op.minimum_coarse_size) {
-            s->err = "the multigrid hierarchy and precision are fixed at nk_create";
-            return NK_UNSUPPORTED;
-        }
+    // nk_create sized the multigrid hierarchy, the precision's buffers, the Galerkin stencils and the LDS tails for
+    // its options. A handle created with the multigrid preconditioner may switch to the diagonal one (or none) and
+    // back; every other change of these needs a new handle.
+    const nk_solver_options &c = s->op0;
+    const bool mg0 = c.preconditioner == NK_PRECONDITIONER_MULTIGRID, mg = o->preconditioner == NK_PRECONDITIONER_MULTIGRID;
+    if ((mg && !mg0) || o->precision != c.precision ||
+        (mg0 && (o->minimum_coarse_size != c.minimum_coarse_size || o->smoother != c.smoother ||
+                 o->coarse_operator != c.coarse_operator)) ||
+        (!mg && o->coarse_operator != NK_COARSE_REDISCRETIZED)) {
+        s->err = "the multigrid hierarchy, smoother, coarse operators and precision are fixed at nk_create";
+        return NK_UNSUPPORTED;
     }
     s->op = *o;
-    if (s->chunk_graph) { (void)hipGraphExecDestroy(s->chunk_graph); s->chunk_graph = nullptr; }
-    s->chunk_graph_key = -1;
+    drop_chunk_graph(s);
     return NK_OK;
 }
 
@@ -1730,6 +1748,10 @@ nk_status nk_solve(nk_solver *s, nk_info *info) {
     if (info) info->total_linear_iterations = 0;
     nk_status st = corners_flags(s, 0, false);
     if (st == NK_OK) st = solve_linear(s, info);
+    if (st == NK_OK && s->op.report_not_converged && s->op.linear.kind == NK_CONVERGENCE && !s->S_host->done_by_tol) {
+        s->err = "PCG stopped at maxiter before meeting its tolerance";
+        st = NK_NOT_CONVERGED;
+    }
     return t.done(st);
 }
 
@@ -2086,6 +2108,7 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
         CHK(read_scalars(s));
         D = s->S_host->picard_D;
         M = s->S_host->picard_X;
+        if (!std::isfinite(D) || !std::isfinite(M)) { s->err = "the adjoint fixed point's change or norm is not finite"; return NK_NONFINITE; }
         it++;
         if (s->verbose) {
             std::fprintf(stderr, "[nunatak_hip] adjoint %d: linear %d, change %.3e, norm %.3e\n", it, cg, D, M);

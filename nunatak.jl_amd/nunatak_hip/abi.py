@@ -78,7 +78,7 @@ class SolverOptions(ct.Structure):
                 ("smoother", ct.c_int32), ("smoothing_sweeps", ct.c_int32), ("coarsest_sweeps", ct.c_int32),
                 ("minimum_coarse_size", ct.c_int32), ("precision", ct.c_int32), ("coarse_operator", ct.c_int32),
                 ("check_interval", ct.c_int32), ("profile_kernel", ct.c_int32), ("graphs", ct.c_int32),
-                ("pad", ct.c_int32)]
+                ("report_not_converged", ct.c_int32)]
 
 
 class Info(ct.Structure):
@@ -150,7 +150,7 @@ def default_options(**kw):
     o = SolverOptions(linear=convergence(1e-8, 0.0, 10_000), picard=convergence(1e-6, 0.0, 100),
                       preconditioner=PRECONDITIONER_DIAGONAL, smoother=POINT_JACOBI, smoothing_sweeps=2,
                       coarsest_sweeps=20, minimum_coarse_size=2, precision=PRECISION_FP64,
-                      coarse_operator=COARSE_REDISCRETIZED, check_interval=8, profile_kernel=0, graphs=1, pad=0)
+                      coarse_operator=COARSE_REDISCRETIZED, check_interval=8, profile_kernel=0, graphs=1, report_not_converged=0)
     for k, v in kw.items():
         setattr(o, k, v)
     return o

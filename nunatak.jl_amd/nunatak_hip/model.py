@@ -32,7 +32,11 @@ _lib = None
 
 
 class NunatakError(RuntimeError):
-    pass
+    """A call into libnunatak_hip.so failed; ``status`` holds its nk_status (None when no call was made)."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 def load(path=LIB_PATH):
@@ -116,11 +120,11 @@ class HIPModel:
             if h.value:
                 self._L.nk_destroy(h)
                 self._h = ct.c_void_p()
-            raise NunatakError(f"nk_create: {abi.STATUS_NAMES.get(st, st)} {msg}")
+            raise NunatakError(f"nk_create: {abi.STATUS_NAMES.get(st, st)} {msg}", st)
 
     def _check(self, st, what):
         if st != abi.NK_OK:
-            raise NunatakError(f"{what}: {abi.STATUS_NAMES.get(st, st)}: {self._L.nk_last_error(self._h).decode()}")
+            raise NunatakError(f"{what}: {abi.STATUS_NAMES.get(st, st)}: {self._L.nk_last_error(self._h).decode()}", st)
 
     def close(self):
         if self._h and self._h.value:

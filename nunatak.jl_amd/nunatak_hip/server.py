@@ -60,7 +60,7 @@ class Server:
                                       ct.byref(rid))
         if st != abi.NK_OK:
             raise NunatakError(f"nk_server_submit: {abi.STATUS_NAMES.get(st, st)}: "
-                               f"{self._L.nk_server_last_error(self._h).decode()}")
+                               f"{self._L.nk_server_last_error(self._h).decode()}", st)
         self._shapes[rid.value] = (abi.packed_shape(grid, physics, "U"), abi.packed_shape(grid, physics, "V"))
         return rid.value
 
@@ -81,7 +81,7 @@ class Server:
                                     v.ctypes.data_as(ct.POINTER(ct.c_double)), ct.byref(info))
         if st != abi.NK_OK:
             raise NunatakError(f"request {rid}: {abi.STATUS_NAMES.get(st, st)}: "
-                               f"{self._L.nk_server_last_error(self._h).decode()}")
+                               f"{self._L.nk_server_last_error(self._h).decode()}", st)
         return u, v, info.as_dict()
 
 

@@ -176,6 +176,79 @@ def serving_stream(n=2000, rate_hz=4.0, seed=5, smallest=64, largest=1024, preci
     return out
 
 
+# MISMIP+ marine ice stream of the reference's multigrid scaling benchmark (benchmarks/multigrid_scaling.jl:11-37)
+MISMIP_LX, MISMIP_LY = 640e3, 80e3
+
+
+def mismip_bed(X, Y):
+    """MISMIP+ bed (Asay-Davis et al., 2016) as benchmarks/multigrid_scaling.jl:11-16 writes it."""
+    B0, B2, B4, B6, xbar = -150.0, -728.8, 343.91, -50.57, 300e3
+    fc, dc, wc, z_deep = 4e3, 500.0, 24e3, -720.0
+    r = X / xbar
+    along = B0 + B2 * r ** 2 + B4 * r ** 4 + B6 * r ** 6
+    across = dc / (1 + np.exp(-2 * (Y - MISMIP_LY / 2 - wc) / fc)) + dc / (1 + np.exp(2 * (Y - MISMIP_LY / 2 + wc) / fc))
+    return np.maximum(along + across, z_deep)
+
+
+def mismip_scaling(dx, preconditioner=abi.PRECONDITIONER_MULTIGRID, precision=abi.PRECISION_FP64,
+                   coarse_operator=abi.COARSE_REDISCRETIZED):
+    """scaling_model(arch, Δx) of benchmarks/multigrid_scaling.jl:20-37: 640 km × 80 km at dx (Bounded², halo 4),
+    H = 1500·√(1 − x/650 km) for x < 600 km (calving front at 600 km), MISMIP+ bed, Glen A = 2.0392e-17 /yr with
+    ε̇₀ = 1e-5 /yr, regularized Coulomb C = 1e4·yr^(1/3), α² = 0.5, m = 1/3, u₀ = 1e-5 /yr, free-slip walls,
+    SubgridGroundedFraction(), SmoothIceMask and the IceSheetConstants defaults; Picard FixedIterations(5) with the
+    multigrid PCG at reltol 1e-6 (maxiter 50 000). set!(model, H) runs update_velocities! once with these; the timed
+    solve then starts from zero with the coefficients that Picard left (benchmarks/multigrid_scaling.jl:39-50)."""
+    Nx, Ny = int(round(MISMIP_LX / dx)), int(round(MISMIP_LY / dx))
+    g = abi.default_grid(Nx, Ny, dx, dx, topology=(abi.BOUNDED, abi.BOUNDED), halo=4)
+    X, Y = _centers(Nx, Ny, dx, dx)
+    H = np.where(X < 600e3, 1500.0 * np.sqrt(np.maximum(1.0 - X / 650e3, 0.0)), 0.0)
+    phys = abi.default_physics(grounding=abi.GROUNDING_SUBGRID, friction=abi.FRICTION_REGULARIZED_COULOMB,
+                               friction_exponent=1.0 / 3.0, coulomb_coefficient=0.5,
+                               velocity_regularization=1e-5 / YEAR, strain_rate_regularization=1e-5 / YEAR,
+                               walls=abi.FREE_SLIP)
+    opts = abi.default_options(linear=abi.convergence(1e-6, 0.0, 50_000), picard=abi.fixed_iterations(5),
+                               preconditioner=preconditioner, precision=precision, coarse_operator=coarse_operator)
+    fields = {"H": H, "BED": mismip_bed(X, Y), "A": np.full((Ny, Nx), 2.0392e-17 / YEAR),
+              "C": np.full((Ny, Nx), 1e4 * YEAR ** (1.0 / 3.0))}
+    return g, phys, opts, fields
+
+
+# Iterations of one solve from zero to reltol 1e-8 published with the benchmark (docs/src/model/numerics.md:44-49):
+# Δx (m) → (Jacobi-diagonal PCG, multigrid PCG)
+MISMIP_SCALING_ITERATIONS = {8000.0: (42, 14), 4000.0: (91, 18), 2000.0: (188, 20), 1000.0: (392, 22),
+                             500.0: (817, 24), 250.0: (1769, 27)}
+# Published times of the same solves (s): Δx → (CPU Jacobi, CPU multigrid, H200 Jacobi, H200 multigrid); CPU is one
+# thread of an AMD EPYC 9745 (docs/src/model/numerics.md:44-52)
+MISMIP_SCALING_TIMES = {8000.0: (0.001, 0.002, 0.07, 0.03), 4000.0: (0.004, 0.007, 0.05, 0.08),
+                        2000.0: (0.03, 0.02, 0.11, 0.04), 1000.0: (0.22, 0.10, 0.24, 0.06),
+                        500.0: (1.9, 0.43, 0.59, 0.06), 250.0: (None, None, 1.3, 0.08)}
+
+
+def mismip_timed_solves(model, reltol=1e-8, maxiter=100_000, jacobi=True, multigrid=True):
+    """timed_solve of benchmarks/multigrid_scaling.jl:39-50 on a model (HIPModel or the oracle) created with the
+    multigrid preconditioner and loaded with mismip_scaling's fields: update_velocities! (as set! runs it), then
+    each preconditioner solves A x = τ from x = 0 to reltol with the coefficients Picard left. Returns
+    {"jacobi": (iterations, u, v, info), "multigrid": (...)}."""
+    base = model.options
+    model.update_velocities()
+    out = {}
+    for name, pre, want in (("jacobi", abi.PRECONDITIONER_DIAGONAL, jacobi), ("multigrid", abi.PRECONDITIONER_MULTIGRID, multigrid)):
+        if not want:
+            continue
+        o = abi.default_options(**{k: getattr(base, k) for k, _ in base._fields_})
+        o.linear = abi.convergence(reltol, 0.0, maxiter)
+        o.preconditioner = pre
+        if pre != abi.PRECONDITIONER_MULTIGRID:
+            o.precision, o.coarse_operator = abi.PRECISION_FP64, abi.COARSE_REDISCRETIZED
+        model.set_options(o)
+        model.set("U", 0.0)
+        model.set("V", 0.0)
+        info = model.solve()
+        out[name] = (info["linear_iterations"], model.get("U").copy(), model.get("V").copy(), info)
+    model.set_options(base)
+    return out
+
+
 def centre_speed(u, v):
     """|u| at cell centres of a Bounded × Bounded grid: √(ℑx(u)² + ℑy(v)²), u shaped (Ny, Nx + 1), v (Ny + 1, Nx)."""
     uc = (u[:, :-1] + u[:, 1:]) / 2

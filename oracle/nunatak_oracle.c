@@ -25,6 +25,14 @@
 
 #define EXPORT __attribute__((visibility("default")))
 
+/* OpenMP (the timed CPU baseline): loops over grid rows run in parallel when a level is large enough to pay for
+ * it. Every parallel loop writes disjoint points with the same per-point arithmetic, and the pairwise sums split at
+ * the same indices as the serial recursion, so results are bit for bit those of one thread. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#define OMP_BIG(g) ((long)(g)->Nx * (long)(g)->Ny >= 16384)
+
 enum { C = 0, F = 1 };
 
 typedef struct {
@@ -179,6 +187,7 @@ static double y_stress_balance(const og *g, int i, int j, const double *u, const
 /* _ssa_operator! over :xyz (ssa_operator.jl:81-89); the halos of p, η̄H, η̄Hᶠᶠ must be filled. */
 static void ssa_operator(const og *g, double *qu, double *qv, const double *pu, const double *pv, const double *eta,
                          const double *etaff, const double *bu, const double *bv, int walls) {
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= g->Ny; j++)
         for (int i = 1; i <= g->Nx; i++) {
             int wu = inactive_u(g, i, j, eta), wv = inactive_v(g, i, j, eta);
@@ -221,8 +230,10 @@ static double y_diagonal(const og *g, int i, int j, const double *eta, const dou
 }
 static void diagonal_precondition(const og *g, double *zu, double *zv, const double *ru, const double *rv,
                                   const double *eta, const double *etaff, const double *bu, const double *bv, int walls) {
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= ny_of(g, C); j++)
         for (int i = 1; i <= nx_of(g, F); i++) AT(g, zu, i, j) = AT(g, ru, i, j) / x_diagonal(g, i, j, eta, etaff, bu, walls);
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= ny_of(g, F); j++)
         for (int i = 1; i <= nx_of(g, C); i++) AT(g, zv, i, j) = AT(g, rv, i, j) / y_diagonal(g, i, j, eta, etaff, bv, walls);
 }
@@ -233,6 +244,7 @@ static void diagonal_precondition(const og *g, double *zu, double *zv, const dou
 
 /* _linear_combination! over :xyz: z = a x + b y (velocity_vector.jl:54-62). */
 static void lincomb(const og *g, double *z, double a, const double *x, double b, const double *y) {
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= g->Ny; j++)
         for (int i = 1; i <= g->Nx; i++) AT(g, z, i, j) = a * AT(g, x, i, j) + b * AT(g, y, i, j);
 }
@@ -253,14 +265,34 @@ static double pairwise_sum(const double *w, long lo, long hi) {
     long mid = lo + ((hi - lo) >> 1);
     return pairwise_sum(w, lo, mid) + pairwise_sum(w, mid + 1, hi);
 }
+/* The same recursion with the halves of large ranges summed as parallel tasks (same split points, same result). */
+static double pairwise_sum_task(const double *w, long lo, long hi) {
+    if (hi - lo < (1L << 16)) return pairwise_sum(w, lo, hi);
+    long mid = lo + ((hi - lo) >> 1);
+    double a = 0.0, b = 0.0;
+#pragma omp task shared(a)
+    a = pairwise_sum_task(w, lo, mid);
+    b = pairwise_sum_task(w, mid + 1, hi);
+#pragma omp taskwait
+    return a + b;
+}
+static double pairwise_sum_par(const double *w, long lo, long hi) {
+    if (hi - lo < (1L << 16)) return pairwise_sum(w, lo, hi);
+    double r = 0.0;
+#pragma omp parallel
+#pragma omp single
+    r = pairwise_sum_task(w, lo, hi);
+    return r;
+}
 
 /* component_dot! (velocity_vector.jl:129-133): product over :xyz into a zeroed parent workspace, then the sum of
  * the whole parent array. */
 static double component_dot(const og *g, double *w, const double *x, const double *y) {
     memset(w, 0, g->plane * sizeof(double));
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= g->Ny; j++)
         for (int i = 1; i <= g->Nx; i++) AT(g, w, i, j) = AT(g, x, i, j) * AT(g, y, i, j);
-    return pairwise_sum(w, 0, (long)g->plane - 1);
+    return pairwise_sum_par(w, 0, (long)g->plane - 1);
 }
 static double vdot(const og *g, double *w, const double *xu, const double *xv, const double *yu, const double *yv) {
     return component_dot(g, w, xu, yu) + component_dot(g, w, xv, yv);
@@ -348,6 +380,7 @@ static void smooth(const og *g, double *xu, double *xv, const double *bu_, const
         ssa_operator(g, qu, qv, xu, xv, c.eta, c.etaff, c.bu, c.bv, walls);
         if (smoother == NK_POINT_JACOBI) {
             /* _smooth! (multigrid_preconditioner.jl:213-219) */
+            #pragma omp parallel for schedule(static) if (OMP_BIG(g))
             for (int j = 1; j <= g->Ny; j++)
                 for (int i = 1; i <= g->Nx; i++) {
                     AT(g, xu, i, j) += (AT(g, bu_, i, j) - AT(g, qu, i, j)) / x_abs_row_sum(g, i, j, c.eta, c.etaff, c.bu, walls);
@@ -355,6 +388,7 @@ static void smooth(const og *g, double *xu, double *xv, const double *bu_, const
                 }
         } else {
             /* _line_smooth! over :xz (multigrid_preconditioner.jl:255-259) */
+            #pragma omp parallel for schedule(static) if (OMP_BIG(g))
             for (int i = 1; i <= g->Nx; i++) {
                 line_relax(g, xu, bu_, qu, t, i, 0, c.eta, c.etaff, c.bu, walls);
                 line_relax(g, xv, bv_, qv, t, i, 1, c.eta, c.etaff, c.bv, walls);
@@ -435,6 +469,7 @@ static double prolongate_at(const og *fg, const og *cg, int i, int j, int lx, in
 
 /* _interpolate_to_corners! (src/StressBalances/stress_balance_coefficients.jl:80-83): ℑxyᶠᶠ = ℑy∘ℑx. */
 static void interpolate_to_corners(const og *g, double *etaff, const double *eta) {
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= ny_of(g, F); j++)
         for (int i = 1; i <= nx_of(g, F); i++)
             AT(g, etaff, i, j) = ((AT(g, eta, i - 1, j - 1) + AT(g, eta, i, j - 1)) / 2.0 +
@@ -450,6 +485,7 @@ static void coarsen_coefficients(nko_mg *mg, const og *fg, const double *eta, co
         nko_level *L = &mg->lev[l];
         const og *g = &L->g;
         int fx = L->cx ? 2 : 1, fy = L->cy ? 2 : 1;
+        #pragma omp parallel for schedule(static) if (OMP_BIG(g))
         for (int J = 1; J <= g->Ny; J++)
             for (int I = 1; I <= g->Nx; I++) {
                 int i = fx * (I - 1) + 1, j = fy * (J - 1) + 1;
@@ -472,6 +508,7 @@ static void coarsen_coefficients(nko_mg *mg, const og *fg, const double *eta, co
 /* _restrict! (multigrid_preconditioner.jl:346-352) */
 static void restrict_vec(const og *cg, const og *fg, int cx, int cy, double *bu, double *bv, const double *ru,
                          const double *rv, const double *ceta, const double *feta) {
+#pragma omp parallel for schedule(static) if (OMP_BIG(cg))
     for (int J = 1; J <= cg->Ny; J++)
         for (int I = 1; I <= cg->Nx; I++) {
             phi_fn pu = {0, ru, feta}, pv = {1, rv, feta};
@@ -483,6 +520,7 @@ static void restrict_vec(const og *cg, const og *fg, int cx, int cy, double *bu,
 /* _prolongate! (multigrid_preconditioner.jl:354-360) */
 static void prolongate_vec(const og *fg, const og *cg, int cx, int cy, double *xu, double *xv, const double *cu,
                            const double *cv, const double *feta, const double *ceta) {
+#pragma omp parallel for schedule(static) if (OMP_BIG(fg))
     for (int j = 1; j <= fg->Ny; j++)
         for (int i = 1; i <= fg->Nx; i++) {
             phi_fn pu = {0, cu, ceta}, pv = {1, cv, ceta};
@@ -632,6 +670,29 @@ EXPORT void nko_destroy(nko_model *m) {
 }
 
 EXPORT void nko_set_physics(nko_model *m, const nk_physics *ph) { m->ph = *ph; }
+
+/* New options on a live model; the multigrid hierarchy is rebuilt when the preconditioner needs one it lacks or
+ * its coarsening rule changed (StressBalanceMultigridPreconditioner(grid; ...) builds it per grid). */
+EXPORT void nko_set_options(nko_model *m, const nk_solver_options *op) {
+    int rebuild = op->preconditioner == NK_PRECONDITIONER_MULTIGRID &&
+                  (!m->mg || op->minimum_coarse_size != m->op.minimum_coarse_size);
+    m->op = *op;
+    if (rebuild) {
+        mg_destroy(m->mg);
+        m->mg = mg_create(&m->g, m->grid.halo, op->minimum_coarse_size);
+    }
+}
+
+/* Threads of the parallel row loops (the timed CPU baseline); returns the count in effect. */
+EXPORT int nko_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 EXPORT void nko_field_shape(const nko_model *m, int id, int32_t *shape) {
     shape[0] = nx_of(&m->g, LOCS[id].lx);
@@ -856,6 +917,7 @@ static void compute_viscosity(nko_model *m, const double *u, const double *v) {
     double n = m->ph.glen_exponent, e0 = m->ph.strain_rate_regularization;
     double *es = m->f[NK_FIELD_ETASIGMA];
     for (int k = 1; k <= m->Ns; k++)
+        #pragma omp parallel for schedule(static) if (OMP_BIG(g))
         for (int j = 1; j <= g->Ny; j++)
             for (int i = 1; i <= g->Nx; i++) {
                 double e2 = eff_strain2(g, i, j, u, v, m->ph.walls) + e0 * e0;
@@ -870,6 +932,7 @@ static void compute_shear_viscosity(nko_model *m, const double *u, const double 
     double *es = m->f[NK_FIELD_ETASIGMA];
     const double *bbu = m->f[NK_FIELD_BETABU], *bbv = m->f[NK_FIELD_BETABV], *ub = m->f[NK_FIELD_UB], *vb = m->f[NK_FIELD_VB];
     for (int k = 1; k <= m->Ns; k++)
+        #pragma omp parallel for schedule(static) if (OMP_BIG(g))
         for (int j = 1; j <= g->Ny; j++)
             for (int i = 1; i <= g->Nx; i++) {
                 double zeta = m->zc[k];
@@ -891,6 +954,7 @@ static void compute_column_integrals(nko_model *m) {
     const double *es = m->f[NK_FIELD_ETASIGMA], *H = m->f[NK_FIELD_H];
     const double *mu = m->ph.masking ? m->f[NK_FIELD_MU] : NULL;
     double *eta = m->f[NK_FIELD_ETAH], *F2 = m->f[NK_FIELD_F2];
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= g->Ny; j++)
         for (int i = 1; i <= g->Nx; i++) {
             double e = 0.0, Fs = 0.0;
@@ -921,6 +985,7 @@ static void compute_basal_drag(nko_model *m, double *bu, double *bv, const doubl
     const double *Cc = m->f[NK_FIELD_C], *N = m->f[NK_FIELD_N], *phiu = m->f[NK_FIELD_PHIU], *phiv = m->f[NK_FIELD_PHIV];
     int useN = ph->friction == NK_FRICTION_REGULARIZED_COULOMB;
     double u0 = ph->velocity_regularization;
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= ny_of(g, C); j++)
         for (int i = 1; i <= nx_of(g, F); i++) {
             double vm = ((AT(g, v, i - 1, j) + AT(g, v, i, j)) / 2.0 + (AT(g, v, i - 1, j + 1) + AT(g, v, i, j + 1)) / 2.0) / 2.0;
@@ -930,6 +995,7 @@ static void compute_basal_drag(nko_model *m, double *bu, double *bv, const doubl
             double beta = AT(g, phiu, i, j) * nko_basal_drag(U, Cf, Nf, ph->friction, ph->friction_exponent, ph->coulomb_coefficient);
             AT(g, bu, i, j) = periph(g, i, j, F, C) ? 0.0 : beta;
         }
+    #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= ny_of(g, F); j++)
         for (int i = 1; i <= nx_of(g, C); i++) {
             double um = ((AT(g, u, i, j - 1) + AT(g, u, i + 1, j - 1)) / 2.0 + (AT(g, u, i, j) + AT(g, u, i + 1, j)) / 2.0) / 2.0;
@@ -1167,6 +1233,13 @@ static void post_process(nko_model *m) {
             }
     }
     free(S);
+}
+
+/* compute_velocity_profile! / extend_depth_average! alone, after a picard_solve! (test/test_diva.jl:33-34) */
+EXPORT void nko_compute_velocity_profile(nko_model *m) {
+    fill_field_halo(m, NK_FIELD_U);
+    fill_field_halo(m, NK_FIELD_V);
+    post_process(m);
 }
 
 /* update_velocities! (src/Models/ice_sheet_model_methods.jl:119-147) */

@@ -38,13 +38,17 @@ def lib():
         L.nko_create.argtypes = [ct.POINTER(abi.Grid), ct.POINTER(abi.Physics), ct.POINTER(abi.SolverOptions)]
         L.nko_destroy.argtypes = [P]
         L.nko_set_physics.argtypes = [P, ct.POINTER(abi.Physics)]
+        L.nko_set_options.argtypes = [P, ct.POINTER(abi.SolverOptions)]
+        L.nko_set_threads.argtypes = [I]
+        L.nko_set_threads.restype = I
         L.nko_field_shape.argtypes = [P, I, ct.POINTER(ct.c_int32)]
         L.nko_set_field.argtypes = [P, I, dp, ct.c_int64, ct.c_int64]
         L.nko_get_field.argtypes = [P, I, dp, ct.c_int64, ct.c_int64]
         L.nko_get_classes.argtypes = [P, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64]
         for name in ["nko_update_geometry", "nko_compute_driving_stress", "nko_update_coefficients",
                      "nko_apply_operator", "nko_precondition", "nko_rate_factor_from_temperature",
-                     "nko_thermal_diagnostics", "nko_cache_thickness", "nko_compute_strain_heating"]:
+                     "nko_thermal_diagnostics", "nko_cache_thickness", "nko_compute_strain_heating",
+                     "nko_compute_velocity_profile"]:
             getattr(L, name).argtypes = [P]
         L.nko_solve.argtypes = [P, ct.POINTER(abi.Info)]
         L.nko_picard_solve.argtypes = [P, ct.POINTER(abi.Info)]
@@ -97,6 +101,10 @@ class OracleModel:
     def set_physics(self, physics):
         self.physics = physics
         lib().nko_set_physics(self._h, ct.byref(physics))
+
+    def set_options(self, options):
+        self.options = options
+        lib().nko_set_options(self._h, ct.byref(options))
 
     def shape(self, name):
         s = (ct.c_int32 * 3)()
@@ -176,6 +184,9 @@ class OracleModel:
         lib().nko_picard_solve(self._h, ct.byref(info))
         return info.as_dict()
 
+    def compute_velocity_profile(self):
+        lib().nko_compute_velocity_profile(self._h)
+
     def update_velocities(self, geometry=True):
         info = abi.Info()
         lib().nko_update_velocities(self._h, 1 if geometry else 0, ct.byref(info))
@@ -204,6 +215,11 @@ class OracleModel:
         zf, zc, dz, w2 = (np.zeros(Ns + 1), np.zeros(Ns), np.zeros(Ns), np.zeros(Ns))
         lib().nko_sigma_metrics(self._h, _dptr(zf), _dptr(zc), _dptr(dz), _dptr(w2))
         return zf, zc, dz, w2
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel row loops (0: leave as is); returns the count in effect."""
+    return lib().nko_set_threads(int(n))
 
 
 def ice_mask_value(H, hmin, htr):

@@ -301,3 +301,69 @@ def test_thermal_diagnostics_parity():
     a, b = h.get("A"), o.get("A")
     assert np.allclose(a, b, rtol=1e-13, atol=0)
     assert np.any(o.get("WATER_FRACTION") > 0)
+
+
+# ---- full-size configurations and the reference's published benchmark ----------------------------------------------
+
+def test_marine_sheet_640x80_full_size_parity():
+    """The grounded + floating marine sheet at its stated size (640 × 80 at 1 km, FP64 MG-PCG with the reference
+    defaults): ≤ 1e-9 relative L2 in (u, v), equal Picard counts, masks and classes bit for bit."""
+    say("marine sheet 640x80 at 1 km, FP64 MG")
+    h, o, ih, io = run_pair(*W.marine_sheet())
+    err = rel_l2([h.get("U"), h.get("V")], [o.get("U"), o.get("V")])
+    say(f"  picard {ih['picard_iterations']} vs {io['picard_iterations']}, cg {ih['total_linear_iterations']} vs "
+        f"{io['total_linear_iterations']}, rel L2 {err:.2e}")
+    assert ih["picard_iterations"] == io["picard_iterations"]
+    assert err <= 1e-9
+    for name in ("MU", "PHI", "PHIU", "PHIV", "S", "N", "TAUU", "TAUV"):
+        assert np.array_equal(h.get(name), o.get(name)), name
+    assert np.array_equal(h.classes()[0], o.classes()[0]) and np.array_equal(h.classes()[1], o.classes()[1])
+
+
+def test_diva_shelf_256x256_parity():
+    """The DIVA shelf (Arrhenius A on 10 σ levels, regularized Coulomb, no-slip walls) at 256² (2 km), FP64 MG-PCG:
+    ≤ 1e-9 relative L2 in (u, v) and in the layer velocities, equal Picard counts."""
+    say("DIVA shelf 256x256 at 2 km, FP64 MG")
+    g, phys, opts, f = W.diva_shelf(N=256, dx=2e3)
+    h, o, ih, io = run_pair(g, phys, opts, f)
+    err = rel_l2([h.get("U"), h.get("V")], [o.get("U"), o.get("V")])
+    say(f"  picard {ih['picard_iterations']} vs {io['picard_iterations']}, cg {ih['total_linear_iterations']} vs "
+        f"{io['total_linear_iterations']}, rel L2 {err:.2e}")
+    assert ih["picard_iterations"] == io["picard_iterations"]
+    assert err <= 1e-9
+    assert rel_l2([h.get("USIGMA"), h.get("VSIGMA")], [o.get("USIGMA"), o.get("VSIGMA")]) <= 1e-9
+
+
+def test_diva_slab_profile_on_device():
+    """test/test_diva.jl:48-58 on the device (Picard, then nk_compute_velocity_profile): profile < 1e-2, mean
+    < 1e-2, basal < 1e-6, v = 0, and second order in Nσ; the layer velocities also match the oracle's (≤ 1e-9)."""
+    from test_oracle_solvers import check_slab, slab_problem
+    say("DIVA slab, Nσ = 10 and 40")
+    coarse, _ = slab_problem(HIPModel, 10)
+    fine, mh = slab_problem(HIPModel, 40)
+    _, mo = slab_problem(O.OracleModel, 40)
+    say(f"  fine {fine}")
+    check_slab(coarse, fine)
+    assert rel_l2([mh.get("USIGMA")], [mo.get("USIGMA")]) <= 1e-9
+
+
+@pytest.mark.parametrize("dx", [8000.0, 4000.0, 2000.0, 1000.0, 500.0, 250.0])
+def test_mismip_scaling_iterations_on_device(dx):
+    """The reference's multigrid scaling benchmark (benchmarks/multigrid_scaling.jl) through the C ABI: FP64 Jacobi-
+    diagonal and multigrid PCG counts of the timed solve equal the published ones (docs/src/model/numerics.md:44-49)
+    at every resolution from 80 × 10 to 2560 × 320; up to 1 km the two solutions also match the oracle's ≤ 1e-9."""
+    g, p, o, f = W.mismip_scaling(dx)
+    say(f"MISMIP+ scaling {g.Nx}x{g.Ny}")
+    h = HIPModel(g, p, o)
+    W.load_into(h, f)
+    r = W.mismip_timed_solves(h)
+    got = (r["jacobi"][0], r["multigrid"][0])
+    say(f"  iterations {got} published {W.MISMIP_SCALING_ITERATIONS[dx]}; device ms jacobi "
+        f"{r['jacobi'][3]['elapsed_ms']:.2f} mg {r['multigrid'][3]['elapsed_ms']:.2f}")
+    assert got == W.MISMIP_SCALING_ITERATIONS[dx]
+    if dx >= 1000.0:
+        m = O.OracleModel(g, p, o)
+        W.load_into(m, f)
+        ro = W.mismip_timed_solves(m)
+        for name in ("jacobi", "multigrid"):
+            assert rel_l2(r[name][1:3], ro[name][1:3]) <= 1e-9, name

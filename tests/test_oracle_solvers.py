@@ -392,13 +392,16 @@ def test_diva_faster_than_ssa_with_sliding():
     assert np.abs(np.tensordot(dz, us, axes=(0, 0)) - ubar).max() < 1e-2 * np.abs(ubar).max()
 
 
-def slab_profile(Ns, H0=1000.0, tau0=1e4, A=1e-21, Cc=1e10):
-    # test/test_diva.jl:119-160
+def slab_problem(make_model, Ns, H0=1000.0, tau0=1e4, A=1e-21, Cc=1e10):
+    """slab_profile_error of test/test_diva.jl:5-48: a uniform slab on a 4 × 4 doubly periodic grid under a uniform
+    driving stress with linear sliding, Picard to 1e-12, then compute_velocity_profile!. Returns the reference's
+    error measures (iterations, profile, mean, basal, v) and the model."""
     g = grid(4, 4, Lx=4e3, Ly=4e3, topology=(P, P), Nsigma=Ns)
     phys = abi.default_physics(masking=0, strain_rate_regularization=1e-16, friction=abi.FRICTION_WEERTMAN,
                                friction_exponent=1.0, stress_balance=abi.DIVA)
-    m = model(g, phys, linear=abi.convergence(1e-12, 0.0, 100), picard=abi.convergence(1e-12, 0.0, 300),
-              preconditioner=abi.PRECONDITIONER_NONE)
+    opts = abi.default_options(linear=abi.convergence(1e-12, 0.0, 100), picard=abi.convergence(1e-12, 0.0, 300),
+                               preconditioner=abi.PRECONDITIONER_NONE)
+    m = make_model(g, phys, opts)
     m.set("H", H0)
     m.set("PHIU", 1.0)
     m.set("PHIV", 1.0)
@@ -409,24 +412,57 @@ def slab_profile(Ns, H0=1000.0, tau0=1e4, A=1e-21, Cc=1e10):
     m.set("U", 0.0)
     m.set("V", 0.0)
     info = m.picard_solve()
-    # compute_velocity_profile! = post-processing of update_velocities!; run it through the same code path by
-    # setting the driving stress again (driving stress with masking off and S = 0 is zero, so restore τ after)
-    return m, info
+    m.compute_velocity_profile()
+    n = 3.0
+    ub = tau0 / Cc
+    U = 2 * A * tau0 ** n * H0 / (n + 1)
+    zc = (m.sigma_metrics()[1] if hasattr(m, "sigma_metrics") else None)
+    if zc is None:
+        zf = (np.arange(Ns + 1) / Ns) ** 2
+        zc = (zf[:-1] + zf[1:]) / 2
+    exact = ub + U * (1 - (1 - zc) ** (n + 1))
+    us = m.get("USIGMA")
+    ubar_e = ub + 2 * A * tau0 ** n * H0 / (n + 2)
+    err = {"iterations": info["picard_iterations"],
+           "profile": float(np.max(np.abs(us - exact[:, None, None])) / U),
+           "mean": float(np.max(np.abs(m.get("U") - ubar_e)) / (ubar_e - ub)),
+           "basal": float(np.max(np.abs(m.get("UB") - ub)) / ub),
+           "v": float(np.max(np.abs(m.get("VSIGMA"))))}
+    return err, m
+
+
+def check_slab(coarse, fine):
+    """The assertions of test/test_diva.jl:50-57."""
+    assert 0 < fine["iterations"] < 300
+    assert fine["profile"] < 1e-2
+    assert fine["mean"] < 1e-2
+    assert fine["basal"] < 1e-6
+    assert fine["v"] == 0
+    assert fine["profile"] < coarse["profile"] / 4
 
 
 def test_diva_slab_profile():
-    res = {}
-    for Ns in (10, 40):
-        m, info = slab_profile(Ns)
-        res[Ns] = (m, info)
-    n, tau0, A, H0, Cc = 3.0, 1e4, 1e-21, 1000.0, 1e10
-    ubase = tau0 / Cc
-    U = 2 * A * tau0 ** n * H0 / (n + 1)
-    m, info = res[40]
-    assert 0 < info["picard_iterations"] < 300
-    ubar_e = ubase + 2 * A * tau0 ** n * H0 / (n + 2)
-    assert np.abs(m.get("U") - ubar_e).max() / (ubar_e - ubase) < 1e-2
-    assert np.all(m.get("V") == 0)
+    coarse, _ = slab_problem(O.OracleModel, 10)
+    fine, _ = slab_problem(O.OracleModel, 40)
+    check_slab(coarse, fine)
+
+
+# ---- the reference's multigrid scaling benchmark (benchmarks/multigrid_scaling.jl, docs/src/model/numerics.md) --
+
+@pytest.mark.parametrize("dx", [8000.0, 4000.0, 2000.0, 1000.0])
+def test_mismip_scaling_iterations_equal_published(dx):
+    """The published CG iteration counts of one solve from zero to 1e-8 on the MISMIP+ ice stream, after
+    FixedIterations(5) Picard with the multigrid PCG at 1e-6 (docs/src/model/numerics.md:44-49): the diagonal
+    and the multigrid preconditioner, iteration for iteration. The two solutions agree as the benchmark reports
+    (max |Δu| / max |u| of the two solves, both at 1e-8)."""
+    from nunatak_hip import workloads as W
+    g, p, o, f = W.mismip_scaling(dx)
+    m = O.OracleModel(g, p, o)
+    W.load_into(m, f)
+    r = W.mismip_timed_solves(m)
+    assert (r["jacobi"][0], r["multigrid"][0]) == W.MISMIP_SCALING_ITERATIONS[dx]
+    uj, um = r["jacobi"][1], r["multigrid"][1]
+    assert np.max(np.abs(uj - um)) / np.max(np.abs(uj)) < 1e-6
 
 
 # ---- multigrid: test/test_multigrid_preconditioner.jl ---------------------------------------------------------------
