@@ -163,12 +163,23 @@ def serving_request(N, seed, dx=1e3, precision=abi.PRECISION_MIXED, linear_relto
     return g, phys, opts, fields
 
 
-def serving_stream(n=2000, rate_hz=4.0, seed=5, smallest=64, largest=1024, precision=abi.PRECISION_MIXED):
-    """Config 5: n requests with Poisson arrivals at rate_hz; side = 16·round(exp(U[ln 64, ln 1024]) / 16); request
-    k is serving_request(side, seed·100003 + k). Returns [(arrival_s, grid, physics, options, fields), ...]."""
+def serving_schedule(n=2000, rate_hz=4.0, seed=5, smallest=64, largest=1024):
+    """(arrival_s, side) arrays of the serving stream without building its requests."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sides = 16 * np.round(np.exp(rng.uniform(np.log(smallest), np.log(largest), size=n)) / 16).astype(int)
     arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, size=n))
+    return arrivals, sides
+
+
+def serving_stream_request(k, sides, seed=5, precision=abi.PRECISION_MIXED):
+    """Request k of the serving stream whose sides serving_schedule drew."""
+    return serving_request(int(sides[k]), seed * 100003 + k, precision=precision)
+
+
+def serving_stream(n=2000, rate_hz=4.0, seed=5, smallest=64, largest=1024, precision=abi.PRECISION_MIXED):
+    """Config 5: n requests with Poisson arrivals at rate_hz; side = 16·round(exp(U[ln 64, ln 1024]) / 16); request
+    k is serving_request(side, seed·100003 + k). Returns [(arrival_s, grid, physics, options, fields), ...]."""
+    arrivals, sides = serving_schedule(n, rate_hz, seed, smallest, largest)
     out = []
     for k in range(n):
         g, p, o, f = serving_request(int(sides[k]), seed * 100003 + k, precision=precision)

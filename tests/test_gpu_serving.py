@@ -108,3 +108,23 @@ def test_other_request_kinds_same_alone_and_in_the_mix(kind):
     out = serve([(0.0, *r) for r in reqs], slots=3, keep_results=True)
     for k, (u, v) in enumerate(out["results"]):
         assert np.array_equal(u, ref[k][0]) and np.array_equal(v, ref[k][1]), f"request {k} differs"
+
+
+def test_stream_requests_with_large_grids_same_alone_and_in_the_mix():
+    """Requests of the serving stream itself (seed 5): its first 12 — sides 592, 608, 272, 144, 80, 192, 192, 80, 80,
+    1024, 384, 128 — and request 174 (side 512), all arriving at once on 4 slots, so the 1024² and 512² solves overlap
+    the small ones. Each result equals the same request solved alone on a fresh handle, bit for bit."""
+    arrivals, sides = W.serving_schedule(2000)
+    picks = list(range(12)) + [174]
+    assert 1024 in sides[picks] and 512 in sides[picks]
+    reqs = [W.serving_stream_request(k, sides) for k in picks]
+    say(f"[serve] stream requests {picks}, sides {[int(sides[k]) for k in picks]}: alone")
+    ref = [alone(*r) for r in reqs]
+    say("[serve] the same in the mix on 4 slots")
+    out = serve([(0.0, *r) for r in reqs], slots=4, keep_results=True)
+    overlap = sum(1 for a in out["records"] for b in out["records"]
+                  if a is not b and a["start_ms"] < b["finish_ms"] and b["start_ms"] < a["finish_ms"])
+    say(f"[serve] overlapping pairs {overlap}, p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms")
+    assert overlap >= 1
+    for k, ((u, v), (ur, vr)) in enumerate(zip(out["results"], ref)):
+        assert np.array_equal(u, ur) and np.array_equal(v, vr), f"stream request {picks[k]} (side {sides[picks[k]]}) differs"
