@@ -239,7 +239,7 @@ void nk_default_options(nk_solver_options *options);
 nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_solver_options *options,
                     nk_solver **out);
 nk_status nk_destroy(nk_solver *solver);
-const char *nk_last_error(const nk_solver *solver);
+const char *nk_last_error(const nk_solver *solver);   /* NULL: the last handle-free call's error on this thread */
 
 /* Change physics or options of an existing handle (same grid). */
 nk_status nk_set_physics(nk_solver *solver, const nk_physics *physics);
@@ -268,6 +268,23 @@ nk_status nk_compute_driving_stress(nk_solver *solver);
 nk_status nk_update_coefficients(nk_solver *solver);
 /* ssa_operator!(q, p, η̄H, η̄Hᶠᶠ, βᵘ, βᵛ, walls): (QU, QV) = A (PU, PV). */
 nk_status nk_apply_operator(nk_solver *solver);
+
+/* Allocation-free variant of nk_apply_operator. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * q = A p (ssa_operator!, src/StressBalances/ssa_operator.jl:81-89) on caller-owned device arrays in the library's
+ * plane layout: element (i, j) of every field at (j − 1)·pitch + (i − 1), pitch = Nx + 1 rounded up to 64 elements,
+ * (Ny + 1) rows (nk_field_device_pointer of a handle with the same grid returns such planes). eta: η̄H at centres,
+ * etaff: η̄Hᶠᶠ at corners, betau/betav: β at u/v points, (pu, pv) in, (qu, qv) out over the Nx × Ny :xyz extent;
+ * walls is NK_FREE_SLIP or NK_NO_SLIP. The activity flags and neighbour masks live in one fixed device array of
+ * NK_NOALLOC_SCRATCH_BYTES per GPU (3 bytes per plane point: grids up to about 2048² cells); larger grids return
+ * NK_INVALID_ARGUMENT. The call makes no allocation of any kind and does not synchronize: it enqueues three kernels
+ * on `stream` (a hipStream_t; NULL = the null stream) of the current device. Calls that overlap on one GPU share the
+ * array and race; serialize them (one stream, or events). Errors are read with nk_last_error(NULL) on the calling
+ * thread. Results are nk_apply_operator's bit for bit. */
+#define NK_NOALLOC_SCRATCH_BYTES (16u << 20)
+nk_status nk_apply_operator_noalloc(const nk_grid *grid, int32_t walls, const double *eta, const double *etaff,
+                                    const double *betau, const double *betav, const double *pu, const double *pv,
+                                    double *qu, double *qv, int64_t pitch, void *stream);
 /* precondition!(z, P, r, coefficients...): (QU, QV) = M⁻¹ (PU, PV) with the handle's preconditioner; the multigrid
  * coarse coefficients are rebuilt first (update_preconditioner!). */
 nk_status nk_precondition(nk_solver *solver);

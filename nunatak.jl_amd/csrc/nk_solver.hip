@@ -165,6 +165,9 @@ namespace {
 // Serialises graph capture against device allocation and release across the handles of one process.
 std::mutex g_device_mu;
 
+// Message of the last failed handle-free call on this thread (nk_last_error(NULL)).
+thread_local std::string g_free_err = "null handle";
+
 KG make_kg(int Nx, int Ny, int tx, int ty, double dx, double dy) {
     KG g;
     g.Nx = Nx; g.Ny = Ny; g.tx = tx; g.ty = ty; g.dx = dx; g.dy = dy;
@@ -1546,7 +1549,7 @@ nk_status nk_destroy(nk_solver *s) {
     return NK_OK;
 }
 
-const char *nk_last_error(const nk_solver *s) { return s ? s->err.c_str() : "null handle"; }
+const char *nk_last_error(const nk_solver *s) { return s ? s->err.c_str() : g_free_err.c_str(); }
 
 nk_status nk_set_physics(nk_solver *s, const nk_physics *physics) {
     if (!s || !physics) return NK_INVALID_ARGUMENT;
@@ -2148,6 +2151,88 @@ nk_status nk_multigrid_levels(const nk_solver *s, int32_t *levels, int32_t *shap
         shapes[2 * l] = s->lev[l + 1].g.Nx;
         shapes[2 * l + 1] = s->lev[l + 1].g.Ny;
     }
+    return NK_OK;
+}
+
+// ---- allocation-free operator apply ------------------------------------------------------------------------------
+// q = A p on caller buffers with the activity flags and neighbour masks in a fixed device array instead of a handle's
+// planes. The array is per device (each GPU the library runs on holds its own copy until the process ends); its device
+// address is taken once per device with hipGetSymbolAddress and only that address is used on the host side.
+
+}  // extern "C"
+
+namespace {
+
+static __device__ __align__(256) uint8_t scratch[NK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+nk_status free_fail(nk_status st, const std::string &msg) {
+    g_free_err = msg;
+    return st;
+}
+
+nk_status scratch_for_current_device(void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return free_fail(NK_DEVICE_ERROR, "nk_apply_operator_noalloc: no current HIP device");
+    if (dev >= NOALLOC_MAX_DEVICES) return free_fail(NK_UNSUPPORTED, "nk_apply_operator_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return free_fail(NK_DEVICE_ERROR, std::string("nk_apply_operator_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return NK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+nk_status nk_apply_operator_noalloc(const nk_grid *grid, int32_t walls, const double *eta, const double *etaff,
+                                    const double *betau, const double *betav, const double *pu, const double *pv, double *qu,
+                                    double *qv, int64_t pitch, void *stream) {
+    if (!grid || !eta || !etaff || !betau || !betav || !pu || !pv || !qu || !qv)
+        return free_fail(NK_INVALID_ARGUMENT, "nk_apply_operator_noalloc: null argument");
+    if (grid->Nx < 1 || grid->Ny < 1 || !(grid->dx > 0) || !(grid->dy > 0) ||
+        (grid->topology_x != NK_PERIODIC && grid->topology_x != NK_BOUNDED) ||
+        (grid->topology_y != NK_PERIODIC && grid->topology_y != NK_BOUNDED))
+        return free_fail(NK_INVALID_ARGUMENT, "nk_apply_operator_noalloc: grid sizes, spacings or topologies invalid");
+    if (walls != NK_FREE_SLIP && walls != NK_NO_SLIP)
+        return free_fail(NK_INVALID_ARGUMENT, "nk_apply_operator_noalloc: walls must be NK_FREE_SLIP or NK_NO_SLIP");
+    const KG g = make_kg(grid->Nx, grid->Ny, grid->topology_x, grid->topology_y, grid->dx, grid->dy);
+    if (pitch != g.pitch)
+        return free_fail(NK_INVALID_ARGUMENT, "nk_apply_operator_noalloc: pitch must be Nx + 1 rounded up to 64 elements (" +
+                                                  std::to_string(g.pitch) + " here)");
+    const size_t plane = (size_t)plane_of(g);
+    const size_t nm_off = (plane + 255) / 256 * 256;
+    const size_t need = nm_off + 2 * plane;
+    if (need > (size_t)NK_NOALLOC_SCRATCH_BYTES)
+        return free_fail(NK_INVALID_ARGUMENT, "nk_apply_operator_noalloc: grid too large for the fixed scratch array (needs " +
+                                                  std::to_string(need) + " bytes, NK_NOALLOC_SCRATCH_BYTES = " +
+                                                  std::to_string((size_t)NK_NOALLOC_SCRATCH_BYTES) + ")");
+    void *base = nullptr;
+    nk_status st = scratch_for_current_device(&base);
+    if (st != NK_OK) return st;
+    uint8_t *act = (uint8_t *)base;
+    uint16_t *nm = (uint16_t *)((uint8_t *)base + nm_off);
+    hipStream_t hs = (hipStream_t)stream;
+    // every flag and mask word the operator reads is rewritten here, so the previous call's contents never leak in
+    k_corners_flags<<<blocks_for(g.Nx + 1, g.Ny + 1), THREADS, 0, hs>>>(g, eta, nullptr, act);
+    NK_LAUNCH_TOPO2(g, k_nmask, blocks_for(g.Nx, g.Ny), THREADS, 0, hs, g, act, nm);
+    Coef<double> c;
+    c.eta = eta; c.etaff = etaff; c.bu = betau; c.bv = betav; c.act = act;
+    Red r;
+    r.partials = nullptr; r.S = nullptr; r.fin = FIN_NONE;
+    NK_LAUNCH_TOPO(g, k_apply_mt, double, blocks_for(g.Nx, g.Ny), THREADS, 0, hs, g, pu, pv, c, (const uint16_t *)nm,
+                   walls == NK_NO_SLIP ? 2.0 : 0.0, qu, qv, (const double *)nullptr, (const double *)nullptr, r, 0, (const Scal *)nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return free_fail(NK_DEVICE_ERROR, std::string("nk_apply_operator_noalloc: launch: ") + hipGetErrorString(e));
     return NK_OK;
 }
 

@@ -26,7 +26,7 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_till_effective_pressure",
            "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint", "nk_device_bytes",
            "nk_estimate_device_bytes", "nk_server_create", "nk_server_destroy", "nk_server_last_error",
-           "nk_server_submit", "nk_server_poll", "nk_server_wait"]
+           "nk_server_submit", "nk_server_poll", "nk_server_wait", "nk_apply_operator_noalloc"]
 
 _lib = None
 
@@ -94,6 +94,7 @@ def load(path=LIB_PATH):
                                    ct.POINTER(I32), ct.POINTER(dp), ct.POINTER(I64)]
     L.nk_server_poll.argtypes = [P, I64, ct.POINTER(I32)]
     L.nk_server_wait.argtypes = [P, I64, dp, dp, ct.POINTER(abi.RequestInfo)]
+    L.nk_apply_operator_noalloc.argtypes = [ct.POINTER(abi.Grid), I32] + [P] * 8 + [I64, P]
     for name in EXPORTS:
         if name not in ("nk_last_error", "nk_stream", "nk_abi_version", "nk_default_grid", "nk_default_physics",
                         "nk_default_options", "nk_server_last_error"):
