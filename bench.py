@@ -3,20 +3,26 @@
 A step is one full velocity solve (update_velocities!: driving stress, Picard with DIVA coefficient updates and
 multigrid-preconditioned CG, velocity-profile reconstruction) of the 1024 × 1024 floating-channel workload
 (the DIVA shelf, 512 km square at dx = 500 m), from u = v = 0 and zero DIVA state, with every input already in HBM.
-`value` is the mixed-precision form's solves/s; the FP64 form's solves/s and the mixed result's relative L2 error
-against it are reported beside it.
+`value` is the mixed-precision form's solves/s (FP32 V-cycle with Galerkin coarse operators inside FP64 PCG). Beside
+it: the FP64 form (the reference algorithm: rediscretized coarse levels) and FP64 with the same Galerkin coarse
+operators as the mixed form, so mixed ÷ FP64-Galerkin is the precision factor alone and FP64-Galerkin ÷ FP64 the
+preconditioner's; each result's relative L2 error against the FP64 solve.
 
-Roofline: the three HBM-bound operator/smoother kernels of the headline solve — the finest-grid l1-Jacobi sweep
-(k_sweep_m), the PCG operator (k_apply_mt<double>) and the Galerkin stencil sweep on the first coarse level
-(k_gp_sweep_t)
-— each timed with HIP events on the solver's own stream (nk_solver_options.profile_kernel) during one extra solve;
-`roofline` is the one with the most device time per solve, the others are listed under roofline.kernels with their
-algorithmic bytes per launch. (The LDS-resident coarse tail, k_g_tail, is latency-bound on one workgroup and has no
-HBM roofline.)
+Roofline: the HBM-bound kernels of the headline solve with ≥ 5 % of its device time — the PCG operator
+(k_apply_mt<double>), the PCG update (k_update_xr<double>), the finest-grid l1-Jacobi sweep (k_sweep_m) and the
+Galerkin stencil sweep on the first coarse level (k_gp_sweep_t) — each timed with HIP events on the solver's own
+stream (nk_solver_options.profile_kernel) during one extra solve; `roofline` is the one with the most device time per
+solve, the others are listed under roofline.kernels with their algorithmic bytes per launch. (The LDS-resident coarse
+tail, k_g_tail, and the one-block finishers are latency-bound and have no HBM roofline.)
 
-CPU baseline: the CPU oracle (plain C, one thread) timed on a bounded sample of the same workload — one DIVA
-coefficient update and three MG-PCG iterations at 1024² — and scaled to one solve with the FP64 GPU run's Picard
-and CG counts.
+CPU baseline: the CPU oracle (the reference algorithm in C, OpenMP over rows) on every host thread this process may
+use. `cpu_baseline` is the 1024² workload extrapolated from a bounded sample (one DIVA coefficient update and three
+MG-PCG iterations, scaled with the FP64 GPU run's Picard and CG counts); `cpu_full_solves` times complete solves on all
+threads and on one: the marine sheet (640 × 80, full Picard) and the reference's MISMIP+ benchmark solves at 1 km,
+beside the reference's published one-thread times.
+
+MISMIP+ scaling (`mismip_scaling`): the reference's only published benchmark (benchmarks/multigrid_scaling.jl) on the
+device at 8 km to 0.25 km, iterations and times beside the published H200 ones.
 
 Serving (bounded sample): --serve requests of the seeded serving stream (sides 64–1024) replayed at their Poisson arrival times
 through nk_server (--serve-slots workers); p50/p99 of finish − arrival and solves/s over the run, beside serial FP64
@@ -83,10 +89,26 @@ def timed_solves(m, warmup, steps, label):
     return wall, infos
 
 
-def cpu_baseline(setup, fp64_info):
-    """Oracle (C, 1 thread) on a bounded sample of the workload, scaled to one solve."""
+def host_cores():
+    """(threads this process may run on, CPU model) of the host."""
+    n = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline(setup, fp64_info, threads):
+    """Oracle (C, OpenMP over rows on `threads` host threads) on a bounded sample of the 1024² workload, scaled to one
+    solve with the FP64 GPU run's trip counts (extrapolated: a complete 1024² solve takes hours on the host)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    oracle.set_threads(threads)
 
     g, phys, opts, fields = setup
     o_opts = abi.default_options(preconditioner=abi.PRECONDITIONER_MULTIGRID, linear=abi.fixed_iterations(3))
@@ -113,12 +135,101 @@ def cpu_baseline(setup, fp64_info):
     picard = fp64_info["picard_iterations"]
     cg = fp64_info["total_linear_iterations"]
     t_solve = picard * (t_coef + t_setup) + cg * t_iter
-    log(f"[bench] cpu oracle: coefficient update {t_coef:.2f} s, solve setup {t_setup:.2f} s, CG iteration {t_iter:.2f} s "
-        f"→ {t_solve:.1f} s per solve ({picard} Picard, {cg} CG iterations)")
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle at 1024²: one DIVA coefficient update ({t_coef:.2f} s), MG-PCG setup ({t_setup:.2f} s) and "
-                      f"3 MG-PCG iterations ({t_iter:.2f} s each), scaled by the FP64 GPU solve's {picard} Picard and "
-                      f"{cg} CG iterations"}
+    log(f"[bench] cpu oracle ({threads} threads): coefficient update {t_coef:.2f} s, solve setup {t_setup:.2f} s, CG iteration "
+        f"{t_iter:.3f} s → {t_solve:.1f} s per solve ({picard} Picard, {cg} CG iterations; extrapolated)")
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"extrapolated: CPU oracle at 1024² on {threads} threads, one DIVA coefficient update ({t_coef:.2f} s), "
+                      f"MG-PCG setup ({t_setup:.2f} s) and 3 MG-PCG iterations ({t_iter:.3f} s each), scaled by the FP64 "
+                      f"GPU solve's {picard} Picard and {cg} CG iterations; complete solves timed in cpu_full_solves"}
+
+
+def cpu_full_solves(threads_all):
+    """Complete solves of the oracle (the reference algorithm in C) on the host, all visible threads and one:
+    the marine sheet (640 × 80 at 1 km, FP64 MG-PCG, full Picard) and the timed solves of the reference's
+    multigrid scaling benchmark at 1 km (MISMIP+, one SSA solve from zero to 1e-8 after FixedIterations(5) Picard;
+    benchmarks/multigrid_scaling.jl), beside the reference's published one-thread EPYC 9745 times."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    out = {}
+    for threads in sorted({threads_all, 1}, reverse=True):
+        oracle.set_threads(threads)
+        g, p, o, f = workloads.marine_sheet()
+        m = oracle.OracleModel(g, p, o)
+        workloads.load_into(m, f)
+        t0 = time.perf_counter()
+        info = m.update_velocities()
+        t_marine = time.perf_counter() - t0
+        g, p, o, f = workloads.mismip_scaling(1000.0)
+        m = oracle.OracleModel(g, p, o)
+        workloads.load_into(m, f)
+        m.update_velocities()
+        times = {}
+        for name, pre in (("jacobi", abi.PRECONDITIONER_DIAGONAL), ("multigrid", abi.PRECONDITIONER_MULTIGRID)):
+            oo = abi.default_options(linear=abi.convergence(1e-8, 0.0, 100_000), picard=o.picard, preconditioner=pre)
+            m.set_options(oo)
+            m.set("U", 0.0)
+            m.set("V", 0.0)
+            t0 = time.perf_counter()
+            it = m.solve()["linear_iterations"]
+            times[name] = (time.perf_counter() - t0, it)
+        log(f"[bench] cpu oracle, {threads} threads: marine sheet 640x80 {t_marine:.2f} s ({info['picard_iterations']} Picard, "
+            f"{info['total_linear_iterations']} CG); MISMIP+ 1 km solve Jacobi {times['jacobi'][0]:.3f} s ({times['jacobi'][1]} it), "
+            f"multigrid {times['multigrid'][0]:.3f} s ({times['multigrid'][1]} it)")
+        out[f"threads_{threads}"] = {"marine_sheet_640x80_s": t_marine, "marine_picard": info["picard_iterations"],
+                                     "marine_cg": info["total_linear_iterations"],
+                                     "mismip_1km_jacobi_s": times["jacobi"][0], "mismip_1km_jacobi_it": times["jacobi"][1],
+                                     "mismip_1km_multigrid_s": times["multigrid"][0],
+                                     "mismip_1km_multigrid_it": times["multigrid"][1]}
+    out["published_epyc9745_1thread_mismip_1km_s"] = {"jacobi": 0.22, "multigrid": 0.10}
+    t1 = out["threads_1"]
+    # the oracle is a checker written for clarity (every halo, mask and division as the reference states it), not a
+    # tuned CPU solver: its one-thread time over the reference's published one is stated so no ratio is overread
+    out["oracle_over_published_1thread"] = {"jacobi": t1["mismip_1km_jacobi_s"] / 0.22, "multigrid": t1["mismip_1km_multigrid_s"] / 0.10}
+    return out
+
+
+def mismip_leg(resolutions):
+    """The reference's only published benchmark (benchmarks/multigrid_scaling.jl; table at docs/src/model/numerics.md:
+    42-49) on the HIP path: per resolution, update_velocities! with FixedIterations(5) Picard (as set! runs it), then
+    one solve from zero to 1e-8 with each preconditioner, timed on the host clock around the second of two solves (the
+    reference times the second solve! too), beside the published iterations and H200 times. FP64 is the reference
+    algorithm; the mixed form (FP32 Galerkin V-cycle) is listed beside it."""
+    rows = []
+    for dx in resolutions:
+        row = {"dx_m": dx}
+        for form, prec, coarse in (("fp64", abi.PRECISION_FP64, abi.COARSE_REDISCRETIZED),
+                                   ("mixed", abi.PRECISION_MIXED, abi.COARSE_GALERKIN)):
+            g, p, o, f = workloads.mismip_scaling(dx, precision=prec, coarse_operator=coarse)
+            m = HIPModel(g, p, o)
+            workloads.load_into(m, f)
+            m.update_velocities()
+            pres = (("multigrid", abi.PRECONDITIONER_MULTIGRID),) if form == "mixed" else \
+                (("jacobi", abi.PRECONDITIONER_DIAGONAL), ("multigrid", abi.PRECONDITIONER_MULTIGRID))
+            for name, pre in pres:
+                oo = abi.default_options(**{k: getattr(o, k) for k, _ in o._fields_})
+                oo.linear, oo.preconditioner = abi.convergence(1e-8, 0.0, 100_000), pre
+                if pre != abi.PRECONDITIONER_MULTIGRID:
+                    oo.precision, oo.coarse_operator = abi.PRECISION_FP64, abi.COARSE_REDISCRETIZED
+                m.set_options(oo)
+                for rep in range(2):
+                    m.set("U", 0.0)
+                    m.set("V", 0.0)
+                    t0 = time.perf_counter()
+                    info = m.solve()
+                    wall = time.perf_counter() - t0
+                row[f"{form}_{name}_it"] = info["linear_iterations"]
+                row[f"{form}_{name}_s"] = wall
+            row["grid"] = [g.Nx, g.Ny]
+            m.close()
+        pub_it, pub_t = workloads.MISMIP_SCALING_ITERATIONS[dx], workloads.MISMIP_SCALING_TIMES[dx]
+        row["published_it"] = {"jacobi": pub_it[0], "multigrid": pub_it[1]}
+        row["published_h200_s"] = {"jacobi": pub_t[2], "multigrid": pub_t[3]}
+        row["published_cpu_1thread_s"] = {"jacobi": pub_t[0], "multigrid": pub_t[1]}
+        log(f"[bench] MISMIP+ {row['grid'][0]}x{row['grid'][1]}: FP64 Jacobi {row['fp64_jacobi_it']} it {1e3 * row['fp64_jacobi_s']:.1f} ms, "
+            f"multigrid {row['fp64_multigrid_it']} it {1e3 * row['fp64_multigrid_s']:.1f} ms (published {pub_it}, H200 "
+            f"{pub_t[2]} / {pub_t[3]} s); mixed multigrid {row['mixed_multigrid_it']} it {1e3 * row['mixed_multigrid_s']:.1f} ms")
+        rows.append(row)
+    return rows
 
 
 def serving_leg(n, rate_hz, slots, serial_sample):
@@ -247,7 +358,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--no-cpu-full", action="store_true", help="skip the complete CPU solves (marine sheet, MISMIP+ 1 km)")
+    ap.add_argument("--no-fp64-galerkin", action="store_true", help="skip the FP64 solve with Galerkin coarse operators")
+    ap.add_argument("--mismip", default="8000,4000,2000,1000,500,250",
+                    help="resolutions (m) of the reference's multigrid scaling benchmark run on the device ('' to skip)")
+    ap.add_argument("--only", choices=["fp64", "fp64_galerkin", "mixed"], default=None)
     ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
     ap.add_argument("--graphs", type=int, default=1, help="0: launch kernel by kernel (for external profilers)")
     ap.add_argument("--no-roofline", action="store_true")
@@ -315,12 +430,17 @@ def main():
     log(f"[bench] building FP64 and mixed handles at {N}x{N}")
     results = {}
     models = {}
-    for label, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
+    setups = {}
+    legs = [("fp64", abi.PRECISION_FP64), ("fp64_galerkin", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)]
+    for label, prec in legs:
         if args.only and args.only != label:
             continue
-        coarse = abi.COARSE_GALERKIN if (label == "mixed" and args.coarse == "galerkin") else abi.COARSE_REDISCRETIZED
+        if label == "fp64_galerkin" and (args.no_fp64_galerkin or args.coarse != "galerkin"):
+            continue
+        coarse = abi.COARSE_GALERKIN if (label != "fp64" and args.coarse == "galerkin") else abi.COARSE_REDISCRETIZED
         m, setup = make(N, prec, picard=args.picard, graphs=args.graphs, coarse=coarse)
         models[label] = (m, setup)
+        setups[label] = setup
         wall, infos = timed_solves(m, Wm, K, label)
         results[label] = {"wall": wall, "infos": infos, "u": m.get("U"), "v": m.get("V")}
 
@@ -332,7 +452,7 @@ def main():
         err = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     else:
         err = None
-    main_label = "mixed" if "mixed" in results else "fp64"
+    main_label = "mixed" if "mixed" in results else ("fp64" if "fp64" in results else "fp64_galerkin")
     r = results[main_label]
     out["value"] = K / r["wall"]
     out["ms_per_step"] = 1e3 * r["wall"] / K
@@ -353,6 +473,20 @@ def main():
     if err is not None:
         out["mixed_rel_l2_err_vs_fp64"] = err
         out["mixed_speedup_vs_fp64"] = out["value"] / out["fp64_solves_per_s"]
+    if "fp64_galerkin" in results:
+        fg = results["fp64_galerkin"]
+        out["fp64_galerkin_solves_per_s"] = K / fg["wall"]
+        out["fp64_galerkin_ms_per_solve"] = 1e3 * fg["wall"] / K
+        out["fp64_galerkin_picard_iterations"] = fg["infos"][-1]["picard_iterations"]
+        out["fp64_galerkin_cg_iterations"] = fg["infos"][-1]["total_linear_iterations"]
+        if "fp64" in results:
+            a = np.concatenate([fg["u"].ravel(), fg["v"].ravel()])
+            b = np.concatenate([results["fp64"]["u"].ravel(), results["fp64"]["v"].ravel()])
+            out["fp64_galerkin_rel_l2_err_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            out["fp64_galerkin_speedup_vs_fp64"] = out["fp64_galerkin_solves_per_s"] / out["fp64_solves_per_s"]
+        if "mixed" in results:
+            # the precision factor alone: the same Galerkin hierarchy and trip structure, FP32 against FP64 V-cycle
+            out["mixed_speedup_vs_fp64_galerkin"] = out["value"] / out["fp64_galerkin_solves_per_s"]
 
     # roofline: one more solve per kernel with HIP-event brackets on the solver's stream, for the three HBM-bound
     # operator/smoother kernels of the headline solve; the one with the largest device time per solve is `roofline`
@@ -368,6 +502,10 @@ def main():
               n0 * (10 * T + 2), f"{N}^2 x (10 planes x {T} B: x, b, eta, etaff, beta_u, beta_v read, x written; 2 B mask)"),
              ("operator", abi.PROFILE_OPERATOR, "k_apply_mt<double> (PCG operator q = A p with p.q, p through LDS)",
               n0 * (8 * 8 + 2), f"{N}^2 x (8 planes x 8 B: p, eta, etaff, beta_u, beta_v read, q written; 2 B mask)")]
+    kinds.append(("update", abi.PROFILE_UPDATE, "k_update_xr<double> (PCG x += a p, r -= a q, r.r" +
+                  ("; FP32 copy of r)" if main_label == "mixed" else ")"),
+                  n0 * (12 * 8 + (8 if main_label == "mixed" else 0)),
+                  f"{N}^2 x (12 x 8 B: x, p, q, r read, x, r written" + ("; r as FP32 written, 8 B)" if main_label == "mixed" else ")")))
     if main_label == "mixed" and args.coarse == "galerkin":
         kinds.append(("galerkin_sweep", abi.PROFILE_GALERKIN_SWEEP,
                       f"k_gp_sweep_t<float> (Galerkin stencil sweep, symmetric packed planes, x through LDS, {N // 2}^2 level)",
@@ -398,10 +536,20 @@ def main():
                        "launches": top["launches"], "algorithmic_bytes_per_launch": top["algorithmic_bytes_per_launch"],
                        "kernels": kernels}
     if not args.no_cpu_baseline and "fp64" in results:
-        out["cpu_baseline"] = cpu_baseline(models["fp64"][1], results["fp64"]["infos"][-1])
+        cores, cpu_model = host_cores()
+        for mm, _ in models.values():
+            mm.close()
+        models = {}
+        out["cpu_baseline"] = cpu_baseline(setups["fp64"], results["fp64"]["infos"][-1], cores)
+        out["cpu_baseline"]["cpu_model"] = cpu_model
+        if not args.no_cpu_full:
+            out["cpu_full_solves"] = cpu_full_solves(cores)
+    if args.mismip:
+        out["mismip_scaling"] = mismip_leg([float(x) for x in args.mismip.split(",")])
     if args.serve > 0:
         for m, _ in models.values():
             m.close()
+        models = {}
         out["serving"] = serving_leg(args.serve, args.serve_rate, args.serve_slots, min(args.serve_serial, args.serve))
         out["served_p50_ms"] = out["serving"]["p50_ms"]
         out["served_p99_ms"] = out["serving"]["p99_ms"]
