@@ -72,8 +72,8 @@ enum { NK_RATE_FACTOR_2D = 0, NK_RATE_FACTOR_3D = 1 };
 
 /* Physics of the model: IceSheetConstants, SmoothIceMask, GlenFlowLaw, the friction law, the grounding treatment
  * and DepthIntegrated (src/Models/ice_sheet_constants.jl:33, src/Models/smooth_ice_mask.jl:41,
- * src/Rheologies/glen_flow_law.jl:114, src/Frictions/frictions.jl:255,286, src/Groundings/subgrid_grounded_fraction.jl:146,
- * src/StressBalances/depth_integrated.jl:138). */
+ * src/Rheologies/glen_flow_law.jl:36, src/Frictions/frictions.jl:34,65, src/Groundings/subgrid_grounded_fraction.jl:23,
+ * src/StressBalances/depth_integrated.jl:29). */
 typedef struct nk_physics {
     double ice_density, ocean_density, gravitational_acceleration;   /* 918, 1028, 9.81 */
     int32_t grounding;                                               /* NK_GROUNDING_* */
@@ -106,7 +106,7 @@ enum { NK_PRECISION_FP64 = 0, NK_PRECISION_MIXED = 1 };
  * coarse cell straddles the front and an ice-free strip narrower than the cell). */
 enum { NK_COARSE_REDISCRETIZED = 0, NK_COARSE_GALERKIN = 1 };
 
-/* StressBalanceSolver options (src/StressBalances/stress_balance_solver.jl:126-135) and the multigrid defaults
+/* StressBalanceSolver options (src/StressBalances/stress_balance_solver.jl:43-52) and the multigrid defaults
  * (src/StressBalances/multigrid_preconditioner.jl:57). */
 typedef struct nk_solver_options {
     nk_control linear;                /* Convergence(reltol = 1e-8, maxiter = 10_000) */
@@ -194,6 +194,8 @@ typedef enum nk_field_id {
     NK_FIELD_TSURF,        /* surface temperature Tₛ (K)  (Center, Center)    */
     NK_FIELD_TOCEAN,       /* ocean temperature under floating ice Tₒ (K) (Center, Center) */
     NK_FIELD_STRAIN_HEATING, /* strain heating Φ (W m⁻³) (Center, Center, σ)  */
+    NK_FIELD_HBAR,         /* ∂J/∂H from nk_adjoint (Center, Center): through geometry, driving stress, coefficients */
+    NK_FIELD_BEDBAR,       /* ∂J/∂b from nk_adjoint (Center, Center)          */
     NK_FIELD_COUNT
 } nk_field_id;
 
@@ -233,8 +235,8 @@ void nk_default_grid(nk_grid *grid);
 void nk_default_physics(nk_physics *physics);
 void nk_default_options(nk_solver_options *options);
 
-/* StressBalanceSolver(grid; ...) + the field set of IceSheetModel (src/StressBalances/stress_balance_solver.jl:126,
- * src/Models/ice_sheet_fields.jl:182-233), and StressBalanceMultigridPreconditioner(grid; ...) when options ask for
+/* StressBalanceSolver(grid; ...) + the field set of IceSheetModel (src/StressBalances/stress_balance_solver.jl:43-52,
+ * src/Models/ice_sheet_fields.jl:8-59), and StressBalanceMultigridPreconditioner(grid; ...) when options ask for
  * it (src/StressBalances/multigrid_preconditioner.jl:57). Allocates all device state of the handle. */
 nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_solver_options *options,
                     nk_solver **out);
@@ -359,7 +361,12 @@ nk_status nk_till_effective_pressure(nk_solver *solver, const nk_till *till);
  * solve being the handle's PCG. Outputs: λ in (LAMBDAU, LAMBDAV) — the gradient with respect to the driving stress —
  * and the parameter gradients ∂J/∂C (CBAR) and ∂J/∂A (ABAR). DIVA carries the lagged state (βᵇ, F₂, ησ) through the
  * fixed point as the reference rule does; a cost on the layer velocities enters through (USIGMABAR, VSIGMABAR),
- * whose pullback through compute_velocity_profile! (src/StressBalances/diva.jl:89-109) gives the state's shadows. */
+ * whose pullback through compute_velocity_profile! (src/StressBalances/diva.jl:89-109) gives the state's shadows.
+ * Geometry: ∂J/∂H (HBAR) and ∂J/∂b (BEDBAR) through update_geometry! (s, μ, φᵘ, φᵛ, N;
+ * src/Models/ice_sheet_model_methods.jl:103-111), the driving stress with its calving-front and grounding-line terms
+ * (src/StressBalances/driving_stress.jl:56-125), the H μ factor of η̄H, the H factor of F₂ (DIVA) and the drag's φ
+ * and N — the gradient of the whole update_velocities!(model) with geometry computed on the device (nk_update_velocities
+ * with geometry = 1), as the reference's test differentiates update_state! (test/test_enzyme.jl:341-356). */
 nk_status nk_adjoint(nk_solver *solver, const nk_control *control, nk_info *info);
 
 /* Multigrid level shapes of the handle (StressBalanceMultigridPreconditioner(grid).levels). Writes up to

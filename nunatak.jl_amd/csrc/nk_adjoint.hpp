@@ -1,7 +1,7 @@
 // nk_adjoint.hpp — transpose of the SSA Picard coefficient step, for the fixed-point adjoint of picard_solve!
 // (ext/NunatakEnzymeExt.jl:113-240). With the solution x* frozen as the operand, one step maps x to
 // q(x) = A(c(x)) x*, c = (η̄H, η̄Hᶠᶠ = ℑxy(η̄H), βᵘ, βᵛ) from update_stress_balance_coefficients!
-// (src/StressBalances/stress_balance_coefficients.jl:64-74). These kernels form ȳ = (∂q/∂x)ᵀ q̄ and, in the parameter
+// (src/StressBalances/stress_balance_coefficients.jl:10-20). These kernels form ȳ = (∂q/∂x)ᵀ q̄ and, in the parameter
 // pass, C̄ = (∂q/∂C)ᵀ q̄ and Ā = (∂q/∂A)ᵀ q̄, following the forward kernels' index maps (a clamped or wrapped
 // neighbour receives the adjoint of every read that landed on it, which is the adjoint of Oceananigans' halo fill).
 // Contributions that several threads add to one point use FP64 atomics.
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(NT) void k_adj_shear(KG g, double wf, const double 
     if (!periph_cf(g, i - 1, j)) aadd(&yv[off(g, cx(g, i - 1), fy(g, j))], -rb / g.dx);
 }
 
-// d basal_drag / dU and / dC (src/Frictions/frictions.jl:304-310)
+// d basal_drag / dU and / dC (src/Frictions/frictions.jl:83-89)
 __device__ __forceinline__ void drag_derivatives(const PhysK &p, double U, double C, double Nf, double &dU, double &dC) {
     const double m = p.m;
     if (p.friction == 1) {
@@ -441,6 +441,235 @@ __global__ __launch_bounds__(NT) void k_add(KG g, const double *__restrict__ au,
     const long o = off(g, i, j);
     cu[o] = au[o] + 1.0 * bu[o];
     cv[o] = av[o] + 1.0 * bv[o];
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Geometry pullback of the converged solve: ∂J/∂H and ∂J/∂b. H and b enter the velocity solve through the geometry
+// diagnostics (s, μ, φᵘ, φᵛ, N of update_geometry!, src/Models/ice_sheet_model_methods.jl:103-111), the driving
+// stress with its calving-front and grounding-line terms (src/StressBalances/driving_stress.jl:56-125), the H μ
+// factor of η̄H and the H factor of F₂ (src/StressBalances/column_integrals.jl:70-71) and the drag (φ and N,
+// src/Frictions/frictions.jl:83-89). With τ̄ = λ, the coefficient shadows of the final pullback (η̄H̄ after the
+// corner interpolation, F̄₂ for DIVA) and β̄ = Az x* q̄ (or the DIVA basal β̄ᵇ), the pullback runs in two passes
+// without atomics: every face writes what it sends to its two cells (the cells its forward reads landed on, clamped
+// or wrapped), then every cell adds the sends of the faces that read it, in a fixed order.
+// ---------------------------------------------------------------------------------------------------------------
+
+// front_force(H, s) = g (ρᵢ H² − ρw D²)/2, D = clamp(H − s, 0, H): derivatives in H and s (one-sided at the kinks)
+__device__ __forceinline__ void front_force_d(double H, double s, double ri, double rw, double gr, double &dH, double &ds) {
+    const double d = H - s;
+    double D = d, dDdH = 1.0, dDds = -1.0;
+    if (d > H) { D = H; dDdH = 1.0; dDds = 0.0; }
+    else if (d < 0.0) { D = 0.0; dDdH = 0.0; dDds = 0.0; }
+    dH = gr * (ri * H - rw * D * dDdH);
+    ds = gr * (-rw * D * dDds);
+}
+
+// grounding_line_stress (driving_stress.jl:92-103) derivatives in (H₁, H₂, s₁, s₂, b₁, b₂)
+__device__ __forceinline__ void gl_stress_d(double H1, double H2, double s1, double s2, double b1, double b2, double ri, double rw,
+                                            double gr, double d[6]) {
+    const double f1 = H1 + rw / ri * b1, f2 = H2 + rw / ri * b2;
+    const double den = f1 - f2;
+    const double theta = f1 / den;
+    const double Hg = H1 + theta * (H2 - H1);
+    const double sg = (1.0 - ri / rw) * Hg;
+    const double c = -ri * gr / 2.0, kap = 1.0 - ri / rw;
+    const double dHg = c * ((s2 - s1) + kap * (H1 - H2));
+    const double t1 = -f2 / (den * den), t2 = f1 / (den * den);   // ∂θ/∂f₁, ∂θ/∂f₂
+    const double dH = H2 - H1;
+    d[0] = c * (sg - s1) + dHg * (1.0 - theta + dH * t1);
+    d[1] = c * (s2 - sg) + dHg * (theta + dH * t2);
+    d[2] = -c * (H1 + Hg);
+    d[3] = c * (Hg + H2);
+    d[4] = dHg * dH * t1 * (rw / ri);
+    d[5] = dHg * dH * t2 * (rw / ri);
+}
+
+// segment_fraction(f₁, f₂) (subgrid_grounded_fraction.jl:31-35) derivatives
+__device__ __forceinline__ void segment_fraction_d(double f1, double f2, double &d1, double &d2) {
+    const double fp = jmax(f1, 0.0) + jmax(f2, 0.0), sf = fabs(f1) + fabs(f2);
+    if (sf == 0.0) { d1 = d2 = 0.0; return; }
+    const double p1 = signbit(f1 - 0.0) ? 0.0 : 1.0, p2 = signbit(f2 - 0.0) ? 0.0 : 1.0;
+    const double g1 = f1 < 0.0 ? -1.0 : 1.0, g2 = f2 < 0.0 ? -1.0 : 1.0;
+    d1 = (p1 * sf - fp * g1) / (sf * sf);
+    d2 = (p2 * sf - fp * g2) / (sf * sf);
+}
+
+// ∂s/∂H, ∂s/∂b and ∂N/∂H, ∂N/∂b at a cell (update_geometry!: surface elevation and effective pressure)
+struct GeoD { double sH, sb, NH, Nb; };
+__device__ __forceinline__ GeoD geometry_d(const PhysK &p, double H, double b) {
+    GeoD r;
+    if (p.grounding != 0) {
+        const bool grounded = !signbit((b + H) - (1.0 - p.ri_rw) * H);   // jmax keeps its first argument
+        r.sH = grounded ? 1.0 : 1.0 - p.ri_rw;
+        r.sb = grounded ? 1.0 : 0.0;
+    } else {
+        r.sH = 1.0; r.sb = 1.0;
+    }
+    const bool bneg = signbit(b - 0.0);                                   // jmin(b, 0) keeps b
+    const double P = p.rig * H + p.rwg * (bneg ? b : 0.0);
+    const bool pos = !signbit(P - 0.0);
+    r.NH = pos ? p.rig : 0.0;
+    r.Nb = (pos && bneg) ? p.rwg : 0.0;
+    return r;
+}
+
+// d basal_drag / dN̄ for regularized Coulomb (τc = α² N̄)
+__device__ __forceinline__ double drag_dN(const PhysK &p, double U, double C, double Nf) {
+    if (p.friction != 2) return 0.0;
+    const double m = p.m, tc = p.alpha2 * Nf;
+    const double D = pow(C, 1.0 / m) * U + pow(tc, 1.0 / m);
+    return p.alpha2 * C * pow(U, m - 1.0) * pow(D, -m - 1.0) * pow(C, 1.0 / m) * U;
+}
+
+// Pass 1, one thread per face: the H̄ and b̄ a u face sends to its cells (lo = the cell read as i − 1, hi = as i) and
+// likewise for v faces (lo = j − 1, hi = j). β̄ is Az x* q̄ on active rows (SSA), or the given basal β̄ᵇ (DIVA, with
+// xu, xv the basal velocities).
+__global__ __launch_bounds__(NT) void k_adj_geometry_faces(KG g, PhysK p, const uint8_t *__restrict__ act, const double *__restrict__ H,
+                                                           const double *__restrict__ bed, const double *__restrict__ s,
+                                                           const double *__restrict__ mu, const double *__restrict__ N,
+                                                           const double *__restrict__ C, const double *__restrict__ phiu,
+                                                           const double *__restrict__ phiv, const double *__restrict__ xu,
+                                                           const double *__restrict__ xv, const double *__restrict__ lu,
+                                                           const double *__restrict__ lv, const double *__restrict__ qbu,
+                                                           const double *__restrict__ qbv, const double *__restrict__ bbar_u,
+                                                           const double *__restrict__ bbar_v, int gl, double *__restrict__ hlo_u,
+                                                           double *__restrict__ hhi_u, double *__restrict__ blo_u,
+                                                           double *__restrict__ bhi_u, double *__restrict__ hlo_v,
+                                                           double *__restrict__ hhi_v, double *__restrict__ blo_v,
+                                                           double *__restrict__ bhi_v) {
+    NK_IJ
+    if (i > g.Nx + 1 || j > g.Ny + 1) return;
+    const long o = off(g, i, j);
+    const double Az = g.dx * g.dy, ri = p.ri, rw = p.rw, gr = p.gr;
+    for (int dir = 0; dir < 2; dir++) {
+        const bool in = dir == 0 ? (i <= nxf(g) && j <= g.Ny) : (i <= g.Nx && j <= nyf(g));
+        if (!in) continue;
+        // the two cells this face reads, (a0, c0) below and (a1, c1) at the face's own index
+        const int a0 = dir == 0 ? cx(g, i - 1) : cx(g, i), c0 = dir == 0 ? cy(g, j) : cy(g, j - 1);
+        const int a1 = cx(g, i), c1 = cy(g, j);
+        const long o0 = off(g, a0, c0), o1 = off(g, a1, c1);
+        const double H0 = H[o0], H1 = H[o1], s0 = s[o0], s1 = s[o1], b0 = bed[o0], b1 = bed[o1];
+        const double len = dir == 0 ? g.dy : g.dx, h = dir == 0 ? g.dx : g.dy;
+        double hb0 = 0.0, hb1 = 0.0, bb0 = 0.0, bb1 = 0.0, sb0 = 0.0, sb1 = 0.0, nb0 = 0.0, nb1 = 0.0;
+        const bool wall = dir == 0 ? periph_fc(g, i, j) : periph_cf(g, i, j);
+        // driving stress, τ̄ = λ
+        const double tb = dir == 0 ? lu[o] : lv[o];
+        bool icefree = false, west = false, east = false;
+        if (mu) {
+            const double m0 = mu[o0], m1 = mu[o1];
+            icefree = (m0 + m1) == 0.0;
+            west = (m0 > 0) & !(m1 > 0);
+            east = !(m0 > 0) & (m1 > 0);
+        }
+        if (!wall && !icefree && tb != 0.0) {
+            bool st = false;
+            if (gl) {
+                const double f1 = H0 + rw / ri * b0, f2 = H1 + rw / ri * b1;
+                st = ((f1 > 0) != (f2 > 0)) & (H0 > 0) & (H1 > 0);
+            }
+            if (west || east) {
+                double dH, ds;
+                if (west) { front_force_d(H0, s0, ri, rw, gr, dH, ds); hb0 += tb * len * dH; sb0 += tb * len * ds; }
+                if (east) { front_force_d(H1, s1, ri, rw, gr, dH, ds); hb1 -= tb * len * dH; sb1 -= tb * len * ds; }
+            } else if (st) {
+                double d[6];
+                gl_stress_d(H0, H1, s0, s1, b0, b1, ri, rw, gr, d);
+                hb0 += tb * len * d[0]; hb1 += tb * len * d[1];
+                sb0 += tb * len * d[2]; sb1 += tb * len * d[3];
+                bb0 += tb * len * d[4]; bb1 += tb * len * d[5];
+            } else {
+                const double c = -Az * ri * gr, Hf = (H0 + H1) / 2.0, ds = (s1 - s0) / h;
+                hb0 += tb * c * ds / 2.0;
+                hb1 += tb * c * ds / 2.0;
+                sb1 += tb * c * Hf / h;
+                sb0 -= tb * c * Hf / h;
+            }
+        }
+        // drag: β = φ · basal_drag(U, C̄, N̄)
+        if (p.friction != 0 && !wall) {
+            double bb;
+            if (dir == 0) bb = bbar_u ? bbar_u[o] : qbar_u(g, act, qbu, i, j) * Az * xu[o];
+            else bb = bbar_v ? bbar_v[o] : qbar_v(g, act, qbv, i, j) * Az * xv[o];
+            if (bb != 0.0) {
+                double U;
+                if (dir == 0) {
+                    const double vm = ((ldcf(g, xv, i - 1, j) + ldcf(g, xv, i, j)) / 2.0 + (ldcf(g, xv, i - 1, j + 1) + ldcf(g, xv, i, j + 1)) / 2.0) / 2.0;
+                    U = sqrt(xu[o] * xu[o] + vm * vm + p.u0 * p.u0);
+                } else {
+                    const double um = ((ldfc(g, xu, i, j - 1) + ldfc(g, xu, i + 1, j - 1)) / 2.0 + (ldfc(g, xu, i, j) + ldfc(g, xu, i + 1, j)) / 2.0) / 2.0;
+                    U = sqrt(xv[o] * xv[o] + um * um + p.u0 * p.u0);
+                }
+                const double Cf = (C[o0] + C[o1]) / 2.0;
+                const double Nf = p.friction == 2 ? (N[o0] + N[o1]) / 2.0 : 0.0;
+                const double ph = dir == 0 ? phiu[o] : phiv[o];
+                if (p.grounding != 0) {
+                    const double phb = bb * basal_drag(p, U, Cf, Nf);
+                    double d0, d1;
+                    segment_fraction_d(H0 + p.rw_ri * b0, H1 + p.rw_ri * b1, d0, d1);
+                    hb0 += phb * d0; bb0 += phb * d0 * p.rw_ri;
+                    hb1 += phb * d1; bb1 += phb * d1 * p.rw_ri;
+                }
+                if (p.friction == 2) {
+                    const double nb = bb * ph * drag_dN(p, U, Cf, Nf) / 2.0;
+                    nb0 += nb;
+                    nb1 += nb;
+                }
+            }
+        }
+        // s̄ and N̄ of the two cells to H̄ and b̄ through their own geometry
+        const GeoD g0 = geometry_d(p, H0, b0), g1 = geometry_d(p, H1, b1);
+        hb0 += sb0 * g0.sH + nb0 * g0.NH;
+        bb0 += sb0 * g0.sb + nb0 * g0.Nb;
+        hb1 += sb1 * g1.sH + nb1 * g1.NH;
+        bb1 += sb1 * g1.sb + nb1 * g1.Nb;
+        if (dir == 0) { hlo_u[o] = hb0; hhi_u[o] = hb1; blo_u[o] = bb0; bhi_u[o] = bb1; }
+        else { hlo_v[o] = hb0; hhi_v[o] = hb1; blo_v[o] = bb0; bhi_v[o] = bb1; }
+    }
+}
+
+// Pass 2, one thread per cell: H̄ = η̄H̄ μ Σ ησ Δζ (+ F̄₂ Σ w₂/ησ, DIVA) + the mask's share (∂μ/∂H · η̄H̄ H Σ ησ Δζ)
+// + the sends of the faces that read the cell: along x the face at the cell's own index (read as i) and the one
+// above it (read as i − 1), and on a Bounded edge the boundary face that the clamp maps here, on a Periodic one
+// the first face, which wraps to the last cell; likewise along y. b̄ from the faces alone.
+__global__ __launch_bounds__(NT) void k_adj_geometry_cells(KG g, PhysK p, Sig sg, const double *__restrict__ H,
+                                                           const double *__restrict__ es, long plane, const double *__restrict__ etab,
+                                                           const double *__restrict__ F2b, const double *__restrict__ hlo_u,
+                                                           const double *__restrict__ hhi_u, const double *__restrict__ blo_u,
+                                                           const double *__restrict__ bhi_u, const double *__restrict__ hlo_v,
+                                                           const double *__restrict__ hhi_v, const double *__restrict__ blo_v,
+                                                           const double *__restrict__ bhi_v, double *__restrict__ Hb,
+                                                           double *__restrict__ Bb) {
+    NK_IJ
+    if (i > g.Nx || j > g.Ny) return;
+    const long o = off(g, i, j);
+    double e = 0.0, w = 0.0;
+    for (int k = 0; k < p.Ns; k++) {
+        const double ek = es[o + (long)k * plane];
+        e += ek * sg.dz[k];
+        w += (ek == 0.0 ? 0.0 : sg.w2[k] / ek);
+    }
+    const double eb = etab[o];
+    double mu = 1.0, dmu = 0.0;
+    if (p.masking) {
+        const double t = jclamp((H[o] - p.hmin) / p.htr, 0.0, 1.0);
+        mu = t * t * (3.0 - 2.0 * t);
+        dmu = (t > 0.0 && t < 1.0) ? 6.0 * t * (1.0 - t) / p.htr : 0.0;
+    }
+    double hb = eb * e * mu + eb * e * H[o] * dmu;
+    if (F2b) hb += F2b[o] * w;
+    double bb = 0.0;
+    // x: face i (read as i), face i + 1 (read as i − 1), and the edge face mapped here
+    hb += hhi_u[o]; bb += bhi_u[o];
+    if (i + 1 <= nxf(g)) { const long q = off(g, i + 1, j); hb += hlo_u[q]; bb += blo_u[q]; }
+    if (g.tx && i == g.Nx) { const long q = off(g, g.Nx + 1, j); hb += hhi_u[q]; bb += bhi_u[q]; }
+    if ((g.tx && i == 1) || (!g.tx && i == g.Nx)) { const long q = off(g, 1, j); hb += hlo_u[q]; bb += blo_u[q]; }
+    // y likewise
+    hb += hhi_v[o]; bb += bhi_v[o];
+    if (j + 1 <= nyf(g)) { const long q = off(g, i, j + 1); hb += hlo_v[q]; bb += blo_v[q]; }
+    if (g.ty && j == g.Ny) { const long q = off(g, i, g.Ny + 1); hb += hhi_v[q]; bb += bhi_v[q]; }
+    if ((g.ty && j == 1) || (!g.ty && j == g.Ny)) { const long q = off(g, i, 1); hb += hlo_v[q]; bb += blo_v[q]; }
+    Hb[o] = hb;
+    Bb[o] = bb;
 }
 
 }  // namespace nk

@@ -75,7 +75,7 @@ __device__ __forceinline__ double jclamp(double x, double lo, double hi) { retur
 
 // ---------------------------------------------------------------------------------------------------------------
 // Activity flags: bit 0 = u point active, bit 1 = v point active (not a wall, not ice free on both sides).
-// inactiveᶠᶜᶜ / inactiveᶜᶠᶜ of src/StressBalances/ssa_operator.jl:91-92 evaluated once per coefficient update.
+// inactiveᶠᶜᶜ / inactiveᶜᶠᶜ of src/StressBalances/strain_rates.jl:2-3 evaluated once per coefficient update.
 // ---------------------------------------------------------------------------------------------------------------
 // Branch-free: the flag byte is always loaded from an in-range (clamped or wrapped) address and the range test is a
 // select, so every load of a stencil point can be issued before any of them returns.
@@ -91,7 +91,7 @@ __device__ __forceinline__ bool vact(const KG &g, const uint8_t *act, int i, int
     const int jj = g.ty ? clampi(j, 1, g.Ny + 1) : wrapi(j, g.Ny);
     return in & (bool)((ld32(act, uoff(g, ii, jj)) >> 1) & 1);
 }
-// interior_u / interior_v with the stiffness mask (ssa_operator.jl:94-95)
+// interior_u / interior_v with the stiffness mask (src/StressBalances/strain_rates.jl:5-6)
 template <typename T> __device__ __forceinline__ T iu(const KG &g, const uint8_t *act, const T *u, int i, int j) {
     const T val = ld32(u, uoff(g, fx(g, i), cy(g, j)));
     return uact(g, act, i, j) ? val : T(0);
@@ -1266,7 +1266,7 @@ __global__ __launch_bounds__(NT) void k_coarsen(KG cg, KG fg, int hx, int hy, co
     bv[o] = sv / cAz;
 }
 
-// η̄Hᶠᶠ = ℑxyᶠᶠ(η̄H) (src/StressBalances/stress_balance_coefficients.jl:80-83) and the activity flags, over the
+// η̄Hᶠᶠ = ℑxyᶠᶠ(η̄H) (src/StressBalances/stress_balance_coefficients.jl:26-29) and the activity flags, over the
 // (Nx + 1) × (Ny + 1) extent of one level.
 __global__ __launch_bounds__(NT) void k_corners_flags(KG g, const double *__restrict__ eta, double *__restrict__ etaff,
                                                       uint8_t *__restrict__ act) {

@@ -605,15 +605,18 @@ constexpr int GDENSE_STOP = 128;  // a Galerkin hierarchy ends at the first coar
 
 __device__ __forceinline__ int g_unknown(const KG &g, int comp, int i, int j) { return comp * g.Nx * g.Ny + (j - 1) * g.Nx + (i - 1); }
 
-template <typename T, int TX, int TY>
-__global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const T *__restrict__ st, T *__restrict__ M) {
+// The inverse is formed and stored in FP64 whatever the V-cycle's precision (the mixed form keeps the coarsest-grid
+// solve in FP64); a null direction is recognised relative to the rounding of the stencil it was assembled from.
+template <typename TS, int TX, int TY>
+__global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__restrict__ st, double *__restrict__ M) {
+    using T = double;
     g.tx = TX; g.ty = TY;
     __shared__ T fcol[GDENSE_MAX];
     __shared__ T dorig[GDENSE_MAX];
     __shared__ unsigned char sing[GDENSE_MAX];
     __shared__ T piv;
     __shared__ int skip;
-    const T tol = sizeof(T) == 4 ? T(1e-5) : T(1e-10);
+    const T tol = sizeof(TS) == 4 ? T(1e-5) : T(1e-10);
     const int n = 2 * g.Nx * g.Ny, tid = threadIdx.x, nth = blockDim.x;
     const long P = gplane(g);
     for (int e = tid; e < n * n; e += nth) M[e] = T(0);
@@ -626,7 +629,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const T *__res
             const GBlk b = gblk(rc, cc);
             for (int dx = b.x0; dx <= b.x1; dx++)
                 for (int dy = b.y0; dy <= b.y1; dy++) {
-                    const T a = st[gidx(b, dx, dy) * P + o];
+                    const T a = (T)st[gidx(b, dx, dy) * P + o];
                     if (a == T(0)) continue;
                     int ic = i + dx, jc = j + dy;
                     if (!g.tx) ic = wrapi(ic, g.Nx);
@@ -682,23 +685,23 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const T *__res
     }
 }
 
-// x = M⁻¹ b on the coarsest level (one workgroup)
+// x = M⁻¹ b on the coarsest level (one workgroup), in FP64 (b promoted exactly, x rounded once to the level's precision)
 template <typename T>
-__global__ __launch_bounds__(TAIL_THREADS) void k_g_dense_solve(KG g, const T *__restrict__ M, const T *__restrict__ bu,
+__global__ __launch_bounds__(TAIL_THREADS) void k_g_dense_solve(KG g, const double *__restrict__ M, const T *__restrict__ bu,
                                                                 const T *__restrict__ bv, T *__restrict__ xu, T *__restrict__ xv,
                                                                 const Scal *gate) {
     if (solve_done(gate)) return;
     const int n = 2 * g.Nx * g.Ny;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        T acc = T(0);
+        double acc = 0.0;
         for (int c = 0; c < n; c++) {
             const int cc = c >= n / 2, k = c - cc * (n / 2);
             const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-            acc += M[(long)c * n + r] * (cc ? bv[o] : bu[o]);
+            acc += M[(long)c * n + r] * (double)(cc ? bv[o] : bu[o]);
         }
         const int rc = r >= n / 2, k = r - rc * (n / 2);
         const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-        (rc ? xv : xu)[o] = acc;
+        (rc ? xv : xu)[o] = (T)acc;
     }
 }
 
@@ -740,10 +743,10 @@ template <typename T> struct GTailLevel {
 
 template <typename T> struct GTailDesc {
     int n, sweeps, coarsest;
-    int dgroups, o_dp;      // dense solve: column groups and the LDS offset of their partial products (G·n values)
+    int dgroups, o_dp;      // dense solve: column groups and the LDS byte offset of their FP64 partial products (G·n)
     const T *gbu, *gbv;     // right-hand side of the top tail level (global)
     T *gxu, *gxv;           // result of the top tail level (global)
-    const T *cinv;          // dense inverse of the coarsest level, or null (sweeps)
+    const double *cinv;     // FP64 dense inverse of the coarsest level, or null (sweeps)
     GTailLevel<T> lv[GTAIL_MAX];
 };
 
@@ -893,25 +896,26 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         // (coalesced: M is column-major), then the groups' partials added in group order
         const GTailLevel<T> &L = d.lv[l];
         const KG g = geom(L.lg);
+        // in FP64 whatever the V-cycle's precision: b promoted exactly, x rounded once to the level's precision
         const int n = 2 * g.Nx * g.Ny, G = d.dgroups, chunk = (n + G - 1) / G;
         const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
-        T *dp = sm + d.o_dp;
+        double *dp = reinterpret_cast<double *>(smem + d.o_dp);
         if (tid < G * n) {
             const int r = tid % n, q = tid / n;
             const int c0 = q * chunk, c1 = c0 + chunk < n ? c0 + chunk : n;
-            T acc = T(0);
+            double acc = 0.0;
             constexpr int U = 16;   // loads of a batch issued together
             for (int cb = c0; cb < c1; cb += U) {
-                T m[U];
+                double m[U];
 #pragma unroll
-                for (int t = 0; t < U; t++) m[t] = cb + t < c1 ? d.cinv[(long)(cb + t) * n + r] : T(0);
+                for (int t = 0; t < U; t++) m[t] = cb + t < c1 ? d.cinv[(long)(cb + t) * n + r] : 0.0;
 #pragma unroll
                 for (int t = 0; t < U; t++) {
                     const int c = cb + t;
                     if (c < c1) {
                         const int cc = c >= n / 2, k = c - cc * (n / 2);
                         const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-                        acc += m[t] * (cc ? bv[o] : bu[o]);
+                        acc += m[t] * (double)(cc ? bv[o] : bu[o]);
                     }
                 }
             }
@@ -920,11 +924,11 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         __syncthreads();
         T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
         for (int r = tid; r < n; r += nth) {
-            T acc = T(0);
+            double acc = 0.0;
             for (int q = 0; q < G; q++) acc += dp[q * n + r];
             const int rc = r >= n / 2, k = r - rc * (n / 2);
             const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-            (rc ? xv : xu)[o] = acc;
+            (rc ? xv : xu)[o] = (T)acc;
         }
         __syncthreads();
     } else {

@@ -25,7 +25,7 @@ struct Sig {
 };
 
 // ---------------------------------------------------------------------------------------------------------------
-// Geometry (src/Groundings/subgrid_grounded_fraction.jl:156-357, src/Models/smooth_ice_mask.jl:50-69). Bit-exact.
+// Geometry (src/Groundings/subgrid_grounded_fraction.jl:31-232, src/Models/smooth_ice_mask.jl:50-69). Bit-exact.
 // ---------------------------------------------------------------------------------------------------------------
 __device__ __forceinline__ double nonzero(double x) { return x == 0.0 ? 1.0 : x; }
 __device__ __forceinline__ double ice_mask_value(double H, double hmin, double htr) {
@@ -151,10 +151,10 @@ __global__ __launch_bounds__(NT) void k_driving_stress(KG g, PhysK p, const doub
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Viscosity and column integrals (src/Rheologies/glen_flow_law.jl:185-221, src/StressBalances/column_integrals.jl:201-212)
+// Viscosity and column integrals (src/Rheologies/glen_flow_law.jl:97-143, src/StressBalances/column_integrals.jl:54-72)
 // ---------------------------------------------------------------------------------------------------------------
 
-// effective_strain_rate² (src/StressBalances/ssa_operator.jl:116-121): strain rates see walls only.
+// effective_strain_rate² (src/StressBalances/strain_rates.jl:27-32): strain rates see walls only.
 __device__ __forceinline__ double eff_strain2(const KG &g, const double *u, const double *v, int i, int j, double wf) {
     const double exx = (iu_w(g, u, i + 1, j) - iu_w(g, u, i, j)) / g.dx;
     const double eyy = (iv_w(g, v, i, j + 1) - iv_w(g, v, i, j)) / g.dy;
@@ -210,14 +210,14 @@ __global__ __launch_bounds__(NT) void k_viscosity(KG g, PhysK p, Sig sg, int mod
     F2[o] = Fs * H[o];
 }
 
-// basal_drag (src/Frictions/frictions.jl:304-310)
+// basal_drag (src/Frictions/frictions.jl:83-89)
 __device__ __forceinline__ double basal_drag(const PhysK &p, double U, double Cf, double Nf) {
     if (p.friction == 1) return Cf * pow(U, p.m - 1.0);
     const double tc = p.alpha2 * Nf;
     return Cf * pow(U, p.m - 1.0) * tc / pow(pow(Cf, 1.0 / p.m) * U + pow(tc, 1.0 / p.m), p.m);
 }
 
-// compute_basal_drag! (frictions.jl:348-381): β from velocities (u, v) at faces, zero at walls.
+// compute_basal_drag! (src/Frictions/frictions.jl:127-160): β from velocities (u, v) at faces, zero at walls.
 __global__ __launch_bounds__(NT) void k_drag(KG g, PhysK p, const double *__restrict__ u, const double *__restrict__ v,
                                              const double *__restrict__ C, const double *__restrict__ N,
                                              const double *__restrict__ phiu, const double *__restrict__ phiv,
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(NT) void k_extend(KG g, int Ns, long plane, const d
 }
 
 // Arrhenius rate factor from temperature through the enthalpy round trip
-// (src/Thermodynamics/enthalpy_conversions.jl:116-174, src/Thermodynamics/column_energy_solve.jl:105-117,
+// (src/Thermodynamics/enthalpy_conversions.jl:1-66, src/Thermodynamics/column_energy_solve.jl:105-117,
 //  src/Rheologies/arrhenius_rate_factor.jl:102-110), default parameters of the reference.
 // Defaults of EnthalpyThermodynamics (src/Thermodynamics/enthalpy_thermodynamics.jl:96-105) and ArrheniusRateFactor
 // (src/Rheologies/arrhenius_rate_factor.jl:60-69).
@@ -320,7 +320,7 @@ struct Thermo {
     static constexpr double Ac = 3.61e-13, Aw = 1.73e3, Qc = 6.0e4, Qw = 13.9e4, Tt = 263.15, Tw = 1.0, R = 8.31441,
                             cw = 181.25, wmax = 0.01;
 };
-// temperature / water_fraction of enthalpy E at pressure p (enthalpy_conversions.jl:116-152), then
+// temperature / water_fraction of enthalpy E at pressure p (src/Thermodynamics/enthalpy_conversions.jl:35-52), then
 // arrhenius_rate_factor(T − Tₚₘ + T₀, ω) (arrhenius_rate_factor.jl:102-110, column_energy_solve.jl:106-111)
 __device__ __forceinline__ void thermal_state(double E, double pr, double &Tk, double &w, double &A) {
     using C = Thermo;

@@ -34,7 +34,7 @@ const LocInfo LOC[NK_FIELD_COUNT] = {
     {1, 0, 0}, {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0},
     {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 2}, {1, 0, 1}, {0, 1, 1}, {0, 0, 1},
     {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
-    {0, 0, 0}, {0, 0, 0}, {0, 0, 1},
+    {0, 0, 0}, {0, 0, 0}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0},
 };
 static_assert(sizeof(LOC) / sizeof(LOC[0]) == NK_FIELD_COUNT, "one location per field id");
 
@@ -136,7 +136,7 @@ struct nk_solver {
     const void *chunk_graph_x = nullptr;  // solution buffer the captured chunk updates
     int *gerr = nullptr;                   // Galerkin products: couplings that fell outside the stencil pattern
     int gtail_start = -1;                  // first level of the Galerkin LDS tail (-1: none)
-    void *cinv = nullptr;                  // dense inverse of the coarsest Galerkin operator (V-cycle precision), or null
+    void *cinv = nullptr;                  // FP64 dense inverse of the coarsest Galerkin operator (both precisions), or null
     // opt-in event timing of one finest-level kernel (nk_solver_options.profile_kernel)
     std::vector<hipEvent_t> prof_events;
     size_t prof_used = 0;
@@ -400,7 +400,7 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
     }
     if (s->cinv) {
         Level &C = s->lev.back();
-        NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, 0, s->stream, C.g, gs<T>(C), (T *)s->cinv);
+        NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, 0, s->stream, C.g, gs<T>(C), (double *)s->cinv);
         CHK(launch_check(s));
     }
     for (size_t l = 1; l < s->lev.size(); l++) {
@@ -663,7 +663,7 @@ nk_status galerkin_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv
     d.sweeps = s->op.smoothing_sweeps;
     d.coarsest = s->op.coarsest_sweeps;
     d.gbu = bu; d.gbv = bv; d.gxu = x.u; d.gxv = x.v;
-    d.cinv = (const T *)s->cinv;
+    d.cinv = (const double *)s->cinv;
     int off_t = 0;
     for (int k = 0; k < d.n; k++) {
         Level &L = s->lev[l + k];
@@ -679,12 +679,14 @@ nk_status galerkin_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv
         int *offs[] = {&t.o_x0u, &t.o_x0v, &t.o_x1u, &t.o_x1v, &t.o_bu, &t.o_bv, &t.o_ru, &t.o_rv};
         for (int *o : offs) { *o = off_t; off_t += plane; }
     }
-    if (d.cinv) {   // partial products of the dense solve
+    if (d.cinv) {   // FP64 partial products of the dense solve, 8-byte aligned
         const KG &gc = s->lev.back().g;
         const int n = 2 * gc.Nx * gc.Ny;
         d.dgroups = TAIL_THREADS / n > 0 ? TAIL_THREADS / n : 1;
-        d.o_dp = off_t;
-        off_t += d.dgroups * n;
+        constexpr int per8 = 8 / (int)sizeof(T);
+        const int at = (off_t + per8 - 1) / per8 * per8;
+        d.o_dp = at * (int)sizeof(T);
+        off_t = at + d.dgroups * n * per8;
     }
     {   // copy the stencils of the smallest levels into LDS while they fit (the coarsest needs none with a dense inverse)
         int flags = 0;
@@ -760,7 +762,7 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     Pair<T> cur = x0, oth = x1;
     if (l == nlev - 1) {
         if (s->cinv) {
-            k_g_dense_solve<T><<<1, TAIL_THREADS, 0, s->stream>>>(L.g, (const T *)s->cinv, bu, bv, cur.u, cur.v, gate);
+            k_g_dense_solve<T><<<1, TAIL_THREADS, 0, s->stream>>>(L.g, (const double *)s->cinv, bu, bv, cur.u, cur.v, gate);
             CHK(launch_check(s));
         } else {
             CHK(smooth(cur, oth, s->op.coarsest_sweeps, true));
@@ -1457,8 +1459,7 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if (options->coarse_operator == NK_COARSE_GALERKIN && s->lev.size() > 1) {
             const KG &gc = s->lev.back().g;
             const size_t n = 2 * (size_t)gc.Nx * (size_t)gc.Ny;
-            const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
-            if (n <= (size_t)GDENSE_MAX && (st = alloc_bytes(s, &s->cinv, n * n * es)) != NK_OK) return fail(st);
+            if (n <= (size_t)GDENSE_MAX && (st = alloc_bytes(s, &s->cinv, n * n * sizeof(double))) != NK_OK) return fail(st);
         }
         if ((st = alloc_bytes(s, (void **)&s->S, sizeof(Scal))) != NK_OK) return fail(st);
         if (hipHostMalloc((void **)&s->S_host, sizeof(Scal), hipHostMallocDefault) != hipSuccess) { s->err = "hipHostMalloc failed"; return fail(NK_OUT_OF_MEMORY); }
@@ -1511,7 +1512,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
     if (o->coarse_operator == NK_COARSE_GALERKIN) {
         b += 64;
         const size_t n = 2 * (size_t)lev.back().Nx * (size_t)lev.back().Ny;
-        if (lev.size() > 1 && n <= (size_t)GDENSE_MAX) b += n * n * (mixed ? sizeof(float) : sizeof(double));
+        if (lev.size() > 1 && n <= (size_t)GDENSE_MAX) b += n * n * sizeof(double);
     }
     return b;
 }
@@ -2125,6 +2126,21 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
     CHK(launch_check(s));
     if (diva) CHK(adjoint_step_diva(s, true));
     else CHK(adjoint_step(s, false, true));
+    // geometry pullback: H̄, b̄ (two passes, faces then cells; the loop's spare planes carry the face sends)
+    {
+        const PhysK p = physk(s);
+        k_adj_geometry_faces<<<Be, THREADS, 0, s->stream>>>(
+            g, p, s->lev[0].act, f[NK_FIELD_H], f[NK_FIELD_BED], f[NK_FIELD_S], s->ph.masking ? f[NK_FIELD_MU] : nullptr,
+            f[NK_FIELD_N], f[NK_FIELD_C], f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], diva ? f[NK_FIELD_UB] : f[NK_FIELD_U],
+            diva ? f[NK_FIELD_VB] : f[NK_FIELD_V], lu, lv, s->ad_qu, s->ad_qv, diva ? s->ad_bbu : nullptr, diva ? s->ad_bbv : nullptr,
+            s->ph.grounding == NK_GROUNDING_SUBGRID_DRIVING_STRESS ? 1 : 0, s->ad_yu, s->ad_ru, s->ad_lu, s->xmu, s->ad_yv, s->ad_rv,
+            s->ad_lv, s->xmv);
+        CHK(launch_check(s));
+        k_adj_geometry_cells<<<B, THREADS, 0, s->stream>>>(g, p, sig_tables(s), f[NK_FIELD_H], f[NK_FIELD_ETASIGMA], s->plane, s->ad_eta,
+                                                          diva ? s->ad_F2n : nullptr, s->ad_yu, s->ad_ru, s->ad_lu, s->xmu, s->ad_yv,
+                                                          s->ad_rv, s->ad_lv, s->xmv, f[NK_FIELD_HBAR], f[NK_FIELD_BEDBAR]);
+        CHK(launch_check(s));
+    }
     if (info) {
         info->picard_iterations = it;
         info->total_linear_iterations = total;

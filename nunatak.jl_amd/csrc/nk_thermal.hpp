@@ -30,7 +30,7 @@ __device__ __forceinline__ double unit_ramp(double x, double w) { return jclamp(
 __device__ __forceinline__ double pmT(double p) { return EP::T0 - EP::beta * p; }
 __device__ __forceinline__ double pmE(double p) { return EP::ci * (pmT(p) - EP::Tr); }
 __device__ __forceinline__ double enth(double T, double w, double p) { return EP::ci * (jmin(T, pmT(p)) - EP::Tr) + w * EP::L; }
-// layer_diffusivity (enthalpy_conversions.jl:182-188)
+// layer_diffusivity (src/Thermodynamics/enthalpy_conversions.jl:72-78)
 __device__ __forceinline__ double layer_diff(double E, double p) {
     const double w = unit_ramp(E - pmE(p) + EP::dE, EP::dE);
     const double Kc = EP::kc / EP::ci;

@@ -49,7 +49,7 @@ struct NKOptions
     linear::NKControl; picard::NKControl
     preconditioner::Int32; smoother::Int32; smoothing_sweeps::Int32; coarsest_sweeps::Int32
     minimum_coarse_size::Int32; precision::Int32
-    coarse_operator::Int32; check_interval::Int32; profile_kernel::Int32; graphs::Int32; pad::Int32
+    coarse_operator::Int32; check_interval::Int32; profile_kernel::Int32; graphs::Int32; report_not_converged::Int32
 end
 
 mutable struct NKInfo
@@ -64,7 +64,8 @@ end
 const U, V, H, BED, S, PHI, PHIU, PHIV, N, MU, TAUU, TAUV, ETAH, ETAHFF, BETAU, BETAV, F2, ETASIGMA,
       BETABU, BETABV, UB, VB, USIGMA, VSIGMA, A, C, TEMPERATURE, PU, PV, QU, QV,
       UBAR, VBAR, LAMBDAU, LAMBDAV, CBAR, ABAR, USIGMABAR, VSIGMABAR, ENTHALPY, WATER_FRACTION,
-      SMB, BASAL_MELT, GROUNDED_MELT, HPREV, GH, HCLIP, HW, HN, GEOTHERMAL, TSURF, TOCEAN, STRAIN_HEATING = 0:52
+      SMB, BASAL_MELT, GROUNDED_MELT, HPREV, GH, HCLIP, HW, HN, GEOTHERMAL, TSURF, TOCEAN, STRAIN_HEATING,
+      HBAR, BEDBAR = 0:54
 
 control(c::Convergence) = NKControl(0, c.maxiter, 0, 0, c.reltol, c.abstol)
 control(c::FixedIterations) = NKControl(1, c.iterations, c.iterations, 0, 0.0, 0.0)
@@ -73,13 +74,17 @@ control(c::FixedIterations) = NKControl(1, c.iterations, c.iterations, 0, 0.0, 0
     HIPStressBalanceSolver(grid; linear_iteration_control, picard_iteration_control, preconditioner, precision, schedule)
 
 Drop-in for `StressBalanceSolver` whose velocity solve runs in libnunatak_hip.so on an MI355X. `preconditioner` is
-`:diagonal` (the reference default) or `:multigrid`; `precision` is `:fp64` or `:mixed` (FP32 V-cycle and inner PCG
-inside FP64 iterative refinement). `iteration` holds the Picard count of the last solve, like the reference's.
+`:diagonal` (the reference default) or `:multigrid`; `precision` is `:fp64` or `:mixed` (FP64 PCG — operator, residual,
+α/β and stopping rule — preconditioned by an FP32 multigrid V-cycle whose coarsest-grid solve is FP64; multigrid
+only, best with `coarse_operator = :galerkin`). `Nσ` and `bed_refinement` are the model's σ grid (IceSheetModel's
+defaults); the first solve checks them against `model.σgrid` and rebuilds the handle if they differ. `iteration`
+holds the Picard count of the last solve, like the reference's.
 """
 mutable struct HIPStressBalanceSolver{Sc}
     handle::Ptr{Cvoid}
     schedule::Sc
     iteration::Int
+    grid::NKGrid
     options::NKOptions
     info::NKInfo
 end
@@ -94,21 +99,41 @@ end
 function HIPStressBalanceSolver(grid; linear_iteration_control = Convergence(eltype(grid); reltol = 1e-8, maxiter = 10_000),
                                 picard_iteration_control = Convergence(eltype(grid); reltol = 1e-6, maxiter = 100),
                                 preconditioner = :diagonal, precision = :fp64, coarse_operator = :rediscretized,
-                                schedule = IterationInterval(1),
+                                schedule = IterationInterval(1), Nσ = 10, bed_refinement = 2,
                                 physics = nothing)
     Nx, Ny, _ = size(grid)
     TX, TY, _ = topology(grid)
     Hx, _, _ = halo_size(grid)
-    g = NKGrid(Nx, Ny, TX === Bounded, TY === Bounded, Hx, 10, grid.Δxᶜᵃᵃ, grid.Δyᵃᶜᵃ, 2.0)
+    g = NKGrid(Nx, Ny, TX === Bounded, TY === Bounded, Hx, Nσ, grid.Δxᶜᵃᵃ, grid.Δyᵃᶜᵃ, bed_refinement)
     o = NKOptions(control(linear_iteration_control), control(picard_iteration_control),
                   preconditioner === :multigrid ? 2 : 1, 0, 2, 20, 2, precision === :mixed ? 1 : 0,
                   coarse_operator === :galerkin ? 1 : 0, 8, 0, 1, 0)
+    solver = HIPStressBalanceSolver(create_handle(g, o), schedule, 0, g, o, NKInfo())
+    finalizer(s -> ccall((:nk_destroy, libnunatak_hip), Cint, (Ptr{Cvoid},), s.handle), solver)
+    return solver
+end
+
+function create_handle(g, o)
     h = Ref{Ptr{Cvoid}}(C_NULL)   # physics are set from the model at every solve
     check(ccall((:nk_create, libnunatak_hip), Cint, (Ref{NKGrid}, Ref{NKPhysics}, Ref{NKOptions}, Ref{Ptr{Cvoid}}),
                 g, default_physics(), o, h), h[], "nk_create")
-    solver = HIPStressBalanceSolver(h[], schedule, 0, o, NKInfo())
-    finalizer(s -> ccall((:nk_destroy, libnunatak_hip), Cint, (Ptr{Cvoid},), s.handle), solver)
-    return solver
+    return h[]
+end
+
+# Nσ and bed_refinement of the model's SigmaGrid (src/Grids/sigma_grid.jl:20-22,43: faces (k/Nσ)^bed_refinement); the
+# handle is rebuilt when they differ from the ones it was created with, so the column weights always match the model.
+function match_sigma_grid!(solver::HIPStressBalanceSolver, model)
+    σgrid = model.σgrid
+    Nσ = size(σgrid, 3)
+    ζ = znodes(σgrid, Face())
+    refinement = Nσ > 1 ? log(ζ[2]) / log(1 / Nσ) : solver.grid.bed_refinement
+    refinement = isapprox(refinement, round(refinement); rtol = 1e-12) ? round(refinement) : refinement
+    (Nσ == solver.grid.Nsigma && refinement == solver.grid.bed_refinement) && return nothing
+    g = solver.grid
+    solver.grid = NKGrid(g.Nx, g.Ny, g.topology_x, g.topology_y, g.halo, Nσ, g.dx, g.dy, refinement)
+    ccall((:nk_destroy, libnunatak_hip), Cint, (Ptr{Cvoid},), solver.handle)
+    solver.handle = create_handle(solver.grid, solver.options)
+    return nothing
 end
 
 function default_physics()
@@ -160,6 +185,7 @@ end
 # update_velocities!(model, solver, ::SSA / ::DIVA) (src/Models/ice_sheet_model_methods.jl:119-147): the geometry
 # fields come from update_geometry!, the velocities are the initial guess and are written back with the solution.
 function update_velocities!(model, solver::HIPStressBalanceSolver, stress_balance)
+    match_sigma_grid!(solver, model)
     check(ccall((:nk_set_physics, libnunatak_hip), Cint, (Ptr{Cvoid}, Ref{NKPhysics}), solver.handle, physics(model)),
           solver.handle, "nk_set_physics")
     (; H, s, φᵘ, φᵛ, N, ice_mask) = model.geometry
@@ -188,10 +214,11 @@ update_velocities!(model, solver::HIPStressBalanceSolver) = update_velocities!(m
 
 # Reverse pass of a converged SSA or DIVA velocity solve (the fixed-point rule of ext/NunatakEnzymeExt.jl:175-240): given the
 # cost gradient with respect to the velocities (ū, v̄) and, for DIVA, the layer velocities (ūσ, v̄σ, pulled back through
-# compute_velocity_profile!), accumulate ∂J/∂τ into (τ̄ᵘ, τ̄ᵛ), ∂J/∂C into C̄ and ∂J/∂A into Ā. The handle must still
-# hold the converged solution of the forward call.
+# compute_velocity_profile!), accumulate ∂J/∂τ into (τ̄ᵘ, τ̄ᵛ), ∂J/∂C into C̄, ∂J/∂A into Ā, and ∂J/∂H, ∂J/∂b into H̄, b̄
+# (through the driving stress, the geometry diagnostics s, μ, φ, N of update_geometry!, the H μ factor of η̄H and the
+# drag). Any output may be `nothing`. The handle must still hold the converged solution of the forward call.
 function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalanceSolver, ū, v̄; ūσ = nothing, v̄σ = nothing,
-                               control = nothing)
+                               H̄ = nothing, b̄ = nothing, control = nothing)
     set_field!(solver, UBAR, ū); set_field!(solver, VBAR, v̄)
     for (id, f) in ((USIGMABAR, ūσ), (VSIGMABAR, v̄σ))
         f === nothing ? check(ccall((:nk_fill_field, libnunatak_hip), Cint, (Ptr{Cvoid}, Cint, Float64), solver.handle, id, 0.0),
@@ -201,7 +228,7 @@ function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalan
     c = control === nothing ? C_NULL : Ref(NunatakHIP.control(control))
     check(ccall((:nk_adjoint, libnunatak_hip), Cint, (Ptr{Cvoid}, Ptr{NKControl}, Ref{NKInfo}), solver.handle, c, info),
           solver.handle, "nk_adjoint")
-    for (f, id) in ((τ̄ᵘ, LAMBDAU), (τ̄ᵛ, LAMBDAV), (C̄, CBAR), (Ā, ABAR))
+    for (f, id) in ((τ̄ᵘ, LAMBDAU), (τ̄ᵛ, LAMBDAV), (C̄, CBAR), (Ā, ABAR), (H̄, HBAR), (b̄, BEDBAR))
         f === nothing && continue
         g = similar(f); get_field!(g, solver, id)
         interior(f) .+= interior(g)

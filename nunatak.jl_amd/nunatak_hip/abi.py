@@ -28,7 +28,7 @@ FIELDS = ["U", "V", "H", "BED", "S", "PHI", "PHIU", "PHIV", "N", "MU", "TAUU", "
           "BETAU", "BETAV", "F2", "ETASIGMA", "BETABU", "BETABV", "UB", "VB", "USIGMA", "VSIGMA", "A", "C",
           "TEMPERATURE", "PU", "PV", "QU", "QV", "UBAR", "VBAR", "LAMBDAU", "LAMBDAV", "CBAR", "ABAR", "USIGMABAR",
           "VSIGMABAR", "ENTHALPY", "WATER_FRACTION", "SMB", "BASAL_MELT", "GROUNDED_MELT", "HPREV", "GH", "HCLIP",
-          "HW", "HN", "GEOTHERMAL", "TSURF", "TOCEAN", "STRAIN_HEATING"]
+          "HW", "HN", "GEOTHERMAL", "TSURF", "TOCEAN", "STRAIN_HEATING", "HBAR", "BEDBAR"]
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 
 # (x location, y location, on the σ grid); "f" = Face, "c" = Center.
@@ -48,6 +48,7 @@ LOCATIONS = {
     "GROUNDED_MELT": ("c", "c", False), "HPREV": ("c", "c", False), "GH": ("c", "c", False), "HCLIP": ("c", "c", False),
     "HW": ("c", "c", False), "HN": ("c", "c", False), "GEOTHERMAL": ("c", "c", False), "TSURF": ("c", "c", False),
     "TOCEAN": ("c", "c", False), "STRAIN_HEATING": ("c", "c", True),
+    "HBAR": ("c", "c", False), "BEDBAR": ("c", "c", False),
 }
 ADVECTION_UPWIND1, ADVECTION_CENTERED2 = 0, 1
 

@@ -58,6 +58,7 @@ static const nko_loc LOCS[NK_FIELD_COUNT] = {
     [NK_FIELD_HPREV] = {C, C, 0},  [NK_FIELD_GH] = {C, C, 0},       [NK_FIELD_HCLIP] = {C, C, 0},
     [NK_FIELD_HW] = {C, C, 0},     [NK_FIELD_HN] = {C, C, 0},       [NK_FIELD_GEOTHERMAL] = {C, C, 0},
     [NK_FIELD_TSURF] = {C, C, 0},  [NK_FIELD_TOCEAN] = {C, C, 0},   [NK_FIELD_STRAIN_HEATING] = {C, C, 1},
+    [NK_FIELD_HBAR] = {C, C, 0},   [NK_FIELD_BEDBAR] = {C, C, 0},
 };
 
 /* ------------------------------------------------------------------------------------------------------------ */
@@ -126,31 +127,31 @@ static double jmin(double x, double y) { double d = x - y; return (isnan(x) || i
 static double jclamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
 
 /* ------------------------------------------------------------------------------------------------------------ */
-/* SSA operator (src/StressBalances/ssa_operator.jl:47-121)                                                      */
+/* SSA operator (src/StressBalances/ssa_operator.jl:47-89, strain_rates.jl)                                                      */
 /* ------------------------------------------------------------------------------------------------------------ */
 
 /* ice_freeᶠᶜᶜ / ice_freeᶜᶠᶜ (src/Models/smooth_ice_mask.jl:72-75); μ or η̄H may be absent (NULL). */
 static int ice_free_u(const og *g, int i, int j, const double *m) { return m && (AT(g, m, i - 1, j) + AT(g, m, i, j) == 0.0); }
 static int ice_free_v(const og *g, int i, int j, const double *m) { return m && (AT(g, m, i, j - 1) + AT(g, m, i, j) == 0.0); }
 
-/* inactiveᶠᶜᶜ / inactiveᶜᶠᶜ (ssa_operator.jl:91-92) */
+/* inactiveᶠᶜᶜ / inactiveᶜᶠᶜ (strain_rates.jl:2-3) */
 static int inactive_u(const og *g, int i, int j, const double *m) { return periph(g, i, j, F, C) | ice_free_u(g, i, j, m); }
 static int inactive_v(const og *g, int i, int j, const double *m) { return periph(g, i, j, C, F) | ice_free_v(g, i, j, m); }
 
-/* interior_u / interior_v (ssa_operator.jl:94-95) */
+/* interior_u / interior_v (strain_rates.jl:5-6) */
 static double interior_u(const og *g, int i, int j, const double *u, const double *m) { return inactive_u(g, i, j, m) ? 0.0 : AT(g, u, i, j); }
 static double interior_v(const og *g, int i, int j, const double *v, const double *m) { return inactive_v(g, i, j, m) ? 0.0 : AT(g, v, i, j); }
 
-/* strain_rate_xx / yy (ssa_operator.jl:97-98): ∂xᶜᶜᶜ = δxᶜ / Δxᶜᶜᶜ */
+/* strain_rate_xx / yy (strain_rates.jl:8-9): ∂xᶜᶜᶜ = δxᶜ / Δxᶜᶜᶜ */
 static double strain_xx(const og *g, int i, int j, const double *u, const double *m) {
     return (interior_u(g, i + 1, j, u, m) - interior_u(g, i, j, u, m)) / g->dx;
 }
 static double strain_yy(const og *g, int i, int j, const double *v, const double *m) {
     return (interior_v(g, i, j + 1, v, m) - interior_v(g, i, j, v, m)) / g->dy;
 }
-/* wall_shear_factor (ssa_operator.jl:100-101) */
+/* wall_shear_factor (strain_rates.jl:11-12) */
 static double wall_factor(int walls) { return walls == NK_NO_SLIP ? 2.0 : 0.0; }
-/* strain_rate_xy (ssa_operator.jl:103-106) */
+/* strain_rate_xy (strain_rates.jl:14-17) */
 static double strain_xy(const og *g, int i, int j, const double *u, const double *v, int walls, const double *m) {
     double e = ((interior_u(g, i, j, u, m) - interior_u(g, i, j - 1, u, m)) / g->dy +
                 (interior_v(g, i, j, v, m) - interior_v(g, i - 1, j, v, m)) / g->dx) / 2.0;
@@ -196,7 +197,7 @@ static void ssa_operator(const og *g, double *qu, double *qv, const double *pu, 
         }
 }
 
-/* effective_strain_rate² (ssa_operator.jl:116-121): strain rates without η̄H (walls only); ℑxyᶜᶜ of ε̇xy². */
+/* effective_strain_rate² (strain_rates.jl:27-32): strain rates without η̄H (walls only); ℑxyᶜᶜ of ε̇xy². */
 static double eff_strain2(const og *g, int i, int j, const double *u, const double *v, int walls) {
     double exx = strain_xx(g, i, j, u, NULL), eyy = strain_yy(g, i, j, v, NULL);
     double a = strain_xy(g, i, j, u, v, walls, NULL), b = strain_xy(g, i + 1, j, u, v, walls, NULL);
@@ -467,7 +468,7 @@ static double prolongate_at(const og *fg, const og *cg, int i, int j, int lx, in
            wx2 * (wy1 * phi_eval(cg, I2, J1, p) + wy2 * phi_eval(cg, I2, J2, p));
 }
 
-/* _interpolate_to_corners! (src/StressBalances/stress_balance_coefficients.jl:80-83): ℑxyᶠᶠ = ℑy∘ℑx. */
+/* _interpolate_to_corners! (src/StressBalances/stress_balance_coefficients.jl:26-29): ℑxyᶠᶠ = ℑy∘ℑx. */
 static void interpolate_to_corners(const og *g, double *etaff, const double *eta) {
     #pragma omp parallel for schedule(static) if (OMP_BIG(g))
     for (int j = 1; j <= ny_of(g, F); j++)
@@ -606,7 +607,7 @@ typedef struct nko_model {
     double *f[NK_FIELD_COUNT];
     uint8_t *class_u, *class_v;
     double *zf, *zc, *dz, *w2;  /* σ faces (1..Ns+1), centers, layer thickness, layer_weight(k, 2) */
-    /* CG workspace (src/Solvers/conjugate_gradient.jl:193-243) and Picard previous velocity */
+    /* CG workspace (src/Solvers/conjugate_gradient.jl:93-116) and Picard previous velocity */
     double *ru, *rv, *zu, *zv, *pu, *pv, *qu, *qv, *w, *xmu, *xmv;
     nko_mg *mg;
     char err[256];
@@ -618,7 +619,7 @@ static int nz_of(const nko_model *m, int id) {
     return 1;
 }
 
-/* sigma_faces (src/Grids/sigma_grid.jl:190-192) and the σ-grid metrics used by column_integrals.jl. Julia's
+/* sigma_faces (src/Grids/sigma_grid.jl:20-22) and the σ-grid metrics used by column_integrals.jl. Julia's
  * Float64^Int is correctly rounded up to its compensated power; long double reproduces that. */
 static double jpow_int(double x, int n) {
     long double r = 1.0L;
@@ -782,7 +783,7 @@ static double grounded_fraction_cc(const og *g, int i, int j, const double *H, c
 }
 
 /* update_geometry! (src/Models/ice_sheet_model_methods.jl:103-111) with compute_surface_elevation!,
- * compute_grounded_fractions!, compute_effective_pressure! (subgrid_grounded_fraction.jl:234-357) and
+ * compute_grounded_fractions!, compute_effective_pressure! (subgrid_grounded_fraction.jl:109-232) and
  * compute_ice_mask! (smooth_ice_mask.jl:60-69). */
 EXPORT void nko_update_geometry(nko_model *m) {
     const og *g = &m->g;
@@ -911,7 +912,7 @@ static double rate_factor(const nko_model *m, int i, int j, int k) {
     return AT(g, A, i, j);
 }
 
-/* _compute_viscosity! (src/Rheologies/glen_flow_law.jl:185-189) over the σ grid :xyz */
+/* _compute_viscosity! (src/Rheologies/glen_flow_law.jl:107-111) over the σ grid :xyz */
 static void compute_viscosity(nko_model *m, const double *u, const double *v) {
     const og *g = &m->g;
     double n = m->ph.glen_exponent, e0 = m->ph.strain_rate_regularization;
@@ -925,7 +926,7 @@ static void compute_viscosity(nko_model *m, const double *u, const double *v) {
             }
 }
 
-/* _compute_shear_viscosity! (glen_flow_law.jl:212-221): the DIVA vertical shear with the lagged ησ */
+/* _compute_shear_viscosity! (src/Rheologies/glen_flow_law.jl:134-143): the DIVA vertical shear with the lagged ησ */
 static void compute_shear_viscosity(nko_model *m, const double *u, const double *v) {
     const og *g = &m->g;
     double n = m->ph.glen_exponent, e0 = m->ph.strain_rate_regularization;
@@ -948,7 +949,7 @@ static void compute_shear_viscosity(nko_model *m, const double *u, const double 
             }
 }
 
-/* _compute_column_integrals! (src/StressBalances/column_integrals.jl:201-212) */
+/* _compute_column_integrals! (src/StressBalances/column_integrals.jl:61-72) */
 static void compute_column_integrals(nko_model *m) {
     const og *g = &m->g;
     const double *es = m->f[NK_FIELD_ETASIGMA], *H = m->f[NK_FIELD_H];
@@ -970,14 +971,14 @@ static void compute_column_integrals(nko_model *m) {
     fill_field_halo(m, NK_FIELD_F2);
 }
 
-/* basal_drag (src/Frictions/frictions.jl:304-310) */
+/* basal_drag (src/Frictions/frictions.jl:83-89) */
 EXPORT double nko_basal_drag(double U, double Cf, double Nf, int law, double m_, double alpha2) {
     if (law == NK_FRICTION_WEERTMAN) return Cf * pow(U, m_ - 1.0);
     double tc = alpha2 * Nf;
     return Cf * pow(U, m_ - 1.0) * tc / pow(pow(Cf, 1.0 / m_) * U + pow(tc, 1.0 / m_), m_);
 }
 
-/* compute_basal_drag! (frictions.jl:348-381): β at velocity points from (u, v). */
+/* compute_basal_drag! (src/Frictions/frictions.jl:127-160): β at velocity points from (u, v). */
 static void compute_basal_drag(nko_model *m, double *bu, double *bv, const double *u, const double *v) {
     const nk_physics *ph = &m->ph;
     if (ph->friction == NK_FRICTION_NONE) return;
@@ -1031,7 +1032,7 @@ static void compute_basal_velocities(nko_model *m) {
     fill_field_halo(m, NK_FIELD_VB);
 }
 
-/* update_stress_balance_coefficients! (stress_balance_coefficients.jl:64-74) and update_diva_coefficients!
+/* update_stress_balance_coefficients! (src/StressBalances/stress_balance_coefficients.jl:10-20) and update_diva_coefficients!
  * (diva.jl:61-75), at the velocity (U, V). */
 EXPORT void nko_update_coefficients(nko_model *m) {
     const og *g = &m->g;
@@ -1108,7 +1109,7 @@ EXPORT void nko_precondition(nko_model *m) {
     precondition(m, m->f[NK_FIELD_QU], m->f[NK_FIELD_QV], m->f[NK_FIELD_PU], m->f[NK_FIELD_PV]);
 }
 
-/* conjugate_gradient! and the step (src/Solvers/conjugate_gradient.jl:299-364). Returns the iteration count; the
+/* conjugate_gradient! and the step (src/Solvers/conjugate_gradient.jl:158-223). Returns the iteration count; the
  * final residual norm goes to *rnorm. */
 static int conjugate_gradient(nko_model *m, double *xu, double *xv, const double *bu_, const double *bv_, double *rnorm) {
     const og *g = &m->g;
@@ -1208,7 +1209,7 @@ static void post_process(nko_model *m) {
     compute_basal_velocities(m);
     const double *es = m->f[NK_FIELD_ETASIGMA], *H = m->f[NK_FIELD_H];
     const double *bbu = m->f[NK_FIELD_BETABU], *bbv = m->f[NK_FIELD_BETABV];
-    /* shear_integral (column_integrals.jl:182) with profile_weight (:149-154), p = 1 */
+    /* shear_integral (src/StressBalances/column_integrals.jl:42) with profile_weight (:149-154), p = 1 */
     double *S = calloc(g->plane, sizeof(double));
     for (int k = 1; k <= Ns; k++) {
         for (int j = 1 - g->H; j <= g->Ny + 1 + g->H; j++)
@@ -1259,7 +1260,7 @@ EXPORT int nko_multigrid_levels(const nko_model *m, int32_t *shapes, int max_lev
 }
 
 /* Transfers between the finest grid and the first coarse level, after coarsen_coefficients!, for the
- * restriction = prolongationᵀ check of test/test_multigrid_preconditioner.jl:219-226. Arrays are interior-packed
+ * restriction = prolongationᵀ check of test/test_multigrid_preconditioner.jl:40-51. Arrays are interior-packed
  * at the respective grids' velocity shapes. */
 static void copy_in(const og *g, double *f, const double *a, int lx, int ly) {
     int nx = nx_of(g, lx), ny = ny_of(g, ly);
@@ -1309,7 +1310,7 @@ EXPORT double nko_arrhenius_rate_factor(double T, double omega) {
     double soft = 1.0 + C_WATER * jmin(omega, W_MAX);
     return exp((1.0 - s) * lc + s * lw) * soft;
 }
-static double pm_temperature(double p) { return E_T0 - E_BETA * p; }                 /* enthalpy_conversions.jl:116 */
+static double pm_temperature(double p) { return E_T0 - E_BETA * p; }                 /* enthalpy_conversions.jl:6 */
 static double pm_enthalpy(double p) { return E_CI * (pm_temperature(p) - E_TR); }    /* :123 */
 static double e_temperature(double E, double p) { return jmin(pm_temperature(p), E_TR + E / E_CI); }  /* :145 */
 static double e_water(double E, double p) { return jmax(0.0, E - pm_enthalpy(p)) / E_L; }            /* :152 */
@@ -1326,7 +1327,7 @@ EXPORT void nko_rate_factor_from_temperature(nko_model *m) {
         for (int j = 1; j <= g->Ny; j++)
             for (int i = 1; i <= g->Nx; i++) {
                 size_t o = (size_t)(k - 1) * g->plane + IDX(g, i, j);
-                double p = rg * AT(g, H, i, j) * (1.0 - m->zc[k]);   /* layer_pressure (enthalpy_conversions.jl:171-174) */
+                double p = rg * AT(g, H, i, j) * (1.0 - m->zc[k]);   /* layer_pressure (enthalpy_conversions.jl:61-64) */
                 double E = e_enthalpy(T[o], 0.0, p);
                 m->f[NK_FIELD_ENTHALPY][o] = E;
                 double Tk = e_temperature(E, p), w = e_water(E, p);
@@ -1348,7 +1349,7 @@ EXPORT void nko_thermal_diagnostics(nko_model *m) {
         for (int j = 1; j <= g->Ny; j++)
             for (int i = 1; i <= g->Nx; i++) {
                 size_t o = (size_t)(k - 1) * g->plane + IDX(g, i, j);
-                double p = rg * AT(g, H, i, j) * (1.0 - m->zc[k]);   /* layer_pressure (enthalpy_conversions.jl:171-174) */
+                double p = rg * AT(g, H, i, j) * (1.0 - m->zc[k]);   /* layer_pressure (enthalpy_conversions.jl:61-64) */
                 T[o] = e_temperature(E[o], p);
                 W[o] = e_water(E[o], p);
                 A[o] = nko_arrhenius_rate_factor(T[o] - pm_temperature(p) + E_T0, W[o]);
@@ -1358,7 +1359,7 @@ EXPORT void nko_thermal_diagnostics(nko_model *m) {
     fill_field_halo(m, NK_FIELD_A);
 }
 
-/* enthalpy (enthalpy_conversions.jl:159-162) and its inverses at pressure p, for tests */
+/* enthalpy (enthalpy_conversions.jl:49-52) and its inverses at pressure p, for tests */
 EXPORT void nko_enthalpy_conversions(double T, double w, double p, double *E, double *T_of_E, double *w_of_E) {
     *E = e_enthalpy(T, w, p);
     *T_of_E = e_temperature(*E, p);
@@ -1433,7 +1434,7 @@ EXPORT double nko_cell_advection_timescale(const nko_model *m) {
 /* EnthalpyParameters defaults not above (enthalpy_thermodynamics.jl:151-166) */
 static const double E_K = 2.1, E_RATIO = 1e-3, E_DE = 1.0, E_HWB = 1e-3, E_HMIN = 1.0, E_WD = 0.01, E_RMAX = 1.58443823077064e-9;
 
-static double unit_ramp(double x, double w) { return jclamp(x / w, 0.0, 1.0); }                 /* enthalpy_conversions.jl:177 */
+static double unit_ramp(double x, double w) { return jclamp(x / w, 0.0, 1.0); }                 /* enthalpy_conversions.jl:69 */
 static double col_thickness(double H) { return jmax(H, E_HMIN); }                              /* :166 */
 /* layer_diffusivity (:180-188) at enthalpy E and pressure p */
 static double layer_diffusivity(double E, double p) {
@@ -1507,7 +1508,7 @@ EXPORT void nko_energy_step(nko_model *m, double dt, int coupled, const nk_till 
     double *c = malloc(sizeof(double) * (size_t)(Ns + 1)), *f = malloc(sizeof(double) * (size_t)(Ns + 1)), *t = calloc((size_t)(Ns + 1), sizeof(double));
     double *x = malloc(sizeof(double) * (size_t)(Ns + 1));
 #define EK(i, j, k) E[(size_t)((k) - 1) * g->plane + IDX(g, i, j)]
-#define LP(i, j, k) (rg * AT(g, H, i, j) * (1.0 - m->zc[k]))   /* layer_pressure (enthalpy_conversions.jl:171-174) */
+#define LP(i, j, k) (rg * AT(g, H, i, j) * (1.0 - m->zc[k]))   /* layer_pressure (enthalpy_conversions.jl:61-64) */
     for (int j = 1; j <= g->Ny; j++)
         for (int i = 1; i <= g->Nx; i++) {
             double Hc = AT(g, H, i, j), Ht = col_thickness(Hc), Htn = col_thickness(AT(g, Hn, i, j));

@@ -39,12 +39,12 @@ def test_defaults_match_reference_constructors():
     L.nk_default_options(ct.byref(o))
     L.nk_default_grid(ct.byref(g))
     # IceSheetConstants (src/Models/ice_sheet_constants.jl:33), GlenFlowLaw (src/Rheologies/glen_flow_law.jl:114),
-    # WeertmanFriction (src/Frictions/frictions.jl:255), SmoothIceMask (src/Models/smooth_ice_mask.jl:41)
+    # WeertmanFriction (src/Frictions/frictions.jl:34), SmoothIceMask (src/Models/smooth_ice_mask.jl:41)
     assert (p.ice_density, p.ocean_density, p.gravitational_acceleration) == (918.0, 1028.0, 9.81)
     assert (p.glen_exponent, p.strain_rate_regularization) == (3.0, 1e-13)
     assert (p.friction_exponent, p.velocity_regularization) == (1 / 3, 1e-8)
     assert (p.mask_minimum_thickness, p.mask_transition_thickness) == (1.0, 1.0)
-    # StressBalanceSolver (src/StressBalances/stress_balance_solver.jl:126-131), multigrid (multigrid_preconditioner.jl:57)
+    # StressBalanceSolver (src/StressBalances/stress_balance_solver.jl:43-52), multigrid (multigrid_preconditioner.jl:57)
     assert (o.linear.reltol, o.linear.maxiter, o.picard.reltol, o.picard.maxiter) == (1e-8, 10_000, 1e-6, 100)
     assert o.preconditioner == abi.PRECONDITIONER_DIAGONAL
     assert (o.smoother, o.smoothing_sweeps, o.coarsest_sweeps, o.minimum_coarse_size) == (abi.POINT_JACOBI, 2, 20, 2)

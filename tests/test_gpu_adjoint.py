@@ -375,3 +375,110 @@ def test_friction_inversion_gradient_along_random_directions(N, chunk):
         worst = max(worst, rel)
         say(f"[adjoint] direction {k}: adjoint {ad:.10e}, fd {fd:.10e}, rel {rel:.2e}")
     assert worst < 1e-8
+
+
+# ---- geometry gradients: ∂J/∂H, ∂J/∂b -------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("preconditioner", [abi.PRECONDITIONER_DIAGONAL, abi.PRECONDITIONER_MULTIGRID])
+def test_thickness_gradient_matches_finite_differences(preconditioner):
+    """∂J/∂H₀ of the reference's ridge model through the converged SSA Picard solve with the geometry on the device
+    (test/test_enzyme.jl:341-356: cells (6, 2) and (10, 3), centred differences with h = 1e-4 H, bar 1e-5)."""
+    say(f"[adjoint] ridge 16x4 thickness gradient, preconditioner {preconditioner}")
+    m, g = ridge(preconditioner)
+    adjoint_of_cost(m, g)
+    Hb = m.get("HBAR").copy()
+    H = m.get("H")
+    for index in ((1, 5), (2, 9)):   # the reference's cells, its step and bar
+        fd = centred_difference(m, g, "H", index, 1e-4 * H[index])
+        say(f"[adjoint] dJ/dH{index}: adjoint {Hb[index]:.10e}, fd {fd:.10e}")
+        assert abs(Hb[index] - fd) < 1e-5 * abs(fd)
+    # cells at the Bounded walls too, Richardson-extrapolated; bar 1e-7 of the gradient's largest entry (a cell whose
+    # own gradient is 10³ smaller sits at the cost's rounding level in the differences)
+    scale = np.max(np.abs(Hb))
+    for index in ((0, 0), (3, 15), (1, 5)):
+        h = 1e-3 * H[index]
+        fd = (4 * centred_difference(m, g, "H", index, h / 2) - centred_difference(m, g, "H", index, h)) / 3
+        say(f"[adjoint] dJ/dH{index} (Richardson): adjoint {Hb[index]:.10e}, fd {fd:.10e}")
+        assert abs(Hb[index] - fd) < 1e-7 * scale
+
+
+def test_thickness_gradient_multigrid_equals_diagonal():
+    out = []
+    for pre in (abi.PRECONDITIONER_DIAGONAL, abi.PRECONDITIONER_MULTIGRID):
+        m, g = ridge(pre)
+        adjoint_of_cost(m, g)
+        out.append(m.get("HBAR").copy())
+    rel = np.max(np.abs(out[1] - out[0])) / np.max(np.abs(out[0]))
+    say(f"[adjoint] H̄ multigrid vs diagonal: {rel:.2e}")
+    assert rel < 1e-8
+
+
+def marine_geometry(N, grounding, friction, diva=False):
+    """The friction-inversion geometry (grounded upstream, subgrid grounding line, floating shelf, calving front at
+    the last column) at N × N, with the given grounding treatment and friction law."""
+    g, phys, opts, fields, _ = W.friction_inversion(N=N, dx=512e3 / N)
+    phys.grounding = grounding
+    phys.friction = friction
+    if friction == abi.FRICTION_REGULARIZED_COULOMB:
+        phys.coulomb_coefficient, phys.velocity_regularization = 0.5, 300.0 / W.YEAR
+        fields = dict(fields, C=np.full_like(fields["C"], 3e4))
+    if diva:
+        phys.stress_balance = abi.DIVA
+    opts.picard = abi.convergence(1e-12, 0.0, 500)
+    opts.linear = abi.convergence(1e-13, 0.0, 20_000)
+    m = HIPModel(g, phys, opts)
+    W.load_into(m, fields)
+    return m, g, fields
+
+
+def speed_cost(m, g, fields, H, B):
+    m.set("H", H)
+    m.set("BED", B)
+    for name in ("U", "V", "ETASIGMA", "F2", "BETABU", "BETABV", "UB", "VB"):
+        m.fill(name, 0.0)
+    m.update_velocities(geometry=True)
+    u, v = m.get("U"), m.get("V")
+    return float(np.sum(u[: g.Ny, : g.Nx] ** 2) + np.sum(v[: g.Ny, : g.Nx] ** 2)) * W.YEAR ** 2
+
+
+@pytest.mark.parametrize("case", ["subgrid-weertman", "gl-split-coulomb", "gl-split-weertman-diva"])
+def test_geometry_gradients_along_random_directions(case):
+    """∂J/∂H · d and ∂J/∂b · d on the marine geometry (masks, calving-front force, subgrid grounding line with and
+    without the split driving stress, Weertman and regularized Coulomb drag with φ and N), SSA and DIVA, against
+    Richardson-extrapolated centred differences along 3 seeded smooth directions each; d is zero on the open-ocean
+    column so the ice mask does not move. Bar: 1e-6 relative."""
+    grounding = abi.GROUNDING_SUBGRID if case.startswith("subgrid") else abi.GROUNDING_SUBGRID_DRIVING_STRESS
+    friction = abi.FRICTION_REGULARIZED_COULOMB if "coulomb" in case else abi.FRICTION_WEERTMAN
+    m, g, fields = marine_geometry(48, grounding, friction, diva=case.endswith("diva"))
+    H0, B0 = fields["H"].copy(), fields["BED"].copy()
+    say(f"[adjoint] marine geometry 48x48 {case}")
+    J0 = speed_cost(m, g, fields, H0, B0)
+    u, v = m.get("U"), m.get("V")
+    ub, vb = np.zeros_like(u), np.zeros_like(v)
+    ub[: g.Ny, : g.Nx] = 2 * u[: g.Ny, : g.Nx] * W.YEAR ** 2
+    vb[: g.Ny, : g.Nx] = 2 * v[: g.Ny, : g.Nx] * W.YEAR ** 2
+    m.set("UBAR", ub)
+    m.set("VBAR", vb)
+    info = m.adjoint()
+    Hb, Bb = m.get("HBAR").copy(), m.get("BEDBAR").copy()
+    say(f"[adjoint] J0 = {J0:.8e}, adjoint iterations {info['picard_iterations']}")
+    rng = np.random.Generator(np.random.PCG64(61))
+    ice = H0 > 0
+    worst = 0.0
+    for name, grad in (("H", Hb), ("BED", Bb)):
+        for k in range(3):
+            d = W.gaussian_random_field(rng, H0.shape, (g.dx, g.dy), 40e3, 1.0)
+            d = np.where(ice, d / np.max(np.abs(d)), 0.0)
+
+            def D(h):
+                Hp, Bp = (H0 + h * d, B0) if name == "H" else (H0, B0 + h * d)
+                Hm, Bm = (H0 - h * d, B0) if name == "H" else (H0, B0 - h * d)
+                return (speed_cost(m, g, fields, Hp, Bp) - speed_cost(m, g, fields, Hm, Bm)) / (2 * h)
+
+            h = 1e-2
+            fd = (4 * D(h / 2) - D(h)) / 3
+            ad = float(np.sum(grad * d))
+            rel = abs(ad - fd) / abs(fd)
+            worst = max(worst, rel)
+            say(f"[adjoint] {case} d{name} direction {k}: adjoint {ad:.10e}, fd {fd:.10e}, rel {rel:.2e}")
+    assert worst < 1e-6

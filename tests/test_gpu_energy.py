@@ -119,7 +119,7 @@ def test_uniform_enthalpy_stays_uniform_on_the_device():
 
 
 def test_till_pressure_and_drainage_on_the_device():
-    """test/test_till_effective_pressure.jl:54-80 through the device, and nk_till_effective_pressure against the
+    """test/test_till_effective_pressure.jl:54-79 through the device, and nk_till_effective_pressure against the
     oracle on a random sheet (device pow: 1e-13)."""
     say("energy: till")
     TE.test_the_till_drains_and_is_capped_and_the_model_sees_the_till_pressure(cls=HIPModel)

@@ -87,14 +87,14 @@ def test_effective_pressure_and_surface():
     assert np.allclose(N[:4], ri * gr * 1000, rtol=1e-14)
     assert np.allclose(N[4:8], ri * gr * 1000 - rw * gr * 500, rtol=1e-14)
     assert np.all(N[8:] == 0)
-    # doctest subgrid_grounded_fraction.jl:285-290: N = [10, 5]
+    # doctest subgrid_grounded_fraction.jl:149-166: N = [10, 5]
     g2 = grid(2, 1, Lx=2, Ly=1, topology=(B, P))
     m2 = model(g2, ice_density=1.0, ocean_density=1.0, gravitational_acceleration=1.0)
     set_field(m2, "H", 10)
     set_field(m2, "BED", lambda x, y: np.where(x < 1, 5.0, -5.0))
     m2.update_geometry()
     assert list(m2.get("N")[0]) == [10.0, 5.0]
-    # doctest subgrid_grounded_fraction.jl:328-333: s = [80, 50]
+    # doctest subgrid_grounded_fraction.jl:192-209: s = [80, 50]
     m3 = model(g2, grounding=abi.GROUNDING_SUBGRID, ice_density=1.0, ocean_density=2.0)
     set_field(m3, "H", 100)
     set_field(m3, "BED", lambda x, y: np.where(x < 1, -20.0, -500.0))
@@ -183,7 +183,7 @@ def test_grounding_line_split(Nx, Ny):
     assert [face(t, k) for k in others] == [face(t0, k) for k in others]
 
 
-# ---- viscosity and column integrals: test/test_glen_viscosity.jl:3-55, glen_flow_law.jl:169-172, test_sigma_grid -----
+# ---- viscosity and column integrals: test/test_glen_viscosity.jl:3-55, glen_flow_law.jl:78-94, test_sigma_grid -----
 
 def test_viscosity_doctest():
     g = grid(4, 4, topology=(P, P), Nsigma=3)
@@ -253,7 +253,7 @@ def test_depth_varying_rate_factor_integrals():
     assert math.isclose(m.get("F2")[2, 2], 1000 * np.sum(weights / scale) / e0, rel_tol=1e-13)
 
 
-# ---- friction: test/test_friction.jl:59-68, doctest frictions.jl:342-345 --------------------------------------------
+# ---- friction: test/test_friction.jl:59-68, doctest frictions.jl:102-125 --------------------------------------------
 
 def test_drag_doctest_and_limits():
     g = grid(4, 4, topology=(P, P))
@@ -332,7 +332,7 @@ def test_rate_factor_from_cold_temperature_is_arrhenius():
         assert Tp > T[k, 0, 0]
 
 
-# ---- update_thermal_diagnostics!: doctest enthalpy_conversions.jl:127-139, column_energy_solve.jl:85-117 ------------
+# ---- update_thermal_diagnostics!: doctest enthalpy_conversions.jl:20-33, column_energy_solve.jl:85-117 ------------
 
 def test_enthalpy_conversions_doctest_and_temperate_branch():
     # doctest: E = enthalpy(263.15, 0, 1e6) → (temperature, water_fraction) = (263.15, 0.0)

@@ -75,7 +75,7 @@ def test_operator_dense_assembly_is_symmetric_and_spd():
         A[:, col] = np.concatenate([qu[: g.Ny, : g.Nx].ravel(), qv[: g.Ny, : g.Nx].ravel()])
     assert np.allclose(A, A.T, rtol=1e-13, atol=1e-13 * np.abs(A).max())
     assert np.all(np.linalg.eigvalsh((A + A.T) / 2) > 0)
-    # the Jacobi diagonal preconditioner divides by exactly this diagonal (test/test_friction.jl:164-175)
+    # the Jacobi diagonal preconditioner divides by exactly this diagonal (test/test_friction.jl:83-94)
     ones_u, ones_v = np.ones((su[1], su[0])), np.ones((sv[1], sv[0]))
     m.set("PU", ones_u)
     m.set("PV", ones_v)
@@ -103,7 +103,7 @@ def test_free_slip_translation_feels_only_drag():
 
 
 def test_ice_free_rows_are_identity():
-    # test/test_ice_mask.jl:297-320
+    # test/test_ice_mask.jl:38-61
     Nx, Ny = 8, 6
     g = grid(Nx, Ny, topology=(B, P))
     m = model(g)
@@ -202,7 +202,7 @@ def test_no_slip_stiffer_than_free_slip():
 
 
 def test_jacobi_fewer_iterations_same_solution():
-    # test/test_friction.jl:177-191
+    # test/test_friction.jl:96-111
     g = grid(48, 32, topology=(B, P))
     b = dict(TAUU=lambda x, y: np.sin(np.pi * x) * np.cos(2 * np.pi * y), TAUV=lambda x, y: x * np.sin(2 * np.pi * y))
     res = []
@@ -244,7 +244,7 @@ def test_ice_stream_picard():
     assert r[1][0] < 1e-3 and r[0][0] / r[1][0] > 3.5
 
 
-# ---- model-level solves: test/test_calving_front.jl:23-66, test/test_ice_mask.jl:322-338, test/test_model.jl -------
+# ---- model-level solves: test/test_calving_front.jl:23-66, test/test_ice_mask.jl:63-79, test/test_model.jl -------
 
 def confined_shelf(Nx=32, L=64e3, x_front=40e3, H0=500.0, A=1e-24):
     g = grid(Nx, 4, Lx=L, Ly=4 * L / Nx)
@@ -558,7 +558,7 @@ def test_v_cycle_symmetric_positive(topology, size, smoother, walls):
 
 
 def test_multigrid_beats_jacobi_on_ice_cap():
-    # test/test_multigrid_preconditioner.jl:241-257: same coefficients, solve from zero with both preconditioners
+    # test/test_multigrid_preconditioner.jl:66-82: same coefficients, solve from zero with both preconditioners
     g, m, _ = ice_cap(Nx=128, hmin=10.0, htr=400.0)
     fields = {k: m.get(k) for k in ("ETAH", "ETAHFF", "BETAU", "BETAV", "TAUU", "TAUV")}
     out = []

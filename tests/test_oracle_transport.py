@@ -1,6 +1,6 @@
 """Thickness transport (src/Models/time_stepping.jl:1-72) in the oracle, with the stage loop of the product's
 `nunatak_hip.transport.time_step`: the properties the reference's own thickness tests assert for
-UpwindBiased(order = 1) (test/test_thickness.jl:1-139) — volume conservation, the source budget, the
+UpwindBiased(order = 1) (test/test_thickness.jl:1-113) — volume conservation, the source budget, the
 centre-of-mass speed, the clip budget, melt on the floating part only and the advection timescale — plus the
 textbook forms of the two face-thickness schemes on a uniform flow. The face-thickness schemes and the split
 Runge-Kutta coefficients come from Oceananigans (0.117), which is not in the reference tree: restated from its
