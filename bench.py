@@ -261,7 +261,7 @@ def serving_leg(n, rate_hz, slots, serial_sample):
                         "(FP32 Galerkin V-cycle), MG-PCG linear 1e-8, Picard 1e-6"}
 
 
-def transport_leg(N, steps):
+def transport_leg(N, steps, advection=abi.ADVECTION_UPWIND1):
     """Thickness transport at N² on the device: `steps` split Runge-Kutta steps (cache, then three stages of tendency
     and substep) with prescribed velocities, upwind face thickness and sources, no geometry update between stages, so
     the time is the transport kernels' (each entry point synchronises). Algorithmic bytes per step: three tendencies
@@ -283,7 +283,7 @@ def transport_leg(N, steps):
     def step():
         m.cache_thickness()
         for k, beta in enumerate((3.0, 2.0, 1.0)):
-            m.compute_thickness_tendency(abi.ADVECTION_UPWIND1)
+            m.compute_thickness_tendency(advection)
             m.thickness_substep(dt / beta, k == 2)
 
     step()
@@ -293,10 +293,14 @@ def transport_leg(N, steps):
         step()
     m.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / steps
-    bytes_step = 8.0 * N * N * (3 * 8 + 3 * 5 + 2)
-    log(f"[bench] transport {N}x{N}: {ms:.3f} ms per RK3 step, {bytes_step / (ms * 1e-3) / 1e9:.0f} GB/s (algorithmic, wall)")
+    # WENO(order = 5, bounds): the limiter pass reads H and writes θ, and the tendency reads θ too
+    bounded = advection == abi.ADVECTION_WENO5_BOUNDED
+    bytes_step = 8.0 * N * N * (3 * (8 + (3 if bounded else 0)) + 3 * 5 + 2)
+    name = {abi.ADVECTION_UPWIND1: "UpwindBiased(order = 1)", abi.ADVECTION_CENTERED2: "Centered(order = 2)",
+            abi.ADVECTION_WENO5: "WENO(order = 5)", abi.ADVECTION_WENO5_BOUNDED: "WENO(order = 5, bounds = (0, 1e5))"}[advection]
+    log(f"[bench] transport {N}x{N} {name}: {ms:.3f} ms per RK3 step, {bytes_step / (ms * 1e-3) / 1e9:.0f} GB/s (algorithmic, wall)")
     m.close()
-    return {"workload": f"thickness transport {N}x{N}, (Bounded, Periodic), UpwindBiased(order = 1), SplitRungeKutta3, "
+    return {"workload": f"thickness transport {N}x{N}, (Bounded, Periodic), {name}, SplitRungeKutta3, "
                         "prescribed velocities, sources, no geometry update", "ms_per_step": ms,
             "algorithmic_bytes_per_step": bytes_step, "GB_s_wall": bytes_step / (ms * 1e-3) / 1e9, "steps": steps}
 
@@ -388,7 +392,9 @@ def main():
         sys.exit(2)
     N, K, Wm = args.size, args.steps, args.warmup
     if args.transport_only:
-        print(json.dumps({"transport": transport_leg(N, max(args.transport, 1)), "energy": energy_leg(N, 10)}), flush=True)
+        print(json.dumps({"transport": transport_leg(N, max(args.transport, 1)),
+                          "transport_weno_bounded": transport_leg(N, max(args.transport, 1), abi.ADVECTION_WENO5_BOUNDED),
+                          "energy": energy_leg(N, 10)}), flush=True)
         return
     os.environ.setdefault("NK_VERBOSE", "1")  # one library progress line per Picard iteration
     if args.stream_curve:
@@ -555,6 +561,7 @@ def main():
         out["served_p99_ms"] = out["serving"]["p99_ms"]
     if args.transport > 0:
         out["transport"] = transport_leg(N, args.transport)
+        out["transport_weno_bounded"] = transport_leg(N, args.transport, abi.ADVECTION_WENO5_BOUNDED)
         out["energy"] = energy_leg(N, 10)
     print(json.dumps(out), flush=True)
 

@@ -312,12 +312,16 @@ nk_status nk_rate_factor_from_temperature(nk_solver *solver);
 nk_status nk_thermal_diagnostics(nk_solver *solver);
 
 /* ---- Thickness transport: the step after the velocity solve in time_step! ---------------------------------------- */
-/* Advection schemes of the thickness flux (Oceananigans UpwindBiased(order = 1) and Centered(order = 2)). */
-enum { NK_ADVECTION_UPWIND1 = 0, NK_ADVECTION_CENTERED2 = 1 };
+/* Advection schemes of the thickness flux: Oceananigans UpwindBiased(order = 1), Centered(order = 2), WENO(order = 5)
+ * and WENO(order = 5, bounds) with the bounds-preserving limiter (the reference's default for SSA and DIVA). */
+enum { NK_ADVECTION_UPWIND1 = 0, NK_ADVECTION_CENTERED2 = 1, NK_ADVECTION_WENO5 = 2, NK_ADVECTION_WENO5_BOUNDED = 3 };
 /* compute_thickness_tendency! (src/Models/time_stepping.jl:34-63): GH = a − m·(1 − φ if grounding else 1) − mᵍ
  * − ∇·(ū H) with the face thickness of the scheme; U, V are the advecting depth-averaged velocities; mᵍ
  * (GROUNDED_MELT) enters only with grounded_melt_coupling (coupled_grounded_melt, :54-55). */
 nk_status nk_compute_thickness_tendency(nk_solver *solver, int32_t advection, int32_t grounded_melt_coupling);
+/* Bounds of NK_ADVECTION_WENO5_BOUNDED (WENO(order = 5, bounds = (lower, upper)); default (0, 1e5), the reference's
+ * default thickness advection, src/Models/ice_sheet_model.jl:123). */
+nk_status nk_set_thickness_bounds(nk_solver *solver, double lower, double upper);
 /* _thickness_substep! (time_stepping.jl:65-72): H★ = HPREV + Δτ·GH; H = max(0, H★); in the final stage
  * HCLIP += max(0, −H★). */
 nk_status nk_thickness_substep(nk_solver *solver, double dtau, int32_t final_stage);

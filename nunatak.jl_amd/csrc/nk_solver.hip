@@ -120,6 +120,10 @@ struct nk_solver {
     // enthalpy step workspace, allocated by the first nk_energy_step: face volume fluxes (Nσ + 1 planes), the column
     // system a, b, c, f and the solver's t (Nσ planes each), the basal state rᵇ, sᵇ, Dᵇ, qᵍ (4 planes)
     double *en_W = nullptr, *en_a = nullptr, *en_b = nullptr, *en_c = nullptr, *en_f = nullptr, *en_t = nullptr, *en_bas = nullptr;
+    // bounds-preserving WENO thickness advection: the per-cell limiter θ (allocated by the first WENO tendency) and the
+    // bounds (the reference's default WENO(order = 5, bounds = (0, 1e5)))
+    double *weno_theta = nullptr;
+    double thick_lower = 0.0, thick_upper = 1e5;
     std::vector<Level> lev;  // lev[0] = the finest grid
     double *partials = nullptr;
     Scal *S = nullptr;
@@ -1791,16 +1795,40 @@ nk_status nk_update_velocities(nk_solver *s, int32_t geometry, nk_info *info) {
 
 nk_status nk_compute_thickness_tendency(nk_solver *s, int32_t advection, int32_t grounded_melt_coupling) {
     if (!s) return NK_INVALID_ARGUMENT;
-    if (advection != NK_ADVECTION_UPWIND1 && advection != NK_ADVECTION_CENTERED2) {
-        s->err = "thickness advection: NK_ADVECTION_UPWIND1 or NK_ADVECTION_CENTERED2";
+    if (advection < NK_ADVECTION_UPWIND1 || advection > NK_ADVECTION_WENO5_BOUNDED) {
+        s->err = "thickness advection: NK_ADVECTION_UPWIND1, _CENTERED2, _WENO5 or _WENO5_BOUNDED";
         return NK_INVALID_ARGUMENT;
     }
     double **f = s->f;
+    const double *theta = nullptr;
+    if (advection == NK_ADVECTION_WENO5_BOUNDED) {
+        if (!s->weno_theta) {
+            std::lock_guard<std::mutex> lk(g_device_mu);
+            nk_status st = alloc_plane(s, &s->weno_theta, s->g);
+            if (st != NK_OK) return st;
+        }
+        // update_bounds_preserving_limiter! before the tendency (src/Models/time_stepping.jl:37)
+        k_weno_limiter<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(s->g, f[NK_FIELD_H], s->thick_lower, s->thick_upper,
+                                                                                   s->weno_theta);
+        CHK(launch_check(s));
+        theta = s->weno_theta;
+    }
     k_thickness_tendency<<<blocks_for(s->g.Nx, s->g.Ny), THREADS, 0, s->stream>>>(
         s->g, advection, s->ph.grounding != NK_GROUNDING_NONE ? 1 : 0, grounded_melt_coupling ? 1 : 0, f[NK_FIELD_U], f[NK_FIELD_V],
-        f[NK_FIELD_H], f[NK_FIELD_SMB], f[NK_FIELD_BASAL_MELT], f[NK_FIELD_PHI], f[NK_FIELD_GROUNDED_MELT], f[NK_FIELD_GH]);
+        f[NK_FIELD_H], f[NK_FIELD_SMB], f[NK_FIELD_BASAL_MELT], f[NK_FIELD_PHI], f[NK_FIELD_GROUNDED_MELT], f[NK_FIELD_GH], theta);
     CHK(launch_check(s));
     HIPCHK(hipStreamSynchronize(s->stream));
+    return NK_OK;
+}
+
+nk_status nk_set_thickness_bounds(nk_solver *s, double lower, double upper) {
+    if (!s) return NK_INVALID_ARGUMENT;
+    if (!(lower <= upper) || std::isnan(lower) || std::isnan(upper)) {
+        s->err = "thickness bounds: lower ≤ upper";
+        return NK_INVALID_ARGUMENT;
+    }
+    s->thick_lower = lower;
+    s->thick_upper = upper;
     return NK_OK;
 }
 

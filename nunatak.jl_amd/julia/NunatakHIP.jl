@@ -236,10 +236,27 @@ function hip_velocity_adjoint!(τ̄ᵘ, τ̄ᵛ, C̄, Ā, solver::HIPStressBalan
     return info.picard_iterations
 end
 
+# The library's code of the model's thickness advection: UpwindBiased(order = 1) 0, Centered(order = 2) 1, WENO(order = 5)
+# 2, WENO(order = 5, bounds) 3 (the reference's default, src/Models/ice_sheet_model.jl:123; its bounds are passed on).
+advection_code(a) = throw(ArgumentError("thickness advection $(summary(a)) is not built in libnunatak_hip"))
+advection_code(::Oceananigans.Advection.UpwindBiased{1}) = 0
+advection_code(::Oceananigans.Advection.Centered{1}) = 1
+advection_code(::Oceananigans.Advection.WENO{3}) = 2
+advection_code(::Oceananigans.Advection.BoundsPreservingWENO{3}) = 3
+
+# (lower, upper) of a bounds-preserving scheme's bounds object (which also carries the limiter field)
+thickness_bounds(b::Tuple) = (b[1], b[2])
+thickness_bounds(b) = hasproperty(b, :lower) ? (b.lower, b.upper) : (b.bounds[1], b.bounds[2])
+
 # One split Runge-Kutta stage of the thickness on the device (rk_substep!, src/Models/time_stepping.jl:15-72) for a
-# model whose velocities are current: the tendency with UpwindBiased(order = 1) (advection = 0) or Centered(order = 2)
-# (advection = 1), then H = max(0, H⁻ + Δτ G), the clip accumulated in the final stage.
-function hip_rk_substep!(model, solver::HIPStressBalanceSolver, Δτ, final_stage; advection = 0)
+# model whose velocities are current: the tendency with the model's advection scheme (advection_code), then
+# H = max(0, H⁻ + Δτ G), the clip accumulated in the final stage.
+function hip_rk_substep!(model, solver::HIPStressBalanceSolver, Δτ, final_stage; advection = advection_code(model.advection))
+    if advection == 3
+        lower, upper = thickness_bounds(model.advection.bounds)
+        check(ccall((:nk_set_thickness_bounds, libnunatak_hip), Cint, (Ptr{Cvoid}, Float64, Float64), solver.handle, lower, upper),
+              solver.handle, "nk_set_thickness_bounds")
+    end
     geometry, vel = model.geometry, model.velocities
     set_field!(solver, NunatakHIP.H, geometry.H); set_field!(solver, HPREV, model.timestepper.Ψ⁻.H)
     set_field!(solver, U, vel.u); set_field!(solver, V, vel.v); set_field!(solver, PHI, geometry.φ)

@@ -50,7 +50,7 @@ LOCATIONS = {
     "TOCEAN": ("c", "c", False), "STRAIN_HEATING": ("c", "c", True),
     "HBAR": ("c", "c", False), "BEDBAR": ("c", "c", False),
 }
-ADVECTION_UPWIND1, ADVECTION_CENTERED2 = 0, 1
+ADVECTION_UPWIND1, ADVECTION_CENTERED2, ADVECTION_WENO5, ADVECTION_WENO5_BOUNDED = 0, 1, 2, 3
 
 
 class Grid(ct.Structure):

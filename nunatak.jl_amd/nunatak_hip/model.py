@@ -26,7 +26,7 @@ EXPORTS = ["nk_abi_version", "nk_default_grid", "nk_default_physics", "nk_defaul
            "nk_till_effective_pressure",
            "nk_rate_factor_from_temperature", "nk_multigrid_levels", "nk_adjoint", "nk_device_bytes",
            "nk_estimate_device_bytes", "nk_server_create", "nk_server_destroy", "nk_server_last_error",
-           "nk_server_submit", "nk_server_poll", "nk_server_wait", "nk_apply_operator_noalloc"]
+           "nk_server_submit", "nk_server_poll", "nk_server_wait", "nk_apply_operator_noalloc", "nk_set_thickness_bounds"]
 
 _lib = None
 
@@ -78,6 +78,7 @@ def load(path=LIB_PATH):
     L.nk_adjoint.argtypes = [P, ct.POINTER(abi.Control), ct.POINTER(abi.Info)]
     L.nk_compute_thickness_tendency.argtypes = [P, I32, I32]
     L.nk_thickness_substep.argtypes = [P, D, I32]
+    L.nk_set_thickness_bounds.argtypes = [P, D, D]
     L.nk_cell_advection_timescale.argtypes = [P, ct.POINTER(D)]
     L.nk_energy_step.argtypes = [P, D, I32, ct.POINTER(abi.Till)]
     L.nk_till_effective_pressure.argtypes = [P, ct.POINTER(abi.Till)]
@@ -203,6 +204,9 @@ class HIPModel:
     def compute_thickness_tendency(self, advection=abi.ADVECTION_UPWIND1, grounded_melt_coupling=False):
         self._check(self._L.nk_compute_thickness_tendency(self._h, advection, 1 if grounded_melt_coupling else 0),
                     "nk_compute_thickness_tendency")
+
+    def set_thickness_bounds(self, lower, upper):
+        self._check(self._L.nk_set_thickness_bounds(self._h, lower, upper), "nk_set_thickness_bounds")
 
     def thickness_substep(self, dtau, final_stage):
         self._check(self._L.nk_thickness_substep(self._h, dtau, 1 if final_stage else 0), "nk_thickness_substep")

@@ -610,6 +610,7 @@ typedef struct nko_model {
     /* CG workspace (src/Solvers/conjugate_gradient.jl:93-116) and Picard previous velocity */
     double *ru, *rv, *zu, *zv, *pu, *pv, *qu, *qv, *w, *xmu, *xmv;
     nko_mg *mg;
+    double weno_bounds[2];  /* WENO(order = 5, bounds): (0, 1e5) unless set */
     char err[256];
 } nko_model;
 
@@ -638,6 +639,7 @@ static double power_integral(double zm, double zp, int p) { return (jpow_int(1.0
 EXPORT nko_model *nko_create(const nk_grid *grid, const nk_physics *ph, const nk_solver_options *op) {
     nko_model *m = calloc(1, sizeof(nko_model));
     m->grid = *grid; m->ph = *ph; m->op = *op;
+    m->weno_bounds[0] = 0.0; m->weno_bounds[1] = 1e5;
     og_init(&m->g, grid->Nx, grid->Ny, grid->topology_x, grid->topology_y, grid->dx, grid->dy);
     m->Ns = grid->Nsigma;
     for (int id = 0; id < NK_FIELD_COUNT; id++)
@@ -1375,25 +1377,126 @@ EXPORT void nko_enthalpy_conversions(double T, double w, double p, double *E, do
  * Centered(order = 2) the mean of the two cells (halo cells hold the zero-gradient / periodic fill) */
 static double face_H(double left, double right, double vel, int adv) { return adv == 0 ? (vel > 0.0 ? left : right) : (left + right) / 2.0; }
 
-/* compute_thickness_tendency! / _compute_thickness_tendency! (time_stepping.jl:34-63) */
+/* WENO(order = 5) face thickness and the bounds-preserving limiter of WENO(order = 5, bounds) — the reference's
+ * default thickness advection (src/Models/ice_sheet_model.jl:123; limiter updated before each tendency,
+ * src/Models/time_stepping.jl:37; face values rescaled per cell as in src/Models/partial_cell_front.jl:139-151).
+ * The scheme is Oceananigans' (0.117, outside the reference tree), restated from its published form: Jiang & Shu
+ * (1996) stencils and smoothness indicators with WENO-Z weights (Borges et al. 2008, αₖ = dₖ(1 + τ/(βₖ + ε)),
+ * ε = 1e-8), third order and then first-order upwind where a stencil would leave a Bounded direction, and the
+ * Zhang & Shu (2010) scaling limiter with ω̂ = 5/18, one θ per cell over both directions. Parity with Oceananigans'
+ * own operation order and boundary rule is unpinned; the reference's transport tests and the scheme's order pin it. */
+#define WENO_EPS 1e-8
+#define WENO_OMEGA (5.0 / 18.0)
+static double weno5z(double a, double b, double c, double d, double e) {
+    double q0 = (2.0 * a - 7.0 * b + 11.0 * c) / 6.0;
+    double q1 = (-b + 5.0 * c + 2.0 * d) / 6.0;
+    double q2 = (2.0 * c + 5.0 * d - e) / 6.0;
+    double s0 = a - 2.0 * b + c, t0 = a - 4.0 * b + 3.0 * c;
+    double s1 = b - 2.0 * c + d, t1 = b - d;
+    double s2 = c - 2.0 * d + e, t2 = 3.0 * c - 4.0 * d + e;
+    double b0 = 13.0 / 12.0 * (s0 * s0) + 0.25 * (t0 * t0);
+    double b1 = 13.0 / 12.0 * (s1 * s1) + 0.25 * (t1 * t1);
+    double b2 = 13.0 / 12.0 * (s2 * s2) + 0.25 * (t2 * t2);
+    double tau = fabs(b0 - b2);
+    double r0 = tau / (b0 + WENO_EPS), r1 = tau / (b1 + WENO_EPS), r2 = tau / (b2 + WENO_EPS);
+    double a0 = 0.1 * (1.0 + r0), a1 = 0.6 * (1.0 + r1), a2 = 0.3 * (1.0 + r2);
+    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+static double weno3z(double a, double b, double c) {
+    double q0 = (-a + 3.0 * b) / 2.0, q1 = (b + c) / 2.0;
+    double b0 = (b - a) * (b - a), b1 = (c - b) * (c - b);
+    double tau = fabs(b0 - b1);
+    double r0 = tau / (b0 + WENO_EPS), r1 = tau / (b1 + WENO_EPS);
+    double a0 = (1.0 / 3.0) * (1.0 + r0), a1 = (2.0 / 3.0) * (1.0 + r1);
+    return (a0 * q0 + a1 * q1) / (a0 + a1);
+}
+EXPORT double nko_weno5z(double a, double b, double c, double d, double e) { return weno5z(a, b, c, d, e); }
+EXPORT double nko_weno3z(double a, double b, double c) { return weno3z(a, b, c); }
+static int weno_order(int k, int N, int bounded) {
+    if (!bounded) return 5;
+    if (k >= 4 && k <= N - 2) return 5;
+    if (k >= 3 && k <= N - 1) return 3;
+    return 1;
+}
+/* reconstruction at face (i, j) of direction dir from the upwind cell below the face (left) or at it (right); the
+ * halo (3 cells) holds the clamped or wrapped values */
+static double weno_face(const og *g, const double *H, int dir, int i, int j, int left) {
+#define WC(d) (dir == 0 ? AT(g, H, i + (d), j) : AT(g, H, i, j + (d)))
+    int order = dir == 0 ? weno_order(i, g->Nx, g->tx) : weno_order(j, g->Ny, g->ty);
+    double r;
+    if (left) r = order == 5 ? weno5z(WC(-3), WC(-2), WC(-1), WC(0), WC(1)) : (order == 3 ? weno3z(WC(-2), WC(-1), WC(0)) : WC(-1));
+    else r = order == 5 ? weno5z(WC(2), WC(1), WC(0), WC(-1), WC(-2)) : (order == 3 ? weno3z(WC(1), WC(0), WC(-1)) : WC(0));
+#undef WC
+    return r;
+}
+static double limit_ratio(double bound, double c, double ext) {
+    double d = ext - c;
+    return d == 0.0 ? 1.0 : fabs((bound - c) / d);
+}
+EXPORT void nko_set_thickness_bounds(nko_model *m, double lower, double upper) {
+    m->weno_bounds[0] = lower;
+    m->weno_bounds[1] = upper;
+}
+/* the face thickness of the WENO schemes: the upwind cell's reconstruction, rescaled by its θ when bounded;
+ * upwind_biased_product ((u + |u|) ψᴸ + (u − |u|) ψᴿ) / 2 */
+static double weno_flux_H(const og *g, const double *H, const double *theta, int dir, int i, int j, double vel) {
+    double hl = weno_face(g, H, dir, i, j, 1), hr = weno_face(g, H, dir, i, j, 0);
+    double ul = hl, ur = hr;
+    if (theta) {
+        int il = dir == 0 ? i - 1 : i, jl = dir == 0 ? j : j - 1;
+        double cl = AT(g, H, il, jl), cr = AT(g, H, i, j);
+        ul = AT(g, theta, il, jl) * (hl - cl) + cl;
+        ur = AT(g, theta, i, j) * (hr - cr) + cr;
+    }
+    return ((vel + fabs(vel)) * ul + (vel - fabs(vel)) * ur) / 2.0;
+}
+
+/* compute_thickness_tendency! / _compute_thickness_tendency! (time_stepping.jl:34-63); adv 0 UpwindBiased(1),
+ * 1 Centered(2), 2 WENO(5), 3 WENO(5) with the bounds-preserving limiter */
 EXPORT void nko_thickness_tendency(nko_model *m, int adv, int coupled) {
     const og *g = &m->g;
     const double *u = m->f[NK_FIELD_U], *v = m->f[NK_FIELD_V], *H = m->f[NK_FIELD_H];
     const double *a = m->f[NK_FIELD_SMB], *mm = m->f[NK_FIELD_BASAL_MELT], *phi = m->f[NK_FIELD_PHI], *mg = m->f[NK_FIELD_GROUNDED_MELT];
     double *G = m->f[NK_FIELD_GH];
+    double *theta = NULL;
+    if (adv == 3) {   /* update_bounds_preserving_limiter!: θ per cell, halo filled as H's */
+        theta = og_alloc(g, 1);
+        for (int j = 1; j <= g->Ny; j++)
+            for (int i = 1; i <= g->Nx; i++) {
+                double c = AT(g, H, i, j), th = 1.0;
+                for (int dir = 0; dir < 2; dir++) {
+                    double pm = weno_face(g, H, dir, i, j, 0);
+                    double pp = dir == 0 ? weno_face(g, H, 0, i + 1, j, 1) : weno_face(g, H, 1, i, j + 1, 1);
+                    double pt = (c - WENO_OMEGA * pm - WENO_OMEGA * pp) / (1.0 - 2.0 * WENO_OMEGA);
+                    double M = fmax(pt, fmax(pm, pp)), mn = fmin(pt, fmin(pm, pp));
+                    th = fmin(th, fmin(limit_ratio(m->weno_bounds[1], c, M), limit_ratio(m->weno_bounds[0], c, mn)));
+                }
+                AT(g, theta, i, j) = th;
+            }
+        fill_halo(g, theta, C, C, 1);
+    }
     for (int j = 1; j <= g->Ny; j++)
         for (int i = 1; i <= g->Nx; i++) {
             double u0 = AT(g, u, i, j), u1 = AT(g, u, i + 1, j), v0 = AT(g, v, i, j), v1 = AT(g, v, i, j + 1);
-            double Fx0 = g->dy * u0 * face_H(AT(g, H, i - 1, j), AT(g, H, i, j), u0, adv);
-            double Fx1 = g->dy * u1 * face_H(AT(g, H, i, j), AT(g, H, i + 1, j), u1, adv);
-            double Fy0 = g->dx * v0 * face_H(AT(g, H, i, j - 1), AT(g, H, i, j), v0, adv);
-            double Fy1 = g->dx * v1 * face_H(AT(g, H, i, j), AT(g, H, i, j + 1), v1, adv);
+            double Fx0, Fx1, Fy0, Fy1;
+            if (adv >= 2) {
+                Fx0 = g->dy * weno_flux_H(g, H, theta, 0, i, j, u0);
+                Fx1 = g->dy * weno_flux_H(g, H, theta, 0, i + 1, j, u1);
+                Fy0 = g->dx * weno_flux_H(g, H, theta, 1, i, j, v0);
+                Fy1 = g->dx * weno_flux_H(g, H, theta, 1, i, j + 1, v1);
+            } else {
+                Fx0 = g->dy * u0 * face_H(AT(g, H, i - 1, j), AT(g, H, i, j), u0, adv);
+                Fx1 = g->dy * u1 * face_H(AT(g, H, i, j), AT(g, H, i + 1, j), u1, adv);
+                Fy0 = g->dx * v0 * face_H(AT(g, H, i, j - 1), AT(g, H, i, j), v0, adv);
+                Fy1 = g->dx * v1 * face_H(AT(g, H, i, j), AT(g, H, i, j + 1), v1, adv);
+            }
             double div = (1.0 / (g->dx * g->dy)) * ((Fx1 - Fx0) + (Fy1 - Fy0));
             /* basal_melt (:51-52): (1 − φ) m with a grounding treatment; grounded_melt (:57-58): mᵍ */
             double melt = m->ph.grounding != NK_GROUNDING_NONE ? (1.0 - AT(g, phi, i, j)) * AT(g, mm, i, j) : AT(g, mm, i, j);
             double S = (AT(g, a, i, j) - melt) - (coupled ? AT(g, mg, i, j) : 0.0);
             AT(g, G, i, j) = S - div;
         }
+    free(theta);
     fill_field_halo(m, NK_FIELD_GH);
 }
 

@@ -60,12 +60,13 @@ def lib():
         L.nko_sigma_metrics.argtypes = [P, dp, dp, dp, dp]
         L.nko_enthalpy_conversions.argtypes = [D, D, D, dp, dp, dp]
         L.nko_thickness_tendency.argtypes = [P, I, I]
+        L.nko_set_thickness_bounds.argtypes = [P, D, D]
         L.nko_thickness_substep.argtypes = [P, D, I]
         L.nko_cell_advection_timescale.argtypes = [P]
         L.nko_cell_advection_timescale.restype = D
         L.nko_energy_step.argtypes = [P, D, I, ct.c_void_p]
         L.nko_till_effective_pressure.argtypes = [P, ct.c_void_p]
-        for name, args in [("nko_ice_mask_value", [D, D, D]), ("nko_segment_grounded_fraction", [D, D]),
+        for name, args in [("nko_weno5z", [D] * 5), ("nko_weno3z", [D] * 3), ("nko_ice_mask_value", [D, D, D]), ("nko_segment_grounded_fraction", [D, D]),
                            ("nko_triangle_grounded_fraction", [D, D, D]), ("nko_arrhenius_rate_factor", [D, D]),
                            ("nko_basal_drag", [D, D, D, I, D, D])]:
             getattr(L, name).argtypes = args
@@ -156,6 +157,9 @@ class OracleModel:
     def compute_thickness_tendency(self, advection=0, grounded_melt_coupling=False):
         lib().nko_thickness_tendency(self._h, advection, 1 if grounded_melt_coupling else 0)
 
+    def set_thickness_bounds(self, lower, upper):
+        lib().nko_set_thickness_bounds(self._h, lower, upper)
+
     def thickness_substep(self, dtau, final_stage):
         lib().nko_thickness_substep(self._h, dtau, 1 if final_stage else 0)
 
@@ -220,6 +224,15 @@ class OracleModel:
 def set_threads(n):
     """Threads of the oracle's parallel row loops (0: leave as is); returns the count in effect."""
     return lib().nko_set_threads(int(n))
+
+
+def weno5z(a, b, c, d, e):
+    """Fifth-order WENO-Z value at the right face of cell c from cell averages (a, b, c, d, e)."""
+    return lib().nko_weno5z(a, b, c, d, e)
+
+
+def weno3z(a, b, c):
+    return lib().nko_weno3z(a, b, c)
 
 
 def ice_mask_value(H, hmin, htr):

@@ -26,8 +26,11 @@ def ulps(a, b):
     return int(np.max(np.abs(ia - ib)))
 
 
+ALL_ADVECTION = [abi.ADVECTION_UPWIND1, abi.ADVECTION_CENTERED2, abi.ADVECTION_WENO5, abi.ADVECTION_WENO5_BOUNDED]
+
+
 @pytest.mark.parametrize("topology", [(B, B), (B, P), (P, B), (P, P)])
-@pytest.mark.parametrize("advection", [abi.ADVECTION_UPWIND1, abi.ADVECTION_CENTERED2])
+@pytest.mark.parametrize("advection", ALL_ADVECTION)
 @pytest.mark.parametrize("grounding", [abi.GROUNDING_NONE, abi.GROUNDING_SUBGRID])
 def test_tendency_and_substep_bit_exact(topology, advection, grounding):
     say(f"transport {topology} advection {advection} grounding {grounding}")
@@ -53,21 +56,22 @@ def test_tendency_and_substep_bit_exact(topology, advection, grounding):
     assert h.get("H").min() == 0.0 and h.cell_advection_timescale() == o.cell_advection_timescale()
 
 
-def test_split_rk3_steps_bit_exact():
-    say("transport steps")
+@pytest.mark.parametrize("advection", ALL_ADVECTION)
+def test_split_rk3_steps_bit_exact(advection):
+    say(f"transport steps, advection {advection}")
     models = [transport_model(background=0.0, width=3e3, melt=lambda x, y: np.where(x > 30e3, 1e-7, 0.0), cls=c)
               for c in (HIPModel, O.OracleModel)]
     dt = 0.5 * models[1][0].cell_advection_timescale()
     for k in range(12):
         for m, _ in models:
-            transport.time_step(m, dt, first=(k == 0))
+            transport.time_step(m, dt, advection=advection, first=(k == 0))
     (h, _), (o, _) = models
     for name in ("H", "HCLIP", "HPREV", "GH"):
         assert ulps(h.get(name), o.get(name)) == 0, name
     assert h.get("HCLIP").max() > 0
 
 
-@pytest.mark.parametrize("advection", [abi.ADVECTION_UPWIND1, abi.ADVECTION_CENTERED2])
+@pytest.mark.parametrize("advection", ALL_ADVECTION)
 def test_volume_is_conserved_at_1024(advection):
     say(f"transport 1024 advection {advection}")
     m, g = transport_model(advection=advection, Nx=1024, L=1024e3, background=100.0, cls=HIPModel)
@@ -76,3 +80,26 @@ def test_volume_is_conserved_at_1024(advection):
     for k in range(32):
         transport.time_step(m, dt, advection=advection, first=(k == 0))
     assert abs(volume(m, g) - V0) / V0 < 1e-13
+
+
+def test_bounded_weno_keeps_a_compact_bump_nonnegative_on_the_device():
+    """test/test_thickness.jl:60-70 on the device: bounded WENO loses < 1e-12 of the volume to the clip, plain WENO
+    > 1e-9, and the device's bounded steps equal the oracle's bit for bit."""
+    drift, H = [], {}
+    for advection, bounds in ((abi.ADVECTION_WENO5_BOUNDED, (0.0, np.inf)), (abi.ADVECTION_WENO5, None)):
+        for cls in (HIPModel, O.OracleModel):
+            m, g = transport_model(width=2e3, cls=cls)
+            if bounds:
+                m.set_thickness_bounds(*bounds)
+            V0 = volume(m, g)
+            dt = 0.25 * m.cell_advection_timescale()
+            for k in range(64):
+                transport.time_step(m, dt, advection=advection, first=(k == 0))
+            H[(advection, cls)] = m.get("H")
+            if cls is HIPModel:
+                assert m.get("H").min() >= 0
+                drift.append(abs(volume(m, g) - V0) / V0)
+    say(f"bounded / plain WENO volume lost to the clip: {drift}")
+    assert drift[0] < 1e-12 and drift[1] > 1e-9
+    for adv in (abi.ADVECTION_WENO5_BOUNDED, abi.ADVECTION_WENO5):
+        assert ulps(H[(adv, HIPModel)], H[(adv, O.OracleModel)]) == 0
