@@ -234,31 +234,36 @@ def mismip_leg(resolutions):
 
 def serving_leg(n, rate_hz, slots, serial_sample):
     """Config 5 on a bounded sample: n requests of the stated stream (Poisson arrivals at rate_hz, sides log-uniform
-    in [64, 1024] rounded to 16) through the server; the serial FP64 baseline solves the first serial_sample
-    requests one after another on fresh FP64 handles."""
+    in [64, 1024] rounded to 16) through the server at their arrival times (latency at a stated load), then the same
+    requests all at t = 0 (saturated: the server's capacity), and the first serial_sample of them solved one at a
+    time (one slot, saturated) in the mixed form and in FP64 on the reference's rediscretized hierarchy."""
     from nunatak_hip.server import serve
     reqs = workloads.serving_stream(n=n, rate_hz=rate_hz, seed=5)
     log(f"[bench] serving {n} requests at {rate_hz} /s on {slots} slots")
     out = serve(reqs, slots=slots, progress=max(1, n // 10))
-    t_serial = 0.0
-    for k, (_, g, p, o, f) in enumerate(reqs[:serial_sample]):
+    log(f"[bench] served at {rate_hz} /s: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, {out['solves_per_s']:.2f} solves/s")
+    sat = serve([(0.0,) + r[1:] for r in reqs], slots=slots)
+    log(f"[bench] saturated ({n} requests at t = 0, {slots} slots): {sat['solves_per_s']:.2f} solves/s, p50 {sat['p50_ms']:.0f} ms, "
+        f"p99 {sat['p99_ms']:.0f} ms")
+    sample = [(0.0,) + r[1:] for r in reqs[:serial_sample]]
+    serial_mixed = serve(sample, slots=1)
+    sample64 = []
+    for _, g, p, o, f in sample:
         o64 = abi.default_options(linear=o.linear, picard=o.picard, preconditioner=o.preconditioner,
                                   precision=abi.PRECISION_FP64, coarse_operator=abi.COARSE_REDISCRETIZED)
-        m = HIPModel(g, p, o64)
-        workloads.load_into(m, f)
-        t0 = time.perf_counter()
-        m.update_velocities(geometry=True)
-        t_serial += time.perf_counter() - t0
-        m.close()
-    serial = serial_sample / t_serial if serial_sample else None
+        sample64.append((0.0, g, p, o64, f))
+    serial64 = serve(sample64, slots=1)
     sides = [r[1].Nx for r in reqs]
-    log(f"[bench] served: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, {out['solves_per_s']:.2f} solves/s; "
-        f"serial FP64 {serial:.2f} solves/s on the first {serial_sample}")
+    log(f"[bench] serial (one slot) on the first {serial_sample}: mixed {serial_mixed['solves_per_s']:.2f} solves/s, "
+        f"FP64 {serial64['solves_per_s']:.2f} solves/s")
     return {"requests": n, "rate_hz": rate_hz, "slots": slots, "sides_min_max": [min(sides), max(sides)],
             "p50_ms": out["p50_ms"], "p99_ms": out["p99_ms"], "solves_per_s": out["solves_per_s"],
-            "serial_fp64_solves_per_s": serial, "serial_sample": serial_sample,
+            "saturated_solves_per_s": sat["solves_per_s"], "saturated_p50_ms": sat["p50_ms"], "saturated_p99_ms": sat["p99_ms"],
+            "serial_sample": serial_sample, "serial_mixed_solves_per_s": serial_mixed["solves_per_s"],
+            "serial_fp64_solves_per_s": serial64["solves_per_s"],
             "workload": "serving stream (seed 5): SSA grounded/floating channel per request, mixed precision "
-                        "(FP32 Galerkin V-cycle), MG-PCG linear 1e-8, Picard 1e-6"}
+                        "(FP32 Galerkin V-cycle), MG-PCG linear 1e-8, Picard 1e-6; serial baselines: the first "
+                        f"{serial_sample} requests through one slot"}
 
 
 def transport_leg(N, steps, advection=abi.ADVECTION_UPWIND1):
@@ -364,9 +369,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cpu-full", action="store_true", help="skip the complete CPU solves (marine sheet, MISMIP+ 1 km)")
     ap.add_argument("--no-fp64-galerkin", action="store_true", help="skip the FP64 solve with Galerkin coarse operators")
+    ap.add_argument("--refined", action="store_true",
+                    help="also time the FP32-PCG-in-FP64-refinement form (≈ 50 s per 1024² solve: opt-in)")
     ap.add_argument("--mismip", default="8000,4000,2000,1000,500,250",
                     help="resolutions (m) of the reference's multigrid scaling benchmark run on the device ('' to skip)")
-    ap.add_argument("--only", choices=["fp64", "fp64_galerkin", "mixed"], default=None)
+    ap.add_argument("--only", choices=["fp64", "fp64_galerkin", "mixed", "mixed_refined"], default=None)
     ap.add_argument("--picard", type=int, default=None, help="FixedIterations(n) Picard (profiling runs only)")
     ap.add_argument("--graphs", type=int, default=1, help="0: launch kernel by kernel (for external profilers)")
     ap.add_argument("--no-roofline", action="store_true")
@@ -381,7 +388,7 @@ def main():
     ap.add_argument("--serve", type=int, default=48, help="requests in the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
-    ap.add_argument("--serve-serial", type=int, default=12, help="requests timed serially in FP64 for the baseline")
+    ap.add_argument("--serve-serial", type=int, default=24, help="requests timed serially (mixed and FP64) for the baselines")
     ap.add_argument("--transport", type=int, default=20, help="RK3 thickness steps timed in the transport leg (0: skip)")
     ap.add_argument("--transport-only", action="store_true",
                     help="run only the thickness-transport and enthalpy legs; prints their JSON objects")
@@ -437,11 +444,15 @@ def main():
     results = {}
     models = {}
     setups = {}
-    legs = [("fp64", abi.PRECISION_FP64), ("fp64_galerkin", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)]
+    legs = [("fp64", abi.PRECISION_FP64), ("fp64_galerkin", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED),
+            ("mixed_refined", abi.PRECISION_MIXED_REFINED)]
     for label, prec in legs:
         if args.only and args.only != label:
             continue
         if label == "fp64_galerkin" and (args.no_fp64_galerkin or args.coarse != "galerkin"):
+            continue
+        if label == "mixed_refined" and not (args.refined or args.only == "mixed_refined") or \
+                (label == "mixed_refined" and args.coarse != "galerkin"):
             continue
         coarse = abi.COARSE_GALERKIN if (label != "fp64" and args.coarse == "galerkin") else abi.COARSE_REDISCRETIZED
         m, setup = make(N, prec, picard=args.picard, graphs=args.graphs, coarse=coarse)
@@ -458,7 +469,7 @@ def main():
         err = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     else:
         err = None
-    main_label = "mixed" if "mixed" in results else ("fp64" if "fp64" in results else "fp64_galerkin")
+    main_label = "mixed" if "mixed" in results else next(k for k in ("fp64", "fp64_galerkin", "mixed_refined") if k in results)
     r = results[main_label]
     out["value"] = K / r["wall"]
     out["ms_per_step"] = 1e3 * r["wall"] / K
@@ -479,6 +490,16 @@ def main():
     if err is not None:
         out["mixed_rel_l2_err_vs_fp64"] = err
         out["mixed_speedup_vs_fp64"] = out["value"] / out["fp64_solves_per_s"]
+    if "mixed_refined" in results:
+        mr = results["mixed_refined"]
+        out["mixed_refined_solves_per_s"] = K / mr["wall"]
+        out["mixed_refined_ms_per_solve"] = 1e3 * mr["wall"] / K
+        out["mixed_refined_picard_iterations"] = mr["infos"][-1]["picard_iterations"]
+        out["mixed_refined_inner_cg_iterations"] = mr["infos"][-1]["total_linear_iterations"]
+        if "fp64" in results:
+            a = np.concatenate([mr["u"].ravel(), mr["v"].ravel()])
+            b = np.concatenate([results["fp64"]["u"].ravel(), results["fp64"]["v"].ravel()])
+            out["mixed_refined_rel_l2_err_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     if "fp64_galerkin" in results:
         fg = results["fp64_galerkin"]
         out["fp64_galerkin_solves_per_s"] = K / fg["wall"]

@@ -99,7 +99,11 @@ typedef struct nk_control {
 
 enum { NK_PRECONDITIONER_NONE = 0, NK_PRECONDITIONER_DIAGONAL = 1, NK_PRECONDITIONER_MULTIGRID = 2 };
 enum { NK_POINT_JACOBI = 0, NK_LINE_JACOBI = 1 };
-enum { NK_PRECISION_FP64 = 0, NK_PRECISION_MIXED = 1 };
+/* NK_PRECISION_MIXED: FP64 PCG (operator, residual, α/β, stopping rule) preconditioned by an FP32 multigrid V-cycle
+ * whose coarsest-grid solve is FP64. NK_PRECISION_MIXED_REFINED: FP64 iterative refinement — the defect r = b − A x
+ * and its norm in FP64, the stopping rule on it — around an FP32 PCG (FP32 operator, vectors and V-cycle, FP64 dots
+ * and α/β) that solves A e = r to a relative 1e-4 from zero each step. Both need the multigrid preconditioner. */
+enum { NK_PRECISION_FP64 = 0, NK_PRECISION_MIXED = 1, NK_PRECISION_MIXED_REFINED = 2 };
 /* Coarse-level operators of the multigrid preconditioner: rediscretized from coarsened coefficients, as the reference
  * builds them (src/StressBalances/multigrid_preconditioner.jl:128-141), or Galerkin products Pᵀ A P of the level
  * above with the same transfers (no reference counterpart; keeps a calving front free on coarse levels where a
@@ -212,7 +216,7 @@ typedef struct nk_info {
     int32_t picard_iterations;      /* solver.iteration of StressBalanceSolver */
     int32_t linear_iterations;      /* iterations of the last linear solve (VelocityConjugateGradientSolver.iteration) */
     int64_t total_linear_iterations;
-    int32_t refinement_steps;       /* mixed precision: FP64 defect corrections in the last linear solve */
+    int32_t refinement_steps;       /* NK_PRECISION_MIXED_REFINED: FP64 defect corrections in the last linear solve */
     int32_t pad;
     double last_picard_change;      /* Δ = ‖x − x⁻‖ */
     double last_picard_norm;        /* X = ‖x‖ */

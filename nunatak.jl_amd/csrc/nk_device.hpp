@@ -1301,15 +1301,16 @@ __global__ __launch_bounds__(NT) void k_cast_coeffs(KG g, const double *__restri
     d32[o] = (float)d[o];
 }
 
-// Mixed precision: r32 = (float) r64 and d32 = 0 over :xyz (inner solve from zero)
+// Iterative refinement: r32 = (float) (r64 · zs) and d32 = 0 over :xyz (the inner FP32 solve from zero; zs a power of
+// two that brings ‖r‖ to O(1))
 __global__ __launch_bounds__(NT) void k_demote(KG g, const double *__restrict__ ru, const double *__restrict__ rv,
                                                float *__restrict__ r32u, float *__restrict__ r32v, float *__restrict__ du,
-                                               float *__restrict__ dv) {
+                                               float *__restrict__ dv, double zs) {
     NK_IJ
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
-    r32u[o] = (float)ru[o];
-    r32v[o] = (float)rv[o];
+    r32u[o] = (float)(ru[o] * zs);
+    r32v[o] = (float)(rv[o] * zs);
     du[o] = 0.0f;
     dv[o] = 0.0f;
 }
@@ -1347,14 +1348,14 @@ __global__ __launch_bounds__(NT) void k_promote_dot(KG g, const float *__restric
     reduce_finish<2>(acc, red);
 }
 
-// Mixed precision: x64 += (double) d32 over :xyz
+// Iterative refinement: x64 += (double) d32 · zi over :xyz (zi = 1/zs of k_demote)
 __global__ __launch_bounds__(NT) void k_promote_add(KG g, const float *__restrict__ du, const float *__restrict__ dv,
-                                                    double *__restrict__ xu, double *__restrict__ xv) {
+                                                    double *__restrict__ xu, double *__restrict__ xv, double zi) {
     NK_IJ
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
-    xu[o] += (double)du[o];
-    xv[o] += (double)dv[o];
+    xu[o] += (double)du[o] * zi;
+    xv[o] += (double)dv[o] * zi;
 }
 
 // fill over the whole extent (used for fill!(x, 0) and workspace clearing)

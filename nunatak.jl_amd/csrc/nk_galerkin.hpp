@@ -607,10 +607,14 @@ __device__ __forceinline__ int g_unknown(const KG &g, int comp, int i, int j) { 
 
 // The inverse is formed and stored in FP64 whatever the V-cycle's precision (the mixed form keeps the coarsest-grid
 // solve in FP64); a null direction is recognised relative to the rounding of the stencil it was assembled from.
+// Systems of up to GDENSE_LDS unknowns are eliminated in LDS (n² doubles of dynamic shared memory) and only the result
+// is written to M; larger ones are eliminated in M itself.
+constexpr int GDENSE_LDS = 128;
 template <typename TS, int TX, int TY>
-__global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__restrict__ st, double *__restrict__ M) {
+__global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__restrict__ st, double *__restrict__ Mout) {
     using T = double;
     g.tx = TX; g.ty = TY;
+    extern __shared__ __align__(16) double gsm[];
     __shared__ T fcol[GDENSE_MAX];
     __shared__ T dorig[GDENSE_MAX];
     __shared__ unsigned char sing[GDENSE_MAX];
@@ -618,6 +622,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__re
     __shared__ int skip;
     const T tol = sizeof(TS) == 4 ? T(1e-5) : T(1e-10);
     const int n = 2 * g.Nx * g.Ny, tid = threadIdx.x, nth = blockDim.x;
+    const bool lds = n <= GDENSE_LDS;
+    T *M = lds ? gsm : Mout;
     const long P = gplane(g);
     for (int e = tid; e < n * n; e += nth) M[e] = T(0);
     __syncthreads();
@@ -675,6 +681,13 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__re
         if (sing[e / n] || sing[e % n]) M[e] = T(0);
     __syncthreads();
     // store column-major (element (r, c) at c·n + r) so that the solves read it with coalesced loads
+    if (lds) {
+        for (int e = tid; e < n * n; e += nth) {
+            const int c = e / n, r = e % n;
+            Mout[e] = M[(long)r * n + c];
+        }
+        return;
+    }
     for (int e = tid; e < n * n; e += nth) {
         const int i = e / n, j = e % n;
         if (i < j) {

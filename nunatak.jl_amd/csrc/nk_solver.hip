@@ -106,6 +106,8 @@ struct nk_solver {
     const double *fz_r64u = nullptr, *fz_r64v = nullptr;
     double *fz_z64u = nullptr, *fz_z64v = nullptr;
     float *rl1u32 = nullptr, *rl1v32 = nullptr;   // mixed multigrid: 1/ℓ1 of the finest rows in FP32 (k_l1_recip)
+    float *b32u = nullptr, *b32v = nullptr;       // iterative refinement: the scaled FP32 defect, right-hand side of the inner PCG
+    double refine_inner = 1e-4;                   // ... and the inner PCG's relative tolerance (NK_REFINE_INNER_RELTOL: tuning)
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     bool fuse_pro = true;   // packed Galerkin levels: prolongation inside the first post-sweep (NK_NO_PRO_FUSE=1: k_prolongate)
     long pro_fuse_points = 256L * 256;   // ... on levels of at most this many points (NK_PRO_FUSE_POINTS: tuning)
@@ -404,7 +406,9 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
     }
     if (s->cinv) {
         Level &C = s->lev.back();
-        NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, 0, s->stream, C.g, gs<T>(C), (double *)s->cinv);
+        const int n = 2 * C.g.Nx * C.g.Ny;
+        const size_t shm = n <= GDENSE_LDS ? sizeof(double) * (size_t)n * (size_t)n : 0;
+        NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, shm, s->stream, C.g, gs<T>(C), (double *)s->cinv);
         CHK(launch_check(s));
     }
     for (size_t l = 1; l < s->lev.size(); l++) {
@@ -438,7 +442,7 @@ nk_status update_preconditioner(nk_solver *s) {
             CHK(corners_flags(s, (int)l, true));
         }
     }
-    if (s->op.precision == NK_PRECISION_MIXED) {
+    if (s->op.precision != NK_PRECISION_FP64) {
         for (size_t l = 0; l < s->lev.size(); l++) {
             Level &L = s->lev[l];
             k_cast_coeffs<<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, L.eta, L.etaff, L.bu, L.bv, L.eta32,
@@ -454,7 +458,7 @@ nk_status update_preconditioner(nk_solver *s) {
         }
     }
     if (s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID && s->op.coarse_operator == NK_COARSE_GALERKIN && s->lev.size() > 1) {
-        if (s->op.precision == NK_PRECISION_MIXED) CHK(galerkin_build<float>(s));
+        if (s->op.precision != NK_PRECISION_FP64) CHK(galerkin_build<float>(s));
         else CHK(galerkin_build<double>(s));
     }
     return NK_OK;
@@ -917,7 +921,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         CHK(launch_check(s));
     }
     Pair<T> z{W.zu, W.zv};
-    const bool mixed = sizeof(T) == 8 && s->op.precision == NK_PRECISION_MIXED && pre == NK_PRECONDITIONER_MULTIGRID;
+    const bool mixed = sizeof(T) == 8 && s->op.precision != NK_PRECISION_FP64 && pre == NK_PRECONDITIONER_MULTIGRID;
     if (mixed) {
         CHK(precondition_fp32(s, FIN_INIT, nullptr));
         k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, W.zu, W.zv, W.pu, W.pv, 1, nullptr);
@@ -979,7 +983,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
     // The chunk's launches have fixed arguments for a given solve path and vector buffers, so they are captured once
     // and replayed; the kernels read the device stop flag, so a replay after convergence does no work.
     const bool use_graph = s->op.graphs && s->op.profile_kernel == NK_PROFILE_NONE;
-    const int key = (int)sizeof(T) * 1000 + (mixed ? 500 : 0) + pre * 100 + (xu == s->f[NK_FIELD_U] ? 1 : 0) + chunk * 10000;
+    const int key = (int)sizeof(T) * 1000 + (mixed ? 500 : 0) + pre * 100 + ((const void *)xu == (const void *)s->f[NK_FIELD_U] ? 1 : 0) + chunk * 10000;
     if (use_graph && (s->chunk_graph_key != key || s->chunk_graph_x != (const void *)xu)) {
         if (s->chunk_graph) { (void)hipGraphExecDestroy(s->chunk_graph); s->chunk_graph = nullptr; }
         hipGraph_t graph;
@@ -1014,7 +1018,76 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
     return NK_OK;
 }
 
+// NK_PRECISION_MIXED_REFINED: FP64 iterative refinement around an FP32 PCG. The defect r = b − A x and its norm are
+// FP64 (the reference's operator and stopping rule ‖r‖ ≤ max(reltol ‖r₀‖, abstol) on the true residual); each step
+// solves A₃₂ e = r·zs from zero with the FP32 PCG (FP32 operator through the neighbour masks, FP32 vectors and V-cycle,
+// FP64 dots and α/β) to a relative refine_inner, zs = 2^−⌊log₂‖r‖⌋ keeping the FP32 right-hand side O(1), then
+// x += e/zs in FP64. The linear iteration count is the inner PCG's total.
+nk_status solve_refined(nk_solver *s, nk_info *info) {
+    CHK(update_preconditioner(s));
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny);
+    Level &L0 = s->lev[0];
+    Work<double> &W = s->w64;
+    const nk_control &ctl = s->op.linear;
+    double *xu = s->f[NK_FIELD_U], *xv = s->f[NK_FIELD_V];
+    const double *bu = s->f[NK_FIELD_TAUU], *bv = s->f[NK_FIELD_TAUV];
+    const double wf = s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0;
+    auto defect = [&](double *res) -> nk_status {
+        launch_apply0<double>(s, coef<double>(L0), wf, xu, xv, W.ru, W.rv, bu, bv, red(s, FIN_NONE), 0, nullptr);
+        CHK(launch_check(s));
+        k_dot<double><<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, W.ru, W.rv, nullptr, nullptr, red(s, FIN_NONE), nullptr);
+        CHK(launch_check(s));
+        CHK(finish_red(s, B, 1, FIN_NONE, nullptr));
+        CHK(read_scalars(s));
+        *res = std::sqrt(s->S_host->out[0]);
+        if (!std::isfinite(*res)) { s->err = "the refinement defect is not finite (NaN or Inf in the inputs?)"; return NK_NONFINITE; }
+        return NK_OK;
+    };
+    double res = 0.0;
+    CHK(defect(&res));
+    const bool fixed = ctl.kind == NK_FIXED_ITERATIONS;
+    const int maxit = fixed ? ctl.iterations : ctl.maxiter;
+    const double tol = jmax_h(ctl.reltol * res, ctl.abstol);
+    int total = 0, steps = 0;
+    while (total < maxit && (fixed || res > tol) && res > 0.0) {
+        const int e = std::ilogb(res);
+        const int k = e < -1000 ? 1000 : (e > 1000 ? -1000 : -e);
+        const double zs = std::ldexp(1.0, k), zi = std::ldexp(1.0, -k);
+        k_demote<<<B, THREADS, 0, s->stream>>>(g, W.ru, W.rv, s->b32u, s->b32v, s->w32.du, s->w32.dv, zs);
+        CHK(launch_check(s));
+        nk_control inner;
+        std::memset(&inner, 0, sizeof(inner));
+        if (fixed) {   // FixedIterations: the remaining count in one step
+            inner.kind = NK_FIXED_ITERATIONS;
+            inner.iterations = maxit - total;
+        } else {
+            inner.kind = NK_CONVERGENCE;
+            inner.reltol = s->refine_inner;
+            inner.maxiter = maxit - total;
+        }
+        int it = 0;
+        double r32 = 0.0;
+        CHK(pcg<float>(s, s->w32.du, s->w32.dv, s->b32u, s->b32v, inner, false, &it, &r32));
+        k_promote_add<<<B, THREADS, 0, s->stream>>>(g, s->w32.du, s->w32.dv, xu, xv, zi);
+        CHK(launch_check(s));
+        total += it;
+        steps++;
+        CHK(defect(&res));
+        if (it == 0 || fixed) break;
+    }
+    if (info) {
+        info->linear_iterations = total;
+        info->total_linear_iterations += total;
+        info->last_residual_norm = res;
+        info->refinement_steps = steps;
+    }
+    s->S_host->done_by_tol = !fixed && res <= tol;
+    return NK_OK;
+}
+
 nk_status solve_linear(nk_solver *s, nk_info *info) {
+    if (s->op.precision == NK_PRECISION_MIXED_REFINED && s->op.preconditioner == NK_PRECONDITIONER_MULTIGRID) return solve_refined(s, info);
     CHK(update_preconditioner(s));
     int it = 0;
     double res = 0.0;
@@ -1205,11 +1278,11 @@ nk_status validate(const nk_grid *g, const nk_physics *ph, const nk_solver_optio
         err = "the coefficients must include the side walls, FreeSlipWalls() or NoSlipWalls()";  // ssa_operator.jl:56
         return NK_INVALID_ARGUMENT;
     }
-    if (op->preconditioner < 0 || op->preconditioner > 2 || op->precision < 0 || op->precision > 1) {
+    if (op->preconditioner < 0 || op->preconditioner > 2 || op->precision < 0 || op->precision > 2) {
         err = "unknown preconditioner or precision";
         return NK_INVALID_ARGUMENT;
     }
-    if (op->precision == NK_PRECISION_MIXED && op->preconditioner != NK_PRECONDITIONER_MULTIGRID) {
+    if (op->precision != NK_PRECISION_FP64 && op->preconditioner != NK_PRECONDITIONER_MULTIGRID) {
         err = "mixed precision runs the multigrid V-cycle in FP32: it needs the multigrid preconditioner";
         return NK_INVALID_ARGUMENT;
     }
@@ -1355,17 +1428,17 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
             if ((st = alloc_plane(s, &L.act, L.g)) != NK_OK) return fail(st);
             double **v64[] = {&L.v64.x0u, &L.v64.x0v, &L.v64.x1u, &L.v64.x1v, &L.v64.bu, &L.v64.bv, &L.v64.qu, &L.v64.qv, &L.v64.t};
             for (auto p : v64) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
-            if (options->precision == NK_PRECISION_MIXED) {
+            if (options->precision != NK_PRECISION_FP64) {
                 float **c32[] = {&L.eta32, &L.etaff32, &L.bu32, &L.bv32};
                 for (auto p : c32) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
                 float **v32[] = {&L.v32.x0u, &L.v32.x0v, &L.v32.x1u, &L.v32.x1v, &L.v32.bu, &L.v32.bv, &L.v32.qu, &L.v32.qv, &L.v32.t};
                 for (auto p : v32) if ((st = alloc_plane(s, p, L.g)) != NK_OK) return fail(st);
             }
             if (options->coarse_operator == NK_COARSE_GALERKIN) {
-                if (options->precision == NK_PRECISION_MIXED) st = alloc_plane(s, &L.gs32, L.g, GS_ALLOC);
+                if (options->precision != NK_PRECISION_FP64) st = alloc_plane(s, &L.gs32, L.g, GS_ALLOC);
                 else st = alloc_plane(s, &L.gs64, L.g, GS_ALLOC);
                 if (st != NK_OK) return fail(st);
-                const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
+                const size_t es = options->precision != NK_PRECISION_FP64 ? 4 : 8;
                 if (wants_gpack(L.g, (int)l) && (st = alloc_bytes(s, &L.gpk, GP_PLANES * es * (size_t)gp_plane(L.g))) != NK_OK)
                     return fail(st);
                 if (wants_gcst(L.g, (int)l, es) &&
@@ -1392,10 +1465,15 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->use_z32 = !(nz && nz[0] == '1');
         const char *np = std::getenv("NK_NO_GPACK");
         s->use_gpack = !(np && np[0] == '1');
-        if (options->precision == NK_PRECISION_MIXED) {
+        if (options->precision != NK_PRECISION_FP64) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
                              &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv, &s->rl1u32, &s->rl1v32};
             for (auto p : w32) if ((st = alloc_plane(s, p, s->g)) != NK_OK) return fail(st);
+            if (options->precision == NK_PRECISION_MIXED_REFINED) {
+                if ((st = alloc_plane(s, &s->b32u, s->g)) != NK_OK || (st = alloc_plane(s, &s->b32v, s->g)) != NK_OK) return fail(st);
+                const char *ir = std::getenv("NK_REFINE_INNER_RELTOL");
+                if (ir) s->refine_inner = std::strtod(ir, nullptr);
+            }
         }
     }
     // coarse tail: the longest run of coarsest levels whose LDS image fits the budget (point Jacobi only)
@@ -1431,8 +1509,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
         // the LDS tail runs the rediscretized operator: levels with Galerkin operators have a tail of their own
         if (options->coarse_operator == NK_COARSE_GALERKIN) {
-            s->tail_start[options->precision == NK_PRECISION_MIXED ? 1 : 0] = -1;
-            const size_t es = options->precision == NK_PRECISION_MIXED ? 4 : 8;
+            s->tail_start[options->precision != NK_PRECISION_FP64 ? 1 : 0] = -1;
+            const size_t es = options->precision != NK_PRECISION_FP64 ? 4 : 8;
             const char *gb = std::getenv("NK_GTAIL_BYTES");   // tuning: LDS bytes of vectors the tail may take
             const size_t gbudget = gb ? (size_t)std::strtoul(gb, nullptr, 10) : GTAIL_BUDGET;
             size_t bytes = 0, pts = 0;
@@ -1453,6 +1531,16 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
                                 (const void *)k_g_tail<float, 1, 0>, (const void *)k_g_tail<float, 1, 1>};
             bool ok = true;
             for (const void *f : gt) ok &= hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+            // the dense inverse eliminates systems of up to GDENSE_LDS unknowns in LDS
+            const void *gi[] = {(const void *)k_g_inverse<double, 0, 0>, (const void *)k_g_inverse<double, 0, 1>,
+                                (const void *)k_g_inverse<double, 1, 0>, (const void *)k_g_inverse<double, 1, 1>,
+                                (const void *)k_g_inverse<float, 0, 0>, (const void *)k_g_inverse<float, 0, 1>,
+                                (const void *)k_g_inverse<float, 1, 0>, (const void *)k_g_inverse<float, 1, 1>};
+            for (const void *f : gi)   // n² doubles beside its 9 KB of static shared memory
+                if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * GDENSE_LDS * GDENSE_LDS)) != hipSuccess) {
+                    s->err = "hipFuncSetAttribute (dense coarsest inverse)";
+                    return fail(NK_DEVICE_ERROR);
+                }
             if (ok && start >= 1 && (int)s->lev.size() - start >= 2 && !(nt && nt[0] == '1')) s->gtail_start = start;
         }
     }
@@ -1479,7 +1567,7 @@ namespace {
 size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
     const KG g0 = make_kg(grid->Nx, grid->Ny, grid->topology_x, grid->topology_y, grid->dx, grid->dy);
     const size_t p0 = (size_t)plane_of(g0), Ns = (size_t)grid->Nsigma;
-    const bool mixed = o->precision == NK_PRECISION_MIXED;
+    const bool mixed = o->precision != NK_PRECISION_FP64;
     size_t b = 0;
     for (int id = 0; id < NK_FIELD_COUNT; id++) b += sizeof(double) * p0 * (LOC[id].sigma ? Ns : 1);
     b += 2 * p0;                                                              // face classes
@@ -1511,6 +1599,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
     b += (24 + 16) * sizeof(double) * p0 + 3 * Ns * sizeof(double) * p0;
     b += sizeof(uint16_t) * p0;                                               // neighbour masks of the finest level
     if (mixed) b += 14 * sizeof(float) * p0;
+    if (o->precision == NK_PRECISION_MIXED_REFINED) b += 2 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
     b += sizeof(double) * MAXRED * nb + sizeof(Scal);
     if (o->coarse_operator == NK_COARSE_GALERKIN) {
@@ -1725,7 +1814,7 @@ nk_status nk_apply_operator(nk_solver *s) {
 
 nk_status nk_precondition(nk_solver *s) {
     if (!s) return NK_INVALID_ARGUMENT;
-    if (s->op.precision == NK_PRECISION_MIXED) { s->err = "nk_precondition runs the FP64 preconditioner only"; return NK_UNSUPPORTED; }
+    if (s->op.precision != NK_PRECISION_FP64) { s->err = "nk_precondition runs the FP64 preconditioner only"; return NK_UNSUPPORTED; }
     CHK(corners_flags(s, 0, false));
     const nk_control fixed0 = {NK_FIXED_ITERATIONS, 0, 0, 0, 0.0, 0.0};
     CHK(write_scalars(s, fixed0));  // done = 0: no gating

@@ -19,7 +19,7 @@ CONVERGENCE, FIXED_ITERATIONS = 0, 1
 PRECONDITIONER_NONE, PRECONDITIONER_DIAGONAL, PRECONDITIONER_MULTIGRID = 0, 1, 2
 POINT_JACOBI, LINE_JACOBI = 0, 1
 COARSE_REDISCRETIZED, COARSE_GALERKIN = 0, 1
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_REFINED = 0, 1, 2
 
 CLASS_WALL, CLASS_ICE_FREE, CLASS_FRONT, CLASS_GROUNDING = 1, 2, 4, 8
 PROFILE_NONE, PROFILE_OPERATOR, PROFILE_SWEEP, PROFILE_UPDATE, PROFILE_GALERKIN_SWEEP = 0, 1, 2, 3, 4

@@ -286,3 +286,26 @@ def test_mixed_linear_solve_scales_exactly_with_a_tiny_right_hand_side(coarse):
     say(f"[galerkin] mixed solve with rhs x 2^-200 (coarse {coarse}): CG {n1} / {n2}")
     assert n1 == n2 and n1 > 0
     assert np.array_equal(u2, u1 * 2.0 ** -200) and np.array_equal(v2, v1 * 2.0 ** -200)
+
+
+@pytest.mark.parametrize("N", [128, 256])
+def test_refined_mixed_form_matches_fp64(N):
+    """NK_PRECISION_MIXED_REFINED — FP64 iterative refinement (defect, norm and stopping rule in FP64) around an FP32
+    PCG with the FP32 Galerkin V-cycle — on the DIVA shelf: ≤ 1e-8 relative L2 in (u, v) against the FP64 solve, at
+    least one refinement step per linear solve, and the last linear solve stopped on the FP64 defect."""
+    out = {}
+    for prec, coarse in ((abi.PRECISION_FP64, abi.COARSE_REDISCRETIZED), (abi.PRECISION_MIXED_REFINED, abi.COARSE_GALERKIN)):
+        g, phys, opts, f = W.diva_shelf(N=N, dx=512e3 / N, precision=prec)
+        opts.coarse_operator = coarse
+        m = HIPModel(g, phys, opts)
+        W.load_into(m, f)
+        info = m.update_velocities()
+        out[prec] = (m.get("U"), m.get("V"), info)
+    a = np.concatenate([x.ravel() for x in out[abi.PRECISION_MIXED_REFINED][:2]])
+    b = np.concatenate([x.ravel() for x in out[abi.PRECISION_FP64][:2]])
+    err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    info = out[abi.PRECISION_MIXED_REFINED][2]
+    say(f"refined {N}^2: picard {info['picard_iterations']} inner CG {info['total_linear_iterations']} refinement steps "
+        f"(last solve) {info['refinement_steps']}; FP64 picard {out[abi.PRECISION_FP64][2]['picard_iterations']}; rel L2 {err:.2e}")
+    assert err <= 1e-8
+    assert info["refinement_steps"] >= 1
