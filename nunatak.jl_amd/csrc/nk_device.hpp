@@ -786,6 +786,15 @@ __device__ __forceinline__ MPoint<T> ssa_apply_m(const KG &g, int i, int j, unsi
     return P;
 }
 
+// Marks loaded values as used here (an empty asm statement reading their registers), so the compiler issues their
+// loads before this point and waits for them here instead of sinking them below a following barrier: a tiled kernel
+// then has its tile's loads and its own point's loads in flight together, one round trip to memory instead of two.
+template <typename T> __device__ __forceinline__ void keep_loaded(const T &v) { asm volatile("" ::"v"(v)); }
+template <typename T, int N> __device__ __forceinline__ void keep_loaded(const T (&a)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; k++) asm volatile("" ::"v"(a[k]));
+}
+
 constexpr int ATW = BX + 2, ATH = BY + 2;   // LDS tile of the iterate for a 64 × 4 block (k_apply_mt)
 // ssa_apply_m with the iterate from LDS tiles tu, tv whose element [1][1] is point (i0, j0) (see k_apply_mt).
 template <typename T>
@@ -1015,33 +1024,69 @@ __global__ __launch_bounds__(NT) void k_sweep_pre_t(KG g, Coef<T> c, T wf, const
     auto cly = [&](int j) { return j < 0 ? 0 : (j > g.Ny + 1 ? g.Ny + 1 : j); };
     auto x1u = [&](unsigned q) { return T(0) + ld32(bu, q) * ld32(rlu, q); };   // as k_sweep_zero forms it
     auto x1v = [&](unsigned q) { return T(0) + ld32(bv, q) * ld32(rlv, q); };
-    for (int e = tid; e < TTH * TTW; e += NT) {
+    const int i = i0 + threadIdx.x, jb = j0 + threadIdx.y * SROWS;
+    // the tile elements this thread stages (x₁ at the iterate tile, η and η̄ᶠᶠ at theirs), each round unrolled so all
+    // loads are in flight together; idle lanes of the last round reread the last element
+    constexpr int XT_ITER = (TTH * TTW + NT - 1) / NT, CT_ITER = ((TTH - 1) * (TTW - 1) + NT - 1) / NT;
+    T xa[XT_ITER], xb[XT_ITER], ea[CT_ITER], fa[CT_ITER];
+#pragma unroll
+    for (int k = 0; k < XT_ITER; k++) {
+        const int e0 = tid + k * NT, e = e0 < TTH * TTW ? e0 : TTH * TTW - 1;
         const int tj = e / TTW, ti = e - tj * TTW;
-        const int i = clx(i0 - 1 + ti), j = cly(j0 - 1 + tj);
-        su[tj][ti] = iu_f<T>(g, c.act, x1u, i, j);
-        sv[tj][ti] = iv_f<T>(g, c.act, x1v, i, j);
+        const int ii = clx(i0 - 1 + ti), jj = cly(j0 - 1 + tj);
+        xa[k] = iu_f<T>(g, c.act, x1u, ii, jj);
+        xb[k] = iv_f<T>(g, c.act, x1v, ii, jj);
     }
-    for (int e = tid; e < (TTH - 1) * (TTW - 1); e += NT) {
+#pragma unroll
+    for (int k = 0; k < CT_ITER; k++) {
+        const int e0 = tid + k * NT, e = e0 < (TTH - 1) * (TTW - 1) ? e0 : (TTH - 1) * (TTW - 1) - 1;
         const int tj = e / (TTW - 1), ti = e - tj * (TTW - 1);
-        se[tj][ti] = ldc(g, c.eta, clx(i0 - 1 + ti), cly(j0 - 1 + tj));
-        sf[tj][ti] = ldff(g, c.etaff, clx(i0 + ti), cly(j0 + tj));
+        ea[k] = ldc(g, c.eta, clx(i0 - 1 + ti), cly(j0 - 1 + tj));
+        fa[k] = ldff(g, c.etaff, clx(i0 + ti), cly(j0 + tj));
     }
+    // the points' own values while the tile's loads are in flight
+    T b_u[SROWS], b_v[SROWS], r_u[SROWS], r_v[SROWS], be_u[SROWS], be_v[SROWS];
+    uint8_t a_c[SROWS];
+#pragma unroll
+    for (int r = 0; r < SROWS; r++) {
+        const int j = jb + r;
+        const unsigned o = (i <= g.Nx && j <= g.Ny) ? uoff(g, i, j) : 0u;
+        b_u[r] = ld32(bu, o); b_v[r] = ld32(bv, o);
+        r_u[r] = ld32(rlu, o); r_v[r] = ld32(rlv, o);
+        be_u[r] = ld32(c.bu, o); be_v[r] = ld32(c.bv, o);
+        a_c[r] = ld32(c.act, o);
+    }
+    // idle lanes write the last element's value into it again (no branch)
+#pragma unroll
+    for (int k = 0; k < XT_ITER; k++) {
+        const int e0 = tid + k * NT, e = e0 < TTH * TTW ? e0 : TTH * TTW - 1;
+        const int tj = e / TTW, ti = e - tj * TTW;
+        su[tj][ti] = xa[k];
+        sv[tj][ti] = xb[k];
+    }
+#pragma unroll
+    for (int k = 0; k < CT_ITER; k++) {
+        const int e0 = tid + k * NT, e = e0 < (TTH - 1) * (TTW - 1) ? e0 : (TTH - 1) * (TTW - 1) - 1;
+        const int tj = e / (TTW - 1), ti = e - tj * (TTW - 1);
+        se[tj][ti] = ea[k];
+        sf[tj][ti] = fa[k];
+    }
+    keep_loaded(b_u); keep_loaded(b_v); keep_loaded(r_u); keep_loaded(r_v); keep_loaded(be_u); keep_loaded(be_v);
     __syncthreads();
     auto U = [&](int p, int q) { return su[q - j0 + 1][p - i0 + 1]; };
     auto V = [&](int p, int q) { return sv[q - j0 + 1][p - i0 + 1]; };
     auto E = [&](int p, int q) { return se[q - j0 + 1][p - i0 + 1]; };
     auto Ff = [&](int p, int q) { return sf[q - j0][p - i0]; };
-    const int i = i0 + threadIdx.x, jb = j0 + threadIdx.y * SROWS;
 #pragma unroll
     for (int r = 0; r < SROWS; r++) {
         const int j = jb + r;
         if (i <= g.Nx && j <= g.Ny) {
             const unsigned o = uoff(g, i, j);
-            const T pu0 = x1u(o), pv0 = x1v(o);
+            const T pu0 = T(0) + b_u[r] * r_u[r], pv0 = T(0) + b_v[r] * r_v[r];
             T qa, qb;
-            ssa_core<T>(g, i, j, U, V, E, Ff, pu0, pv0, ld32(c.bu, o), ld32(c.bv, o), ld32(c.act, o), wf, qa, qb);
-            st32(ou, o, pu0 + (ld32(bu, o) - qa) * ld32(rlu, o));
-            st32(ov, o, pv0 + (ld32(bv, o) - qb) * ld32(rlv, o));
+            ssa_core<T>(g, i, j, U, V, E, Ff, pu0, pv0, be_u[r], be_v[r], a_c[r], wf, qa, qb);
+            st32(ou, o, pu0 + (b_u[r] - qa) * r_u[r]);
+            st32(ov, o, pv0 + (b_v[r] - qb) * r_v[r]);
         }
     }
 }

@@ -160,6 +160,52 @@ __device__ __forceinline__ T gp_row_d(const KG &g, const T *__restrict__ pk, int
     }
     return q;
 }
+// gp_row_d in two halves: the 31 couplings of row RC at (i, j) loaded into registers (gp_coefs), then summed against
+// the iterate in gp_row_d's order (gp_row_cf). A tiled kernel issues the coupling loads before it stages its iterate
+// tile, so they are in flight together instead of one after the other across the block's barrier.
+template <typename T, int RC>
+__device__ __forceinline__ void gp_coefs(const KG &g, const T *__restrict__ pk, int i, int j, T (&cf)[31]) {
+    const long PP = gp_plane(g), p = gp_off(g, i, j);
+    int k = 0;
+    if constexpr (RC == 0) {
+        constexpr GBlk bu = gblk(0, 0), bv = gblk(0, 1);
+#pragma unroll
+        for (int dx = bu.x0; dx <= bu.x1; dx++)
+#pragma unroll
+            for (int dy = bu.y0; dy <= bu.y1; dy++)
+                cf[k++] = gp_stored(dx, dy) ? pk[gp_uu(dx, dy) * PP + p] : pk[gp_uu(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)];
+#pragma unroll
+        for (int dx = bv.x0; dx <= bv.x1; dx++)
+#pragma unroll
+            for (int dy = bv.y0; dy <= bv.y1; dy++) cf[k++] = pk[gp_uv(dx, dy) * PP + p];
+    } else {
+        constexpr GBlk bu = gblk(1, 0), bv = gblk(1, 1);
+#pragma unroll
+        for (int dx = bu.x0; dx <= bu.x1; dx++)
+#pragma unroll
+            for (int dy = bu.y0; dy <= bu.y1; dy++) cf[k++] = pk[gp_uv(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)];
+#pragma unroll
+        for (int dx = bv.x0; dx <= bv.x1; dx++)
+#pragma unroll
+            for (int dy = bv.y0; dy <= bv.y1; dy++)
+                cf[k++] = gp_stored(dx, dy) ? pk[gp_vv(dx, dy) * PP + p] : pk[gp_vv(-dx, -dy) * PP + gp_nb(g, i + dx, j + dy)];
+    }
+}
+template <typename T, int RC, typename FU, typename FV>
+__device__ __forceinline__ T gp_row_cf(const T (&cf)[31], FU XU, FV XV) {
+    constexpr GBlk bu = gblk(RC, 0), bv = gblk(RC, 1);
+    T q = T(0);
+    int k = 0;
+#pragma unroll
+    for (int dx = bu.x0; dx <= bu.x1; dx++)
+#pragma unroll
+        for (int dy = bu.y0; dy <= bu.y1; dy++) q += cf[k++] * XU(dx, dy);
+#pragma unroll
+    for (int dx = bv.x0; dx <= bv.x1; dx++)
+#pragma unroll
+        for (int dy = bv.y0; dy <= bv.y1; dy++) q += cf[k++] * XV(dx, dy);
+    return q;
+}
 // x through accessors fu(offset), fv(offset) as in g_row_f
 template <typename T, int RC, typename FU, typename FV>
 __device__ __forceinline__ T gp_row_f(const KG &g, const T *__restrict__ pk, int i, int j, FU fu, FV fv) {
@@ -478,6 +524,35 @@ template <typename T> struct ProSrc {
     const uint8_t *fact, *cact;
     const T *cu, *cv;
 };
+// The tile is read in two phases: every element a thread stages is loaded into registers first (a fixed number of
+// rounds, unrolled, so all of a thread's loads are in flight together instead of one round trip per element), then
+// written to LDS; a kernel issues the loads of its own point's operands between the two, so they overlap too.
+constexpr int GT_ITER = (GTH * GTW + NT - 1) / NT;
+template <typename T>
+__device__ __forceinline__ void fetch_gtile(const KG &g, const T *__restrict__ xu, const T *__restrict__ xv, int i0, int j0,
+                                            T (&va)[GT_ITER], T (&vb)[GT_ITER]) {
+    const int tid = threadIdx.y * BX + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < GT_ITER; k++) {
+        const int e0 = tid + k * NT, e = e0 < GTH * GTW ? e0 : GTH * GTW - 1;   // idle lanes reread the last element
+        const int tj = e / GTW, ti = e - tj * GTW;
+        // every column a real point reads lies in [−1, N + 2]; beyond it the clamp only keeps idle elements in range
+        const int i = clampi(i0 - 2 + ti, -1, g.Nx + 2), j = clampi(j0 - 2 + tj, -1, g.Ny + 2);
+        va[k] = xu[off(g, fx(g, i), cy(g, j))];
+        vb[k] = xv[off(g, cx(g, i), fy(g, j))];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void store_gtile(T (*tu)[GTW], T (*tv)[GTW], const T (&va)[GT_ITER], const T (&vb)[GT_ITER]) {
+    const int tid = threadIdx.y * BX + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < GT_ITER; k++) {   // idle lanes write the last element's value into it again (no branch)
+        const int e0 = tid + k * NT, e = e0 < GTH * GTW ? e0 : GTH * GTW - 1;
+        const int tj = e / GTW, ti = e - tj * GTW;
+        tu[tj][ti] = va[k];
+        tv[tj][ti] = vb[k];
+    }
+}
 template <typename T, bool PRO = false>
 __device__ __forceinline__ void load_gtile(const KG &g, const T *__restrict__ xu, const T *__restrict__ xv, T (*tu)[GTW], T (*tv)[GTW],
                                            int i0, int j0, const ProSrc<T> *pr = nullptr) {
@@ -526,6 +601,49 @@ __device__ __forceinline__ void gp_pass_t(KG g, const T *__restrict__ st, const 
         const T su = st[GS_PLANES * P + o], sv = st[(GS_PLANES + 1) * P + o];
         ou[o] = XU(0, 0) + (bu[o] - qu) * su;
         ov[o] = XV(0, 0) + (bv[o] - qv) * sv;
+    }
+}
+// gp_pass_t for the latency-bound levels (a few thousand to tens of thousands of points, where a launch is a handful
+// of round trips to memory): every tile element's loads and the row's couplings are issued together and waited for
+// once before the barrier. Same values and order, so the results are gp_pass_t's bit for bit. On the bandwidth-bound
+// levels the 62 couplings held in registers cost occupancy, so those keep gp_pass_t.
+template <typename T, int TX, int TY, bool RES>
+__device__ __forceinline__ void gp_pass_tl(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ xu,
+                                           const T *__restrict__ xv, const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ou,
+                                           T *__restrict__ ov) {
+    __shared__ T tu[GTH][GTW], tv[GTH][GTW];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int i0 = bx_ * BX + 1, j0 = by_ * BY + 1;
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool live = i <= g.Nx && j <= g.Ny;
+    const long o = live ? off(g, i, j) : 0, P = gplane(g);
+    T va[GT_ITER], vb[GT_ITER];
+    fetch_gtile<T>(g, xu, xv, i0, j0, va, vb);
+    // the row's couplings, right-hand side and 1/ℓ1 while the tile's loads are in flight (branch-free: a thread past
+    // the level's edge loads point (1, 1)'s and discards them, so the wait before the tile is written counts only the
+    // tile's loads)
+    T cu[31], cv[31];
+    const int ic = live ? i : 1, jc = live ? j : 1;
+    const long oc = off(g, ic, jc);
+    gp_coefs<T, 0>(g, pk, ic, jc, cu);
+    gp_coefs<T, 1>(g, pk, ic, jc, cv);
+    const T b_u = bu[oc], b_v = bv[oc];
+    const T s_u = RES ? T(0) : st[GS_PLANES * P + oc], s_v = RES ? T(0) : st[(GS_PLANES + 1) * P + oc];
+    store_gtile<T>(tu, tv, va, vb);
+    keep_loaded(cu); keep_loaded(cv); keep_loaded(b_u); keep_loaded(b_v); keep_loaded(s_u); keep_loaded(s_v);
+    __syncthreads();
+    if (!live) return;
+    auto XU = [&](int dx, int dy) { return tu[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    auto XV = [&](int dx, int dy) { return tv[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    const T qu = gp_row_cf<T, 0>(cu, XU, XV);
+    const T qv = gp_row_cf<T, 1>(cv, XU, XV);
+    if (RES) {
+        ou[o] = b_u - qu;
+        ov[o] = b_v - qv;
+    } else {
+        ou[o] = XU(0, 0) + (b_u - qu) * s_u;
+        ov[o] = XV(0, 0) + (b_v - qv) * s_v;
     }
 }
 template <typename T, int TX, int TY>
@@ -590,6 +708,77 @@ __global__ __launch_bounds__(NT) void k_gp_sweep_pre_t(KG g, const T *__restrict
     const T qv = gp_row_d<T, 1>(g, pk, i, j, XU, XV);
     ou[o] = bu[o] * su + (bu[o] - qu) * su;
     ov[o] = bv[o] * sv + (bv[o] - qv) * sv;
+}
+
+// k_gp_sweep_t and k_gp_residual_t for the latency-bound levels (gp_pass_tl): results bit for bit the same.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_sweep_tl(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ xu,
+                                                    const T *__restrict__ xv, const T *__restrict__ bu, const T *__restrict__ bv,
+                                                    T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    gp_pass_tl<T, TX, TY, false>(g, st, pk, xu, xv, bu, bv, ou, ov);
+}
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_residual_tl(KG g, const T *__restrict__ pk, const T *__restrict__ xu, const T *__restrict__ xv,
+                                                       const T *__restrict__ bu, const T *__restrict__ bv, T *__restrict__ ru,
+                                                       T *__restrict__ rv, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    gp_pass_tl<T, TX, TY, true>(g, nullptr, pk, xu, xv, bu, bv, ru, rv);
+}
+
+// k_gp_sweep_pre_t for the latency-bound levels, as gp_pass_tl: the same values and order, bit for bit.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_gp_sweep_pre_tl(KG g, const T *__restrict__ st, const T *__restrict__ pk, const T *__restrict__ bu,
+                                                        const T *__restrict__ bv, T *__restrict__ ou, T *__restrict__ ov, const Scal *gate) {
+    g.tx = TX; g.ty = TY;
+    if (solve_done(gate)) return;
+    __shared__ T tu[GTH][GTW], tv[GTH][GTW];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int i0 = bx_ * BX + 1, j0 = by_ * BY + 1;
+    const long P = gplane(g);
+    const T *su_p = st + GS_PLANES * P, *sv_p = st + (GS_PLANES + 1) * P;
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool live = i <= g.Nx && j <= g.Ny;
+    const long o = live ? off(g, i, j) : 0;
+    // b and 1/ℓ1 of the tile elements (all loads of a thread issued together), then the row's couplings, right-hand
+    // side and 1/ℓ1 while those are in flight; x₁ = b·(1/ℓ1) is formed as the tile is written
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    T tb_u[GT_ITER], ts_u[GT_ITER], tb_v[GT_ITER], ts_v[GT_ITER];
+#pragma unroll
+    for (int k = 0; k < GT_ITER; k++) {
+        const int e0 = tid + k * NT, e = e0 < GTH * GTW ? e0 : GTH * GTW - 1;   // idle lanes reread the last element
+        const int tj = e / GTW, ti = e - tj * GTW;
+        const int ii = clampi(i0 - 2 + ti, -1, g.Nx + 2), jj = clampi(j0 - 2 + tj, -1, g.Ny + 2);
+        const long qu = off(g, fx(g, ii), cy(g, jj)), qv = off(g, cx(g, ii), fy(g, jj));
+        tb_u[k] = bu[qu]; ts_u[k] = su_p[qu];
+        tb_v[k] = bv[qv]; ts_v[k] = sv_p[qv];
+    }
+    // (branch-free: a thread past the level's edge loads point (1, 1)'s and discards them)
+    T cu[31], cv[31];
+    const int ic = live ? i : 1, jc = live ? j : 1;
+    const long oc = off(g, ic, jc);
+    gp_coefs<T, 0>(g, pk, ic, jc, cu);
+    gp_coefs<T, 1>(g, pk, ic, jc, cv);
+    const T b_u = bu[oc], b_v = bv[oc], s_u = su_p[oc], s_v = sv_p[oc];
+#pragma unroll
+    for (int k = 0; k < GT_ITER; k++) {   // idle lanes write the last element's value into it again (no branch)
+        const int e0 = tid + k * NT, e = e0 < GTH * GTW ? e0 : GTH * GTW - 1;
+        const int tj = e / GTW, ti = e - tj * GTW;
+        tu[tj][ti] = tb_u[k] * ts_u[k];
+        tv[tj][ti] = tb_v[k] * ts_v[k];
+    }
+    keep_loaded(cu); keep_loaded(cv); keep_loaded(b_u); keep_loaded(b_v); keep_loaded(s_u); keep_loaded(s_v);
+    __syncthreads();
+    if (!live) return;
+    auto XU = [&](int dx, int dy) { return tu[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    auto XV = [&](int dx, int dy) { return tv[j - j0 + 2 + dy][i - i0 + 2 + dx]; };
+    const T qu = gp_row_cf<T, 0>(cu, XU, XV);
+    const T qv = gp_row_cf<T, 1>(cv, XU, XV);
+    ou[o] = b_u * s_u + (b_u - qu) * s_u;
+    ov[o] = b_v * s_v + (b_v - qv) * s_v;
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -756,27 +945,34 @@ template <typename T> struct GTailLevel {
 
 template <typename T> struct GTailDesc {
     int n, sweeps, coarsest;
-    int dgroups, o_dp;      // dense solve: column groups and the LDS byte offset of their FP64 partial products (G·n)
+    int dgroups, o_dp;      // dense solve: column groups and the LDS byte offset of their FP64 partial products (G·n),
+                            // followed by the coarsest right-hand side in FP64 (n, unknown order)
     const T *gbu, *gbv;     // right-hand side of the top tail level (global)
     T *gxu, *gxv;           // result of the top tail level (global)
     const double *cinv;     // FP64 dense inverse of the coarsest level, or null (sweeps)
     GTailLevel<T> lv[GTAIL_MAX];
 };
 
-// Development aid: built with -DNK_TAIL_TIMING, thread 0 of the tail stamps the device clock (100 MHz) at each phase
-// (load, each level down, coarsest, each level up, store) and prints the 41st call's phase times. Off by default.
+// Development aid: built with -DNK_TAIL_TIMING, thread 0 of the tail stamps the device clock (100 MHz) after the
+// load and after every step of the cycle and prints the 41st call's step times. Off by default.
 #ifdef NK_TAIL_TIMING
 __device__ unsigned int nk_tail_calls = 0;
-#define NK_TT(k) do { const int k_ = (k); if (threadIdx.x == 0 && k_ < 40) tt[k_] = wall_clock64(); } while (0)
+#define NK_TT(k) do { const int k_ = (k); if (threadIdx.x == 0 && k_ < 64) tt[k_] = wall_clock64(); } while (0)
 #else
 #define NK_TT(k) do { } while (0)
 #endif
+
+// The cycle is one loop over its steps (sweeps, residual, restriction, coarsest solve, prolongation), so each
+// operation's code appears once, and the rows of a pass are spread one per thread (u rows, then v rows), so a 16²
+// level keeps 512 threads busy instead of 256. Per row the arithmetic is that of g_row, k_restrict and k_prolongate.
+
+enum TailOp : int { TAIL_SWEEP = 0, TAIL_RESIDUAL = 1, TAIL_RESTRICT = 2, TAIL_DENSE = 3, TAIL_PROLONG = 4 };
 
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const Scal *gate) {
     if (solve_done(gate)) return;
 #ifdef NK_TAIL_TIMING
-    unsigned long long tt[40];
+    unsigned long long tt[64];
     int nt = 0;
 #endif
     NK_TT(nt++);
@@ -784,6 +980,10 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
     extern __shared__ __align__(16) unsigned char smem[];
     T *sm = reinterpret_cast<T *>(smem);
     const int tid = threadIdx.x, nth = blockDim.x;
+    // the coarsest right-hand side in FP64 in unknown order, written by the restriction into the coarsest level, so
+    // the dense solve reads b[c] directly (the promotion to FP64 is exact)
+    const int ncoarse = 2 * d.lv[d.n - 1].lg.Nx * d.lv[d.n - 1].lg.Ny;
+    double *bvec = d.cinv ? reinterpret_cast<double *>(smem + d.o_dp) + d.dgroups * ncoarse : nullptr;
     for (int l = 0; l < d.n; l++) {
         const GTailLevel<T> &L = d.lv[l];
         const int nx = L.lg.Nx + 1, ny = L.lg.Ny + 1;
@@ -819,160 +1019,148 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
         return sm + (which == 0 ? (comp == 0 ? L.o_x0u : L.o_x0v) : (comp == 0 ? L.o_x1u : L.o_x1v));
     };
     // One pass over the level's rows with the compact stencil at stb (LDS copy or global): an ℓ1 sweep
-    // o = x + (b − A x)/ℓ1 (o = b/ℓ1 when z), or the residual o = b − A x (res). Called with a pointer of one
-    // address space per call site, so the stencil loads are LDS or global loads, not flat ones.
-    auto pass = [&](const T *stb, const KG &g, const T *xu, const T *xv, const T *bu, const T *bv, T *ou, T *ov, bool z,
-                    bool res) __attribute__((always_inline)) {
+    // o = x + (b − A x)/ℓ1 (mode 0: o = b/ℓ1, the sweep from zero; mode 1: the sweep), or the residual o = b − A x
+    // (mode 2). Called with a pointer of one address space per call site, so the stencil loads are LDS or global
+    // loads, not flat ones.
+    auto rows = [&](const T *stb, const KG &g, const T *xu, const T *xv, const T *bu, const T *bv, T *ou, T *ov,
+                    int mode) __attribute__((always_inline)) {
         const long NP = (long)(g.Nx + 1) * (long)(g.Ny + 1);
-        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
+        const int npts = g.Nx * g.Ny;
+        for (int r = tid; r < 2 * npts; r += nth) {
+            const int rc = r >= npts, p = r - rc * npts;
             const int i = p % g.Nx + 1, j = p / g.Nx + 1;
             const long o = off(g, i, j);   // pitch Nx + 1: also the point's index in the compact stencil
-            const T su = stb[GS_PLANES * NP + o], sv = stb[(GS_PLANES + 1) * NP + o];
-            if (z) {
-                ou[o] = bu[o] * su;
-                ov[o] = bv[o] * sv;
+            const T s = stb[(GS_PLANES + rc) * NP + o];
+            const T b = rc ? bv[o] : bu[o];
+            T out;
+            if (mode == 0) {
+                out = b * s;
             } else {
-                const T qu = g_row<T, 0>(g, stb, NP, o, i, j, xu, xv);
-                const T qv = g_row<T, 1>(g, stb, NP, o, i, j, xu, xv);
-                if (res) {
-                    ou[o] = bu[o] - qu;
-                    ov[o] = bv[o] - qv;
-                } else {
-                    ou[o] = xu[o] + (bu[o] - qu) * su;
-                    ov[o] = xv[o] + (bv[o] - qv) * sv;
-                }
+                const T q = rc ? g_row<T, 1>(g, stb, NP, o, i, j, xu, xv) : g_row<T, 0>(g, stb, NP, o, i, j, xu, xv);
+                out = mode == 2 ? b - q : (rc ? xv[o] : xu[o]) + (b - q) * s;
             }
+            (rc ? ov : ou)[o] = out;
         }
     };
-    auto level_pass = [&](const GTailLevel<T> &L, const KG &g, const T *xu, const T *xv, const T *bu, const T *bv, T *ou, T *ov,
-                          bool z, bool res) {
-        if (L.o_st >= 0) pass(sm + L.o_st, g, xu, xv, bu, bv, ou, ov, z, res);
-        else pass(L.cst, g, xu, xv, bu, bv, ou, ov, z, res);
-    };
-    // sweeps of x ← x + (b − A x)/ℓ1 (x = b/ℓ1 first when from_zero); one barrier per sweep (ping-pong buffers)
-    auto smooth = [&](int l, int sweeps, bool from_zero) {
+    const int S = d.sweeps, nl = d.n;
+    const int per_down = S + 2, n_down = (nl - 1) * per_down;
+    const int n_coarse = d.cinv ? 1 : d.coarsest;
+    const int per_up = S + 1, n_steps = n_down + n_coarse + (nl - 1) * per_up;
+    for (int step = 0; step < n_steps; step++) {
+        int l, op, z = 0;
+        if (step < n_down) {            // down leg: S sweeps from zero, the residual, the restriction
+            l = step / per_down;
+            const int k = step - l * per_down;
+            op = k < S ? TAIL_SWEEP : (k == S ? TAIL_RESIDUAL : TAIL_RESTRICT);
+            z = k == 0;
+        } else if (step < n_down + n_coarse) {   // coarsest level: dense inverse, or sweeps from zero
+            l = nl - 1;
+            op = d.cinv ? TAIL_DENSE : TAIL_SWEEP;
+            z = step == n_down;
+        } else {                        // up leg: the prolongation, S sweeps
+            const int t = step - n_down - n_coarse, q = t / per_up;
+            l = nl - 2 - q;
+            op = t - q * per_up == 0 ? TAIL_PROLONG : TAIL_SWEEP;
+        }
         const GTailLevel<T> &L = d.lv[l];
         const KG g = geom(L.lg);
-        const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
-        for (int k = 0; k < sweeps; k++) {
-            const bool z = from_zero && k == 0;
-            const int src = C_(l), dst = z ? src : 1 - src;
-            level_pass(L, g, X(l, src, 0), X(l, src, 1), bu, bv, X(l, dst, 0), X(l, dst, 1), z, false);
-            cur = (cur & ~(1u << l)) | ((unsigned)dst << l);
+        if (op == TAIL_SWEEP || op == TAIL_RESIDUAL) {
+            const int src = C_(l), dst = op == TAIL_RESIDUAL ? src : (z ? src : 1 - src);
+            T *ou = op == TAIL_RESIDUAL ? sm + L.o_ru : X(l, dst, 0), *ov = op == TAIL_RESIDUAL ? sm + L.o_rv : X(l, dst, 1);
+            const int mode = op == TAIL_RESIDUAL ? 2 : (z ? 0 : 1);
+            if (L.o_st >= 0) rows(sm + L.o_st, g, X(l, src, 0), X(l, src, 1), sm + L.o_bu, sm + L.o_bv, ou, ov, mode);
+            else rows(L.cst, g, X(l, src, 0), X(l, src, 1), sm + L.o_bu, sm + L.o_bv, ou, ov, mode);
+            if (op == TAIL_SWEEP) cur = (cur & ~(1u << l)) | ((unsigned)dst << l);
+        } else if (op == TAIL_RESTRICT) {
+            const GTailLevel<T> &C = d.lv[l + 1];
+            const KG cg = geom(C.lg);
+            const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
+            const T *ru = sm + L.o_ru, *rv = sm + L.o_rv;
+            const int cn = cg.Nx * cg.Ny;
+            for (int r = tid; r < 2 * cn; r += nth) {
+                const int rc = r >= cn, p = r - rc * cn;
+                const int I = p % cg.Nx + 1, J = p / cg.Nx + 1;
+                T sum = T(0);
+                // fixed trip counts (as k_restrict): all reads issued together, off-support terms masked
+                if (rc == 0) {   // u: Face along x, Center along y
+#pragma unroll
+                    for (int a = -1; a <= 1; a++)
+#pragma unroll
+                        for (int b = -1; b <= 2; b++) {
+                            const bool on = (C.hx || a == 0) && (C.hy || b == 0);
+                            const int i = r_index(I, a, g.Nx, g.tx, 1, C.hx), j = r_index(J, b, g.Ny, g.ty, 0, C.hy);
+                            const T w = r_weight<T>(I, a, cg.Nx, g.tx, 1, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 0, C.hy);
+                            const T val = iu(g, fact, ru, i, j);
+                            if (on) sum += w * val;
+                        }
+                } else {         // v: Center along x, Face along y
+#pragma unroll
+                    for (int a = -1; a <= 2; a++)
+#pragma unroll
+                        for (int b = -1; b <= 1; b++) {
+                            const bool on = (C.hx || a == 0) && (C.hy || b == 0);
+                            const int i = r_index(I, a, g.Nx, g.tx, 0, C.hx), j = r_index(J, b, g.Ny, g.ty, 1, C.hy);
+                            const T w = r_weight<T>(I, a, cg.Nx, g.tx, 0, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 1, C.hy);
+                            const T val = iv(g, fact, rv, i, j);
+                            if (on) sum += w * val;
+                        }
+                }
+                const long o = off(cg, I, J);
+                const T bval = (cact[o] & (rc ? 2 : 1)) ? sum : T(0);
+                (rc ? sm + C.o_bv : sm + C.o_bu)[o] = bval;
+                if (d.cinv && l + 1 == nl - 1) bvec[r] = (double)bval;   // row r is unknown r of the dense solve
+            }
+        } else if (op == TAIL_DENSE) {
+            // x = M⁻¹ b: d.dgroups groups of n threads, group q summing columns [q·chunk, (q + 1)·chunk) of row r
+            // (coalesced: M is column-major), then the groups' partials added in group order; in FP64 whatever the
+            // V-cycle's precision (b promoted exactly, x rounded once to the level's precision)
+            const int n = 2 * g.Nx * g.Ny, G = d.dgroups, chunk = (n + G - 1) / G;
+            double *dp = reinterpret_cast<double *>(smem + d.o_dp);
+            if (tid < G * n) {
+                const int r = tid % n, q = tid / n;
+                const int c0 = q * chunk, c1 = c0 + chunk < n ? c0 + chunk : n;
+                double acc = 0.0;
+                constexpr int U = 16;   // loads of a batch issued together
+                for (int cb = c0; cb < c1; cb += U) {
+                    double m[U];
+#pragma unroll
+                    for (int t = 0; t < U; t++) m[t] = cb + t < c1 ? d.cinv[(long)(cb + t) * n + r] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < U; t++)
+                        if (cb + t < c1) acc += m[t] * bvec[cb + t];
+                }
+                dp[q * n + r] = acc;
+            }
             __syncthreads();
-        }
-    };
-    int l = 0;
-    for (; l < d.n - 1; l++) {
-        const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
-        const KG g = geom(L.lg), cg = geom(C.lg);
-        smooth(l, d.sweeps, true);
-        T *ru = sm + L.o_ru, *rv = sm + L.o_rv;
-        level_pass(L, g, X(l, C_(l), 0), X(l, C_(l), 1), sm + L.o_bu, sm + L.o_bv, ru, rv, false, true);
-        __syncthreads();
-        const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
-        T *cbu = sm + C.o_bu, *cbv = sm + C.o_bv;
-        for (int p = tid; p < cg.Nx * cg.Ny; p += nth) {
-            const int I = p % cg.Nx + 1, J = p / cg.Nx + 1;
-            T su = T(0), sv = T(0);
-            // fixed trip counts (as k_restrict): all reads issued together, off-support terms masked
-#pragma unroll
-            for (int a = -1; a <= 1; a++)
-#pragma unroll
-                for (int b = -1; b <= 2; b++) {
-                    const bool on = (C.hx || a == 0) && (C.hy || b == 0);
-                    const int i = r_index(I, a, g.Nx, g.tx, 1, C.hx), j = r_index(J, b, g.Ny, g.ty, 0, C.hy);
-                    const T w = r_weight<T>(I, a, cg.Nx, g.tx, 1, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 0, C.hy);
-                    const T val = iu(g, fact, ru, i, j);
-                    if (on) su += w * val;
-                }
-#pragma unroll
-            for (int a = -1; a <= 2; a++)
-#pragma unroll
-                for (int b = -1; b <= 1; b++) {
-                    const bool on = (C.hx || a == 0) && (C.hy || b == 0);
-                    const int i = r_index(I, a, g.Nx, g.tx, 0, C.hx), j = r_index(J, b, g.Ny, g.ty, 1, C.hy);
-                    const T w = r_weight<T>(I, a, cg.Nx, g.tx, 0, C.hx) * r_weight<T>(J, b, cg.Ny, g.ty, 1, C.hy);
-                    const T val = iv(g, fact, rv, i, j);
-                    if (on) sv += w * val;
-                }
-            const long o = off(cg, I, J);
-            cbu[o] = (cact[o] & 1) ? su : T(0);
-            cbv[o] = (cact[o] & 2) ? sv : T(0);
-        }
-        __syncthreads();
-        NK_TT(nt++);
-    }
-    // coarsest level: dense inverse, or sweeps
-    if (d.cinv) {
-        // x = M⁻¹ b: d.dgroups groups of n threads, group q summing columns [q·chunk, (q + 1)·chunk) of row r
-        // (coalesced: M is column-major), then the groups' partials added in group order
-        const GTailLevel<T> &L = d.lv[l];
-        const KG g = geom(L.lg);
-        // in FP64 whatever the V-cycle's precision: b promoted exactly, x rounded once to the level's precision
-        const int n = 2 * g.Nx * g.Ny, G = d.dgroups, chunk = (n + G - 1) / G;
-        const T *bu = sm + L.o_bu, *bv = sm + L.o_bv;
-        double *dp = reinterpret_cast<double *>(smem + d.o_dp);
-        if (tid < G * n) {
-            const int r = tid % n, q = tid / n;
-            const int c0 = q * chunk, c1 = c0 + chunk < n ? c0 + chunk : n;
-            double acc = 0.0;
-            constexpr int U = 16;   // loads of a batch issued together
-            for (int cb = c0; cb < c1; cb += U) {
-                double m[U];
-#pragma unroll
-                for (int t = 0; t < U; t++) m[t] = cb + t < c1 ? d.cinv[(long)(cb + t) * n + r] : 0.0;
-#pragma unroll
-                for (int t = 0; t < U; t++) {
-                    const int c = cb + t;
-                    if (c < c1) {
-                        const int cc = c >= n / 2, k = c - cc * (n / 2);
-                        const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-                        acc += m[t] * (double)(cc ? bv[o] : bu[o]);
-                    }
-                }
+            T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
+            for (int r = tid; r < n; r += nth) {
+                double acc = 0.0;
+                for (int q = 0; q < G; q++) acc += dp[q * n + r];
+                const int rc = r >= n / 2, k = r - rc * (n / 2);
+                const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
+                (rc ? xv : xu)[o] = (T)acc;
             }
-            dp[q * n + r] = acc;
+        } else {   // TAIL_PROLONG: x at level l += P (x at level l + 1)
+            const GTailLevel<T> &C = d.lv[l + 1];
+            const KG cg = geom(C.lg);
+            const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
+            const T *cu = X(l + 1, C_(l + 1), 0), *cv = X(l + 1, C_(l + 1), 1);
+            T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
+            const int npts = g.Nx * g.Ny;
+            for (int r = tid; r < 2 * npts; r += nth) {
+                const int rc = r >= npts, p = r - rc * npts;
+                const int i = p % g.Nx + 1, j = p / g.Nx + 1;
+                int I1, I2, J1, J2;
+                T wx1, wx2, wy1, wy2;
+                parents<T>(i, cg.Nx, g.tx, rc ? 0 : 1, C.hx, I1, I2, wx1, wx2);
+                parents<T>(j, cg.Ny, g.ty, rc ? 1 : 0, C.hy, J1, J2, wy1, wy2);
+                auto e = [&](int I, int J) { return rc ? iv(cg, cact, cv, I, J) : iu(cg, cact, cu, I, J); };
+                const T pr = wx1 * (wy1 * e(I1, J1) + wy2 * e(I1, J2)) + wx2 * (wy1 * e(I2, J1) + wy2 * e(I2, J2));
+                const long o = off(g, i, j);
+                (rc ? xv : xu)[o] += (fact[o] & (rc ? 2 : 1)) ? pr : T(0);
+            }
         }
         __syncthreads();
-        T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
-        for (int r = tid; r < n; r += nth) {
-            double acc = 0.0;
-            for (int q = 0; q < G; q++) acc += dp[q * n + r];
-            const int rc = r >= n / 2, k = r - rc * (n / 2);
-            const long o = off(g, k % g.Nx + 1, k / g.Nx + 1);
-            (rc ? xv : xu)[o] = (T)acc;
-        }
-        __syncthreads();
-    } else {
-        smooth(l, d.coarsest, true);
-    }
-    NK_TT(nt++);
-    // up leg
-    for (l = d.n - 2; l >= 0; l--) {
-        const GTailLevel<T> &L = d.lv[l], &C = d.lv[l + 1];
-        const KG g = geom(L.lg), cg = geom(C.lg);
-        const uint8_t *cact = smem + C.o_act, *fact = smem + L.o_act;
-        const T *cu = X(l + 1, C_(l + 1), 0), *cv = X(l + 1, C_(l + 1), 1);
-        T *xu = X(l, C_(l), 0), *xv = X(l, C_(l), 1);
-        for (int p = tid; p < g.Nx * g.Ny; p += nth) {
-            const int i = p % g.Nx + 1, j = p / g.Nx + 1;
-            int I1, I2, J1, J2;
-            T wx1, wx2, wy1, wy2;
-            parents<T>(i, cg.Nx, g.tx, 1, C.hx, I1, I2, wx1, wx2);
-            parents<T>(j, cg.Ny, g.ty, 0, C.hy, J1, J2, wy1, wy2);
-            const T pu = wx1 * (wy1 * iu(cg, cact, cu, I1, J1) + wy2 * iu(cg, cact, cu, I1, J2)) +
-                         wx2 * (wy1 * iu(cg, cact, cu, I2, J1) + wy2 * iu(cg, cact, cu, I2, J2));
-            parents<T>(i, cg.Nx, g.tx, 0, C.hx, I1, I2, wx1, wx2);
-            parents<T>(j, cg.Ny, g.ty, 1, C.hy, J1, J2, wy1, wy2);
-            const T pv = wx1 * (wy1 * iv(cg, cact, cv, I1, J1) + wy2 * iv(cg, cact, cv, I1, J2)) +
-                         wx2 * (wy1 * iv(cg, cact, cv, I2, J1) + wy2 * iv(cg, cact, cv, I2, J2));
-            const long o = off(g, i, j);
-            xu[o] += (fact[o] & 1) ? pu : T(0);
-            xv[o] += (fact[o] & 2) ? pv : T(0);
-        }
-        __syncthreads();
-        smooth(l, d.sweeps, false);
         NK_TT(nt++);
     }
     const GTailLevel<T> &L = d.lv[0];
@@ -988,7 +1176,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_tail(GTailDesc<T> d, const S
     if (threadIdx.x == 0 && atomicAdd(&nk_tail_calls, 1u) == 40u) {
         printf("tail n=%d cinv=%d sweeps=%d coarsest=%d\n", d.n, d.cinv != nullptr, d.sweeps, d.coarsest);
         for (int k = 0; k < d.n; k++) printf("  level %d: %d x %d stencil %s\n", k, d.lv[k].lg.Nx, d.lv[k].lg.Ny, d.lv[k].o_st >= 0 ? "LDS" : "global");
-        for (int k = 1; k < nt && k < 40; k++) printf("  stamp %d: +%.2f us\n", k, (double)(tt[k] - tt[k - 1]) * 0.01);
+        for (int k = 1; k < nt && k < 64; k++) printf("  stamp %d: +%.2f us\n", k, (double)(tt[k] - tt[k - 1]) * 0.01);
     }
 #endif
 }

@@ -111,6 +111,8 @@ struct nk_solver {
     bool fuse_pre = true;   // first two sweeps from zero in one kernel (NK_NO_PRE_FUSE=1 at creation: separate kernels)
     bool fuse_pro = true;   // packed Galerkin levels: prolongation inside the first post-sweep (NK_NO_PRO_FUSE=1: k_prolongate)
     long pro_fuse_points = 256L * 256;   // ... on levels of at most this many points (NK_PRO_FUSE_POINTS: tuning)
+    long latency_points = 256L * 256;    // packed Galerkin levels of at most this many points launch the latency-bound
+                                         // forms k_gp_sweep_tl, k_gp_residual_tl, k_gp_sweep_pre_tl (NK_LATENCY_POINTS: tuning)
     uint16_t *nm16 = nullptr;   // neighbour masks of the finest points (k_nmask), rebuilt with the finest activity flags
     bool use_z32 = true;        // mixed: the direction update reads the FP32 z (NK_NO_Z32=1: FP64 z via z64)
     bool apply_tile = true;     // FP64 finest operator applies and sweeps, and the packed Galerkin sweeps and residuals,
@@ -687,14 +689,14 @@ nk_status galerkin_tail(nk_solver *s, int l, Pair<T> x, const T *bu, const T *bv
         int *offs[] = {&t.o_x0u, &t.o_x0v, &t.o_x1u, &t.o_x1v, &t.o_bu, &t.o_bv, &t.o_ru, &t.o_rv};
         for (int *o : offs) { *o = off_t; off_t += plane; }
     }
-    if (d.cinv) {   // FP64 partial products of the dense solve, 8-byte aligned
+    if (d.cinv) {   // FP64 partial products of the dense solve and its FP64 right-hand side, 8-byte aligned
         const KG &gc = s->lev.back().g;
         const int n = 2 * gc.Nx * gc.Ny;
         d.dgroups = TAIL_THREADS / n > 0 ? TAIL_THREADS / n : 1;
         constexpr int per8 = 8 / (int)sizeof(T);
         const int at = (off_t + per8 - 1) / per8 * per8;
         d.o_dp = at * (int)sizeof(T);
-        off_t = at + d.dgroups * n * per8;
+        off_t = at + (d.dgroups + 1) * n * per8;   // + the coarsest right-hand side in FP64, in unknown order
     }
     {   // copy the stencils of the smallest levels into LDS while they fit (the coarsest needs none with a dense inverse)
         int flags = 0;
@@ -733,6 +735,7 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     const T *st = gs<T>(L);
     const T *pk = (s->use_gpack && L.gpk) ? (const T *)L.gpk : nullptr;
     const dim3 B = blocks_for(L.g.Nx, L.g.Ny);
+    const bool lat = (long)L.g.Nx * L.g.Ny <= s->latency_points;   // a latency-bound level
     auto smooth = [&](Pair<T> &cur, Pair<T> &oth, int sweeps, bool from_zero) -> nk_status {
         if (from_zero && sweeps == 0) {
             k_fill2<T><<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, cur.u, cur.v, T(0), gate);
@@ -740,7 +743,8 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
         }
         int k0 = 0;
         if (from_zero && sweeps >= 2 && s->fuse_pre) {   // the first two sweeps in one pass
-            if (pk && s->apply_tile) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre_t, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
+            if (pk && s->apply_tile && lat) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre_tl, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
+            else if (pk && s->apply_tile) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre_t, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
             else if (pk) NK_LAUNCH_TOPO(L.g, k_gp_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, pk, bu, bv, oth.u, oth.v, gate);
             else NK_LAUNCH_TOPO(L.g, k_g_sweep_pre, T, B, THREADS, 0, s->stream, L.g, st, bu, bv, oth.u, oth.v, gate);
             CHK(launch_check(s));
@@ -752,7 +756,10 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
                 NK_LAUNCH_TOPO(L.g, k_g_sweep, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, cur.u, cur.v, 1, gate);
             } else {
                 ProfScope ps(s, NK_PROFILE_GALERKIN_SWEEP, l, (int)sizeof(T));
-                if (pk && s->apply_tile)
+                if (pk && s->apply_tile && lat)
+                    NK_LAUNCH_TOPO_EV(L.g, k_gp_sweep_tl, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, pk, cur.u, cur.v, bu, bv,
+                                      oth.u, oth.v, gate);
+                else if (pk && s->apply_tile)
                     NK_LAUNCH_TOPO_EV(L.g, k_gp_sweep_t, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, L.g, st, pk, cur.u, cur.v, bu, bv,
                                       oth.u, oth.v, gate);
                 else if (pk)
@@ -781,7 +788,9 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     CHK(smooth(cur, oth, s->op.smoothing_sweeps, true));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
-    if (pk && s->apply_tile)
+    if (pk && s->apply_tile && lat)
+        NK_LAUNCH_TOPO(L.g, k_gp_residual_tl, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+    else if (pk && s->apply_tile)
         NK_LAUNCH_TOPO(L.g, k_gp_residual_t, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
     else if (pk) NK_LAUNCH_TOPO(L.g, k_gp_residual, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
     else NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
@@ -1505,6 +1514,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->fuse_pro = !(npro && npro[0] == '1');
         const char *nproc = std::getenv("NK_PRO_FUSE_POINTS");
         if (nproc) s->pro_fuse_points = std::strtol(nproc, nullptr, 10);
+        const char *nlat = std::getenv("NK_LATENCY_POINTS");
+        if (nlat) s->latency_points = std::strtol(nlat, nullptr, 10);
         const char *nt = std::getenv("NK_NO_TAIL");
         if (nt && nt[0] == '1') s->tail_start[0] = s->tail_start[1] = -1;
         // the LDS tail runs the rediscretized operator: levels with Galerkin operators have a tail of their own

@@ -309,3 +309,23 @@ def test_refined_mixed_form_matches_fp64(N):
         f"(last solve) {info['refinement_steps']}; FP64 picard {out[abi.PRECISION_FP64][2]['picard_iterations']}; rel L2 {err:.2e}")
     assert err <= 1e-8
     assert info["refinement_steps"] >= 1
+
+
+@pytest.mark.parametrize("precision,topology,N,Ny", [(MIX, (B, B), 128, None), (MIX, (P, P), 96, None), (MIX, (B, P), 160, 96),
+                                                     (F64, (P, B), 128, None)])
+def test_latency_bound_forms_equal_bandwidth_forms(monkeypatch, precision, topology, N, Ny):
+    """On small packed Galerkin levels the sweeps and residuals issue every load of a thread (tile elements and the
+    row's 31 couplings) before one wait (k_gp_sweep_tl, k_gp_residual_tl, k_gp_sweep_pre_tl); larger levels load the
+    tile, then the couplings after the block's barrier. NK_LATENCY_POINTS at creation sets the largest level that
+    takes the first forms: 0 (none) and 10⁸ (all) must give the same solutions and iteration counts bit for bit."""
+    out = []
+    for points in ("0", "100000000"):
+        monkeypatch.setenv("NK_LATENCY_POINTS", points)
+        m, info = channel(N, abi.COARSE_GALERKIN, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    monkeypatch.delenv("NK_LATENCY_POINTS", raising=False)
+    say(f"[galerkin] latency-bound vs bandwidth-bound level kernels (precision {precision}, {topology}, {N}x{Ny or N}): "
+        f"CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
