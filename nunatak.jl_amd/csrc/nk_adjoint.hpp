@@ -4,14 +4,31 @@
 // (src/StressBalances/stress_balance_coefficients.jl:10-20). These kernels form ȳ = (∂q/∂x)ᵀ q̄ and, in the parameter
 // pass, C̄ = (∂q/∂C)ᵀ q̄ and Ā = (∂q/∂A)ᵀ q̄, following the forward kernels' index maps (a clamped or wrapped
 // neighbour receives the adjoint of every read that landed on it, which is the adjoint of Oceananigans' halo fill).
-// Contributions that several threads add to one point use FP64 atomics.
+//
+// Scatter without atomics. A scatter kernel's thread (i, j) adds into points (i + a, j + b) with |a|, |b| ≤ 1 after
+// the forward kernel's clamp or wrap. It runs as one launch per colour class: the points of a class lie at least
+// three apart along each direction (cyclically on a Periodic one), so no two threads of a launch touch one address,
+// every add is a plain load-add-store, and each sum is formed in the same order on every run (bit-reproducible
+// gradients). The host enumerates the classes (adjoint_classes in nk_solver.hip).
 #pragma once
 #include "nk_device.hpp"
 #include "nk_physics.hpp"
 
 namespace nk {
 
-__device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
+// A colour class: i = ox + sx·t for t < nx, j = oy + sy·u for u < ny.
+struct Colour {
+    int ox, sx, nx, oy, sy, ny;
+};
+#define NK_IJ_COL(col)                                                   \
+    int bx_, by_;                                                        \
+    xcd_tile(bx_, by_);                                                  \
+    const int ti_ = bx_ * BX + threadIdx.x, tj_ = by_ * BY + threadIdx.y; \
+    if (ti_ >= col.nx || tj_ >= col.ny) return;                          \
+    const int i = col.ox + col.sx * ti_, j = col.oy + col.sy * tj_;
+
+// Plain add: the launch's colour class guarantees that no other thread of the launch writes *p.
+__device__ __forceinline__ void aadd(double *p, double v) { *p += v; }
 
 // q̄ of an operator row, zero where the row is an identity row (inactive)
 __device__ __forceinline__ double qbar_u(const KG &g, const uint8_t *act, const double *qb, int i, int j) {
@@ -37,10 +54,10 @@ __global__ __launch_bounds__(NT) void k_adj_centres(KG g, const uint8_t *__restr
 }
 
 // Corners: η̄Hᶠᶠ̄ = T̄xy · 2ε̇xy, then η̄H̄ += ℑxyᵀ(η̄Hᶠᶠ̄) onto the (clamped / wrapped) centres.
-__global__ __launch_bounds__(NT) void k_adj_corners(KG g, const uint8_t *__restrict__ act, double wf, const double *__restrict__ xu,
+__global__ __launch_bounds__(NT) void k_adj_corners(KG g, Colour col, const uint8_t *__restrict__ act, double wf, const double *__restrict__ xu,
                                                     const double *__restrict__ xv, const double *__restrict__ qbu,
                                                     const double *__restrict__ qbv, double *__restrict__ etab) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > nxf(g) || j > nyf(g)) return;
     double exy = ((iu(g, act, xu, i, j) - iu(g, act, xu, i, j - 1)) / g.dy + (iv(g, act, xv, i, j) - iv(g, act, xv, i - 1, j)) / g.dx) / 2.0;
     exy = periph_ff(g, i, j) ? wf * exy : exy;
@@ -56,13 +73,13 @@ __global__ __launch_bounds__(NT) void k_adj_corners(KG g, const uint8_t *__restr
 
 // Centres: η̄H̄ → Ē (η̄H ∝ Eᵉ, e = (1 − n)/2n) → strain-rate adjoints (viscosity masks: walls only); and, in the
 // parameter pass, Ā (η̄H ∝ A^(−1/n) per layer).
-__global__ __launch_bounds__(NT) void k_adj_viscosity(KG g, PhysK p, Sig sg, const double *__restrict__ xu, const double *__restrict__ xv,
+__global__ __launch_bounds__(NT) void k_adj_viscosity(KG g, Colour col, PhysK p, Sig sg, const double *__restrict__ xu, const double *__restrict__ xv,
                                                       const double *__restrict__ eta, const double *__restrict__ es, long plane,
                                                       const double *__restrict__ A, const double *__restrict__ H,
                                                       const double *__restrict__ mu, const double *__restrict__ etab,
                                                       double *__restrict__ yu, double *__restrict__ yv, double *__restrict__ exyb,
                                                       double *__restrict__ Ab, int velocity, int parameters) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
     const double eb = etab[o];
@@ -101,9 +118,9 @@ __global__ __launch_bounds__(NT) void k_adj_viscosity(KG g, PhysK p, Sig sg, con
 }
 
 // Corners: ε̇xy adjoint → velocity adjoint (viscosity masks)
-__global__ __launch_bounds__(NT) void k_adj_shear(KG g, double wf, const double *__restrict__ exyb, double *__restrict__ yu,
+__global__ __launch_bounds__(NT) void k_adj_shear(KG g, Colour col, double wf, const double *__restrict__ exyb, double *__restrict__ yu,
                                                   double *__restrict__ yv) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > nxf(g) || j > nyf(g)) return;
     const double b = exyb[off(g, i, j)];
     if (b == 0.0) return;
@@ -131,14 +148,14 @@ __device__ __forceinline__ void drag_derivatives(const PhysK &p, double U, doubl
 
 // Faces: β̄ = Az x* q̄ on active rows (or the given β̄ — the DIVA chain passes βᵇ̄); β = φ·drag(U(u, v), ℑx C, ℑx N)
 // → velocity adjoint and, in the parameter pass, C̄.
-__global__ __launch_bounds__(NT) void k_adj_drag(KG g, PhysK p, const uint8_t *__restrict__ act, const double *__restrict__ xu,
+__global__ __launch_bounds__(NT) void k_adj_drag(KG g, Colour col, PhysK p, const uint8_t *__restrict__ act, const double *__restrict__ xu,
                                                  const double *__restrict__ xv, const double *__restrict__ qbu,
                                                  const double *__restrict__ qbv, const double *__restrict__ C,
                                                  const double *__restrict__ N, const double *__restrict__ phiu,
                                                  const double *__restrict__ phiv, double *__restrict__ yu, double *__restrict__ yv,
                                                  double *__restrict__ Cb, int velocity, int parameters,
                                                  const double *__restrict__ bbar_u, const double *__restrict__ bbar_v) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > g.Nx + 1 || j > g.Ny + 1) return;
     if (p.friction == 0) return;
     const long o = off(g, i, j);
@@ -210,13 +227,13 @@ __global__ __launch_bounds__(NT) void k_adj_drag(KG g, PhysK p, const uint8_t *_
 
 // Faces: β̄ = q̄·Az·x* (operator rows) through β = βᵇ/(1 + βᵇ F̄ˣ): βᵇ̄ += β̄/den², F₂̄ += −β̄ βᵇ²/den² / 2 to the two
 // centres (diva.jl:35-51).
-__global__ __launch_bounds__(NT) void k_adj_effective_drag(KG g, const uint8_t *__restrict__ act, const double *__restrict__ xu,
+__global__ __launch_bounds__(NT) void k_adj_effective_drag(KG g, Colour col, const uint8_t *__restrict__ act, const double *__restrict__ xu,
                                                            const double *__restrict__ xv, const double *__restrict__ qbu,
                                                            const double *__restrict__ qbv, const double *__restrict__ bbu,
                                                            const double *__restrict__ bbv, const double *__restrict__ F2,
                                                            double *__restrict__ bbu_b, double *__restrict__ bbv_b,
                                                            double *__restrict__ F2_b) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > g.Nx + 1 || j > g.Ny + 1) return;
     const long o = off(g, i, j);
     const double Az = g.dx * g.dy;
@@ -248,7 +265,7 @@ __global__ __launch_bounds__(NT) void k_adj_effective_drag(KG g, const uint8_t *
 // η̄H̄ (etab), F̄₂′ (F2_b), ησ̄′ (es_b, σ layers). Outputs: strain-rate adjoints onto u, v and the corners (as the SSA
 // kernel), βᵇ̄ and uᵇ̄ of the faces i, i + 1 (j, j + 1) through the basal shear, ησ̄ of the lagged layers (es_b_old),
 // and Ā in the parameter pass.
-__global__ __launch_bounds__(NT) void k_adj_viscosity_diva(KG g, PhysK p, Sig sg, const double *__restrict__ xu,
+__global__ __launch_bounds__(NT) void k_adj_viscosity_diva(KG g, Colour col, PhysK p, Sig sg, const double *__restrict__ xu,
                                                            const double *__restrict__ xv, const double *__restrict__ es, long plane,
                                                            const double *__restrict__ A, const double *__restrict__ H,
                                                            const double *__restrict__ mu, const double *__restrict__ bbu,
@@ -260,7 +277,7 @@ __global__ __launch_bounds__(NT) void k_adj_viscosity_diva(KG g, PhysK p, Sig sg
                                                            double *__restrict__ ub_b, double *__restrict__ vb_b,
                                                            double *__restrict__ es_b_old, double *__restrict__ Ab, int velocity,
                                                            int parameters) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
     const double Hm = H[o] * (mu ? mu[o] : 1.0);
@@ -328,13 +345,13 @@ __global__ __launch_bounds__(NT) void k_adj_viscosity_diva(KG g, PhysK p, Sig sg
 
 // Faces: uᵇ = u / (1 + βᵇ ℑx F₂) backwards (diva.jl:1-27): ū += uᵇ̄/D; the lagged βᵇ and F₂ get −uᵇ̄ uᵇ/D times
 // the other factor (F₂ halved onto the two centres).
-__global__ __launch_bounds__(NT) void k_adj_basal_velocity(KG g, const double *__restrict__ ub, const double *__restrict__ vb,
+__global__ __launch_bounds__(NT) void k_adj_basal_velocity(KG g, Colour col, const double *__restrict__ ub, const double *__restrict__ vb,
                                                            const double *__restrict__ bbu, const double *__restrict__ bbv,
                                                            const double *__restrict__ F2, const double *__restrict__ ub_b,
                                                            const double *__restrict__ vb_b, double *__restrict__ yu,
                                                            double *__restrict__ yv, double *__restrict__ bbu_b_old,
                                                            double *__restrict__ bbv_b_old, double *__restrict__ F2_b_old) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > g.Nx + 1 || j > g.Ny + 1) return;
     const long o = off(g, i, j);
     if (i <= nxf(g) && j <= g.Ny) {
@@ -368,14 +385,14 @@ __global__ __launch_bounds__(NT) void k_adj_basal_velocity(KG g, const double *_
 // Faces: compute_velocity_profile! backwards (diva.jl:89-109): uσₖ = uᵇ (1 + βᵇ F₁ₖ), F₁ₖ = ℑx SIₖ,
 // SIₖ(c) = H(c) Σ_{k′≤k} pw[k][k′] / ησ(c, k′). From ūσ: uᵇ̄ += Σ ūσₖ (1 + βᵇ F₁ₖ), βᵇ̄ += Σ ūσₖ uᵇ F₁ₖ, and
 // ησ̄(c, k′) −= ūσₖ uᵇ βᵇ / 2 · H pw[k][k′] / ησ(c, k′)² for both centres c of the face.
-__global__ __launch_bounds__(NT) void k_adj_velocity_profile(KG g, Sig sg, int Ns, long plane, const double *__restrict__ es,
+__global__ __launch_bounds__(NT) void k_adj_velocity_profile(KG g, Colour col, Sig sg, int Ns, long plane, const double *__restrict__ es,
                                                              const double *__restrict__ H, const double *__restrict__ ub,
                                                              const double *__restrict__ vb, const double *__restrict__ bbu,
                                                              const double *__restrict__ bbv, const double *__restrict__ usb,
                                                              const double *__restrict__ vsb, double *__restrict__ ub_b,
                                                              double *__restrict__ vb_b, double *__restrict__ bbu_b,
                                                              double *__restrict__ bbv_b, double *__restrict__ es_b) {
-    NK_IJ
+    NK_IJ_COL(col)
     if (i > g.Nx + 1 || j > g.Ny + 1) return;
     const long o = off(g, i, j);
     auto SI = [&](long oc, int k) {

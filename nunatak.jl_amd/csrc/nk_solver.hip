@@ -2056,6 +2056,51 @@ nk_status nk_rate_factor_from_temperature(nk_solver *s) {
 
 namespace {
 
+// Colour classes of one direction for the scatter kernels of nk_adjoint.hpp, whose thread at point i adds into
+// points i − 1 .. i + 1 (after the clamp or wrap), over points 1 .. extent. Bounded: the points ≡ 1, 2, 0 (mod 3).
+// Periodic (only points 1 .. N write anything): the first M = 3⌊N/3⌋ points in three classes of stride 3 — two points
+// of one class are then at least 3 apart also across the wrap, the N − M remaining points lying between the class's
+// last and first — and each of the N − M remaining points in a class of its own.
+extern "C++" {
+struct DirClasses {
+    int n = 0;
+    int o[5], s[5], c[5];
+};
+DirClasses adjoint_classes(int N, int extent, bool periodic) {
+    DirClasses d;
+    const int span = periodic ? 3 * (N / 3) : extent;
+    for (int k = 0; k < 3 && k < span; k++) {
+        d.o[d.n] = 1 + k;
+        d.s[d.n] = 3;
+        d.c[d.n] = (span - 1 - k) / 3 + 1;
+        d.n++;
+    }
+    if (periodic)
+        for (int k = span; k < N; k++) {
+            d.o[d.n] = k + 1;
+            d.s[d.n] = 1;
+            d.c[d.n] = 1;
+            d.n++;
+        }
+    return d;
+}
+
+// Runs launch(blocks, colour) once per colour class of the points 1..ex × 1..ey, in a fixed order on the handle's
+// stream.
+template <typename Launch>
+nk_status coloured(nk_solver *s, int ex, int ey, Launch launch) {
+    const DirClasses X = adjoint_classes(s->g.Nx, ex, s->g.tx == NK_PERIODIC);
+    const DirClasses Y = adjoint_classes(s->g.Ny, ey, s->g.ty == NK_PERIODIC);
+    for (int b = 0; b < Y.n; b++)
+        for (int a = 0; a < X.n; a++) {
+            const Colour col{X.o[a], X.s[a], X.c[a], Y.o[b], Y.s[b], Y.c[b]};
+            launch(blocks_for(col.nx, col.ny), col);
+            CHK(launch_check(s));
+        }
+    return NK_OK;
+}
+}   // extern "C++"
+
 // One transposed coefficient step: ȳ = (∂q/∂x)ᵀ q̄ (velocity) and/or C̄, Ā (parameters), with q̄ in ad_q.
 nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters) {
     const KG &g = s->g;
@@ -2071,21 +2116,24 @@ nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters) {
     }
     k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
     CHK(launch_check(s));
-    k_adj_corners<<<Be, THREADS, 0, s->stream>>>(g, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
-    CHK(launch_check(s));
-    k_adj_viscosity<<<Bc, THREADS, 0, s->stream>>>(g, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETAH],
-                                                    f[NK_FIELD_ETASIGMA], s->plane, f[NK_FIELD_A], f[NK_FIELD_H],
-                                                    s->ph.masking ? f[NK_FIELD_MU] : nullptr, s->ad_eta, s->ad_yu, s->ad_yv,
-                                                    s->ad_exy, f[NK_FIELD_ABAR], velocity ? 1 : 0, parameters ? 1 : 0);
-    CHK(launch_check(s));
-    if (velocity) {
-        k_adj_shear<<<Be, THREADS, 0, s->stream>>>(g, p.wf, s->ad_exy, s->ad_yu, s->ad_yv);
-        CHK(launch_check(s));
-    }
-    k_adj_drag<<<Be, THREADS, 0, s->stream>>>(g, p, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, f[NK_FIELD_C], f[NK_FIELD_N],
-                                               f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_yu, s->ad_yv, f[NK_FIELD_CBAR],
-                                               velocity ? 1 : 0, parameters ? 1 : 0, nullptr, nullptr);
-    return launch_check(s);
+    CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+        k_adj_corners<<<B, THREADS, 0, s->stream>>>(g, col, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    }));
+    CHK(coloured(s, g.Nx, g.Ny, [&](dim3 B, Colour col) {
+        k_adj_viscosity<<<B, THREADS, 0, s->stream>>>(g, col, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETAH],
+                                                      f[NK_FIELD_ETASIGMA], s->plane, f[NK_FIELD_A], f[NK_FIELD_H],
+                                                      s->ph.masking ? f[NK_FIELD_MU] : nullptr, s->ad_eta, s->ad_yu, s->ad_yv,
+                                                      s->ad_exy, f[NK_FIELD_ABAR], velocity ? 1 : 0, parameters ? 1 : 0);
+    }));
+    if (velocity)
+        CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+            k_adj_shear<<<B, THREADS, 0, s->stream>>>(g, col, p.wf, s->ad_exy, s->ad_yu, s->ad_yv);
+        }));
+    return coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+        k_adj_drag<<<B, THREADS, 0, s->stream>>>(g, col, p, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, f[NK_FIELD_C],
+                                                 f[NK_FIELD_N], f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_yu, s->ad_yv, f[NK_FIELD_CBAR],
+                                                 velocity ? 1 : 0, parameters ? 1 : 0, nullptr, nullptr);
+    });
 }
 
 // One transposed DIVA coefficient step (update_diva_coefficients!, src/StressBalances/diva.jl:60-75), q̄ in ad_q and the
@@ -2111,28 +2159,35 @@ nk_status adjoint_step_diva(nk_solver *s, bool parameters) {
     CHK(launch_check(s));
     k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
     CHK(launch_check(s));
-    k_adj_corners<<<Be, THREADS, 0, s->stream>>>(g, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
-    CHK(launch_check(s));
-    k_adj_effective_drag<<<Be, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, f[NK_FIELD_BETABU],
-                                                         f[NK_FIELD_BETABV], f[NK_FIELD_F2], s->ad_bbu, s->ad_bbv, s->ad_F2n);
-    CHK(launch_check(s));
-    k_adj_viscosity_diva<<<Bc, THREADS, 0, s->stream>>>(g, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETASIGMA], s->plane,
-                                                         f[NK_FIELD_A], f[NK_FIELD_H], s->ph.masking ? f[NK_FIELD_MU] : nullptr,
-                                                         f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_UB], f[NK_FIELD_VB],
-                                                         s->ad_eta, s->ad_F2n, s->st_es, s->ad_yu, s->ad_yv, s->ad_exy, s->ad_bbu,
-                                                         s->ad_bbv, s->ad_ubb, s->ad_vbb, s->out_es, f[NK_FIELD_ABAR], 1,
-                                                         parameters ? 1 : 0);
-    CHK(launch_check(s));
-    k_adj_shear<<<Be, THREADS, 0, s->stream>>>(g, p.wf, s->ad_exy, s->ad_yu, s->ad_yv);
-    CHK(launch_check(s));
-    k_adj_drag<<<Be, THREADS, 0, s->stream>>>(g, p, act, f[NK_FIELD_UB], f[NK_FIELD_VB], s->ad_qu, s->ad_qv, f[NK_FIELD_C], f[NK_FIELD_N],
-                                               f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_ubb, s->ad_vbb, f[NK_FIELD_CBAR], 1,
-                                               parameters ? 1 : 0, s->ad_bbu, s->ad_bbv);
-    CHK(launch_check(s));
-    k_adj_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
-                                                         f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->out_bbu,
-                                                         s->out_bbv, s->out_F2);
-    return launch_check(s);
+    CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+        k_adj_corners<<<B, THREADS, 0, s->stream>>>(g, col, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    }));
+    CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+        k_adj_effective_drag<<<B, THREADS, 0, s->stream>>>(g, col, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv,
+                                                           f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_F2], s->ad_bbu,
+                                                           s->ad_bbv, s->ad_F2n);
+    }));
+    CHK(coloured(s, g.Nx, g.Ny, [&](dim3 B, Colour col) {
+        k_adj_viscosity_diva<<<B, THREADS, 0, s->stream>>>(g, col, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETASIGMA],
+                                                           s->plane, f[NK_FIELD_A], f[NK_FIELD_H],
+                                                           s->ph.masking ? f[NK_FIELD_MU] : nullptr, f[NK_FIELD_BETABU],
+                                                           f[NK_FIELD_BETABV], f[NK_FIELD_UB], f[NK_FIELD_VB], s->ad_eta, s->ad_F2n,
+                                                           s->st_es, s->ad_yu, s->ad_yv, s->ad_exy, s->ad_bbu, s->ad_bbv, s->ad_ubb,
+                                                           s->ad_vbb, s->out_es, f[NK_FIELD_ABAR], 1, parameters ? 1 : 0);
+    }));
+    CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+        k_adj_shear<<<B, THREADS, 0, s->stream>>>(g, col, p.wf, s->ad_exy, s->ad_yu, s->ad_yv);
+    }));
+    CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+        k_adj_drag<<<B, THREADS, 0, s->stream>>>(g, col, p, act, f[NK_FIELD_UB], f[NK_FIELD_VB], s->ad_qu, s->ad_qv, f[NK_FIELD_C],
+                                                 f[NK_FIELD_N], f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_ubb, s->ad_vbb,
+                                                 f[NK_FIELD_CBAR], 1, parameters ? 1 : 0, s->ad_bbu, s->ad_bbv);
+    }));
+    return coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
+        k_adj_basal_velocity<<<B, THREADS, 0, s->stream>>>(g, col, f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_BETABU],
+                                                           f[NK_FIELD_BETABV], f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu,
+                                                           s->ad_yv, s->out_bbu, s->out_bbv, s->out_F2);
+    });
 }
 
 // EnzymeRules.reverse for converged picard_solve! (ext/NunatakEnzymeExt.jl:175-240), SSA coefficients.
@@ -2174,15 +2229,17 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
         k_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
                                                          f[NK_FIELD_F2], ubp, vbp);
         CHK(launch_check(s));
-        k_adj_velocity_profile<<<Be, THREADS, 0, s->stream>>>(g, sig_tables(s), s->Ns, s->plane, f[NK_FIELD_ETASIGMA], f[NK_FIELD_H],
-                                                               ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
-                                                               f[NK_FIELD_USIGMABAR], f[NK_FIELD_VSIGMABAR], s->ad_ubb, s->ad_vbb,
-                                                               s->cs_bbu, s->cs_bbv, s->cs_es);
-        CHK(launch_check(s));
-        k_adj_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
-                                                             f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->cs_bbu,
-                                                             s->cs_bbv, s->cs_F2);
-        CHK(launch_check(s));
+        CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 Bk, Colour col) {
+            k_adj_velocity_profile<<<Bk, THREADS, 0, s->stream>>>(g, col, sig_tables(s), s->Ns, s->plane, f[NK_FIELD_ETASIGMA],
+                                                                  f[NK_FIELD_H], ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                                  f[NK_FIELD_USIGMABAR], f[NK_FIELD_VSIGMABAR], s->ad_ubb, s->ad_vbb,
+                                                                  s->cs_bbu, s->cs_bbv, s->cs_es);
+        }));
+        CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 Bk, Colour col) {
+            k_adj_basal_velocity<<<Bk, THREADS, 0, s->stream>>>(g, col, ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                                f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->cs_bbu,
+                                                                s->cs_bbv, s->cs_F2);
+        }));
         k_add<<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_yu, s->ad_yv, s->ad_xbu, s->ad_xbv);
         CHK(launch_check(s));
         xbu = s->ad_xbu;

@@ -482,3 +482,42 @@ def test_geometry_gradients_along_random_directions(case):
             worst = max(worst, rel)
             say(f"[adjoint] {case} d{name} direction {k}: adjoint {ad:.10e}, fd {fd:.10e}, rel {rel:.2e}")
     assert worst < 1e-6
+
+
+# ---- reproducibility -------------------------------------------------------------------------------------------------
+
+def test_adjoint_is_bit_reproducible():
+    """The transposed coefficient kernels add into their neighbours in coloured passes (no atomics), so two adjoints
+    of the same solve on fresh handles give the same gradients bit for bit and the same iteration counts: the ridge
+    model (Bounded × Periodic with Ny = 4, so a Periodic direction has a point outside the stride-3 classes) through
+    the mixed Galerkin form, and the marine geometry (50², DIVA, split driving stress) with its geometry gradients."""
+    def ridge_mixed():
+        m, g = ridge(abi.PRECONDITIONER_MULTIGRID)
+        o = m.options
+        o.precision, o.coarse_operator = abi.PRECISION_MIXED, abi.COARSE_GALERKIN
+        m2 = HIPModel(g, m.physics, o)
+        for name in ("H", "BED", "A", "C"):
+            m2.set(name, m.get(name))
+        _, info = adjoint_of_cost(m2, g)
+        return m2, info
+
+    def marine_diva():
+        m, g, fields = marine_geometry(50, abi.GROUNDING_SUBGRID_DRIVING_STRESS, abi.FRICTION_WEERTMAN, diva=True)
+        speed_cost(m, g, fields, fields["H"], fields["BED"])
+        u, v = m.get("U"), m.get("V")
+        m.set("UBAR", 2 * u * W.YEAR ** 2)
+        m.set("VBAR", 2 * v * W.YEAR ** 2)
+        return m, m.adjoint()
+
+    for name, make in (("ridge mixed", ridge_mixed), ("marine DIVA 50x50", marine_diva)):
+        runs = []
+        for _ in range(2):
+            m, info = make()
+            runs.append(({k: m.get(k).copy() for k in ("CBAR", "ABAR", "HBAR", "BEDBAR", "LAMBDAU", "LAMBDAV")},
+                         (info["picard_iterations"], info["total_linear_iterations"])))
+        (a, ia), (b, ib) = runs
+        say(f"[adjoint] {name}: iterations {ia} and {ib}")
+        assert ia == ib
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+        assert np.max(np.abs(a["CBAR"])) > 0 and np.max(np.abs(a["HBAR"])) > 0
