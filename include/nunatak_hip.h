@@ -374,7 +374,15 @@ nk_status nk_till_effective_pressure(nk_solver *solver, const nk_till *till);
  * src/Models/ice_sheet_model_methods.jl:103-111), the driving stress with its calving-front and grounding-line terms
  * (src/StressBalances/driving_stress.jl:56-125), the H μ factor of η̄H, the H factor of F₂ (DIVA) and the drag's φ
  * and N — the gradient of the whole update_velocities!(model) with geometry computed on the device (nk_update_velocities
- * with geometry = 1), as the reference's test differentiates update_state! (test/test_enzyme.jl:341-356). */
+ * with geometry = 1), as the reference's test differentiates update_state! (test/test_enzyme.jl:341-356).
+ * With a FixedIterations(n) Picard control the converged-Picard rule does not apply (ext/NunatakEnzymeExt.jl:121 matches
+ * Convergence only) and the reference's Enzyme differentiates the n unrolled steps: so does this call (SSA; DIVA returns
+ * NK_UNSUPPORTED). It replays the handle's last FixedIterations Picard solve from the initial guess that solve
+ * recorded (NK_INVALID_ARGUMENT if there was none), keeps the n + 1 iterates on a device tape (2(n + 2) planes,
+ * allocated on first use and counted by nk_device_bytes), and runs each step's reverse: λ_k = A(c(x_{k−1}))⁻¹ x̄_k
+ * (solve!'s rule, ext/NunatakEnzymeExt.jl:68-103), LAMBDA += λ_k, and the coefficient pullback of
+ * A(c(x_{k−1})) x_k with −λ_k into x̄_{k−1}, CBAR, ABAR, HBAR and BEDBAR; `control` is not used. The handle is left
+ * with (U, V) = x_n. info: picard_iterations = n, total_linear_iterations of the n adjoint solves. */
 nk_status nk_adjoint(nk_solver *solver, const nk_control *control, nk_info *info);
 
 /* Multigrid level shapes of the handle (StressBalanceMultigridPreconditioner(grid).levels). Writes up to

@@ -147,14 +147,16 @@ __device__ __forceinline__ void drag_derivatives(const PhysK &p, double U, doubl
 }
 
 // Faces: β̄ = Az x* q̄ on active rows (or the given β̄ — the DIVA chain passes βᵇ̄); β = φ·drag(U(u, v), ℑx C, ℑx N)
-// → velocity adjoint and, in the parameter pass, C̄.
+// → velocity adjoint and, in the parameter pass, C̄. (xu, xv) is the point the drag is linearized at, (xsu, xsv) the
+// vector the operator multiplies (both x* at the fixed point; x_{k−1} and x_k in an unrolled Picard step).
 __global__ __launch_bounds__(NT) void k_adj_drag(KG g, Colour col, PhysK p, const uint8_t *__restrict__ act, const double *__restrict__ xu,
                                                  const double *__restrict__ xv, const double *__restrict__ qbu,
                                                  const double *__restrict__ qbv, const double *__restrict__ C,
                                                  const double *__restrict__ N, const double *__restrict__ phiu,
                                                  const double *__restrict__ phiv, double *__restrict__ yu, double *__restrict__ yv,
                                                  double *__restrict__ Cb, int velocity, int parameters,
-                                                 const double *__restrict__ bbar_u, const double *__restrict__ bbar_v) {
+                                                 const double *__restrict__ bbar_u, const double *__restrict__ bbar_v,
+                                                 const double *__restrict__ xsu, const double *__restrict__ xsv) {
     NK_IJ_COL(col)
     if (i > g.Nx + 1 || j > g.Ny + 1) return;
     if (p.friction == 0) return;
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(NT) void k_adj_drag(KG g, Colour col, PhysK p, cons
     const bool useN = p.friction == 2;
     const double u0 = p.u0;
     if (i <= nxf(g) && j <= g.Ny && !periph_fc(g, i, j)) {
-        const double bb = bbar_u ? bbar_u[o] : qbar_u(g, act, qbu, i, j) * Az * xu[o];
+        const double bb = bbar_u ? bbar_u[o] : qbar_u(g, act, qbu, i, j) * Az * xsu[o];
         if (bb != 0.0) {
             const double vm = ((ldcf(g, xv, i - 1, j) + ldcf(g, xv, i, j)) / 2.0 + (ldcf(g, xv, i - 1, j + 1) + ldcf(g, xv, i, j + 1)) / 2.0) / 2.0;
             const double uu = xu[o];
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(NT) void k_adj_drag(KG g, Colour col, PhysK p, cons
         }
     }
     if (i <= g.Nx && j <= nyf(g) && !periph_cf(g, i, j)) {
-        const double bb = bbar_v ? bbar_v[o] : qbar_v(g, act, qbv, i, j) * Az * xv[o];
+        const double bb = bbar_v ? bbar_v[o] : qbar_v(g, act, qbv, i, j) * Az * xsv[o];
         if (bb != 0.0) {
             const double um = ((ldfc(g, xu, i, j - 1) + ldfc(g, xu, i + 1, j - 1)) / 2.0 + (ldfc(g, xu, i, j) + ldfc(g, xu, i + 1, j)) / 2.0) / 2.0;
             const double vv = xv[o];
@@ -539,8 +541,8 @@ __device__ __forceinline__ double drag_dN(const PhysK &p, double U, double C, do
 }
 
 // Pass 1, one thread per face: the H̄ and b̄ a u face sends to its cells (lo = the cell read as i − 1, hi = as i) and
-// likewise for v faces (lo = j − 1, hi = j). β̄ is Az x* q̄ on active rows (SSA), or the given basal β̄ᵇ (DIVA, with
-// xu, xv the basal velocities).
+// likewise for v faces (lo = j − 1, hi = j). β̄ is Az xs q̄ on active rows (SSA; xs the vector the operator multiplies,
+// xu, xv the point the drag is linearized at), or the given basal β̄ᵇ (DIVA, with xu, xv the basal velocities).
 __global__ __launch_bounds__(NT) void k_adj_geometry_faces(KG g, PhysK p, const uint8_t *__restrict__ act, const double *__restrict__ H,
                                                            const double *__restrict__ bed, const double *__restrict__ s,
                                                            const double *__restrict__ mu, const double *__restrict__ N,
@@ -549,7 +551,8 @@ __global__ __launch_bounds__(NT) void k_adj_geometry_faces(KG g, PhysK p, const 
                                                            const double *__restrict__ xv, const double *__restrict__ lu,
                                                            const double *__restrict__ lv, const double *__restrict__ qbu,
                                                            const double *__restrict__ qbv, const double *__restrict__ bbar_u,
-                                                           const double *__restrict__ bbar_v, int gl, double *__restrict__ hlo_u,
+                                                           const double *__restrict__ bbar_v, const double *__restrict__ xsu,
+                                                           const double *__restrict__ xsv, int gl, double *__restrict__ hlo_u,
                                                            double *__restrict__ hhi_u, double *__restrict__ blo_u,
                                                            double *__restrict__ bhi_u, double *__restrict__ hlo_v,
                                                            double *__restrict__ hhi_v, double *__restrict__ blo_v,
@@ -605,8 +608,8 @@ __global__ __launch_bounds__(NT) void k_adj_geometry_faces(KG g, PhysK p, const 
         // drag: β = φ · basal_drag(U, C̄, N̄)
         if (p.friction != 0 && !wall) {
             double bb;
-            if (dir == 0) bb = bbar_u ? bbar_u[o] : qbar_u(g, act, qbu, i, j) * Az * xu[o];
-            else bb = bbar_v ? bbar_v[o] : qbar_v(g, act, qbv, i, j) * Az * xv[o];
+            if (dir == 0) bb = bbar_u ? bbar_u[o] : qbar_u(g, act, qbu, i, j) * Az * xsu[o];
+            else bb = bbar_v ? bbar_v[o] : qbar_v(g, act, qbv, i, j) * Az * xsv[o];
             if (bb != 0.0) {
                 double U;
                 if (dir == 0) {
@@ -644,7 +647,8 @@ __global__ __launch_bounds__(NT) void k_adj_geometry_faces(KG g, PhysK p, const 
     }
 }
 
-// Pass 2, one thread per cell: H̄ = η̄H̄ μ Σ ησ Δζ (+ F̄₂ Σ w₂/ησ, DIVA) + the mask's share (∂μ/∂H · η̄H̄ H Σ ησ Δζ)
+// Pass 2, one thread per cell (adding to H̄, b̄ when accumulate, e.g. over the steps of an unrolled Picard loop):
+// H̄ = η̄H̄ μ Σ ησ Δζ (+ F̄₂ Σ w₂/ησ, DIVA) + the mask's share (∂μ/∂H · η̄H̄ H Σ ησ Δζ)
 // + the sends of the faces that read the cell: along x the face at the cell's own index (read as i) and the one
 // above it (read as i − 1), and on a Bounded edge the boundary face that the clamp maps here, on a Periodic one
 // the first face, which wraps to the last cell; likewise along y. b̄ from the faces alone.
@@ -655,7 +659,7 @@ __global__ __launch_bounds__(NT) void k_adj_geometry_cells(KG g, PhysK p, Sig sg
                                                            const double *__restrict__ bhi_u, const double *__restrict__ hlo_v,
                                                            const double *__restrict__ hhi_v, const double *__restrict__ blo_v,
                                                            const double *__restrict__ bhi_v, double *__restrict__ Hb,
-                                                           double *__restrict__ Bb) {
+                                                           double *__restrict__ Bb, int accumulate) {
     NK_IJ
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j);
@@ -685,8 +689,8 @@ __global__ __launch_bounds__(NT) void k_adj_geometry_cells(KG g, PhysK p, Sig sg
     if (j + 1 <= nyf(g)) { const long q = off(g, i, j + 1); hb += hlo_v[q]; bb += blo_v[q]; }
     if (g.ty && j == g.Ny) { const long q = off(g, i, g.Ny + 1); hb += hhi_v[q]; bb += bhi_v[q]; }
     if ((g.ty && j == 1) || (!g.ty && j == g.Ny)) { const long q = off(g, i, 1); hb += hlo_v[q]; bb += blo_v[q]; }
-    Hb[o] = hb;
-    Bb[o] = bb;
+    Hb[o] = accumulate ? Hb[o] + hb : hb;
+    Bb[o] = accumulate ? Bb[o] + bb : bb;
 }
 
 }  // namespace nk

@@ -102,6 +102,12 @@ struct nk_solver {
     double *st_es = nullptr, *out_es = nullptr;
     // shadows the cost gives the DIVA state (through the layer velocities) and the resulting x̄ of the adjoint
     double *cs_bbu = nullptr, *cs_bbv = nullptr, *cs_F2 = nullptr, *cs_es = nullptr, *ad_xbu = nullptr, *ad_xbv = nullptr;
+    // unrolled adjoint of FixedIterations Picard (SSA): the initial guess of the last such Picard solve (recorded by
+    // picard()), and the tape of its iterates x_0 .. x_n (2 planes each) plus λ_k, allocated by the first nk_adjoint
+    double *x0u = nullptr, *x0v = nullptr;
+    bool x0_valid = false;
+    double *tape = nullptr;
+    long tape_planes = 0;
     // mixed precision: while set, the finest level's last FP32 sweep also writes z64 and sums r64·z64
     const double *fz_r64u = nullptr, *fz_r64v = nullptr;
     double *fz_z64u = nullptr, *fz_z64v = nullptr;
@@ -1188,6 +1194,18 @@ nk_status picard(nk_solver *s, nk_info *info) {
     double D = INFINITY, X = 0.0;
     int it = 0;
     if (info) info->total_linear_iterations = 0;
+    if (fixed && s->ph.stress_balance == NK_SSA) {
+        // the initial guess, for the unrolled adjoint (nk_adjoint replays the steps from it)
+        if (!s->x0u) {
+            CHK(alloc_plane(s, &s->x0u, g));
+            CHK(alloc_plane(s, &s->x0v, g));
+        }
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->f[NK_FIELD_U], s->f[NK_FIELD_V], s->x0u, s->x0v);
+        CHK(launch_check(s));
+        s->x0_valid = true;
+    } else {
+        s->x0_valid = false;
+    }
     for (;;) {
         if (fixed) { if (it >= ctl.iterations) break; }
         else if (!(D > jmax_h(ctl.reltol * X, ctl.abstol) && it < ctl.maxiter)) break;
@@ -1644,6 +1662,7 @@ nk_status nk_destroy(nk_solver *s) {
     std::lock_guard<std::mutex> lk(g_device_mu);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void *p : s->allocs) (void)hipFree(p);
+    if (s->tape) (void)hipFree(s->tape);
     if (s->S_host) (void)hipHostFree(s->S_host);
     if (s->chunk_graph) (void)hipGraphExecDestroy(s->chunk_graph);
     for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
@@ -2101,11 +2120,17 @@ nk_status coloured(nk_solver *s, int ex, int ey, Launch launch) {
 }
 }   // extern "C++"
 
-// One transposed coefficient step: ȳ = (∂q/∂x)ᵀ q̄ (velocity) and/or C̄, Ā (parameters), with q̄ in ad_q.
-nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters) {
+// One transposed coefficient step: ȳ = (∂q/∂x)ᵀ q̄ (velocity) and/or C̄, Ā (parameters), with q̄ in ad_q. q(x) =
+// A(c(x)) xs is linearized at x = (U, V) with xs = (xsu, xsv): the same vector at the fixed point (null: U, V), the step's
+// result x_k at the input x_{k−1} of an unrolled Picard step.
+nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters, const double *xsu = nullptr, const double *xsv = nullptr) {
     const KG &g = s->g;
     const dim3 Bc = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
     double **f = s->f;
+    if (!xsu) {
+        xsu = f[NK_FIELD_U];
+        xsv = f[NK_FIELD_V];
+    }
     const PhysK p = physk(s);
     const uint8_t *act = s->lev[0].act;
     k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->ad_eta, s->ad_exy, 0.0, nullptr);
@@ -2114,10 +2139,10 @@ nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters) {
         k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->ad_yu, s->ad_yv, 0.0, nullptr);
         CHK(launch_check(s));
     }
-    k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, xsu, xsv, s->ad_qu, s->ad_qv, s->ad_eta);
     CHK(launch_check(s));
     CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
-        k_adj_corners<<<B, THREADS, 0, s->stream>>>(g, col, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+        k_adj_corners<<<B, THREADS, 0, s->stream>>>(g, col, act, p.wf, xsu, xsv, s->ad_qu, s->ad_qv, s->ad_eta);
     }));
     CHK(coloured(s, g.Nx, g.Ny, [&](dim3 B, Colour col) {
         k_adj_viscosity<<<B, THREADS, 0, s->stream>>>(g, col, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETAH],
@@ -2132,7 +2157,7 @@ nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters) {
     return coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
         k_adj_drag<<<B, THREADS, 0, s->stream>>>(g, col, p, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, f[NK_FIELD_C],
                                                  f[NK_FIELD_N], f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_yu, s->ad_yv, f[NK_FIELD_CBAR],
-                                                 velocity ? 1 : 0, parameters ? 1 : 0, nullptr, nullptr);
+                                                 velocity ? 1 : 0, parameters ? 1 : 0, nullptr, nullptr, xsu, xsv);
     });
 }
 
@@ -2181,13 +2206,41 @@ nk_status adjoint_step_diva(nk_solver *s, bool parameters) {
     CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
         k_adj_drag<<<B, THREADS, 0, s->stream>>>(g, col, p, act, f[NK_FIELD_UB], f[NK_FIELD_VB], s->ad_qu, s->ad_qv, f[NK_FIELD_C],
                                                  f[NK_FIELD_N], f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], s->ad_ubb, s->ad_vbb,
-                                                 f[NK_FIELD_CBAR], 1, parameters ? 1 : 0, s->ad_bbu, s->ad_bbv);
+                                                 f[NK_FIELD_CBAR], 1, parameters ? 1 : 0, s->ad_bbu, s->ad_bbv, f[NK_FIELD_UB],
+                                                 f[NK_FIELD_VB]);
     }));
     return coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
         k_adj_basal_velocity<<<B, THREADS, 0, s->stream>>>(g, col, f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_BETABU],
                                                            f[NK_FIELD_BETABV], f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu,
                                                            s->ad_yv, s->out_bbu, s->out_bbv, s->out_F2);
     });
+}
+
+// Geometry pullback of one coefficient step: H̄, b̄ from τ̄ = (lu, lv), q̄ in ad_q and η̄H̄ in ad_eta (two passes,
+// faces then cells; the adjoint loop's spare planes carry the face sends), added to HBAR/BEDBAR when accumulate.
+// (xsu, xsv): the vector the operator multiplies (null: U, V).
+nk_status geometry_pullback(nk_solver *s, const double *lu, const double *lv, const double *xsu, const double *xsv, bool accumulate) {
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    double **f = s->f;
+    const bool diva = s->ph.stress_balance == NK_DIVA;
+    const PhysK p = physk(s);
+    if (!xsu) {
+        xsu = diva ? f[NK_FIELD_UB] : f[NK_FIELD_U];
+        xsv = diva ? f[NK_FIELD_VB] : f[NK_FIELD_V];
+    }
+    k_adj_geometry_faces<<<Be, THREADS, 0, s->stream>>>(
+        g, p, s->lev[0].act, f[NK_FIELD_H], f[NK_FIELD_BED], f[NK_FIELD_S], s->ph.masking ? f[NK_FIELD_MU] : nullptr,
+        f[NK_FIELD_N], f[NK_FIELD_C], f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], diva ? f[NK_FIELD_UB] : f[NK_FIELD_U],
+        diva ? f[NK_FIELD_VB] : f[NK_FIELD_V], lu, lv, s->ad_qu, s->ad_qv, diva ? s->ad_bbu : nullptr, diva ? s->ad_bbv : nullptr,
+        xsu, xsv, s->ph.grounding == NK_GROUNDING_SUBGRID_DRIVING_STRESS ? 1 : 0, s->ad_yu, s->ad_ru, s->ad_lu, s->xmu, s->ad_yv,
+        s->ad_rv, s->ad_lv, s->xmv);
+    CHK(launch_check(s));
+    k_adj_geometry_cells<<<B, THREADS, 0, s->stream>>>(g, p, sig_tables(s), f[NK_FIELD_H], f[NK_FIELD_ETASIGMA], s->plane, s->ad_eta,
+                                                      diva ? s->ad_F2n : nullptr, s->ad_yu, s->ad_ru, s->ad_lu, s->xmu, s->ad_yv,
+                                                      s->ad_rv, s->ad_lv, s->xmv, f[NK_FIELD_HBAR], f[NK_FIELD_BEDBAR],
+                                                      accumulate ? 1 : 0);
+    return launch_check(s);
 }
 
 // EnzymeRules.reverse for converged picard_solve! (ext/NunatakEnzymeExt.jl:175-240), SSA coefficients.
@@ -2311,21 +2364,7 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
     CHK(launch_check(s));
     if (diva) CHK(adjoint_step_diva(s, true));
     else CHK(adjoint_step(s, false, true));
-    // geometry pullback: H̄, b̄ (two passes, faces then cells; the loop's spare planes carry the face sends)
-    {
-        const PhysK p = physk(s);
-        k_adj_geometry_faces<<<Be, THREADS, 0, s->stream>>>(
-            g, p, s->lev[0].act, f[NK_FIELD_H], f[NK_FIELD_BED], f[NK_FIELD_S], s->ph.masking ? f[NK_FIELD_MU] : nullptr,
-            f[NK_FIELD_N], f[NK_FIELD_C], f[NK_FIELD_PHIU], f[NK_FIELD_PHIV], diva ? f[NK_FIELD_UB] : f[NK_FIELD_U],
-            diva ? f[NK_FIELD_VB] : f[NK_FIELD_V], lu, lv, s->ad_qu, s->ad_qv, diva ? s->ad_bbu : nullptr, diva ? s->ad_bbv : nullptr,
-            s->ph.grounding == NK_GROUNDING_SUBGRID_DRIVING_STRESS ? 1 : 0, s->ad_yu, s->ad_ru, s->ad_lu, s->xmu, s->ad_yv, s->ad_rv,
-            s->ad_lv, s->xmv);
-        CHK(launch_check(s));
-        k_adj_geometry_cells<<<B, THREADS, 0, s->stream>>>(g, p, sig_tables(s), f[NK_FIELD_H], f[NK_FIELD_ETASIGMA], s->plane, s->ad_eta,
-                                                          diva ? s->ad_F2n : nullptr, s->ad_yu, s->ad_ru, s->ad_lu, s->xmu, s->ad_yv,
-                                                          s->ad_rv, s->ad_lv, s->xmv, f[NK_FIELD_HBAR], f[NK_FIELD_BEDBAR]);
-        CHK(launch_check(s));
-    }
+    CHK(geometry_pullback(s, lu, lv, nullptr, nullptr, false));
     if (info) {
         info->picard_iterations = it;
         info->total_linear_iterations = total;
@@ -2335,11 +2374,120 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
     return NK_OK;
 }
 
+
+// Reverse of picard_solve! with FixedIterations(n) (src/Solvers/picard.jl:63-69), differentiated step by step as
+// Enzyme does where the converged-Picard rule does not apply (that rule is only for Convergence controls,
+// ext/NunatakEnzymeExt.jl:121). Step k is x_k = A(c_k)⁻¹ τ with c_k = c(x_{k−1}), warm-started at x_{k−1}; its solve
+// has the implicit reverse rule of solve! (ext/NunatakEnzymeExt.jl:68-103): λ_k = A(c_k)⁻¹ x̄_k from zero, τ̄ += λ_k,
+// and the coefficient pullback of q = A(c(x_{k−1})) x_k with q̄ = −λ_k gives x̄_{k−1} and the shadows of C, A, H and b;
+// the initial guess gets no shadow. The iterates come from replaying the forward steps from the initial guess that
+// picard() recorded, into a tape of n + 1 iterates (2(n + 2) planes with λ_k, allocated on first use). SSA only.
+nk_status adjoint_unrolled(nk_solver *s, int n, nk_info *info) {
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    double **f = s->f;
+    if (s->ph.stress_balance != NK_SSA) {
+        s->err = "the unrolled adjoint of FixedIterations Picard is built for SSA only (DIVA: use a Convergence control)";
+        return NK_UNSUPPORTED;
+    }
+    if (!s->x0_valid) {
+        s->err = "the unrolled adjoint of FixedIterations Picard replays the last FixedIterations Picard solve of this handle: run one first";
+        return NK_INVALID_ARGUMENT;
+    }
+    const long P = plane_of(g);
+    const long need = 2L * (n + 2);
+    if (s->tape_planes < need) {
+        if (s->tape) {
+            HIPCHK(hipStreamSynchronize(s->stream));
+            (void)hipFree(s->tape);
+            s->device_bytes -= (size_t)s->tape_planes * (size_t)P * sizeof(double);
+            s->tape = nullptr;
+            s->tape_planes = 0;
+        }
+        if (hipMalloc((void **)&s->tape, (size_t)need * (size_t)P * sizeof(double)) != hipSuccess) {
+            s->tape = nullptr;
+            s->err = "hipMalloc of the unrolled adjoint's tape failed";
+            return NK_OUT_OF_MEMORY;
+        }
+        s->tape_planes = need;
+        s->device_bytes += (size_t)need * (size_t)P * sizeof(double);
+    }
+    auto tu = [&](int k) { return s->tape + (2L * k) * P; };
+    auto tv = [&](int k) { return s->tape + (2L * k + 1) * P; };
+    double *tlu = s->tape + 2L * (n + 1) * P, *tlv = tlu + P;
+    // forward replay from the recorded initial guess
+    k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->x0u, s->x0v, f[NK_FIELD_U], f[NK_FIELD_V]);
+    CHK(launch_check(s));
+    k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->x0u, s->x0v, tu(0), tv(0));
+    CHK(launch_check(s));
+    nk_info fwd{};
+    for (int k = 1; k <= n; k++) {
+        CHK(update_coefficients(s));
+        CHK(solve_linear(s, &fwd));
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], tu(k), tv(k));
+        CHK(launch_check(s));
+    }
+    // reverse sweep
+    double *lu = f[NK_FIELD_LAMBDAU], *lv = f[NK_FIELD_LAMBDAV];
+    double *zero2[][2] = {{lu, lv}, {f[NK_FIELD_CBAR], nullptr}, {f[NK_FIELD_HBAR], f[NK_FIELD_BEDBAR]}};
+    for (auto &z : zero2) {
+        k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, z[0], z[1], 0.0, nullptr);
+        CHK(launch_check(s));
+    }
+    for (int k = 0; k < (s->ph.rate_factor_depth == NK_RATE_FACTOR_3D ? s->Ns : 1); k++) {
+        k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_ABAR] + (long)k * s->plane, nullptr, 0.0, nullptr);
+        CHK(launch_check(s));
+    }
+    k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_xbu, s->ad_xbv);   // x̄_n
+    CHK(launch_check(s));
+    long total = 0;
+    double res = 0.0;
+    for (int k = n; k >= 1; k--) {
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, tu(k - 1), tv(k - 1), f[NK_FIELD_U], f[NK_FIELD_V]);
+        CHK(launch_check(s));
+        CHK(update_coefficients(s));                                              // c_k = c(x_{k−1})
+        CHK(update_preconditioner(s));
+        k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, tlu, tlv, 0.0, nullptr);
+        CHK(launch_check(s));
+        int cg = 0;
+        CHK(pcg<double>(s, tlu, tlv, s->ad_xbu, s->ad_xbv, s->op.linear, false, &cg, &res));   // λ_k = A(c_k)⁻¹ x̄_k
+        total += cg;
+        k_add<<<B, THREADS, 0, s->stream>>>(g, lu, lv, tlu, tlv, lu, lv);      // τ̄ += λ_k
+        CHK(launch_check(s));
+        k_negate<<<B, THREADS, 0, s->stream>>>(g, tlu, tlv, s->ad_qu, s->ad_qv);   // q̄ = −λ_k
+        CHK(launch_check(s));
+        CHK(adjoint_step(s, true, true, tu(k), tv(k)));                          // x̄_{k−1}, C̄ and Ā += ...
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->ad_yu, s->ad_yv, s->ad_xbu, s->ad_xbv);
+        CHK(launch_check(s));
+        CHK(geometry_pullback(s, tlu, tlv, tu(k), tv(k), true));                 // H̄, b̄ += ...
+        if (s->verbose) {
+            std::fprintf(stderr, "[nunatak_hip] unrolled adjoint step %d: linear %d\n", k, cg);
+            std::fflush(stderr);
+        }
+    }
+    // the handle as the forward solve left it: c(x_{n−1}) and (U, V) = x_n
+    if (n >= 1) {
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, tu(n - 1), tv(n - 1), f[NK_FIELD_U], f[NK_FIELD_V]);
+        CHK(launch_check(s));
+        CHK(update_coefficients(s));
+    }
+    k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, tu(n), tv(n), f[NK_FIELD_U], f[NK_FIELD_V]);
+    CHK(launch_check(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (info) {
+        info->picard_iterations = n;
+        info->total_linear_iterations = total;
+        info->last_residual_norm = res;
+    }
+    return NK_OK;
+}
+
 }  // namespace
 
 nk_status nk_adjoint(nk_solver *s, const nk_control *control, nk_info *info) {
     if (!s) return NK_INVALID_ARGUMENT;
     Timer t(s, info);
+    if (s->op.picard.kind == NK_FIXED_ITERATIONS) return t.done(adjoint_unrolled(s, s->op.picard.iterations, info));
     const nk_control ctl = control ? *control : s->op.picard;
     return t.done(adjoint(s, ctl, info));
 }

@@ -521,3 +521,73 @@ def test_adjoint_is_bit_reproducible():
         for k in a:
             assert np.array_equal(a[k], b[k]), k
         assert np.max(np.abs(a["CBAR"])) > 0 and np.max(np.abs(a["HBAR"])) > 0
+
+
+# ---- unrolled adjoint of FixedIterations Picard ------------------------------------------------------------------------
+
+def ridge_fixed(n, preconditioner=abi.PRECONDITIONER_DIAGONAL):
+    """The ridge model with Picard FixedIterations(n): nk_adjoint then differentiates the n unrolled steps."""
+    m, g = ridge(preconditioner)
+    o = m.options
+    o.picard = abi.fixed_iterations(n)
+    m2 = HIPModel(g, m.physics, o)
+    for name in ("H", "BED", "A", "C"):
+        m2.set(name, m.get(name))
+    return m2, g
+
+
+def close_rtol(a, b, rtol):
+    """Julia's isapprox on arrays: ‖a − b‖ ≤ rtol · max(‖a‖, ‖b‖)."""
+    return np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(a), np.linalg.norm(b))
+
+
+def test_fixed_point_adjoint_equals_unrolled_adjoint_of_60_iterations():
+    """test/test_enzyme.jl:341-365: the fixed-point adjoint of the converged ridge solve (Convergence(reltol = 1e-12))
+    against the taped adjoint of FixedIterations(60) — here the unrolled reverse of the 60 Picard steps — for ∂J/∂H₀
+    and ∂J/∂C, rtol 1e-8 (the reference's bar)."""
+    m, g = ridge(abi.PRECONDITIONER_DIAGONAL)
+    adjoint_of_cost(m, g)
+    fp = {k: m.get(k).copy() for k in ("HBAR", "CBAR", "LAMBDAU")}
+    t, gt = ridge_fixed(60)
+    _, info = adjoint_of_cost(t, gt)
+    say(f"[adjoint] unrolled FixedIterations(60): {info['picard_iterations']} steps, {info['total_linear_iterations']} CG")
+    assert info["picard_iterations"] == 60
+    for k in ("HBAR", "CBAR", "LAMBDAU"):
+        a, b = fp[k], t.get(k)
+        say(f"[adjoint] fixed point vs unrolled {k}: {np.max(np.abs(a - b)) / np.max(np.abs(b)):.2e}")
+        assert close_rtol(a, b, 1e-8), k
+
+
+def test_unrolled_adjoint_of_three_iterations_matches_finite_differences():
+    """Three Picard steps from zero are far from the fixed point, so their derivative differs from the fixed-point
+    adjoint's: the unrolled reverse (each step linearized at its own input and multiplying its own result) against
+    centred differences of J after FixedIterations(3), for C, A and H; bar 1e-5 (the reference's for the ridge)."""
+    m, g = ridge_fixed(3)
+    J, info = adjoint_of_cost(m, g)
+    assert info["picard_iterations"] == 3
+    Cb, Ab, Hb = m.get("CBAR").copy(), m.get("ABAR").copy(), m.get("HBAR").copy()
+    C, A, H = m.get("C"), m.get("A"), m.get("H")
+    for name, grad, field, index in (("C", Cb, C, (1, 6)), ("C", Cb, C, (3, 11)), ("A", Ab, A, (2, 9)),
+                                     ("H", Hb, H, (1, 5)), ("H", Hb, H, (2, 9))):
+        fd = centred_difference(m, g, name, index, 1e-4 * field[index])
+        say(f"[adjoint] FixedIterations(3) dJ/d{name}{index}: unrolled {grad[index]:.10e}, fd {fd:.10e}")
+        assert abs(grad[index] - fd) < 1e-5 * abs(fd)
+    # the converged solve's gradient is a different quantity here
+    mc, gc = ridge(abi.PRECONDITIONER_DIAGONAL)
+    adjoint_of_cost(mc, gc)
+    assert np.max(np.abs(mc.get("CBAR") - Cb)) > 1e-3 * np.max(np.abs(Cb))
+
+
+def test_unrolled_adjoint_multigrid_equals_diagonal_and_restores_the_forward_state():
+    """The unrolled reverse through the multigrid preconditioner gives the diagonal one's gradients (1e-8), and leaves
+    the handle's (U, V) as the forward solve did."""
+    out = []
+    for pre in (abi.PRECONDITIONER_DIAGONAL, abi.PRECONDITIONER_MULTIGRID):
+        m, g = ridge_fixed(8, pre)
+        adjoint_of_cost(m, g)
+        u_after = m.get("U").copy()
+        cost(m, g)
+        assert np.array_equal(u_after, m.get("U"))
+        out.append((m.get("CBAR").copy(), m.get("HBAR").copy()))
+    for d, mg in zip(*out):
+        assert close_rtol(mg, d, 1e-8)
