@@ -232,7 +232,7 @@ def mismip_leg(resolutions):
     return rows
 
 
-def serving_leg(n, rate_hz, slots, serial_sample):
+def serving_leg(n, rate_hz, slots, serial_sample, cpu_oracle=True):
     """Config 5 on a bounded sample: n requests of the stated stream (Poisson arrivals at rate_hz, sides log-uniform
     in [64, 1024] rounded to 16) through the server at their arrival times (latency at a stated load), then the same
     requests all at t = 0 (saturated: the server's capacity), and the first serial_sample of them solved one at a
@@ -256,14 +256,49 @@ def serving_leg(n, rate_hz, slots, serial_sample):
     sides = [r[1].Nx for r in reqs]
     log(f"[bench] serial (one slot) on the first {serial_sample}: mixed {serial_mixed['solves_per_s']:.2f} solves/s, "
         f"FP64 {serial64['solves_per_s']:.2f} solves/s")
+    cpu_small = serving_cpu_small(sample64, serial64["records"], serial_mixed["records"]) if cpu_oracle else None
     return {"requests": n, "rate_hz": rate_hz, "slots": slots, "sides_min_max": [min(sides), max(sides)],
             "p50_ms": out["p50_ms"], "p99_ms": out["p99_ms"], "solves_per_s": out["solves_per_s"],
             "saturated_solves_per_s": sat["solves_per_s"], "saturated_p50_ms": sat["p50_ms"], "saturated_p99_ms": sat["p99_ms"],
             "serial_sample": serial_sample, "serial_mixed_solves_per_s": serial_mixed["solves_per_s"],
-            "serial_fp64_solves_per_s": serial64["solves_per_s"],
+            "serial_fp64_solves_per_s": serial64["solves_per_s"], "cpu_oracle_smallest": cpu_small,
             "workload": "serving stream (seed 5): SSA grounded/floating channel per request, mixed precision "
                         "(FP32 Galerkin V-cycle), MG-PCG linear 1e-8, Picard 1e-6; serial baselines: the first "
                         f"{serial_sample} requests through one slot"}
+
+
+def serving_cpu_small(sample64, rec64, rec_mixed, budget_s=20.0):
+    """The reference algorithm on the host for the serving stream: the oracle (FP64, rediscretized multigrid, all host
+    threads) solves the sample's requests from the smallest up until budget_s of host time is spent; the same requests'
+    device times (serial, one slot) in FP64 and mixed come from the serial runs' records (start → finish)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads, cpu_model = host_cores()
+    oracle.set_threads(threads)
+    order = sorted(range(len(sample64)), key=lambda k: sample64[k][1].Nx * sample64[k][1].Ny)
+    spent, done = 0.0, []
+    for k in order:
+        _, g, p, o, f = sample64[k]
+        m = oracle.OracleModel(g, p, o)
+        workloads.load_into(m, f)
+        t0 = time.perf_counter()
+        m.update_velocities()
+        dt = time.perf_counter() - t0
+        m.close()
+        spent += dt
+        done.append(k)
+        if spent >= budget_s:
+            break
+    gpu64 = sum((rec64[k]["finish_ms"] - rec64[k]["start_ms"]) / 1e3 for k in done)
+    gpumx = sum((rec_mixed[k]["finish_ms"] - rec_mixed[k]["start_ms"]) / 1e3 for k in done)
+    sides = sorted({sample64[k][1].Nx for k in done})
+    log(f"[bench] cpu oracle ({threads} threads) on the {len(done)} smallest serving requests (sides {sides[0]}–{sides[-1]}): "
+        f"{len(done) / spent:.2f} solves/s; device serial FP64 {len(done) / gpu64:.2f}, mixed {len(done) / gpumx:.2f} solves/s")
+    return {"requests": len(done), "sides": sides, "threads": threads, "cpu_model": cpu_model,
+            "cpu_solves_per_s": len(done) / spent, "device_serial_fp64_solves_per_s": len(done) / gpu64,
+            "device_serial_mixed_solves_per_s": len(done) / gpumx,
+            "note": "oracle = the reference algorithm restated in C (FP64, rediscretized multigrid), complete solves; "
+                    "device times are start → finish of the same requests in the serial runs (inputs to results on host)"}
 
 
 def transport_leg(N, steps, advection=abi.ADVECTION_UPWIND1):
@@ -388,7 +423,7 @@ def main():
     ap.add_argument("--serve", type=int, default=48, help="requests in the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
-    ap.add_argument("--serve-serial", type=int, default=24, help="requests timed serially (mixed and FP64) for the baselines")
+    ap.add_argument("--serve-serial", type=int, default=48, help="requests timed serially (mixed and FP64) for the baselines")
     ap.add_argument("--transport", type=int, default=20, help="RK3 thickness steps timed in the transport leg (0: skip)")
     ap.add_argument("--transport-only", action="store_true",
                     help="run only the thickness-transport and enthalpy legs; prints their JSON objects")
@@ -577,7 +612,8 @@ def main():
         for m, _ in models.values():
             m.close()
         models = {}
-        out["serving"] = serving_leg(args.serve, args.serve_rate, args.serve_slots, min(args.serve_serial, args.serve))
+        out["serving"] = serving_leg(args.serve, args.serve_rate, args.serve_slots, min(args.serve_serial, args.serve),
+                                     cpu_oracle=not args.no_cpu_baseline)
         out["served_p50_ms"] = out["serving"]["p50_ms"]
         out["served_p99_ms"] = out["serving"]["p99_ms"]
     if args.transport > 0:
