@@ -376,13 +376,14 @@ nk_status nk_till_effective_pressure(nk_solver *solver, const nk_till *till);
  * and N — the gradient of the whole update_velocities!(model) with geometry computed on the device (nk_update_velocities
  * with geometry = 1), as the reference's test differentiates update_state! (test/test_enzyme.jl:341-356).
  * With a FixedIterations(n) Picard control the converged-Picard rule does not apply (ext/NunatakEnzymeExt.jl:121 matches
- * Convergence only) and the reference's Enzyme differentiates the n unrolled steps: so does this call (SSA; DIVA returns
- * NK_UNSUPPORTED). It replays the handle's last FixedIterations Picard solve from the initial guess that solve
- * recorded (NK_INVALID_ARGUMENT if there was none), keeps the n + 1 iterates on a device tape (2(n + 2) planes,
- * allocated on first use and counted by nk_device_bytes), and runs each step's reverse: λ_k = A(c(x_{k−1}))⁻¹ x̄_k
- * (solve!'s rule, ext/NunatakEnzymeExt.jl:68-103), LAMBDA += λ_k, and the coefficient pullback of
- * A(c(x_{k−1})) x_k with −λ_k into x̄_{k−1}, CBAR, ABAR, HBAR and BEDBAR; `control` is not used. The handle is left
- * with (U, V) = x_n. info: picard_iterations = n, total_linear_iterations of the n adjoint solves. */
+ * Convergence only) and the reference's Enzyme differentiates the n unrolled steps: so does this call, SSA and DIVA.
+ * It replays the handle's last FixedIterations Picard solve from the initial guess (and DIVA lagged state) that solve
+ * recorded (NK_INVALID_ARGUMENT if there was none), keeps the n + 1 iterates (and states) on a device tape (2 planes
+ * per step, 5 + Nσ for DIVA, allocated on first use and counted by nk_device_bytes), and runs each step's reverse:
+ * λ_k = A(c(x_{k−1}))⁻¹ x̄_k (solve!'s rule, ext/NunatakEnzymeExt.jl:68-103), LAMBDA += λ_k, and the coefficient
+ * pullback of A(c(x_{k−1})) x_k with −λ_k into x̄_{k−1} (DIVA: the lagged state's shadows too), CBAR, ABAR, HBAR and
+ * BEDBAR; `control` is not used. The handle is left as the forward solve left it ((U, V) = x_n, its coefficients and
+ * post-processing). info: picard_iterations = n, total_linear_iterations of the n adjoint solves. */
 nk_status nk_adjoint(nk_solver *solver, const nk_control *control, nk_info *info);
 
 /* Multigrid level shapes of the handle (StressBalanceMultigridPreconditioner(grid).levels). Writes up to

@@ -102,9 +102,11 @@ struct nk_solver {
     double *st_es = nullptr, *out_es = nullptr;
     // shadows the cost gives the DIVA state (through the layer velocities) and the resulting x̄ of the adjoint
     double *cs_bbu = nullptr, *cs_bbv = nullptr, *cs_F2 = nullptr, *cs_es = nullptr, *ad_xbu = nullptr, *ad_xbv = nullptr;
-    // unrolled adjoint of FixedIterations Picard (SSA): the initial guess of the last such Picard solve (recorded by
-    // picard()), and the tape of its iterates x_0 .. x_n (2 planes each) plus λ_k, allocated by the first nk_adjoint
+    // unrolled adjoint of FixedIterations Picard: the initial guess of the last such Picard solve (recorded by picard()),
+    // and the tape of its iterates x_0 .. x_n (2 planes each; DIVA adds the lagged state βᵇ, F₂, ησ of each step) plus
+    // λ_k, allocated by the first nk_adjoint
     double *x0u = nullptr, *x0v = nullptr;
+    double *s0_bbu = nullptr, *s0_bbv = nullptr, *s0_F2 = nullptr, *s0_es = nullptr;   // DIVA: the lagged state it started from
     bool x0_valid = false;
     double *tape = nullptr;
     long tape_planes = 0;
@@ -1185,6 +1187,24 @@ nk_status post_process(nk_solver *s) {
     return launch_check(s);
 }
 
+// Copies DIVA's lagged state (βᵇ at u and v faces, F₂, ησ on Nσ layers) from one set of planes to another.
+nk_status copy_diva_state(nk_solver *s, const double *bbu, const double *bbv, const double *F2, const double *es, double *bbu2,
+                          double *bbv2, double *F22, double *es2) {
+    const KG &g = s->g;
+    const dim3 Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    const double *src[] = {bbu, bbv, F2};
+    double *dst[] = {bbu2, bbv2, F22};
+    for (int k = 0; k < 3; k++) {
+        k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, src[k], dst[k]);
+        CHK(launch_check(s));
+    }
+    for (int k = 0; k < s->Ns; k++) {
+        k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, es + (long)k * s->plane, es2 + (long)k * s->plane);
+        CHK(launch_check(s));
+    }
+    return NK_OK;
+}
+
 // picard_solve! (src/Solvers/picard.jl:71-90)
 nk_status picard(nk_solver *s, nk_info *info) {
     const nk_control &ctl = s->op.picard;
@@ -1194,14 +1214,24 @@ nk_status picard(nk_solver *s, nk_info *info) {
     double D = INFINITY, X = 0.0;
     int it = 0;
     if (info) info->total_linear_iterations = 0;
-    if (fixed && s->ph.stress_balance == NK_SSA) {
-        // the initial guess, for the unrolled adjoint (nk_adjoint replays the steps from it)
+    if (fixed) {
+        // the initial guess (and DIVA's lagged state), for the unrolled adjoint: nk_adjoint replays the steps from it
         if (!s->x0u) {
             CHK(alloc_plane(s, &s->x0u, g));
             CHK(alloc_plane(s, &s->x0v, g));
         }
         k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->f[NK_FIELD_U], s->f[NK_FIELD_V], s->x0u, s->x0v);
         CHK(launch_check(s));
+        if (s->ph.stress_balance == NK_DIVA) {
+            if (!s->s0_bbu) {
+                CHK(alloc_plane(s, &s->s0_bbu, g));
+                CHK(alloc_plane(s, &s->s0_bbv, g));
+                CHK(alloc_plane(s, &s->s0_F2, g));
+                CHK(alloc_plane(s, &s->s0_es, g, s->Ns));
+            }
+            CHK(copy_diva_state(s, s->f[NK_FIELD_BETABU], s->f[NK_FIELD_BETABV], s->f[NK_FIELD_F2], s->f[NK_FIELD_ETASIGMA],
+                                s->s0_bbu, s->s0_bbv, s->s0_F2, s->s0_es));
+        }
         s->x0_valid = true;
     } else {
         s->x0_valid = false;
@@ -2161,13 +2191,27 @@ nk_status adjoint_step(nk_solver *s, bool velocity, bool parameters, const doubl
     });
 }
 
+// The lagged DIVA state a step read (βᵇ at u and v faces, F₂, ησ): null members are the handle's fields, which hold it
+// at the fixed point; an unrolled step passes its input state from the tape.
+struct DivaLag {
+    const double *bbu = nullptr, *bbv = nullptr, *F2 = nullptr, *es = nullptr;
+};
+
 // One transposed DIVA coefficient step (update_diva_coefficients!, src/StressBalances/diva.jl:60-75), q̄ in ad_q and the
 // shadows of the step's state (βᵇ′, F₂′, ησ′) in st_*: ȳ in ad_y, the lagged state's shadows in out_*, and C̄, Ā in the
 // parameter pass. The whole chain runs in both passes (C̄ needs βᵇ̄ from the basal-shear terms of the viscosity).
-nk_status adjoint_step_diva(nk_solver *s, bool parameters) {
+// The step is linearized at (U, V) with the lagged state lag, and (xsu, xsv) is the vector the operator multiplies
+// (null: U, V); the step's own outputs (βᵇ′, F₂′, uᵇ) are read from the fields.
+nk_status adjoint_step_diva(nk_solver *s, bool parameters, const double *xsu = nullptr, const double *xsv = nullptr,
+                            DivaLag lag = DivaLag{}) {
     const KG &g = s->g;
     const dim3 Bc = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
     double **f = s->f;
+    if (!xsu) {
+        xsu = f[NK_FIELD_U];
+        xsv = f[NK_FIELD_V];
+    }
+    if (!lag.bbu) lag = DivaLag{f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_F2], f[NK_FIELD_ETASIGMA]};
     const PhysK p = physk(s);
     const uint8_t *act = s->lev[0].act;
     auto fill = [&](double *a, double *b) { k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, a, b, 0.0, nullptr); return launch_check(s); };
@@ -2182,18 +2226,18 @@ nk_status adjoint_step_diva(nk_solver *s, bool parameters) {
     CHK(launch_check(s));
     k_copy_plane<<<Be, THREADS, 0, s->stream>>>(g, s->st_F2, s->ad_F2n);
     CHK(launch_check(s));
-    k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+    k_adj_centres<<<Bc, THREADS, 0, s->stream>>>(g, act, xsu, xsv, s->ad_qu, s->ad_qv, s->ad_eta);
     CHK(launch_check(s));
     CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
-        k_adj_corners<<<B, THREADS, 0, s->stream>>>(g, col, act, p.wf, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv, s->ad_eta);
+        k_adj_corners<<<B, THREADS, 0, s->stream>>>(g, col, act, p.wf, xsu, xsv, s->ad_qu, s->ad_qv, s->ad_eta);
     }));
     CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
-        k_adj_effective_drag<<<B, THREADS, 0, s->stream>>>(g, col, act, f[NK_FIELD_U], f[NK_FIELD_V], s->ad_qu, s->ad_qv,
+        k_adj_effective_drag<<<B, THREADS, 0, s->stream>>>(g, col, act, xsu, xsv, s->ad_qu, s->ad_qv,
                                                            f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_F2], s->ad_bbu,
                                                            s->ad_bbv, s->ad_F2n);
     }));
     CHK(coloured(s, g.Nx, g.Ny, [&](dim3 B, Colour col) {
-        k_adj_viscosity_diva<<<B, THREADS, 0, s->stream>>>(g, col, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_ETASIGMA],
+        k_adj_viscosity_diva<<<B, THREADS, 0, s->stream>>>(g, col, p, sig_tables(s), f[NK_FIELD_U], f[NK_FIELD_V], lag.es,
                                                            s->plane, f[NK_FIELD_A], f[NK_FIELD_H],
                                                            s->ph.masking ? f[NK_FIELD_MU] : nullptr, f[NK_FIELD_BETABU],
                                                            f[NK_FIELD_BETABV], f[NK_FIELD_UB], f[NK_FIELD_VB], s->ad_eta, s->ad_F2n,
@@ -2210,10 +2254,49 @@ nk_status adjoint_step_diva(nk_solver *s, bool parameters) {
                                                  f[NK_FIELD_VB]);
     }));
     return coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 B, Colour col) {
-        k_adj_basal_velocity<<<B, THREADS, 0, s->stream>>>(g, col, f[NK_FIELD_UB], f[NK_FIELD_VB], f[NK_FIELD_BETABU],
-                                                           f[NK_FIELD_BETABV], f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu,
-                                                           s->ad_yv, s->out_bbu, s->out_bbv, s->out_F2);
+        k_adj_basal_velocity<<<B, THREADS, 0, s->stream>>>(g, col, f[NK_FIELD_UB], f[NK_FIELD_VB], lag.bbu, lag.bbv, lag.F2,
+                                                           s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->out_bbu, s->out_bbv,
+                                                           s->out_F2);
     });
+}
+
+// The cost's pullback through DIVA's post-processing (compute_velocity_profile!, after the Picard loop) at the current
+// state: x̄ = (UBAR, VBAR) + ū from the layer velocities' shadows (USIGMABAR, VSIGMABAR) into ad_xb, and the shadows
+// s̄ = (βᵇ̄, F̄₂, ησ̄) the cost gives the state into cs_*.
+nk_status diva_profile_pullback(nk_solver *s) {
+    const KG &g = s->g;
+    const dim3 B = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
+    double **f = s->f;
+    // the cost's pullback through the layer velocities (compute_velocity_profile!, after the Picard loop): ū joins x̄,
+    // and the DIVA state gets the shadows s̄ that every fixed-point step adds to its sweep
+    double *z[] = {s->cs_bbu, s->cs_bbv, s->cs_F2, s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv};
+    for (double *a : z) {
+        k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, a, nullptr, 0.0, nullptr);
+        CHK(launch_check(s));
+    }
+    for (int k = 0; k < s->Ns; k++) {
+        k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->cs_es + (long)k * s->plane, s->out_es + (long)k * s->plane, 0.0, nullptr);
+        CHK(launch_check(s));
+    }
+    // uᵇ of the post-processing, at x* and the state there, into out_bbu/out_bbv (free until the first step clears them)
+    double *ubp = s->out_bbu, *vbp = s->out_bbv;
+    k_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                     f[NK_FIELD_F2], ubp, vbp);
+    CHK(launch_check(s));
+    CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 Bk, Colour col) {
+        k_adj_velocity_profile<<<Bk, THREADS, 0, s->stream>>>(g, col, sig_tables(s), s->Ns, s->plane, f[NK_FIELD_ETASIGMA],
+                                                              f[NK_FIELD_H], ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                              f[NK_FIELD_USIGMABAR], f[NK_FIELD_VSIGMABAR], s->ad_ubb, s->ad_vbb,
+                                                              s->cs_bbu, s->cs_bbv, s->cs_es);
+    }));
+    CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 Bk, Colour col) {
+        k_adj_basal_velocity<<<Bk, THREADS, 0, s->stream>>>(g, col, ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
+                                                            f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->cs_bbu,
+                                                            s->cs_bbv, s->cs_F2);
+    }));
+    k_add<<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_yu, s->ad_yv, s->ad_xbu, s->ad_xbv);
+    CHK(launch_check(s));
+    return NK_OK;
 }
 
 // Geometry pullback of one coefficient step: H̄, b̄ from τ̄ = (lu, lv), q̄ in ad_q and η̄H̄ in ad_eta (two passes,
@@ -2266,35 +2349,7 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
     const bool diva = s->ph.stress_balance == NK_DIVA;
     const double *xbu = f[NK_FIELD_UBAR], *xbv = f[NK_FIELD_VBAR];
     if (diva) {
-        // the cost's pullback through the layer velocities (compute_velocity_profile!, after the Picard loop): ū joins x̄,
-        // and the DIVA state gets the shadows s̄ that every fixed-point step adds to its sweep
-        double *z[] = {s->cs_bbu, s->cs_bbv, s->cs_F2, s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv};
-        for (double *a : z) {
-            k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, a, nullptr, 0.0, nullptr);
-            CHK(launch_check(s));
-        }
-        for (int k = 0; k < s->Ns; k++) {
-            k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, s->cs_es + (long)k * s->plane, s->out_es + (long)k * s->plane, 0.0, nullptr);
-            CHK(launch_check(s));
-        }
-        // uᵇ of the post-processing, at x* and the state there, into out_bbu/out_bbv (free until the first step clears them)
-        double *ubp = s->out_bbu, *vbp = s->out_bbv;
-        k_basal_velocity<<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
-                                                         f[NK_FIELD_F2], ubp, vbp);
-        CHK(launch_check(s));
-        CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 Bk, Colour col) {
-            k_adj_velocity_profile<<<Bk, THREADS, 0, s->stream>>>(g, col, sig_tables(s), s->Ns, s->plane, f[NK_FIELD_ETASIGMA],
-                                                                  f[NK_FIELD_H], ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
-                                                                  f[NK_FIELD_USIGMABAR], f[NK_FIELD_VSIGMABAR], s->ad_ubb, s->ad_vbb,
-                                                                  s->cs_bbu, s->cs_bbv, s->cs_es);
-        }));
-        CHK(coloured(s, g.Nx + 1, g.Ny + 1, [&](dim3 Bk, Colour col) {
-            k_adj_basal_velocity<<<Bk, THREADS, 0, s->stream>>>(g, col, ubp, vbp, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV],
-                                                                f[NK_FIELD_F2], s->ad_ubb, s->ad_vbb, s->ad_yu, s->ad_yv, s->cs_bbu,
-                                                                s->cs_bbv, s->cs_F2);
-        }));
-        k_add<<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_yu, s->ad_yv, s->ad_xbu, s->ad_xbv);
-        CHK(launch_check(s));
+        CHK(diva_profile_pullback(s));
         xbu = s->ad_xbu;
         xbv = s->ad_xbv;
         // first step's state shadows: s̄ + 0
@@ -2377,25 +2432,24 @@ nk_status adjoint(nk_solver *s, const nk_control &ctl, nk_info *info) {
 
 // Reverse of picard_solve! with FixedIterations(n) (src/Solvers/picard.jl:63-69), differentiated step by step as
 // Enzyme does where the converged-Picard rule does not apply (that rule is only for Convergence controls,
-// ext/NunatakEnzymeExt.jl:121). Step k is x_k = A(c_k)⁻¹ τ with c_k = c(x_{k−1}), warm-started at x_{k−1}; its solve
-// has the implicit reverse rule of solve! (ext/NunatakEnzymeExt.jl:68-103): λ_k = A(c_k)⁻¹ x̄_k from zero, τ̄ += λ_k,
-// and the coefficient pullback of q = A(c(x_{k−1})) x_k with q̄ = −λ_k gives x̄_{k−1} and the shadows of C, A, H and b;
+// ext/NunatakEnzymeExt.jl:121). Step k is x_k = A(c_k)⁻¹ τ with c_k = c(x_{k−1}) (DIVA: c_k, s_k = c(x_{k−1}, s_{k−1}) with
+// the lagged state s = (βᵇ, F₂, ησ)), warm-started at x_{k−1}; its solve has the implicit reverse rule of solve!
+// (ext/NunatakEnzymeExt.jl:68-103): λ_k = A(c_k)⁻¹ x̄_k from zero, τ̄ += λ_k, and the coefficient pullback of
+// q = A(c(x_{k−1})) x_k with q̄ = −λ_k gives x̄_{k−1} (DIVA: also s̄_{k−1} from s̄_k) and the shadows of C, A, H and b;
 // the initial guess gets no shadow. The iterates come from replaying the forward steps from the initial guess that
-// picard() recorded, into a tape of n + 1 iterates (2(n + 2) planes with λ_k, allocated on first use). SSA only.
+// picard() recorded, into a tape of the n + 1 iterates (and states), allocated on first use.
 nk_status adjoint_unrolled(nk_solver *s, int n, nk_info *info) {
     const KG &g = s->g;
     const dim3 B = blocks_for(g.Nx, g.Ny), Be = blocks_for(g.Nx + 1, g.Ny + 1);
     double **f = s->f;
-    if (s->ph.stress_balance != NK_SSA) {
-        s->err = "the unrolled adjoint of FixedIterations Picard is built for SSA only (DIVA: use a Convergence control)";
-        return NK_UNSUPPORTED;
-    }
+    const bool diva = s->ph.stress_balance == NK_DIVA;
     if (!s->x0_valid) {
         s->err = "the unrolled adjoint of FixedIterations Picard replays the last FixedIterations Picard solve of this handle: run one first";
         return NK_INVALID_ARGUMENT;
     }
     const long P = plane_of(g);
-    const long need = 2L * (n + 2);
+    const long SP = diva ? 5L + s->Ns : 2L;   // planes per tape step: x (u, v) and, DIVA, βᵇu, βᵇv, F₂, ησ
+    const long need = SP * (n + 1) + 2;
     if (s->tape_planes < need) {
         if (s->tape) {
             HIPCHK(hipStreamSynchronize(s->stream));
@@ -2412,20 +2466,45 @@ nk_status adjoint_unrolled(nk_solver *s, int n, nk_info *info) {
         s->tape_planes = need;
         s->device_bytes += (size_t)need * (size_t)P * sizeof(double);
     }
-    auto tu = [&](int k) { return s->tape + (2L * k) * P; };
-    auto tv = [&](int k) { return s->tape + (2L * k + 1) * P; };
-    double *tlu = s->tape + 2L * (n + 1) * P, *tlv = tlu + P;
-    // forward replay from the recorded initial guess
-    k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->x0u, s->x0v, f[NK_FIELD_U], f[NK_FIELD_V]);
-    CHK(launch_check(s));
+    auto step = [&](int k) { return s->tape + SP * k * P; };
+    auto tu = [&](int k) { return step(k); };
+    auto tv = [&](int k) { return step(k) + P; };
+    auto lag_of = [&](int k) { return DivaLag{step(k) + 2 * P, step(k) + 3 * P, step(k) + 4 * P, step(k) + 5 * P}; };
+    double *tlu = s->tape + SP * (n + 1) * P, *tlv = tlu + P;
+    auto set_state = [&](int k) -> nk_status {   // (U, V) = x_k and, DIVA, the fields of the lagged state = s_k
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, tu(k), tv(k), f[NK_FIELD_U], f[NK_FIELD_V]);
+        CHK(launch_check(s));
+        if (diva) {
+            const DivaLag L = lag_of(k);
+            CHK(copy_diva_state(s, L.bbu, L.bbv, L.F2, L.es, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_F2],
+                                f[NK_FIELD_ETASIGMA]));
+        }
+        return NK_OK;
+    };
+    auto record = [&](int k) -> nk_status {      // tape step k = (U, V) and, DIVA, the fields' state
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], tu(k), tv(k));
+        CHK(launch_check(s));
+        if (diva) {
+            const DivaLag L = lag_of(k);
+            CHK(copy_diva_state(s, f[NK_FIELD_BETABU], f[NK_FIELD_BETABV], f[NK_FIELD_F2], f[NK_FIELD_ETASIGMA],
+                                (double *)L.bbu, (double *)L.bbv, (double *)L.F2, (double *)L.es));
+        }
+        return NK_OK;
+    };
+    // forward replay from the recorded initial guess (and lagged state)
     k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->x0u, s->x0v, tu(0), tv(0));
     CHK(launch_check(s));
+    if (diva) {
+        const DivaLag L = lag_of(0);
+        CHK(copy_diva_state(s, s->s0_bbu, s->s0_bbv, s->s0_F2, s->s0_es, (double *)L.bbu, (double *)L.bbv, (double *)L.F2,
+                            (double *)L.es));
+    }
+    CHK(set_state(0));
     nk_info fwd{};
     for (int k = 1; k <= n; k++) {
         CHK(update_coefficients(s));
         CHK(solve_linear(s, &fwd));
-        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_U], f[NK_FIELD_V], tu(k), tv(k));
-        CHK(launch_check(s));
+        CHK(record(k));
     }
     // reverse sweep
     double *lu = f[NK_FIELD_LAMBDAU], *lv = f[NK_FIELD_LAMBDAV];
@@ -2438,14 +2517,19 @@ nk_status adjoint_unrolled(nk_solver *s, int n, nk_info *info) {
         k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, f[NK_FIELD_ABAR] + (long)k * s->plane, nullptr, 0.0, nullptr);
         CHK(launch_check(s));
     }
-    k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_xbu, s->ad_xbv);   // x̄_n
-    CHK(launch_check(s));
+    if (diva) {
+        // the post-processing's pullback at the final state: x̄_n = x̄ + ū, and s̄_n from the layer velocities
+        CHK(diva_profile_pullback(s));
+        CHK(copy_diva_state(s, s->cs_bbu, s->cs_bbv, s->cs_F2, s->cs_es, s->st_bbu, s->st_bbv, s->st_F2, s->st_es));
+    } else {
+        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, f[NK_FIELD_UBAR], f[NK_FIELD_VBAR], s->ad_xbu, s->ad_xbv);   // x̄_n
+        CHK(launch_check(s));
+    }
     long total = 0;
     double res = 0.0;
     for (int k = n; k >= 1; k--) {
-        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, tu(k - 1), tv(k - 1), f[NK_FIELD_U], f[NK_FIELD_V]);
-        CHK(launch_check(s));
-        CHK(update_coefficients(s));                                              // c_k = c(x_{k−1})
+        CHK(set_state(k - 1));
+        CHK(update_coefficients(s));                                              // c_k (and s_k) from x_{k−1} (and s_{k−1})
         CHK(update_preconditioner(s));
         k_fill2<double><<<Be, THREADS, 0, s->stream>>>(g, tlu, tlv, 0.0, nullptr);
         CHK(launch_check(s));
@@ -2456,23 +2540,28 @@ nk_status adjoint_unrolled(nk_solver *s, int n, nk_info *info) {
         CHK(launch_check(s));
         k_negate<<<B, THREADS, 0, s->stream>>>(g, tlu, tlv, s->ad_qu, s->ad_qv);   // q̄ = −λ_k
         CHK(launch_check(s));
-        CHK(adjoint_step(s, true, true, tu(k), tv(k)));                          // x̄_{k−1}, C̄ and Ā += ...
+        if (diva) {
+            CHK(adjoint_step_diva(s, true, tu(k), tv(k), lag_of(k - 1)));       // x̄_{k−1}, s̄_{k−1} (out_*), C̄ and Ā += ...
+            CHK(copy_diva_state(s, s->out_bbu, s->out_bbv, s->out_F2, s->out_es, s->st_bbu, s->st_bbv, s->st_F2, s->st_es));
+        } else {
+            CHK(adjoint_step(s, true, true, tu(k), tv(k)));                      // x̄_{k−1}, C̄ and Ā += ...
+        }
         k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, s->ad_yu, s->ad_yv, s->ad_xbu, s->ad_xbv);
         CHK(launch_check(s));
-        CHK(geometry_pullback(s, tlu, tlv, tu(k), tv(k), true));                 // H̄, b̄ += ...
+        CHK(geometry_pullback(s, tlu, tlv, diva ? nullptr : tu(k), diva ? nullptr : tv(k), true));   // H̄, b̄ += ...
         if (s->verbose) {
             std::fprintf(stderr, "[nunatak_hip] unrolled adjoint step %d: linear %d\n", k, cg);
             std::fflush(stderr);
         }
     }
-    // the handle as the forward solve left it: c(x_{n−1}) and (U, V) = x_n
+    // the handle as the forward solve left it: the coefficients (and state) of step n, (U, V) = x_n, post-processing
     if (n >= 1) {
-        k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, tu(n - 1), tv(n - 1), f[NK_FIELD_U], f[NK_FIELD_V]);
-        CHK(launch_check(s));
+        CHK(set_state(n - 1));
         CHK(update_coefficients(s));
     }
     k_copy2<double><<<B, THREADS, 0, s->stream>>>(g, tu(n), tv(n), f[NK_FIELD_U], f[NK_FIELD_V]);
     CHK(launch_check(s));
+    CHK(post_process(s));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (info) {
         info->picard_iterations = n;

@@ -210,14 +210,15 @@ def test_unconverged_adjoint_warns(capfd):
     assert "Picard adjoint did not converge" in err
 
 
-def diva_problem():
+def diva_problem(picard=None):
     """diva_problem of test/test_enzyme.jl:63-90: 8 × 6 (Bounded, Periodic), Nσ = 6, τ given, φ = 1, no ice mask,
-    A = 1e-21(1 + cos(2πy/Ly)/5), Weertman C = 1e6(1 + x/Lx), PCG 1e-14, Picard 1e-12."""
+    A = 1e-21(1 + cos(2πy/Ly)/5), Weertman C = 1e6(1 + x/Lx), PCG 1e-14, Picard 1e-12 (or the given control)."""
     Lx, Ly, tau0 = 8e4, 6e4, 1e4
     g = grid(8, 6, Lx, Ly, topology=(B, P), halo=4, Nsigma=6)
     phys = abi.default_physics(stress_balance=abi.DIVA, masking=0, friction=abi.FRICTION_WEERTMAN,
                                strain_rate_regularization=1e-16)
-    opts = abi.default_options(linear=abi.convergence(1e-14, 0.0, 10_000), picard=abi.convergence(1e-12, 0.0, 300))
+    opts = abi.default_options(linear=abi.convergence(1e-14, 0.0, 10_000),
+                               picard=picard if picard is not None else abi.convergence(1e-12, 0.0, 300))
     m = HIPModel(g, phys, opts)
     Az = (Lx / 8) * (Ly / 6)
     m.set("H", field_values(g, "H", lambda x, y: 1000 + x / 1e2 + 100 * np.sin(2 * np.pi * y / Ly)))
@@ -591,3 +592,59 @@ def test_unrolled_adjoint_multigrid_equals_diagonal_and_restores_the_forward_sta
         out.append((m.get("CBAR").copy(), m.get("HBAR").copy()))
     for d, mg in zip(*out):
         assert close_rtol(mg, d, 1e-8)
+
+
+def diva_layer_adjoint(m, g):
+    """diva_layer_cost and its adjoint; returns J and copies of (CBAR, ABAR, LAMBDAU)."""
+    J, (u, v, us, vs) = diva_layer_cost(m, g)
+    m.set("UBAR", 2 * u)
+    m.set("VBAR", 2 * v)
+    m.set("USIGMABAR", 2 * us)
+    m.set("VSIGMABAR", 2 * vs)
+    info = m.adjoint()
+    return J, info, {k: m.get(k).copy() for k in ("CBAR", "ABAR", "LAMBDAU")}
+
+
+def test_diva_fixed_point_adjoint_equals_unrolled_adjoint_of_60_iterations():
+    """test/test_enzyme.jl:306-337: with the reference's full DIVA cost (u, v and the layer velocities), the fixed-point
+    adjoint of Convergence(reltol = 1e-12) against the taped adjoint of FixedIterations(60) — the unrolled reverse
+    carrying the lagged state (βᵇ, F₂, ησ) of each step — for ∂J/∂A and ∂J/∂C, rtol 1e-8 (the reference's bar)."""
+    m, g = diva_problem()
+    _, _, fp = diva_layer_adjoint(m, g)
+    t, gt = diva_problem(abi.fixed_iterations(60))
+    _, info, un = diva_layer_adjoint(t, gt)
+    assert info["picard_iterations"] == 60
+    for k in ("ABAR", "CBAR", "LAMBDAU"):
+        say(f"[adjoint] DIVA fixed point vs unrolled {k}: {np.max(np.abs(fp[k] - un[k])) / np.max(np.abs(un[k])):.2e}")
+    # the reference checks the parameter gradients; τ̄ = Σ λ_k of the 60 steps converges more slowly through the lagged
+    # state (observed 5e-8), so it gets a looser bar
+    for k in ("ABAR", "CBAR"):
+        assert close_rtol(fp[k], un[k], 1e-8), k
+    assert close_rtol(fp["LAMBDAU"], un["LAMBDAU"], 1e-6)
+
+
+def test_diva_unrolled_adjoint_of_three_iterations_matches_finite_differences():
+    """Three DIVA Picard steps from zero with the full cost: the unrolled reverse (lagged state from the tape, each step
+    linearized at its input) against centred differences of J after FixedIterations(3) for A and C (h = 1e-4 of the
+    value); bar 1e-5."""
+    m, g = diva_problem(abi.fixed_iterations(3))
+    J, info, gr = diva_layer_adjoint(m, g)
+    assert info["picard_iterations"] == 3
+    C, A = m.get("C").copy(), m.get("A").copy()
+
+    def fd(name, arr, index, h):
+        a = arr.copy()
+        a[index] += h
+        m.set(name, a)
+        Jp = diva_layer_cost(m, g)[0]
+        a[index] -= 2 * h
+        m.set(name, a)
+        Jm = diva_layer_cost(m, g)[0]
+        m.set(name, arr)
+        return (Jp - Jm) / (2 * h)
+
+    for name, grad, arr, index in (("A", gr["ABAR"], A, (1, 2)), ("A", gr["ABAR"], A, (4, 5)), ("C", gr["CBAR"], C, (2, 3)),
+                                   ("C", gr["CBAR"], C, (5, 6))):
+        d = fd(name, arr, index, 1e-4 * abs(arr[index]))
+        say(f"[adjoint] DIVA FixedIterations(3) dJ/d{name}{index}: unrolled {grad[index]:.10e}, fd {d:.10e}")
+        assert abs(grad[index] - d) < 1e-5 * abs(d)
