@@ -799,8 +799,83 @@ __device__ __forceinline__ int g_unknown(const KG &g, int comp, int i, int j) { 
 // Systems of up to GDENSE_LDS unknowns are eliminated in LDS (n² doubles of dynamic shared memory) and only the result
 // is written to M; larger ones are eliminated in M itself.
 constexpr int GDENSE_LDS = 128;
+// Gauss-Jordan of an n × n system (n ≤ GDENSE_LDS) held in registers: with GINV_ROWS × GDENSE_LDS threads, thread t keeps
+// column j = t mod GDENSE_LDS of rows t / GDENSE_LDS + GINV_ROWS·r. Each step publishes the pivot, then the scaled
+// pivot row and the pivot column through LDS (double-buffered, two barriers per step), and every thread updates its
+// elements with the same operations, in the same order per element, as the shared-memory elimination below — so the
+// inverse is that loop's bit for bit, without its per-element index divisions and LDS round trips.
+constexpr int GINV_ROWS = TAIL_THREADS / GDENSE_LDS;
+constexpr int GINV_R = GDENSE_LDS / GINV_ROWS;   // rows per thread
+__device__ __forceinline__ void g_inverse_regs(const double *__restrict__ Ml, double *__restrict__ Mout, int n, double tol,
+                                               const double *dorig, unsigned char *sing) {
+    __shared__ double rowk[2][GDENSE_LDS], colk[2][GDENSE_LDS];
+    __shared__ double pivs[2];
+    __shared__ int skips[2];
+    const int tid = threadIdx.x, j = tid % GDENSE_LDS, rb = tid / GDENSE_LDS;
+    const bool cj = j < n;
+    double R[GINV_R];
+#pragma unroll
+    for (int r = 0; r < GINV_R; r++) {
+        const int i = rb + GINV_ROWS * r;
+        R[r] = (cj && i < n) ? Ml[(long)i * n + j] : 0.0;
+    }
+    __syncthreads();   // every thread has its elements before the caller's LDS matrix may be reused
+    for (int k = 0; k < n; k++) {
+        const int b = k & 1, kr = k / GINV_ROWS;
+        const bool rowk_owner = rb == k % GINV_ROWS;
+        if (rowk_owner && j == k) {   // the pivot element's owner
+            double pk = 0.0;
+#pragma unroll
+            for (int r = 0; r < GINV_R; r++) pk = r == kr ? R[r] : pk;
+            const int sk = !(pk > tol * fabs(dorig[k]));
+            skips[b] = sk;
+            if (sk) sing[k] = 1;
+            else pivs[b] = 1.0 / pk;
+        }
+        __syncthreads();
+        if (skips[b]) continue;
+        const double piv = pivs[b];
+        if (rowk_owner && cj) {   // row k: M[k][k] = 1 first, then the whole row times 1/p
+#pragma unroll
+            for (int r = 0; r < GINV_R; r++)
+                if (r == kr) {
+                    const double v = (j == k ? 1.0 : R[r]) * piv;
+                    R[r] = v;
+                    rowk[b][j] = v;
+                }
+        }
+        if (j == k) {   // column k after the row scaling
+#pragma unroll
+            for (int r = 0; r < GINV_R; r++) {
+                const int i = rb + GINV_ROWS * r;
+                if (i < n) colk[b][i] = R[r];
+            }
+        }
+        __syncthreads();
+        if (cj) {
+            const double mkj = rowk[b][j], mkk = rowk[b][k];
+#pragma unroll
+            for (int r = 0; r < GINV_R; r++) {
+                const int i = rb + GINV_ROWS * r;
+                if (i >= n || i == k) continue;
+                const double f = colk[b][i];
+                R[r] = (j == k) ? -f * mkk : R[r] - f * mkj;
+            }
+        }
+    }
+    __syncthreads();
+    // null directions dropped; stored column-major (element (r, c) at c·n + r) for coalesced solves
+    if (cj) {
+#pragma unroll
+        for (int r = 0; r < GINV_R; r++) {
+            const int i = rb + GINV_ROWS * r;
+            if (i < n) Mout[(long)j * n + i] = (sing[i] || sing[j]) ? 0.0 : R[r];
+        }
+    }
+}
+
 template <typename TS, int TX, int TY>
-__global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__restrict__ st, double *__restrict__ Mout) {
+__global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__restrict__ st, double *__restrict__ Mout, int regs) {
     using T = double;
     g.tx = TX; g.ty = TY;
     extern __shared__ __align__(16) double gsm[];
@@ -840,6 +915,10 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__re
         sing[i] = 0;
     }
     __syncthreads();
+    if (regs && lds && nth == GINV_ROWS * GDENSE_LDS) {
+        g_inverse_regs(M, Mout, n, tol, dorig, sing);
+        return;
+    }
     for (int k = 0; k < n; k++) {
         if (tid == 0) {
             const T p = M[(long)k * n + k];

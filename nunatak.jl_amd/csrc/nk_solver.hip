@@ -127,6 +127,8 @@ struct nk_solver {
                                 // stage the iterate through LDS (NK_NO_ATILE=1: k_apply_m, k_sweep_m, k_sweep_m_dot,
                                 // k_gp_sweep, k_gp_sweep_pre, k_gp_residual)
     bool use_gpack = true;      // launched Galerkin sweeps read the packed stencil (NK_NO_GPACK=1 at creation: full)
+    bool ginv_regs = true;      // the coarsest Galerkin inverse eliminates in registers (NK_NO_GINV_REGS=1 at creation:
+                                // in LDS, element by element)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
                                 // flag-byte kernels k_apply, k_sweep, k_sweep_dot, k_sweep_z64)
     // enthalpy step workspace, allocated by the first nk_energy_step: face volume fluxes (Nσ + 1 planes), the column
@@ -418,7 +420,7 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
         Level &C = s->lev.back();
         const int n = 2 * C.g.Nx * C.g.Ny;
         const size_t shm = n <= GDENSE_LDS ? sizeof(double) * (size_t)n * (size_t)n : 0;
-        NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, shm, s->stream, C.g, gs<T>(C), (double *)s->cinv);
+        NK_LAUNCH_TOPO(C.g, k_g_inverse, T, 1, TAIL_THREADS, shm, s->stream, C.g, gs<T>(C), (double *)s->cinv, s->ginv_regs ? 1 : 0);
         CHK(launch_check(s));
     }
     for (size_t l = 1; l < s->lev.size(); l++) {
@@ -1522,6 +1524,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->use_z32 = !(nz && nz[0] == '1');
         const char *np = std::getenv("NK_NO_GPACK");
         s->use_gpack = !(np && np[0] == '1');
+        const char *ngr = std::getenv("NK_NO_GINV_REGS");
+        s->ginv_regs = !(ngr && ngr[0] == '1');
         if (options->precision != NK_PRECISION_FP64) {
             float **w32[] = {&s->w32.ru, &s->w32.rv, &s->w32.zu, &s->w32.zv, &s->w32.z2u, &s->w32.z2v, &s->w32.pu, &s->w32.pv,
                              &s->w32.qu, &s->w32.qv, &s->w32.du, &s->w32.dv, &s->rl1u32, &s->rl1v32};

@@ -329,3 +329,27 @@ def test_latency_bound_forms_equal_bandwidth_forms(monkeypatch, precision, topol
         f"CG {out[0][2]} / {out[1][2]}")
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision,topology,N,Ny", [(MIX, (B, B), 128, None), (MIX, (P, P), 96, None), (F64, (B, P), 128, 64),
+                                                     (MIX, (P, B), 160, 96), (F64, (P, P), 64, None)])
+def test_register_inverse_equals_lds_inverse(monkeypatch, precision, topology, N, Ny):
+    """The coarsest Galerkin level's dense generalized inverse is eliminated with the matrix in registers (a column of
+    16 rows per thread, pivot row and column shared through LDS); with NK_NO_GINV_REGS=1 at creation the same
+    Gauss-Jordan steps run element by element on the matrix in LDS. Each element sees the same operations in the same
+    order, so solutions and iteration counts must be identical bit for bit — also on doubly periodic floating shelves,
+    whose rigid translations are null directions that the elimination drops."""
+    out = []
+    for regs in (True, False):
+        if regs:
+            monkeypatch.delenv("NK_NO_GINV_REGS", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_GINV_REGS", "1")
+        m, info = channel(N, abi.COARSE_GALERKIN, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    monkeypatch.delenv("NK_NO_GINV_REGS", raising=False)
+    say(f"[galerkin] register vs LDS coarsest inverse (precision {precision}, {topology}, {N}x{Ny or N}): "
+        f"CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
