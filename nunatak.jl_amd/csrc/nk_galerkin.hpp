@@ -266,11 +266,21 @@ __device__ __forceinline__ bool g_fold(int &d, int lo, int hi, int N, int period
     return d >= lo && d <= hi;
 }
 
+// Activity of the coarse columns within ±2 of (I, J) as bits of the mask g_rap_point tests: bit (dx + 2)·5 + (dy + 2)
+// for u, + 25 for v; one offset (dx, dy) = (k / 5 − 2, k % 5 − 2) of the 25.
+__device__ __forceinline__ uint64_t g_cmask_bits(const KG &cg, const uint8_t *cact, int I, int J, int k) {
+    const int dx = k / 5 - 2, dy = k % 5 - 2;
+    int ic = I + dx, jc = J + dy;
+    if (!cg.tx) ic = wrapi(ic, cg.Nx);
+    if (!cg.ty) jc = wrapi(jc, cg.Ny);
+    return ((uint64_t)uact(cg, cact, ic, jc) << k) | ((uint64_t)vact(cg, cact, ic, jc) << (25 + k));
+}
+
 // Contribution of one fine point f = support point (a, b) of the restriction of coarse row RC at (I, J):
-// w_f · Σ over the fine couplings f′ of f of A_l(f, f′) · P(f′, C), over the coarse parents C of f′ (coarse-active),
-// added into acc[0..31) (the row's couplings in pattern order).
+// w_f · Σ over the fine couplings f′ of f of A_l(f, f′) · P(f′, C), over the coarse parents C of f′ (coarse-active,
+// from cmask: g_cmask_bits over the 25 offsets), added into acc[0..31) (the row's couplings in pattern order).
 template <typename T, int RC>
-__device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, int hy, const uint8_t *cact, const uint8_t *fact,
+__device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, int hy, uint64_t cmask, const uint8_t *fact,
                                             const T *__restrict__ fst, long fplane, int I, int J, int a, int b, T *acc, int *err) {
     constexpr int base = gblk(RC, 0).base;
     const int fr = RC == 0 ? 1 : 0;   // restriction: u is Face along x, v is Face along y
@@ -278,19 +288,6 @@ __device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, 
     const T w = r_weight<T>(I, a, cg.Nx, fg.tx, fr, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 1 - fr, hy);
     if (w == T(0)) return;
     if (!(RC == 0 ? uact(fg, fact, i, j) : vact(fg, fact, i, j))) return;
-    // activity of the coarse columns within ±2 of (I, J): bit (dx + 2)·5 + (dy + 2) for u, + 25 for v
-    uint64_t cmask = 0;
-#pragma unroll
-    for (int dx = -2; dx <= 2; dx++)
-#pragma unroll
-        for (int dy = -2; dy <= 2; dy++) {
-            const int bit = (dx + 2) * 5 + (dy + 2);
-            int ic = I + dx, jc = J + dy;
-            if (!cg.tx) ic = wrapi(ic, cg.Nx);
-            if (!cg.ty) jc = wrapi(jc, cg.Ny);
-            cmask |= (uint64_t)uact(cg, cact, ic, jc) << bit;
-            cmask |= (uint64_t)vact(cg, cact, ic, jc) << (25 + bit);
-        }
     const long fo = off(fg, i, j);
     T av[31];
 #pragma unroll
@@ -350,12 +347,21 @@ __global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, cons
     cg.tx = TX; cg.ty = TY;
     fg.tx = TX; fg.ty = TY;
     __shared__ T part[2 * GRAP_POINTS][12][31];
+    __shared__ unsigned long long cmask[GRAP_POINTS];
     const int tid = threadIdx.y * BX + threadIdx.x;
     const int slot = tid / 16, sub = tid % 16, rc = slot & 1;
     const int npts = cg.Nx * cg.Ny;
     const int q = blockIdx.x * GRAP_POINTS + (slot >> 1);
     const int I = q % cg.Nx + 1, J = q / cg.Nx + 1;
     const bool active = q < npts && (rc == 0 ? uact(cg, cact, I, J) : vact(cg, cact, I, J));
+    // the coarse activity around each of the block's points, formed once per point (32 threads each, 25 offsets)
+    if (tid < GRAP_POINTS) cmask[tid] = 0ull;
+    __syncthreads();
+    {
+        const int p = tid / 32, k = tid % 32, qp = blockIdx.x * GRAP_POINTS + p;
+        if (k < 25 && qp < npts) atomicOr(&cmask[p], (unsigned long long)g_cmask_bits(cg, cact, qp % cg.Nx + 1, qp / cg.Nx + 1, k));
+    }
+    __syncthreads();
     if (sub < 12) {
         T *acc = part[slot][sub];
         for (int k = 0; k < 31; k++) acc[k] = T(0);
@@ -364,8 +370,9 @@ __global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, cons
             const bool on = (hx || a == 0) && (hy || b == 0);
             if (on) {
                 const long fP = gplane(fg);
-                if (rc == 0) g_rap_point<T, 0>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, a, b, acc, err);
-                else g_rap_point<T, 1>(cg, fg, hx, hy, cact, fact, fst, fP, I, J, a, b, acc, err);
+                const uint64_t cm = cmask[slot >> 1];
+                if (rc == 0) g_rap_point<T, 0>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                else g_rap_point<T, 1>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
             }
         }
     }
