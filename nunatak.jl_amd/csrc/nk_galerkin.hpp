@@ -881,6 +881,35 @@ __device__ __forceinline__ void g_inverse_regs(const double *__restrict__ Ml, do
     }
 }
 
+// Row RC at (i, j) of the dense coarsest operator: its 31 couplings added into their columns of Mrow in pattern order
+// (an out-of-range column carries a zero coupling; on a small Periodic level two offsets may wrap onto one column and
+// add in that order). The couplings are loaded first, all in flight together, then added.
+template <typename T, typename TS, int RC>
+__device__ __forceinline__ void g_inverse_row(const KG &g, const TS *__restrict__ st, long P, int i, int j, T *Mrow) {
+    constexpr int base = gblk(RC, 0).base;
+    const long o = off(g, i, j);
+    T cf[31];
+#pragma unroll
+    for (int k = 0; k < 31; k++) cf[k] = (T)st[(base + k) * P + o];
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+        constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
+        const GBlk b = cc == 0 ? B0 : B1;
+#pragma unroll
+        for (int dx = b.x0; dx <= b.x1; dx++)
+#pragma unroll
+            for (int dy = b.y0; dy <= b.y1; dy++) {
+                const T a = cf[gidx(b, dx, dy) - base];
+                if (a == T(0)) continue;
+                int ic = i + dx, jc = j + dy;
+                if (!g.tx) ic = wrapi(ic, g.Nx);
+                if (!g.ty) jc = wrapi(jc, g.Ny);
+                if (ic < 1 || ic > g.Nx || jc < 1 || jc > g.Ny) continue;
+                Mrow[g_unknown(g, cc, ic, jc)] += a;
+            }
+    }
+}
+
 template <typename TS, int TX, int TY>
 __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__restrict__ st, double *__restrict__ Mout, int regs) {
     using T = double;
@@ -901,20 +930,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_g_inverse(KG g, const TS *__re
     // assemble: row r of component rc at (i, j) from its 31 couplings (out-of-range columns carry zero couplings)
     for (int r = tid; r < n; r += nth) {
         const int rc = r >= n / 2, k = r - rc * (n / 2), i = k % g.Nx + 1, j = k / g.Nx + 1;
-        const long o = off(g, i, j);
-        for (int cc = 0; cc < 2; cc++) {
-            const GBlk b = gblk(rc, cc);
-            for (int dx = b.x0; dx <= b.x1; dx++)
-                for (int dy = b.y0; dy <= b.y1; dy++) {
-                    const T a = (T)st[gidx(b, dx, dy) * P + o];
-                    if (a == T(0)) continue;
-                    int ic = i + dx, jc = j + dy;
-                    if (!g.tx) ic = wrapi(ic, g.Nx);
-                    if (!g.ty) jc = wrapi(jc, g.Ny);
-                    if (ic < 1 || ic > g.Nx || jc < 1 || jc > g.Ny) continue;
-                    M[(long)r * n + g_unknown(g, cc, ic, jc)] += a;
-                }
-        }
+        if (rc == 0) g_inverse_row<T, TS, 0>(g, st, P, i, j, M + (long)r * n);
+        else g_inverse_row<T, TS, 1>(g, st, P, i, j, M + (long)r * n);
     }
     __syncthreads();
     for (int i = tid; i < n; i += nth) {
