@@ -99,3 +99,19 @@ def test_enum_mirrors_match_header():
         for name, _ in pairs:
             if name.startswith(family):
                 assert hasattr(abi, name), name
+
+
+def test_julia_binding_field_ids_match_header():
+    """The Julia binding numbers the fields as consecutive constants; they must follow nk_field_id's order in
+    include/nunatak_hip.h exactly (a shifted id would copy one model field into another)."""
+    text = open(HEADER).read()
+    body = text[text.index("typedef enum nk_field_id {"):text.index("} nk_field_id;")]
+    header_ids = re.findall(r"^\s*NK_FIELD_([A-Z0-9_]+)\s*[,=]?", body, re.M)   # enumerators, not mentions in comments
+    assert header_ids[-1] == "COUNT"
+    header_ids = header_ids[:-1]
+    jl = open(os.path.join(ROOT, "nunatak.jl_amd", "julia", "NunatakHIP.jl")).read()
+    m = re.search(r"# field ids \(nk_field_id\)\nconst (.*?) = 0:(\d+)", jl, re.S)
+    assert m, "field id block not found in NunatakHIP.jl"
+    names = [n.strip() for n in m.group(1).replace("\n", " ").split(",")]
+    assert names == header_ids
+    assert int(m.group(2)) == len(header_ids) - 1
