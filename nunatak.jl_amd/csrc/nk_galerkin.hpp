@@ -788,6 +788,98 @@ __global__ __launch_bounds__(NT) void k_gp_sweep_pre_tl(KG g, const T *__restric
     ov[o] = b_v * s_v + (b_v - qv) * s_v;
 }
 
+// The residual of a latency-bound packed Galerkin level and its restriction to the level below in one launch. A block
+// owns an RRX × RRY tile of the coarse level. Each of its threads forms one fine residual r = b − A x that the tile's
+// restriction reads — at the fine index k_restrict reads it (r_index's map) and masked with the fine activity as
+// k_restrict masks it — into LDS, with every load it needs (the row's 31 couplings, the 31 iterate values, b and the
+// activity byte) issued at once; then 2·RRX·RRY threads form b = R r of the tile's points with k_restrict's weights,
+// order and coarse activity and store it. The residual arithmetic is k_gp_residual's (gp_row_cf in gp_row_d's order)
+// and the restriction restrict_u / restrict_v's, so b is k_gp_residual_tl + k_restrict's bit for bit, with one
+// launch and one round trip to memory fewer; r itself is not stored (only the restriction reads it).
+constexpr int RRX = 32, RRY = 2;                          // coarse tile of a block
+constexpr int RRU_W = 2 * RRX + 1, RRU_H = 2 * RRY + 2;   // fine u residuals of the tile (both directions halved)
+constexpr int RRV_W = 2 * RRX + 2, RRV_H = 2 * RRY + 1;   // fine v residuals
+constexpr int RR_THREADS = 768;                           // ≥ RRU_W·RRU_H + RRV_W·RRV_H = 720 slots, one per thread
+static_assert(RRU_W * RRU_H + RRV_W * RRV_H <= RR_THREADS, "one residual slot per thread");
+// r_index's fine index for an unmapped one, x = 2I − 1 + a on a halved direction (and I itself on an un-halved one)
+__device__ __forceinline__ int rr_map(int x, int N, int bounded, int face, int h) {
+    if (!h) return x;
+    if (!bounded) return wrapi(x, N);
+    if (face) return x > 1 ? x : 1;
+    return clampi(x, 1, N);
+}
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(RR_THREADS) void k_gp_residual_restrict(KG fg, const T *__restrict__ pk, const T *__restrict__ xu,
+                                                                    const T *__restrict__ xv, const T *__restrict__ bu,
+                                                                    const T *__restrict__ bv, const uint8_t *__restrict__ fact,
+                                                                    KG cg, int hx, int hy, const uint8_t *__restrict__ cact,
+                                                                    T *__restrict__ cbu, T *__restrict__ cbv, const Scal *gate) {
+    fg.tx = TX; fg.ty = TY;
+    cg.tx = TX; cg.ty = TY;
+    if (solve_done(gate)) return;
+    __shared__ T su[RRU_H][RRU_W], sv[RRV_H][RRV_W];
+    int bx_, by_;
+    xcd_tile(bx_, by_);
+    const int I0 = bx_ * RRX + 1, J0 = by_ * RRY + 1;
+    // unmapped fine index of slot 0 along each direction, and the slot extents of the two components
+    const int xo = hx ? 2 * I0 - 2 : I0, yo = hy ? 2 * J0 - 2 : J0;
+    const int uw = hx ? RRU_W : RRX, uh = hy ? RRU_H : RRY, vw = hx ? RRV_W : RRX, vh = hy ? RRV_H : RRY;
+    const int tid = threadIdx.x, nu = uw * uh;
+    const bool isu = tid < nu, live = tid < nu + vw * vh;
+    const int t = isu ? tid : tid - nu, w = isu ? uw : vw;
+    const int si = live ? t % w : 0, sj = live ? t / w : 0;
+    // unmapped indices beyond the ones a coarse point of the level reads are clamped, so every map and address stays
+    // in range (a packed level has at least 16 points a side, so the clamp never moves a slot that is read)
+    const int xi = clampi(xo + si, -fg.Nx, 2 * fg.Nx), yj = clampi(yo + sj, -fg.Ny, 2 * fg.Ny);
+    const int mi = rr_map(xi, fg.Nx, fg.tx, isu ? 1 : 0, hx), mj = rr_map(yj, fg.Ny, fg.ty, isu ? 0 : 1, hy);
+    // the residual is formed at an in-range point (the k_gp_residual range); an out-of-range one is a wall, inactive
+    const int ci = clampi(mi, 1, fg.Nx), cj = clampi(mj, 1, fg.Ny);
+    const long o = off(fg, ci, cj);
+    const bool act = isu ? uact(fg, fact, mi, mj) : vact(fg, fact, mi, mj);
+    T cf[31];
+    if (isu) gp_coefs<T, 0>(fg, pk, ci, cj, cf);
+    else gp_coefs<T, 1>(fg, pk, ci, cj, cf);
+    const T b0 = isu ? bu[o] : bv[o];
+    auto XU = [&](int dx, int dy) { return xu[off(fg, fx(fg, ci + dx), cy(fg, cj + dy))]; };
+    auto XV = [&](int dx, int dy) { return xv[off(fg, cx(fg, ci + dx), fy(fg, cj + dy))]; };
+    const T q = isu ? gp_row_cf<T, 0>(cf, XU, XV) : gp_row_cf<T, 1>(cf, XU, XV);
+    const T r = b0 - q;
+    if (live) {
+        if (isu) su[sj][si] = act ? r : T(0);
+        else sv[sj][si] = act ? r : T(0);
+    }
+    __syncthreads();
+    if (tid >= 2 * RRX * RRY) return;
+    const int comp = tid / (RRX * RRY), pp = tid % (RRX * RRY);
+    const int I = I0 + pp % RRX, J = J0 + pp / RRX;
+    if (I > cg.Nx || J > cg.Ny) return;
+    const long oc = off(cg, I, J);
+    T sum = T(0);
+    if (comp == 0) {   // restrict_u's loop
+#pragma unroll
+        for (int a = -1; a <= 1; a++)
+#pragma unroll
+            for (int b = -1; b <= 2; b++) {
+                const bool on = (hx || a == 0) && (hy || b == 0);
+                const T wgt = r_weight<T>(I, a, cg.Nx, fg.tx, 1, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 0, hy);
+                const T val = su[(hy ? 2 * J - 1 + b : J) - yo][(hx ? 2 * I - 1 + a : I) - xo];
+                if (on) sum += wgt * val;
+            }
+        cbu[oc] = (cact[oc] & 1) ? sum : T(0);
+    } else {           // restrict_v's loop
+#pragma unroll
+        for (int a = -1; a <= 2; a++)
+#pragma unroll
+            for (int b = -1; b <= 1; b++) {
+                const bool on = (hx || a == 0) && (hy || b == 0);
+                const T wgt = r_weight<T>(I, a, cg.Nx, fg.tx, 0, hx) * r_weight<T>(J, b, cg.Ny, fg.ty, 1, hy);
+                const T val = sv[(hy ? 2 * J - 1 + b : J) - yo][(hx ? 2 * I - 1 + a : I) - xo];
+                if (on) sum += wgt * val;
+            }
+        cbv[oc] = (cact[oc] & 2) ? sum : T(0);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Coarsest level: the Galerkin operator as a dense matrix, inverted once per build (Gauss-Jordan without pivoting:
 // the operator is symmetric positive semidefinite, an inactive row is an identity row). A pivot that falls below

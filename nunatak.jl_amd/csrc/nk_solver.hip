@@ -127,6 +127,8 @@ struct nk_solver {
                                 // stage the iterate through LDS (NK_NO_ATILE=1: k_apply_m, k_sweep_m, k_sweep_m_dot,
                                 // k_gp_sweep, k_gp_sweep_pre, k_gp_residual)
     bool use_gpack = true;      // launched Galerkin sweeps read the packed stencil (NK_NO_GPACK=1 at creation: full)
+    bool fuse_rr = true;        // latency-bound packed Galerkin levels: residual and restriction in one launch
+                                // (k_gp_residual_restrict; NK_NO_RR_FUSE=1 at creation: k_gp_residual_tl + k_restrict)
     bool ginv_regs = true;      // the coarsest Galerkin inverse eliminates in registers (NK_NO_GINV_REGS=1 at creation:
                                 // in LDS, element by element)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
@@ -798,15 +800,23 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     CHK(smooth(cur, oth, s->op.smoothing_sweeps, true));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
-    if (pk && s->apply_tile && lat)
-        NK_LAUNCH_TOPO(L.g, k_gp_residual_tl, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
-    else if (pk && s->apply_tile)
-        NK_LAUNCH_TOPO(L.g, k_gp_residual_t, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
-    else if (pk) NK_LAUNCH_TOPO(L.g, k_gp_residual, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
-    else NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
-    CHK(launch_check(s));
-    NK_LAUNCH_TOPO(C.g, k_restrict, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, L.g, C.cx, C.cy, C.act, L.act, V.qu, V.qv, CV.bu, CV.bv, gate);
-    CHK(launch_check(s));
+    if (pk && s->apply_tile && lat && s->fuse_rr) {   // residual and restriction in one launch
+        const dim3 Brr((C.g.Nx + RRX - 1) / RRX, (C.g.Ny + RRY - 1) / RRY);
+        NK_LAUNCH_TOPO(L.g, k_gp_residual_restrict, T, Brr, dim3(RR_THREADS), 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, L.act, C.g,
+                       C.cx, C.cy, C.act, CV.bu, CV.bv, gate);
+        CHK(launch_check(s));
+    } else {
+        if (pk && s->apply_tile && lat)
+            NK_LAUNCH_TOPO(L.g, k_gp_residual_tl, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+        else if (pk && s->apply_tile)
+            NK_LAUNCH_TOPO(L.g, k_gp_residual_t, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+        else if (pk) NK_LAUNCH_TOPO(L.g, k_gp_residual, T, B, THREADS, 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+        else NK_LAUNCH_TOPO(L.g, k_g_residual, T, B, THREADS, 0, s->stream, L.g, st, cur.u, cur.v, bu, bv, V.qu, V.qv, gate);
+        CHK(launch_check(s));
+        NK_LAUNCH_TOPO(C.g, k_restrict, T, blocks_for(C.g.Nx, C.g.Ny), THREADS, 0, s->stream, C.g, L.g, C.cx, C.cy, C.act, L.act, V.qu,
+                       V.qv, CV.bu, CV.bv, gate);
+        CHK(launch_check(s));
+    }
     Pair<T> cres;
     CHK(v_cycle<T>(s, l + 1, Pair<T>{CV.x0u, CV.x0v}, Pair<T>{CV.x1u, CV.x1v}, CV.bu, CV.bv, false, gate, &cres));
     // prolongation inside the first post-sweep on the launch-latency-bound levels (≤ 256 × 256 points by default)
@@ -1524,6 +1534,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->use_z32 = !(nz && nz[0] == '1');
         const char *np = std::getenv("NK_NO_GPACK");
         s->use_gpack = !(np && np[0] == '1');
+        const char *nrr = std::getenv("NK_NO_RR_FUSE");
+        s->fuse_rr = !(nrr && nrr[0] == '1');
         const char *ngr = std::getenv("NK_NO_GINV_REGS");
         s->ginv_regs = !(ngr && ngr[0] == '1');
         if (options->precision != NK_PRECISION_FP64) {

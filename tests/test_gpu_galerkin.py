@@ -353,3 +353,33 @@ def test_register_inverse_equals_lds_inverse(monkeypatch, precision, topology, N
         f"CG {out[0][2]} / {out[1][2]}")
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision,topology,N,Ny,latency", [(MIX, (B, B), 128, None, None), (MIX, (P, P), 96, None, None),
+                                                             (MIX, (B, P), 160, 96, None), (F64, (P, B), 128, None, None),
+                                                             (MIX, (B, B), 256, None, "100000000"),
+                                                             (MIX, (P, B), 200, 120, "100000000"),
+                                                             (F64, (B, P), 144, 48, "100000000")])
+def test_residual_and_restriction_in_one_launch_equal_separate_kernels(monkeypatch, precision, topology, N, Ny, latency):
+    """On latency-bound packed Galerkin levels the residual and the restriction to the level below run as one kernel
+    (k_gp_residual_restrict: the residuals the restriction reads formed in LDS at the fine indices k_restrict reads);
+    with NK_NO_RR_FUSE=1 at creation k_gp_residual_tl stores r and k_restrict reads it. The same values in the same order,
+    so solutions and iteration counts must be identical bit for bit — on all four topologies, on semi-coarsened
+    hierarchies and with every level below the finest taking the fused form (NK_LATENCY_POINTS = 10⁸)."""
+    out = []
+    if latency:
+        monkeypatch.setenv("NK_LATENCY_POINTS", latency)
+    for fused in (True, False):
+        if fused:
+            monkeypatch.delenv("NK_NO_RR_FUSE", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_RR_FUSE", "1")
+        m, info = channel(N, abi.COARSE_GALERKIN, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    monkeypatch.delenv("NK_NO_RR_FUSE", raising=False)
+    monkeypatch.delenv("NK_LATENCY_POINTS", raising=False)
+    say(f"[galerkin] residual + restriction fused vs separate (precision {precision}, {topology}, {N}x{Ny or N}, latency "
+        f"points {latency}): CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
