@@ -127,8 +127,9 @@ struct nk_solver {
                                 // stage the iterate through LDS (NK_NO_ATILE=1: k_apply_m, k_sweep_m, k_sweep_m_dot,
                                 // k_gp_sweep, k_gp_sweep_pre, k_gp_residual)
     bool use_gpack = true;      // launched Galerkin sweeps read the packed stencil (NK_NO_GPACK=1 at creation: full)
-    bool fuse_rr = true;        // latency-bound packed Galerkin levels: residual and restriction in one launch
-                                // (k_gp_residual_restrict; NK_NO_RR_FUSE=1 at creation: k_gp_residual_tl + k_restrict)
+    bool fuse_rr = true;        // packed Galerkin levels of up to rr_points points: residual and restriction in one
+                                // launch (k_gp_residual_restrict; NK_NO_RR_FUSE=1 at creation: a residual kernel + k_restrict)
+    long rr_points = 256L * 256;   // (NK_RR_POINTS: tuning)
     bool ginv_regs = true;      // the coarsest Galerkin inverse eliminates in registers (NK_NO_GINV_REGS=1 at creation:
                                 // in LDS, element by element)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
@@ -800,7 +801,7 @@ nk_status v_cycle_galerkin(nk_solver *s, int l, Pair<T> x0, Pair<T> x1, const T 
     CHK(smooth(cur, oth, s->op.smoothing_sweeps, true));
     Level &C = s->lev[l + 1];
     Vecs<T> &CV = vecs<T>(C);
-    if (pk && s->apply_tile && lat && s->fuse_rr) {   // residual and restriction in one launch
+    if (pk && s->apply_tile && s->fuse_rr && (long)L.g.Nx * L.g.Ny <= s->rr_points) {   // residual and restriction in one launch
         const dim3 Brr((C.g.Nx + RRX - 1) / RRX, (C.g.Ny + RRY - 1) / RRY);
         NK_LAUNCH_TOPO(L.g, k_gp_residual_restrict, T, Brr, dim3(RR_THREADS), 0, s->stream, L.g, pk, cur.u, cur.v, bu, bv, L.act, C.g,
                        C.cx, C.cy, C.act, CV.bu, CV.bv, gate);
@@ -1536,6 +1537,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->use_gpack = !(np && np[0] == '1');
         const char *nrr = std::getenv("NK_NO_RR_FUSE");
         s->fuse_rr = !(nrr && nrr[0] == '1');
+        const char *nrrp = std::getenv("NK_RR_POINTS");
+        if (nrrp) s->rr_points = std::strtol(nrrp, nullptr, 10);
         const char *ngr = std::getenv("NK_NO_GINV_REGS");
         s->ginv_regs = !(ngr && ngr[0] == '1');
         if (options->precision != NK_PRECISION_FP64) {
