@@ -899,18 +899,21 @@ __device__ __forceinline__ int g_unknown(const KG &g, int comp, int i, int j) { 
 // is written to M; larger ones are eliminated in M itself.
 constexpr int GDENSE_LDS = 128;
 // Gauss-Jordan of an n × n system (n ≤ GDENSE_LDS) held in registers: with GINV_ROWS × GDENSE_LDS threads, thread t keeps
-// column j = t mod GDENSE_LDS of rows t / GDENSE_LDS + GINV_ROWS·r. Each step publishes the pivot, then the scaled
-// pivot row and the pivot column through LDS (double-buffered, two barriers per step), and every thread updates its
-// elements with the same operations, in the same order per element, as the shared-memory elimination below — so the
-// inverse is that loop's bit for bit, without its per-element index divisions and LDS round trips.
+// column j = t mod GDENSE_LDS of rows t / GDENSE_LDS + GINV_ROWS·r (a wave's 64 lanes share one row group, so row
+// indices are wave-uniform). Step k reads row k, column k and 1/M[k][k] as step k − 1 left them from LDS (double
+// buffered, one barrier per step): the owners of row k + 1 and column k + 1 publish them, and 1/M[k+1][k+1], as the
+// step produces them. Every element is updated with the operations, in the order, of the shared-memory elimination
+// below — row k: (j == k ? 1 : M[k][j])·(1/p); other rows: M[i][j] − M[i][k]·m_kj; column k: (−0) − M[i][k]·(1/p),
+// which is −(M[i][k]·(1/p)) bit for bit, zeros' signs included — so the inverse is that loop's bit for bit.
+// The update of the other rows has no selects (the pivot row is overwritten afterwards by its owners alone): the
+// FP64 multiply and subtract of 16 rows per thread per step set the pace.
 constexpr int GINV_ROWS = TAIL_THREADS / GDENSE_LDS;
 constexpr int GINV_R = GDENSE_LDS / GINV_ROWS;   // rows per thread
 __device__ __forceinline__ void g_inverse_regs(const double *__restrict__ Ml, double *__restrict__ Mout, int n, double tol,
                                                const double *dorig, unsigned char *sing) {
-    __shared__ double rowk[2][GDENSE_LDS], colk[2][GDENSE_LDS];
-    __shared__ double pivs[2];
-    __shared__ int skips[2];
-    const int tid = threadIdx.x, j = tid % GDENSE_LDS, rb = tid / GDENSE_LDS;
+    __shared__ double rowb[2][GDENSE_LDS], colb[2][GDENSE_LDS], pivb[2];
+    const int tid = threadIdx.x, j = tid % GDENSE_LDS;
+    const int rb = __builtin_amdgcn_readfirstlane(tid / GDENSE_LDS);
     const bool cj = j < n;
     double R[GINV_R];
 #pragma unroll
@@ -918,51 +921,59 @@ __device__ __forceinline__ void g_inverse_regs(const double *__restrict__ Ml, do
         const int i = rb + GINV_ROWS * r;
         R[r] = (cj && i < n) ? Ml[(long)i * n + j] : 0.0;
     }
-    __syncthreads();   // every thread has its elements before the caller's LDS matrix may be reused
-    for (int k = 0; k < n; k++) {
-        const int b = k & 1, kr = k / GINV_ROWS;
-        const bool rowk_owner = rb == k % GINV_ROWS;
-        if (rowk_owner && j == k) {   // the pivot element's owner
-            double pk = 0.0;
+    if (rb == 0 && cj) rowb[0][j] = R[0];
+    if (tid == 0) pivb[0] = 1.0 / R[0];
+    if (j == 0) {
 #pragma unroll
-            for (int r = 0; r < GINV_R; r++) pk = r == kr ? R[r] : pk;
-            const int sk = !(pk > tol * fabs(dorig[k]));
-            skips[b] = sk;
-            if (sk) sing[k] = 1;
-            else pivs[b] = 1.0 / pk;
-        }
-        __syncthreads();
-        if (skips[b]) continue;
-        const double piv = pivs[b];
-        if (rowk_owner && cj) {   // row k: M[k][k] = 1 first, then the whole row times 1/p
-#pragma unroll
-            for (int r = 0; r < GINV_R; r++)
-                if (r == kr) {
-                    const double v = (j == k ? 1.0 : R[r]) * piv;
-                    R[r] = v;
-                    rowk[b][j] = v;
-                }
-        }
-        if (j == k) {   // column k after the row scaling
-#pragma unroll
-            for (int r = 0; r < GINV_R; r++) {
-                const int i = rb + GINV_ROWS * r;
-                if (i < n) colk[b][i] = R[r];
-            }
-        }
-        __syncthreads();
-        if (cj) {
-            const double mkj = rowk[b][j], mkk = rowk[b][k];
-#pragma unroll
-            for (int r = 0; r < GINV_R; r++) {
-                const int i = rb + GINV_ROWS * r;
-                if (i >= n || i == k) continue;
-                const double f = colk[b][i];
-                R[r] = (j == k) ? -f * mkk : R[r] - f * mkj;
-            }
+        for (int r = 0; r < GINV_R; r++) {
+            const int i = rb + GINV_ROWS * r;
+            if (i < n) colb[0][i] = R[r];
         }
     }
     __syncthreads();
+    for (int k = 0; k < n; k++) {
+        const int b = k & 1, nb = b ^ 1, k1 = k + 1;
+        // every read of the step in flight together (column entries at i ≥ n and row entries at j ≥ n are never used)
+        const double pk = rowb[b][k], piv = pivb[b], rkj = rowb[b][j];
+        double f[GINV_R];
+#pragma unroll
+        for (int r = 0; r < GINV_R; r++) f[r] = colb[b][rb + GINV_ROWS * r];
+        if (!(pk > tol * fabs(dorig[k]))) {   // a null direction: unknown k is left out
+            if (tid == 0) sing[k] = 1;
+        } else {
+            const double mkj = (j == k ? 1.0 : rkj) * piv;
+            if (j == k) {
+#pragma unroll
+                for (int r = 0; r < GINV_R; r++)
+                    if (rb + GINV_ROWS * r != k) R[r] = -0.0;
+            }
+#pragma unroll
+            for (int r = 0; r < GINV_R; r++) R[r] = R[r] - f[r] * mkj;   // rows ≥ n are padding, never read
+            if (rb == k % GINV_ROWS) {   // the pivot row: (j == k ? 1 : M[k][j])·(1/p)
+                const int kr = k / GINV_ROWS;
+#pragma unroll
+                for (int r = 0; r < GINV_R; r++) R[r] = r == kr ? mkj : R[r];
+            }
+        }
+        if (k1 < n) {   // row k + 1, its reciprocal pivot and column k + 1 as this step leaves them
+            if (rb == k1 % GINV_ROWS) {
+                const int kr = k1 / GINV_ROWS;
+                double v = 0.0;
+#pragma unroll
+                for (int r = 0; r < GINV_R; r++) v = r == kr ? R[r] : v;
+                if (cj) rowb[nb][j] = v;
+                if (j == k1) pivb[nb] = 1.0 / v;   // unused when step k + 1 leaves the unknown out
+            }
+            if (j == k1) {
+#pragma unroll
+                for (int r = 0; r < GINV_R; r++) {
+                    const int i = rb + GINV_ROWS * r;
+                    if (i < n) colb[nb][i] = R[r];
+                }
+            }
+        }
+        __syncthreads();
+    }
     // null directions dropped; stored column-major (element (r, c) at c·n + r) for coalesced solves
     if (cj) {
 #pragma unroll

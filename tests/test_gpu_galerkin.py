@@ -335,7 +335,8 @@ def test_latency_bound_forms_equal_bandwidth_forms(monkeypatch, precision, topol
                                                      (MIX, (P, B), 160, 96), (F64, (P, P), 64, None)])
 def test_register_inverse_equals_lds_inverse(monkeypatch, precision, topology, N, Ny):
     """The coarsest Galerkin level's dense generalized inverse is eliminated with the matrix in registers (a column of
-    16 rows per thread, pivot row and column shared through LDS); with NK_NO_GINV_REGS=1 at creation the same
+    16 rows per thread; the pivot row, its reciprocal pivot and the pivot column shared through LDS, one barrier per
+    step, the pivot column formed as (−0) − f·(1/p)); with NK_NO_GINV_REGS=1 at creation the same
     Gauss-Jordan steps run element by element on the matrix in LDS. Each element sees the same operations in the same
     order, so solutions and iteration counts must be identical bit for bit — also on doubly periodic floating shelves,
     whose rigid translations are null directions that the elimination drops."""
