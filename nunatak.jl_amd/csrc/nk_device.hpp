@@ -288,6 +288,10 @@ struct Scal {
     // solve to 1e-14 relative takes ‖r‖ to 1e-27 and below, where z32 = r32/ℓ1 would fall into FP32 denormals). A power of
     // two scales every FP32 operation exactly: without underflow, results are the unscaled ones bit for bit.
     double zs, zi;
+    // r·z of the latest preconditioning. A finisher folded into k_update_p / k_update_p32 (each block forms β from
+    // the slots itself) stores r·z here rather than in rho, which its other blocks still read; the α finisher then
+    // moves it into rho. The finishers of k_finish keep rho_new equal to rho.
+    double rho_new;
 };
 
 enum Fin : int {
@@ -302,12 +306,13 @@ enum Fin : int {
 };
 
 // zs = 2^−⌊log₂ n⌋ and zi = 1/zs for a norm n (1 for n = 0 or not finite; the exponent kept within ±1000)
-__device__ __forceinline__ void set_scale(Scal *S, double n) {
+__device__ __forceinline__ void scale_of(double n, double &zs, double &zi) {
     const int e = (n > 0.0 && n <= DBL_MAX) ? ilogb(n) : 0;
     const int k = e < -1000 ? 1000 : (e > 1000 ? -1000 : -e);
-    S->zs = ldexp(1.0, k);
-    S->zi = ldexp(1.0, -k);
+    zs = ldexp(1.0, k);
+    zi = ldexp(1.0, -k);
 }
+__device__ __forceinline__ void set_scale(Scal *S, double n) { scale_of(n, S->zs, S->zi); }
 
 // The stopping rule of iterate_conjugate_gradient! (src/Solvers/conjugate_gradient.jl:178-200): FixedIterations stops after
 // maxiter steps, Convergence when ‖r‖ ≤ tol or at maxiter. A residual norm that is not finite stops the solve and is
@@ -322,6 +327,7 @@ __device__ __forceinline__ void set_done(Scal *S) {
 __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
     switch (fin) {
     case FIN_ALPHA:
+        S->rho = S->rho_new;
         S->pq = t[0];
         S->alpha = t[0] == 0.0 ? 0.0 : S->rho / t[0];
         set_scale(S, S->res);
@@ -331,6 +337,7 @@ __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
         double rz = t[0];
         S->beta = S->rho == 0.0 ? 0.0 : rz / S->rho;
         S->rho = rz;
+        S->rho_new = rz;
         S->rr = t[1];
         S->res = sqrt(t[1]);
         S->it += 1;
@@ -339,6 +346,7 @@ __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
     }
     case FIN_INIT: {
         S->rho = t[0];
+        S->rho_new = t[0];
         S->rr = t[1];
         S->r0 = sqrt(t[1]);
         double a = S->reltol * S->r0, b = S->abstol;
@@ -355,7 +363,7 @@ __device__ __forceinline__ void finish(Scal *S, int fin, const double *t) {
         S->it += 1;
         set_done(S);
         break;
-    case FIN_BETA: S->beta = S->rho == 0.0 ? 0.0 : t[0] / S->rho; S->rho = t[0]; break;
+    case FIN_BETA: S->beta = S->rho == 0.0 ? 0.0 : t[0] / S->rho; S->rho = t[0]; S->rho_new = t[0]; break;
     case FIN_PICARD: S->picard_D = sqrt(t[0]); S->picard_X = sqrt(t[1]); break;
     default: break;
     }
@@ -401,43 +409,60 @@ template <int NV> __device__ void reduce_finish(double (&v)[NV], const Red &r) {
 
 __device__ __forceinline__ bool solve_done(const Scal *S) { return S != nullptr && S->done; }
 
-// Sum NV block partials of the preceding kernel and run the finisher (one block). Each thread first loads its
-// share of every value's slots in batches of FIN_BATCH (all loads of a batch in flight together), then adds them in a
-// fixed order, so the totals are the same on every run.
+// Sum the nv values' block slots of the preceding kernel (nblocks slots each) into tot[] (LDS, MAXRED entries; the
+// entries past nv are 0), visible to every thread of the block on return. Each thread first loads its share of every
+// value's slots in batches of FIN_BATCH (all loads of a batch in flight together), then adds them in a fixed order,
+// so the totals are the same on every run and in every block that forms them (the block must have NT threads). NVM is
+// the most values the caller may ask for (each value's total is formed alike whatever NVM is).
 constexpr int FIN_BATCH = 8;
-__global__ __launch_bounds__(NT) void k_finish(const double *__restrict__ partials, int nblocks, int nv, Scal *S, int fin,
-                                               const Scal *gate) {
-    if (solve_done(gate)) return;
-    __shared__ double sh[MAXRED][NT / 64];
+template <int NVM>
+__device__ __forceinline__ void fold_sum(const double *__restrict__ partials, int nblocks, int nv, double *tot) {
+    __shared__ double sh[NVM][NT / 64];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int nth = blockDim.x * blockDim.y, nwaves = nth >> 6;
-    double s[MAXRED] = {0.0, 0.0, 0.0, 0.0};
-    for (int b0 = 0; b0 < nblocks; b0 += nth * FIN_BATCH) {
-        double x[MAXRED][FIN_BATCH];
+    double s[NVM];
 #pragma unroll
-        for (int k = 0; k < MAXRED; k++)
+    for (int k = 0; k < NVM; k++) s[k] = 0.0;
+    for (int b0 = 0; b0 < nblocks; b0 += nth * FIN_BATCH) {
+        double x[NVM][FIN_BATCH];
+#pragma unroll
+        for (int k = 0; k < NVM; k++)
 #pragma unroll
             for (int u = 0; u < FIN_BATCH; u++) {
                 const int b = b0 + u * nth + tid;
                 x[k][u] = (k < nv && b < nblocks) ? partials[(long)k * nblocks + b] : 0.0;
             }
 #pragma unroll
-        for (int k = 0; k < MAXRED; k++)
+        for (int k = 0; k < NVM; k++)
 #pragma unroll
             for (int u = 0; u < FIN_BATCH; u++) s[k] += x[k][u];
     }
 #pragma unroll
-    for (int k = 0; k < MAXRED; k++) {
+    for (int k = 0; k < NVM; k++) {
         const double w = wave_sum(s[k]);
         if (lane == 0) sh[k][wave] = w;
     }
     __syncthreads();
     if (tid == 0) {
-        double tot[MAXRED] = {0.0, 0.0, 0.0, 0.0};
-        for (int k = 0; k < nv; k++)
-            for (int w = 0; w < nwaves; w++) tot[k] += sh[k][w];
-        finish(S, fin, tot);
+        double t[MAXRED] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < nv && k < NVM; k++)
+            for (int w = 0; w < nwaves; w++) t[k] += sh[k][w];
+        for (int k = 0; k < MAXRED; k++) tot[k] = t[k];
+    }
+    __syncthreads();
+}
+
+// Sum NV block partials of the preceding kernel and run the finisher (one block).
+__global__ __launch_bounds__(NT) void k_finish(const double *__restrict__ partials, int nblocks, int nv, Scal *S, int fin,
+                                               const Scal *gate) {
+    if (solve_done(gate)) return;
+    __shared__ double tot[MAXRED];
+    fold_sum<MAXRED>(partials, nblocks, nv, tot);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double t[MAXRED];
+        for (int k = 0; k < MAXRED; k++) t[k] = tot[k];
+        finish(S, fin, t);
     }
 }
 
@@ -548,26 +573,67 @@ __global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int p
     reduce_finish<2>(acc, red);
 }
 
+// The α finisher inside the consumer: every block sums the operator's p·q slots (fold_nb of them) as k_finish would
+// and forms α = ρ/(p·q) and the FP32 scale itself; block 0 stores them (and ρ) for the kernels that follow.
+__device__ __forceinline__ void fold_alpha(const double *fold, int fold_nb, Scal *S, double &alpha, double &zs) {
+    __shared__ double tot[MAXRED];
+    fold_sum<1>(fold, fold_nb, 1, tot);
+    const double pq = tot[0], rho = S->rho_new;
+    alpha = pq == 0.0 ? 0.0 : rho / pq;
+    double zi;
+    scale_of(S->res, zs, zi);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        S->rho = rho;
+        S->pq = pq;
+        S->alpha = alpha;
+        S->zs = zs;
+        S->zi = zi;
+        for (int k = 0; k < MAXRED; k++) S->out[k] = tot[k];
+    }
+}
+
+// The β finisher inside the consumer: every block sums the preconditioner's r·z slots (the first value of its
+// reduction) and forms β = r·z / ρ with the previous ρ (which no block of the kernel writes); block 0 stores β and r·z
+// (as rho_new, moved into rho by α).
+__device__ __forceinline__ double fold_beta(const double *fold, int fold_nb, Scal *S) {
+    __shared__ double tot[MAXRED];
+    fold_sum<1>(fold, fold_nb, 1, tot);
+    const double rz = tot[0], beta = S->rho == 0.0 ? 0.0 : rz / S->rho;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        S->beta = beta;
+        S->rho_new = rz;
+        for (int k = 0; k < MAXRED; k++) S->out[k] = tot[k];
+    }
+    return beta;
+}
+
 // Multigrid PCG: x += α p; r += (−α) q; r·r → res, it, done.
 // With r32u: also r32 = (float) (r·zs), the FP32 preconditioner's right-hand side (mixed precision; zs set with α).
+// With fold: α comes from the operator's p·q slots (fold_alpha) instead of a k_finish launch; the r·r slots this kernel
+// writes must then be another buffer than fold.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_update_xr(KG g, T *__restrict__ xu, T *__restrict__ xv, const T *__restrict__ pu,
                                                   const T *__restrict__ pv, const T *__restrict__ qu, const T *__restrict__ qv,
                                                   T *__restrict__ ru, T *__restrict__ rv, float *__restrict__ r32u,
-                                                  float *__restrict__ r32v, Red red, const Scal *S) {
+                                                  float *__restrict__ r32v, Red red, Scal *S, const double *fold, int fold_nb) {
     if (solve_done(S)) return;
+    double alpha, zs;
+    if (fold) fold_alpha(fold, fold_nb, S, alpha, zs);
+    else {
+        alpha = S->alpha;
+        zs = S->zs;
+    }
     NK_IJ
     double acc[1] = {0.0};
     if (i <= g.Nx && j <= g.Ny) {
         const long o = off(g, i, j);
-        const T al = (T)S->alpha;
+        const T al = (T)alpha;
         xu[o] = T(1) * xu[o] + al * pu[o];
         xv[o] = T(1) * xv[o] + al * pv[o];
         const T a = T(1) * ru[o] + (-al) * qu[o], b = T(1) * rv[o] + (-al) * qv[o];
         ru[o] = a;
         rv[o] = b;
         if (r32u) {
-            const double zs = S->zs;
             r32u[o] = (float)((double)a * zs);
             r32v[o] = (float)((double)b * zs);
         }
@@ -576,11 +642,13 @@ __global__ __launch_bounds__(NT) void k_update_xr(KG g, T *__restrict__ xu, T *_
     reduce_finish<1>(acc, red);
 }
 
-// p = z + β p (or p = z when init)
+// p = z + β p (or p = z when init). With fold: β from the preconditioner's r·z slots (fold_beta).
 template <typename T>
 __global__ __launch_bounds__(NT) void k_update_p(KG g, const T *__restrict__ zu, const T *__restrict__ zv, T *__restrict__ pu,
-                                                 T *__restrict__ pv, int init, const Scal *S) {
+                                                 T *__restrict__ pv, int init, Scal *S, const double *fold, int fold_nb) {
     if (solve_done(S) && !init) return;
+    double beta = 0.0;
+    if (!init) beta = fold ? fold_beta(fold, fold_nb, S) : S->beta;
     NK_IJ
     if (i <= g.Nx && j <= g.Ny) {
         const long o = off(g, i, j);
@@ -588,7 +656,7 @@ __global__ __launch_bounds__(NT) void k_update_p(KG g, const T *__restrict__ zu,
             pu[o] = zu[o];
             pv[o] = zv[o];
         } else {
-            const T be = (T)S->beta;
+            const T be = (T)beta;
             pu[o] = T(1) * zu[o] + be * pu[o];
             pv[o] = T(1) * zv[o] + be * pv[o];
         }
@@ -599,12 +667,13 @@ __global__ __launch_bounds__(NT) void k_update_p(KG g, const T *__restrict__ zu,
 // stored as z64, so p is k_update_p's bit for bit without the FP64 copy of z being written and read.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_update_p32(KG g, const float *__restrict__ zu, const float *__restrict__ zv, T *__restrict__ pu,
-                                                   T *__restrict__ pv, const Scal *S) {
+                                                   T *__restrict__ pv, Scal *S, const double *fold, int fold_nb) {
     if (solve_done(S)) return;
+    const double beta = fold ? fold_beta(fold, fold_nb, S) : S->beta;
     NK_IJ
     if (i <= g.Nx && j <= g.Ny) {
         const long o = off(g, i, j);
-        const T be = (T)S->beta, zi = (T)S->zi;
+        const T be = (T)beta, zi = (T)S->zi;
         pu[o] = T(1) * ((T)zu[o] * zi) + be * pu[o];
         pv[o] = T(1) * ((T)zv[o] * zi) + be * pv[o];
     }

@@ -24,7 +24,8 @@ def say(*a):
     sys.stdout.flush()
 
 
-def channel(N, coarse, precision=abi.PRECISION_FP64, picard=1, linear=1e-10, topology=(B, B), Ny=None, walls=None):
+def channel(N, coarse, precision=abi.PRECISION_FP64, picard=1, linear=1e-10, topology=(B, B), Ny=None, walls=None,
+            smoother=None):
     g, phys, opts, fields = W.diva_shelf(N=N, dx=512e3 / N)
     if walls is not None:
         phys.walls = walls
@@ -41,6 +42,8 @@ def channel(N, coarse, precision=abi.PRECISION_FP64, picard=1, linear=1e-10, top
     opts.picard = abi.fixed_iterations(picard)
     opts.coarse_operator = coarse
     opts.precision = precision
+    if smoother is not None:
+        opts.smoother = smoother
     m = HIPModel(g, phys, opts)
     W.load_into(m, fields)
     info = m.update_velocities(geometry=True)
@@ -382,5 +385,38 @@ def test_residual_and_restriction_in_one_launch_equal_separate_kernels(monkeypat
     monkeypatch.delenv("NK_LATENCY_POINTS", raising=False)
     say(f"[galerkin] residual + restriction fused vs separate (precision {precision}, {topology}, {N}x{Ny or N}, latency "
         f"points {latency}): CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision,coarse,topology,N,Ny,smoother,blocks",
+                         [(MIX, abi.COARSE_GALERKIN, (B, B), 128, None, None, None),
+                          (MIX, abi.COARSE_GALERKIN, (P, P), 96, None, None, None),
+                          (F64, abi.COARSE_REDISCRETIZED, (P, B), 128, None, None, None),
+                          (MIX, abi.COARSE_REDISCRETIZED, (B, P), 160, 96, abi.LINE_JACOBI, None),
+                          (F64, abi.COARSE_GALERKIN, (B, B), 256, None, None, None),
+                          (MIX, abi.COARSE_GALERKIN, (B, B), 1024, 520, None, "1000000000")])
+def test_folded_finishers_equal_k_finish(monkeypatch, precision, coarse, topology, N, Ny, smoother, blocks):
+    """On grids of at most NK_FOLD_BLOCKS workgroups the multigrid PCG's α and β finishers run inside k_update_xr and
+    the direction update (every block sums the reduction slots itself, in k_finish's order) instead of as k_finish
+    launches; NK_NO_FOLD=1 at creation keeps the launches. Solutions and iteration counts must be identical bit for bit
+    — with the FP32 V-cycle's z read directly (k_update_p32), through the FP64 z (line smoother: k_promote_dot's r·z
+    with r·r beside it), in the FP64 form (k_update_p), and on a 1024 × 520 grid with folding forced (2,080 slots per
+    value, more than one load batch per thread)."""
+    out = []
+    if blocks:
+        monkeypatch.setenv("NK_FOLD_BLOCKS", blocks)
+    for folded in (True, False):
+        if folded:
+            monkeypatch.delenv("NK_NO_FOLD", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_FOLD", "1")
+        m, info = channel(N, coarse, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny, smoother=smoother)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    monkeypatch.delenv("NK_NO_FOLD", raising=False)
+    monkeypatch.delenv("NK_FOLD_BLOCKS", raising=False)
+    say(f"[pcg] folded vs launched finishers (precision {precision}, coarse {coarse}, {topology}, {N}x{Ny or N}, "
+        f"smoother {smoother}): CG {out[0][2]} / {out[1][2]}")
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
