@@ -1715,7 +1715,7 @@ size_t planned_bytes(const nk_grid *grid, const nk_solver_options *o) {
     if (mixed) b += 14 * sizeof(float) * p0;
     if (o->precision == NK_PRECISION_MIXED_REFINED) b += 2 * sizeof(float) * p0;
     const size_t nb = (size_t)((g0.Nx + BX - 1) / BX) * (size_t)((g0.Ny + BY - 1) / BY);
-    b += sizeof(double) * MAXRED * nb + sizeof(Scal);
+    b += 2 * sizeof(double) * MAXRED * nb + sizeof(Scal);                     // two slot buffers (partials, partials_xr)
     if (o->coarse_operator == NK_COARSE_GALERKIN) {
         b += 64;
         const size_t n = 2 * (size_t)lev.back().Nx * (size_t)lev.back().Ny;
