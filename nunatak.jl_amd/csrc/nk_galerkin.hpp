@@ -30,6 +30,22 @@ __host__ __device__ constexpr GBlk gblk(int rc, int cc) {
 }
 __host__ __device__ constexpr int gidx(const GBlk &b, int dx, int dy) { return b.base + (dx - b.x0) * (b.y1 - b.y0 + 1) + (dy - b.y0); }
 
+// The couplings the finest level's assembled stencil can hold (k_g_assemble: the rediscretized operator's 5-point
+// blocks and its 4 cross couplings per row, 9 of the 31 slots); its other slots stay zero from allocation.
+__host__ __device__ constexpr bool fin_slot(int rc, int cc, int dx, int dy) {
+    return rc == cc ? ((dx == 0 && dy >= -1 && dy <= 1) || (dy == 0 && (dx == -1 || dx == 1)))
+                    : (rc == 0 ? (dx >= -1 && dx <= 0 && dy >= 0 && dy <= 1) : (dx >= 0 && dx <= 1 && dy >= -1 && dy <= 0));
+}
+__host__ __device__ constexpr bool fin_slot_k(int rc, int k) {   // slot k (0..62) of row type rc in pattern order
+    for (int cc = 0; cc < 2; cc++) {
+        const GBlk b = gblk(rc, cc);
+        for (int dx = b.x0; dx <= b.x1; dx++)
+            for (int dy = b.y0; dy <= b.y1; dy++)
+                if (gidx(b, dx, dy) == k) return fin_slot(rc, cc, dx, dy);
+    }
+    return false;
+}
+
 __device__ __forceinline__ long gplane(const KG &g) { return (long)g.pitch * (long)(g.Ny + 1); }
 
 // Row RC (0 = u, 1 = v) of A x at (i, j); x read through the location's index maps (wrap on Periodic, clamp on
@@ -221,7 +237,11 @@ __global__ __launch_bounds__(NT) void k_g_assemble(KG g, Coef<T> c, T wf, T *__r
     if (i > g.Nx || j > g.Ny) return;
     const long o = off(g, i, j), P = gplane(g);
     const T dx = (T)g.dx, dy = (T)g.dy, Az = (T)(g.dx * g.dy);
-    for (int k = 0; k < GS_PLANES; k++) st[k * P + o] = T(0);
+    // only the 18 slots the rediscretized operator can fill are written (zero first); the other 44 stay zero from
+    // allocation, and the triple product of the first coarse level reads only these (fin_slot)
+#pragma unroll
+    for (int k = 0; k < GS_PLANES; k++)
+        if (fin_slot_k(k < 31 ? 0 : 1, k)) st[k * P + o] = T(0);
     const T ec = ldc(g, c.eta, i, j), ew = ldc(g, c.eta, i - 1, j), es = ldc(g, c.eta, i, j - 1);
     const T f00 = ldff(g, c.etaff, i, j) * (periph_ff(g, i, j) ? wf : T(1));
     const T f01 = ldff(g, c.etaff, i, j + 1) * (periph_ff(g, i, j + 1) ? wf : T(1));
@@ -279,7 +299,9 @@ __device__ __forceinline__ uint64_t g_cmask_bits(const KG &cg, const uint8_t *ca
 // Contribution of one fine point f = support point (a, b) of the restriction of coarse row RC at (I, J):
 // w_f · Σ over the fine couplings f′ of f of A_l(f, f′) · P(f′, C), over the coarse parents C of f′ (coarse-active,
 // from cmask: g_cmask_bits over the 25 offsets), added into acc[0..31) (the row's couplings in pattern order).
-template <typename T, int RC>
+// FIN: the fine level is the finest, whose stencil holds only the fin_slot couplings (the others are exact zeros,
+// which this loop skips anyway), so only those are loaded and visited.
+template <typename T, int RC, bool FIN>
 __device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, int hy, uint64_t cmask, const uint8_t *fact,
                                             const T *__restrict__ fst, long fplane, int I, int J, int a, int b, T *acc, int *err) {
     constexpr int base = gblk(RC, 0).base;
@@ -291,7 +313,7 @@ __device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, 
     const long fo = off(fg, i, j);
     T av[31];
 #pragma unroll
-    for (int k = 0; k < 31; k++) av[k] = fst[(base + k) * fplane + fo];
+    for (int k = 0; k < 31; k++) av[k] = (!FIN || fin_slot_k(RC, base + k)) ? fst[(base + k) * fplane + fo] : T(0);
 #pragma unroll
     for (int cc = 0; cc < 2; cc++) {
         constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
@@ -300,6 +322,7 @@ __device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, 
         for (int dx = fb.x0; dx <= fb.x1; dx++)
 #pragma unroll
             for (int dy = fb.y0; dy <= fb.y1; dy++) {
+                if (FIN && !fin_slot(RC, cc, dx, dy)) continue;
                 const T c = av[gidx(fb, dx, dy) - base];
                 if (c == T(0)) continue;
                 int i2 = i + dx, j2 = j + dy;
@@ -343,7 +366,7 @@ constexpr int GRAP_POINTS = 8;
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
                                               const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
-                                              int *err) {
+                                              int *err, int fin) {
     cg.tx = TX; cg.ty = TY;
     fg.tx = TX; fg.ty = TY;
     __shared__ T part[2 * GRAP_POINTS][12][31];
@@ -371,8 +394,13 @@ __global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, cons
             if (on) {
                 const long fP = gplane(fg);
                 const uint64_t cm = cmask[slot >> 1];
-                if (rc == 0) g_rap_point<T, 0>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
-                else g_rap_point<T, 1>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                if (fin) {
+                    if (rc == 0) g_rap_point<T, 0, true>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                    else g_rap_point<T, 1, true>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                } else {
+                    if (rc == 0) g_rap_point<T, 0, false>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                    else g_rap_point<T, 1, false>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                }
             }
         }
     }

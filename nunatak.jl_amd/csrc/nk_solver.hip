@@ -130,6 +130,8 @@ struct nk_solver {
     bool fuse_rr = true;        // packed Galerkin levels of up to rr_points points: residual and restriction in one
                                 // launch (k_gp_residual_restrict; NK_NO_RR_FUSE=1 at creation: a residual kernel + k_restrict)
     long rr_points = 256L * 256;   // (NK_RR_POINTS: tuning)
+    bool fin_pattern = true;    // the first triple product reads only the finest stencil's 9 couplings per row
+                                // (NK_NO_FIN_PATTERN=1 at creation: all 31, the other 22 zeros skipped as they are read)
     bool ginv_regs = true;      // the coarsest Galerkin inverse eliminates in registers (NK_NO_GINV_REGS=1 at creation:
                                 // in LDS, element by element)
     bool use_nmask = true;      // finest-level operator applies and sweeps read nm16 (NK_NO_NMASK=1 at creation: the
@@ -436,7 +438,7 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
         Level &F = s->lev[l - 1], &C = s->lev[l];
         const dim3 Br((C.g.Nx * C.g.Ny + GRAP_POINTS - 1) / GRAP_POINTS);
         NK_LAUNCH_TOPO(C.g, k_g_rap, T, Br, THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
-                       gs<T>(C), s->gerr);
+                       gs<T>(C), s->gerr, (l == 1 && s->fin_pattern) ? 1 : 0);
         CHK(launch_check(s));
     }
     if (s->cinv) {
@@ -1570,6 +1572,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->fuse_rr = !(nrr && nrr[0] == '1');
         const char *nrrp = std::getenv("NK_RR_POINTS");
         if (nrrp) s->rr_points = std::strtol(nrrp, nullptr, 10);
+        const char *nfp = std::getenv("NK_NO_FIN_PATTERN");
+        s->fin_pattern = !(nfp && nfp[0] == '1');
         const char *ngr = std::getenv("NK_NO_GINV_REGS");
         s->ginv_regs = !(ngr && ngr[0] == '1');
         if (options->precision != NK_PRECISION_FP64) {

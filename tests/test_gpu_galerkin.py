@@ -420,3 +420,28 @@ def test_folded_finishers_equal_k_finish(monkeypatch, precision, coarse, topolog
         f"smoother {smoother}): CG {out[0][2]} / {out[1][2]}")
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision,topology,N,Ny,walls", [(MIX, (B, B), 128, None, None), (MIX, (P, P), 96, None, None),
+                                                           (F64, (P, B), 128, 64, None), (MIX, (B, P), 160, 96, None),
+                                                           (MIX, (B, B), 96, None, abi.FREE_SLIP)])
+def test_first_product_over_the_finest_pattern_equals_full_pattern(monkeypatch, precision, topology, N, Ny, walls):
+    """The first Galerkin product P^T A P reads, of the finest level's assembled stencil, only the 9 couplings per row
+    the rediscretized operator fills (k_g_assemble writes only those); with NK_NO_FIN_PATTERN=1 at creation it reads all
+    31 and skips the zeros as it meets them. The same terms in the same order, so solutions and iteration counts must
+    be identical bit for bit."""
+    out = []
+    for fin in (True, False):
+        if fin:
+            monkeypatch.delenv("NK_NO_FIN_PATTERN", raising=False)
+        else:
+            monkeypatch.setenv("NK_NO_FIN_PATTERN", "1")
+        m, info = channel(N, abi.COARSE_GALERKIN, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny,
+                          walls=walls)
+        out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
+        m.close()
+    monkeypatch.delenv("NK_NO_FIN_PATTERN", raising=False)
+    say(f"[galerkin] first product over the finest pattern vs all 31 couplings (precision {precision}, {topology}, "
+        f"{N}x{Ny or N}, walls {walls}): CG {out[0][2]} / {out[1][2]}")
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
