@@ -301,9 +301,13 @@ __device__ __forceinline__ uint64_t g_cmask_bits(const KG &cg, const uint8_t *ca
 // from cmask: g_cmask_bits over the 25 offsets), added into acc[0..31) (the row's couplings in pattern order).
 // FIN: the fine level is the finest, whose stencil holds only the fin_slot couplings (the others are exact zeros,
 // which this loop skips anyway), so only those are loaded and visited.
+// half ≥ 0: only the fine couplings to component half (columns of that block of the pattern, whose coarse targets are
+// that block's slots, disjoint from the other's), so two lanes can share a support point without changing any slot's
+// order of additions.
 template <typename T, int RC, bool FIN>
 __device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, int hy, uint64_t cmask, const uint8_t *fact,
-                                            const T *__restrict__ fst, long fplane, int I, int J, int a, int b, T *acc, int *err) {
+                                            const T *__restrict__ fst, long fplane, int I, int J, int a, int b, T *acc, int *err,
+                                            int half) {
     constexpr int base = gblk(RC, 0).base;
     const int fr = RC == 0 ? 1 : 0;   // restriction: u is Face along x, v is Face along y
     const int i = r_index(I, a, fg.Nx, fg.tx, fr, hx), j = r_index(J, b, fg.Ny, fg.ty, 1 - fr, hy);
@@ -311,11 +315,14 @@ __device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, 
     if (w == T(0)) return;
     if (!(RC == 0 ? uact(fg, fact, i, j) : vact(fg, fact, i, j))) return;
     const long fo = off(fg, i, j);
+    constexpr int nu = gblk(RC, 1).base - base;   // couplings to u columns; the rest are to v columns
     T av[31];
 #pragma unroll
-    for (int k = 0; k < 31; k++) av[k] = (!FIN || fin_slot_k(RC, base + k)) ? fst[(base + k) * fplane + fo] : T(0);
+    for (int k = 0; k < 31; k++)
+        av[k] = ((!FIN || fin_slot_k(RC, base + k)) && (half < 0 || (k < nu) == (half == 0))) ? fst[(base + k) * fplane + fo] : T(0);
 #pragma unroll
     for (int cc = 0; cc < 2; cc++) {
+        if (half >= 0 && cc != half) continue;
         constexpr GBlk B0 = gblk(RC, 0), B1 = gblk(RC, 1);
         const GBlk fb = cc == 0 ? B0 : B1;
 #pragma unroll
@@ -358,83 +365,100 @@ __device__ __forceinline__ void g_rap_point(const KG &cg, const KG &fg, int hx, 
     }
 }
 
-// A_{l+1} = Pᵀ A_l P and 1/ℓ1 of its rows; inactive coarse rows become identity rows. One 256-thread block per 8
-// coarse points: 16 row slots (u and v of each point) × 16 lanes, lane s < 12 handling support point s of the row's
-// restriction into its own 31 partial sums in LDS; the partials are then summed over the lanes in lane order
-// (deterministic). One-dimensional grid of ⌈Nx·Ny / 8⌉ blocks.
+// A_{l+1} = Pᵀ A_l P and 1/ℓ1 of its rows; inactive coarse rows become identity rows. One 256-thread block per
+// GRAP_POINTS coarse points (k_g_rap): 16 row slots (u and v of each point) × 16 lanes, lane m < 12 handling support
+// point m of the row's restriction into its own 31 partial sums in LDS. k_g_rap_split: half as many points per block
+// and each support point's couplings divided between two lanes by column component (threads 0–127 the u columns,
+// 128–255 the v columns, so a wave takes one half only): the halves fill disjoint slots, every slot's additions keep
+// their order, and each lane's chain of loads and LDS updates is about half as long. The partials are then summed over
+// the support points in order (deterministic). One-dimensional grid of ⌈Nx·Ny / points per block⌉ blocks.
 constexpr int GRAP_POINTS = 8;
-template <typename T, int TX, int TY>
-__global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
-                                              const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
-                                              int *err, int fin) {
+template <typename T, int TX, int TY, int SPLIT>
+__device__ __forceinline__ void g_rap_block(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
+                                            const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
+                                            int *err, int fin) {
     cg.tx = TX; cg.ty = TY;
     fg.tx = TX; fg.ty = TY;
-    __shared__ T part[2 * GRAP_POINTS][12][31];
-    __shared__ unsigned long long cmask[GRAP_POINTS];
+    constexpr int npb = SPLIT ? GRAP_POINTS / 2 : GRAP_POINTS;   // coarse points per block
+    __shared__ T part[2 * npb][12][31];
+    __shared__ T absv[2 * npb][31];
+    __shared__ unsigned long long cmask[npb];
+    __shared__ int sact[2 * npb];
     const int tid = threadIdx.y * BX + threadIdx.x;
-    const int slot = tid / 16, sub = tid % 16, rc = slot & 1;
+    const int half = SPLIT ? (tid >> 7) : -1;
+    const int slot = SPLIT ? ((tid & 127) >> 4) : (tid >> 4), m = tid & 15, rc = slot & 1;
     const int npts = cg.Nx * cg.Ny;
-    const int q = blockIdx.x * GRAP_POINTS + (slot >> 1);
+    const int q = blockIdx.x * npb + (slot >> 1);
     const int I = q % cg.Nx + 1, J = q / cg.Nx + 1;
     const bool active = q < npts && (rc == 0 ? uact(cg, cact, I, J) : vact(cg, cact, I, J));
-    // the coarse activity around each of the block's points, formed once per point (32 threads each, 25 offsets)
-    if (tid < GRAP_POINTS) cmask[tid] = 0ull;
+    if (tid < npb) cmask[tid] = 0ull;
+    if (m == 0 && half <= 0) sact[slot] = active ? 1 : 0;
     __syncthreads();
-    {
-        const int p = tid / 32, k = tid % 32, qp = blockIdx.x * GRAP_POINTS + p;
-        if (k < 25 && qp < npts) atomicOr(&cmask[p], (unsigned long long)g_cmask_bits(cg, cact, qp % cg.Nx + 1, qp / cg.Nx + 1, k));
+    {   // the coarse activity around each of the block's points, formed once per point (32 threads each, 25 offsets)
+        const int p = tid / 32, k = tid % 32, qp = blockIdx.x * npb + p;
+        if (p < npb && k < 25 && qp < npts)
+            atomicOr(&cmask[p], (unsigned long long)g_cmask_bits(cg, cact, qp % cg.Nx + 1, qp / cg.Nx + 1, k));
     }
     __syncthreads();
-    if (sub < 12) {
-        T *acc = part[slot][sub];
-        for (int k = 0; k < 31; k++) acc[k] = T(0);
+    if (m < 12) {
+        T *acc = part[slot][m];
+        const int nu = gblk(rc, 1).base - gblk(rc, 0).base;
+        const int k0 = half == 1 ? nu : 0, k1 = half == 0 ? nu : 31;
+        for (int k = k0; k < k1; k++) acc[k] = T(0);
         if (active) {
-            const int a = rc == 0 ? sub / 4 - 1 : sub / 3 - 1, b = rc == 0 ? sub % 4 - 1 : sub % 3 - 1;
+            const int a = rc == 0 ? m / 4 - 1 : m / 3 - 1, b = rc == 0 ? m % 4 - 1 : m % 3 - 1;
             const bool on = (hx || a == 0) && (hy || b == 0);
             if (on) {
                 const long fP = gplane(fg);
                 const uint64_t cm = cmask[slot >> 1];
                 if (fin) {
-                    if (rc == 0) g_rap_point<T, 0, true>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
-                    else g_rap_point<T, 1, true>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                    if (rc == 0) g_rap_point<T, 0, true>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err, half);
+                    else g_rap_point<T, 1, true>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err, half);
                 } else {
-                    if (rc == 0) g_rap_point<T, 0, false>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
-                    else g_rap_point<T, 1, false>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err);
+                    if (rc == 0) g_rap_point<T, 0, false>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err, half);
+                    else g_rap_point<T, 1, false>(cg, fg, hx, hy, cm, fact, fst, fP, I, J, a, b, acc, err, half);
                 }
             }
         }
     }
     __syncthreads();
-    // lanes of each slot sum couplings k = lane, lane + 16 over the 12 support points, in support order
-    T vals[2] = {T(0), T(0)};
+    // each coupling k of each row: the partials summed over the 12 support points in support order
     const long P = gplane(cg);
-    const int base = gblk(rc, 0).base, diag = gidx(gblk(rc, rc), 0, 0) - base;
-    if (q < npts) {
-        const long o = off(cg, I, J);
-        for (int t = 0; t < 2; t++) {
-            const int k = sub + 16 * t;
-            if (k >= 31) continue;
-            T v = T(0);
-            if (active) {
-                for (int m = 0; m < 12; m++) v += part[slot][m][k];
-            } else {
-                v = k == diag ? T(1) : T(0);
-            }
-            cst[(base + k) * P + o] = v;
-            vals[t] = fabs(v);
+    for (int e = tid; e < 2 * npb * 31; e += NT) {
+        const int ss = e / 31, k = e % 31, rs = ss & 1, qs = blockIdx.x * npb + (ss >> 1);
+        if (qs >= npts) continue;
+        const int base = gblk(rs, 0).base, diag = gidx(gblk(rs, rs), 0, 0) - base;
+        T v = T(0);
+        if (sact[ss]) {
+            for (int mm = 0; mm < 12; mm++) v += part[ss][mm][k];
+        } else {
+            v = k == diag ? T(1) : T(0);
+        }
+        cst[(base + k) * P + off(cg, qs % cg.Nx + 1, qs / cg.Nx + 1)] = v;
+        absv[ss][k] = fabs(v);
+    }
+    __syncthreads();
+    if (tid < 2 * npb) {   // 1/ℓ1 of each row, its |couplings| added in pattern order
+        const int ss = tid, rs = ss & 1, qs = blockIdx.x * npb + (ss >> 1);
+        if (qs < npts) {
+            T l1 = T(0);
+            for (int k = 0; k < 31; k++) l1 += absv[ss][k];
+            cst[(GS_PLANES + rs) * P + off(cg, qs % cg.Nx + 1, qs / cg.Nx + 1)] = l1 > T(0) ? T(1) / l1 : T(1);
         }
     }
-    __syncthreads();
-    if (q < npts) {
-        part[slot][0][sub] = vals[0];
-        if (sub < 15) part[slot][1][sub] = vals[1];
-    }
-    __syncthreads();
-    if (sub == 0 && q < npts) {
-        T l1 = T(0);
-        for (int k = 0; k < 31; k++) l1 += k < 16 ? part[slot][0][k] : part[slot][1][k - 16];
-        cst[(GS_PLANES + rc) * P + off(cg, I, J)] = l1 > T(0) ? T(1) / l1 : T(1);
-    }
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
+                                              const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
+                                              int *err, int fin) {
+    g_rap_block<T, TX, TY, 0>(cg, fg, hx, hy, cact, fact, fst, cst, err, fin);
+}
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_g_rap_split(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
+                                                    const uint8_t *__restrict__ fact, const T *__restrict__ fst,
+                                                    T *__restrict__ cst, int *err, int fin) {
+    g_rap_block<T, TX, TY, 1>(cg, fg, hx, hy, cact, fact, fst, cst, err, fin);
 }
 
 // ℓ1-Jacobi sweep with a stencil operator: x_out = x + (b − A x)/ℓ1, or b/ℓ1 from a zero x.

@@ -130,6 +130,11 @@ struct nk_solver {
     bool fuse_rr = true;        // packed Galerkin levels of up to rr_points points: residual and restriction in one
                                 // launch (k_gp_residual_restrict; NK_NO_RR_FUSE=1 at creation: a residual kernel + k_restrict)
     long rr_points = 256L * 256;   // (NK_RR_POINTS: tuning)
+    bool rap_split = true;      // triple products into levels of at most rap_split_points points split each support
+                                // point's couplings between two lanes by column component (k_g_rap_split;
+                                // NK_NO_RAP_SPLIT=1 at creation: one lane per support point, k_g_rap)
+    long rap_split_points = 256L * 256;   // (NK_RAP_SPLIT_POINTS: tuning; into 512² from 1024² the split measured
+                                          // 491 against 444 µs, into 256² and below 335 against 344 µs down to 30 against 53)
     bool fin_pattern = true;    // the first triple product reads only the finest stencil's 9 couplings per row
                                 // (NK_NO_FIN_PATTERN=1 at creation: all 31, the other 22 zeros skipped as they are read)
     bool ginv_regs = true;      // the coarsest Galerkin inverse eliminates in registers (NK_NO_GINV_REGS=1 at creation:
@@ -436,9 +441,15 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
     HIPCHK(hipMemsetAsync(s->gerr, 0, sizeof(int), s->stream));
     for (size_t l = 1; l < s->lev.size(); l++) {
         Level &F = s->lev[l - 1], &C = s->lev[l];
-        const dim3 Br((C.g.Nx * C.g.Ny + GRAP_POINTS - 1) / GRAP_POINTS);
-        NK_LAUNCH_TOPO(C.g, k_g_rap, T, Br, THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
-                       gs<T>(C), s->gerr, (l == 1 && s->fin_pattern) ? 1 : 0);
+        const bool split = s->rap_split && (long)C.g.Nx * C.g.Ny <= s->rap_split_points;
+        const int npb = split ? GRAP_POINTS / 2 : GRAP_POINTS, fin = (l == 1 && s->fin_pattern) ? 1 : 0;
+        const dim3 Br((C.g.Nx * C.g.Ny + npb - 1) / npb);
+        if (split)
+            NK_LAUNCH_TOPO(C.g, k_g_rap_split, T, Br, THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
+                           gs<T>(C), s->gerr, fin);
+        else
+            NK_LAUNCH_TOPO(C.g, k_g_rap, T, Br, THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
+                           gs<T>(C), s->gerr, fin);
         CHK(launch_check(s));
     }
     if (s->cinv) {
@@ -1572,6 +1583,10 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         s->fuse_rr = !(nrr && nrr[0] == '1');
         const char *nrrp = std::getenv("NK_RR_POINTS");
         if (nrrp) s->rr_points = std::strtol(nrrp, nullptr, 10);
+        const char *nrs = std::getenv("NK_NO_RAP_SPLIT");
+        s->rap_split = !(nrs && nrs[0] == '1');
+        const char *nrsp = std::getenv("NK_RAP_SPLIT_POINTS");
+        if (nrsp) s->rap_split_points = std::strtol(nrsp, nullptr, 10);
         const char *nfp = std::getenv("NK_NO_FIN_PATTERN");
         s->fin_pattern = !(nfp && nfp[0] == '1');
         const char *ngr = std::getenv("NK_NO_GINV_REGS");
