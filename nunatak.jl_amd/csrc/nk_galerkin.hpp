@@ -376,7 +376,7 @@ constexpr int GRAP_POINTS = 8;
 template <typename T, int TX, int TY, int SPLIT>
 __device__ __forceinline__ void g_rap_block(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
                                             const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
-                                            int *err, int fin) {
+                                            int *err, int fin, T *__restrict__ ccst) {
     cg.tx = TX; cg.ty = TY;
     fg.tx = TX; fg.ty = TY;
     constexpr int npb = SPLIT ? GRAP_POINTS / 2 : GRAP_POINTS;   // coarse points per block
@@ -422,8 +422,10 @@ __device__ __forceinline__ void g_rap_block(KG cg, KG fg, int hx, int hy, const 
         }
     }
     __syncthreads();
-    // each coupling k of each row: the partials summed over the 12 support points in support order
-    const long P = gplane(cg);
+    // each coupling k of each row: the partials summed over the 12 support points in support order. With ccst the
+    // same values also go to the level's compact copy (k_g_compact's layout; its padding points stay zero from
+    // allocation), which the LDS tail reads.
+    const long P = gplane(cg), NP = (long)(cg.Nx + 1) * (cg.Ny + 1);
     for (int e = tid; e < 2 * npb * 31; e += NT) {
         const int ss = e / 31, k = e % 31, rs = ss & 1, qs = blockIdx.x * npb + (ss >> 1);
         if (qs >= npts) continue;
@@ -434,7 +436,9 @@ __device__ __forceinline__ void g_rap_block(KG cg, KG fg, int hx, int hy, const 
         } else {
             v = k == diag ? T(1) : T(0);
         }
-        cst[(base + k) * P + off(cg, qs % cg.Nx + 1, qs / cg.Nx + 1)] = v;
+        const int Is = qs % cg.Nx + 1, Js = qs / cg.Nx + 1;
+        cst[(base + k) * P + off(cg, Is, Js)] = v;
+        if (ccst) ccst[(base + k) * NP + (long)(Js - 1) * (cg.Nx + 1) + (Is - 1)] = v;
         absv[ss][k] = fabs(v);
     }
     __syncthreads();
@@ -443,7 +447,10 @@ __device__ __forceinline__ void g_rap_block(KG cg, KG fg, int hx, int hy, const 
         if (qs < npts) {
             T l1 = T(0);
             for (int k = 0; k < 31; k++) l1 += absv[ss][k];
-            cst[(GS_PLANES + rs) * P + off(cg, qs % cg.Nx + 1, qs / cg.Nx + 1)] = l1 > T(0) ? T(1) / l1 : T(1);
+            const int Is = qs % cg.Nx + 1, Js = qs / cg.Nx + 1;
+            const T r = l1 > T(0) ? T(1) / l1 : T(1);
+            cst[(GS_PLANES + rs) * P + off(cg, Is, Js)] = r;
+            if (ccst) ccst[(GS_PLANES + rs) * NP + (long)(Js - 1) * (cg.Nx + 1) + (Is - 1)] = r;
         }
     }
 }
@@ -451,14 +458,14 @@ __device__ __forceinline__ void g_rap_block(KG cg, KG fg, int hx, int hy, const 
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_g_rap(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
                                               const uint8_t *__restrict__ fact, const T *__restrict__ fst, T *__restrict__ cst,
-                                              int *err, int fin) {
-    g_rap_block<T, TX, TY, 0>(cg, fg, hx, hy, cact, fact, fst, cst, err, fin);
+                                              int *err, int fin, T *__restrict__ ccst) {
+    g_rap_block<T, TX, TY, 0>(cg, fg, hx, hy, cact, fact, fst, cst, err, fin, ccst);
 }
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_g_rap_split(KG cg, KG fg, int hx, int hy, const uint8_t *__restrict__ cact,
                                                     const uint8_t *__restrict__ fact, const T *__restrict__ fst,
-                                                    T *__restrict__ cst, int *err, int fin) {
-    g_rap_block<T, TX, TY, 1>(cg, fg, hx, hy, cact, fact, fst, cst, err, fin);
+                                                    T *__restrict__ cst, int *err, int fin, T *__restrict__ ccst) {
+    g_rap_block<T, TX, TY, 1>(cg, fg, hx, hy, cact, fact, fst, cst, err, fin, ccst);
 }
 
 // ℓ1-Jacobi sweep with a stencil operator: x_out = x + (b − A x)/ℓ1, or b/ℓ1 from a zero x.

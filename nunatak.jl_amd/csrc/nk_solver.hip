@@ -133,6 +133,8 @@ struct nk_solver {
     bool rap_split = true;      // triple products into levels of at most rap_split_points points split each support
                                 // point's couplings between two lanes by column component (k_g_rap_split;
                                 // NK_NO_RAP_SPLIT=1 at creation: one lane per support point, k_g_rap)
+    bool rap_compact = true;    // the products also write the tail levels' compact stencil copies (NK_NO_RAP_COMPACT=1
+                                // at creation: k_g_compact copies them after the products)
     long rap_split_points = 256L * 256;   // (NK_RAP_SPLIT_POINTS: tuning; into 512² from 1024² the split measured
                                           // 491 against 444 µs, into 256² and below 335 against 344 µs down to 30 against 53)
     bool fin_pattern = true;    // the first triple product reads only the finest stencil's 9 couplings per row
@@ -444,12 +446,13 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
         const bool split = s->rap_split && (long)C.g.Nx * C.g.Ny <= s->rap_split_points;
         const int npb = split ? GRAP_POINTS / 2 : GRAP_POINTS, fin = (l == 1 && s->fin_pattern) ? 1 : 0;
         const dim3 Br((C.g.Nx * C.g.Ny + npb - 1) / npb);
+        T *cc = s->rap_compact ? (T *)C.gcst : nullptr;   // the tail's compact copy, written by the product itself
         if (split)
             NK_LAUNCH_TOPO(C.g, k_g_rap_split, T, Br, THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
-                           gs<T>(C), s->gerr, fin);
+                           gs<T>(C), s->gerr, fin, cc);
         else
             NK_LAUNCH_TOPO(C.g, k_g_rap, T, Br, THREADS, 0, s->stream, C.g, F.g, C.cx, C.cy, C.act, F.act, gs<T>(F),
-                           gs<T>(C), s->gerr, fin);
+                           gs<T>(C), s->gerr, fin, cc);
         CHK(launch_check(s));
     }
     if (s->cinv) {
@@ -465,7 +468,7 @@ template <typename T> nk_status galerkin_build(nk_solver *s) {
         k_g_pack<T><<<blocks_for(L.g.Nx, L.g.Ny), THREADS, 0, s->stream>>>(L.g, gs<T>(L), (T *)L.gpk);
         CHK(launch_check(s));
     }
-    for (size_t l = 1; l < s->lev.size(); l++) {
+    for (size_t l = 1; l < s->lev.size() && !s->rap_compact; l++) {
         Level &L = s->lev[l];
         if (!L.gcst) continue;
         k_g_compact<T><<<blocks_for(L.g.Nx + 1, L.g.Ny + 1), THREADS, 0, s->stream>>>(L.g, gs<T>(L), (T *)L.gcst);
@@ -1585,6 +1588,8 @@ nk_status nk_create(const nk_grid *grid, const nk_physics *physics, const nk_sol
         if (nrrp) s->rr_points = std::strtol(nrrp, nullptr, 10);
         const char *nrs = std::getenv("NK_NO_RAP_SPLIT");
         s->rap_split = !(nrs && nrs[0] == '1');
+        const char *nrc = std::getenv("NK_NO_RAP_COMPACT");
+        s->rap_compact = !(nrc && nrc[0] == '1');
         const char *nrsp = std::getenv("NK_RAP_SPLIT_POINTS");
         if (nrsp) s->rap_split_points = std::strtol(nrsp, nullptr, 10);
         const char *nfp = std::getenv("NK_NO_FIN_PATTERN");

@@ -450,23 +450,26 @@ def test_first_product_over_the_finest_pattern_equals_full_pattern(monkeypatch, 
 @pytest.mark.parametrize("precision,topology,N,Ny,walls", [(MIX, (B, B), 128, None, None), (MIX, (P, P), 96, None, None),
                                                            (F64, (P, B), 128, 64, None), (MIX, (B, P), 160, 96, None),
                                                            (MIX, (B, B), 96, None, abi.FREE_SLIP), (F64, (B, B), 144, 48, None)])
-def test_split_triple_product_equals_one_lane_per_support_point(monkeypatch, precision, topology, N, Ny, walls):
+@pytest.mark.parametrize("switch", ["NK_NO_RAP_SPLIT", "NK_NO_RAP_COMPACT"])
+def test_split_triple_product_equals_one_lane_per_support_point(monkeypatch, switch, precision, topology, N, Ny, walls):
     """The Galerkin products divide each restriction support point's fine couplings between two lanes by column
     component (their coarse targets are disjoint slots); with NK_NO_RAP_SPLIT=1 at creation one lane takes them all.
-    Every coarse coupling receives the same terms in the same order, so solutions and iteration counts must be
-    identical bit for bit — on all four topologies, with free-slip walls and on a semi-coarsened hierarchy."""
+    The products also write the LDS tail's compact stencil copies; with NK_NO_RAP_COMPACT=1 k_g_compact copies them
+    afterwards. Every coarse coupling receives the same terms in the same order and the tail reads the same values, so
+    solutions and iteration counts must be identical bit for bit — on all four topologies, with free-slip walls and on
+    a semi-coarsened hierarchy."""
     out = []
     for split in (True, False):
         if split:
-            monkeypatch.delenv("NK_NO_RAP_SPLIT", raising=False)
+            monkeypatch.delenv(switch, raising=False)
         else:
-            monkeypatch.setenv("NK_NO_RAP_SPLIT", "1")
+            monkeypatch.setenv(switch, "1")
         m, info = channel(N, abi.COARSE_GALERKIN, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny,
                           walls=walls)
         out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
         m.close()
-    monkeypatch.delenv("NK_NO_RAP_SPLIT", raising=False)
-    say(f"[galerkin] split vs one-lane triple product (precision {precision}, {topology}, {N}x{Ny or N}, walls {walls}): "
+    monkeypatch.delenv(switch, raising=False)
+    say(f"[galerkin] {switch} off vs on (precision {precision}, {topology}, {N}x{Ny or N}, walls {walls}): "
         f"CG {out[0][2]} / {out[1][2]}")
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
