@@ -526,6 +526,25 @@ __global__ __launch_bounds__(NT) void k_apply(KG g, const T *__restrict__ pu, co
     if (nred) reduce_finish<1>(acc, red);
 }
 
+// The α finisher inside the consumer: every block sums the operator's p·q slots (fold_nb of them) as k_finish would
+// and forms α = ρ/(p·q) and the FP32 scale itself; block 0 stores them (and ρ) for the kernels that follow.
+__device__ __forceinline__ void fold_alpha(const double *fold, int fold_nb, Scal *S, double &alpha, double &zs) {
+    __shared__ double tot[MAXRED];
+    fold_sum<1>(fold, fold_nb, 1, tot);
+    const double pq = tot[0], rho = S->rho_new;
+    alpha = pq == 0.0 ? 0.0 : rho / pq;
+    double zi;
+    scale_of(S->res, zs, zi);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        S->rho = rho;
+        S->pq = pq;
+        S->alpha = alpha;
+        S->zs = zs;
+        S->zi = zi;
+        for (int k = 0; k < MAXRED; k++) S->out[k] = tot[k];
+    }
+}
+
 // Jacobi step of the PCG with the diagonal preconditioner:
 //   x += α p; r += (−α) q; z = r / d; (r·z, r·r) → β, ρ, res, done.  With init != 0: z = r / d, p = z, (r·z, r·r).
 template <typename T, int TX, int TY>
@@ -533,9 +552,15 @@ __global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int p
                                                     const T *__restrict__ pu, const T *__restrict__ pv, const T *__restrict__ qu,
                                                     const T *__restrict__ qv, T *__restrict__ ru, T *__restrict__ rv,
                                                     T *__restrict__ zu, T *__restrict__ zv, T *__restrict__ pu_out,
-                                                    T *__restrict__ pv_out, int init, Red red, const Scal *S) {
+                                                    T *__restrict__ pv_out, int init, Red red, Scal *S, const double *fold,
+                                                    int fold_nb) {
     g.tx = TX; g.ty = TY;
     if (solve_done(S)) return;
+    double alpha = 0.0, zs;
+    if (!init) {   // with fold: α from the operator's p·q slots (the r·z, r·r slots then go to another buffer)
+        if (fold) fold_alpha(fold, fold_nb, S, alpha, zs);
+        else alpha = S->alpha;
+    }
     NK_IJ
     double acc[2] = {0.0, 0.0};
     if (i <= g.Nx && j <= g.Ny) {
@@ -545,7 +570,7 @@ __global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int p
             r_u = ru[o];
             r_v = rv[o];
         } else {
-            const T al = (T)S->alpha;
+            const T al = (T)alpha;
             xu[o] = T(1) * xu[o] + al * pu[o];
             xv[o] = T(1) * xv[o] + al * pv[o];
             r_u = T(1) * ru[o] + (-al) * qu[o];
@@ -571,25 +596,6 @@ __global__ __launch_bounds__(NT) void k_update_diag(KG g, Coef<T> c, T wf, int p
         acc[1] = (double)r_u * (double)r_u + (double)r_v * (double)r_v;
     }
     reduce_finish<2>(acc, red);
-}
-
-// The α finisher inside the consumer: every block sums the operator's p·q slots (fold_nb of them) as k_finish would
-// and forms α = ρ/(p·q) and the FP32 scale itself; block 0 stores them (and ρ) for the kernels that follow.
-__device__ __forceinline__ void fold_alpha(const double *fold, int fold_nb, Scal *S, double &alpha, double &zs) {
-    __shared__ double tot[MAXRED];
-    fold_sum<1>(fold, fold_nb, 1, tot);
-    const double pq = tot[0], rho = S->rho_new;
-    alpha = pq == 0.0 ? 0.0 : rho / pq;
-    double zi;
-    scale_of(S->res, zs, zi);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
-        S->rho = rho;
-        S->pq = pq;
-        S->alpha = alpha;
-        S->zs = zs;
-        S->zi = zi;
-        for (int k = 0; k < MAXRED; k++) S->out[k] = tot[k];
-    }
 }
 
 // The β finisher inside the consumer: every block sums the preconditioner's r·z slots (the first value of its

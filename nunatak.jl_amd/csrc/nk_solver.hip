@@ -1001,12 +1001,13 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
         CHK(launch_check(s));
     } else {
         NK_LAUNCH_TOPO(g, k_update_diag, T, B, THREADS, 0, s->stream, g, c, wf, pre, xu, xv, W.pu, W.pv, W.qu, W.qv, W.ru, W.rv, W.zu, W.zv, W.pu,
-                                                        W.pv, 1, red(s, FIN_INIT), nullptr);
+                                                        W.pv, 1, red(s, FIN_INIT), nullptr, nullptr, 0);
         CHK(launch_check(s));
         CHK(finish_red(s, B, 2, FIN_INIT, nullptr));
     }
     const int chunk = s->op.check_interval > 0 ? s->op.check_interval : 8;
-    const bool fold = s->fold && pre == NK_PRECONDITIONER_MULTIGRID;
+    const bool fold = s->fold && pre == NK_PRECONDITIONER_MULTIGRID;   // α and β folded
+    const bool fold_a = s->fold;                                       // α folded (also into k_update_diag)
     auto launch_chunk = [&]() -> nk_status {
         for (int k = 0; k < chunk; k++) {
             {
@@ -1015,7 +1016,7 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
                                  red(s, FIN_ALPHA), 1, (const Scal *)s->S, ps.e0, ps.e1);
             }
             CHK(launch_check(s));
-            if (!fold) CHK(finish_red(s, B, 1, FIN_ALPHA, s->S));
+            if (!fold_a) CHK(finish_red(s, B, 1, FIN_ALPHA, s->S));
             if (pre == NK_PRECONDITIONER_MULTIGRID) {
                 {
                     ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
@@ -1045,12 +1046,14 @@ nk_status pcg(nk_solver *s, T *xu, T *xv, const T *bu, const T *bv, const nk_con
             } else {
                 {
                     ProfScope ps(s, NK_PROFILE_UPDATE, 0, (int)sizeof(T));
+                    Red rd = red(s, FIN_BETA_RES);
+                    rd.partials = s->partials_xr;
                     NK_LAUNCH_TOPO_EV(g, k_update_diag, T, B, THREADS, 0, s->stream, ps.e0, ps.e1, g, c, wf, pre, xu, xv, (const T *)W.pu,
                                       (const T *)W.pv, (const T *)W.qu, (const T *)W.qv, W.ru, W.rv, W.zu, W.zv, (T *)nullptr, (T *)nullptr, 0,
-                                      red(s, FIN_BETA_RES), (const Scal *)s->S);
+                                      rd, s->S, fold_a ? (const double *)s->partials : nullptr, (int)(B.x * B.y));
                 }
                 CHK(launch_check(s));
-                CHK(finish_red(s, B, 2, FIN_BETA_RES, s->S));
+                CHK(finish_red(s, B, 2, FIN_BETA_RES, s->S, s->partials_xr));
                 k_update_p<T><<<B, THREADS, 0, s->stream>>>(g, W.zu, W.zv, W.pu, W.pv, 0, s->S, nullptr, 0);
                 CHK(launch_check(s));
             }
@@ -1971,7 +1974,7 @@ nk_status nk_precondition(nk_solver *s) {
     } else {
         NK_LAUNCH_TOPO(g, k_update_diag, double, B, THREADS, 0, s->stream, g, coef<double>(L), s->ph.walls == NK_NO_SLIP ? 2.0 : 0.0, pre, nullptr, nullptr,
                                                              nullptr, nullptr, nullptr, nullptr, ru, rv, zu, zv, s->w64.pu, s->w64.pv, 1,
-                                                             red(s, FIN_NONE), nullptr);
+                                                             red(s, FIN_NONE), nullptr, nullptr, 0);
         CHK(launch_check(s));
     }
     HIPCHK(hipStreamSynchronize(s->stream));

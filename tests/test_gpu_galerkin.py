@@ -25,7 +25,7 @@ def say(*a):
 
 
 def channel(N, coarse, precision=abi.PRECISION_FP64, picard=1, linear=1e-10, topology=(B, B), Ny=None, walls=None,
-            smoother=None):
+            smoother=None, preconditioner=None):
     g, phys, opts, fields = W.diva_shelf(N=N, dx=512e3 / N)
     if walls is not None:
         phys.walls = walls
@@ -44,6 +44,8 @@ def channel(N, coarse, precision=abi.PRECISION_FP64, picard=1, linear=1e-10, top
     opts.precision = precision
     if smoother is not None:
         opts.smoother = smoother
+    if preconditioner is not None:
+        opts.preconditioner = preconditioner
     m = HIPModel(g, phys, opts)
     W.load_into(m, fields)
     info = m.update_velocities(geometry=True)
@@ -390,7 +392,8 @@ def test_residual_and_restriction_in_one_launch_equal_separate_kernels(monkeypat
 
 
 @pytest.mark.parametrize("precision,coarse,topology,N,Ny,smoother,blocks",
-                         [(MIX, abi.COARSE_GALERKIN, (B, B), 128, None, None, None),
+                         [(F64, "diagonal", (B, B), 96, None, None, None),
+                          (MIX, abi.COARSE_GALERKIN, (B, B), 128, None, None, None),
                           (MIX, abi.COARSE_GALERKIN, (P, P), 96, None, None, None),
                           (F64, abi.COARSE_REDISCRETIZED, (P, B), 128, None, None, None),
                           (MIX, abi.COARSE_REDISCRETIZED, (B, P), 160, 96, abi.LINE_JACOBI, None),
@@ -402,7 +405,7 @@ def test_folded_finishers_equal_k_finish(monkeypatch, precision, coarse, topolog
     launches; NK_NO_FOLD=1 at creation keeps the launches. Solutions and iteration counts must be identical bit for bit
     — with the FP32 V-cycle's z read directly (k_update_p32), through the FP64 z (line smoother: k_promote_dot's r·z
     with r·r beside it), in the FP64 form (k_update_p), and on a 1024 × 520 grid with folding forced (2,080 slots per
-    value, more than one load batch per thread)."""
+    value, more than one load batch per thread). With the diagonal preconditioner only α is folded (into k_update_diag)."""
     out = []
     if blocks:
         monkeypatch.setenv("NK_FOLD_BLOCKS", blocks)
@@ -411,7 +414,11 @@ def test_folded_finishers_equal_k_finish(monkeypatch, precision, coarse, topolog
             monkeypatch.delenv("NK_NO_FOLD", raising=False)
         else:
             monkeypatch.setenv("NK_NO_FOLD", "1")
-        m, info = channel(N, coarse, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny, smoother=smoother)
+        if coarse == "diagonal":
+            m, info = channel(N, abi.COARSE_REDISCRETIZED, precision=precision, picard=2, linear=1e-8, topology=topology,
+                              Ny=Ny, preconditioner=abi.PRECONDITIONER_DIAGONAL)
+        else:
+            m, info = channel(N, coarse, precision=precision, picard=2, linear=1e-8, topology=topology, Ny=Ny, smoother=smoother)
         out.append((m.get("U"), m.get("V"), info["total_linear_iterations"]))
         m.close()
     monkeypatch.delenv("NK_NO_FOLD", raising=False)
