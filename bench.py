@@ -267,6 +267,35 @@ def serving_leg(n, rate_hz, slots, serial_sample, cpu_oracle=True):
                         f"{serial_sample} requests through one slot"}
 
 
+def serving_request_leg(side, warmup, steps, graphs):
+    """One request of the serving sample (the first of the 48-request schedule with this side) solved alone on one
+    handle: warmup solves, then steps timed solves from zero velocity (wall time around update_velocities)."""
+    _, sides = workloads.serving_schedule(48, 2.0, 5, 64, 1024)
+    picks = [k for k, sd in enumerate(sides) if sd == side]
+    if not picks:
+        raise SystemExit(f"no request with side {side} in the serving sample (sides: {sorted(set(sides.tolist()))})")
+    k = picks[0]
+    g, p, o, f = workloads.serving_stream_request(k, sides)
+    o.graphs = graphs
+    m = HIPModel(g, p, o)
+    times, info = [], None
+    for rep in range(warmup + steps):
+        workloads.load_into(m, f)
+        m.synchronize()
+        t0 = time.perf_counter()
+        info = m.update_velocities()
+        m.synchronize()
+        if rep >= warmup:
+            times.append((time.perf_counter() - t0) * 1e3)
+    m.close()
+    log(f"[bench] serving request {k} ({side}²): {np.median(times):.1f} ms median of {steps}, picard "
+        f"{info['picard_iterations']}, cg {info['total_linear_iterations']}")
+    return {"request": k, "side": side, "ms": times, "median_ms": float(np.median(times)), "graphs": graphs,
+            "picard_iterations": info["picard_iterations"], "cg_iterations": info["total_linear_iterations"],
+            "workload": "serving stream (seed 5) request: SSA grounded/floating channel, mixed precision (FP32 Galerkin "
+                        "V-cycle), MG-PCG linear 1e-8, Picard 1e-6"}
+
+
 def serving_cpu_small(sample64, rec64, rec_mixed, budget_s=20.0):
     """The reference algorithm on the host for the serving stream: the oracle (FP64, rediscretized multigrid, all host
     threads) solves the sample's requests from the smallest up until budget_s of host time is spent; the same requests'
@@ -427,6 +456,9 @@ def main():
     ap.add_argument("--transport", type=int, default=20, help="RK3 thickness steps timed in the transport leg (0: skip)")
     ap.add_argument("--transport-only", action="store_true",
                     help="run only the thickness-transport and enthalpy legs; prints their JSON objects")
+    ap.add_argument("--serve-request", type=int, default=0,
+                    help="time one request of the serving sample with this side alone (--warmup, then --steps solves on "
+                         "one handle, mixed form; --graphs 0 for external profilers); prints one JSON object and exits")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the velocity solve of one grid is not "
@@ -437,6 +469,9 @@ def main():
         print(json.dumps({"transport": transport_leg(N, max(args.transport, 1)),
                           "transport_weno_bounded": transport_leg(N, max(args.transport, 1), abi.ADVECTION_WENO5_BOUNDED),
                           "energy": energy_leg(N, 10)}), flush=True)
+        return
+    if args.serve_request:
+        print(json.dumps({"serving_request": serving_request_leg(args.serve_request, Wm, K, args.graphs)}), flush=True)
         return
     os.environ.setdefault("NK_VERBOSE", "1")  # one library progress line per Picard iteration
     if args.stream_curve:
